@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 Mcell-steps/s of the decaying-turbulence time step
+(big_periodic_3D path: AB2 convection/diffusion + FFT pressure Poisson +
+projection, triply periodic, 256^3, nu=1/1000, dt=5e-4) on one MI355X.
+
+A step is one full time step of the whole 256^3 field; inputs are resident
+in HBM before the timed region.  The per-kernel roofline numbers come from
+HIP events recorded on the solver's own stream (ns_step_profiled) over the
+same number of steps.  The CPU baseline times the oracle (the repository's
+C restatement of the reference step, OpenMP) on a short sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic FP64 arrays streamed per interior cell by each phase (one read or
+# write of one full array = 8 B/cell; the r2c half spectrum counts as one real array)
+PHASES = ["predictor", "vel_bc", "div_fft_x_fwd", "fft_y_fwd", "fft_z_solve", "fft_y_inv", "fft_x_inv_dp",
+          "pr_bc", "project"]
+ARRAYS_PER_CELL = {"predictor": 12, "vel_bc": 0, "div_fft_x_fwd": 4, "fft_y_fwd": 2, "fft_z_solve": 2,
+                   "fft_y_inv": 2, "fft_x_inv_dp": 2, "pr_bc": 0, "project": 8}
+KERNEL_NAMES = {"predictor": "k_predictor", "div_fft_x_fwd": "k_fft_x_fwd", "fft_y_fwd": "k_fft_yz<1,-1,0>",
+                "fft_z_solve": "k_fft_yz<2,-1,1>", "fft_y_inv": "k_fft_yz<1,1,0>", "fft_x_inv_dp": "k_fft_x_inv",
+                "project": "k_project"}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_256.json"))
+    return ap.parse_args()
+
+
+def cpu_baseline(cfg, u, v, w, p, nsteps):
+    from oracle import oracle as O
+
+    st = O.OracleState(cfg, u, v, w, p, first=True)
+    st.step()  # untimed Euler first step
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        st.step()
+    dt = time.perf_counter() - t0
+    n3 = cfg.nx * cfg.ny * cfg.nz
+    return {
+        "value": n3 * nsteps / dt / 1e6,
+        "unit": "Mcell-steps/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"{nsteps} AB2 steps of the same {cfg.nx}^3 field after 1 untimed Euler step "
+                  f"({dt:.1f} s, oracle/ns_oracle.c, OpenMP)",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU decomposition is "
+              "described in DESIGN.md and is not built.", file=sys.stderr)
+        return 2
+    import torch
+
+    from ns_3d_staggered_dp_amd import AB2, FLAG_GAUGE_CORNER, NSConfig, fields
+    from ns_3d_staggered_dp_amd import _abi
+    from ns_3d_staggered_dp_amd.solver import NSSolver
+
+    n = a.n
+    cfg = NSConfig.cube(n, nu=1e-3, dt=5e-4, scheme=AB2, flags=FLAG_GAUGE_CORNER)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=2)
+    s = NSSolver(cfg)
+    s.upload(u, v, w, p, first=True)
+    s.step(a.warmup)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(a.steps)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    n3 = n ** 3
+    ms_step = elapsed / a.steps * 1e3
+    value = n3 * a.steps / elapsed / 1e6
+    finite = s.check_finite()
+    maxdiv = s.max_div()
+
+    # per-phase GPU time on the solver stream, same number of steps
+    import ctypes
+
+    ph = (ctypes.c_double * 9)()
+    _abi.call("ns_step_profiled", s.h, a.steps, ph)
+    phase_ms = {k: ph[i] / a.steps for i, k in enumerate(PHASES)}
+    dom = max((k for k in PHASES if ARRAYS_PER_CELL[k] > 0), key=lambda k: phase_ms[k])
+    bytes_launch = ARRAYS_PER_CELL[dom] * 8 * n3
+    achieved = bytes_launch / (phase_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(a.traffic_json):
+        try:
+            tj = json.load(open(a.traffic_json))
+            traffic = tj.get(KERNEL_NAMES.get(dom, dom))
+        except Exception:
+            traffic = None
+    total_alg = sum(ARRAYS_PER_CELL.values()) * 8
+    step_gbs = total_alg * n3 / (ms_step * 1e-3) / 1e9
+    s.close()
+
+    line = {
+        "metric": "FP64 Mcell-steps/s at 256^3 + % HBM roofline; adjoint/forward cost ratio",
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: random solenoidal field, E(k)~k^4 exp(-2(k/4)^2), seed 2, energy 0.5",
+        "config": {"workload": f"decaying turbulence {n}^3 periodic, AB2 + FFT Poisson, nu=1e-3, dt=5e-4",
+                   "grid": [n, n, n], "scheme": "AB2", "poisson": "FFT", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": bytes_launch, "avg_ms": round(phase_ms[dom], 4)},
+        "step_roofline": {"alg_bytes_per_cell_step": total_alg, "achieved_GBs": round(step_gbs, 1),
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
+        "checks": {"finite": bool(finite), "max_div": maxdiv},
+    }
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(cfg, u, v, w, p, a.cpu_steps)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,149 @@
+/*
+ * ns_hip.h -- C ABI of the MI355X (gfx950) FP64 staggered-grid Navier-Stokes
+ * time step.  Drop-in boundary for the Fortran host of NS_3D_Staggered_dp.
+ *
+ * Conventions
+ *  - Every array is FP64, Fortran column-major, with the reference's ghosted
+ *    shapes: u(nx+1,ny+2,nz+2), v(nx+2,ny+1,nz+2), w(nx+2,ny+2,nz+1),
+ *    p,dp(nx+2,ny+2,nz+2) (big_periodic_3D.f90:14-18,27-28); interior-only
+ *    arrays are (nx,ny,nz).
+ *  - Pointers may be host or device (HIP unified addressing); host arrays are
+ *    staged through the context.  Device-resident callers pay no copies.
+ *  - A boundary-plane set is one contiguous FP64 vector: the 18 velocity
+ *    planes in vel_bc_staggered's argument order (NS_functions.f90:213-217:
+ *    bx_u_1, bx_u_nx, by_u_1, by_u_ny, bz_u_1, bz_u_nz, then v, then w), the
+ *    6 pressure planes (bx_p_1 ... bz_p_nz), then the scalar mean_dp.  Its
+ *    length is ns_plane_set_length().
+ *  - Every entry point returns an int status (NS_OK = 0, negative on error);
+ *    nothing aborts across the ABI (the reference stops with stop(1),
+ *    re_simulation_sub_tstep_interBC_combine.f90:393-394).
+ *  - One context per host thread; calls on one context are serialised on its
+ *    HIP stream.  Contexts share no mutable state.
+ */
+#ifndef NS_HIP_H
+#define NS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NS_OK 0
+#define NS_ERR_INVALID (-1)     /* bad dimensions / codes / null pointer */
+#define NS_ERR_UNSUPPORTED (-2) /* combination the reference does not implement either */
+#define NS_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define NS_ERR_NONFINITE (-4)   /* NaN/Inf detected by ns_check_finite */
+#define NS_ERR_NOMEM (-5)
+
+#define NS_EULER 0
+#define NS_AB2 1
+#define NS_AB2_CN 2
+
+/* dp = dp - dp(1,1,1) after the pressure solve, as the periodic driver does
+ * (big_periodic_3D.f90:562). */
+#define NS_FLAG_GAUGE_CORNER 1
+/* Periodic CN rows use the wrap stencil.  Without it the periodic CN matrix
+ * is mat_CN's (NS_functions.f90:1273-1278), whose row 1 couples to column
+ * nx+1 (== u(1)) instead of nx. */
+#define NS_FLAG_CN_CYCLIC 2
+
+/* Run configuration: the driver parameter block (big_periodic_3D.f90:9-21,
+ * 104-108; re_simulation_sub_tstep_interBC_combine.f90:100-112).
+ * bc/pbc codes (NS_functions.f90:219-220): 1 periodic, 2 Dirichlet on the
+ * cell wall, 3 Neumann on the cell wall, 4 Dirichlet on the ghost cell. */
+typedef struct ns_config {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    double nu, dt;
+    int scheme;  /* NS_EULER, NS_AB2, NS_AB2_CN */
+    int bc[3];   /* velocity: 1, 2 or 4 */
+    int pbc[3];  /* pressure: all 1 (FFT) or all in {2,3} (DST-II/DCT-II) */
+    int dp_flag; /* 1: p accumulates dp and -dt*grad p enters the predictor */
+    int flags;   /* NS_FLAG_* */
+} ns_config;
+
+typedef struct ns_ctx ns_ctx;
+
+/* ---- lifecycle and state (replaces the drivers' array setup,
+ *      big_periodic_3D.f90:26-33,157-169, and the HDF5 restart set
+ *      u,v,w,p,rhs_*_previous, big_periodic_3D.f90:309-325) ---- */
+int ns_create(const ns_config *cfg, ns_ctx **out);
+int ns_destroy(ns_ctx *ctx);
+long ns_plane_set_length(int nx, int ny, int nz);
+/* first != 0: the next AB2/AB2-CN step is the Euler-type first step
+ * (big_periodic_3D.f90:377, ...interBC_combine.f90:569); 0 = restart. */
+int ns_upload_state(ns_ctx *ctx, const double *u, const double *v, const double *w, const double *p,
+                    const double *hx, const double *hy, const double *hz, int first);
+int ns_download_state(ns_ctx *ctx, double *u, double *v, double *w, double *p, double *hx, double *hy, double *hz);
+/* u*, v*, w* and dp of the last step (dp gauged like the driver's). Any may be NULL. */
+int ns_download_work(ns_ctx *ctx, double *u_star, double *v_star, double *w_star, double *dp);
+int ns_synchronize(ns_ctx *ctx);
+/* device pointers of the resident state, in the order u,v,w,p,hx,hy,hz,u*,v*,w*,dp */
+int ns_state_pointers(ns_ctx *ctx, double **ptrs11);
+
+/* ---- fused time step (the loop body of big_periodic_3D.f90:326-571 and
+ *      re_simulation_sub_tstep_interBC_combine.f90:508-802) ---- */
+/* current boundary-plane set used by the following steps (host or device) */
+int ns_set_bc_planes(ns_ctx *ctx, const double *planes);
+/* current set = (1-theta)*a + theta*b on the device: inter-boundary time
+ * interpolation (MATLAB interp1 of sub_tstep_BC_combined_interpolation.m:61-64,
+ * linear form) */
+int ns_bc_interp(ns_ctx *ctx, const double *set_a, const double *set_b, double theta);
+int ns_step(ns_ctx *ctx, int nsteps);
+/* ns_step with HIP events around each phase on the context stream; adds the
+ * per-phase GPU milliseconds, summed over the steps, into phase_ms[NS_NPHASES]:
+ * 0 predictor, 1 velocity BC fill or CN2 rows + ADI sweeps, 2 divergence +
+ * x forward transform, 3 y forward, 4 z forward + modal solve + z inverse,
+ * 5 y inverse, 6 x inverse (dp), 7 pressure ghost fill, 8 projection.
+ * (The reference times the same phases with SYSTEM_CLOCK,
+ * big_periodic_3D.f90:328-347,491-492,515-519,545-546,568-569.) */
+#define NS_NPHASES 9
+int ns_step_profiled(ns_ctx *ctx, int nsteps, double *phase_ms);
+/* steps with a plane set per step, linearly interpolated between parent
+ * snapshots: snapshot s is at time s*snap_dt, step m uses
+ * t = t0 + (m+1)*dt (the boundary data of the step's end time). */
+int ns_resim_advance(ns_ctx *ctx, const double *snapshots, int nsnap, double snap_dt, double t0, int nsteps);
+/* max |div u| over interior cells (big_periodic_3D.f90:576-579) */
+int ns_max_div(ns_ctx *ctx, double *out);
+int ns_check_finite(ns_ctx *ctx);
+
+/* ---- per-subroutine mirrors with the reference's arguments ---- */
+/* cal_conv NS_functions.f90:53-59; conv_* are (nx or nx-1,ny,nz) etc. */
+int ns_cal_conv(ns_ctx *ctx, const double *u, const double *v, const double *w, int bc_x, int bc_y, int bc_z,
+                double dx, double dy, double dz, double *conv_x, double *conv_y, double *conv_z);
+/* cal_diff NS_functions.f90:161-168 */
+int ns_cal_diff(ns_ctx *ctx, const double *u, const double *v, const double *w, int bc_x, int bc_y, int bc_z,
+                double dx2, double dy2, double dz2, double *diff_x, double *diff_y, double *diff_z);
+/* vel_bc_staggered NS_functions.f90:213-217 (planes: packed set) */
+int ns_vel_bc_staggered(ns_ctx *ctx, double *u_star, double *v_star, double *w_star, const double *planes,
+                        int bc_x, int bc_y, int bc_z);
+/* vel_bc_staggered_CN2 NS_functions.f90:313 (the dir-th plane pairs of the packed set) */
+int ns_vel_bc_staggered_cn2(ns_ctx *ctx, double *u_star, double *v_star, double *w_star, const double *planes,
+                            int bc, int dir);
+/* ADI sweep along dir with the context's CN matrices (getrs/dttrsb loops,
+ * big_periodic_3D.f90:425-483, ...interBC_combine.f90:656-701) */
+int ns_adi_sweep(ns_ctx *ctx, double *u_star, double *v_star, double *w_star, int dir);
+/* get_velpr_bc NS_functions.f90:414-420 (writes the packed set; p_previous
+ * used only when !(dp_flag || using_ustar)) */
+int ns_get_velpr_bc(ns_ctx *ctx, const double *u_bc, const double *v_bc, const double *w_bc, const double *p_bc,
+                    double dt, int dp_flag, int using_ustar, const double *p_previous, double *planes);
+/* get_pr_bc NS_functions.f90:518-521 */
+int ns_get_pr_bc(ns_ctx *ctx, const double *p_bc, double *planes);
+/* divergence RHS (big_periodic_3D.f90:498-501): rhs(nx,ny,nz) = div(u*)/dt */
+int ns_divergence(ns_ctx *ctx, const double *u_star, const double *v_star, const double *w_star, double dt,
+                  double *rhs);
+/* pr_bc_staggered_modify_rhs NS_functions.f90:1345 */
+int ns_pr_bc_staggered_modify_rhs(ns_ctx *ctx, double *rhs, const double *planes);
+/* periodic FFT Poisson (big_periodic_3D.f90:526-548): rhs(nx,ny,nz) -> sol(nx,ny,nz) */
+int ns_poisson_fft(ns_ctx *ctx, const double *rhs, double *sol);
+/* DCT_poisson_solver NS_functions.f90:1505 on rhs(nx,ny,nz) in place */
+int ns_dct_poisson_solver(ns_ctx *ctx, double *rhs, double mean_dp);
+/* pr_bc_staggered NS_functions.f90:1403 */
+int ns_pr_bc_staggered(ns_ctx *ctx, double *dp, const double *planes);
+/* projection (big_periodic_3D.f90:564-567) over the full ghosted arrays */
+int ns_project(ns_ctx *ctx, const double *u_star, const double *v_star, const double *w_star, const double *dp,
+               double dt, int dp_flag, double *u, double *v, double *w, double *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,897 @@
+// C ABI of libns_hip: context lifecycle, host-side plans (FFT radices and
+// twiddles, Poisson eigenvalues, CN line systems), the fused time step and
+// the per-subroutine mirrors declared in include/ns_hip.h.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ns_internal.h"
+
+using namespace ns;
+
+struct ns_ctx {
+    ns_config cfg;
+    Grid g;
+    PlaneLayout L;
+    hipStream_t st = nullptr;
+    // resident state (Fortran ghosted layout)
+    double *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
+    double *hx = nullptr, *hy = nullptr, *hz = nullptr;
+    double *us = nullptr, *vs = nullptr, *ws = nullptr, *dp = nullptr;
+    double *planes = nullptr;   // current plane set
+    double *snaps = nullptr;    // snapshot history for ns_resim_advance
+    size_t snaps_len = 0;
+    double2 *spec = nullptr;    // periodic spectrum [z][y][kx]
+    double *work = nullptr;     // DCT work / interior scratch
+    double *partial = nullptr;  // max-div partials
+    unsigned int *counter = nullptr;
+    PoissonPlans pp{};
+    LineSys cn[3][2]{};
+    int first = 1;
+    int poisson_ok = 0;  // 0: mixed periodic / non-periodic codes (no transform solver)
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+const double PI = 3.1415926535897932;  // big_periodic_3D.f90:6
+
+#define HIPCHK(x)                          \
+    do {                                   \
+        if ((x) != hipSuccess) {           \
+            (void)hipGetLastError();       \
+            return NS_ERR_HIP;             \
+        }                                  \
+    } while (0)
+
+template <class T>
+int dalloc(ns_ctx *c, T **p, size_t n) {
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        return NS_ERR_NOMEM;
+    }
+    c->allocs.push_back(q);
+    *p = (T *)q;
+    return NS_OK;
+}
+
+template <class T>
+int dupload(ns_ctx *c, T **p, const std::vector<T> &h) {
+    int rc = dalloc(c, p, h.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return NS_OK;
+}
+
+bool factor_radices(int n, FFTPlan &pl) {
+    pl.n = n;
+    pl.nst = 0;
+    int m = n;
+    while (m % 8 == 0 && m > 8) { pl.radix[pl.nst++] = 8; m /= 8; }
+    // remaining power of two part
+    if (m % 8 == 0) { pl.radix[pl.nst++] = 8; m /= 8; }
+    if (m % 4 == 0) { pl.radix[pl.nst++] = 4; m /= 4; }
+    if (m % 2 == 0) { pl.radix[pl.nst++] = 2; m /= 2; }
+    const int odd[3] = {3, 5, 7};
+    for (int q = 0; q < 3; q++)
+        while (m % odd[q] == 0) {
+            if (pl.nst >= 10) return false;
+            pl.radix[pl.nst++] = odd[q];
+            m /= odd[q];
+        }
+    return m == 1 && pl.nst > 0 && pl.nst <= 10;
+}
+
+int make_fft_plan(ns_ctx *c, int n, FFTPlan &pl) {
+    if (n < 2 || !factor_radices(n, pl)) return NS_ERR_UNSUPPORTED;
+    std::vector<double2> tw(n);
+    const long double tp = 2.0L * acosl(-1.0L);
+    for (int m = 0; m < n; m++) {
+        long double a = tp * (long double)m / (long double)n;
+        tw[m] = make_double2((double)cosl(a), (double)-sinl(a));
+    }
+    double2 *d;
+    int rc = dupload(c, &d, tw);
+    pl.tw = d;
+    return rc;
+}
+
+int make_quarter_wave(ns_ctx *c, int n, const double2 **out) {
+    std::vector<double2> q(n);
+    const long double pi = acosl(-1.0L);
+    for (int k = 0; k < n; k++) {
+        long double a = pi * (long double)k / (2.0L * n);
+        q[k] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    double2 *d;
+    int rc = dupload(c, &d, q);
+    *out = d;
+    return rc;
+}
+
+// Poisson eigenvalue parts, evaluated like the reference:
+// FFT: (sin(pi*(i-1)/nx)/dx)**2 (big_periodic_3D.f90:242-252)
+// DCT: (sin((i-1)*pi/2/nx))**2/dx2 for pbc 3, (sin(i*pi/2/nx))**2/dx2 for pbc 2 (...interBC:321-351)
+int make_eigs(ns_ctx *c) {
+    const ns_config &f = c->cfg;
+    const int n[3] = {f.nx, f.ny, f.nz};
+    const double d[3] = {f.dx, f.dy, f.dz};
+    const double *outp[3];
+    for (int ax = 0; ax < 3; ax++) {
+        std::vector<double> e;
+        if (!c->pp.dct) {
+            int len = (ax == 0) ? n[0] / 2 + 1 : n[ax];
+            e.resize(len);
+            for (int i = 1; i <= len; i++) {
+                double s = sin(PI * (i - 1) / n[ax]) / d[ax];
+                e[i - 1] = s * s;
+            }
+        } else {
+            e.resize(n[ax]);
+            double d2 = d[ax] * d[ax];
+            for (int i = 1; i <= n[ax]; i++) {
+                double s = (f.pbc[ax] == 2) ? sin(i * PI / 2 / n[ax]) : sin((i - 1) * PI / 2 / n[ax]);
+                e[i - 1] = s * s / d2;
+            }
+        }
+        double *dd;
+        int rc = dupload(c, &dd, e);
+        if (rc) return rc;
+        outp[ax] = dd;
+    }
+    c->pp.ex = outp[0];
+    c->pp.ey = outp[1];
+    c->pp.ez = outp[2];
+    return NS_OK;
+}
+
+// Thomas factorisation of a tridiagonal (lo_i: row i col i-1, d_i, up_i: row i col i+1)
+void thomas_factor(const std::vector<double> &lo, const std::vector<double> &d, const std::vector<double> &up,
+                   std::vector<double> &inv, std::vector<double> &cp) {
+    size_t m = d.size();
+    inv.assign(m, 0.0);
+    cp.assign(m, 0.0);
+    inv[0] = 1.0 / d[0];
+    cp[0] = up[0] * inv[0];
+    for (size_t i = 1; i < m; i++) {
+        double den = d[i] - lo[i] * cp[i - 1];
+        inv[i] = 1.0 / den;
+        cp[i] = up[i] * inv[i];
+    }
+}
+std::vector<double> thomas_solve(const std::vector<double> &lo, const std::vector<double> &inv,
+                                 const std::vector<double> &cp, std::vector<double> r) {
+    size_t m = r.size();
+    r[0] = r[0] * inv[0];
+    for (size_t i = 1; i < m; i++) r[i] = (r[i] - lo[i] * r[i - 1]) * inv[i];
+    for (size_t i = m - 1; i-- > 0;) r[i] = r[i] - cp[i] * r[i + 1];
+    return r;
+}
+
+// CN line system for direction ax (0..2) and type A (face-normal, n+1) or B (n+2)
+int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S) {
+    const ns_config &f = c->cfg;
+    const int n = (ax == 0) ? f.nx : (ax == 1) ? f.ny : f.nz;
+    const double h = (ax == 0) ? f.dx : (ax == 1) ? f.dy : f.dz;
+    const int bc = f.bc[ax];
+    const double dx2 = h * h, idx2 = 1.0 / dx2, dt = f.dt, nu = f.nu;
+    // mat_CN entries (NS_functions.f90:1261-1269)
+    const double dg = 1.0 + 0.5 * dt * nu * 2.0 * idx2;
+    const double a = 0.5 * dt * nu * 1.0 * idx2;
+    S = LineSys{};
+    S.a = a;
+    std::vector<double> lo, d, up, inv, cp;
+    if (bc == 1) {
+        const int m = n;
+        S.periodic = 1;
+        S.m = m;
+        S.N = typeB ? n + 2 : n + 1;
+        S.off = typeB ? 1 : 0;
+        S.quirk = (!typeB && !(f.flags & NS_FLAG_CN_CYCLIC)) ? 1 : 0;
+        lo.assign(m, -a);
+        up.assign(m, -a);
+        d.assign(m, dg);
+        lo[0] = 0.0;
+        up[m - 1] = 0.0;
+        double alpha = -a, beta = -a;  // corners (0,m-1) and (m-1,0)
+        if (S.quirk) {
+            d[0] = dg - a;  // (1+2a)u1 - a u_N with u_N = u_1 + r_N
+            alpha = 0.0;
+        }
+        double gamma = -d[0];
+        std::vector<double> dt_ = d;
+        dt_[0] = d[0] - gamma;
+        dt_[m - 1] = d[m - 1] - alpha * beta / gamma;
+        thomas_factor(lo, dt_, up, inv, cp);
+        std::vector<double> uvec(m, 0.0);
+        uvec[0] = gamma;
+        uvec[m - 1] = beta;
+        std::vector<double> z = thomas_solve(lo, inv, cp, uvec);
+        S.s = alpha / gamma;
+        S.den = 1.0 + z[0] + S.s * z[m - 1];
+        double *dz_;
+        int rc = dupload(c, &dz_, z);
+        if (rc) return rc;
+        S.z = dz_;
+    } else if (bc == 2 || bc == 4) {
+        const int N = typeB ? n + 2 : n + 1;
+        S.periodic = 0;
+        S.N = N;
+        S.m = N;
+        S.off = 0;
+        lo.assign(N, -a);
+        up.assign(N, -a);
+        d.assign(N, dg);
+        lo[0] = 0.0;
+        up[N - 1] = 0.0;
+        if (!typeB || bc == 4) {
+            d[0] = 1.0; up[0] = 0.0;
+            d[N - 1] = 1.0; lo[N - 1] = 0.0;
+        } else {
+            d[0] = 0.5; up[0] = 0.5;
+            d[N - 1] = 0.5; lo[N - 1] = 0.5;
+        }
+        thomas_factor(lo, d, up, inv, cp);
+        S.z = nullptr;
+    } else {
+        return NS_ERR_UNSUPPORTED;
+    }
+    double *dl, *di, *dc;
+    int rc = dupload(c, &dl, lo);
+    if (rc) return rc;
+    rc = dupload(c, &di, inv);
+    if (rc) return rc;
+    rc = dupload(c, &dc, cp);
+    if (rc) return rc;
+    S.lo = dl;
+    S.inv = di;
+    S.cp = dc;
+    return NS_OK;
+}
+
+bool is_dev_ptr(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t at;
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+// host-or-device argument staged onto the device for a mirror call
+struct Arg {
+    double *dev = nullptr;
+    void *user = nullptr;
+    size_t n = 0;
+    bool owned = false;
+};
+
+int arg_in(ns_ctx *c, Arg &a, const double *p, size_t n, bool copy_in) {
+    a.user = (void *)p;
+    a.n = n;
+    if (!p) return NS_ERR_INVALID;
+    if (is_dev_ptr(p)) {
+        a.dev = (double *)p;
+        a.owned = false;
+        return NS_OK;
+    }
+    if (hipMalloc(&a.dev, n * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return NS_ERR_NOMEM;
+    }
+    a.owned = true;
+    if (copy_in) HIPCHK(hipMemcpyAsync(a.dev, p, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    return NS_OK;
+}
+
+int arg_out(ns_ctx *c, Arg &a) {
+    if (a.owned) HIPCHK(hipMemcpyAsync(a.user, a.dev, a.n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    return NS_OK;
+}
+
+void arg_free(Arg &a) {
+    if (a.owned && a.dev) (void)hipFree(a.dev);
+    a.dev = nullptr;
+}
+
+struct ArgList {
+    std::vector<Arg> v;
+    ~ArgList() {
+        for (auto &a : v) arg_free(a);
+    }
+    Arg &add() {
+        v.emplace_back();
+        return v.back();
+    }
+};
+
+bool valid_cfg(const ns_config *f) {
+    if (!f) return false;
+    if (f->nx < 2 || f->ny < 2 || f->nz < 2) return false;
+    if (!(f->dx > 0) || !(f->dy > 0) || !(f->dz > 0) || !(f->dt > 0) || !(f->nu >= 0)) return false;
+    if (f->scheme < NS_EULER || f->scheme > NS_AB2_CN) return false;
+    for (int a = 0; a < 3; a++) {
+        if (f->bc[a] < 1 || f->bc[a] > 4 || f->bc[a] == 3) return false;
+        if (f->pbc[a] < 1 || f->pbc[a] > 4) return false;
+    }
+    if (f->dp_flag != 0 && f->dp_flag != 1) return false;
+    return true;
+}
+
+inline size_t comp_size(const Grid &g, int c) { return c == 0 ? g.nu() : c == 1 ? g.nv() : c == 2 ? g.nw() : g.np(); }
+
+int run_poisson(ns_ctx *c, const double *us, const double *vs, const double *ws, const double *rhs_in,
+                const double *planes, const double *mean_dev, double mean_host, double *dp_out, double *sol_out,
+                hipEvent_t *ev = nullptr) {
+    const ns_config &f = c->cfg;
+    if (!c->poisson_ok) return NS_ERR_UNSUPPORTED;
+    if (!c->pp.dct)
+        poisson_fft_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, c->spec, dp_out, sol_out, c->st, ev);
+    else
+        poisson_dct_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, planes, mean_dev, mean_host,
+                          c->work, dp_out, sol_out, c->st, ev);
+    return NS_OK;
+}
+
+// one time step on the context stream; ev (optional) receives NS_NPHASES+1
+// events: before the predictor and after each phase
+int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
+    const ns_config &f = c->cfg;
+    const Grid &g = c->g;
+    if (ev) (void)hipEventRecord(ev[0], c->st);
+    launch_predictor(f, c->first, c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz, c->us, c->vs, c->ws, c->st);
+    if (ev) (void)hipEventRecord(ev[1], c->st);
+    if (f.scheme != NS_AB2_CN) {
+        launch_vel_bc(g, f.bc, c->planes, c->us, c->vs, c->ws, c->st);
+    } else {
+        for (int dir = 1; dir <= 3; dir++) {
+            launch_vel_bc_cn2(g, f.bc[dir - 1], dir, c->planes, c->us, c->vs, c->ws, c->st);
+            launch_adi_sweep(g, dir, c->cn[dir - 1][0], c->cn[dir - 1][1], c->us, c->vs, c->ws, c->st);
+        }
+    }
+    if (ev) (void)hipEventRecord(ev[2], c->st);
+    int rc = run_poisson(c, c->us, c->vs, c->ws, nullptr, c->planes, c->planes + c->L.off[24], 0.0, c->dp, nullptr,
+                         ev ? ev + 3 : nullptr);
+    if (rc) return rc;
+    launch_pr_bc(g, f.pbc, c->planes, f.dx, f.dy, f.dz, c->dp, c->st);
+    if (ev) (void)hipEventRecord(ev[8], c->st);
+    const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
+    launch_project(g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
+                   c->st);
+    if (ev) (void)hipEventRecord(ev[9], c->st);
+    c->first = 0;
+    return NS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+long ns_plane_set_length(int nx, int ny, int nz) { return plane_layout(nx, ny, nz).total; }
+
+int ns_create(const ns_config *cfg, ns_ctx **out) {
+    if (!out) return NS_ERR_INVALID;
+    *out = nullptr;
+    if (!valid_cfg(cfg)) return NS_ERR_INVALID;
+    ns_ctx *c = new ns_ctx();
+    c->cfg = *cfg;
+    c->g = Grid{cfg->nx, cfg->ny, cfg->nz};
+    c->L = plane_layout(cfg->nx, cfg->ny, cfg->nz);
+    int rc = NS_OK;
+    auto fail = [&](int r) {
+        ns_destroy(c);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        c->st = nullptr;
+        return fail(NS_ERR_HIP);
+    }
+    const Grid &g = c->g;
+    if ((rc = dalloc(c, &c->u, g.nu())) || (rc = dalloc(c, &c->v, g.nv())) || (rc = dalloc(c, &c->w, g.nw())) ||
+        (rc = dalloc(c, &c->p, g.np())) || (rc = dalloc(c, &c->hx, g.nu())) || (rc = dalloc(c, &c->hy, g.nv())) ||
+        (rc = dalloc(c, &c->hz, g.nw())) || (rc = dalloc(c, &c->us, g.nu())) || (rc = dalloc(c, &c->vs, g.nv())) ||
+        (rc = dalloc(c, &c->ws, g.nw())) || (rc = dalloc(c, &c->dp, g.np())) ||
+        (rc = dalloc(c, &c->planes, (size_t)c->L.total)) || (rc = dalloc(c, &c->partial, (size_t)4096)) ||
+        (rc = dalloc(c, &c->counter, (size_t)1)))
+        return fail(rc);
+    const bool allp = cfg->pbc[0] == 1 && cfg->pbc[1] == 1 && cfg->pbc[2] == 1;
+    const bool nonp = cfg->pbc[0] != 1 && cfg->pbc[1] != 1 && cfg->pbc[2] != 1;
+    const bool dct_ok = nonp && cfg->pbc[0] != 4 && cfg->pbc[1] != 4 && cfg->pbc[2] != 4;
+    c->poisson_ok = allp || dct_ok;
+    if (c->poisson_ok) {
+        c->pp.dct = allp ? 0 : 1;
+        for (int a = 0; a < 3; a++) c->pp.pbc[a] = cfg->pbc[a];
+        if ((rc = make_fft_plan(c, cfg->nx, c->pp.px)) || (rc = make_fft_plan(c, cfg->ny, c->pp.py)) ||
+            (rc = make_fft_plan(c, cfg->nz, c->pp.pz)))
+            return fail(rc);
+        if ((rc = make_eigs(c))) return fail(rc);
+        if (allp) {
+            if ((rc = dalloc(c, &c->spec, (size_t)(cfg->nx / 2 + 1) * cfg->ny * cfg->nz))) return fail(rc);
+        } else {
+            if ((rc = make_quarter_wave(c, cfg->nx, &c->pp.wq_x)) || (rc = make_quarter_wave(c, cfg->ny, &c->pp.wq_y)) ||
+                (rc = make_quarter_wave(c, cfg->nz, &c->pp.wq_z)))
+                return fail(rc);
+        }
+    }
+    if ((rc = dalloc(c, &c->work, g.ni()))) return fail(rc);
+    if (cfg->scheme == NS_AB2_CN) {
+        for (int ax = 0; ax < 3; ax++)
+            for (int t = 0; t < 2; t++)
+                if ((rc = make_line_sys(c, ax, t, c->cn[ax][t]))) return fail(rc);
+    }
+    // zero everything so unset ghosts and history read as the reference's zero-initialised arrays
+    HIPCHK(hipMemsetAsync(c->u, 0, g.nu() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->v, 0, g.nv() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->w, 0, g.nw() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->p, 0, g.np() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->hx, 0, g.nu() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->hy, 0, g.nv() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->hz, 0, g.nw() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->us, 0, g.nu() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->vs, 0, g.nv() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->ws, 0, g.nw() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->dp, 0, g.np() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->planes, 0, (size_t)c->L.total * 8, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *out = c;
+    return NS_OK;
+}
+
+int ns_destroy(ns_ctx *c) {
+    if (!c) return NS_ERR_INVALID;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void *q : c->allocs) (void)hipFree(q);
+    if (c->snaps) (void)hipFree(c->snaps);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return NS_OK;
+}
+
+int ns_synchronize(ns_ctx *c) {
+    if (!c) return NS_ERR_INVALID;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_upload_state(ns_ctx *c, const double *u, const double *v, const double *w, const double *p, const double *hx,
+                    const double *hy, const double *hz, int first) {
+    if (!c || !u || !v || !w) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    HIPCHK(hipMemcpyAsync(c->u, u, g.nu() * 8, hipMemcpyDefault, c->st));
+    HIPCHK(hipMemcpyAsync(c->v, v, g.nv() * 8, hipMemcpyDefault, c->st));
+    HIPCHK(hipMemcpyAsync(c->w, w, g.nw() * 8, hipMemcpyDefault, c->st));
+    if (p) HIPCHK(hipMemcpyAsync(c->p, p, g.np() * 8, hipMemcpyDefault, c->st));
+    else HIPCHK(hipMemsetAsync(c->p, 0, g.np() * 8, c->st));
+    if (hx) HIPCHK(hipMemcpyAsync(c->hx, hx, g.nu() * 8, hipMemcpyDefault, c->st));
+    else HIPCHK(hipMemsetAsync(c->hx, 0, g.nu() * 8, c->st));
+    if (hy) HIPCHK(hipMemcpyAsync(c->hy, hy, g.nv() * 8, hipMemcpyDefault, c->st));
+    else HIPCHK(hipMemsetAsync(c->hy, 0, g.nv() * 8, c->st));
+    if (hz) HIPCHK(hipMemcpyAsync(c->hz, hz, g.nw() * 8, hipMemcpyDefault, c->st));
+    else HIPCHK(hipMemsetAsync(c->hz, 0, g.nw() * 8, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->first = first ? 1 : 0;
+    return NS_OK;
+}
+
+int ns_download_state(ns_ctx *c, double *u, double *v, double *w, double *p, double *hx, double *hy, double *hz) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    if (u) HIPCHK(hipMemcpyAsync(u, c->u, g.nu() * 8, hipMemcpyDefault, c->st));
+    if (v) HIPCHK(hipMemcpyAsync(v, c->v, g.nv() * 8, hipMemcpyDefault, c->st));
+    if (w) HIPCHK(hipMemcpyAsync(w, c->w, g.nw() * 8, hipMemcpyDefault, c->st));
+    if (p) HIPCHK(hipMemcpyAsync(p, c->p, g.np() * 8, hipMemcpyDefault, c->st));
+    if (hx) HIPCHK(hipMemcpyAsync(hx, c->hx, g.nu() * 8, hipMemcpyDefault, c->st));
+    if (hy) HIPCHK(hipMemcpyAsync(hy, c->hy, g.nv() * 8, hipMemcpyDefault, c->st));
+    if (hz) HIPCHK(hipMemcpyAsync(hz, c->hz, g.nw() * 8, hipMemcpyDefault, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_download_work(ns_ctx *c, double *u_star, double *v_star, double *w_star, double *dp) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    if (u_star) HIPCHK(hipMemcpyAsync(u_star, c->us, g.nu() * 8, hipMemcpyDefault, c->st));
+    if (v_star) HIPCHK(hipMemcpyAsync(v_star, c->vs, g.nv() * 8, hipMemcpyDefault, c->st));
+    if (w_star) HIPCHK(hipMemcpyAsync(w_star, c->ws, g.nw() * 8, hipMemcpyDefault, c->st));
+    if (dp) {
+        if (c->cfg.flags & NS_FLAG_GAUGE_CORNER) {
+            double *tmp = nullptr;
+            HIPCHK(hipMalloc(&tmp, g.np() * 8));
+            launch_sub_gauge(c->dp, c->dp, tmp, g.np(), c->st);
+            hipError_t e = hipMemcpyAsync(dp, tmp, g.np() * 8, hipMemcpyDefault, c->st);
+            (void)hipStreamSynchronize(c->st);
+            (void)hipFree(tmp);
+            HIPCHK(e);
+        } else {
+            HIPCHK(hipMemcpyAsync(dp, c->dp, g.np() * 8, hipMemcpyDefault, c->st));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_state_pointers(ns_ctx *c, double **p) {
+    if (!c || !p) return NS_ERR_INVALID;
+    double *q[11] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz, c->us, c->vs, c->ws, c->dp};
+    for (int i = 0; i < 11; i++) p[i] = q[i];
+    return NS_OK;
+}
+
+int ns_set_bc_planes(ns_ctx *c, const double *planes) {
+    if (!c || !planes) return NS_ERR_INVALID;
+    HIPCHK(hipMemcpyAsync(c->planes, planes, (size_t)c->L.total * 8, hipMemcpyDefault, c->st));
+    return NS_OK;
+}
+
+int ns_bc_interp(ns_ctx *c, const double *a, const double *b, double theta) {
+    if (!c || !a || !b) return NS_ERR_INVALID;
+    ArgList args;
+    Arg &A = args.add();
+    int rc = arg_in(c, A, a, (size_t)c->L.total, true);
+    if (rc) return rc;
+    Arg &B = args.add();
+    rc = arg_in(c, B, b, (size_t)c->L.total, true);
+    if (rc) return rc;
+    launch_axpby(A.dev, B.dev, theta, c->planes, c->L.total, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_step(ns_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return NS_ERR_INVALID;
+    for (int s = 0; s < nsteps; s++) {
+        int rc = one_step(c);
+        if (rc) return rc;
+    }
+    if (hipGetLastError() != hipSuccess) return NS_ERR_HIP;
+    return NS_OK;
+}
+
+int ns_step_profiled(ns_ctx *c, int nsteps, double *phase_ms) {
+    if (!c || nsteps < 0 || !phase_ms) return NS_ERR_INVALID;
+    hipEvent_t ev[NS_NPHASES + 1];
+    for (int i = 0; i <= NS_NPHASES; i++) HIPCHK(hipEventCreate(&ev[i]));
+    for (int i = 0; i < NS_NPHASES; i++) phase_ms[i] = 0.0;
+    int rc = NS_OK;
+    for (int s = 0; s < nsteps && rc == NS_OK; s++) {
+        rc = one_step(c, ev);
+        if (rc) break;
+        if (hipEventSynchronize(ev[NS_NPHASES]) != hipSuccess) { rc = NS_ERR_HIP; break; }
+        for (int i = 0; i < NS_NPHASES; i++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) { rc = NS_ERR_HIP; break; }
+            phase_ms[i] += ms;
+        }
+    }
+    for (int i = 0; i <= NS_NPHASES; i++) (void)hipEventDestroy(ev[i]);
+    (void)hipGetLastError();
+    return rc;
+}
+
+int ns_resim_advance(ns_ctx *c, const double *snapshots, int nsnap, double snap_dt, double t0, int nsteps) {
+    if (!c || !snapshots || nsnap < 2 || !(snap_dt > 0) || nsteps < 0) return NS_ERR_INVALID;
+    const size_t len = (size_t)c->L.total;
+    const double *base = snapshots;
+    if (!is_dev_ptr(snapshots)) {
+        size_t need = len * nsnap;
+        if (c->snaps_len < need) {
+            if (c->snaps) (void)hipFree(c->snaps);
+            c->snaps = nullptr;
+            c->snaps_len = 0;
+            HIPCHK(hipMalloc(&c->snaps, need * 8));
+            c->snaps_len = need;
+        }
+        HIPCHK(hipMemcpyAsync(c->snaps, snapshots, need * 8, hipMemcpyHostToDevice, c->st));
+        base = c->snaps;
+    }
+    for (int m = 0; m < nsteps; m++) {
+        double t = t0 + (m + 1) * c->cfg.dt;
+        int s = (int)floor(t / snap_dt);
+        if (s < 0) s = 0;
+        if (s > nsnap - 2) s = nsnap - 2;
+        double th = (t - s * snap_dt) / snap_dt;
+        launch_axpby(base + len * s, base + len * (s + 1), th, c->planes, (long)len, c->st);
+        int rc = one_step(c);
+        if (rc) return rc;
+    }
+    if (hipGetLastError() != hipSuccess) return NS_ERR_HIP;
+    return NS_OK;
+}
+
+int ns_max_div(ns_ctx *c, double *out) {
+    if (!c || !out) return NS_ERR_INVALID;
+    const ns_config &f = c->cfg;
+    int nb = launch_max_div(c->g, f.dx, f.dy, f.dz, c->u, c->v, c->w, c->partial, c->st);
+    std::vector<double> h(nb);
+    HIPCHK(hipMemcpyAsync(h.data(), c->partial, nb * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double m = 0;
+    for (double x : h) m = x > m ? x : m;
+    *out = m;
+    return NS_OK;
+}
+
+int ns_check_finite(ns_ctx *c) {
+    if (!c) return NS_ERR_INVALID;
+    HIPCHK(hipMemsetAsync(c->counter, 0, sizeof(unsigned int), c->st));
+    launch_count_nonfinite(c->u, c->g.nu(), c->counter, c->st);
+    launch_count_nonfinite(c->v, c->g.nv(), c->counter, c->st);
+    launch_count_nonfinite(c->w, c->g.nw(), c->counter, c->st);
+    launch_count_nonfinite(c->p, c->g.np(), c->counter, c->st);
+    unsigned int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->counter, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return h ? NS_ERR_NONFINITE : NS_OK;
+}
+
+// ---------------- mirrors ----------------
+
+int ns_cal_conv(ns_ctx *c, const double *u, const double *v, const double *w, int bc_x, int bc_y, int bc_z, double dx,
+                double dy, double dz, double *conv_x, double *conv_y, double *conv_z) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    int bc[3] = {bc_x, bc_y, bc_z};
+    for (int a = 0; a < 3; a++)
+        if (bc[a] < 1 || bc[a] > 4) return NS_ERR_INVALID;
+    size_t m[3];
+    for (int k = 0; k < 3; k++) {
+        size_t s = g.ni();
+        int n = (k == 0) ? g.nx : (k == 1) ? g.ny : g.nz;
+        m[k] = (bc[k] == 1) ? s : s / n * (n - 1);
+    }
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w, g.nw(), true))) return rc;
+    Arg &ox = A.add(); if ((rc = arg_in(c, ox, conv_x, m[0], false))) return rc;
+    Arg &oy = A.add(); if ((rc = arg_in(c, oy, conv_y, m[1], false))) return rc;
+    Arg &oz = A.add(); if ((rc = arg_in(c, oz, conv_z, m[2], false))) return rc;
+    launch_conv_diff(g, bc, dx, dy, dz, 0, au.dev, av.dev, aw.dev, ox.dev, oy.dev, oz.dev, c->st);
+    if ((rc = arg_out(c, ox)) || (rc = arg_out(c, oy)) || (rc = arg_out(c, oz))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_cal_diff(ns_ctx *c, const double *u, const double *v, const double *w, int bc_x, int bc_y, int bc_z, double dx2,
+                double dy2, double dz2, double *diff_x, double *diff_y, double *diff_z) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    int bc[3] = {bc_x, bc_y, bc_z};
+    for (int a = 0; a < 3; a++)
+        if (bc[a] < 1 || bc[a] > 4) return NS_ERR_INVALID;
+    size_t m[3];
+    for (int k = 0; k < 3; k++) {
+        size_t s = g.ni();
+        int n = (k == 0) ? g.nx : (k == 1) ? g.ny : g.nz;
+        m[k] = (bc[k] == 1) ? s : s / n * (n - 1);
+    }
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w, g.nw(), true))) return rc;
+    Arg &ox = A.add(); if ((rc = arg_in(c, ox, diff_x, m[0], false))) return rc;
+    Arg &oy = A.add(); if ((rc = arg_in(c, oy, diff_y, m[1], false))) return rc;
+    Arg &oz = A.add(); if ((rc = arg_in(c, oz, diff_z, m[2], false))) return rc;
+    launch_conv_diff(g, bc, dx2, dy2, dz2, 1, au.dev, av.dev, aw.dev, ox.dev, oy.dev, oz.dev, c->st);
+    if ((rc = arg_out(c, ox)) || (rc = arg_out(c, oy)) || (rc = arg_out(c, oz))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_vel_bc_staggered(ns_ctx *c, double *u_star, double *v_star, double *w_star, const double *planes, int bc_x,
+                        int bc_y, int bc_z) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    int bc[3] = {bc_x, bc_y, bc_z};
+    for (int a = 0; a < 3; a++) {
+        if (bc[a] < 1 || bc[a] > 4) return NS_ERR_INVALID;
+        if (bc[a] != 1 && bc[a] != 3 && !planes) return NS_ERR_INVALID;
+    }
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_star, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_star, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_star, g.nw(), true))) return rc;
+    const double *pl = nullptr;
+    if (planes) {
+        Arg &ap = A.add();
+        if ((rc = arg_in(c, ap, planes, (size_t)c->L.total, true))) return rc;
+        pl = ap.dev;
+    }
+    launch_vel_bc(g, bc, pl, au.dev, av.dev, aw.dev, c->st);
+    if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_vel_bc_staggered_cn2(ns_ctx *c, double *u_star, double *v_star, double *w_star, const double *planes, int bc,
+                            int dir) {
+    if (!c || dir < 1 || dir > 3 || bc < 1 || bc > 4) return NS_ERR_INVALID;
+    if ((bc == 2 || bc == 4) && !planes) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_star, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_star, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_star, g.nw(), true))) return rc;
+    const double *pl = nullptr;
+    if (planes) {
+        Arg &ap = A.add();
+        if ((rc = arg_in(c, ap, planes, (size_t)c->L.total, true))) return rc;
+        pl = ap.dev;
+    }
+    launch_vel_bc_cn2(g, bc, dir, pl, au.dev, av.dev, aw.dev, c->st);
+    if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_adi_sweep(ns_ctx *c, double *u_star, double *v_star, double *w_star, int dir) {
+    if (!c || dir < 1 || dir > 3) return NS_ERR_INVALID;
+    if (c->cfg.scheme != NS_AB2_CN) return NS_ERR_UNSUPPORTED;
+    const Grid &g = c->g;
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_star, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_star, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_star, g.nw(), true))) return rc;
+    launch_adi_sweep(g, dir, c->cn[dir - 1][0], c->cn[dir - 1][1], au.dev, av.dev, aw.dev, c->st);
+    if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_get_velpr_bc(ns_ctx *c, const double *u_bc, const double *v_bc, const double *w_bc, const double *p_bc, double dt,
+                    int dp_flag, int using_ustar, const double *p_previous, double *planes) {
+    if (!c || !planes) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    const int corr = !(dp_flag || using_ustar);
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_bc, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_bc, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_bc, g.nw(), true))) return rc;
+    Arg &ap = A.add(); if ((rc = arg_in(c, ap, p_bc, g.np(), true))) return rc;
+    const double *pprev = nullptr;
+    if (corr) {
+        Arg &aq = A.add();
+        if ((rc = arg_in(c, aq, p_previous, g.np(), true))) return rc;
+        pprev = aq.dev;
+    }
+    Arg &ao = A.add(); if ((rc = arg_in(c, ao, planes, (size_t)c->L.total, true))) return rc;
+    launch_get_velpr_bc(g, f.bc, f.pbc, f.dx, f.dy, f.dz, dt, corr, au.dev, av.dev, aw.dev, ap.dev, pprev, ao.dev,
+                        c->st);
+    if ((rc = arg_out(c, ao))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_get_pr_bc(ns_ctx *c, const double *p_bc, double *planes) {
+    if (!c || !planes) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    ArgList A;
+    int rc;
+    Arg &ap = A.add(); if ((rc = arg_in(c, ap, p_bc, g.np(), true))) return rc;
+    Arg &ao = A.add(); if ((rc = arg_in(c, ao, planes, (size_t)c->L.total, true))) return rc;
+    launch_get_pr_bc(g, f.pbc, f.dx, f.dy, f.dz, ap.dev, ao.dev, c->st);
+    if ((rc = arg_out(c, ao))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_divergence(ns_ctx *c, const double *u_star, const double *v_star, const double *w_star, double dt, double *rhs) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_star, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_star, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_star, g.nw(), true))) return rc;
+    Arg &ao = A.add(); if ((rc = arg_in(c, ao, rhs, g.ni(), false))) return rc;
+    launch_divergence(g, f.dx, f.dy, f.dz, dt, au.dev, av.dev, aw.dev, ao.dev, c->st);
+    if ((rc = arg_out(c, ao))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_pr_bc_staggered_modify_rhs(ns_ctx *c, double *rhs, const double *planes) {
+    if (!c || !planes) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    ArgList A;
+    int rc;
+    Arg &ar = A.add(); if ((rc = arg_in(c, ar, rhs, g.ni(), true))) return rc;
+    Arg &ap = A.add(); if ((rc = arg_in(c, ap, planes, (size_t)c->L.total, true))) return rc;
+    launch_modify_rhs(g, f.pbc, ap.dev, f.dx, f.dy, f.dz, ar.dev, c->st);
+    if ((rc = arg_out(c, ar))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_poisson_fft(ns_ctx *c, const double *rhs, double *sol) {
+    if (!c) return NS_ERR_INVALID;
+    if (!c->poisson_ok || c->pp.dct) return NS_ERR_UNSUPPORTED;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    ArgList A;
+    int rc;
+    Arg &ar = A.add(); if ((rc = arg_in(c, ar, rhs, g.ni(), true))) return rc;
+    Arg &ao = A.add(); if ((rc = arg_in(c, ao, sol, g.ni(), false))) return rc;
+    poisson_fft_solve(g, c->pp, f.dx, f.dy, f.dz, f.dt, nullptr, nullptr, nullptr, ar.dev, c->spec, nullptr, ao.dev,
+                      c->st);
+    if ((rc = arg_out(c, ao))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_dct_poisson_solver(ns_ctx *c, double *rhs, double mean_dp) {
+    if (!c) return NS_ERR_INVALID;
+    if (!c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    ArgList A;
+    int rc;
+    Arg &ar = A.add(); if ((rc = arg_in(c, ar, rhs, g.ni(), true))) return rc;
+    poisson_dct_solve(g, c->pp, f.dx, f.dy, f.dz, f.dt, nullptr, nullptr, nullptr, ar.dev, nullptr, nullptr, mean_dp,
+                      c->work, nullptr, ar.dev, c->st);
+    if ((rc = arg_out(c, ar))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_pr_bc_staggered(ns_ctx *c, double *dp, const double *planes) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    bool need = f.pbc[0] != 1 || f.pbc[1] != 1 || f.pbc[2] != 1;
+    if (need && !planes) return NS_ERR_INVALID;
+    ArgList A;
+    int rc;
+    Arg &ad = A.add(); if ((rc = arg_in(c, ad, dp, g.np(), true))) return rc;
+    const double *pl = c->planes;
+    if (planes) {
+        Arg &ap = A.add();
+        if ((rc = arg_in(c, ap, planes, (size_t)c->L.total, true))) return rc;
+        pl = ap.dev;
+    }
+    launch_pr_bc(g, f.pbc, pl, f.dx, f.dy, f.dz, ad.dev, c->st);
+    if ((rc = arg_out(c, ad))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_project(ns_ctx *c, const double *u_star, const double *v_star, const double *w_star, const double *dp, double dt,
+               int dp_flag, double *u, double *v, double *w, double *p) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    ArgList A;
+    int rc;
+    Arg &a1 = A.add(); if ((rc = arg_in(c, a1, u_star, g.nu(), true))) return rc;
+    Arg &a2 = A.add(); if ((rc = arg_in(c, a2, v_star, g.nv(), true))) return rc;
+    Arg &a3 = A.add(); if ((rc = arg_in(c, a3, w_star, g.nw(), true))) return rc;
+    Arg &a4 = A.add(); if ((rc = arg_in(c, a4, dp, g.np(), true))) return rc;
+    Arg &o1 = A.add(); if ((rc = arg_in(c, o1, u, g.nu(), false))) return rc;
+    Arg &o2 = A.add(); if ((rc = arg_in(c, o2, v, g.nv(), false))) return rc;
+    Arg &o3 = A.add(); if ((rc = arg_in(c, o3, w, g.nw(), false))) return rc;
+    Arg &o4 = A.add(); if ((rc = arg_in(c, o4, p, g.np(), true))) return rc;
+    launch_project(g, f.dx, f.dy, f.dz, dt, dp_flag, a1.dev, a2.dev, a3.dev, a4.dev, nullptr, o1.dev, o2.dev, o3.dev,
+                   o4.dev, c->st);
+    if ((rc = arg_out(c, o1)) || (rc = arg_out(c, o2)) || (rc = arg_out(c, o3)) || (rc = arg_out(c, o4))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Internal declarations shared by the HIP translation units of libns_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ns_hip.h"
+
+// 1-based Fortran index into an (n1,n2,*) column-major array
+#define FIX(i, j, k, n1, n2) ((size_t)((i) - 1) + (size_t)(n1) * ((size_t)((j) - 1) + (size_t)(n2) * (size_t)((k) - 1)))
+
+namespace ns {
+
+struct Grid {
+    int nx, ny, nz;
+    __host__ __device__ size_t nu() const { return (size_t)(nx + 1) * (ny + 2) * (nz + 2); }
+    __host__ __device__ size_t nv() const { return (size_t)(nx + 2) * (ny + 1) * (nz + 2); }
+    __host__ __device__ size_t nw() const { return (size_t)(nx + 2) * (ny + 2) * (nz + 1); }
+    __host__ __device__ size_t np() const { return (size_t)(nx + 2) * (ny + 2) * (nz + 2); }
+    __host__ __device__ size_t ni() const { return (size_t)nx * ny * nz; }
+};
+
+// component dims: c=0 u, 1 v, 2 w, 3 p
+__host__ __device__ inline void comp_dims(const Grid &g, int c, int n[3]) {
+    n[0] = g.nx + 2; n[1] = g.ny + 2; n[2] = g.nz + 2;
+    if (c < 3) n[c] -= 1;
+}
+
+// packed plane-set layout (18 velocity + 6 pressure planes + mean_dp)
+struct PlaneLayout {
+    long off[25];
+    long size[24];
+    long total;  // including the trailing scalar
+};
+PlaneLayout plane_layout(int nx, int ny, int nz);
+
+// Stockham plan for one transform length, radices in {2,3,4,5,7,8}
+struct FFTPlan {
+    int n;
+    int nst;
+    int radix[10];
+    const double2 *tw;  // device table, tw[m] = exp(-2 pi i m / n), m = 0..n-1
+};
+
+// Tridiagonal line system of one ADI direction and one matrix type
+// (A: the face-normal component, n+1 long; B: the other two, n+2 long).
+struct LineSys {
+    int N;        // line length (n+1 or n+2)
+    int periodic; // 1: reduced cyclic system on m = n unknowns
+    int m;        // number of unknowns solved by the Thomas sweep
+    int off;      // index (0-based) of the first unknown in the line
+    double a;     // CN off-diagonal magnitude: 0.5*dt*nu/h^2
+    int quirk;    // periodic A-type with mat_CN's row 1
+    // Thomas coefficients for the m unknowns: d'_i = (r_i - lo_i d'_{i-1}) * inv_i ; x_i = d'_i - cp_i x_{i+1}
+    const double *lo, *inv, *cp;
+    // Sherman-Morrison correction (periodic only): x -= f * z, f = (y_0 + s*y_{m-1}) / den
+    const double *z;
+    double s, den;
+};
+
+struct Ctx;
+
+// ---- stencil / BC / projection launchers (ns_stencil.hip) ----
+void launch_predictor(const ns_config &c, int first, const double *u, const double *v, const double *w, const double *p,
+                      double *hx, double *hy, double *hz, double *us, double *vs, double *ws, hipStream_t st);
+void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff,
+                      const double *u, const double *v, const double *w, double *ox, double *oy, double *oz,
+                      hipStream_t st);
+void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w, hipStream_t st);
+void launch_vel_bc_cn2(const Grid &g, int bc, int dir, const double *planes, double *u, double *v, double *w,
+                       hipStream_t st);
+void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,
+                  hipStream_t st);
+void launch_modify_rhs(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz,
+                       double *rhs, hipStream_t st);
+void launch_divergence(const Grid &g, double dx, double dy, double dz, double dt, const double *us, const double *vs,
+                       const double *ws, double *rhs, hipStream_t st);
+// gauge: if non-null, dp values are taken as dp - *gauge (the corner gauge)
+void launch_project(const Grid &g, double dx, double dy, double dz, double dt, int dp_flag, const double *us,
+                    const double *vs, const double *ws, const double *dp, const double *gauge, double *u, double *v,
+                    double *w, double *p, hipStream_t st);
+void launch_get_velpr_bc(const Grid &g, const int bc[3], const int pbc[3], double dx, double dy, double dz, double dt,
+                         int corr, const double *ubc, const double *vbc, const double *wbc, const double *pbc_arr,
+                         const double *pprev, double *planes, hipStream_t st);
+void launch_get_pr_bc(const Grid &g, const int pbc[3], double dx, double dy, double dz, const double *p,
+                      double *planes, hipStream_t st);
+void launch_axpby(const double *a, const double *b, double th, double *out, long n, hipStream_t st);
+void launch_copy_interior(const Grid &g, const double *sol, double *dp, hipStream_t st);
+void launch_sub_gauge(const double *src, const double *gauge, double *dst, size_t n, hipStream_t st);
+// block-wise partial maxima of |div| into partial[], returns number of partials
+int launch_max_div(const Grid &g, double dx, double dy, double dz, const double *u, const double *v, const double *w,
+                   double *partial, hipStream_t st);
+int max_div_partials(const Grid &g);
+void launch_count_nonfinite(const double *a, size_t n, unsigned int *count, hipStream_t st);
+
+// ---- Poisson (ns_poisson.hip) ----
+struct PoissonPlans {
+    FFTPlan px, py, pz;  // transform lengths nx, ny, nz
+    int dct;             // 0: periodic FFT; 1: DCT/DST family
+    int pbc[3];
+    const double *ex, *ey, *ez;  // per-axis eigenvalue parts (FFT: nx/2+1 / ny / nz; DCT: nx/ny/nz)
+    const double2 *wq_x, *wq_y, *wq_z;  // Makhoul quarter-wave twiddles exp(-i pi k/(2n)), DCT only
+};
+// ev (optional): 5 events recorded after the five passes
+void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
+                       const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr);
+void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
+                       const double *us, const double *vs, const double *ws, const double *rhs_in,
+                       const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr);
+
+// ---- ADI (ns_adi.hip) ----
+void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
+                      hipStream_t st);
+
+}  // namespace ns

@@ -1,0 +1,711 @@
+// Pressure Poisson solves for gfx950, FP64.
+//
+// Periodic (big_periodic_3D.f90:215-256,526-548): r2c forward transform of
+// the divergence (unscaled), division by
+//   eig = -4*((sin(pi*kx/nx)/dx)^2 + (sin(pi*ky/ny)/dy)^2 + (sin(pi*kz/nz)/dz)^2),
+// mode (0,0,0) -> 0, c2r backward transform scaled by 1/(nx*ny*nz).
+// Neumann/Dirichlet walls (DCT_poisson_solver, NS_functions.f90:1505-1630):
+// per-axis DCT-II (pbc=3) or DST-II (pbc=2, sign flip + DCT-II + reversal,
+// :1522-1525), division by the eigenvalues of ...interBC_combine.f90:321-351,
+// zero mode := mean_dp*nx*ny*nz (:1577), exact inverse transforms.
+//
+// Structure: five line passes over the volume (x forward fused with the
+// divergence and the boundary RHS terms; y forward; z forward + modal
+// division + z inverse; y inverse; x inverse writing dp).  Each pass stages
+// L lines in LDS and runs a mixed-radix Stockham FFT (radices 2,3,4,5,7,8).
+// Real lines are transformed two at a time as one complex line.  The DCT is
+// Makhoul's (IEEE TASSP 28(1), 1980): even/odd reorder, complex FFT of length
+// n, quarter-wave twiddle.
+#include "ns_internal.h"
+
+namespace ns {
+
+namespace {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by sign*i
+__device__ __forceinline__ double2 cmuli(double2 a, int sign) {
+    return sign < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+// small DFTs: X[k] = sum_r v[r] exp(sign*2*pi*i*r*k/R)
+template <int R> struct SmallDFT;
+
+template <> struct SmallDFT<2> {
+    __device__ static void run(double2 *v, int) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+template <> struct SmallDFT<4> {
+    __device__ static void run(double2 *v, int s) {
+        double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        double2 t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), s);
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+template <> struct SmallDFT<8> {
+    __device__ static void run(double2 *v, int s) {
+        const double h = 0.70710678118654752440;
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        SmallDFT<4>::run(e, s);
+        SmallDFT<4>::run(o, s);
+        // o[k] *= W8^k, W8 = exp(s*2*pi*i/8)
+        double2 w1 = make_double2(h, s * h), w3 = make_double2(-h, s * h);
+        o[1] = cmul(o[1], w1);
+        o[2] = cmuli(o[2], s);
+        o[3] = cmul(o[3], w3);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+};
+// odd radices: direct sums with cos/sin(2*pi*m/R)
+template <int R> struct OddTab;
+template <> struct OddTab<3> {
+    __device__ static double c(int m) { const double t[3] = {1.0, -0.5, -0.5}; return t[m]; }
+    __device__ static double s(int m) { const double t[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676}; return t[m]; }
+};
+template <> struct OddTab<5> {
+    __device__ static double c(int m) {
+        const double t[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
+                             0.30901699437494742410};
+        return t[m];
+    }
+    __device__ static double s(int m) {
+        const double t[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
+                             -0.95105651629515357212};
+        return t[m];
+    }
+};
+template <> struct OddTab<7> {
+    __device__ static double c(int m) {
+        const double t[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429, -0.90096886790241912624,
+                             -0.90096886790241912624, -0.22252093395631440429, 0.62348980185873353053};
+        return t[m];
+    }
+    __device__ static double s(int m) {
+        const double t[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702, 0.43388373911755812048,
+                             -0.43388373911755812048, -0.97492791218182360702, -0.78183148246802980871};
+        return t[m];
+    }
+};
+template <int R> struct SmallDFTOdd {
+    __device__ static void run(double2 *v, int s) {
+        double2 out[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            double2 acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                int m = (r * k) % R;
+                double2 w = make_double2(OddTab<R>::c(m), s * OddTab<R>::s(m));
+                acc = cadd(acc, cmul(v[r], w));
+            }
+            out[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = out[k];
+    }
+};
+template <> struct SmallDFT<3> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<3>::run(v, s); } };
+template <> struct SmallDFT<5> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<5>::run(v, s); } };
+template <> struct SmallDFT<7> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<7>::run(v, s); } };
+
+// one Stockham stage over L lines of length N (line stride LS) in LDS
+template <int R>
+__device__ __forceinline__ void stage(const double2 *__restrict__ A, double2 *__restrict__ B, int N, int LS, int L,
+                                      int Ns, const double2 *__restrict__ tw, int sign) {
+    const int NR = N / R;
+    const int nb = L * NR;
+    const int tstep = N / (Ns * R);
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        int line = t / NR, j = t - line * NR;
+        const double2 *in = A + line * LS;
+        double2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = in[j + r * NR];
+        int jm = j % Ns;
+        if (jm) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                double2 w = tw[jm * r * tstep];
+                if (sign > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        SmallDFT<R>::run(v, sign);
+        int idxD = (j - jm) * R + jm;
+        double2 *out = B + line * LS;
+#pragma unroll
+        for (int r = 0; r < R; r++) out[idxD + r * Ns] = v[r];
+    }
+}
+
+// full transform; data starts in A, result pointer returned (A or B)
+__device__ double2 *fft_lds(double2 *A, double2 *B, const FFTPlan &pl, int LS, int L, int sign) {
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; s++) {
+        switch (pl.radix[s]) {
+            case 8: stage<8>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
+            case 4: stage<4>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
+            case 2: stage<2>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
+            case 3: stage<3>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
+            case 5: stage<5>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
+            case 7: stage<7>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
+        }
+        __syncthreads();
+        Ns *= pl.radix[s];
+        double2 *t = A;
+        A = B;
+        B = t;
+    }
+    return A;
+}
+
+struct DivSrc {
+    // divergence source: either rhs (interior array) or the staggered u*, v*, w*
+    const double *rhs;
+    const double *us, *vs, *ws;
+    double dx, dy, dz, dt;
+    // boundary RHS terms (pr_bc_staggered_modify_rhs), DCT family only
+    const double *planes;
+    PlaneLayout L;
+    int pbc[3];
+};
+
+// divergence RHS at interior cell (a,b,c), 0-based (big_periodic_3D.f90:498-501)
+__device__ __forceinline__ double div_at(const Grid &g, const DivSrc &S, int a, int b, int c) {
+    double s;
+    if (S.rhs) {
+        s = S.rhs[(size_t)a + (size_t)g.nx * ((size_t)b + (size_t)g.ny * c)];
+    } else {
+        const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+        const int A = a + 1, B = b + 1, C = c + 1;
+        s = (S.us[FIX(A + 1, B + 1, C + 1, U1, U2)] - S.us[FIX(A, B + 1, C + 1, U1, U2)]) / S.dx;
+        s = s + (S.vs[FIX(A + 1, B + 1, C + 1, V1, V2)] - S.vs[FIX(A + 1, B, C + 1, V1, V2)]) / S.dy;
+        s = s + (S.ws[FIX(A + 1, B + 1, C + 1, W1, W2)] - S.ws[FIX(A + 1, B + 1, C, W1, W2)]) / S.dz;
+        s = s / S.dt;
+    }
+    if (S.planes) {
+        // x, then y, then z terms, in the reference's order (NS_functions.f90:1362-1398)
+        const int id[3] = {a, b, c}, n[3] = {g.nx, g.ny, g.nz}, np_[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+        const double d[3] = {S.dx, S.dy, S.dz};
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            int pbc = S.pbc[ax];
+            if (pbc == 1) continue;
+            int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+            size_t pl = (size_t)(id[o1] + 1) + (size_t)np_[o1] * (size_t)(id[o2] + 1);
+            double dd = d[ax], d2 = d[ax] * d[ax];
+            if (id[ax] == 0) {
+                double b1 = S.planes[S.L.off[18 + 2 * ax] + pl];
+                if (pbc == 2) s = s - 2 * b1 / d2;
+                else if (pbc == 3) s = s + b1 / dd;
+                else if (pbc == 4) s = s - b1 / d2;
+            }
+            if (id[ax] == n[ax] - 1) {
+                double bn = S.planes[S.L.off[19 + 2 * ax] + pl];
+                if (pbc == 2) s = s - 2 * bn / d2;
+                else if (pbc == 3) s = s - bn / dd;
+                else if (pbc == 4) s = s - bn / d2;
+            }
+        }
+    }
+    return s;
+}
+
+struct PassParams {
+    Grid g;
+    FFTPlan pl;  // plan of this pass's axis
+    int L;       // complex lines per block
+    int LS;      // LDS line stride (complex)
+    int nxh;     // nx/2+1 (periodic)
+    DivSrc src;
+    double2 *spec;                          // periodic spectrum [z][y][kx]
+    double *work;                           // DCT work array [z][y][x]
+    const double *ex, *ey, *ez;             // eigenvalue parts
+    const double2 *wq;                      // Makhoul twiddles of this axis
+    int pbc;                                // this axis' pressure code (DCT)
+    const double *mean_dp_dev;              // DCT zero mode source (device) or null
+    double mean_dp;                         // host value if mean_dp_dev is null
+    double scale;                           // backward scale (periodic)
+    double *dp_out;                         // ghosted dp (interior written) or null
+    double *sol_out;                        // interior (nx,ny,nz) or null
+};
+
+// ---------------- periodic passes ----------------
+
+__global__ void __launch_bounds__(256) k_fft_x_fwd(PassParams P) {
+    extern __shared__ double2 smem[];
+    const Grid g = P.g;
+    const int N = g.nx, L = P.L, LS = P.LS;
+    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const long RL = (long)g.ny * g.nz;
+    const long rl0 = (long)blockIdx.x * 2 * L;
+    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
+        int rline = f / N, x = f - rline * N;
+        long rl = rl0 + rline;
+        double val = 0.0;
+        if (rl < RL) val = div_at(g, P.src, x, (int)(rl % g.ny), (int)(rl / g.ny));
+        double *slot = (double *)&A[(rline >> 1) * LS + x];
+        slot[rline & 1] = val;
+    }
+    __syncthreads();
+    double2 *Z = fft_lds(A, B, P.pl, LS, L, -1);
+    const int nxh = P.nxh;
+    for (int f = threadIdx.x; f < 2 * L * nxh; f += blockDim.x) {
+        int rline = f / nxh, kx = f - rline * nxh;
+        long rl = rl0 + rline;
+        if (rl >= RL) continue;
+        const double2 *z = Z + (rline >> 1) * LS;
+        double2 a = z[kx], b = z[(N - kx) % N];
+        double2 X;
+        if ((rline & 1) == 0) X = make_double2((a.x + b.x) * 0.5, (a.y - b.y) * 0.5);
+        else X = make_double2((a.y + b.y) * 0.5, (b.x - a.x) * 0.5);
+        P.spec[(size_t)rl * nxh + kx] = X;
+    }
+}
+
+// strided complex lines: along y (stride nxh) for fixed (kx,z), or along z
+// (stride nxh*ny) for fixed (kx,y); L consecutive kx per block
+template <int AXIS>
+__device__ __forceinline__ size_t spec_index(const Grid &g, int nxh, int kx, int pos, int other) {
+    if (AXIS == 1) return ((size_t)other * g.ny + pos) * nxh + kx;
+    return ((size_t)pos * g.ny + other) * nxh + kx;
+}
+
+template <int AXIS, int SIGN, int SOLVE>
+__global__ void __launch_bounds__(256) k_fft_yz(PassParams P) {
+    extern __shared__ double2 smem[];
+    const Grid g = P.g;
+    const int N = (AXIS == 1) ? g.ny : g.nz, L = P.L, LS = P.LS, nxh = P.nxh;
+    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const int ngrp = (nxh + L - 1) / L;
+    const int kx0 = (blockIdx.x % ngrp) * L;
+    const int other = blockIdx.x / ngrp;
+    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+        int l = f % L, pos = f / L;
+        int kx = kx0 + l;
+        A[l * LS + pos] = (kx < nxh) ? P.spec[spec_index<AXIS>(g, nxh, kx, pos, other)] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double2 *Z = fft_lds(A, B, P.pl, LS, L, SIGN);
+    if (SOLVE) {
+        // modal division (z pass): eig = -4*((ex + ey) + ez); mode (0,0,0) -> 0
+        for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+            int l = f / N, kz = f - l * N;
+            int kx = kx0 + l;
+            if (kx >= nxh) continue;
+            double2 *z = Z + l * LS + kz;
+            double e = P.ex[kx];
+            e = e + P.ey[other];
+            e = e + P.ez[kz];
+            e = -4.0 * e;
+            if (kx == 0 && other == 0 && kz == 0) *z = make_double2(0.0, 0.0);
+            else *z = make_double2(z->x / e, z->y / e);
+        }
+        __syncthreads();
+        double2 *other_buf = (Z == A) ? B : A;
+        Z = fft_lds(Z, other_buf, P.pl, LS, L, +1);
+    }
+    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+        int l = f % L, pos = f / L;
+        int kx = kx0 + l;
+        if (kx < nxh) P.spec[spec_index<AXIS>(g, nxh, kx, pos, other)] = Z[l * LS + pos];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fft_x_inv(PassParams P) {
+    extern __shared__ double2 smem[];
+    const Grid g = P.g;
+    const int N = g.nx, L = P.L, LS = P.LS, nxh = P.nxh;
+    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const long RL = (long)g.ny * g.nz;
+    const long rl0 = (long)blockIdx.x * 2 * L;
+    // stage the half spectra of the 2L real lines in B (line r at B + r*nxh)
+    for (int f = threadIdx.x; f < 2 * L * nxh; f += blockDim.x) {
+        int rline = f / nxh, kx = f - rline * nxh;
+        long rl = rl0 + rline;
+        double2 X = (rl < RL) ? P.spec[(size_t)rl * nxh + kx] : make_double2(0.0, 0.0);
+        if (kx == 0 || 2 * kx == N) X.y = 0.0;  // c2r ignores the imaginary parts of DC and Nyquist
+        B[rline * nxh + kx] = X;
+    }
+    __syncthreads();
+    // Z[m] = Xa[m] + i Xb[m] with Hermitian extension
+    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+        int l = f / N, m = f - l * N;
+        double2 a, b;
+        if (m < nxh) {
+            a = B[(2 * l) * nxh + m];
+            b = B[(2 * l + 1) * nxh + m];
+        } else {
+            a = B[(2 * l) * nxh + (N - m)];
+            b = B[(2 * l + 1) * nxh + (N - m)];
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        A[l * LS + m] = make_double2(a.x - b.y, a.y + b.x);
+    }
+    __syncthreads();
+    double2 *Z = fft_lds(A, B, P.pl, LS, L, +1);
+    const double sc = P.scale;
+    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
+        int rline = f / N, x = f - rline * N;
+        long rl = rl0 + rline;
+        if (rl >= RL) continue;
+        double2 z = Z[(rline >> 1) * LS + x];
+        double val = ((rline & 1) ? z.y : z.x) * sc;
+        int j = (int)(rl % g.ny), k = (int)(rl / g.ny);
+        if (P.dp_out) P.dp_out[FIX(x + 2, j + 2, k + 2, g.nx + 2, g.ny + 2)] = val;
+        if (P.sol_out) P.sol_out[(size_t)x + (size_t)g.nx * rl] = val;
+    }
+}
+
+// ---------------- DCT family passes ----------------
+// Makhoul reorder: v[m] = x[src(m)]
+__device__ __forceinline__ int mk_src(int m, int N) {
+    int h = (N + 1) / 2;
+    return (m < h) ? 2 * m : 2 * (N - 1 - m) + 1;
+}
+
+// forward DCT-II (pbc 3) or DST-II (pbc 2) of the pair of real lines held as
+// Z (after the forward FFT of the reordered data); writes X_a, X_b into out
+__device__ __forceinline__ void makhoul_post(const double2 *Z, int N, int k, const double2 *wq, double &xa, double &xb) {
+    double2 a = Z[k], b = Z[(N - k) % N];
+    // Va = (Z[k] + conj(Z[N-k]))/2, Vb = -i (Z[k] - conj(Z[N-k]))/2
+    double2 va = make_double2((a.x + b.x) * 0.5, (a.y - b.y) * 0.5);
+    double2 vb = make_double2((a.y + b.y) * 0.5, (b.x - a.x) * 0.5);
+    double c = wq[k].x, s = wq[k].y;  // exp(-i pi k/(2N)) = c - i s
+    xa = c * va.x + s * va.y;
+    xb = c * vb.x + s * vb.y;
+}
+
+// inverse: given X (length N, real) returns V[k] = exp(i pi k/(2N)) (X[k] - i X[N-k])
+__device__ __forceinline__ double2 makhoul_pre_inv(double xk, double xnk, int k, const double2 *wq) {
+    double c = wq[k].x, s = wq[k].y;
+    return make_double2(c * xk + s * xnk, s * xk - c * xnk);
+}
+
+// generic real-line gather/scatter for the DCT passes.
+// AXIS 0: lines along x, pairs of consecutive flattened (y,z) lines.
+// AXIS 1/2: lines along y/z, pairs of consecutive x at fixed z (AXIS 1) or y (AXIS 2).
+struct RealLines {
+    __device__ static void locate(const Grid &g, int AXIS, int blk, int L, int rline, long &base, long &stride,
+                                  bool &valid) {
+        if (AXIS == 0) {
+            long rl = (long)blk * 2 * L + rline;
+            valid = rl < (long)g.ny * g.nz;
+            base = rl * g.nx;
+            stride = 1;
+        } else {
+            int nxp = (g.nx + 1) / 2;               // pairs per row
+            int ngrp = (nxp + L - 1) / L;
+            int other = blk / ngrp;
+            int x = (blk % ngrp) * 2 * L + rline;
+            valid = x < g.nx;
+            if (AXIS == 1) { base = (long)other * g.nx * g.ny + x; stride = g.nx; }
+            else { base = (long)other * g.nx + x; stride = (long)g.nx * g.ny; }
+        }
+    }
+};
+
+template <int AXIS>
+__device__ __forceinline__ int dct_n(const Grid &g) { return AXIS == 0 ? g.nx : AXIS == 1 ? g.ny : g.nz; }
+
+// forward DCT along AXIS; AXIS 0 reads the divergence (+ boundary RHS terms)
+template <int AXIS, int SOLVE>
+__global__ void __launch_bounds__(256) k_dct_fwd(PassParams P) {
+    extern __shared__ double2 smem[];
+    const Grid g = P.g;
+    const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;
+    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const int blk = blockIdx.x;
+    const bool dst = (P.pbc == 2);
+    // load 2L real lines, reorder into v, pack pairs as complex
+    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
+        int rline, pos;
+        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
+        else { rline = f % (2 * L); pos = f / (2 * L); }
+        long base, stride;
+        bool valid;
+        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
+        double val = 0.0;
+        if (valid) {
+            if (AXIS == 0) {
+                long rl = base / g.nx;
+                val = div_at(g, P.src, pos, (int)(rl % g.ny), (int)(rl / g.ny));
+            } else {
+                val = P.work[base + stride * pos];
+            }
+            if (dst && (pos & 1) == 0) val = -val;
+        }
+        // x[pos] goes to v[m] with mk_src(m) == pos
+        int m = (pos & 1) ? (N - 1 - (pos >> 1)) : (pos >> 1);
+        double *slot = (double *)&A[(rline >> 1) * LS + m];
+        slot[rline & 1] = val;
+    }
+    __syncthreads();
+    double2 *Z = fft_lds(A, B, P.pl, LS, L, -1);
+    double2 *O = (Z == A) ? B : A;
+    // post-twiddle into real coefficients, stored as (xa, xb) pairs in O
+    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+        int l = f / N, k = f - l * N;
+        double xa, xb;
+        makhoul_post(Z + l * LS, N, k, P.wq, xa, xb);
+        int kk = dst ? (N - 1 - k) : k;  // DST-II reverses the output order
+        O[l * LS + kk] = make_double2(xa, xb);
+    }
+    __syncthreads();
+    if (SOLVE) {
+        // modal division; the zero mode carries mean_dp (NS_functions.f90:1576-1577)
+        for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+            int l = f / N, kz = f - l * N;
+            for (int h = 0; h < 2; h++) {
+                int rline = 2 * l + h;
+                long base, stride;
+                bool valid;
+                RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
+                if (!valid) continue;
+                int xi = (int)(base % g.nx), yj = (int)((base / g.nx) % g.ny);
+                double e = P.ex[xi];
+                e = e + P.ey[yj];
+                e = e + P.ez[kz];
+                e = -4.0 * e;
+                double *slot = (double *)&O[l * LS + kz];
+                if (e == 0.0) slot[h] = P.mean_dp_dev ? (*P.mean_dp_dev) * g.nx * g.ny * g.nz : P.mean_dp * g.nx * g.ny * g.nz;
+                else slot[h] = slot[h] / e;
+            }
+        }
+        __syncthreads();
+        // inverse along z: (DST: reverse input) -> V -> inverse FFT -> un-reorder (-> DST sign flip)
+        for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+            int l = f / N, k = f - l * N;
+            const double2 *X = O + l * LS;
+            int k0 = dst ? (N - 1 - k) : k;
+            double2 xk = X[k0];
+            double2 xnk = make_double2(0.0, 0.0);
+            if (k > 0) xnk = X[dst ? (k - 1) : (N - k)];
+            double2 va = makhoul_pre_inv(xk.x, xnk.x, k, P.wq);
+            double2 vb = makhoul_pre_inv(xk.y, xnk.y, k, P.wq);
+            Z[l * LS + k] = make_double2(va.x - vb.y, va.y + vb.x);
+        }
+        __syncthreads();
+        double2 *R = fft_lds(Z, O, P.pl, LS, L, +1);
+        const double inv = 1.0 / N;
+        for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
+            int rline = f % (2 * L), pos = f / (2 * L);
+            long base, stride;
+            bool valid;
+            RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
+            if (!valid) continue;
+            int m = (pos & 1) ? (N - 1 - (pos >> 1)) : (pos >> 1);
+            double2 z = R[(rline >> 1) * LS + m];
+            double val = ((rline & 1) ? z.y : z.x) * inv;
+            if (dst && (pos & 1) == 0) val = -val;
+            P.work[base + stride * pos] = val;
+        }
+        return;
+    }
+    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
+        int rline, pos;
+        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
+        else { rline = f % (2 * L); pos = f / (2 * L); }
+        long base, stride;
+        bool valid;
+        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
+        if (!valid) continue;
+        double2 z = O[(rline >> 1) * LS + pos];
+        P.work[base + stride * pos] = (rline & 1) ? z.y : z.x;
+    }
+}
+
+template <int AXIS>
+__global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
+    extern __shared__ double2 smem[];
+    const Grid g = P.g;
+    const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;
+    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const int blk = blockIdx.x;
+    const bool dst = (P.pbc == 2);
+    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
+        int rline, pos;
+        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
+        else { rline = f % (2 * L); pos = f / (2 * L); }
+        long base, stride;
+        bool valid;
+        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
+        double val = valid ? P.work[base + stride * pos] : 0.0;
+        double *slot = (double *)&B[(rline >> 1) * LS + pos];
+        slot[rline & 1] = val;
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
+        int l = f / N, k = f - l * N;
+        const double2 *X = B + l * LS;
+        int k0 = dst ? (N - 1 - k) : k;
+        double2 xk = X[k0];
+        double2 xnk = make_double2(0.0, 0.0);
+        if (k > 0) xnk = X[dst ? (k - 1) : (N - k)];
+        double2 va = makhoul_pre_inv(xk.x, xnk.x, k, P.wq);
+        double2 vb = makhoul_pre_inv(xk.y, xnk.y, k, P.wq);
+        A[l * LS + k] = make_double2(va.x - vb.y, va.y + vb.x);
+    }
+    __syncthreads();
+    double2 *R = fft_lds(A, B, P.pl, LS, L, +1);
+    const double inv = 1.0 / N;
+    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
+        int rline, pos;
+        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
+        else { rline = f % (2 * L); pos = f / (2 * L); }
+        long base, stride;
+        bool valid;
+        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
+        if (!valid) continue;
+        int m = (pos & 1) ? (N - 1 - (pos >> 1)) : (pos >> 1);
+        double2 z = R[(rline >> 1) * LS + m];
+        double val = ((rline & 1) ? z.y : z.x) * inv;
+        if (dst && (pos & 1) == 0) val = -val;
+        if (AXIS == 0) {
+            long rl = base / g.nx;
+            int j = (int)(rl % g.ny), k = (int)(rl / g.ny);
+            if (P.dp_out) P.dp_out[FIX(pos + 2, j + 2, k + 2, g.nx + 2, g.ny + 2)] = val;
+            if (P.sol_out) P.sol_out[base + pos] = val;
+        } else {
+            P.work[base + stride * pos] = val;
+        }
+    }
+}
+
+inline int lines_per_block(int N) {
+    int L = 2048 / N;
+    return L < 1 ? 1 : L;
+}
+inline size_t lds_bytes(int L, int LS) { return (size_t)2 * L * LS * sizeof(double2); }
+
+}  // namespace
+
+void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
+                       const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
+    PassParams P{};
+    P.g = g;
+    P.nxh = g.nx / 2 + 1;
+    P.src.rhs = rhs_in;
+    P.src.us = us; P.src.vs = vs; P.src.ws = ws;
+    P.src.dx = dx; P.src.dy = dy; P.src.dz = dz; P.src.dt = dt;
+    P.src.planes = nullptr;
+    P.spec = spec;
+    P.ex = pl.ex; P.ey = pl.ey; P.ez = pl.ez;
+    P.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+    P.dp_out = dp_out;
+    P.sol_out = sol_out;
+    const long RL = (long)g.ny * g.nz;
+    // x forward
+    P.pl = pl.px;
+    P.L = lines_per_block(g.nx);
+    P.LS = g.nx + 2;
+    {
+        long CL = (RL + 1) / 2;
+        unsigned nb = (unsigned)((CL + P.L - 1) / P.L);
+        hipLaunchKernelGGL(k_fft_x_fwd, dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    }
+    // y forward
+    P.pl = pl.py;
+    P.L = lines_per_block(g.ny);
+    P.LS = g.ny + 1;
+    {
+        unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);
+        hipLaunchKernelGGL((k_fft_yz<1, -1, 0>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    }
+    // z forward + divide + inverse
+    P.pl = pl.pz;
+    P.L = lines_per_block(g.nz);
+    P.LS = g.nz + 1;
+    {
+        unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.ny);
+        hipLaunchKernelGGL((k_fft_yz<2, -1, 1>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    }
+    // y inverse
+    P.pl = pl.py;
+    P.L = lines_per_block(g.ny);
+    P.LS = g.ny + 1;
+    {
+        unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);
+        hipLaunchKernelGGL((k_fft_yz<1, 1, 0>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    }
+    // x inverse (c2r)
+    P.pl = pl.px;
+    P.L = lines_per_block(g.nx);
+    P.LS = g.nx + 2;
+    {
+        long CL = (RL + 1) / 2;
+        unsigned nb = (unsigned)((CL + P.L - 1) / P.L);
+        hipLaunchKernelGGL(k_fft_x_inv, dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[4], st);
+    }
+}
+
+void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
+                       const double *us, const double *vs, const double *ws, const double *rhs_in,
+                       const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
+    PassParams P{};
+    P.g = g;
+    P.src.rhs = rhs_in;
+    P.src.us = us; P.src.vs = vs; P.src.ws = ws;
+    P.src.dx = dx; P.src.dy = dy; P.src.dz = dz; P.src.dt = dt;
+    P.src.planes = planes;
+    P.src.L = plane_layout(g.nx, g.ny, g.nz);
+    for (int a = 0; a < 3; a++) P.src.pbc[a] = pl.pbc[a];
+    P.work = work;
+    P.ex = pl.ex; P.ey = pl.ey; P.ez = pl.ez;
+    P.mean_dp_dev = mean_dp_dev;
+    P.mean_dp = mean_dp_host;
+    P.dp_out = dp_out;
+    P.sol_out = sol_out;
+    const long RL = (long)g.ny * g.nz;
+    auto nb_x = [&](int L) { return (unsigned)((((RL + 1) / 2) + L - 1) / L); };
+    auto nb_yz = [&](int L, int others) { return (unsigned)(((((g.nx + 1) / 2) + L - 1) / L) * others); };
+    // x forward
+    P.pl = pl.px; P.wq = pl.wq_x; P.pbc = pl.pbc[0];
+    P.L = lines_per_block(g.nx); P.LS = g.nx + 1;
+    hipLaunchKernelGGL((k_dct_fwd<0, 0>), dim3(nb_x(P.L)), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    // y forward
+    P.pl = pl.py; P.wq = pl.wq_y; P.pbc = pl.pbc[1];
+    P.L = lines_per_block(g.ny); P.LS = g.ny + 1;
+    hipLaunchKernelGGL((k_dct_fwd<1, 0>), dim3(nb_yz(P.L, g.nz)), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    // z forward + divide + inverse
+    P.pl = pl.pz; P.wq = pl.wq_z; P.pbc = pl.pbc[2];
+    P.L = lines_per_block(g.nz); P.LS = g.nz + 1;
+    hipLaunchKernelGGL((k_dct_fwd<2, 1>), dim3(nb_yz(P.L, g.ny)), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    // y inverse
+    P.pl = pl.py; P.wq = pl.wq_y; P.pbc = pl.pbc[1];
+    P.L = lines_per_block(g.ny); P.LS = g.ny + 1;
+    hipLaunchKernelGGL((k_dct_inv<1>), dim3(nb_yz(P.L, g.nz)), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    // x inverse
+    P.pl = pl.px; P.wq = pl.wq_x; P.pbc = pl.pbc[0];
+    P.L = lines_per_block(g.nx); P.LS = g.nx + 1;
+    hipLaunchKernelGGL((k_dct_inv<0>), dim3(nb_x(P.L)), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    if (ev) (void)hipEventRecord(ev[4], st);
+}
+
+}  // namespace ns

@@ -1,0 +1,746 @@
+// Staggered-grid stencils, boundary fills, divergence and projection for
+// gfx950, FP64.  Formulas and operation order follow the reference so that
+// each value rounds the same way (paths relative to NS_3D_Staggered_dp/):
+//   cal_conv  NS_functions.f90:53-158   (avg/diff composition, FD_functions.f90:222-410)
+//   cal_diff  NS_functions.f90:161-210  (diff2, FD_functions.f90:413-527)
+//   predictor big_periodic_3D.f90:350-424, ...interBC_combine.f90:543-654
+//   vel_bc_staggered(_CN2) NS_functions.f90:213-410, pr_bc_staggered :1403-1464
+//   get_velpr_bc/get_pr_bc :414-562, pr_bc_staggered_modify_rhs :1345-1400
+//   divergence big_periodic_3D.f90:498-501, projection :562-567
+#include "ns_internal.h"
+
+namespace ns {
+
+namespace {
+
+struct FieldsC {
+    const double *u, *v, *w;
+    int nx, ny, nz;
+};
+
+#define UU(i, j, k) f.u[FIX(i, j, k, f.nx + 1, f.ny + 2)]
+#define VV(i, j, k) f.v[FIX(i, j, k, f.nx + 2, f.ny + 1)]
+#define WW(i, j, k) f.w[FIX(i, j, k, f.nx + 2, f.ny + 2)]
+
+// edge fluxes shared by two components: E = avg_y(u)*avg_x(v) on xy-edges,
+// F = avg_z(u)*avg_x(w) on xz-edges, G = avg_z(v)*avg_y(w) on yz-edges
+__device__ __forceinline__ double fluxE(const FieldsC &f, int i, int j, int k) {
+    return ((UU(i, j + 1, k) + UU(i, j, k)) / 2.0) * ((VV(i + 1, j, k) + VV(i, j, k)) / 2.0);
+}
+__device__ __forceinline__ double fluxF(const FieldsC &f, int i, int j, int k) {
+    return ((UU(i, j, k + 1) + UU(i, j, k)) / 2.0) * ((WW(i + 1, j, k) + WW(i, j, k)) / 2.0);
+}
+__device__ __forceinline__ double fluxG(const FieldsC &f, int i, int j, int k) {
+    return ((VV(i, j, k + 1) + VV(i, j, k)) / 2.0) * ((WW(i, j + 1, k) + WW(i, j, k)) / 2.0);
+}
+
+// conv and diff of one component at one in-range point (1-based ghosted indices)
+__device__ __forceinline__ void conv_diff_u(const FieldsC &f, int per, int fi, int J, int K, double dx, double dy,
+                                            double dz, double dx2, double dy2, double dz2, double &conv, double &dif) {
+    int fl = (per && fi == 1) ? f.nx : fi - 1;
+    double c0 = UU(fi, J, K);
+    double ur = (UU(fi + 1, J, K) + c0) / 2.0;
+    double ul = (c0 + UU(fl, J, K)) / 2.0;
+    double t1 = ur * ur - ul * ul;
+    double t2 = fluxE(f, fi, J, K) - fluxE(f, fi, J - 1, K);
+    double t3 = fluxF(f, fi, J, K) - fluxF(f, fi, J, K - 1);
+    double r = t1 / dx;
+    r = r + t2 / dy;
+    r = r + t3 / dz;
+    conv = r;
+    double d1 = UU(fl, J, K) - 2.0 * c0 + UU(fi + 1, J, K);
+    double d2 = UU(fi, J - 1, K) - 2.0 * c0 + UU(fi, J + 1, K);
+    double d3 = UU(fi, J, K - 1) - 2.0 * c0 + UU(fi, J, K + 1);
+    double s = d1 / dx2;
+    s = s + d2 / dy2;
+    s = s + d3 / dz2;
+    dif = s;
+}
+__device__ __forceinline__ void conv_diff_v(const FieldsC &f, int per, int I, int fj, int K, double dx, double dy,
+                                            double dz, double dx2, double dy2, double dz2, double &conv, double &dif) {
+    int fl = (per && fj == 1) ? f.ny : fj - 1;
+    double c0 = VV(I, fj, K);
+    double t1 = fluxE(f, I, fj, K) - fluxE(f, I - 1, fj, K);
+    double vr = (VV(I, fj + 1, K) + c0) / 2.0;
+    double vl = (c0 + VV(I, fl, K)) / 2.0;
+    double t2 = vr * vr - vl * vl;
+    double t3 = fluxG(f, I, fj, K) - fluxG(f, I, fj, K - 1);
+    double r = t1 / dx;
+    r = r + t2 / dy;
+    r = r + t3 / dz;
+    conv = r;
+    double d1 = VV(I - 1, fj, K) - 2.0 * c0 + VV(I + 1, fj, K);
+    double d2 = VV(I, fl, K) - 2.0 * c0 + VV(I, fj + 1, K);
+    double d3 = VV(I, fj, K - 1) - 2.0 * c0 + VV(I, fj, K + 1);
+    double s = d1 / dx2;
+    s = s + d2 / dy2;
+    s = s + d3 / dz2;
+    dif = s;
+}
+__device__ __forceinline__ void conv_diff_w(const FieldsC &f, int per, int I, int J, int fk, double dx, double dy,
+                                            double dz, double dx2, double dy2, double dz2, double &conv, double &dif) {
+    int fl = (per && fk == 1) ? f.nz : fk - 1;
+    double c0 = WW(I, J, fk);
+    double t1 = fluxF(f, I, J, fk) - fluxF(f, I - 1, J, fk);
+    double t2 = fluxG(f, I, J, fk) - fluxG(f, I, J - 1, fk);
+    double wr = (WW(I, J, fk + 1) + c0) / 2.0;
+    double wl = (c0 + WW(I, J, fl)) / 2.0;
+    double t3 = wr * wr - wl * wl;
+    double r = t1 / dx;
+    r = r + t2 / dy;
+    r = r + t3 / dz;
+    conv = r;
+    double d1 = WW(I - 1, J, fk) - 2.0 * c0 + WW(I + 1, J, fk);
+    double d2 = WW(I, J - 1, fk) - 2.0 * c0 + WW(I, J + 1, fk);
+    double d3 = WW(I, J, fl) - 2.0 * c0 + WW(I, J, fk + 1);
+    double s = d1 / dx2;
+    s = s + d2 / dy2;
+    s = s + d3 / dz2;
+    dif = s;
+}
+
+__device__ __forceinline__ void conv_diff(const FieldsC &f, int c, const int bc[3], int i, int j, int k, double dx,
+                                          double dy, double dz, double dx2, double dy2, double dz2, double &conv,
+                                          double &dif) {
+    if (c == 0) conv_diff_u(f, bc[0] == 1, i, j, k, dx, dy, dz, dx2, dy2, dz2, conv, dif);
+    else if (c == 1) conv_diff_v(f, bc[1] == 1, i, j, k, dx, dy, dz, dx2, dy2, dz2, conv, dif);
+    else conv_diff_w(f, bc[2] == 1, i, j, k, dx, dy, dz, dx2, dy2, dz2, conv, dif);
+}
+
+// range of the rhs assignment (x_range0/x_range1 etc., big_periodic_3D.f90:126-146,356-358)
+__device__ __forceinline__ bool in_range(int c, const int bc[3], int nx, int ny, int nz, int i, int j, int k) {
+    int id[3] = {i, j, k}, n[3] = {nx, ny, nz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int lo, hi;
+        if (a == c) { lo = (bc[a] == 1) ? 1 : 2; hi = n[a]; }
+        else { lo = 2; hi = n[a] + 1; }
+        if (id[a] < lo || id[a] > hi) return false;
+    }
+    return true;
+}
+
+struct PredParams {
+    int nx, ny, nz;
+    double dx, dy, dz, nu, dt;
+    int scheme, first, dp_flag;
+    int bc[3];
+    const double *u, *v, *w, *p;
+    double *h[3];
+    double *out[3];
+};
+
+__global__ void __launch_bounds__(256) k_predictor(PredParams P) {
+    const int c = blockIdx.y;
+    Grid g{P.nx, P.ny, P.nz};
+    int n[3];
+    comp_dims(g, c, n);
+    size_t tot = (size_t)n[0] * n[1] * n[2];
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= tot) return;
+    int i = (int)(q % n[0]) + 1;
+    size_t r = q / n[0];
+    int j = (int)(r % n[1]) + 1;
+    int k = (int)(r / n[1]) + 1;
+    FieldsC f{P.u, P.v, P.w, P.nx, P.ny, P.nz};
+    const double *v0 = (c == 0) ? P.u : (c == 1) ? P.v : P.w;
+    double *H = P.h[c];
+    bool in = in_range(c, P.bc, P.nx, P.ny, P.nz, i, j, k);
+    double cv = 0.0, df = 0.0;
+    if (in) conv_diff(f, c, P.bc, i, j, k, P.dx, P.dy, P.dz, P.dx * P.dx, P.dy * P.dy, P.dz * P.dz, cv, df);
+    double dpd = 0.0;
+    if (P.dp_flag) {
+        int b[3] = {i, j, k};
+        b[c]++;
+        double dd = (c == 0) ? P.dx : (c == 1) ? P.dy : P.dz;
+        dpd = (P.p[FIX(b[0], b[1], b[2], P.nx + 2, P.ny + 2)] - P.p[FIX(i, j, k, P.nx + 2, P.ny + 2)]) / dd;
+    }
+    const double dpf = (double)P.dp_flag;
+    const double dt = P.dt, nu = P.nu;
+    double rhs, val;
+    if (P.scheme == NS_EULER) {
+        rhs = in ? nu * df - cv : 0.0;
+        val = dt * (1.0 * rhs - dpf * dpd) + v0[q];
+        H[q] = 0.0;
+    } else if (P.scheme == NS_AB2) {
+        rhs = in ? nu * df - cv : 0.0;
+        if (P.first) val = dt * (1.0 * rhs - dpf * dpd) + v0[q];
+        else val = dt * (1.5 * rhs - 0.5 * H[q] - dpf * dpd) + v0[q];
+        H[q] = rhs;
+    } else {
+        if (P.first) {
+            rhs = in ? 0.5 * nu * df - cv : 0.0;
+            val = dt * (rhs - dpf * dpd) + v0[q];
+        } else {
+            rhs = in ? 0.5 * nu * df - 1.5 * cv : 0.0;
+            val = dt * (rhs - 0.5 * H[q] - dpf * dpd) + v0[q];
+        }
+        if (in) H[q] = -cv;
+    }
+    P.out[c][q] = val;
+}
+
+struct CDParams {
+    int nx, ny, nz;
+    double h0, h1, h2;
+    int bc[3];
+    int which;  // 0 conv, 1 diff
+    const double *u, *v, *w;
+    double *o[3];
+};
+
+// compact outputs conv_x(m1,ny,nz) etc. (cal_conv/cal_diff mirrors)
+__global__ void __launch_bounds__(256) k_conv_diff(CDParams P) {
+    const int c = blockIdx.y;
+    int m[3] = {P.nx, P.ny, P.nz};
+    if (P.bc[c] != 1) m[c] -= 1;
+    size_t tot = (size_t)m[0] * m[1] * m[2];
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= tot) return;
+    int a = (int)(q % m[0]) + 1;
+    size_t r = q / m[0];
+    int b = (int)(r % m[1]) + 1;
+    int cc = (int)(r / m[1]) + 1;
+    int id[3] = {a + 1, b + 1, cc + 1};
+    id[c] = (P.bc[c] == 1) ? id[c] - 1 : id[c];  // own axis: periodic 1..n, walls 2..n
+    FieldsC f{P.u, P.v, P.w, P.nx, P.ny, P.nz};
+    double cv, df;
+    // conv mirror gets (dx,dy,dz); diff mirror gets (dx2,dy2,dz2) as cal_diff does
+    conv_diff(f, c, P.bc, id[0], id[1], id[2], P.h0, P.h1, P.h2, P.h0, P.h1, P.h2, cv, df);
+    P.o[c][q] = (P.which == 0) ? cv : df;
+}
+
+struct BCParams {
+    int nx, ny, nz;
+    int ax;
+    int bc;
+    int pbc_mode;  // 0 velocity fill, 1 CN2 rows, 2 pressure fill
+    double d;
+    PlaneLayout L;
+    const double *planes;
+    double *a[3];
+};
+
+__device__ __forceinline__ void other_axes(int ax, int &o1, int &o2) {
+    o1 = (ax == 0) ? 1 : 0;
+    o2 = (ax == 2) ? 1 : 2;
+}
+
+// velocity ghost/wall fill along one axis (NS_functions.f90:235-308), all three components
+__global__ void __launch_bounds__(256) k_vel_bc_axis(BCParams P) {
+    Grid g{P.nx, P.ny, P.nz};
+    int o1, o2;
+    other_axes(P.ax, o1, o2);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        size_t tot = (size_t)n[o1] * n[o2];
+        if (t >= tot) continue;
+        int pp = (int)(t % n[o1]) + 1, qq = (int)(t / n[o1]) + 1;
+        int i1[3], i2[3];
+        i1[o1] = i2[o1] = pp;
+        i1[o2] = i2[o2] = qq;
+        double *A = P.a[c];
+        const int N = n[P.ax];
+        const bool normal = (c == P.ax);
+        const double *b1 = P.planes ? P.planes + P.L.off[6 * c + 2 * P.ax] : nullptr;
+        const double *bn = P.planes ? P.planes + P.L.off[6 * c + 2 * P.ax + 1] : nullptr;
+        const size_t pl = t;
+#define AT(ix) A[FIX(ix[0], ix[1], ix[2], n[0], n[1])]
+        if (P.bc == 1) {
+            if (normal) { i1[P.ax] = N; i2[P.ax] = 1; AT(i1) = AT(i2); }
+            else {
+                i1[P.ax] = 1; i2[P.ax] = N - 1; AT(i1) = AT(i2);
+                i1[P.ax] = N; i2[P.ax] = 2; AT(i1) = AT(i2);
+            }
+        } else if (P.bc == 2) {
+            if (normal) {
+                i1[P.ax] = 1; AT(i1) = b1[pl];
+                i1[P.ax] = N; AT(i1) = bn[pl];
+            } else {
+                i1[P.ax] = 1; i2[P.ax] = 2; AT(i1) = 2 * b1[pl] - AT(i2);
+                i1[P.ax] = N; i2[P.ax] = N - 1; AT(i1) = 2 * bn[pl] - AT(i2);
+            }
+        } else if (P.bc == 4) {
+            i1[P.ax] = 1; AT(i1) = b1[pl];
+            i1[P.ax] = N; AT(i1) = bn[pl];
+        }
+    }
+}
+
+// boundary rows before an ADI sweep (vel_bc_staggered_CN2, NS_functions.f90:313-410)
+__global__ void __launch_bounds__(256) k_cn2_axis(BCParams P) {
+    Grid g{P.nx, P.ny, P.nz};
+    int o1, o2;
+    other_axes(P.ax, o1, o2);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        size_t tot = (size_t)n[o1] * n[o2];
+        if (t >= tot) continue;
+        int id[3];
+        id[o1] = (int)(t % n[o1]) + 1;
+        id[o2] = (int)(t / n[o1]) + 1;
+        double *A = P.a[c];
+        const int N = n[P.ax];
+        if (P.bc == 1) {
+            if (c != P.ax) { id[P.ax] = 1; A[FIX(id[0], id[1], id[2], n[0], n[1])] = 0.0; }
+            id[P.ax] = N; A[FIX(id[0], id[1], id[2], n[0], n[1])] = 0.0;
+        } else if (P.bc == 2 || P.bc == 4) {
+            const double *b1 = P.planes + P.L.off[6 * c + 2 * P.ax];
+            const double *bn = P.planes + P.L.off[6 * c + 2 * P.ax + 1];
+            id[P.ax] = 1; A[FIX(id[0], id[1], id[2], n[0], n[1])] = b1[t];
+            id[P.ax] = N; A[FIX(id[0], id[1], id[2], n[0], n[1])] = bn[t];
+        }
+    }
+}
+
+// pressure ghost fill along one axis (pr_bc_staggered, NS_functions.f90:1416-1462)
+__global__ void __launch_bounds__(256) k_pr_bc_axis(BCParams P) {
+    int n[3] = {P.nx + 2, P.ny + 2, P.nz + 2};
+    int o1, o2;
+    other_axes(P.ax, o1, o2);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n[o1] * n[o2]) return;
+    int i1[3], i2[3];
+    i1[o1] = i2[o1] = (int)(t % n[o1]) + 1;
+    i1[o2] = i2[o2] = (int)(t / n[o1]) + 1;
+    double *A = P.a[0];
+    const int N = n[P.ax];
+    const double *b1 = (P.bc != 1) ? P.planes + P.L.off[18 + 2 * P.ax] : nullptr;
+    const double *bn = (P.bc != 1) ? P.planes + P.L.off[19 + 2 * P.ax] : nullptr;
+    if (P.bc == 1) {
+        i1[P.ax] = 1; i2[P.ax] = N - 1; AT(i1) = AT(i2);
+        i1[P.ax] = N; i2[P.ax] = 2; AT(i1) = AT(i2);
+    } else if (P.bc == 2) {
+        i1[P.ax] = 1; i2[P.ax] = 2; AT(i1) = 2 * b1[t] - AT(i2);
+        i1[P.ax] = N; i2[P.ax] = N - 1; AT(i1) = 2 * bn[t] - AT(i2);
+    } else if (P.bc == 3) {
+        i1[P.ax] = 1; i2[P.ax] = 2; AT(i1) = -P.d * b1[t] + AT(i2);
+        i1[P.ax] = N; i2[P.ax] = N - 1; AT(i1) = P.d * bn[t] + AT(i2);
+    } else if (P.bc == 4) {
+        i1[P.ax] = 1; AT(i1) = b1[t];
+        i1[P.ax] = N; AT(i1) = bn[t];
+    }
+#undef AT
+}
+
+// Poisson RHS boundary terms along one axis (NS_functions.f90:1362-1398)
+__global__ void __launch_bounds__(256) k_modify_rhs_axis(BCParams P) {
+    int n[3] = {P.nx, P.ny, P.nz}, np_[3] = {P.nx + 2, P.ny + 2, P.nz + 2};
+    int o1, o2;
+    other_axes(P.ax, o1, o2);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n[o1] * n[o2]) return;
+    int pp = (int)(t % n[o1]) + 1, qq = (int)(t / n[o1]) + 1;
+    int i1[3], iN[3];
+    i1[o1] = iN[o1] = pp;
+    i1[o2] = iN[o2] = qq;
+    i1[P.ax] = 1;
+    iN[P.ax] = n[P.ax];
+    size_t pl = (size_t)pp + (size_t)np_[o1] * (size_t)qq;  // interior (2:np-1) of the full plane
+    const double *b1 = P.planes + P.L.off[18 + 2 * P.ax], *bn = P.planes + P.L.off[19 + 2 * P.ax];
+    double *r1 = &P.a[0][FIX(i1[0], i1[1], i1[2], P.nx, P.ny)];
+    double *rn = &P.a[0][FIX(iN[0], iN[1], iN[2], P.nx, P.ny)];
+    const double d = P.d, d2 = P.d * P.d;
+    if (P.bc == 2) { *r1 = *r1 - 2 * b1[pl] / d2; *rn = *rn - 2 * bn[pl] / d2; }
+    else if (P.bc == 3) { *r1 = *r1 + b1[pl] / d; *rn = *rn - bn[pl] / d; }
+    else if (P.bc == 4) { *r1 = *r1 - b1[pl] / d2; *rn = *rn - bn[pl] / d2; }
+}
+
+__global__ void __launch_bounds__(256) k_divergence(Grid g, double dx, double dy, double dz, double dt,
+                                                    const double *__restrict__ us, const double *__restrict__ vs,
+                                                    const double *__restrict__ ws, double *__restrict__ rhs) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.ni()) return;
+    int a = (int)(q % g.nx) + 1;
+    size_t r = q / g.nx;
+    int b = (int)(r % g.ny) + 1, c = (int)(r / g.ny) + 1;
+    const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+    double s = (us[FIX(a + 1, b + 1, c + 1, U1, U2)] - us[FIX(a, b + 1, c + 1, U1, U2)]) / dx;
+    s = s + (vs[FIX(a + 1, b + 1, c + 1, V1, V2)] - vs[FIX(a + 1, b, c + 1, V1, V2)]) / dy;
+    s = s + (ws[FIX(a + 1, b + 1, c + 1, W1, W2)] - ws[FIX(a + 1, b + 1, c, W1, W2)]) / dz;
+    rhs[q] = s / dt;
+}
+
+struct ProjParams {
+    Grid g;
+    double d[3], dt;
+    int dp_flag;
+    const double *s[3];
+    const double *dp, *gauge;
+    double *o[4];
+};
+
+__global__ void __launch_bounds__(256) k_project(ProjParams P) {
+    const int c = blockIdx.y;
+    int n[3];
+    comp_dims(P.g, c, n);
+    size_t tot = (size_t)n[0] * n[1] * n[2];
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= tot) return;
+    const double gc = P.gauge ? *P.gauge : 0.0;
+    if (c == 3) {
+        double dpv = P.gauge ? P.dp[q] - gc : P.dp[q];
+        P.o[3][q] = (double)P.dp_flag * P.o[3][q] + dpv;
+        return;
+    }
+    int i = (int)(q % n[0]) + 1;
+    size_t r = q / n[0];
+    int j = (int)(r % n[1]) + 1, k = (int)(r / n[1]) + 1;
+    int b[3] = {i, j, k};
+    b[c]++;
+    const int P1 = P.g.nx + 2, P2 = P.g.ny + 2;
+    double hi = P.dp[FIX(b[0], b[1], b[2], P1, P2)], lo = P.dp[FIX(i, j, k, P1, P2)];
+    if (P.gauge) { hi = hi - gc; lo = lo - gc; }
+    double gr = hi - lo;
+    P.o[c][q] = P.s[c][q] - P.dt * gr / P.d[c];
+}
+
+struct GetBCParams {
+    int nx, ny, nz;
+    int ax, bc, corr;
+    double d[3], dt;
+    PlaneLayout L;
+    const double *src[3];
+    const double *pprev;
+    double *planes;
+};
+
+// velocity planes from a (parent) field (get_velpr_bc, NS_functions.f90:446-503)
+__global__ void __launch_bounds__(256) k_get_vel_bc_axis(GetBCParams P) {
+    Grid g{P.nx, P.ny, P.nz};
+    int o1, o2;
+    other_axes(P.ax, o1, o2);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int P1 = P.nx + 2, P2 = P.ny + 2;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        if (t >= (size_t)n[o1] * n[o2]) continue;
+        int id[3];
+        id[o1] = (int)(t % n[o1]) + 1;
+        id[o2] = (int)(t / n[o1]) + 1;
+        const int N = n[P.ax];
+        auto val = [&](int pos) -> double {
+            int a[3] = {id[0], id[1], id[2]};
+            a[P.ax] = pos;
+            double x = P.src[c][FIX(a[0], a[1], a[2], n[0], n[1])];
+            if (P.corr) {
+                int b[3] = {a[0], a[1], a[2]};
+                b[c]++;
+                double gr = P.pprev[FIX(b[0], b[1], b[2], P1, P2)] - P.pprev[FIX(a[0], a[1], a[2], P1, P2)];
+                x = x + gr / P.d[c] * P.dt;
+            }
+            return x;
+        };
+        double *b1 = P.planes + P.L.off[6 * c + 2 * P.ax], *bn = P.planes + P.L.off[6 * c + 2 * P.ax + 1];
+        if (P.bc == 4 || c == P.ax) { b1[t] = val(1); bn[t] = val(N); }
+        else { b1[t] = (val(1) + val(2)) / 2; bn[t] = (val(N) + val(N - 1)) / 2; }
+    }
+}
+
+// pressure planes (get_pr_bc, NS_functions.f90:518-560)
+__global__ void __launch_bounds__(256) k_get_pr_bc_axis(GetBCParams P) {
+    int n[3] = {P.nx + 2, P.ny + 2, P.nz + 2};
+    int o1, o2;
+    other_axes(P.ax, o1, o2);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n[o1] * n[o2]) return;
+    int id[3];
+    id[o1] = (int)(t % n[o1]) + 1;
+    id[o2] = (int)(t / n[o1]) + 1;
+    const double *p = P.src[0];
+    const int N = n[P.ax];
+    auto val = [&](int pos) -> double {
+        int a[3] = {id[0], id[1], id[2]};
+        a[P.ax] = pos;
+        return p[FIX(a[0], a[1], a[2], n[0], n[1])];
+    };
+    double *b1 = P.planes + P.L.off[18 + 2 * P.ax], *bn = P.planes + P.L.off[19 + 2 * P.ax];
+    double d = P.d[P.ax];
+    if (P.bc == 2) { b1[t] = (val(1) + val(2)) / 2; bn[t] = (val(N) + val(N - 1)) / 2; }
+    else if (P.bc == 4) { b1[t] = val(1); bn[t] = val(N); }
+    else if (P.bc == 3) { b1[t] = (val(2) - val(1)) / d; bn[t] = (val(N) - val(N - 1)) / d; }
+}
+
+__global__ void __launch_bounds__(256) k_axpby(const double *__restrict__ a, const double *__restrict__ b, double th,
+                                               double *__restrict__ out, long n) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = (1.0 - th) * a[q] + th * b[q];
+}
+
+__global__ void __launch_bounds__(256) k_copy_interior(Grid g, const double *__restrict__ sol, double *__restrict__ dp) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.ni()) return;
+    int a = (int)(q % g.nx) + 1;
+    size_t r = q / g.nx;
+    int b = (int)(r % g.ny) + 1, c = (int)(r / g.ny) + 1;
+    dp[FIX(a + 1, b + 1, c + 1, g.nx + 2, g.ny + 2)] = sol[q];
+}
+
+__global__ void __launch_bounds__(256) k_sub_gauge(const double *__restrict__ src, const double *gauge,
+                                                   double *__restrict__ dst, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = src[q] - *gauge;
+}
+
+__global__ void __launch_bounds__(256) k_max_div(Grid g, double dx, double dy, double dz, const double *__restrict__ u,
+                                                 const double *__restrict__ v, const double *__restrict__ w,
+                                                 double *partial) {
+    __shared__ double red[256];
+    double m = 0.0;
+    const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < g.ni(); q += (size_t)gridDim.x * blockDim.x) {
+        int a = (int)(q % g.nx) + 1;
+        size_t r = q / g.nx;
+        int b = (int)(r % g.ny) + 1, c = (int)(r / g.ny) + 1;
+        double s = (u[FIX(a + 1, b + 1, c + 1, U1, U2)] - u[FIX(a, b + 1, c + 1, U1, U2)]) / dx;
+        s = s + (v[FIX(a + 1, b + 1, c + 1, V1, V2)] - v[FIX(a + 1, b, c + 1, V1, V2)]) / dy;
+        s = s + (w[FIX(a + 1, b + 1, c + 1, W1, W2)] - w[FIX(a + 1, b + 1, c, W1, W2)]) / dz;
+        m = fmax(m, fabs(s));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_count_nonfinite(const double *__restrict__ a, size_t n, unsigned int *count) {
+    unsigned int local = 0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        if (!isfinite(a[q])) local++;
+    if (local) atomicAdd(count, local);
+}
+
+inline unsigned int nblocks(size_t n, int bs = 256) { return (unsigned int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+PlaneLayout plane_layout(int nx, int ny, int nz) {
+    PlaneLayout L;
+    long d[4][3] = {{nx + 1, ny + 2, nz + 2}, {nx + 2, ny + 1, nz + 2}, {nx + 2, ny + 2, nz + 1}, {nx + 2, ny + 2, nz + 2}};
+    int q = 0;
+    for (int c = 0; c < 4; c++) {
+        long sx = d[c][1] * d[c][2], sy = d[c][0] * d[c][2], sz = d[c][0] * d[c][1];
+        L.size[q++] = sx; L.size[q++] = sx;
+        L.size[q++] = sy; L.size[q++] = sy;
+        L.size[q++] = sz; L.size[q++] = sz;
+    }
+    long off = 0;
+    for (q = 0; q < 24; q++) { L.off[q] = off; off += L.size[q]; }
+    L.off[24] = off;
+    L.total = off + 1;
+    return L;
+}
+
+void launch_predictor(const ns_config &c, int first, const double *u, const double *v, const double *w, const double *p,
+                      double *hx, double *hy, double *hz, double *us, double *vs, double *ws, hipStream_t st) {
+    PredParams P;
+    P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
+    P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
+    P.scheme = c.scheme; P.first = first; P.dp_flag = c.dp_flag;
+    for (int a = 0; a < 3; a++) P.bc[a] = c.bc[a];
+    P.u = u; P.v = v; P.w = w; P.p = p;
+    P.h[0] = hx; P.h[1] = hy; P.h[2] = hz;
+    P.out[0] = us; P.out[1] = vs; P.out[2] = ws;
+    Grid g{c.nx, c.ny, c.nz};
+    size_t mx = g.nu() > g.nv() ? g.nu() : g.nv();
+    mx = mx > g.nw() ? mx : g.nw();
+    hipLaunchKernelGGL(k_predictor, dim3(nblocks(mx), 3), dim3(256), 0, st, P);
+}
+
+void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff, const double *u,
+                      const double *v, const double *w, double *ox, double *oy, double *oz, hipStream_t st) {
+    CDParams P;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.h0 = h0; P.h1 = h1; P.h2 = h2;
+    for (int a = 0; a < 3; a++) P.bc[a] = bc[a];
+    P.which = which_diff;
+    P.u = u; P.v = v; P.w = w;
+    P.o[0] = ox; P.o[1] = oy; P.o[2] = oz;
+    hipLaunchKernelGGL(k_conv_diff, dim3(nblocks(g.ni()), 3), dim3(256), 0, st, P);
+}
+
+static size_t max_plane(const Grid &g, int ax) {
+    size_t m = 0;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        int o1, o2;
+        o1 = (ax == 0) ? 1 : 0;
+        o2 = (ax == 2) ? 1 : 2;
+        size_t s = (size_t)n[o1] * n[o2];
+        m = s > m ? s : m;
+    }
+    return m;
+}
+
+void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w,
+                   hipStream_t st) {
+    BCParams P;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.planes = planes;
+    P.a[0] = u; P.a[1] = v; P.a[2] = w;
+    P.pbc_mode = 0;
+    P.d = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        P.ax = ax;
+        P.bc = bc[ax];
+        if (bc[ax] == 3) continue;
+        hipLaunchKernelGGL(k_vel_bc_axis, dim3(nblocks(max_plane(g, ax))), dim3(256), 0, st, P);
+    }
+}
+
+void launch_vel_bc_cn2(const Grid &g, int bc, int dir, const double *planes, double *u, double *v, double *w,
+                       hipStream_t st) {
+    BCParams P;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.planes = planes;
+    P.a[0] = u; P.a[1] = v; P.a[2] = w;
+    P.ax = dir - 1;
+    P.bc = bc;
+    P.pbc_mode = 1;
+    P.d = 0;
+    if (bc == 3) return;
+    hipLaunchKernelGGL(k_cn2_axis, dim3(nblocks(max_plane(g, P.ax))), dim3(256), 0, st, P);
+}
+
+void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,
+                  hipStream_t st) {
+    BCParams P;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.planes = planes;
+    P.a[0] = dp; P.a[1] = nullptr; P.a[2] = nullptr;
+    P.pbc_mode = 2;
+    double d[3] = {dx, dy, dz};
+    int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+    for (int ax = 0; ax < 3; ax++) {
+        P.ax = ax;
+        P.bc = pbc[ax];
+        P.d = d[ax];
+        int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        hipLaunchKernelGGL(k_pr_bc_axis, dim3(nblocks((size_t)n[o1] * n[o2])), dim3(256), 0, st, P);
+    }
+}
+
+void launch_modify_rhs(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz,
+                       double *rhs, hipStream_t st) {
+    BCParams P;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.planes = planes;
+    P.a[0] = rhs; P.a[1] = nullptr; P.a[2] = nullptr;
+    double d[3] = {dx, dy, dz};
+    int n[3] = {g.nx, g.ny, g.nz};
+    for (int ax = 0; ax < 3; ax++) {
+        if (pbc[ax] == 1) continue;
+        P.ax = ax;
+        P.bc = pbc[ax];
+        P.d = d[ax];
+        int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        hipLaunchKernelGGL(k_modify_rhs_axis, dim3(nblocks((size_t)n[o1] * n[o2])), dim3(256), 0, st, P);
+    }
+}
+
+void launch_divergence(const Grid &g, double dx, double dy, double dz, double dt, const double *us, const double *vs,
+                       const double *ws, double *rhs, hipStream_t st) {
+    hipLaunchKernelGGL(k_divergence, dim3(nblocks(g.ni())), dim3(256), 0, st, g, dx, dy, dz, dt, us, vs, ws, rhs);
+}
+
+void launch_project(const Grid &g, double dx, double dy, double dz, double dt, int dp_flag, const double *us,
+                    const double *vs, const double *ws, const double *dp, const double *gauge, double *u, double *v,
+                    double *w, double *p, hipStream_t st) {
+    ProjParams P;
+    P.g = g;
+    P.d[0] = dx; P.d[1] = dy; P.d[2] = dz;
+    P.dt = dt;
+    P.dp_flag = dp_flag;
+    P.s[0] = us; P.s[1] = vs; P.s[2] = ws;
+    P.dp = dp;
+    P.gauge = gauge;
+    P.o[0] = u; P.o[1] = v; P.o[2] = w; P.o[3] = p;
+    hipLaunchKernelGGL(k_project, dim3(nblocks(g.np()), 4), dim3(256), 0, st, P);
+}
+
+void launch_get_velpr_bc(const Grid &g, const int bc[3], const int pbc[3], double dx, double dy, double dz, double dt,
+                         int corr, const double *ubc, const double *vbc, const double *wbc, const double *pbc_arr,
+                         const double *pprev, double *planes, hipStream_t st) {
+    GetBCParams P;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.d[0] = dx; P.d[1] = dy; P.d[2] = dz;
+    P.dt = dt;
+    P.corr = corr && (bc[0] != 1 || bc[1] != 1 || bc[2] != 1);
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.src[0] = ubc; P.src[1] = vbc; P.src[2] = wbc;
+    P.pprev = pprev;
+    P.planes = planes;
+    for (int ax = 0; ax < 3; ax++) {
+        if (bc[ax] != 2 && bc[ax] != 4) continue;
+        P.ax = ax;
+        P.bc = bc[ax];
+        hipLaunchKernelGGL(k_get_vel_bc_axis, dim3(nblocks(max_plane(g, ax))), dim3(256), 0, st, P);
+    }
+    launch_get_pr_bc(g, pbc, dx, dy, dz, pbc_arr, planes, st);
+}
+
+void launch_get_pr_bc(const Grid &g, const int pbc[3], double dx, double dy, double dz, const double *p,
+                      double *planes, hipStream_t st) {
+    GetBCParams P;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.d[0] = dx; P.d[1] = dy; P.d[2] = dz;
+    P.dt = 0;
+    P.corr = 0;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.src[0] = p; P.src[1] = nullptr; P.src[2] = nullptr;
+    P.pprev = nullptr;
+    P.planes = planes;
+    int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+    for (int ax = 0; ax < 3; ax++) {
+        if (pbc[ax] == 1) continue;
+        P.ax = ax;
+        P.bc = pbc[ax];
+        int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        hipLaunchKernelGGL(k_get_pr_bc_axis, dim3(nblocks((size_t)n[o1] * n[o2])), dim3(256), 0, st, P);
+    }
+}
+
+void launch_axpby(const double *a, const double *b, double th, double *out, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(256), 0, st, a, b, th, out, n);
+}
+
+void launch_copy_interior(const Grid &g, const double *sol, double *dp, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_interior, dim3(nblocks(g.ni())), dim3(256), 0, st, g, sol, dp);
+}
+
+void launch_sub_gauge(const double *src, const double *gauge, double *dst, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_sub_gauge, dim3(nblocks(n)), dim3(256), 0, st, src, gauge, dst, n);
+}
+
+int max_div_partials(const Grid &g) {
+    size_t nb = nblocks(g.ni());
+    return (int)(nb < 2048 ? nb : 2048);
+}
+
+int launch_max_div(const Grid &g, double dx, double dy, double dz, const double *u, const double *v, const double *w,
+                   double *partial, hipStream_t st) {
+    int nb = max_div_partials(g);
+    hipLaunchKernelGGL(k_max_div, dim3(nb), dim3(256), 0, st, g, dx, dy, dz, u, v, w, partial);
+    return nb;
+}
+
+void launch_count_nonfinite(const double *a, size_t n, unsigned int *count, hipStream_t st) {
+    size_t nb = nblocks(n);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)nb), dim3(256), 0, st, a, n, count);
+}
+
+}  // namespace ns

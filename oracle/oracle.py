@@ -1,0 +1,234 @@
+"""ctypes wrapper of the CPU oracle (oracle/ns_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+as the checker, and by bench.py's ``cpu_baseline`` leg as the timed CPU
+baseline.  Arrays are numpy float64 in Fortran order with the reference's
+ghosted shapes.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from ns_3d_staggered_dp_amd.config import CConfig, NSConfig
+from ns_3d_staggered_dp_amd.grid import field_shapes, plane_set_length
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libns_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.nso_max_div.restype = ctypes.c_double
+        _lib.nso_num_threads.restype = ctypes.c_int
+        _lib.nso_step.restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "need float64 Fortran-contiguous"
+    return a.ctypes.data_as(_D)
+
+
+def zeros(shape):
+    return np.zeros(shape, dtype=np.float64, order="F")
+
+
+def avg(A, k, bc):
+    n = list(A.shape)
+    if bc != 1:
+        n[k - 1] -= 1
+    B = zeros(tuple(n))
+    lib().nso_avg(_p(A), *map(ctypes.c_int, A.shape), ctypes.c_int(k), ctypes.c_int(bc), _p(B))
+    return B
+
+
+def diff(A, k):
+    n = list(A.shape)
+    n[k - 1] -= 1
+    B = zeros(tuple(n))
+    lib().nso_diff(_p(A), *map(ctypes.c_int, A.shape), ctypes.c_int(k), _p(B))
+    return B
+
+
+def diff2(A, k, bc):
+    n = list(A.shape)
+    n[k - 1] -= 1 if bc == 1 else 2
+    B = zeros(tuple(n))
+    lib().nso_diff2(_p(A), *map(ctypes.c_int, A.shape), ctypes.c_int(k), ctypes.c_int(bc), _p(B))
+    return B
+
+
+def _conv_shapes(nx, ny, nz, bc):
+    mx = nx if bc[0] == 1 else nx - 1
+    my = ny if bc[1] == 1 else ny - 1
+    mz = nz if bc[2] == 1 else nz - 1
+    return (mx, ny, nz), (nx, my, nz), (nx, ny, mz)
+
+
+def cal_conv(u, v, w, bc, dx, dy, dz):
+    nx, ny, nz = u.shape[0] - 1, v.shape[1] - 1, w.shape[2] - 1
+    sx, sy, sz = _conv_shapes(nx, ny, nz, bc)
+    cx, cy, cz = zeros(sx), zeros(sy), zeros(sz)
+    lib().nso_cal_conv(_p(u), _p(v), _p(w), nx, ny, nz, *bc, ctypes.c_double(dx), ctypes.c_double(dy),
+                       ctypes.c_double(dz), _p(cx), _p(cy), _p(cz))
+    return cx, cy, cz
+
+
+def cal_diff(u, v, w, bc, dx2, dy2, dz2):
+    nx, ny, nz = u.shape[0] - 1, v.shape[1] - 1, w.shape[2] - 1
+    sx, sy, sz = _conv_shapes(nx, ny, nz, bc)
+    cx, cy, cz = zeros(sx), zeros(sy), zeros(sz)
+    lib().nso_cal_diff(_p(u), _p(v), _p(w), nx, ny, nz, *bc, ctypes.c_double(dx2), ctypes.c_double(dy2),
+                       ctypes.c_double(dz2), _p(cx), _p(cy), _p(cz))
+    return cx, cy, cz
+
+
+def vel_bc(u, v, w, planes, bc):
+    nx, ny, nz = u.shape[0] - 1, v.shape[1] - 1, w.shape[2] - 1
+    lib().nso_vel_bc(_p(u), _p(v), _p(w), nx, ny, nz, _p(planes), *bc)
+
+
+def vel_bc_cn2(u, v, w, planes, bc, direction):
+    nx, ny, nz = u.shape[0] - 1, v.shape[1] - 1, w.shape[2] - 1
+    lib().nso_vel_bc_cn2(_p(u), _p(v), _p(w), nx, ny, nz, _p(planes), bc, direction)
+
+
+def get_velpr_bc(u, v, w, p, bc, pbc, dx, dy, dz, dt, dp_flag, using_ustar, p_prev=None):
+    nx, ny, nz = u.shape[0] - 1, v.shape[1] - 1, w.shape[2] - 1
+    planes = np.zeros(plane_set_length(nx, ny, nz))
+    pp = p_prev if p_prev is not None else p
+    lib().nso_get_velpr_bc(_p(u), _p(v), _p(w), _p(p), *bc, *pbc, nx, ny, nz, ctypes.c_double(dx),
+                           ctypes.c_double(dy), ctypes.c_double(dz), ctypes.c_double(dt), int(dp_flag),
+                           int(using_ustar), _p(pp), _p(planes))
+    return planes
+
+
+def get_pr_bc(p, pbc, dx, dy, dz, planes=None):
+    nx, ny, nz = p.shape[0] - 2, p.shape[1] - 2, p.shape[2] - 2
+    if planes is None:
+        planes = np.zeros(plane_set_length(nx, ny, nz))
+    lib().nso_get_pr_bc(_p(p), nx, ny, nz, *pbc, ctypes.c_double(dx), ctypes.c_double(dy), ctypes.c_double(dz),
+                        _p(planes))
+    return planes
+
+
+def pr_bc_modify_rhs(rhs, planes, pbc, dx, dy, dz):
+    nx, ny, nz = rhs.shape
+    lib().nso_pr_bc_modify_rhs(_p(rhs), nx, ny, nz, _p(planes), *pbc, ctypes.c_double(dx), ctypes.c_double(dy),
+                               ctypes.c_double(dz))
+
+
+def pr_bc(dp, planes, pbc, dx, dy, dz):
+    nx, ny, nz = dp.shape[0] - 2, dp.shape[1] - 2, dp.shape[2] - 2
+    lib().nso_pr_bc(_p(dp), nx, ny, nz, _p(planes), *pbc, ctypes.c_double(dx), ctypes.c_double(dy),
+                    ctypes.c_double(dz))
+
+
+def mat_cn(n, bc, dx2, dt, nu, cyclic=False):
+    A = zeros((n + 1, n + 1))
+    B = zeros((n + 2, n + 2))
+    lib().nso_mat_cn(n, bc, ctypes.c_double(dx2), ctypes.c_double(dt), ctypes.c_double(nu), int(cyclic), _p(A), _p(B))
+    return A, B
+
+
+def adi_sweep(u, v, w, cfg: NSConfig, direction):
+    c = cfg.to_c()
+    lib().nso_adi_sweep(_p(u), _p(v), _p(w), ctypes.byref(c), int(direction))
+
+
+def fft3(re, im, sign):
+    re = np.asfortranarray(re, dtype=np.float64).copy(order="F")
+    im = np.asfortranarray(im, dtype=np.float64).copy(order="F")
+    lib().nso_fft3(_p(re), _p(im), *re.shape, int(sign))
+    return re, im
+
+
+def poisson_fft(rhs, dx, dy, dz):
+    rhs = np.asfortranarray(rhs)
+    out = zeros(rhs.shape)
+    lib().nso_poisson_fft(_p(rhs), _p(out), *rhs.shape, ctypes.c_double(dx), ctypes.c_double(dy), ctypes.c_double(dz))
+    return out
+
+
+def poisson_dct(rhs, pbc, dx, dy, dz, mean_dp):
+    out = np.array(rhs, dtype=np.float64, order="F", copy=True)
+    lib().nso_poisson_dct(_p(out), *out.shape, *pbc, ctypes.c_double(dx), ctypes.c_double(dy), ctypes.c_double(dz),
+                          ctypes.c_double(mean_dp))
+    return out
+
+
+def dct1(x, pbc, inverse=False):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    X = np.zeros_like(x)
+    lib().nso_dct1(_p(x), _p(X), len(x), int(pbc), int(inverse))
+    return X
+
+
+def divergence(u, v, w, dx, dy, dz, dt):
+    nx, ny, nz = u.shape[0] - 1, v.shape[1] - 1, w.shape[2] - 1
+    r = zeros((nx, ny, nz))
+    lib().nso_divergence(_p(u), _p(v), _p(w), nx, ny, nz, ctypes.c_double(dx), ctypes.c_double(dy),
+                         ctypes.c_double(dz), ctypes.c_double(dt), _p(r))
+    return r
+
+
+def max_div(u, v, w, dx, dy, dz):
+    nx, ny, nz = u.shape[0] - 1, v.shape[1] - 1, w.shape[2] - 1
+    return lib().nso_max_div(_p(u), _p(v), _p(w), nx, ny, nz, ctypes.c_double(dx), ctypes.c_double(dy),
+                             ctypes.c_double(dz))
+
+
+def num_threads() -> int:
+    return lib().nso_num_threads()
+
+
+class OracleState:
+    """AB2 state (u, v, w, p, rhs_previous) plus the last step's u*, dp
+    (the restart set of big_periodic_3D.f90:620-644)."""
+
+    def __init__(self, cfg: NSConfig, u, v, w, p=None, hx=None, hy=None, hz=None, first=True):
+        self.cfg = cfg
+        sh = field_shapes(cfg.nx, cfg.ny, cfg.nz)
+        self.u = np.array(u, dtype=np.float64, order="F", copy=True)
+        self.v = np.array(v, dtype=np.float64, order="F", copy=True)
+        self.w = np.array(w, dtype=np.float64, order="F", copy=True)
+        self.p = zeros(sh["p"]) if p is None else np.array(p, dtype=np.float64, order="F", copy=True)
+        self.hx = zeros(sh["u"]) if hx is None else np.array(hx, dtype=np.float64, order="F", copy=True)
+        self.hy = zeros(sh["v"]) if hy is None else np.array(hy, dtype=np.float64, order="F", copy=True)
+        self.hz = zeros(sh["w"]) if hz is None else np.array(hz, dtype=np.float64, order="F", copy=True)
+        self.us, self.vs, self.ws = zeros(sh["u"]), zeros(sh["v"]), zeros(sh["w"])
+        self.dp = zeros(sh["p"])
+        self.first = first
+
+    def step(self, planes=None):
+        c = self.cfg.to_c()
+        if planes is None:
+            planes = np.zeros(plane_set_length(self.cfg.nx, self.cfg.ny, self.cfg.nz))
+        rc = lib().nso_step(ctypes.byref(c), _p(self.u), _p(self.v), _p(self.w), _p(self.p), _p(self.hx),
+                            _p(self.hy), _p(self.hz), _p(self.us), _p(self.vs), _p(self.ws), _p(self.dp),
+                            _p(planes), int(self.first))
+        if rc != 0:
+            raise RuntimeError(f"nso_step failed: {rc}")
+        self.first = False
+
+    def max_div(self):
+        c = self.cfg
+        return max_div(self.u, self.v, self.w, c.dx, c.dy, c.dz)
