@@ -44,8 +44,8 @@ class NSConfig:
     dx: float
     dy: float
     dz: float
-    nu: float
-    dt: float
+    nu: float = 1e-3
+    dt: float = 1e-3
     scheme: int = AB2
     bc: tuple = (1, 1, 1)
     pbc: tuple = (1, 1, 1)

@@ -587,8 +587,9 @@ __global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
     }
 }
 
-inline int lines_per_block(int N) {
-    int L = 2048 / N;
+// lines per block so that the two LDS line buffers stay within 64 KiB
+inline int lines_per_block(int LS) {
+    int L = 2048 / LS;
     return L < 1 ? 1 : L;
 }
 inline size_t lds_bytes(int L, int LS) { return (size_t)2 * L * LS * sizeof(double2); }
@@ -613,8 +614,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     const long RL = (long)g.ny * g.nz;
     // x forward
     P.pl = pl.px;
-    P.L = lines_per_block(g.nx);
     P.LS = g.nx + 2;
+    P.L = lines_per_block(P.LS);
     {
         long CL = (RL + 1) / 2;
         unsigned nb = (unsigned)((CL + P.L - 1) / P.L);
@@ -623,8 +624,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     }
     // y forward
     P.pl = pl.py;
-    P.L = lines_per_block(g.ny);
     P.LS = g.ny + 1;
+    P.L = lines_per_block(P.LS);
     {
         unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);
         hipLaunchKernelGGL((k_fft_yz<1, -1, 0>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
@@ -632,8 +633,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     }
     // z forward + divide + inverse
     P.pl = pl.pz;
-    P.L = lines_per_block(g.nz);
     P.LS = g.nz + 1;
+    P.L = lines_per_block(P.LS);
     {
         unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.ny);
         hipLaunchKernelGGL((k_fft_yz<2, -1, 1>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
@@ -641,8 +642,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     }
     // y inverse
     P.pl = pl.py;
-    P.L = lines_per_block(g.ny);
     P.LS = g.ny + 1;
+    P.L = lines_per_block(P.LS);
     {
         unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);
         hipLaunchKernelGGL((k_fft_yz<1, 1, 0>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
@@ -650,8 +651,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     }
     // x inverse (c2r)
     P.pl = pl.px;
-    P.L = lines_per_block(g.nx);
     P.LS = g.nx + 2;
+    P.L = lines_per_block(P.LS);
     {
         long CL = (RL + 1) / 2;
         unsigned nb = (unsigned)((CL + P.L - 1) / P.L);
@@ -683,27 +684,32 @@ void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     auto nb_yz = [&](int L, int others) { return (unsigned)(((((g.nx + 1) / 2) + L - 1) / L) * others); };
     // x forward
     P.pl = pl.px; P.wq = pl.wq_x; P.pbc = pl.pbc[0];
-    P.L = lines_per_block(g.nx); P.LS = g.nx + 1;
+    P.LS = g.nx + 1;
+    P.L = lines_per_block(P.LS);
     hipLaunchKernelGGL((k_dct_fwd<0, 0>), dim3(nb_x(P.L)), dim3(256), lds_bytes(P.L, P.LS), st, P);
     if (ev) (void)hipEventRecord(ev[0], st);
     // y forward
     P.pl = pl.py; P.wq = pl.wq_y; P.pbc = pl.pbc[1];
-    P.L = lines_per_block(g.ny); P.LS = g.ny + 1;
+    P.LS = g.ny + 1;
+    P.L = lines_per_block(P.LS);
     hipLaunchKernelGGL((k_dct_fwd<1, 0>), dim3(nb_yz(P.L, g.nz)), dim3(256), lds_bytes(P.L, P.LS), st, P);
     if (ev) (void)hipEventRecord(ev[1], st);
     // z forward + divide + inverse
     P.pl = pl.pz; P.wq = pl.wq_z; P.pbc = pl.pbc[2];
-    P.L = lines_per_block(g.nz); P.LS = g.nz + 1;
+    P.LS = g.nz + 1;
+    P.L = lines_per_block(P.LS);
     hipLaunchKernelGGL((k_dct_fwd<2, 1>), dim3(nb_yz(P.L, g.ny)), dim3(256), lds_bytes(P.L, P.LS), st, P);
     if (ev) (void)hipEventRecord(ev[2], st);
     // y inverse
     P.pl = pl.py; P.wq = pl.wq_y; P.pbc = pl.pbc[1];
-    P.L = lines_per_block(g.ny); P.LS = g.ny + 1;
+    P.LS = g.ny + 1;
+    P.L = lines_per_block(P.LS);
     hipLaunchKernelGGL((k_dct_inv<1>), dim3(nb_yz(P.L, g.nz)), dim3(256), lds_bytes(P.L, P.LS), st, P);
     if (ev) (void)hipEventRecord(ev[3], st);
     // x inverse
     P.pl = pl.px; P.wq = pl.wq_x; P.pbc = pl.pbc[0];
-    P.L = lines_per_block(g.nx); P.LS = g.nx + 1;
+    P.LS = g.nx + 1;
+    P.L = lines_per_block(P.LS);
     hipLaunchKernelGGL((k_dct_inv<0>), dim3(nb_x(P.L)), dim3(256), lds_bytes(P.L, P.LS), st, P);
     if (ev) (void)hipEventRecord(ev[4], st);
 }

@@ -78,7 +78,7 @@ def _random_potential(cfg: NSConfig, amp_of_k, rng):
     for _ in range(3):
         phase = rng.uniform(0.0, 2.0 * np.pi, size=amp.shape)
         spec = amp * np.exp(1j * phase)
-        comps.append(np.fft.irfftn(spec, s=(nx, ny, nz)))
+        comps.append(np.fft.irfftn(spec, s=(nx, ny, nz), axes=(0, 1, 2)))
     return comps
 
 

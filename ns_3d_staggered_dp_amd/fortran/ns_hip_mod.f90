@@ -1,0 +1,418 @@
+! ns_hip_mod -- ISO_C_BINDING layer over libns_hip (include/ns_hip.h).
+!
+! A Fortran host of NS_3D_Staggered_dp replaces `use NS_functions` by
+! `use ns_hip_mod` and keeps its calls: the wrappers below carry the
+! reference's subroutine names and argument lists (NS_functions.f90):
+!   cal_conv            :53-59      cal_diff              :161-168
+!   vel_bc_staggered    :213-217    vel_bc_staggered_CN2  :313
+!   get_velpr_bc        :414-420    get_pr_bc             :518-521
+!   pr_bc_staggered_modify_rhs :1345-1348   pr_bc_staggered :1403-1406
+!   DCT_poisson_solver  :1505-1507 (MKL handles/ipar/dpar are not needed:
+!                                   the transform plans live in the context)
+! plus the inline driver pieces as subroutines (poisson_fft, project,
+! adi_sweep) and the fused, device-resident step (ns_hip_step).
+! All arrays keep the reference's ghosted shapes.  Errors stop with the
+! failing entry point's name and status, as the drivers' `stop(1)` does.
+module ns_hip_mod
+    use, intrinsic :: iso_c_binding
+    implicit none
+    private
+
+    integer, parameter, public :: NS_EULER = 0, NS_AB2 = 1, NS_AB2_CN = 2
+    integer, parameter, public :: NS_FLAG_GAUGE_CORNER = 1, NS_FLAG_CN_CYCLIC = 2
+
+    type, bind(C), public :: ns_config
+        integer(c_int) :: nx, ny, nz
+        real(c_double) :: dx, dy, dz
+        real(c_double) :: nu, dt
+        integer(c_int) :: scheme
+        integer(c_int) :: bc(3), pbc(3)
+        integer(c_int) :: dp_flag
+        integer(c_int) :: flags
+    end type ns_config
+
+    interface
+        integer(c_int) function c_ns_create(cfg, ctx) bind(C, name='ns_create')
+            import :: c_int, c_ptr, ns_config
+            type(ns_config), intent(in) :: cfg
+            type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function c_ns_destroy(ctx) bind(C, name='ns_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_long) function c_ns_plane_set_length(nx, ny, nz) bind(C, name='ns_plane_set_length')
+            import :: c_int, c_long
+            integer(c_int), value :: nx, ny, nz
+        end function
+        integer(c_int) function c_ns_upload_state(ctx, u, v, w, p, hx, hy, hz, first) bind(C, name='ns_upload_state')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, u, v, w, p, hx, hy, hz
+            integer(c_int), value :: first
+        end function
+        integer(c_int) function c_ns_download_state(ctx, u, v, w, p, hx, hy, hz) bind(C, name='ns_download_state')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, u, v, w, p, hx, hy, hz
+        end function
+        integer(c_int) function c_ns_download_work(ctx, us, vs, ws, dp) bind(C, name='ns_download_work')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, us, vs, ws, dp
+        end function
+        integer(c_int) function c_ns_set_bc_planes(ctx, planes) bind(C, name='ns_set_bc_planes')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, planes
+        end function
+        integer(c_int) function c_ns_step(ctx, nsteps) bind(C, name='ns_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function c_ns_max_div(ctx, out) bind(C, name='ns_max_div')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out
+        end function
+        integer(c_int) function c_ns_cal_conv(ctx, u, v, w, bx, by, bz, dx, dy, dz, cx, cy, cz) bind(C, name='ns_cal_conv')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, u, v, w, cx, cy, cz
+            integer(c_int), value :: bx, by, bz
+            real(c_double), value :: dx, dy, dz
+        end function
+        integer(c_int) function c_ns_cal_diff(ctx, u, v, w, bx, by, bz, dx2, dy2, dz2, fx, fy, fz) bind(C, name='ns_cal_diff')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, u, v, w, fx, fy, fz
+            integer(c_int), value :: bx, by, bz
+            real(c_double), value :: dx2, dy2, dz2
+        end function
+        integer(c_int) function c_ns_vel_bc(ctx, u, v, w, planes, bx, by, bz) bind(C, name='ns_vel_bc_staggered')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, u, v, w, planes
+            integer(c_int), value :: bx, by, bz
+        end function
+        integer(c_int) function c_ns_vel_bc_cn2(ctx, u, v, w, planes, bc, dir) bind(C, name='ns_vel_bc_staggered_cn2')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, u, v, w, planes
+            integer(c_int), value :: bc, dir
+        end function
+        integer(c_int) function c_ns_adi_sweep(ctx, u, v, w, dir) bind(C, name='ns_adi_sweep')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, u, v, w
+            integer(c_int), value :: dir
+        end function
+        integer(c_int) function c_ns_get_velpr_bc(ctx, u, v, w, p, dt, dp_flag, using_ustar, pprev, planes) &
+            bind(C, name='ns_get_velpr_bc')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, u, v, w, p, pprev, planes
+            real(c_double), value :: dt
+            integer(c_int), value :: dp_flag, using_ustar
+        end function
+        integer(c_int) function c_ns_get_pr_bc(ctx, p, planes) bind(C, name='ns_get_pr_bc')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, p, planes
+        end function
+        integer(c_int) function c_ns_divergence(ctx, u, v, w, dt, rhs) bind(C, name='ns_divergence')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, u, v, w, rhs
+            real(c_double), value :: dt
+        end function
+        integer(c_int) function c_ns_modify_rhs(ctx, rhs, planes) bind(C, name='ns_pr_bc_staggered_modify_rhs')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, rhs, planes
+        end function
+        integer(c_int) function c_ns_poisson_fft(ctx, rhs, sol) bind(C, name='ns_poisson_fft')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, rhs, sol
+        end function
+        integer(c_int) function c_ns_dct_poisson(ctx, rhs, mean_dp) bind(C, name='ns_dct_poisson_solver')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, rhs
+            real(c_double), value :: mean_dp
+        end function
+        integer(c_int) function c_ns_pr_bc(ctx, dp, planes) bind(C, name='ns_pr_bc_staggered')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, dp, planes
+        end function
+        integer(c_int) function c_ns_project(ctx, us, vs, ws, dp, dt, dp_flag, u, v, w, p) bind(C, name='ns_project')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, us, vs, ws, dp, u, v, w, p
+            real(c_double), value :: dt
+            integer(c_int), value :: dp_flag
+        end function
+    end interface
+
+    type(c_ptr), save :: ctx = c_null_ptr
+    type(ns_config), save :: cfg_saved
+    real(c_double), allocatable, target, save :: pset(:)
+
+    public :: ns_hip_init, ns_hip_finalize, ns_hip_upload, ns_hip_download, ns_hip_download_work, ns_hip_step
+    public :: ns_hip_set_planes, ns_hip_max_div
+    public :: cal_conv, cal_diff, vel_bc_staggered, vel_bc_staggered_CN2, get_velpr_bc, get_pr_bc
+    public :: pr_bc_staggered_modify_rhs, pr_bc_staggered, DCT_poisson_solver, poisson_fft, divergence, project
+    public :: adi_sweep
+
+contains
+
+    subroutine check(rc, name)
+        integer(c_int), intent(in) :: rc
+        character(*), intent(in) :: name
+        if (rc /= 0) then
+            print '("ns_hip: ", A, " failed with status ", I0)', name, rc
+            stop 1
+        end if
+    end subroutine check
+
+    ! timescheme: "Euler" / "AB2" / "AB2-CN" (big_periodic_3D.f90:105)
+    subroutine ns_hip_init(nx, ny, nz, dx, dy, dz, nu, dt, timescheme, bc_x, bc_y, bc_z, pbc_x, pbc_y, pbc_z, dp_flag, flags)
+        integer, intent(in) :: nx, ny, nz, bc_x, bc_y, bc_z, pbc_x, pbc_y, pbc_z, dp_flag
+        real(8), intent(in) :: dx, dy, dz, nu, dt
+        character(*), intent(in) :: timescheme
+        integer, intent(in), optional :: flags
+        type(ns_config) :: c
+        if (c_associated(ctx)) call ns_hip_finalize()
+        c%nx = nx; c%ny = ny; c%nz = nz
+        c%dx = dx; c%dy = dy; c%dz = dz; c%nu = nu; c%dt = dt
+        select case (trim(timescheme))
+        case ("Euler"); c%scheme = NS_EULER
+        case ("AB2"); c%scheme = NS_AB2
+        case ("AB2-CN"); c%scheme = NS_AB2_CN
+        case default
+            print *, "ns_hip_init: unknown time scheme ", trim(timescheme)
+            stop 1
+        end select
+        c%bc = [bc_x, bc_y, bc_z]
+        c%pbc = [pbc_x, pbc_y, pbc_z]
+        c%dp_flag = dp_flag
+        c%flags = 0
+        if (present(flags)) c%flags = flags
+        call check(c_ns_create(c, ctx), "ns_create")
+        cfg_saved = c
+        if (allocated(pset)) deallocate(pset)
+        allocate(pset(c_ns_plane_set_length(nx, ny, nz)))
+        pset = 0
+    end subroutine ns_hip_init
+
+    subroutine ns_hip_finalize()
+        if (c_associated(ctx)) call check(c_ns_destroy(ctx), "ns_destroy")
+        ctx = c_null_ptr
+    end subroutine ns_hip_finalize
+
+    subroutine ns_hip_upload(u, v, w, p, rhs_x_previous, rhs_y_previous, rhs_z_previous, first)
+        real(8), intent(in), target :: u(:,:,:), v(:,:,:), w(:,:,:), p(:,:,:)
+        real(8), intent(in), target :: rhs_x_previous(:,:,:), rhs_y_previous(:,:,:), rhs_z_previous(:,:,:)
+        logical, intent(in) :: first
+        integer(c_int) :: f
+        f = 0
+        if (first) f = 1
+        call check(c_ns_upload_state(ctx, c_loc(u), c_loc(v), c_loc(w), c_loc(p), c_loc(rhs_x_previous), &
+            c_loc(rhs_y_previous), c_loc(rhs_z_previous), f), "ns_upload_state")
+    end subroutine ns_hip_upload
+
+    subroutine ns_hip_download(u, v, w, p, rhs_x_previous, rhs_y_previous, rhs_z_previous)
+        real(8), intent(inout), target :: u(:,:,:), v(:,:,:), w(:,:,:), p(:,:,:)
+        real(8), intent(inout), target :: rhs_x_previous(:,:,:), rhs_y_previous(:,:,:), rhs_z_previous(:,:,:)
+        call check(c_ns_download_state(ctx, c_loc(u), c_loc(v), c_loc(w), c_loc(p), c_loc(rhs_x_previous), &
+            c_loc(rhs_y_previous), c_loc(rhs_z_previous)), "ns_download_state")
+    end subroutine ns_hip_download
+
+    subroutine ns_hip_download_work(u_star, v_star, w_star, dp)
+        real(8), intent(inout), target :: u_star(:,:,:), v_star(:,:,:), w_star(:,:,:), dp(:,:,:)
+        call check(c_ns_download_work(ctx, c_loc(u_star), c_loc(v_star), c_loc(w_star), c_loc(dp)), "ns_download_work")
+    end subroutine ns_hip_download_work
+
+    subroutine ns_hip_set_planes(planes)
+        real(8), intent(in), target :: planes(:)
+        call check(c_ns_set_bc_planes(ctx, c_loc(planes)), "ns_set_bc_planes")
+    end subroutine ns_hip_set_planes
+
+    subroutine ns_hip_step(nsteps)
+        integer, intent(in) :: nsteps
+        call check(c_ns_step(ctx, int(nsteps, c_int)), "ns_step")
+    end subroutine ns_hip_step
+
+    real(8) function ns_hip_max_div()
+        real(c_double) :: m
+        call check(c_ns_max_div(ctx, m), "ns_max_div")
+        ns_hip_max_div = m
+    end function ns_hip_max_div
+
+    ! ---- mirrors with the reference's argument lists ----
+
+    subroutine cal_conv(u, v, w, bc_x, bc_y, bc_z, dx, dy, dz, conv_x, conv_y, conv_z)
+        integer, intent(in) :: bc_x, bc_y, bc_z
+        real(8), intent(in) :: dx, dy, dz
+        real(8), dimension(:,:,:), intent(in), target :: u, v, w
+        real(8), dimension(:,:,:), intent(out), target :: conv_x, conv_y, conv_z
+        call check(c_ns_cal_conv(ctx, c_loc(u), c_loc(v), c_loc(w), int(bc_x, c_int), int(bc_y, c_int), &
+            int(bc_z, c_int), dx, dy, dz, c_loc(conv_x), c_loc(conv_y), c_loc(conv_z)), "cal_conv")
+    end subroutine cal_conv
+
+    subroutine cal_diff(u, v, w, bc_x, bc_y, bc_z, dx2, dy2, dz2, diff_x, diff_y, diff_z)
+        integer, intent(in) :: bc_x, bc_y, bc_z
+        real(8), intent(in) :: dx2, dy2, dz2
+        real(8), dimension(:,:,:), intent(in), target :: u, v, w
+        real(8), dimension(:,:,:), intent(out), target :: diff_x, diff_y, diff_z
+        call check(c_ns_cal_diff(ctx, c_loc(u), c_loc(v), c_loc(w), int(bc_x, c_int), int(bc_y, c_int), &
+            int(bc_z, c_int), dx2, dy2, dz2, c_loc(diff_x), c_loc(diff_y), c_loc(diff_z)), "cal_diff")
+    end subroutine cal_diff
+
+    ! copy one 2-D plane into / out of the packed set (plane q, 0-based order of include/ns_hip.h)
+    integer function plane_offset(q)
+        integer, intent(in) :: q
+        integer :: d(3, 4), c, k, s, off
+        d(:, 1) = [cfg_saved%nx + 1, cfg_saved%ny + 2, cfg_saved%nz + 2]
+        d(:, 2) = [cfg_saved%nx + 2, cfg_saved%ny + 1, cfg_saved%nz + 2]
+        d(:, 3) = [cfg_saved%nx + 2, cfg_saved%ny + 2, cfg_saved%nz + 1]
+        d(:, 4) = [cfg_saved%nx + 2, cfg_saved%ny + 2, cfg_saved%nz + 2]
+        off = 0
+        do k = 0, q - 1
+            c = k / 6 + 1
+            select case (mod(k, 6) / 2)
+            case (0); s = d(2, c) * d(3, c)
+            case (1); s = d(1, c) * d(3, c)
+            case default; s = d(1, c) * d(2, c)
+            end select
+            off = off + s
+        end do
+        plane_offset = off
+    end function plane_offset
+
+    subroutine put_plane(q, b)
+        integer, intent(in) :: q
+        real(8), intent(in) :: b(:,:)
+        integer :: o
+        o = plane_offset(q)
+        pset(o + 1:o + size(b)) = reshape(b, [size(b)])
+    end subroutine put_plane
+
+    subroutine get_plane(q, b)
+        integer, intent(in) :: q
+        real(8), intent(out) :: b(:,:)
+        integer :: o
+        o = plane_offset(q)
+        b = reshape(pset(o + 1:o + size(b)), shape(b))
+    end subroutine get_plane
+
+    subroutine vel_bc_staggered(u_star, v_star, w_star, &
+        bx_u_1, bx_u_nx, by_u_1, by_u_ny, bz_u_1, bz_u_nz, &
+        bx_v_1, bx_v_nx, by_v_1, by_v_ny, bz_v_1, bz_v_nz, &
+        bx_w_1, bx_w_nx, by_w_1, by_w_ny, bz_w_1, bz_w_nz, bc_x, bc_y, bc_z)
+        integer, intent(in) :: bc_x, bc_y, bc_z
+        real(8), dimension(:,:,:), intent(inout), target :: u_star, v_star, w_star
+        real(8), dimension(:,:), intent(in) :: bx_u_1, bx_u_nx, by_u_1, by_u_ny, bz_u_1, bz_u_nz
+        real(8), dimension(:,:), intent(in) :: bx_v_1, bx_v_nx, by_v_1, by_v_ny, bz_v_1, bz_v_nz
+        real(8), dimension(:,:), intent(in) :: bx_w_1, bx_w_nx, by_w_1, by_w_ny, bz_w_1, bz_w_nz
+        call put_plane(0, bx_u_1); call put_plane(1, bx_u_nx); call put_plane(2, by_u_1)
+        call put_plane(3, by_u_ny); call put_plane(4, bz_u_1); call put_plane(5, bz_u_nz)
+        call put_plane(6, bx_v_1); call put_plane(7, bx_v_nx); call put_plane(8, by_v_1)
+        call put_plane(9, by_v_ny); call put_plane(10, bz_v_1); call put_plane(11, bz_v_nz)
+        call put_plane(12, bx_w_1); call put_plane(13, bx_w_nx); call put_plane(14, by_w_1)
+        call put_plane(15, by_w_ny); call put_plane(16, bz_w_1); call put_plane(17, bz_w_nz)
+        call check(c_ns_vel_bc(ctx, c_loc(u_star), c_loc(v_star), c_loc(w_star), c_loc(pset), int(bc_x, c_int), &
+            int(bc_y, c_int), int(bc_z, c_int)), "vel_bc_staggered")
+    end subroutine vel_bc_staggered
+
+    subroutine vel_bc_staggered_CN2(u_star, v_star, w_star, u_1, u_end, v_1, v_end, w_1, w_end, bc, dir)
+        integer, intent(in) :: bc, dir
+        real(8), dimension(:,:,:), intent(inout), target :: u_star, v_star, w_star
+        real(8), dimension(:,:), intent(in) :: u_1, u_end, v_1, v_end, w_1, w_end
+        integer :: a
+        a = 2 * (dir - 1)
+        if (bc == 2 .or. bc == 4) then
+            call put_plane(a, u_1); call put_plane(a + 1, u_end)
+            call put_plane(6 + a, v_1); call put_plane(7 + a, v_end)
+            call put_plane(12 + a, w_1); call put_plane(13 + a, w_end)
+        end if
+        call check(c_ns_vel_bc_cn2(ctx, c_loc(u_star), c_loc(v_star), c_loc(w_star), c_loc(pset), int(bc, c_int), &
+            int(dir, c_int)), "vel_bc_staggered_CN2")
+    end subroutine vel_bc_staggered_CN2
+
+    subroutine adi_sweep(u_star, v_star, w_star, dir)
+        integer, intent(in) :: dir
+        real(8), dimension(:,:,:), intent(inout), target :: u_star, v_star, w_star
+        call check(c_ns_adi_sweep(ctx, c_loc(u_star), c_loc(v_star), c_loc(w_star), int(dir, c_int)), "adi_sweep")
+    end subroutine adi_sweep
+
+    ! the grid sizes, codes and spacings come from the context (they equal the
+    ! reference's explicit arguments for the arrays it was created for)
+    subroutine get_velpr_bc(u_bc, v_bc, w_bc, p_bc, dt, dp_flag, using_Ustar, p_previous, &
+        bx_u_1, bx_u_nx, by_u_1, by_u_ny, bz_u_1, bz_u_nz, bx_v_1, bx_v_nx, by_v_1, by_v_ny, bz_v_1, bz_v_nz, &
+        bx_w_1, bx_w_nx, by_w_1, by_w_ny, bz_w_1, bz_w_nz, bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz)
+        real(8), intent(in), target :: u_bc(:,:,:), v_bc(:,:,:), w_bc(:,:,:), p_bc(:,:,:), p_previous(:,:,:)
+        real(8), intent(in) :: dt
+        integer, intent(in) :: dp_flag
+        logical, intent(in) :: using_Ustar
+        real(8), dimension(:,:), intent(out) :: bx_u_1, bx_u_nx, by_u_1, by_u_ny, bz_u_1, bz_u_nz
+        real(8), dimension(:,:), intent(out) :: bx_v_1, bx_v_nx, by_v_1, by_v_ny, bz_v_1, bz_v_nz
+        real(8), dimension(:,:), intent(out) :: bx_w_1, bx_w_nx, by_w_1, by_w_ny, bz_w_1, bz_w_nz
+        real(8), dimension(:,:), intent(out) :: bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz
+        integer(c_int) :: us
+        us = 0
+        if (using_Ustar) us = 1
+        pset = 0
+        call check(c_ns_get_velpr_bc(ctx, c_loc(u_bc), c_loc(v_bc), c_loc(w_bc), c_loc(p_bc), dt, int(dp_flag, c_int), &
+            us, c_loc(p_previous), c_loc(pset)), "get_velpr_bc")
+        call get_plane(0, bx_u_1); call get_plane(1, bx_u_nx); call get_plane(2, by_u_1)
+        call get_plane(3, by_u_ny); call get_plane(4, bz_u_1); call get_plane(5, bz_u_nz)
+        call get_plane(6, bx_v_1); call get_plane(7, bx_v_nx); call get_plane(8, by_v_1)
+        call get_plane(9, by_v_ny); call get_plane(10, bz_v_1); call get_plane(11, bz_v_nz)
+        call get_plane(12, bx_w_1); call get_plane(13, bx_w_nx); call get_plane(14, by_w_1)
+        call get_plane(15, by_w_ny); call get_plane(16, bz_w_1); call get_plane(17, bz_w_nz)
+        call get_plane(18, bx_p_1); call get_plane(19, bx_p_nx); call get_plane(20, by_p_1)
+        call get_plane(21, by_p_ny); call get_plane(22, bz_p_1); call get_plane(23, bz_p_nz)
+    end subroutine get_velpr_bc
+
+    subroutine get_pr_bc(p_bc, bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz)
+        real(8), intent(in), target :: p_bc(:,:,:)
+        real(8), dimension(:,:), intent(out) :: bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz
+        pset = 0
+        call check(c_ns_get_pr_bc(ctx, c_loc(p_bc), c_loc(pset)), "get_pr_bc")
+        call get_plane(18, bx_p_1); call get_plane(19, bx_p_nx); call get_plane(20, by_p_1)
+        call get_plane(21, by_p_ny); call get_plane(22, bz_p_1); call get_plane(23, bz_p_nz)
+    end subroutine get_pr_bc
+
+    ! the reference passes the planes' interior sections (2:np-1); here the full planes
+    subroutine pr_bc_staggered_modify_rhs(RHS_poisson_internal, bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz)
+        real(8), intent(inout), target :: RHS_poisson_internal(:,:,:)
+        real(8), dimension(:,:), intent(in) :: bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz
+        call put_plane(18, bx_p_1); call put_plane(19, bx_p_nx); call put_plane(20, by_p_1)
+        call put_plane(21, by_p_ny); call put_plane(22, bz_p_1); call put_plane(23, bz_p_nz)
+        call check(c_ns_modify_rhs(ctx, c_loc(RHS_poisson_internal), c_loc(pset)), "pr_bc_staggered_modify_rhs")
+    end subroutine pr_bc_staggered_modify_rhs
+
+    subroutine pr_bc_staggered(dp, bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz)
+        real(8), intent(inout), target :: dp(:,:,:)
+        real(8), dimension(:,:), intent(in) :: bx_p_1, bx_p_nx, by_p_1, by_p_ny, bz_p_1, bz_p_nz
+        call put_plane(18, bx_p_1); call put_plane(19, bx_p_nx); call put_plane(20, by_p_1)
+        call put_plane(21, by_p_ny); call put_plane(22, bz_p_1); call put_plane(23, bz_p_nz)
+        call check(c_ns_pr_bc(ctx, c_loc(dp), c_loc(pset)), "pr_bc_staggered")
+    end subroutine pr_bc_staggered
+
+    subroutine DCT_poisson_solver(rhs, mean_dp)
+        real(8), intent(inout), target :: rhs(:,:,:)
+        real(8), intent(in) :: mean_dp
+        call check(c_ns_dct_poisson(ctx, c_loc(rhs), mean_dp), "DCT_poisson_solver")
+    end subroutine DCT_poisson_solver
+
+    subroutine poisson_fft(rhs, sol)
+        real(8), intent(in), target :: rhs(:,:,:)
+        real(8), intent(out), target :: sol(:,:,:)
+        call check(c_ns_poisson_fft(ctx, c_loc(rhs), c_loc(sol)), "poisson_fft")
+    end subroutine poisson_fft
+
+    subroutine divergence(u_star, v_star, w_star, dt, rhs)
+        real(8), intent(in), target :: u_star(:,:,:), v_star(:,:,:), w_star(:,:,:)
+        real(8), intent(in) :: dt
+        real(8), intent(out), target :: rhs(:,:,:)
+        call check(c_ns_divergence(ctx, c_loc(u_star), c_loc(v_star), c_loc(w_star), dt, c_loc(rhs)), "divergence")
+    end subroutine divergence
+
+    subroutine project(u_star, v_star, w_star, dp, dt, dp_flag, u, v, w, p)
+        real(8), intent(in), target :: u_star(:,:,:), v_star(:,:,:), w_star(:,:,:), dp(:,:,:)
+        real(8), intent(in) :: dt
+        integer, intent(in) :: dp_flag
+        real(8), intent(inout), target :: u(:,:,:), v(:,:,:), w(:,:,:), p(:,:,:)
+        call check(c_ns_project(ctx, c_loc(u_star), c_loc(v_star), c_loc(w_star), c_loc(dp), dt, int(dp_flag, c_int), &
+            c_loc(u), c_loc(v), c_loc(w), c_loc(p)), "project")
+    end subroutine project
+
+end module ns_hip_mod

@@ -78,3 +78,13 @@ def unpack_planes(nx: int, ny: int, nz: int, buf: np.ndarray) -> dict:
 
 def interior_count(nx: int, ny: int, nz: int) -> int:
     return nx * ny * nz
+
+
+def subbox(a: np.ndarray, comp: str, x0: int, y0: int, z0: int, nx0: int, ny0: int, nz0: int) -> np.ndarray:
+    """Parent-field slice of a sub-box at 1-based offset (x0,y0,z0), as the
+    reference's idx_* index vectors select it (big_periodic_3D.f90:21-25):
+    face axes take x0..x0+n0, centred axes x0..x0+n0+1 (ghosts included)."""
+    ext = {"u": (nx0 + 1, ny0 + 2, nz0 + 2), "v": (nx0 + 2, ny0 + 1, nz0 + 2),
+           "w": (nx0 + 2, ny0 + 2, nz0 + 1), "p": (nx0 + 2, ny0 + 2, nz0 + 2)}[comp]
+    o = (x0 - 1, y0 - 1, z0 - 1)
+    return np.asfortranarray(a[o[0]:o[0] + ext[0], o[1]:o[1] + ext[1], o[2]:o[2] + ext[2]])
