@@ -84,6 +84,7 @@ int ns_state_pointers(ns_ctx *ctx, double **ptrs11);
  *      re_simulation_sub_tstep_interBC_combine.f90:508-802) ---- */
 /* current boundary-plane set used by the following steps (host or device) */
 int ns_set_bc_planes(ns_ctx *ctx, const double *planes);
+int ns_get_bc_planes(ns_ctx *ctx, double *planes);
 /* current set = (1-theta)*a + theta*b on the device: inter-boundary time
  * interpolation (MATLAB interp1 of sub_tstep_BC_combined_interpolation.m:61-64,
  * linear form) */
@@ -102,6 +103,24 @@ int ns_step_profiled(ns_ctx *ctx, int nsteps, double *phase_ms);
  * snapshots: snapshot s is at time s*snap_dt, step m uses
  * t = t0 + (m+1)*dt (the boundary data of the step's end time). */
 int ns_resim_advance(ns_ctx *ctx, const double *snapshots, int nsnap, double snap_dt, double t0, int nsteps);
+/* Parent-run capture of a sub-box's boundary data, the per-step sub-box
+ * dump of big_periodic_3D.f90:433-489,664-712 reduced to what a re-simulation
+ * reads: after ns_capture_begin, every `every`-th step writes one plane set
+ * of the sub-box (its length is ns_plane_set_length(sb->nx,sb->ny,sb->nz))
+ * into dst + k*length for the k-th capture (k < max_sets; dst is device
+ * memory).  Velocity planes follow get_velpr_bc with wall code 2 on u*, v*, w*
+ * (NS_functions.f90:446-503); for AB2-CN they are taken after each ADI
+ * sweep of their axis (the post-sweep planes bx_u_1s ... of
+ * big_periodic_3D.f90:433-489).  Pressure planes are the code-3 gradients of
+ * the gauged dp (get_pr_bc, NS_functions.f90:534-536) and the trailing
+ * scalar is mean(dp) over the sub-box interior (...interBC_combine.f90:771).
+ * The sub-box origin (x0,y0,z0) is 1-based, as the reference's idx_* vectors. */
+typedef struct ns_subbox {
+    int x0, y0, z0;
+    int nx, ny, nz;
+} ns_subbox;
+int ns_capture_begin(ns_ctx *ctx, const ns_subbox *sb, double *dst, int max_sets, int every);
+int ns_capture_end(ns_ctx *ctx, int *nsets_written);
 /* max |div u| over interior cells (big_periodic_3D.f90:576-579) */
 int ns_max_div(ns_ctx *ctx, double *out);
 int ns_check_finite(ns_ctx *ctx);

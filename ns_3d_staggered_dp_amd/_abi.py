@@ -48,10 +48,13 @@ SIGNATURES = {
     "ns_synchronize": [P],
     "ns_state_pointers": [P, P],
     "ns_set_bc_planes": [P, P],
+    "ns_get_bc_planes": [P, P],
     "ns_bc_interp": [P, P, P, D],
     "ns_step": [P, I],
     "ns_step_profiled": [P, I, P],
     "ns_resim_advance": [P, P, I, D, D, I],
+    "ns_capture_begin": [P, P, P, I, I],
+    "ns_capture_end": [P, ctypes.POINTER(I)],
     "ns_max_div": [P, ctypes.POINTER(D)],
     "ns_check_finite": [P],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],

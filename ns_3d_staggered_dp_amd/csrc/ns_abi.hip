@@ -30,6 +30,13 @@ struct ns_ctx {
     PoissonPlans pp{};
     LineSys cn[3][2]{};
     int first = 1;
+    // parent-run capture of a sub-box's boundary plane sets
+    int cap_on = 0, cap_every = 1, cap_max = 0, cap_count = 0, cap_step = 0;
+    Grid cap_g{0, 0, 0};
+    int cap_o[3] = {0, 0, 0};
+    long cap_len = 0;
+    double *cap_dst = nullptr;
+    double *cap_partial = nullptr;
     int poisson_ok = 0;  // 0: mixed periodic / non-periodic codes (no transform solver)
     std::vector<void *> allocs;
 };
@@ -339,20 +346,41 @@ int run_poisson(ns_ctx *c, const double *us, const double *vs, const double *ws,
     return NS_OK;
 }
 
+// capture the planes of axis ax (velocity) from u*, v*, w* into the current set
+void cap_vel(ns_ctx *c, int ax) {
+    const ns_config &f = c->cfg;
+    const double d[3] = {f.dx, f.dy, f.dz};
+    double *dst = c->cap_dst + (size_t)c->cap_len * c->cap_count;
+    launch_capture(c->g, c->cap_g, c->cap_o, ax, 0, d, nullptr, c->us, c->vs, c->ws, dst, c->st);
+}
+void cap_pressure(ns_ctx *c, const double *gauge) {
+    const ns_config &f = c->cfg;
+    const double d[3] = {f.dx, f.dy, f.dz};
+    double *dst = c->cap_dst + (size_t)c->cap_len * c->cap_count;
+    for (int ax = 0; ax < 3; ax++)
+        launch_capture(c->g, c->cap_g, c->cap_o, ax, 1, d, gauge, c->dp, nullptr, nullptr, dst, c->st);
+    PlaneLayout Ls = plane_layout(c->cap_g.nx, c->cap_g.ny, c->cap_g.nz);
+    launch_subbox_mean(c->g, c->cap_g, c->cap_o, gauge, c->dp, c->cap_partial, dst + Ls.off[24], c->st);
+}
+
 // one time step on the context stream; ev (optional) receives NS_NPHASES+1
 // events: before the predictor and after each phase
 int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     const ns_config &f = c->cfg;
     const Grid &g = c->g;
+    const bool cap = c->cap_on && c->cap_count < c->cap_max && ((c->cap_step + 1) % c->cap_every == 0);
     if (ev) (void)hipEventRecord(ev[0], c->st);
     launch_predictor(f, c->first, c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz, c->us, c->vs, c->ws, c->st);
     if (ev) (void)hipEventRecord(ev[1], c->st);
     if (f.scheme != NS_AB2_CN) {
         launch_vel_bc(g, f.bc, c->planes, c->us, c->vs, c->ws, c->st);
+        if (cap)
+            for (int ax = 0; ax < 3; ax++) cap_vel(c, ax);
     } else {
         for (int dir = 1; dir <= 3; dir++) {
             launch_vel_bc_cn2(g, f.bc[dir - 1], dir, c->planes, c->us, c->vs, c->ws, c->st);
             launch_adi_sweep(g, dir, c->cn[dir - 1][0], c->cn[dir - 1][1], c->us, c->vs, c->ws, c->st);
+            if (cap) cap_vel(c, dir - 1);  // post-sweep planes (big_periodic_3D.f90:433-489)
         }
     }
     if (ev) (void)hipEventRecord(ev[2], c->st);
@@ -362,10 +390,15 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     launch_pr_bc(g, f.pbc, c->planes, f.dx, f.dy, f.dz, c->dp, c->st);
     if (ev) (void)hipEventRecord(ev[8], c->st);
     const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
+    if (cap) cap_pressure(c, gauge);
     launch_project(g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
                    c->st);
     if (ev) (void)hipEventRecord(ev[9], c->st);
     c->first = 0;
+    if (c->cap_on) {
+        if (cap) c->cap_count++;
+        c->cap_step++;
+    }
     return NS_OK;
 }
 
@@ -530,6 +563,13 @@ int ns_set_bc_planes(ns_ctx *c, const double *planes) {
     return NS_OK;
 }
 
+int ns_get_bc_planes(ns_ctx *c, double *planes) {
+    if (!c || !planes) return NS_ERR_INVALID;
+    HIPCHK(hipMemcpyAsync(planes, c->planes, (size_t)c->L.total * 8, hipMemcpyDefault, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
 int ns_bc_interp(ns_ctx *c, const double *a, const double *b, double theta) {
     if (!c || !a || !b) return NS_ERR_INVALID;
     ArgList args;
@@ -602,6 +642,42 @@ int ns_resim_advance(ns_ctx *c, const double *snapshots, int nsnap, double snap_
         if (rc) return rc;
     }
     if (hipGetLastError() != hipSuccess) return NS_ERR_HIP;
+    return NS_OK;
+}
+
+int ns_capture_begin(ns_ctx *c, const ns_subbox *sb, double *dst, int max_sets, int every) {
+    if (!c || !sb || !dst || max_sets < 1 || every < 1) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    if (sb->nx < 2 || sb->ny < 2 || sb->nz < 2 || sb->x0 < 1 || sb->y0 < 1 || sb->z0 < 1) return NS_ERR_INVALID;
+    // the sub-box's ghosted extent must lie inside the parent's ghosted arrays
+    if (sb->x0 + sb->nx > g.nx + 1 || sb->y0 + sb->ny > g.ny + 1 || sb->z0 + sb->nz > g.nz + 1)
+        return NS_ERR_INVALID;
+    if (!is_dev_ptr(dst)) return NS_ERR_INVALID;
+    if (!c->cap_partial) {
+        int rc = dalloc(c, &c->cap_partial, (size_t)1024);
+        if (rc) return rc;
+    }
+    c->cap_g = Grid{sb->nx, sb->ny, sb->nz};
+    c->cap_o[0] = sb->x0 - 1;
+    c->cap_o[1] = sb->y0 - 1;
+    c->cap_o[2] = sb->z0 - 1;
+    c->cap_len = plane_layout(sb->nx, sb->ny, sb->nz).total;
+    c->cap_dst = dst;
+    c->cap_max = max_sets;
+    c->cap_every = every;
+    c->cap_count = 0;
+    c->cap_step = 0;
+    c->cap_on = 1;
+    HIPCHK(hipMemsetAsync(dst, 0, (size_t)c->cap_len * max_sets * 8, c->st));
+    return NS_OK;
+}
+
+int ns_capture_end(ns_ctx *c, int *nsets) {
+    if (!c) return NS_ERR_INVALID;
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (nsets) *nsets = c->cap_count;
+    c->cap_on = 0;
+    c->cap_dst = nullptr;
     return NS_OK;
 }
 

@@ -92,6 +92,13 @@ int launch_max_div(const Grid &g, double dx, double dy, double dz, const double 
                    double *partial, hipStream_t st);
 int max_div_partials(const Grid &g);
 void launch_count_nonfinite(const double *a, size_t n, unsigned int *count, hipStream_t st);
+// sub-box plane capture from a parent field (what 0: velocity planes of wall code 2 from
+// s0..s2 = u*,v*,w*; what 1: pressure-gradient planes from s0 = dp); o = 0-based origin
+void launch_capture(const Grid &gp, const Grid &gs, const int o[3], int ax, int what, const double d[3],
+                    const double *gauge, const double *s0, const double *s1, const double *s2, double *planes,
+                    hipStream_t st);
+void launch_subbox_mean(const Grid &gp, const Grid &gs, const int o[3], const double *gauge, const double *dp,
+                        double *partial, double *out, hipStream_t st);
 
 // ---- Poisson (ns_poisson.hip) ----
 struct PoissonPlans {

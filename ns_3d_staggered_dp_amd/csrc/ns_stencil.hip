@@ -518,6 +518,89 @@ __global__ void __launch_bounds__(256) k_count_nonfinite(const double *__restric
     if (local) atomicAdd(count, local);
 }
 
+
+// ---- parent-run capture of a sub-box's boundary planes ----
+// (big_periodic_3D.f90:433-489 stores the post-sweep planes of the sub-box;
+// get_velpr_bc/get_pr_bc, NS_functions.f90:446-560, define the plane values)
+struct CapParams {
+    Grid gp, gs;      // parent and sub-box grids
+    int o[3];         // 0-based offset of the sub-box origin in the parent arrays
+    int ax;           // axis of the planes written by this launch
+    int what;         // 0 velocity planes (wall code 2), 1 pressure-gradient planes (code 3)
+    double d[3];
+    const double *gauge;  // parent corner gauge value (dp(1,1,1)) or null
+    PlaneLayout L;        // layout of the sub-box plane set
+    const double *src[3];
+    double *planes;
+};
+
+__global__ void __launch_bounds__(256) k_capture_axis(CapParams P) {
+    int o1, o2;
+    other_axes(P.ax, o1, o2);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = (P.what == 0) ? 3 : 1;
+    for (int c = 0; c < nc; c++) {
+        const int cc = (P.what == 0) ? c : 3;
+        int ns_[3], np_[3];
+        comp_dims(P.gs, cc, ns_);
+        comp_dims(P.gp, cc, np_);
+        if (t >= (size_t)ns_[o1] * ns_[o2]) continue;
+        int id[3];
+        id[o1] = (int)(t % ns_[o1]);
+        id[o2] = (int)(t / ns_[o1]);
+        const int N = ns_[P.ax];
+        const double *A = P.src[c];
+        const double gc = P.gauge ? *P.gauge : 0.0;
+        auto val = [&](int pos) -> double {  // pos is 1-based along ax in the sub-box
+            int a[3] = {id[0] + P.o[0], id[1] + P.o[1], id[2] + P.o[2]};
+            a[P.ax] = pos - 1 + P.o[P.ax];
+            double x = A[(size_t)a[0] + (size_t)np_[0] * ((size_t)a[1] + (size_t)np_[1] * (size_t)a[2])];
+            return P.gauge ? x - gc : x;
+        };
+        if (P.what == 0) {
+            double *b1 = P.planes + P.L.off[6 * c + 2 * P.ax], *bn = P.planes + P.L.off[6 * c + 2 * P.ax + 1];
+            if (c == P.ax) { b1[t] = val(1); bn[t] = val(N); }
+            else { b1[t] = (val(1) + val(2)) / 2; bn[t] = (val(N) + val(N - 1)) / 2; }
+        } else {
+            double *b1 = P.planes + P.L.off[18 + 2 * P.ax], *bn = P.planes + P.L.off[19 + 2 * P.ax];
+            double dd = P.d[P.ax];
+            b1[t] = (val(2) - val(1)) / dd;
+            bn[t] = (val(N) - val(N - 1)) / dd;
+        }
+    }
+}
+
+// deterministic two-pass mean of the gauged dp over the sub-box interior
+__global__ void __launch_bounds__(256) k_subbox_partial(CapParams P, double *partial) {
+    __shared__ double red[256];
+    const size_t n = P.gs.ni();
+    const double gc = P.gauge ? *P.gauge : 0.0;
+    double s = 0.0;
+    const int P1 = P.gp.nx + 2, P2 = P.gp.ny + 2;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        int a = (int)(q % P.gs.nx);
+        size_t r = q / P.gs.nx;
+        int b = (int)(r % P.gs.ny), c = (int)(r / P.gs.ny);
+        double x = P.src[0][(size_t)(a + 1 + P.o[0]) + (size_t)P1 * ((size_t)(b + 1 + P.o[1]) + (size_t)P2 * (size_t)(c + 1 + P.o[2]))];
+        s += P.gauge ? x - gc : x;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_subbox_mean(const double *partial, int np, double inv_n, double *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < np; i++) s += partial[i];
+        *out = s * inv_n;
+    }
+}
+
 inline unsigned int nblocks(size_t n, int bs = 256) { return (unsigned int)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -741,6 +824,40 @@ void launch_count_nonfinite(const double *a, size_t n, unsigned int *count, hipS
     size_t nb = nblocks(n);
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)nb), dim3(256), 0, st, a, n, count);
+}
+
+
+void launch_capture(const Grid &gp, const Grid &gs, const int o[3], int ax, int what, const double d[3],
+                    const double *gauge, const double *s0, const double *s1, const double *s2, double *planes,
+                    hipStream_t st) {
+    CapParams P;
+    P.gp = gp; P.gs = gs;
+    for (int a = 0; a < 3; a++) { P.o[a] = o[a]; P.d[a] = d[a]; }
+    P.ax = ax; P.what = what;
+    P.gauge = gauge;
+    P.L = plane_layout(gs.nx, gs.ny, gs.nz);
+    P.src[0] = s0; P.src[1] = s1; P.src[2] = s2;
+    P.planes = planes;
+    size_t m = what == 0 ? max_plane(gs, ax) : 0;
+    if (what == 1) {
+        int n[3] = {gs.nx + 2, gs.ny + 2, gs.nz + 2};
+        int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        m = (size_t)n[o1] * n[o2];
+    }
+    hipLaunchKernelGGL(k_capture_axis, dim3(nblocks(m)), dim3(256), 0, st, P);
+}
+
+void launch_subbox_mean(const Grid &gp, const Grid &gs, const int o[3], const double *gauge, const double *dp,
+                        double *partial, double *out, hipStream_t st) {
+    CapParams P{};
+    P.gp = gp; P.gs = gs;
+    for (int a = 0; a < 3; a++) P.o[a] = o[a];
+    P.gauge = gauge;
+    P.src[0] = dp;
+    int nb = (int)nblocks(gs.ni());
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_subbox_partial, dim3(nb), dim3(256), 0, st, P, partial);
+    hipLaunchKernelGGL(k_subbox_mean, dim3(1), dim3(64), 0, st, partial, nb, 1.0 / (double)gs.ni(), out);
 }
 
 }  // namespace ns

@@ -117,6 +117,11 @@ class NSSolver:
     def set_bc_planes(self, planes):
         _abi.call("ns_set_bc_planes", self.h, _ptr(planes))
 
+    def get_bc_planes(self):
+        out = np.zeros(self.nplanes)
+        _abi.call("ns_get_bc_planes", self.h, _ptr(out))
+        return out
+
     def bc_interp(self, a, b, theta: float):
         _abi.call("ns_bc_interp", self.h, _ptr(a), _ptr(b), float(theta))
 

@@ -902,6 +902,82 @@ void nso_project(double *u, double *v, double *w, double *p, const double *us, c
     for (size_t q = 0; q < NP; q++) p[q] = dp_flag * p[q] + dp[q];
 }
 
+
+/* ------------------------------------------------------------------ */
+/* parent-run capture of a sub-box's plane set (big_periodic_3D.f90:  */
+/* 433-489: post-sweep planes; get_velpr_bc/get_pr_bc semantics)      */
+/* ------------------------------------------------------------------ */
+static struct { int on, x0, y0, z0, nx, ny, nz; double *dst; } g_cap;
+
+void nso_set_capture(int x0, int y0, int z0, int nx0, int ny0, int nz0, double *dst)
+{
+    g_cap.on = dst != NULL;
+    g_cap.x0 = x0; g_cap.y0 = y0; g_cap.z0 = z0;
+    g_cap.nx = nx0; g_cap.ny = ny0; g_cap.nz = nz0;
+    g_cap.dst = dst;
+}
+
+/* value of parent component array A (dims np) at sub-box 1-based index (i,j,k) */
+static double sub_at(const double *A, const int np[3], int i, int j, int k)
+{
+    return A[IX(i + g_cap.x0 - 1, j + g_cap.y0 - 1, k + g_cap.z0 - 1, np[0], np[1])];
+}
+
+static void capture_vel_axis(const double *us, const double *vs, const double *ws, const nso_config *cfg, int ax)
+{
+    int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz, sx = g_cap.nx, sy = g_cap.ny, sz = g_cap.nz;
+    int npar[3][3] = {{nx + 1, ny + 2, nz + 2}, {nx + 2, ny + 1, nz + 2}, {nx + 2, ny + 2, nz + 1}};
+    int nsub[3][3] = {{sx + 1, sy + 2, sz + 2}, {sx + 2, sy + 1, sz + 2}, {sx + 2, sy + 2, sz + 1}};
+    const double *src[3] = {us, vs, ws};
+    for (int c = 0; c < 3; c++) {
+        int *n = nsub[c];
+        double *b1 = plane_w(g_cap.dst, sx, sy, sz, 6 * c + 2 * ax), *bn = plane_w(g_cap.dst, sx, sy, sz, 6 * c + 2 * ax + 1);
+        int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        for (int q = 1; q <= n[o2]; q++)
+            for (int pp = 1; pp <= n[o1]; pp++) {
+                int a[4][3];
+                for (int t = 0; t < 4; t++) { a[t][o1] = pp; a[t][o2] = q; }
+                a[0][ax] = 1; a[1][ax] = 2; a[2][ax] = n[ax]; a[3][ax] = n[ax] - 1;
+                double v1 = sub_at(src[c], npar[c], a[0][0], a[0][1], a[0][2]);
+                double v2 = sub_at(src[c], npar[c], a[1][0], a[1][1], a[1][2]);
+                double e1 = sub_at(src[c], npar[c], a[2][0], a[2][1], a[2][2]);
+                double e2 = sub_at(src[c], npar[c], a[3][0], a[3][1], a[3][2]);
+                size_t pl = (size_t)(pp - 1) + (size_t)n[o1] * (size_t)(q - 1);
+                if (c == ax) { b1[pl] = v1; bn[pl] = e1; }
+                else { b1[pl] = (v1 + v2) / 2; bn[pl] = (e1 + e2) / 2; }
+            }
+    }
+}
+
+static void capture_pressure(const double *dp, const nso_config *cfg)
+{
+    int nx = cfg->nx, ny = cfg->ny, sx = g_cap.nx, sy = g_cap.ny, sz = g_cap.nz;
+    int npar[3] = {nx + 2, ny + 2, cfg->nz + 2}, n[3] = {sx + 2, sy + 2, sz + 2};
+    double d[3] = {cfg->dx, cfg->dy, cfg->dz};
+    for (int ax = 0; ax < 3; ax++) {
+        double *b1 = plane_w(g_cap.dst, sx, sy, sz, 18 + 2 * ax), *bn = plane_w(g_cap.dst, sx, sy, sz, 19 + 2 * ax);
+        int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        for (int q = 1; q <= n[o2]; q++)
+            for (int pp = 1; pp <= n[o1]; pp++) {
+                int a[4][3];
+                for (int t = 0; t < 4; t++) { a[t][o1] = pp; a[t][o2] = q; }
+                a[0][ax] = 1; a[1][ax] = 2; a[2][ax] = n[ax]; a[3][ax] = n[ax] - 1;
+                double v1 = sub_at(dp, npar, a[0][0], a[0][1], a[0][2]), v2 = sub_at(dp, npar, a[1][0], a[1][1], a[1][2]);
+                double e1 = sub_at(dp, npar, a[2][0], a[2][1], a[2][2]), e2 = sub_at(dp, npar, a[3][0], a[3][1], a[3][2]);
+                size_t pl = (size_t)(pp - 1) + (size_t)n[o1] * (size_t)(q - 1);
+                b1[pl] = (v2 - v1) / d[ax];
+                bn[pl] = (e1 - e2) / d[ax];
+            }
+    }
+    double s = 0;
+    for (int k = 2; k <= sz + 1; k++)
+        for (int j = 2; j <= sy + 1; j++)
+            for (int i = 2; i <= sx + 1; i++) s += sub_at(dp, npar, i, j, k);
+    long s24[24], o25[25];
+    nso_plane_layout(sx, sy, sz, s24, o25);
+    g_cap.dst[o25[24]] = s / ((double)sx * sy * sz);
+}
+
 /* ------------------------------------------------------------------ */
 /* one time step: the loop body of big_periodic_3D.f90:326-571 and      */
 /* re_simulation_sub_tstep_interBC_combine.f90:508-802                  */
@@ -989,9 +1065,12 @@ int nso_step(const nso_config *cfg, double *u, double *v, double *w, double *p,
         for (int dir = 1; dir <= 3; dir++) {
             nso_vel_bc_cn2(us, vs, ws, nx, ny, nz, planes, bcv[dir - 1], dir);
             adi_sweep(us, vs, ws, cfg, dir);
+            if (g_cap.on) capture_vel_axis(us, vs, ws, cfg, dir - 1);
         }
     } else {
         nso_vel_bc(us, vs, ws, nx, ny, nz, planes, bcx, bcy, bcz);
+        if (g_cap.on)
+            for (int ax = 0; ax < 3; ax++) capture_vel_axis(us, vs, ws, cfg, ax);
     }
 
     double *rhs = malloc(sizeof(double) * NI);
@@ -1015,6 +1094,7 @@ int nso_step(const nso_config *cfg, double *u, double *v, double *w, double *p,
         double c0 = dp[0];
         for (size_t q = 0; q < NP; q++) dp[q] = dp[q] - c0;
     }
+    if (g_cap.on) capture_pressure(dp, cfg);
     nso_project(u, v, w, p, us, vs, ws, dp, nx, ny, nz, dx, dy, dz, dt, cfg->dp_flag);
     free(rhs); free(sol);
     (void)NU; (void)NV; (void)NW;

@@ -232,3 +232,26 @@ class OracleState:
     def max_div(self):
         c = self.cfg
         return max_div(self.u, self.v, self.w, c.dx, c.dy, c.dz)
+
+
+class capture:
+    """Context manager: the oracle's next steps write the sub-box plane set of
+    each step into consecutive rows of the returned array."""
+
+    def __init__(self, x0, y0, z0, nx0, ny0, nz0, nsteps):
+        self.spec = (x0, y0, z0, nx0, ny0, nz0)
+        self.len = plane_set_length(nx0, ny0, nz0)
+        self.buf = np.zeros((nsteps, self.len))
+        self.k = 0
+
+    def __enter__(self):
+        return self
+
+    def arm(self):
+        """point the capture at the next row (call before each step)"""
+        row = self.buf[self.k]
+        lib().nso_set_capture(*self.spec, row.ctypes.data_as(_D))
+        self.k += 1
+
+    def __exit__(self, *exc):
+        lib().nso_set_capture(0, 0, 0, 0, 0, 0, None)

@@ -1,0 +1,100 @@
+"""GPU: parent-run plane capture, device-side time interpolation and the
+walled sub-box re-simulation (re_simulation_sub_tstep_interBC path), checked
+against the oracle and against the parent itself (self-consistency)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from ns_3d_staggered_dp_amd import AB2, AB2_CN, FLAG_GAUGE_CORNER, NSConfig, fields
+from ns_3d_staggered_dp_amd import _abi
+from ns_3d_staggered_dp_amd.grid import plane_set_length, subbox
+from ns_3d_staggered_dp_amd.solver import NSSolver
+
+pytestmark = pytest.mark.gpu
+
+BOX = ((5, 4, 6), (8, 10, 6))
+
+
+class SubBox(ctypes.Structure):
+    _fields_ = [("x0", ctypes.c_int), ("y0", ctypes.c_int), ("z0", ctypes.c_int),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)]
+
+
+def gpu_parent(scheme, nsteps, n=20, dt=4e-3, every=1):
+    import torch
+
+    (x0, y0, z0), (a, b, c) = BOX
+    cfg = NSConfig.cube(n, nu=2e-3, dt=dt, scheme=scheme, dp_flag=1, flags=FLAG_GAUGE_CORNER)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=4)
+    L = plane_set_length(a, b, c)
+    nsets = nsteps // every
+    buf = torch.zeros(nsets * L, dtype=torch.float64, device="cuda")
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, first=True)
+        s.step(3)
+        st = s.download()
+        start = {k: subbox(st[k], cmp, x0, y0, z0, a, b, c)
+                 for cmp, k in (("u", "u"), ("v", "v"), ("w", "w"), ("p", "p"), ("u", "hx"), ("v", "hy"), ("w", "hz"))}
+        sb = SubBox(x0, y0, z0, a, b, c)
+        _abi.call("ns_capture_begin", s.h, ctypes.byref(sb), ctypes.c_void_p(buf.data_ptr()), nsets, every)
+        snaps = []
+        for k in range(nsteps):
+            s.step(1)
+            if (k + 1) % every == 0:
+                st = s.download()
+                snaps.append({kk: subbox(st[kk], cmp, x0, y0, z0, a, b, c)
+                              for cmp, kk in (("u", "u"), ("v", "v"), ("w", "w"), ("p", "p"))})
+        cnt = ctypes.c_int()
+        _abi.call("ns_capture_end", s.h, ctypes.byref(cnt))
+        assert cnt.value == nsets
+    scfg = NSConfig(nx=a, ny=b, nz=c, dx=cfg.dx, dy=cfg.dy, dz=cfg.dz, nu=cfg.nu, dt=cfg.dt, scheme=scheme,
+                    bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
+    return cfg, scfg, start, buf, snaps
+
+
+@pytest.mark.parametrize("scheme", [AB2, AB2_CN])
+def test_capture_matches_oracle(oracle, scheme):
+    """captured plane sets (GPU) == the oracle's (same parent run), to the
+    step-parity bar (the parents themselves agree to ~1e-15)"""
+    from test_oracle import parent_capture_run
+
+    _, _, _, planes_ref, _ = parent_capture_run(oracle, scheme)
+    _, _, _, buf, _ = gpu_parent(scheme, 6)
+    got = buf.cpu().numpy().reshape(planes_ref.shape)
+    assert np.abs(got - planes_ref).max() <= 1e-11 * np.abs(planes_ref).max()
+
+
+@pytest.mark.parametrize("scheme", [AB2, AB2_CN])
+def test_resim_from_captured_history_reproduces_parent(oracle, scheme):
+    """sub-box driven by the device-resident captured history through
+    ns_resim_advance: snapshot k sits at k*dt and step m ends at t0+(m+1)*dt,
+    so t0 = -dt makes step m read snapshot m exactly"""
+    nsteps = 6
+    cfg, scfg, start, buf, snaps = gpu_parent(scheme, nsteps)
+    with NSSolver(scfg) as s:
+        s.upload(start["u"], start["v"], start["w"], start["p"], start["hx"], start["hy"], start["hz"], first=False)
+        s.resim_advance(buf, nsteps, scfg.dt, -scfg.dt, nsteps)
+        got = s.download()
+    ref = snaps[-1]
+    for name in ("u", "v", "w", "p"):
+        err = np.abs(got[name] - ref[name]).max() / np.sqrt(np.mean(ref[name] ** 2))
+        assert err < 1e-12, (name, err)
+
+
+def test_device_time_interpolation_is_linear():
+    """ns_bc_interp and ns_resim_advance between snapshots give the linear
+    interpolation (1-theta)*a + theta*b of the bracketing plane sets"""
+    cfg = NSConfig(nx=8, ny=7, nz=6, dx=0.3, dy=0.3, dz=0.3, nu=1e-3, dt=0.25, bc=(2, 2, 2), pbc=(3, 3, 3),
+                   dp_flag=1)
+    L = plane_set_length(8, 7, 6)
+    snaps = np.random.default_rng(3).standard_normal((4, L))
+    with NSSolver(cfg) as s:
+        s.bc_interp(snaps[1], snaps[2], 0.3)
+        assert np.array_equal(s.get_bc_planes(), (1.0 - 0.3) * snaps[1] + 0.3 * snaps[2])
+        u, v, w, p = fields.random_ghosted(cfg, 5)
+        s.upload(0.01 * u, 0.01 * v, 0.01 * w, p, first=True)
+        # one step from t0=0.2 ends at t=0.45: snapshots 1 (t=0.25) and 2 (t=0.5), theta=0.8
+        s.resim_advance(np.ascontiguousarray(snaps.ravel()), 4, 0.25, 0.2, 1)
+        th = (0.2 + 0.25 - 0.25) / 0.25
+        assert np.abs(s.get_bc_planes() - ((1.0 - th) * snaps[1] + th * snaps[2])).max() < 1e-15
