@@ -77,7 +77,9 @@ int ns_download_state(ns_ctx *ctx, double *u, double *v, double *w, double *p, d
 /* u*, v*, w* and dp of the last step (dp gauged like the driver's). Any may be NULL. */
 int ns_download_work(ns_ctx *ctx, double *u_star, double *v_star, double *w_star, double *dp);
 int ns_synchronize(ns_ctx *ctx);
-/* device pointers of the resident state, in the order u,v,w,p,hx,hy,hz,u*,v*,w*,dp */
+/* device pointers of the resident state, in the order u,v,w,p,hx,hy,hz,u*,v*,w*,dp; valid
+ * until the next step (u* alternates between two buffers when dp_flag == 0, and u,v,w,p
+ * are then formed from u*, dp only by ns_download_state / ns_max_div / ns_check_finite) */
 int ns_state_pointers(ns_ctx *ctx, double **ptrs11);
 
 /* ---- fused time step (the loop body of big_periodic_3D.f90:326-571 and

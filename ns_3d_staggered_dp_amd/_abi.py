@@ -80,6 +80,14 @@ def lib():
     """Load libns_hip.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # PyTorch-ROCm bundles its own libamdhip64.so with the same soname
+        # (libamdhip64.so.7) as /opt/rocm's.  Loading torch first makes this
+        # library bind to that copy, so a process holds one HIP runtime; loading
+        # ours first and torch later would start a second runtime that finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -m ns_3d_staggered_dp_amd.build`"

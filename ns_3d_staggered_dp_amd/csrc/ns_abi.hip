@@ -5,6 +5,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <deque>
+#include <utility>
 #include <vector>
 
 #include "ns_internal.h"
@@ -20,6 +22,10 @@ struct ns_ctx {
     double *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
     double *hx = nullptr, *hy = nullptr, *hz = nullptr;
     double *us = nullptr, *vs = nullptr, *ws = nullptr, *dp = nullptr;
+    double *us2 = nullptr, *vs2 = nullptr, *ws2 = nullptr;  // second u* buffer (lazy projection)
+    // lazy projection (dp_flag == 0): the state is (u*, dp) and u = u* - dt*grad(dp)
+    // is formed inside the next predictor; u, v, w, p are materialised on demand
+    int lazy = 0, stale = 0;
     double *planes = nullptr;   // current plane set
     double *snaps = nullptr;    // snapshot history for ns_resim_advance
     size_t snaps_len = 0;
@@ -307,8 +313,10 @@ void arg_free(Arg &a) {
     a.dev = nullptr;
 }
 
+// std::deque keeps references to earlier elements valid across push_back
+// (a std::vector would reallocate and leave them dangling)
 struct ArgList {
-    std::vector<Arg> v;
+    std::deque<Arg> v;
     ~ArgList() {
         for (auto &a : v) arg_free(a);
     }
@@ -363,6 +371,16 @@ void cap_pressure(ns_ctx *c, const double *gauge) {
     launch_subbox_mean(c->g, c->cap_g, c->cap_o, gauge, c->dp, c->cap_partial, dst + Ls.off[24], c->st);
 }
 
+// lazy mode: form u, v, w, p from the current (u*, dp)
+void materialize(ns_ctx *c) {
+    if (!c->lazy || !c->stale) return;
+    const ns_config &f = c->cfg;
+    const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
+    launch_project(c->g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
+                   c->st);
+    c->stale = 0;
+}
+
 // one time step on the context stream; ev (optional) receives NS_NPHASES+1
 // events: before the predictor and after each phase
 int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
@@ -370,7 +388,17 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     const Grid &g = c->g;
     const bool cap = c->cap_on && c->cap_count < c->cap_max && ((c->cap_step + 1) % c->cap_every == 0);
     if (ev) (void)hipEventRecord(ev[0], c->st);
-    launch_predictor(f, c->first, c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz, c->us, c->vs, c->ws, c->st);
+    if (c->lazy) {
+        const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
+        launch_predict_tiled(f, c->first, true, c->us, c->vs, c->ws, c->dp, gauge, c->p, c->hx, c->hy, c->hz, c->us2,
+                             c->vs2, c->ws2, c->st);
+        std::swap(c->us, c->us2);
+        std::swap(c->vs, c->vs2);
+        std::swap(c->ws, c->ws2);
+    } else {
+        launch_predict_tiled(f, c->first, false, c->u, c->v, c->w, nullptr, nullptr, c->p, c->hx, c->hy, c->hz, c->us,
+                             c->vs, c->ws, c->st);
+    }
     if (ev) (void)hipEventRecord(ev[1], c->st);
     if (f.scheme != NS_AB2_CN) {
         launch_vel_bc(g, f.bc, c->planes, c->us, c->vs, c->ws, c->st);
@@ -391,8 +419,10 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     if (ev) (void)hipEventRecord(ev[8], c->st);
     const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
     if (cap) cap_pressure(c, gauge);
-    launch_project(g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
-                   c->st);
+    if (c->lazy) c->stale = 1;
+    else
+        launch_project(g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
+                       c->st);
     if (ev) (void)hipEventRecord(ev[9], c->st);
     c->first = 0;
     if (c->cap_on) {
@@ -431,6 +461,7 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
         (rc = dalloc(c, &c->p, g.np())) || (rc = dalloc(c, &c->hx, g.nu())) || (rc = dalloc(c, &c->hy, g.nv())) ||
         (rc = dalloc(c, &c->hz, g.nw())) || (rc = dalloc(c, &c->us, g.nu())) || (rc = dalloc(c, &c->vs, g.nv())) ||
         (rc = dalloc(c, &c->ws, g.nw())) || (rc = dalloc(c, &c->dp, g.np())) ||
+        (rc = dalloc(c, &c->us2, g.nu())) || (rc = dalloc(c, &c->vs2, g.nv())) || (rc = dalloc(c, &c->ws2, g.nw())) ||
         (rc = dalloc(c, &c->planes, (size_t)c->L.total)) || (rc = dalloc(c, &c->partial, (size_t)4096)) ||
         (rc = dalloc(c, &c->counter, (size_t)1)))
         return fail(rc);
@@ -471,6 +502,10 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
     HIPCHK(hipMemsetAsync(c->vs, 0, g.nv() * 8, c->st));
     HIPCHK(hipMemsetAsync(c->ws, 0, g.nw() * 8, c->st));
     HIPCHK(hipMemsetAsync(c->dp, 0, g.np() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->us2, 0, g.nu() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->vs2, 0, g.nv() * 8, c->st));
+    HIPCHK(hipMemsetAsync(c->ws2, 0, g.nw() * 8, c->st));
+    c->lazy = (cfg->dp_flag == 0);
     HIPCHK(hipMemsetAsync(c->planes, 0, (size_t)c->L.total * 8, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     *out = c;
@@ -508,6 +543,14 @@ int ns_upload_state(ns_ctx *c, const double *u, const double *v, const double *w
     else HIPCHK(hipMemsetAsync(c->hy, 0, g.nv() * 8, c->st));
     if (hz) HIPCHK(hipMemcpyAsync(c->hz, hz, g.nw() * 8, hipMemcpyDefault, c->st));
     else HIPCHK(hipMemsetAsync(c->hz, 0, g.nw() * 8, c->st));
+    if (c->lazy) {
+        // u = u* - dt*grad(0) reproduces u exactly
+        HIPCHK(hipMemcpyAsync(c->us, c->u, g.nu() * 8, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->vs, c->v, g.nv() * 8, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->ws, c->w, g.nw() * 8, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemsetAsync(c->dp, 0, g.np() * 8, c->st));
+        c->stale = 0;
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     c->first = first ? 1 : 0;
     return NS_OK;
@@ -516,6 +559,7 @@ int ns_upload_state(ns_ctx *c, const double *u, const double *v, const double *w
 int ns_download_state(ns_ctx *c, double *u, double *v, double *w, double *p, double *hx, double *hy, double *hz) {
     if (!c) return NS_ERR_INVALID;
     const Grid &g = c->g;
+    materialize(c);
     if (u) HIPCHK(hipMemcpyAsync(u, c->u, g.nu() * 8, hipMemcpyDefault, c->st));
     if (v) HIPCHK(hipMemcpyAsync(v, c->v, g.nv() * 8, hipMemcpyDefault, c->st));
     if (w) HIPCHK(hipMemcpyAsync(w, c->w, g.nw() * 8, hipMemcpyDefault, c->st));
@@ -683,6 +727,7 @@ int ns_capture_end(ns_ctx *c, int *nsets) {
 
 int ns_max_div(ns_ctx *c, double *out) {
     if (!c || !out) return NS_ERR_INVALID;
+    materialize(c);
     const ns_config &f = c->cfg;
     int nb = launch_max_div(c->g, f.dx, f.dy, f.dz, c->u, c->v, c->w, c->partial, c->st);
     std::vector<double> h(nb);
@@ -696,6 +741,7 @@ int ns_max_div(ns_ctx *c, double *out) {
 
 int ns_check_finite(ns_ctx *c) {
     if (!c) return NS_ERR_INVALID;
+    materialize(c);
     HIPCHK(hipMemsetAsync(c->counter, 0, sizeof(unsigned int), c->st));
     launch_count_nonfinite(c->u, c->g.nu(), c->counter, c->st);
     launch_count_nonfinite(c->v, c->g.nv(), c->counter, c->st);

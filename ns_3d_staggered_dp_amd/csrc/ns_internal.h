@@ -61,8 +61,11 @@ struct LineSys {
 struct Ctx;
 
 // ---- stencil / BC / projection launchers (ns_stencil.hip) ----
-void launch_predictor(const ns_config &c, int first, const double *u, const double *v, const double *w, const double *p,
-                      double *hx, double *hy, double *hz, double *us, double *vs, double *ws, hipStream_t st);
+// fused predictor (ns_predict.hip); lazy: (u,v,w) are the previous u*,v*,w* and the
+// projection with dp (and the corner gauge, if non-null) is applied on load
+void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+                          const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
+                          double *us, double *vs, double *ws, hipStream_t st);
 void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff,
                       const double *u, const double *v, const double *w, double *ox, double *oy, double *oz,
                       hipStream_t st);

@@ -40,7 +40,8 @@ __device__ __forceinline__ void conv_diff_u(const FieldsC &f, int per, int fi, i
     int fl = (per && fi == 1) ? f.nx : fi - 1;
     double c0 = UU(fi, J, K);
     double ur = (UU(fi + 1, J, K) + c0) / 2.0;
-    double ul = (c0 + UU(fl, J, K)) / 2.0;
+    // avg(bc=1) wraps with the array's last two entries: (u(nx+1)+u(nx))/2 (FD_functions.f90:252)
+    double ul = (per && fi == 1) ? (UU(f.nx + 1, J, K) + UU(f.nx, J, K)) / 2.0 : (c0 + UU(fl, J, K)) / 2.0;
     double t1 = ur * ur - ul * ul;
     double t2 = fluxE(f, fi, J, K) - fluxE(f, fi, J - 1, K);
     double t3 = fluxF(f, fi, J, K) - fluxF(f, fi, J, K - 1);
@@ -62,7 +63,7 @@ __device__ __forceinline__ void conv_diff_v(const FieldsC &f, int per, int I, in
     double c0 = VV(I, fj, K);
     double t1 = fluxE(f, I, fj, K) - fluxE(f, I - 1, fj, K);
     double vr = (VV(I, fj + 1, K) + c0) / 2.0;
-    double vl = (c0 + VV(I, fl, K)) / 2.0;
+    double vl = (per && fj == 1) ? (VV(I, f.ny + 1, K) + VV(I, f.ny, K)) / 2.0 : (c0 + VV(I, fl, K)) / 2.0;
     double t2 = vr * vr - vl * vl;
     double t3 = fluxG(f, I, fj, K) - fluxG(f, I, fj, K - 1);
     double r = t1 / dx;
@@ -84,7 +85,7 @@ __device__ __forceinline__ void conv_diff_w(const FieldsC &f, int per, int I, in
     double t1 = fluxF(f, I, J, fk) - fluxF(f, I - 1, J, fk);
     double t2 = fluxG(f, I, J, fk) - fluxG(f, I, J - 1, fk);
     double wr = (WW(I, J, fk + 1) + c0) / 2.0;
-    double wl = (c0 + WW(I, J, fl)) / 2.0;
+    double wl = (per && fk == 1) ? (WW(I, J, f.nz + 1) + WW(I, J, f.nz)) / 2.0 : (c0 + WW(I, J, fl)) / 2.0;
     double t3 = wr * wr - wl * wl;
     double r = t1 / dx;
     r = r + t2 / dy;
@@ -118,66 +119,6 @@ __device__ __forceinline__ bool in_range(int c, const int bc[3], int nx, int ny,
         if (id[a] < lo || id[a] > hi) return false;
     }
     return true;
-}
-
-struct PredParams {
-    int nx, ny, nz;
-    double dx, dy, dz, nu, dt;
-    int scheme, first, dp_flag;
-    int bc[3];
-    const double *u, *v, *w, *p;
-    double *h[3];
-    double *out[3];
-};
-
-__global__ void __launch_bounds__(256) k_predictor(PredParams P) {
-    const int c = blockIdx.y;
-    Grid g{P.nx, P.ny, P.nz};
-    int n[3];
-    comp_dims(g, c, n);
-    size_t tot = (size_t)n[0] * n[1] * n[2];
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= tot) return;
-    int i = (int)(q % n[0]) + 1;
-    size_t r = q / n[0];
-    int j = (int)(r % n[1]) + 1;
-    int k = (int)(r / n[1]) + 1;
-    FieldsC f{P.u, P.v, P.w, P.nx, P.ny, P.nz};
-    const double *v0 = (c == 0) ? P.u : (c == 1) ? P.v : P.w;
-    double *H = P.h[c];
-    bool in = in_range(c, P.bc, P.nx, P.ny, P.nz, i, j, k);
-    double cv = 0.0, df = 0.0;
-    if (in) conv_diff(f, c, P.bc, i, j, k, P.dx, P.dy, P.dz, P.dx * P.dx, P.dy * P.dy, P.dz * P.dz, cv, df);
-    double dpd = 0.0;
-    if (P.dp_flag) {
-        int b[3] = {i, j, k};
-        b[c]++;
-        double dd = (c == 0) ? P.dx : (c == 1) ? P.dy : P.dz;
-        dpd = (P.p[FIX(b[0], b[1], b[2], P.nx + 2, P.ny + 2)] - P.p[FIX(i, j, k, P.nx + 2, P.ny + 2)]) / dd;
-    }
-    const double dpf = (double)P.dp_flag;
-    const double dt = P.dt, nu = P.nu;
-    double rhs, val;
-    if (P.scheme == NS_EULER) {
-        rhs = in ? nu * df - cv : 0.0;
-        val = dt * (1.0 * rhs - dpf * dpd) + v0[q];
-        H[q] = 0.0;
-    } else if (P.scheme == NS_AB2) {
-        rhs = in ? nu * df - cv : 0.0;
-        if (P.first) val = dt * (1.0 * rhs - dpf * dpd) + v0[q];
-        else val = dt * (1.5 * rhs - 0.5 * H[q] - dpf * dpd) + v0[q];
-        H[q] = rhs;
-    } else {
-        if (P.first) {
-            rhs = in ? 0.5 * nu * df - cv : 0.0;
-            val = dt * (rhs - dpf * dpd) + v0[q];
-        } else {
-            rhs = in ? 0.5 * nu * df - 1.5 * cv : 0.0;
-            val = dt * (rhs - 0.5 * H[q] - dpf * dpd) + v0[q];
-        }
-        if (in) H[q] = -cv;
-    }
-    P.out[c][q] = val;
 }
 
 struct CDParams {
@@ -620,22 +561,6 @@ PlaneLayout plane_layout(int nx, int ny, int nz) {
     L.off[24] = off;
     L.total = off + 1;
     return L;
-}
-
-void launch_predictor(const ns_config &c, int first, const double *u, const double *v, const double *w, const double *p,
-                      double *hx, double *hy, double *hz, double *us, double *vs, double *ws, hipStream_t st) {
-    PredParams P;
-    P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
-    P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
-    P.scheme = c.scheme; P.first = first; P.dp_flag = c.dp_flag;
-    for (int a = 0; a < 3; a++) P.bc[a] = c.bc[a];
-    P.u = u; P.v = v; P.w = w; P.p = p;
-    P.h[0] = hx; P.h[1] = hy; P.h[2] = hz;
-    P.out[0] = us; P.out[1] = vs; P.out[2] = ws;
-    Grid g{c.nx, c.ny, c.nz};
-    size_t mx = g.nu() > g.nv() ? g.nu() : g.nv();
-    mx = mx > g.nw() ? mx : g.nw();
-    hipLaunchKernelGGL(k_predictor, dim3(nblocks(mx), 3), dim3(256), 0, st, P);
 }
 
 void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff, const double *u,
