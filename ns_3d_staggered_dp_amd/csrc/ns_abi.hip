@@ -477,7 +477,7 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
             return fail(rc);
         if ((rc = make_eigs(c))) return fail(rc);
         if (allp) {
-            if ((rc = dalloc(c, &c->spec, (size_t)(cfg->nx / 2 + 1) * cfg->ny * cfg->nz))) return fail(rc);
+            if ((rc = dalloc(c, &c->spec, (size_t)spectrum_row(cfg->nx) * cfg->ny * cfg->nz))) return fail(rc);
         } else {
             if ((rc = make_quarter_wave(c, cfg->nx, &c->pp.wq_x)) || (rc = make_quarter_wave(c, cfg->ny, &c->pp.wq_y)) ||
                 (rc = make_quarter_wave(c, cfg->nz, &c->pp.wq_z)))

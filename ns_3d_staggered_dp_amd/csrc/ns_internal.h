@@ -111,6 +111,8 @@ struct PoissonPlans {
     const double *ex, *ey, *ez;  // per-axis eigenvalue parts (FFT: nx/2+1 / ny / nz; DCT: nx/ny/nz)
     const double2 *wq_x, *wq_y, *wq_z;  // Makhoul quarter-wave twiddles exp(-i pi k/(2n)), DCT only
 };
+// row stride (complex) of the periodic half spectrum [z][y][kx]
+int spectrum_row(int nx);
 // ev (optional): 5 events recorded after the five passes
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,

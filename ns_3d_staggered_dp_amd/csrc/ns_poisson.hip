@@ -122,19 +122,31 @@ template <> struct SmallDFT<3> { __device__ static void run(double2 *v, int s) {
 template <> struct SmallDFT<5> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<5>::run(v, s); } };
 template <> struct SmallDFT<7> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<7>::run(v, s); } };
 
-// one Stockham stage over L lines of length N (line stride LS) in LDS
+// LDS layout of L lines of length N: element (line, pos) at line*SL + pos*SP.
+// Line-major (SL = padded length, SP = 1) suits lines that are contiguous in
+// memory; line-interleaved (SL = 1, SP = L) suits strided lines, whose global
+// loads arrive line-fastest.  Threads walk butterflies in the same fastest
+// index so LDS accesses of a wavefront are unit-stride in either layout.
+struct Lay {
+    int SL, SP;
+};
+
+// one Stockham stage over L lines of length N
 template <int R>
-__device__ __forceinline__ void stage(const double2 *__restrict__ A, double2 *__restrict__ B, int N, int LS, int L,
+__device__ __forceinline__ void stage(const double2 *__restrict__ A, double2 *__restrict__ B, int N, Lay ly, int L,
                                       int Ns, const double2 *__restrict__ tw, int sign) {
     const int NR = N / R;
     const int nb = L * NR;
     const int tstep = N / (Ns * R);
+    const bool line_fast = (ly.SL == 1);
     for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-        int line = t / NR, j = t - line * NR;
-        const double2 *in = A + line * LS;
+        int line, j;
+        if (line_fast) { line = t % L; j = t / L; }
+        else { line = t / NR; j = t - line * NR; }
+        const double2 *in = A + line * ly.SL;
         double2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = in[j + r * NR];
+        for (int r = 0; r < R; r++) v[r] = in[(j + r * NR) * ly.SP];
         int jm = j % Ns;
         if (jm) {
 #pragma unroll
@@ -146,23 +158,23 @@ __device__ __forceinline__ void stage(const double2 *__restrict__ A, double2 *__
         }
         SmallDFT<R>::run(v, sign);
         int idxD = (j - jm) * R + jm;
-        double2 *out = B + line * LS;
+        double2 *out = B + line * ly.SL;
 #pragma unroll
-        for (int r = 0; r < R; r++) out[idxD + r * Ns] = v[r];
+        for (int r = 0; r < R; r++) out[(idxD + r * Ns) * ly.SP] = v[r];
     }
 }
 
 // full transform; data starts in A, result pointer returned (A or B)
-__device__ double2 *fft_lds(double2 *A, double2 *B, const FFTPlan &pl, int LS, int L, int sign) {
+__device__ double2 *fft_lds(double2 *A, double2 *B, const FFTPlan &pl, Lay ly, int L, int sign) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; s++) {
         switch (pl.radix[s]) {
-            case 8: stage<8>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
-            case 4: stage<4>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
-            case 2: stage<2>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
-            case 3: stage<3>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
-            case 5: stage<5>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
-            case 7: stage<7>(A, B, pl.n, LS, L, Ns, pl.tw, sign); break;
+            case 8: stage<8>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
+            case 4: stage<4>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
+            case 2: stage<2>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
+            case 3: stage<3>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
+            case 5: stage<5>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
+            case 7: stage<7>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
         }
         __syncthreads();
         Ns *= pl.radix[s];
@@ -231,6 +243,7 @@ struct PassParams {
     int L;       // complex lines per block
     int LS;      // LDS line stride (complex)
     int nxh;     // nx/2+1 (periodic)
+    int nxhp;    // spectrum row stride: nxh rounded up to a multiple of 8
     DivSrc src;
     double2 *spec;                          // periodic spectrum [z][y][kx]
     double *work;                           // DCT work array [z][y][x]
@@ -262,7 +275,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(PassParams P) {
         slot[rline & 1] = val;
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, LS, L, -1);
+    double2 *Z = fft_lds(A, B, P.pl, Lay{LS, 1}, L, -1);
     const int nxh = P.nxh;
     for (int f = threadIdx.x; f < 2 * L * nxh; f += blockDim.x) {
         int rline = f / nxh, kx = f - rline * nxh;
@@ -273,41 +286,44 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(PassParams P) {
         double2 X;
         if ((rline & 1) == 0) X = make_double2((a.x + b.x) * 0.5, (a.y - b.y) * 0.5);
         else X = make_double2((a.y + b.y) * 0.5, (b.x - a.x) * 0.5);
-        P.spec[(size_t)rl * nxh + kx] = X;
+        P.spec[(size_t)rl * P.nxhp + kx] = X;
     }
 }
 
-// strided complex lines: along y (stride nxh) for fixed (kx,z), or along z
-// (stride nxh*ny) for fixed (kx,y); L consecutive kx per block
+// strided complex lines: along y (stride nxhp) for fixed (kx,z), or along z
+// (stride nxhp*ny) for fixed (kx,y); L consecutive kx per block.  Spectrum
+// rows are padded to nxhp (a multiple of 8 complex = 128 B), so a group of 8
+// wavenumbers is one aligned cache line.
 template <int AXIS>
-__device__ __forceinline__ size_t spec_index(const Grid &g, int nxh, int kx, int pos, int other) {
-    if (AXIS == 1) return ((size_t)other * g.ny + pos) * nxh + kx;
-    return ((size_t)pos * g.ny + other) * nxh + kx;
+__device__ __forceinline__ size_t spec_index(const Grid &g, int nxhp, int kx, int pos, int other) {
+    if (AXIS == 1) return ((size_t)other * g.ny + pos) * nxhp + kx;
+    return ((size_t)pos * g.ny + other) * nxhp + kx;
 }
 
 template <int AXIS, int SIGN, int SOLVE>
 __global__ void __launch_bounds__(256) k_fft_yz(PassParams P) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
-    const int N = (AXIS == 1) ? g.ny : g.nz, L = P.L, LS = P.LS, nxh = P.nxh;
-    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const int N = (AXIS == 1) ? g.ny : g.nz, L = P.L, nxh = P.nxh, nxhp = P.nxhp;
+    double2 *A = smem, *B = smem + (size_t)L * N;
+    const Lay ly{1, L};  // line-interleaved: element (l, pos) at pos*L + l
     const int ngrp = (nxh + L - 1) / L;
     const int kx0 = (blockIdx.x % ngrp) * L;
     const int other = blockIdx.x / ngrp;
     for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
         int l = f % L, pos = f / L;
         int kx = kx0 + l;
-        A[l * LS + pos] = (kx < nxh) ? P.spec[spec_index<AXIS>(g, nxh, kx, pos, other)] : make_double2(0.0, 0.0);
+        A[f] = (kx < nxh) ? P.spec[spec_index<AXIS>(g, nxhp, kx, pos, other)] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, LS, L, SIGN);
+    double2 *Z = fft_lds(A, B, P.pl, ly, L, SIGN);
     if (SOLVE) {
         // modal division (z pass): eig = -4*((ex + ey) + ez); mode (0,0,0) -> 0
         for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-            int l = f / N, kz = f - l * N;
+            int l = f % L, kz = f / L;
             int kx = kx0 + l;
             if (kx >= nxh) continue;
-            double2 *z = Z + l * LS + kz;
+            double2 *z = Z + f;
             double e = P.ex[kx];
             e = e + P.ey[other];
             e = e + P.ez[kz];
@@ -317,12 +333,12 @@ __global__ void __launch_bounds__(256) k_fft_yz(PassParams P) {
         }
         __syncthreads();
         double2 *other_buf = (Z == A) ? B : A;
-        Z = fft_lds(Z, other_buf, P.pl, LS, L, +1);
+        Z = fft_lds(Z, other_buf, P.pl, ly, L, +1);
     }
     for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
         int l = f % L, pos = f / L;
         int kx = kx0 + l;
-        if (kx < nxh) P.spec[spec_index<AXIS>(g, nxh, kx, pos, other)] = Z[l * LS + pos];
+        if (kx < nxh) P.spec[spec_index<AXIS>(g, nxhp, kx, pos, other)] = Z[f];
     }
 }
 
@@ -337,7 +353,7 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(PassParams P) {
     for (int f = threadIdx.x; f < 2 * L * nxh; f += blockDim.x) {
         int rline = f / nxh, kx = f - rline * nxh;
         long rl = rl0 + rline;
-        double2 X = (rl < RL) ? P.spec[(size_t)rl * nxh + kx] : make_double2(0.0, 0.0);
+        double2 X = (rl < RL) ? P.spec[(size_t)rl * P.nxhp + kx] : make_double2(0.0, 0.0);
         if (kx == 0 || 2 * kx == N) X.y = 0.0;  // c2r ignores the imaginary parts of DC and Nyquist
         B[rline * nxh + kx] = X;
     }
@@ -358,7 +374,7 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(PassParams P) {
         A[l * LS + m] = make_double2(a.x - b.y, a.y + b.x);
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, LS, L, +1);
+    double2 *Z = fft_lds(A, B, P.pl, Lay{LS, 1}, L, +1);
     const double sc = P.scale;
     for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
         int rline = f / N, x = f - rline * N;
@@ -456,7 +472,7 @@ __global__ void __launch_bounds__(256) k_dct_fwd(PassParams P) {
         slot[rline & 1] = val;
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, LS, L, -1);
+    double2 *Z = fft_lds(A, B, P.pl, Lay{LS, 1}, L, -1);
     double2 *O = (Z == A) ? B : A;
     // post-twiddle into real coefficients, stored as (xa, xb) pairs in O
     for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
@@ -501,7 +517,7 @@ __global__ void __launch_bounds__(256) k_dct_fwd(PassParams P) {
             Z[l * LS + k] = make_double2(va.x - vb.y, va.y + vb.x);
         }
         __syncthreads();
-        double2 *R = fft_lds(Z, O, P.pl, LS, L, +1);
+        double2 *R = fft_lds(Z, O, P.pl, Lay{LS, 1}, L, +1);
         const double inv = 1.0 / N;
         for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
             int rline = f % (2 * L), pos = f / (2 * L);
@@ -562,7 +578,7 @@ __global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
         A[l * LS + k] = make_double2(va.x - vb.y, va.y + vb.x);
     }
     __syncthreads();
-    double2 *R = fft_lds(A, B, P.pl, LS, L, +1);
+    double2 *R = fft_lds(A, B, P.pl, Lay{LS, 1}, L, +1);
     const double inv = 1.0 / N;
     for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
         int rline, pos;
@@ -596,12 +612,19 @@ inline size_t lds_bytes(int L, int LS) { return (size_t)2 * L * LS * sizeof(doub
 
 }  // namespace
 
+int spectrum_row(int nx) { return ((nx / 2 + 1) + 7) / 8 * 8; }
+
+namespace {
+
+}  // namespace
+
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
                        double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
     PassParams P{};
     P.g = g;
     P.nxh = g.nx / 2 + 1;
+    P.nxhp = spectrum_row(g.nx);
     P.src.rhs = rhs_in;
     P.src.us = us; P.src.vs = vs; P.src.ws = ws;
     P.src.dx = dx; P.src.dy = dy; P.src.dz = dz; P.src.dt = dt;
@@ -624,7 +647,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     }
     // y forward
     P.pl = pl.py;
-    P.LS = g.ny + 1;
+    P.LS = g.ny;
     P.L = lines_per_block(P.LS);
     {
         unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);
@@ -633,7 +656,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     }
     // z forward + divide + inverse
     P.pl = pl.pz;
-    P.LS = g.nz + 1;
+    P.LS = g.nz;
     P.L = lines_per_block(P.LS);
     {
         unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.ny);
@@ -642,7 +665,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     }
     // y inverse
     P.pl = pl.py;
-    P.LS = g.ny + 1;
+    P.LS = g.ny;
     P.L = lines_per_block(P.LS);
     {
         unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);

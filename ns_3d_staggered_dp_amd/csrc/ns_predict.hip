@@ -3,10 +3,12 @@
 // (NS_functions.f90:53-210, big_periodic_3D.f90:343-424,
 // re_simulation_sub_tstep_interBC_combine.f90:543-654).
 //
-// Layout of the work: a block owns a 64 x 8 column of cells and streams
+// Layout of the work: a block owns a 64 x 4 column of cells and streams
 // through z ("2.5-D" blocking).  Each component keeps a ring of three z-slices
-// of its halo'd 66 x 10 tile in LDS, so every input value is fetched from
-// memory about 1.3 times (the x/y halo) instead of once per stencil use.  Only
+// of its halo'd 66 x 6 tile in LDS, so every input value is fetched from
+// memory about 1.5 times (the x/y halo, mostly served by L2) instead of once
+// per stencil use; the next slices are prefetched into registers while the
+// current one is computed.  Only
 // the points of the reference's assignment ranges are computed (x_range0 etc.,
 // big_periodic_3D.f90:126-146); the ghost entries of u* are rewritten by the
 // boundary fill or discarded by the ADI sweeps in every scheme, so they are
@@ -18,20 +20,26 @@
 // the projection kernel computes it.  This removes one full read/write pass
 // of u, v, w per step.
 //
-// Operation order inside each formula is the reference's (see ns_stencil.hip),
-// so results are bitwise those of the per-point kernel.
+// Operation order inside each formula is the reference's (see ns_stencil.hip).
+// Divisions by the (per-launch constant) grid spacings are evaluated as
+// q = a*r, q += (a - q*d)*r with r = 1/d rounded on the host (Markstein's
+// correction, two FMAs): the quotient is the correctly rounded a/d except in
+// rare last-bit cases, at a third of the cost of the full IEEE division
+// sequence.  Results therefore agree with the per-point kernel to the last
+// bit almost everywhere and to within one rounding elsewhere.
 #include "ns_internal.h"
 
 namespace ns {
 
 namespace {
 
-constexpr int TX = 64, TY = 8;
+constexpr int TX = 64, TY = 4;
 constexpr int HX = TX + 2, HY = TY + 2;
 
 struct TileParams {
     int nx, ny, nz;
     double dx, dy, dz, nu, dt;
+    double rd[3], rd2[3];  // 1/dx, 1/dy, 1/dz and 1/dx^2, ... rounded on the host
     int scheme, first, dp_flag;
     int per[3];          // own-axis periodicity of u, v, w (bc == 1)
     int zc;              // z cells per block
@@ -42,6 +50,13 @@ struct TileParams {
     double *h[3];
     double *out[3];
 };
+
+// a/d with r = RN(1/d): one product and one FMA correction step
+__device__ __forceinline__ double qdiv(double a, double d, double r) {
+    const double q = a * r;
+    const double e = __builtin_fma(-q, d, a);
+    return __builtin_fma(e, r, q);
+}
 
 __device__ __forceinline__ void cdims(const TileParams &P, int c, int n[3]) {
     n[0] = P.nx + 2; n[1] = P.ny + 2; n[2] = P.nz + 2;
@@ -76,7 +91,7 @@ __device__ __forceinline__ double load_val(const TileParams &P, int c, int gi, i
         }
         double gr = hi - lo;
         const double d = (c == 0) ? P.dx : (c == 1) ? P.dy : P.dz;
-        x = x - P.dt * gr / d;
+        x = x - qdiv(P.dt * gr, d, P.rd[c]);
     }
     return x;
 }
@@ -102,26 +117,103 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
         ox[c] = gidx(c, 0, a0) - 1;
         oy[c] = gidx(c, 1, b0) - 1;
     }
-    // load slice with global z-index gk of component c into its ring slot
-    auto load_slice = [&](int c, int gk) {
-        double(*dst)[HX] = S[c][((gk % 3) + 3) % 3];
-        for (int e = threadIdx.x; e < HX * HY; e += TX * TY) {
-            int la = e % HX, lb = e / HX;
-            dst[lb][la] = load_val<LAZY>(P, c, ox[c] + la, oy[c] + lb, gk);
+    // Per-thread load descriptors: thread e loads halo elements e and e+TX*TY
+    // (HX x HY per slice) of each component.  Their x/y offsets (with the
+    // periodic wrap of the own-axis index 0 and clamping) are fixed for the
+    // whole column, so each slice load is one add and one load.
+    constexpr int NE = 2;
+    unsigned int offc[3][NE], offp[3][NE];  // < 2^32 elements per array (up to ~1600^3)
+    bool valid[NE];
+    int lxy[NE][2];
+    const int P1 = P.nx + 2, P2 = P.ny + 2;
+    const size_t pp = (size_t)P1 * P2;
+    size_t ps[3];
+    int nz3[3];
+    for (int s_ = 0; s_ < NE; s_++) {
+        const int e = threadIdx.x + s_ * TX * TY;
+        valid[s_] = e < HX * HY;
+        const int la = e % HX, lb = e / HX;
+        lxy[s_][0] = la;
+        lxy[s_][1] = lb;
+        for (int c = 0; c < 3; c++) {
+            int n[3];
+            cdims(P, c, n);
+            int gi = ox[c] + la, gj = oy[c] + lb;
+            if (c == 0 && gi == 0 && P.per[0]) gi = P.nx;
+            if (c == 1 && gj == 0 && P.per[1]) gj = P.ny;
+            gi = gi < 1 ? 1 : (gi > n[0] ? n[0] : gi);
+            gj = gj < 1 ? 1 : (gj > n[1] ? n[1] : gj);
+            offc[c][s_] = (unsigned int)((gi - 1) + n[0] * (gj - 1));
+            offp[c][s_] = (unsigned int)((gi - 1) + P1 * (gj - 1));
+            if (s_ == 0) {
+                ps[c] = (size_t)n[0] * n[1];
+                nz3[c] = n[2];
+            }
+        }
+    }
+    const double gauge = (LAZY && P.gauge) ? *P.gauge : 0.0;
+    const double dgrad[3] = {P.dx, P.dy, P.dz};
+    // own-axis step of the dp neighbour used by the lazy projection of component c
+    const size_t dstep[3] = {1, (size_t)P1, pp};
+    auto keff = [&](int c, int gk) {
+        if (c == 2 && gk == 0 && P.per[2]) gk = P.nz;
+        return gk < 1 ? 1 : (gk > nz3[c] ? nz3[c] : gk);
+    };
+    struct Regs {
+        double v[3][NE], lo[3][NE], hi[3][NE];
+    };
+    auto fetch = [&](Regs &R, int c, int gk) {
+        const int k = keff(c, gk);
+        const double *src = P.src[c] + (size_t)(k - 1) * ps[c];
+        const double *dpk = LAZY ? P.dp + (size_t)(k - 1) * pp : nullptr;
+#pragma unroll
+        for (int s_ = 0; s_ < NE; s_++) {
+            if (!valid[s_]) continue;
+            R.v[c][s_] = src[offc[c][s_]];
+            if (LAZY) {
+                R.lo[c][s_] = dpk[offp[c][s_]];
+                R.hi[c][s_] = dpk[offp[c][s_] + dstep[c]];
+            }
         }
     };
+    auto store = [&](const Regs &R, int c, int gk) {
+        double(*dst)[HX] = S[c][gk % 3];
+#pragma unroll
+        for (int s_ = 0; s_ < NE; s_++) {
+            if (!valid[s_]) continue;
+            double x = R.v[c][s_];
+            if (LAZY) {
+                // the projection of big_periodic_3D.f90:562-567, gauge applied first
+                double hi = R.hi[c][s_], lo = R.lo[c][s_];
+                if (P.gauge) {
+                    hi = hi - gauge;
+                    lo = lo - gauge;
+                }
+                double gr = hi - lo;
+                x = x - qdiv(P.dt * gr, dgrad[c], P.rd[c]);
+            }
+            dst[lxy[s_][1]][lxy[s_][0]] = x;
+        }
+    };
+    Regs R;
     // prologue: u, v need slices K-1..K+1 with K = az+2; w needs fk-1..fk+1 with fk = az+1
-    load_slice(0, z0 + 1); load_slice(0, z0 + 2);
-    load_slice(1, z0 + 1); load_slice(1, z0 + 2);
-    load_slice(2, z0 + 0); load_slice(2, z0 + 1);
+    for (int q = 0; q < 3; q++) {
+        fetch(R, 0, z0 + 1 + q); fetch(R, 1, z0 + 1 + q); fetch(R, 2, z0 + q);
+        store(R, 0, z0 + 1 + q); store(R, 1, z0 + 1 + q); store(R, 2, z0 + q);
+    }
+    __syncthreads();
     const double dx = P.dx, dy = P.dy, dz = P.dz, dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+    const double rx = P.rd[0], ry = P.rd[1], rz = P.rd[2], rx2 = P.rd2[0], ry2 = P.rd2[1], rz2 = P.rd2[2];
     const double dt = P.dt, nu = P.nu, dpf = (double)P.dp_flag;
     const int a = a0 + tx, bb = b0 + ty;
     for (int az = z0; az < z1; az++) {
-        load_slice(0, az + 3);
-        load_slice(1, az + 3);
-        load_slice(2, az + 2);
-        __syncthreads();
+        const bool more = az + 1 < z1;
+        // prefetch the slices of the next iteration; their latency overlaps the compute below
+        if (more) {
+            fetch(R, 0, az + 4);
+            fetch(R, 1, az + 4);
+            fetch(R, 2, az + 3);
+        }
         if (a < P.nx && bb < P.ny) {
 #define LU(gi, gj, gk) S[0][(gk) % 3][(gj) - oy[0]][(gi) - ox[0]]
 #define LV(gi, gj, gk) S[1][(gk) % 3][(gj) - oy[1]][(gi) - ox[1]]
@@ -147,16 +239,16 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                     double t1 = ur * ur - ul * ul;
                     double t2 = EF(fi, J, K) - EF(fi, J - 1, K);
                     double t3 = FF(fi, J, K) - FF(fi, J, K - 1);
-                    double r = t1 / dx;
-                    r = r + t2 / dy;
-                    r = r + t3 / dz;
+                    double r = qdiv(t1, dx, rx);
+                    r = r + qdiv(t2, dy, ry);
+                    r = r + qdiv(t3, dz, rz);
                     cv = r;
                     double d1 = LU(fl, J, K) - 2.0 * c0 + LU(fi + 1, J, K);
                     double d2 = LU(fi, J - 1, K) - 2.0 * c0 + LU(fi, J + 1, K);
                     double d3 = LU(fi, J, K - 1) - 2.0 * c0 + LU(fi, J, K + 1);
-                    double s = d1 / dx2;
-                    s = s + d2 / dy2;
-                    s = s + d3 / dz2;
+                    double s = qdiv(d1, dx2, rx2);
+                    s = s + qdiv(d2, dy2, ry2);
+                    s = s + qdiv(d3, dz2, rz2);
                     df = s;
                 } else if (c == 1) {
                     const int I = gi, fj = gj, K = gk;
@@ -169,16 +261,16 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                                     : (c0 + LV(I, fl, K)) / 2.0;
                     double t2 = vr * vr - vl * vl;
                     double t3 = GF(I, fj, K) - GF(I, fj, K - 1);
-                    double r = t1 / dx;
-                    r = r + t2 / dy;
-                    r = r + t3 / dz;
+                    double r = qdiv(t1, dx, rx);
+                    r = r + qdiv(t2, dy, ry);
+                    r = r + qdiv(t3, dz, rz);
                     cv = r;
                     double d1 = LV(I - 1, fj, K) - 2.0 * c0 + LV(I + 1, fj, K);
                     double d2 = LV(I, fl, K) - 2.0 * c0 + LV(I, fj + 1, K);
                     double d3 = LV(I, fj, K - 1) - 2.0 * c0 + LV(I, fj, K + 1);
-                    double s = d1 / dx2;
-                    s = s + d2 / dy2;
-                    s = s + d3 / dz2;
+                    double s = qdiv(d1, dx2, rx2);
+                    s = s + qdiv(d2, dy2, ry2);
+                    s = s + qdiv(d3, dz2, rz2);
                     df = s;
                 } else {
                     const int I = gi, J = gj, fk = gk;
@@ -191,16 +283,16 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                                     ? (load_val<LAZY>(P, 2, I, J, P.nz + 1) + LW(I, J, fl)) / 2.0
                                     : (c0 + LW(I, J, fl)) / 2.0;
                     double t3 = wr * wr - wl * wl;
-                    double r = t1 / dx;
-                    r = r + t2 / dy;
-                    r = r + t3 / dz;
+                    double r = qdiv(t1, dx, rx);
+                    r = r + qdiv(t2, dy, ry);
+                    r = r + qdiv(t3, dz, rz);
                     cv = r;
                     double d1 = LW(I - 1, J, fk) - 2.0 * c0 + LW(I + 1, J, fk);
                     double d2 = LW(I, J - 1, fk) - 2.0 * c0 + LW(I, J + 1, fk);
                     double d3 = LW(I, J, fl) - 2.0 * c0 + LW(I, J, fk + 1);
-                    double s = d1 / dx2;
-                    s = s + d2 / dy2;
-                    s = s + d3 / dz2;
+                    double s = qdiv(d1, dx2, rx2);
+                    s = s + qdiv(d2, dy2, ry2);
+                    s = s + qdiv(d3, dz2, rz2);
                     df = s;
                 }
                 int n[3];
@@ -211,7 +303,8 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                     int bq[3] = {gi, gj, gk};
                     bq[c]++;
                     const double dd = (c == 0) ? dx : (c == 1) ? dy : dz;
-                    dpd = (P.p[FIX(bq[0], bq[1], bq[2], P.nx + 2, P.ny + 2)] - P.p[FIX(gi, gj, gk, P.nx + 2, P.ny + 2)]) / dd;
+                    dpd = qdiv(P.p[FIX(bq[0], bq[1], bq[2], P.nx + 2, P.ny + 2)] - P.p[FIX(gi, gj, gk, P.nx + 2, P.ny + 2)],
+                               dd, P.rd[c]);
                 }
                 double *H = P.h[c];
                 double rhs, val;
@@ -244,6 +337,12 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
 #undef LW
         }
         __syncthreads();
+        if (more) {
+            store(R, 0, az + 4);
+            store(R, 1, az + 4);
+            store(R, 2, az + 3);
+        }
+        __syncthreads();
     }
 }
 
@@ -255,6 +354,11 @@ void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double
     TileParams P;
     P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
     P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
+    const double d[3] = {c.dx, c.dy, c.dz};
+    for (int a = 0; a < 3; a++) {
+        P.rd[a] = 1.0 / d[a];
+        P.rd2[a] = 1.0 / (d[a] * d[a]);
+    }
     P.scheme = c.scheme; P.first = first; P.dp_flag = c.dp_flag;
     for (int a = 0; a < 3; a++) P.per[a] = (c.bc[a] == 1);
     P.zc = 16;
