@@ -505,7 +505,11 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
     HIPCHK(hipMemsetAsync(c->us2, 0, g.nu() * 8, c->st));
     HIPCHK(hipMemsetAsync(c->vs2, 0, g.nv() * 8, c->st));
     HIPCHK(hipMemsetAsync(c->ws2, 0, g.nw() * 8, c->st));
+    // lazy projection (dp folded into the next predictor) needs periodic pressure
+    // on every periodic velocity axis: the predictor reads dp(n+1) as dp(1)
     c->lazy = (cfg->dp_flag == 0);
+    for (int a = 0; a < 3; a++)
+        if (cfg->bc[a] == 1 && cfg->pbc[a] != 1) c->lazy = 0;
     HIPCHK(hipMemsetAsync(c->planes, 0, (size_t)c->L.total * 8, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     *out = c;

@@ -4,21 +4,26 @@
 // re_simulation_sub_tstep_interBC_combine.f90:543-654).
 //
 // Layout of the work: a block owns a 64 x 4 column of cells and streams
-// through z ("2.5-D" blocking).  Each component keeps a ring of three z-slices
+// through z ("2.5-D" blocking).  Each component keeps a ring of four z-slices
 // of its halo'd 66 x 6 tile in LDS, so every input value is fetched from
 // memory about 1.5 times (the x/y halo, mostly served by L2) instead of once
-// per stencil use; the next slices are prefetched into registers while the
-// current one is computed.  Only
-// the points of the reference's assignment ranges are computed (x_range0 etc.,
-// big_periodic_3D.f90:126-146); the ghost entries of u* are rewritten by the
-// boundary fill or discarded by the ADI sweeps in every scheme, so they are
-// left untouched, and so are the ghost entries of the AB2 history.
+// per stencil use.  Each z step stores the slices prefetched into registers
+// during the previous step, issues the loads of the next ones (and of the AB2
+// history of the next cell), passes one barrier and computes; four slots make
+// that single barrier sufficient.  Only the points of the reference's
+// assignment ranges are computed (x_range0 etc., big_periodic_3D.f90:126-146);
+// the ghost entries of u* are rewritten by the boundary fill or discarded by
+// the ADI sweeps in every scheme, so they are left untouched, and so are the
+// ghost entries of the AB2 history.
 //
 // LAZY: the input velocity is the previous step's u* and dp, and the
 // projection u = u* - dt*(dp(i+1)-dp(i))/dx (big_periodic_3D.f90:562-567,
-// with the corner gauge) is evaluated while the tiles are loaded, exactly as
-// the projection kernel computes it.  This removes one full read/write pass
-// of u, v, w per step.
+// with the corner gauge) is applied when the slices are stored to LDS, with
+// dp taken from a three-slice LDS ring of its own (67 x 7 tile), so dp is read
+// from memory once.  This removes one full read/write pass of u, v, w per
+// step.  The periodic wraps use dp(n+1) == dp(1) and dp(n+2) == dp(2), which
+// the periodic pressure fill makes exact copies; the context only runs lazily
+// when every periodic velocity axis has a periodic pressure axis.
 //
 // Operation order inside each formula is the reference's (see ns_stencil.hip).
 // Divisions by the (per-launch constant) grid spacings are evaluated as
@@ -35,6 +40,15 @@ namespace {
 
 constexpr int TX = 64, TY = 4;
 constexpr int HX = TX + 2, HY = TY + 2;
+constexpr int UW = HX + 1;   // u rows: local x = HX holds u(nx+1) (periodic wrap of the x average)
+constexpr int VH = HY + 1;   // v tile: local row HY holds v(ny+1) (periodic wrap of the y average)
+constexpr int DXT = HX + 1, DYT = HY + 1;  // dp tile, covering the lo/hi points of all three components
+constexpr int NVS = 4;       // velocity ring slots: one barrier per z step
+constexpr int NDS = 3;       // dp ring slots
+constexpr int NT = TX * TY;
+constexpr int NE = 2;        // load slots per thread and array (every tile has <= 2*NT elements)
+static_assert(HY * UW <= NE * NT && VH * HX <= NE * NT && HY * HX <= NE * NT && DYT * DXT <= NE * NT,
+              "tile larger than the per-thread load slots");
 
 struct TileParams {
     int nx, ny, nz;
@@ -66,19 +80,18 @@ __device__ __forceinline__ void cdims(const TileParams &P, int c, int n[3]) {
 // global 1-based index of interior cell a (0-based) along axis ax for component c
 __device__ __forceinline__ int gidx(int c, int ax, int a) { return a + ((ax == c) ? 1 : 2); }
 
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// one velocity value with the lazy projection applied, read straight from
+// memory (used for the periodic z wrap of w, once per column)
 template <bool LAZY>
 __device__ __forceinline__ double load_val(const TileParams &P, int c, int gi, int gj, int gk) {
     int n[3];
     cdims(P, c, n);
-    // periodic wrap of the own-axis left neighbour of face 1 (index 0 -> n)
-    if (c == 0 && gi == 0 && P.per[0]) gi = P.nx;
-    if (c == 1 && gj == 0 && P.per[1]) gj = P.ny;
-    if (c == 2 && gk == 0 && P.per[2]) gk = P.nz;
-    gi = gi < 1 ? 1 : (gi > n[0] ? n[0] : gi);
-    gj = gj < 1 ? 1 : (gj > n[1] ? n[1] : gj);
-    gk = gk < 1 ? 1 : (gk > n[2] ? n[2] : gk);
-    size_t q = FIX(gi, gj, gk, n[0], n[1]);
-    double x = P.src[c][q];
+    gi = clampi(gi, 1, n[0]);
+    gj = clampi(gj, 1, n[1]);
+    gk = clampi(gk, 1, n[2]);
+    double x = P.src[c][FIX(gi, gj, gk, n[0], n[1])];
     if (LAZY) {
         const int P1 = P.nx + 2, P2 = P.ny + 2;
         int b[3] = {gi, gj, gk};
@@ -96,14 +109,23 @@ __device__ __forceinline__ double load_val(const TileParams &P, int c, int gi, i
     return x;
 }
 
+// LDS tiles of one block: rings of z-slices of the halo'd tiles
 template <bool LAZY>
-__global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
-    __shared__ double S[3][3][HY][HX];  // [component][slot][y][x]
+struct Tiles {
+    double u[NVS][HY][UW];
+    double v[NVS][VH][HX];
+    double w[NVS][HY][HX];
+    double d[LAZY ? NDS : 1][DYT][DXT];
+};
+
+template <bool LAZY>
+__global__ void __launch_bounds__(NT) k_predict_tiled(TileParams P) {
+    __shared__ Tiles<LAZY> T;
     // XCD-aware tile order: consecutive tiles of one XCD are x/y neighbours
     const int nb = P.ntiles;
-    int b = blockIdx.x;
+    const int b = blockIdx.x;
     const int per_xcd = (nb + 7) / 8;
-    int t = (b % 8) * per_xcd + b / 8;
+    const int t = (b % 8) * per_xcd + b / 8;
     if (t >= nb) return;  // only when nb is not a multiple of 8: padding blocks exit at once
     const int tx_ = t % P.tiles_x;
     const int ty_ = (t / P.tiles_x) % P.tiles_y;
@@ -111,120 +133,211 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
     const int a0 = tx_ * TX, b0 = ty_ * TY, z0 = tz_ * P.zc;
     const int z1 = min(z0 + P.zc, P.nz);
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    // tile origins (global index of local 0) per component
-    int ox[3], oy[3];
-    for (int c = 0; c < 3; c++) {
-        ox[c] = gidx(c, 0, a0) - 1;
-        oy[c] = gidx(c, 1, b0) - 1;
-    }
-    // Per-thread load descriptors: thread e loads halo elements e and e+TX*TY
-    // (HX x HY per slice) of each component.  Their x/y offsets (with the
-    // periodic wrap of the own-axis index 0 and clamping) are fixed for the
-    // whole column, so each slice load is one add and one load.
-    constexpr int NE = 2;
-    unsigned int offc[3][NE], offp[3][NE];  // < 2^32 elements per array (up to ~1600^3)
-    bool valid[NE];
-    int lxy[NE][2];
     const int P1 = P.nx + 2, P2 = P.ny + 2;
     const size_t pp = (size_t)P1 * P2;
-    size_t ps[3];
-    int nz3[3];
+    // tile origins (global index of local 0) per component; the dp tile starts at (a0, b0)
+    const int ox[3] = {a0, a0 + 1, a0 + 1}, oy[3] = {b0 + 1, b0, b0 + 1};
+    int nn[3][3];
+    for (int c = 0; c < 3; c++) cdims(P, c, nn[c]);
+
+    // Per-thread load descriptors (element e = threadIdx.x + s*NT of each tile).
+    // x/y offsets, with the periodic wrap of the own-axis index 0 and clamping,
+    // are fixed for the whole column; each slice load is one add and one load.
+    // Clamped points only feed outputs outside the computed ranges.
+    unsigned int off[3][NE];   // element offset within a z-slice of u, v, w
+    short lds[3][NE];          // LDS position within the slice (-1: no element)
+    unsigned int offd[NE];     // dp tile
+    short ldsd[NE];
+    short plo[3][NE], phi[3][NE];  // dp tile positions of the lazy projection (phi < 0: next dp slice, w)
     for (int s_ = 0; s_ < NE; s_++) {
-        const int e = threadIdx.x + s_ * TX * TY;
-        valid[s_] = e < HX * HY;
-        const int la = e % HX, lb = e / HX;
-        lxy[s_][0] = la;
-        lxy[s_][1] = lb;
-        for (int c = 0; c < 3; c++) {
-            int n[3];
-            cdims(P, c, n);
-            int gi = ox[c] + la, gj = oy[c] + lb;
-            if (c == 0 && gi == 0 && P.per[0]) gi = P.nx;
-            if (c == 1 && gj == 0 && P.per[1]) gj = P.ny;
-            gi = gi < 1 ? 1 : (gi > n[0] ? n[0] : gi);
-            gj = gj < 1 ? 1 : (gj > n[1] ? n[1] : gj);
-            offc[c][s_] = (unsigned int)((gi - 1) + n[0] * (gj - 1));
-            offp[c][s_] = (unsigned int)((gi - 1) + P1 * (gj - 1));
-            if (s_ == 0) {
-                ps[c] = (size_t)n[0] * n[1];
-                nz3[c] = n[2];
-            }
+        const int e = threadIdx.x + s_ * NT;
+        // u: HY rows of UW; column HX is u(nx+1)
+        {
+            const bool ok = e < HY * UW;
+            const int la = e % UW, lb = e / UW;
+            int gi = (la == HX) ? P.nx + 1 : ox[0] + la, gj = oy[0] + lb;
+            if (gi == 0 && P.per[0]) gi = P.nx;
+            gi = clampi(gi, 1, nn[0][0]);
+            gj = clampi(gj, 1, nn[0][1]);
+            off[0][s_] = (unsigned int)((gi - 1) + nn[0][0] * (gj - 1));
+            lds[0][s_] = ok ? (short)(lb * UW + la) : (short)-1;
+            // dp(nx+1) == dp(1) and dp(nx+2) == dp(2) after the periodic pressure fill
+            const int dx0 = (la == HX) ? 1 : la;
+            plo[0][s_] = (short)((lb + 1) * DXT + dx0);
+            phi[0][s_] = (short)((lb + 1) * DXT + dx0 + 1);
+        }
+        // v: VH rows of HX; row HY is v(ny+1)
+        {
+            const bool ok = e < VH * HX;
+            const int la = e % HX, lb = e / HX;
+            int gi = ox[1] + la, gj = (lb == HY) ? P.ny + 1 : oy[1] + lb;
+            if (gj == 0 && P.per[1]) gj = P.ny;
+            gi = clampi(gi, 1, nn[1][0]);
+            gj = clampi(gj, 1, nn[1][1]);
+            off[1][s_] = (unsigned int)((gi - 1) + nn[1][0] * (gj - 1));
+            lds[1][s_] = ok ? (short)(lb * HX + la) : (short)-1;
+            const int dy0 = (lb == HY) ? 1 : lb;
+            plo[1][s_] = (short)(dy0 * DXT + la + 1);
+            phi[1][s_] = (short)((dy0 + 1) * DXT + la + 1);
+        }
+        // w: HY rows of HX
+        {
+            const bool ok = e < HY * HX;
+            const int la = e % HX, lb = e / HX;
+            const int gi = clampi(ox[2] + la, 1, nn[2][0]), gj = clampi(oy[2] + lb, 1, nn[2][1]);
+            off[2][s_] = (unsigned int)((gi - 1) + nn[2][0] * (gj - 1));
+            lds[2][s_] = ok ? (short)(lb * HX + la) : (short)-1;
+            plo[2][s_] = (short)((lb + 1) * DXT + la + 1);
+            phi[2][s_] = plo[2][s_];
+        }
+        // dp: DYT rows of DXT starting at (a0, b0)
+        {
+            const bool ok = LAZY && e < DYT * DXT;
+            const int la = e % DXT, lb = e / DXT;
+            int gi = a0 + la, gj = b0 + lb;
+            if (gi == 0 && P.per[0]) gi = P.nx;
+            if (gj == 0 && P.per[1]) gj = P.ny;
+            gi = clampi(gi, 1, P1);
+            gj = clampi(gj, 1, P2);
+            offd[s_] = (unsigned int)((gi - 1) + P1 * (gj - 1));
+            ldsd[s_] = ok ? (short)e : (short)-1;
         }
     }
+    const size_t ps[3] = {(size_t)nn[0][0] * nn[0][1], (size_t)nn[1][0] * nn[1][1], (size_t)nn[2][0] * nn[2][1]};
     const double gauge = (LAZY && P.gauge) ? *P.gauge : 0.0;
-    const double dgrad[3] = {P.dx, P.dy, P.dz};
-    // own-axis step of the dp neighbour used by the lazy projection of component c
-    const size_t dstep[3] = {1, (size_t)P1, pp};
-    auto keff = [&](int c, int gk) {
-        if (c == 2 && gk == 0 && P.per[2]) gk = P.nz;
-        return gk < 1 ? 1 : (gk > nz3[c] ? nz3[c] : gk);
+    // z index of a velocity slice (0 -> nz for periodic w) and of a dp slice
+    auto kvel = [&](int c, int s) {
+        if (c == 2 && s == 0 && P.per[2]) s = P.nz;
+        return clampi(s, 1, nn[c][2]);
     };
-    struct Regs {
-        double v[3][NE], lo[3][NE], hi[3][NE];
+    auto kdp = [&](int s) {
+        if (s == 0 && P.per[2]) s = P.nz;  // dp(nz+1) == dp(1) after the periodic fill
+        return clampi(s, 1, P.nz + 2);
     };
-    auto fetch = [&](Regs &R, int c, int gk) {
-        const int k = keff(c, gk);
-        const double *src = P.src[c] + (size_t)(k - 1) * ps[c];
-        const double *dpk = LAZY ? P.dp + (size_t)(k - 1) * pp : nullptr;
+    struct VRegs {
+        double v[3][NE];
+    };
+    struct DRegs {
+        double d[NE];
+    };
+    // vel slices: u, v at su, w at sw
+    auto fetch_vel = [&](VRegs &R, int su, int sw) {
 #pragma unroll
-        for (int s_ = 0; s_ < NE; s_++) {
-            if (!valid[s_]) continue;
-            R.v[c][s_] = src[offc[c][s_]];
-            if (LAZY) {
-                R.lo[c][s_] = dpk[offp[c][s_]];
-                R.hi[c][s_] = dpk[offp[c][s_] + dstep[c]];
-            }
+        for (int c = 0; c < 3; c++) {
+            const double *src = P.src[c] + (size_t)(kvel(c, c == 2 ? sw : su) - 1) * ps[c];
+#pragma unroll
+            for (int s_ = 0; s_ < NE; s_++)
+                if (lds[c][s_] >= 0) R.v[c][s_] = src[off[c][s_]];
         }
     };
-    auto store = [&](const Regs &R, int c, int gk) {
-        double(*dst)[HX] = S[c][gk % 3];
+    auto fetch_dp = [&](DRegs &D, int s) {
+        const double *src = P.dp + (size_t)(kdp(s) - 1) * pp;
 #pragma unroll
-        for (int s_ = 0; s_ < NE; s_++) {
-            if (!valid[s_]) continue;
-            double x = R.v[c][s_];
-            if (LAZY) {
-                // the projection of big_periodic_3D.f90:562-567, gauge applied first
-                double hi = R.hi[c][s_], lo = R.lo[c][s_];
-                if (P.gauge) {
-                    hi = hi - gauge;
-                    lo = lo - gauge;
+        for (int s_ = 0; s_ < NE; s_++)
+            if (ldsd[s_] >= 0) D.d[s_] = src[offd[s_]];
+    };
+    auto store_dp = [&](const DRegs &D, int s) {
+        double *dst = &T.d[s % NDS][0][0];
+#pragma unroll
+        for (int s_ = 0; s_ < NE; s_++)
+            if (ldsd[s_] >= 0) dst[ldsd[s_]] = P.gauge ? D.d[s_] - gauge : D.d[s_];
+    };
+    // projection u = u* - dt*(dp(+1) - dp)/d (big_periodic_3D.f90:562-567), with the
+    // dp slices (gauge already applied) taken from the LDS ring
+    auto store_vel = [&](const VRegs &R, int su, int sw) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int s = (c == 2) ? sw : su;
+            double *dst = (c == 0) ? &T.u[s % NVS][0][0] : (c == 1) ? &T.v[s % NVS][0][0] : &T.w[s % NVS][0][0];
+            const double d = (c == 0) ? P.dx : (c == 1) ? P.dy : P.dz;
+#pragma unroll
+            for (int s_ = 0; s_ < NE; s_++) {
+                if (lds[c][s_] < 0) continue;
+                double x = R.v[c][s_];
+                if (LAZY) {
+                    const double *dl = &T.d[s % NDS][0][0];
+                    const double *dh = (c == 2) ? &T.d[(s + 1) % NDS][0][0] : dl;
+                    const double gr = dh[phi[c][s_]] - dl[plo[c][s_]];
+                    x = x - qdiv(P.dt * gr, d, P.rd[c]);
                 }
-                double gr = hi - lo;
-                x = x - qdiv(P.dt * gr, dgrad[c], P.rd[c]);
+                dst[lds[c][s_]] = x;
             }
-            dst[lxy[s_][1]][lxy[s_][0]] = x;
         }
     };
-    Regs R;
-    // prologue: u, v need slices K-1..K+1 with K = az+2; w needs fk-1..fk+1 with fk = az+1
-    for (int q = 0; q < 3; q++) {
-        fetch(R, 0, z0 + 1 + q); fetch(R, 1, z0 + 1 + q); fetch(R, 2, z0 + q);
-        store(R, 0, z0 + 1 + q); store(R, 1, z0 + 1 + q); store(R, 2, z0 + q);
+
+    // output cell of this thread and its first-slice offsets
+    const int a = a0 + tx, bb = b0 + ty;
+    const bool act = a < P.nx && bb < P.ny;
+    const bool needH = P.scheme != NS_EULER && !P.first;
+    size_t q0[3];
+    bool on[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int own = (c == 0) ? a : bb;
+        on[c] = act && (c == 2 || P.per[c] || own != 0);
+        q0[c] = FIX(gidx(c, 0, a), gidx(c, 1, bb), gidx(c, 2, 0), nn[c][0], nn[c][1]);
     }
-    __syncthreads();
+    auto fetch_h = [&](double hr[3], int az) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const bool own_ok = (c != 2) || P.per[2] || az != 0;
+            if (needH && on[c] && own_ok) hr[c] = P.h[c][q0[c] + (size_t)az * ps[c]];
+            else hr[c] = 0.0;
+        }
+    };
+
+    // prologue: dp slices z0..z0+2, then u, v slices z0+1, z0+2 and w slices z0, z0+1
+    VRegs R;
+    DRegs D;
+    if (LAZY) {
+        for (int s = z0; s < z0 + 3; s++) {
+            fetch_dp(D, s);
+            store_dp(D, s);
+        }
+        __syncthreads();
+    }
+    for (int q = 0; q < 2; q++) {
+        fetch_vel(R, z0 + 1 + q, z0 + q);
+        store_vel(R, z0 + 1 + q, z0 + q);
+    }
+    if (LAZY) {
+        fetch_dp(D, z0 + 3);
+        __syncthreads();  // dp slot (z0+3) % 3 was read above
+        store_dp(D, z0 + 3);
+        fetch_dp(D, z0 + 4);
+    }
+    fetch_vel(R, z0 + 3, z0 + 2);
+    double hn[3];
+    fetch_h(hn, z0);
+    if (LAZY) __syncthreads();  // dp slice z0+3 visible to the first store phase
+
     const double dx = P.dx, dy = P.dy, dz = P.dz, dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
     const double rx = P.rd[0], ry = P.rd[1], rz = P.rd[2], rx2 = P.rd2[0], ry2 = P.rd2[1], rz2 = P.rd2[2];
     const double dt = P.dt, nu = P.nu, dpf = (double)P.dp_flag;
-    const int a = a0 + tx, bb = b0 + ty;
     for (int az = z0; az < z1; az++) {
         const bool more = az + 1 < z1;
-        // prefetch the slices of the next iteration; their latency overlaps the compute below
+        // store phase: this step's newest slices (u, v at az+3, w at az+2; dp at az+4)
+        if (LAZY) store_dp(D, az + 4);
+        store_vel(R, az + 3, az + 2);
+        double hc[3] = {hn[0], hn[1], hn[2]};
+        // prefetch for the next step; the loads overlap the barrier and the compute below
         if (more) {
-            fetch(R, 0, az + 4);
-            fetch(R, 1, az + 4);
-            fetch(R, 2, az + 3);
+            fetch_vel(R, az + 4, az + 3);
+            if (LAZY) fetch_dp(D, az + 5);
+            fetch_h(hn, az + 1);
         }
-        if (a < P.nx && bb < P.ny) {
-#define LU(gi, gj, gk) S[0][(gk) % 3][(gj) - oy[0]][(gi) - ox[0]]
-#define LV(gi, gj, gk) S[1][(gk) % 3][(gj) - oy[1]][(gi) - ox[1]]
-#define LW(gi, gj, gk) S[2][(gk) % 3][(gj) - oy[2]][(gi) - ox[2]]
+        __syncthreads();
+        if (act) {
+#define LU(gi, gj, gk) T.u[(gk) % NVS][(gj) - oy[0]][(gi) - ox[0]]
+#define LV(gi, gj, gk) T.v[(gk) % NVS][(gj) - oy[1]][(gi) - ox[1]]
+#define LW(gi, gj, gk) T.w[(gk) % NVS][(gj) - oy[2]][(gi) - ox[2]]
 #define EF(i, j, k) (((LU(i, (j) + 1, k) + LU(i, j, k)) / 2.0) * ((LV((i) + 1, j, k) + LV(i, j, k)) / 2.0))
 #define FF(i, j, k) (((LU(i, j, (k) + 1) + LU(i, j, k)) / 2.0) * ((LW((i) + 1, j, k) + LW(i, j, k)) / 2.0))
 #define GF(i, j, k) (((LV(i, j, (k) + 1) + LV(i, j, k)) / 2.0) * ((LW(i, (j) + 1, k) + LW(i, j, k)) / 2.0))
+#pragma unroll
             for (int c = 0; c < 3; c++) {
                 // own-axis in-range: periodic 0..n-1, walls 1..n-1 (faces 2..n)
-                const int own = (c == 0) ? a : (c == 1) ? bb : az;
-                if (!P.per[c] && own == 0) continue;
+                if (!on[c] || (c == 2 && !P.per[2] && az == 0)) continue;
                 const int gi = gidx(c, 0, a), gj = gidx(c, 1, bb), gk = gidx(c, 2, az);
                 double cv, df, c0;
                 if (c == 0) {
@@ -233,9 +346,8 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                     const int fl = fi - 1;  // tile local index 0 holds the periodic wrap when fi == 1
                     double ur = (LU(fi + 1, J, K) + c0) / 2.0;
                     // avg(bc=1) wraps with u(nx+1) and u(nx) (FD_functions.f90:252)
-                    double ul = (P.per[0] && fi == 1)
-                                    ? (load_val<LAZY>(P, 0, P.nx + 1, J, K) + LU(fl, J, K)) / 2.0
-                                    : (c0 + LU(fl, J, K)) / 2.0;
+                    double ul = (P.per[0] && fi == 1) ? (T.u[K % NVS][J - oy[0]][HX] + LU(fl, J, K)) / 2.0
+                                                      : (c0 + LU(fl, J, K)) / 2.0;
                     double t1 = ur * ur - ul * ul;
                     double t2 = EF(fi, J, K) - EF(fi, J - 1, K);
                     double t3 = FF(fi, J, K) - FF(fi, J, K - 1);
@@ -256,9 +368,8 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                     const int fl = fj - 1;
                     double t1 = EF(I, fj, K) - EF(I - 1, fj, K);
                     double vr = (LV(I, fj + 1, K) + c0) / 2.0;
-                    double vl = (P.per[1] && fj == 1)
-                                    ? (load_val<LAZY>(P, 1, I, P.ny + 1, K) + LV(I, fl, K)) / 2.0
-                                    : (c0 + LV(I, fl, K)) / 2.0;
+                    double vl = (P.per[1] && fj == 1) ? (T.v[K % NVS][HY][I - ox[1]] + LV(I, fl, K)) / 2.0
+                                                      : (c0 + LV(I, fl, K)) / 2.0;
                     double t2 = vr * vr - vl * vl;
                     double t3 = GF(I, fj, K) - GF(I, fj, K - 1);
                     double r = qdiv(t1, dx, rx);
@@ -279,9 +390,8 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                     double t1 = FF(I, J, fk) - FF(I - 1, J, fk);
                     double t2 = GF(I, J, fk) - GF(I, J - 1, fk);
                     double wr = (LW(I, J, fk + 1) + c0) / 2.0;
-                    double wl = (P.per[2] && fk == 1)
-                                    ? (load_val<LAZY>(P, 2, I, J, P.nz + 1) + LW(I, J, fl)) / 2.0
-                                    : (c0 + LW(I, J, fl)) / 2.0;
+                    double wl = (P.per[2] && fk == 1) ? (load_val<LAZY>(P, 2, I, J, P.nz + 1) + LW(I, J, fl)) / 2.0
+                                                      : (c0 + LW(I, J, fl)) / 2.0;
                     double t3 = wr * wr - wl * wl;
                     double r = qdiv(t1, dx, rx);
                     r = r + qdiv(t2, dy, ry);
@@ -295,16 +405,13 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                     s = s + qdiv(d3, dz2, rz2);
                     df = s;
                 }
-                int n[3];
-                cdims(P, c, n);
-                const size_t q = FIX(gi, gj, gk, n[0], n[1]);
+                const size_t q = q0[c] + (size_t)az * ps[c];
                 double dpd = 0.0;
-                if (P.dp_flag) {
+                if (!LAZY && P.dp_flag) {
                     int bq[3] = {gi, gj, gk};
                     bq[c]++;
                     const double dd = (c == 0) ? dx : (c == 1) ? dy : dz;
-                    dpd = qdiv(P.p[FIX(bq[0], bq[1], bq[2], P.nx + 2, P.ny + 2)] - P.p[FIX(gi, gj, gk, P.nx + 2, P.ny + 2)],
-                               dd, P.rd[c]);
+                    dpd = qdiv(P.p[FIX(bq[0], bq[1], bq[2], P1, P2)] - P.p[FIX(gi, gj, gk, P1, P2)], dd, P.rd[c]);
                 }
                 double *H = P.h[c];
                 double rhs, val;
@@ -315,7 +422,7 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                 } else if (P.scheme == NS_AB2) {
                     rhs = nu * df - cv;
                     if (P.first) val = dt * (1.0 * rhs - dpf * dpd) + c0;
-                    else val = dt * (1.5 * rhs - 0.5 * H[q] - dpf * dpd) + c0;
+                    else val = dt * (1.5 * rhs - 0.5 * hc[c] - dpf * dpd) + c0;
                     H[q] = rhs;
                 } else {
                     if (P.first) {
@@ -323,7 +430,7 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
                         val = dt * (rhs - dpf * dpd) + c0;
                     } else {
                         rhs = 0.5 * nu * df - 1.5 * cv;
-                        val = dt * (rhs - 0.5 * H[q] - dpf * dpd) + c0;
+                        val = dt * (rhs - 0.5 * hc[c] - dpf * dpd) + c0;
                     }
                     H[q] = -cv;
                 }
@@ -336,13 +443,6 @@ __global__ void __launch_bounds__(TX *TY) k_predict_tiled(TileParams P) {
 #undef LV
 #undef LW
         }
-        __syncthreads();
-        if (more) {
-            store(R, 0, az + 4);
-            store(R, 1, az + 4);
-            store(R, 2, az + 3);
-        }
-        __syncthreads();
     }
 }
 
