@@ -203,3 +203,46 @@ def test_step_walls(oracle, scheme, bc, pbc):
     out, st, div = run_both(oracle, cfg, u, v, w, p, 4, planes=planes)
     for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
         assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+
+
+def cfg_periodic(nx, ny, nz, **kw):
+    return NSConfig(nx=nx, ny=ny, nz=nz, dx=2 * np.pi / nx, dy=0.5, dz=0.35, nu=2e-3, dt=2e-3, bc=(1, 1, 1),
+                    pbc=(1, 1, 1), flags=FLAG_GAUGE_CORNER, **kw)
+
+
+@pytest.mark.parametrize("scheme,dp_flag", [(AB2, 0), (AB2_CN, 0), (AB2, 1), (EULER, 0)])
+def test_step_periodic_multi_tile(oracle, scheme, dp_flag):
+    """periodic box spanning several predictor tiles (64 x 4 x 16 cells) with
+    partial tiles on every axis: the periodic wraps of u(nx+1), v(ny+1) and
+    w(nz+1) and of the lazily applied pressure gradient cross tile edges"""
+    cfg = cfg_periodic(72, 10, 20, scheme=scheme, dp_flag=dp_flag)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=7)
+    out, st, div = run_both(oracle, cfg, u, v, w, p, 6)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p), ("hx", st.hx), ("hz", st.hz)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+    assert div <= 1e-12
+
+
+def test_step_periodic_unfilled_ghosts(oracle):
+    """periodic run from arrays whose ghost faces are not periodic copies
+    (u(nx+1) != u(1) etc.): the first step's x/y/z averages read those ghost
+    values exactly as avg(bc=1) does (FD_functions.f90:246-301)"""
+    cfg = cfg_periodic(70, 9, 18, scheme=AB2, dp_flag=0)
+    u, v, w, p = rand_fields(cfg, 83)
+    out, st, div = run_both(oracle, cfg, 0.3 * u, 0.3 * v, 0.3 * w, p, 3)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+    assert div <= 1e-12
+
+
+@pytest.mark.parametrize("scheme", [AB2, AB2_CN])
+def test_step_walls_lazy_multi_tile(oracle, scheme):
+    """walls with dp_flag=0 (projection folded into the next predictor) on a
+    box spanning several predictor tiles"""
+    cfg = cfg_box(70, 9, 18, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=scheme, dp_flag=0).replace(dt=1e-3)
+    u, v, w, p = rand_fields(cfg, 84)
+    u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
+    planes = rand_planes(cfg, 85, 0.2)
+    out, st, div = run_both(oracle, cfg, u, v, w, p, 4, planes=planes)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
