@@ -70,52 +70,49 @@ template <> struct SmallDFT<8> {
         }
     }
 };
-// odd radices: direct sums with cos/sin(2*pi*m/R)
+// odd radices: X[k] = v0 + sum_r a_r cos(2 pi r k/R) + s i sum_r b_r sin(2 pi r k/R)
+// over the symmetric pairs a_r = v_r + v_{R-r}, b_r = v_r - v_{R-r}, r = 1..(R-1)/2
 template <int R> struct OddTab;
 template <> struct OddTab<3> {
-    __device__ static double c(int m) { const double t[3] = {1.0, -0.5, -0.5}; return t[m]; }
-    __device__ static double s(int m) { const double t[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676}; return t[m]; }
+    static constexpr double c[2] = {1.0, -0.5};
+    static constexpr double s[2] = {0.0, 0.86602540378443864676};
 };
 template <> struct OddTab<5> {
-    __device__ static double c(int m) {
-        const double t[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
-                             0.30901699437494742410};
-        return t[m];
-    }
-    __device__ static double s(int m) {
-        const double t[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
-                             -0.95105651629515357212};
-        return t[m];
-    }
+    static constexpr double c[3] = {1.0, 0.30901699437494742410, -0.80901699437494742410};
+    static constexpr double s[3] = {0.0, 0.95105651629515357212, 0.58778525229247312917};
 };
 template <> struct OddTab<7> {
-    __device__ static double c(int m) {
-        const double t[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429, -0.90096886790241912624,
-                             -0.90096886790241912624, -0.22252093395631440429, 0.62348980185873353053};
-        return t[m];
-    }
-    __device__ static double s(int m) {
-        const double t[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702, 0.43388373911755812048,
-                             -0.43388373911755812048, -0.97492791218182360702, -0.78183148246802980871};
-        return t[m];
-    }
+    static constexpr double c[4] = {1.0, 0.62348980185873353053, -0.22252093395631440429, -0.90096886790241912624};
+    static constexpr double s[4] = {0.0, 0.78183148246802980871, 0.97492791218182360702, 0.43388373911755812048};
 };
 template <int R> struct SmallDFTOdd {
-    __device__ static void run(double2 *v, int s) {
-        double2 out[R];
+    __device__ __forceinline__ static void run(double2 *v, int sg) {
+        constexpr int H = (R - 1) / 2;
+        double2 a[H + 1], b[H + 1];
+        double2 x0 = v[0];
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            double2 acc = v[0];
-#pragma unroll
-            for (int r = 1; r < R; r++) {
-                int m = (r * k) % R;
-                double2 w = make_double2(OddTab<R>::c(m), s * OddTab<R>::s(m));
-                acc = cadd(acc, cmul(v[r], w));
-            }
-            out[k] = acc;
+        for (int r = 1; r <= H; r++) {
+            a[r] = cadd(v[r], v[R - r]);
+            b[r] = csub(v[r], v[R - r]);
+            x0 = cadd(x0, a[r]);
         }
 #pragma unroll
-        for (int k = 0; k < R; k++) v[k] = out[k];
+        for (int k = 1; k <= H; k++) {
+            double2 cr = v[0], si = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int r = 1; r <= H; r++) {
+                const int m = (r * k) % R;  // cos(2 pi m/R) and sin(2 pi m/R) from the half table
+                const double cm = OddTab<R>::c[m <= H ? m : R - m];
+                const double sm = (m <= H) ? OddTab<R>::s[m] : -OddTab<R>::s[R - m];
+                cr = make_double2(cr.x + a[r].x * cm, cr.y + a[r].y * cm);
+                si = make_double2(si.x + b[r].x * sm, si.y + b[r].y * sm);
+            }
+            // sign*i*si
+            const double2 t = cmuli(si, sg);
+            v[k] = cadd(cr, t);
+            v[R - k] = csub(cr, t);
+        }
+        v[0] = x0;
     }
 };
 template <> struct SmallDFT<3> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<3>::run(v, s); } };
@@ -258,31 +255,167 @@ struct PassParams {
 };
 
 // ---------------- periodic passes ----------------
+//
+// In-place Stockham: each stage reads its butterflies into registers, waits
+// at a barrier, then writes them back, so a block needs one line buffer
+// (L*N <= FFT_ELEMS complex) instead of two, and twice as many blocks fit on
+// a CU.  L is a power of two and threads walk the lines fastest; line-major
+// buffers (x passes) use a row stride of 2 (mod 16) complex so that the L
+// lines of one butterfly index fall in distinct LDS banks.  Twiddles are
+// staged in LDS.
 
-__global__ void __launch_bounds__(256) k_fft_x_fwd(PassParams P) {
-    extern __shared__ double2 smem[];
-    const Grid g = P.g;
-    const int N = g.nx, L = P.L, LS = P.LS;
-    double2 *A = smem, *B = smem + (size_t)L * LS;
-    const long RL = (long)g.ny * g.nz;
-    const long rl0 = (long)blockIdx.x * 2 * L;
-    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
-        int rline = f / N, x = f - rline * N;
-        long rl = rl0 + rline;
-        double val = 0.0;
-        if (rl < RL) val = div_at(g, P.src, x, (int)(rl % g.ny), (int)(rl / g.ny));
-        double *slot = (double *)&A[(rline >> 1) * LS + x];
-        slot[rline & 1] = val;
+constexpr int FFT_NT = 256;      // threads per block
+constexpr int FFT_ELEMS = 2048;  // complex elements per block (L*N)
+
+// x / d for 0 <= x < 2^24 with r = 1/d in single precision (one correction)
+struct IDiv {
+    int d;
+    float r;
+};
+__device__ __forceinline__ int idiv(int x, IDiv D) {
+    int q = (int)((float)x * D.r);
+    const int rem = x - q * D.d;
+    if (rem < 0) q--;
+    else if (rem >= D.d) q++;
+    return q;
+}
+
+template <int R>
+__device__ __forceinline__ void stage_ip(double2 *A, int N, Lay ly, int lsh, int Ns, int tstep,
+                                         const double2 *__restrict__ tw, int sign) {
+    constexpr int MB = (FFT_ELEMS / R + FFT_NT - 1) / FFT_NT;
+    const int NR = N / R;
+    const int nb = NR << lsh;
+    const int lm = (1 << lsh) - 1;
+    const bool p2 = (Ns & (Ns - 1)) == 0;
+    double2 v[MB][R];
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+        const int t = threadIdx.x + m * FFT_NT;
+        if (t < nb) {
+            const int line = t & lm, j = t >> lsh;
+            const double2 *in = A + line * ly.SL;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[m][r] = in[(j + r * NR) * ly.SP];
+            const int jm = p2 ? (j & (Ns - 1)) : (j % Ns);
+            if (jm) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2 w = tw[jm * r * tstep];
+                    if (sign > 0) w.y = -w.y;
+                    v[m][r] = cmul(v[m][r], w);
+                }
+            }
+            SmallDFT<R>::run(v[m], sign);
+        }
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, Lay{LS, 1}, L, -1);
-    const int nxh = P.nxh;
-    for (int f = threadIdx.x; f < 2 * L * nxh; f += blockDim.x) {
-        int rline = f / nxh, kx = f - rline * nxh;
-        long rl = rl0 + rline;
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+        const int t = threadIdx.x + m * FFT_NT;
+        if (t < nb) {
+            const int line = t & lm, j = t >> lsh;
+            const int jm = p2 ? (j & (Ns - 1)) : (j % Ns);
+            const int idxD = (j - jm) * R + jm;
+            double2 *out = A + line * ly.SL;
+#pragma unroll
+            for (int r = 0; r < R; r++) out[(idxD + r * Ns) * ly.SP] = v[m][r];
+        }
+    }
+    __syncthreads();
+}
+
+// full in-place transform of the 2^lsh lines in A (callers synchronise before).
+// ODD: the plan may contain radices 3, 5, 7; power-of-two lengths use the
+// variant without them, whose register footprint is a third.
+template <bool ODD>
+__device__ void fft_ip(double2 *A, const FFTPlan &pl, Lay ly, int lsh, const double2 *tw, int sign) {
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; s++) {
+        const int R = pl.radix[s];
+        const int tstep = pl.n / (Ns * R);
+        switch (R) {
+            case 8: stage_ip<8>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 4: stage_ip<4>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 2: stage_ip<2>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 3: if (ODD) stage_ip<3>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 5: if (ODD) stage_ip<5>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 7: if (ODD) stage_ip<7>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+        }
+        Ns *= R;
+    }
+}
+
+struct FFTPass {
+    Grid g;
+    FFTPlan pl;   // plan of this pass's axis
+    int lsh;      // log2 of the complex lines per block
+    int LS;       // LDS row stride (complex) of line-major buffers
+    int nxh;      // nx/2+1
+    int nxhp;     // spectrum row stride: nxh rounded up to a multiple of 8
+    IDiv dN, dNxh, dNy;
+    // divergence source: rhs (interior array) or the staggered u*, v*, w*
+    const double *rhs;
+    const double *us, *vs, *ws;
+    double dx, dy, dz, dt, rdx, rdy, rdz, rdt;  // spacings and their reciprocals
+    double2 *spec;                // half spectrum [z][y][kx]
+    const double *ex, *ey, *ez;   // eigenvalue parts
+    double scale;                 // backward scale 1/(nx*ny*nz)
+    double *dp_out;               // ghosted dp (interior written) or null
+    double *sol_out;              // interior (nx,ny,nz) or null
+};
+
+// a/d with r = RN(1/d): one product and one FMA correction step (the
+// correctly rounded quotient except in rare last-bit cases)
+__device__ __forceinline__ double qdiv(double a, double d, double r) {
+    const double q = a * r;
+    const double e = __builtin_fma(-q, d, a);
+    return __builtin_fma(e, r, q);
+}
+
+// divergence RHS at interior cell (a,b,c), 0-based (big_periodic_3D.f90:498-501)
+__device__ __forceinline__ double div_rhs(const FFTPass &P, int a, int b, int c) {
+    const Grid &g = P.g;
+    if (P.rhs) return P.rhs[(size_t)a + (size_t)g.nx * ((size_t)b + (size_t)g.ny * c)];
+    const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+    const int A = a + 1, B = b + 1, C = c + 1;
+    double s = qdiv(P.us[FIX(A + 1, B + 1, C + 1, U1, U2)] - P.us[FIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
+    s = s + qdiv(P.vs[FIX(A + 1, B + 1, C + 1, V1, V2)] - P.vs[FIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
+    s = s + qdiv(P.ws[FIX(A + 1, B + 1, C + 1, W1, W2)] - P.ws[FIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
+    return qdiv(s, P.dt, P.rdt);
+}
+
+template <bool ODD>
+__global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd(FFTPass P) {
+    extern __shared__ double2 smem[];
+    const Grid g = P.g;
+    const int N = g.nx, L = 1 << P.lsh, LS = P.LS, nxh = P.nxh;
+    double2 *A = smem, *tws = smem + (size_t)L * LS;
+    for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
+    const int RL = g.ny * g.nz;
+    const int rl0 = blockIdx.x * 2 * L;
+    const int cnt = 2 * L * N;
+#pragma unroll 4
+    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+        const int rline = idiv(f, P.dN), x = f - rline * N;
+        const int rl = rl0 + rline;
+        double val = 0.0;
+        if (rl < RL) {
+            const int k = idiv(rl, P.dNy);
+            val = div_rhs(P, x, rl - k * g.ny, k);
+        }
+        ((double *)&A[(rline >> 1) * LS + x])[rline & 1] = val;
+    }
+    __syncthreads();
+    fft_ip<ODD>(A, P.pl, Lay{LS, 1}, P.lsh, tws, -1);
+    // split the paired transform into the two half spectra
+    const int cnto = 2 * L * nxh;
+    for (int f = threadIdx.x; f < cnto; f += FFT_NT) {
+        const int rline = idiv(f, P.dNxh), kx = f - rline * nxh;
+        const int rl = rl0 + rline;
         if (rl >= RL) continue;
-        const double2 *z = Z + (rline >> 1) * LS;
-        double2 a = z[kx], b = z[(N - kx) % N];
+        const double2 *z = A + (rline >> 1) * LS;
+        const double2 a = z[kx], b = z[kx == 0 ? 0 : N - kx];
         double2 X;
         if ((rline & 1) == 0) X = make_double2((a.x + b.x) * 0.5, (a.y - b.y) * 0.5);
         else X = make_double2((a.y + b.y) * 0.5, (b.x - a.x) * 0.5);
@@ -291,8 +424,9 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(PassParams P) {
 }
 
 // strided complex lines: along y (stride nxhp) for fixed (kx,z), or along z
-// (stride nxhp*ny) for fixed (kx,y); L consecutive kx per block.  Spectrum
-// rows are padded to nxhp (a multiple of 8 complex = 128 B), so a group of 8
+// (stride nxhp*ny) for fixed (kx,y); L consecutive kx per block, held
+// line-interleaved in LDS (element (l,pos) at pos*L + l).  Spectrum rows are
+// padded to nxhp (a multiple of 8 complex = 128 B), so a group of 8
 // wavenumbers is one aligned cache line.
 template <int AXIS>
 __device__ __forceinline__ size_t spec_index(const Grid &g, int nxhp, int kx, int pos, int other) {
@@ -300,30 +434,33 @@ __device__ __forceinline__ size_t spec_index(const Grid &g, int nxhp, int kx, in
     return ((size_t)pos * g.ny + other) * nxhp + kx;
 }
 
-template <int AXIS, int SIGN, int SOLVE>
-__global__ void __launch_bounds__(256) k_fft_yz(PassParams P) {
+template <int AXIS, int SIGN, int SOLVE, bool ODD>
+__global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
-    const int N = (AXIS == 1) ? g.ny : g.nz, L = P.L, nxh = P.nxh, nxhp = P.nxhp;
-    double2 *A = smem, *B = smem + (size_t)L * N;
-    const Lay ly{1, L};  // line-interleaved: element (l, pos) at pos*L + l
-    const int ngrp = (nxh + L - 1) / L;
-    const int kx0 = (blockIdx.x % ngrp) * L;
+    const int N = (AXIS == 1) ? g.ny : g.nz, lsh = P.lsh, L = 1 << lsh, nxh = P.nxh, nxhp = P.nxhp;
+    double2 *A = smem, *tws = smem + (size_t)L * N;
+    for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
+    const int ngrp = (nxh + L - 1) >> lsh;
+    const int kx0 = (blockIdx.x % ngrp) << lsh;
     const int other = blockIdx.x / ngrp;
-    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-        int l = f % L, pos = f / L;
-        int kx = kx0 + l;
+    const int cnt = L * N;
+#pragma unroll 4
+    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+        const int l = f & (L - 1), pos = f >> lsh;
+        const int kx = kx0 + l;
         A[f] = (kx < nxh) ? P.spec[spec_index<AXIS>(g, nxhp, kx, pos, other)] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, ly, L, SIGN);
+    const Lay ly{1, L};
+    fft_ip<ODD>(A, P.pl, ly, lsh, tws, SIGN);
     if (SOLVE) {
         // modal division (z pass): eig = -4*((ex + ey) + ez); mode (0,0,0) -> 0
-        for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-            int l = f % L, kz = f / L;
-            int kx = kx0 + l;
+        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+            const int l = f & (L - 1), kz = f >> lsh;
+            const int kx = kx0 + l;
             if (kx >= nxh) continue;
-            double2 *z = Z + f;
+            double2 *z = A + f;
             double e = P.ex[kx];
             e = e + P.ey[other];
             e = e + P.ez[kz];
@@ -332,57 +469,56 @@ __global__ void __launch_bounds__(256) k_fft_yz(PassParams P) {
             else *z = make_double2(z->x / e, z->y / e);
         }
         __syncthreads();
-        double2 *other_buf = (Z == A) ? B : A;
-        Z = fft_lds(Z, other_buf, P.pl, ly, L, +1);
+        fft_ip<ODD>(A, P.pl, ly, lsh, tws, +1);
     }
-    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-        int l = f % L, pos = f / L;
-        int kx = kx0 + l;
-        if (kx < nxh) P.spec[spec_index<AXIS>(g, nxhp, kx, pos, other)] = Z[f];
+    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+        const int l = f & (L - 1), pos = f >> lsh;
+        const int kx = kx0 + l;
+        if (kx < nxh) P.spec[spec_index<AXIS>(g, nxhp, kx, pos, other)] = A[f];
     }
 }
 
-__global__ void __launch_bounds__(256) k_fft_x_inv(PassParams P) {
+template <bool ODD>
+__global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
-    const int N = g.nx, L = P.L, LS = P.LS, nxh = P.nxh;
-    double2 *A = smem, *B = smem + (size_t)L * LS;
-    const long RL = (long)g.ny * g.nz;
-    const long rl0 = (long)blockIdx.x * 2 * L;
-    // stage the half spectra of the 2L real lines in B (line r at B + r*nxh)
-    for (int f = threadIdx.x; f < 2 * L * nxh; f += blockDim.x) {
-        int rline = f / nxh, kx = f - rline * nxh;
-        long rl = rl0 + rline;
-        double2 X = (rl < RL) ? P.spec[(size_t)rl * P.nxhp + kx] : make_double2(0.0, 0.0);
-        if (kx == 0 || 2 * kx == N) X.y = 0.0;  // c2r ignores the imaginary parts of DC and Nyquist
-        B[rline * nxh + kx] = X;
-    }
-    __syncthreads();
-    // Z[m] = Xa[m] + i Xb[m] with Hermitian extension
-    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-        int l = f / N, m = f - l * N;
-        double2 a, b;
-        if (m < nxh) {
-            a = B[(2 * l) * nxh + m];
-            b = B[(2 * l + 1) * nxh + m];
-        } else {
-            a = B[(2 * l) * nxh + (N - m)];
-            b = B[(2 * l + 1) * nxh + (N - m)];
+    const int N = g.nx, L = 1 << P.lsh, LS = P.LS, nxh = P.nxh;
+    double2 *A = smem, *tws = smem + (size_t)L * LS;
+    for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
+    const int RL = g.ny * g.nz;
+    const int rl0 = blockIdx.x * 2 * L;
+    // Z[m] = Xa[m] + i Xb[m] of the paired lines, Hermitian extension for m >= nxh;
+    // c2r ignores the imaginary parts of DC and Nyquist
+    const int cnt = L * N;
+#pragma unroll 4
+    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+        const int l = idiv(f, P.dN), m = f - l * N;
+        const int ra = rl0 + 2 * l;
+        const bool cj = m >= nxh;
+        const int km = cj ? N - m : m;
+        double2 a = (ra < RL) ? P.spec[(size_t)ra * P.nxhp + km] : make_double2(0.0, 0.0);
+        double2 b = (ra + 1 < RL) ? P.spec[(size_t)(ra + 1) * P.nxhp + km] : make_double2(0.0, 0.0);
+        if (km == 0 || 2 * km == N) {
+            a.y = 0.0;
+            b.y = 0.0;
+        }
+        if (cj) {
             a.y = -a.y;
             b.y = -b.y;
         }
         A[l * LS + m] = make_double2(a.x - b.y, a.y + b.x);
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, Lay{LS, 1}, L, +1);
+    fft_ip<ODD>(A, P.pl, Lay{LS, 1}, P.lsh, tws, +1);
     const double sc = P.scale;
-    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
-        int rline = f / N, x = f - rline * N;
-        long rl = rl0 + rline;
+    const int cnto = 2 * L * N;
+    for (int f = threadIdx.x; f < cnto; f += FFT_NT) {
+        const int rline = idiv(f, P.dN), x = f - rline * N;
+        const int rl = rl0 + rline;
         if (rl >= RL) continue;
-        double2 z = Z[(rline >> 1) * LS + x];
-        double val = ((rline & 1) ? z.y : z.x) * sc;
-        int j = (int)(rl % g.ny), k = (int)(rl / g.ny);
+        const double2 z = A[(rline >> 1) * LS + x];
+        const double val = ((rline & 1) ? z.y : z.x) * sc;
+        const int k = idiv(rl, P.dNy), j = rl - k * g.ny;
         if (P.dp_out) P.dp_out[FIX(x + 2, j + 2, k + 2, g.nx + 2, g.ny + 2)] = val;
         if (P.sol_out) P.sol_out[(size_t)x + (size_t)g.nx * rl] = val;
     }
@@ -616,72 +752,84 @@ int spectrum_row(int nx) { return ((nx / 2 + 1) + 7) / 8 * 8; }
 
 namespace {
 
+// launch with dynamic LDS, raising the kernel's limit above 64 KiB when a
+// long line needs it
+template <typename K>
+void launch_lds(K kernel, unsigned nb, size_t lds, hipStream_t st, const FFTPass &P) {
+    if (lds > 65536) (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(FFT_NT), lds, st, P);
+}
+
+// log2 of the lines per block: the largest power of two L with L*n <= FFT_ELEMS
+int fft_lsh(int n) {
+    int s = 0;
+    while ((2 << s) * n <= FFT_ELEMS) s++;
+    return s;
+}
+// line-major row stride: n rounded up to 2 (mod 16) complex
+int fft_row(int n) { return n + ((2 - n) % 16 + 16) % 16; }
+
+IDiv make_idiv(int d) { return IDiv{d, 1.0f / (float)d}; }
+
 }  // namespace
 
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
                        double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
-    PassParams P{};
+    FFTPass P{};
     P.g = g;
     P.nxh = g.nx / 2 + 1;
     P.nxhp = spectrum_row(g.nx);
-    P.src.rhs = rhs_in;
-    P.src.us = us; P.src.vs = vs; P.src.ws = ws;
-    P.src.dx = dx; P.src.dy = dy; P.src.dz = dz; P.src.dt = dt;
-    P.src.planes = nullptr;
+    P.dN = make_idiv(g.nx);
+    P.dNxh = make_idiv(P.nxh);
+    P.dNy = make_idiv(g.ny);
+    P.rhs = rhs_in;
+    P.us = us; P.vs = vs; P.ws = ws;
+    P.dx = dx; P.dy = dy; P.dz = dz; P.dt = dt;
+    P.rdx = 1.0 / dx; P.rdy = 1.0 / dy; P.rdz = 1.0 / dz; P.rdt = 1.0 / dt;
     P.spec = spec;
     P.ex = pl.ex; P.ey = pl.ey; P.ez = pl.ez;
     P.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
     P.dp_out = dp_out;
     P.sol_out = sol_out;
     const long RL = (long)g.ny * g.nz;
-    // x forward
+    auto lds_x = [&]() { return ((size_t)(1 << P.lsh) * P.LS + g.nx) * sizeof(double2); };
+    auto lds_yz = [&](int n) { return ((size_t)(1 << P.lsh) * n + n) * sizeof(double2); };
+    auto nb_yz = [&](int others) { return (unsigned)((((P.nxh + (1 << P.lsh) - 1) >> P.lsh)) * others); };
+    // x forward (fused with the divergence)
     P.pl = pl.px;
-    P.LS = g.nx + 2;
-    P.L = lines_per_block(P.LS);
-    {
-        long CL = (RL + 1) / 2;
-        unsigned nb = (unsigned)((CL + P.L - 1) / P.L);
-        hipLaunchKernelGGL(k_fft_x_fwd, dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    P.lsh = fft_lsh(g.nx);
+    P.LS = fft_row(g.nx);
+    const unsigned nbx = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
+    auto pow2 = [](int n) { return (n & (n - 1)) == 0; };
+    if (pow2(g.nx)) launch_lds(k_fft_x_fwd<false>, nbx, lds_x(), st, P);
+    else launch_lds(k_fft_x_fwd<true>, nbx, lds_x(), st, P);
     if (ev) (void)hipEventRecord(ev[0], st);
-    }
     // y forward
     P.pl = pl.py;
-    P.LS = g.ny;
-    P.L = lines_per_block(P.LS);
-    {
-        unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);
-        hipLaunchKernelGGL((k_fft_yz<1, -1, 0>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    P.lsh = fft_lsh(g.ny);
+    if (pow2(g.ny)) launch_lds(k_fft_yz<1, -1, 0, false>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    else launch_lds(k_fft_yz<1, -1, 0, true>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[1], st);
-    }
-    // z forward + divide + inverse
+    // z forward + modal division + z inverse
     P.pl = pl.pz;
-    P.LS = g.nz;
-    P.L = lines_per_block(P.LS);
-    {
-        unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.ny);
-        hipLaunchKernelGGL((k_fft_yz<2, -1, 1>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    P.lsh = fft_lsh(g.nz);
+    if (pow2(g.nz)) launch_lds(k_fft_yz<2, -1, 1, false>, nb_yz(g.ny), lds_yz(g.nz), st, P);
+    else launch_lds(k_fft_yz<2, -1, 1, true>, nb_yz(g.ny), lds_yz(g.nz), st, P);
     if (ev) (void)hipEventRecord(ev[2], st);
-    }
     // y inverse
     P.pl = pl.py;
-    P.LS = g.ny;
-    P.L = lines_per_block(P.LS);
-    {
-        unsigned nb = (unsigned)(((P.nxh + P.L - 1) / P.L) * g.nz);
-        hipLaunchKernelGGL((k_fft_yz<1, 1, 0>), dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    P.lsh = fft_lsh(g.ny);
+    if (pow2(g.ny)) launch_lds(k_fft_yz<1, 1, 0, false>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    else launch_lds(k_fft_yz<1, 1, 0, true>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[3], st);
-    }
-    // x inverse (c2r)
+    // x inverse (c2r) writing dp
     P.pl = pl.px;
-    P.LS = g.nx + 2;
-    P.L = lines_per_block(P.LS);
-    {
-        long CL = (RL + 1) / 2;
-        unsigned nb = (unsigned)((CL + P.L - 1) / P.L);
-        hipLaunchKernelGGL(k_fft_x_inv, dim3(nb), dim3(256), lds_bytes(P.L, P.LS), st, P);
+    P.lsh = fft_lsh(g.nx);
+    P.LS = fft_row(g.nx);
+    if (pow2(g.nx)) launch_lds(k_fft_x_inv<false>, nbx, lds_x(), st, P);
+    else launch_lds(k_fft_x_inv<true>, nbx, lds_x(), st, P);
     if (ev) (void)hipEventRecord(ev[4], st);
-    }
 }
 
 void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
