@@ -7,10 +7,10 @@
 // through z ("2.5-D" blocking).  Each component keeps a ring of four z-slices
 // of its halo'd 66 x 6 tile in LDS, so every input value is fetched from
 // memory about 1.5 times (the x/y halo, mostly served by L2) instead of once
-// per stencil use.  Each z step stores the slices prefetched into registers
-// during the previous step, issues the loads of the next ones (and of the AB2
-// history of the next cell), passes one barrier and computes; four slots make
-// that single barrier sufficient.  Only the points of the reference's
+// per stencil use.  Each z step stores the slices loaded into registers two
+// steps earlier, issues the loads for two steps ahead (and of the AB2 history
+// of the next cell), passes one barrier and computes; four slots make that
+// single barrier sufficient.  Only the points of the reference's
 // assignment ranges are computed (x_range0 etc., big_periodic_3D.f90:126-146);
 // the ghost entries of u* are rewritten by the boundary fill or discarded by
 // the ADI sweeps in every scheme, so they are left untouched, and so are the
@@ -119,7 +119,7 @@ struct Tiles {
 };
 
 template <bool LAZY>
-__global__ void __launch_bounds__(NT) k_predict_tiled(TileParams P) {
+__global__ void __launch_bounds__(NT, 3) k_predict_tiled(TileParams P) {
     __shared__ Tiles<LAZY> T;
     // XCD-aware tile order: consecutive tiles of one XCD are x/y neighbours
     const int nb = P.ntiles;
@@ -287,26 +287,30 @@ __global__ void __launch_bounds__(NT) k_predict_tiled(TileParams P) {
     };
 
     // prologue: dp slices z0..z0+2, then u, v slices z0+1, z0+2 and w slices z0, z0+1
-    VRegs R;
-    DRegs D;
+    // Two register sets (A: even steps, B: odd steps) keep the loads of a
+    // step's slices in flight for two steps before they are stored.
+    VRegs RA, RB;
+    DRegs DA, DB;
     if (LAZY) {
         for (int s = z0; s < z0 + 3; s++) {
-            fetch_dp(D, s);
-            store_dp(D, s);
+            fetch_dp(DA, s);
+            store_dp(DA, s);
         }
         __syncthreads();
     }
     for (int q = 0; q < 2; q++) {
-        fetch_vel(R, z0 + 1 + q, z0 + q);
-        store_vel(R, z0 + 1 + q, z0 + q);
+        fetch_vel(RA, z0 + 1 + q, z0 + q);
+        store_vel(RA, z0 + 1 + q, z0 + q);
     }
     if (LAZY) {
-        fetch_dp(D, z0 + 3);
+        fetch_dp(DA, z0 + 3);
         __syncthreads();  // dp slot (z0+3) % 3 was read above
-        store_dp(D, z0 + 3);
-        fetch_dp(D, z0 + 4);
+        store_dp(DA, z0 + 3);
+        fetch_dp(DA, z0 + 4);
+        fetch_dp(DB, z0 + 5);
     }
-    fetch_vel(R, z0 + 3, z0 + 2);
+    fetch_vel(RA, z0 + 3, z0 + 2);
+    fetch_vel(RB, z0 + 4, z0 + 3);
     double hn[3];
     fetch_h(hn, z0);
     if (LAZY) __syncthreads();  // dp slice z0+3 visible to the first store phase
@@ -314,18 +318,18 @@ __global__ void __launch_bounds__(NT) k_predict_tiled(TileParams P) {
     const double dx = P.dx, dy = P.dy, dz = P.dz, dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
     const double rx = P.rd[0], ry = P.rd[1], rz = P.rd[2], rx2 = P.rd2[0], ry2 = P.rd2[1], rz2 = P.rd2[2];
     const double dt = P.dt, nu = P.nu, dpf = (double)P.dp_flag;
-    for (int az = z0; az < z1; az++) {
-        const bool more = az + 1 < z1;
+    auto zstep = [&](const int az, VRegs &R, DRegs &D) {
         // store phase: this step's newest slices (u, v at az+3, w at az+2; dp at az+4)
         if (LAZY) store_dp(D, az + 4);
         store_vel(R, az + 3, az + 2);
         double hc[3] = {hn[0], hn[1], hn[2]};
-        // prefetch for the next step; the loads overlap the barrier and the compute below
-        if (more) {
-            fetch_vel(R, az + 4, az + 3);
-            if (LAZY) fetch_dp(D, az + 5);
-            fetch_h(hn, az + 1);
+        // loads for two steps ahead (same register set) and the next AB2 history;
+        // they overlap the barrier and the compute below
+        if (az + 2 < z1) {
+            fetch_vel(R, az + 5, az + 4);
+            if (LAZY) fetch_dp(D, az + 6);
         }
+        if (az + 1 < z1) fetch_h(hn, az + 1);
         __syncthreads();
         if (act) {
 #define LU(gi, gj, gk) T.u[(gk) % NVS][(gj) - oy[0]][(gi) - ox[0]]
@@ -443,7 +447,50 @@ __global__ void __launch_bounds__(NT) k_predict_tiled(TileParams P) {
 #undef LV
 #undef LW
         }
+    };
+    for (int az = z0; az < z1; az += 2) {
+        zstep(az, RA, DA);
+        if (az + 1 < z1) zstep(az + 1, RB, DB);
     }
+}
+
+}  // namespace
+
+namespace {
+
+// blocks of the predictor resident on the whole device at once (cached per device)
+int resident_blocks(bool lazy) {
+    static int cache[2][64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int &slots = cache[lazy ? 1 : 0][dev & 63];
+    if (!slots) {
+        int per_cu = 0, cus = 0;
+        if (lazy) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<true>, NT, 0);
+        else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<false>, NT, 0);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        slots = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    }
+    return slots;
+}
+
+// z cells per block: the column split into tz chunks so that the tiles fill
+// whole rounds of resident blocks (a partial last round leaves most of the
+// device idle); cost = rounds x (chunk + the ~3 slices of the prologue)
+int choose_zc(int nz, long txy, int slots) {
+    int best_tz = 1;
+    long best = -1;
+    for (int tz = 1; tz <= nz; tz++) {
+        const int zc = (nz + tz - 1) / tz;
+        if (zc < 4 && tz > 1) break;
+        const long nt = txy * tz;
+        const long cost = ((nt + slots - 1) / slots) * (zc + 3);
+        if (best < 0 || cost < best) {
+            best = cost;
+            best_tz = tz;
+        }
+    }
+    return (nz + best_tz - 1) / best_tz;
 }
 
 }  // namespace
@@ -461,9 +508,9 @@ void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double
     }
     P.scheme = c.scheme; P.first = first; P.dp_flag = c.dp_flag;
     for (int a = 0; a < 3; a++) P.per[a] = (c.bc[a] == 1);
-    P.zc = 16;
     P.tiles_x = (c.nx + TX - 1) / TX;
     P.tiles_y = (c.ny + TY - 1) / TY;
+    P.zc = choose_zc(c.nz, (long)P.tiles_x * P.tiles_y, resident_blocks(lazy));
     P.tiles_z = (c.nz + P.zc - 1) / P.zc;
     P.ntiles = P.tiles_x * P.tiles_y * P.tiles_z;
     P.src[0] = u; P.src[1] = v; P.src[2] = w;
