@@ -343,11 +343,12 @@ inline size_t comp_size(const Grid &g, int c) { return c == 0 ? g.nu() : c == 1 
 
 int run_poisson(ns_ctx *c, const double *us, const double *vs, const double *ws, const double *rhs_in,
                 const double *planes, const double *mean_dev, double mean_host, double *dp_out, double *sol_out,
-                hipEvent_t *ev = nullptr) {
+                hipEvent_t *ev = nullptr, bool dp_ghost_x = false) {
     const ns_config &f = c->cfg;
     if (!c->poisson_ok) return NS_ERR_UNSUPPORTED;
     if (!c->pp.dct)
-        poisson_fft_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, c->spec, dp_out, sol_out, c->st, ev);
+        poisson_fft_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, c->spec, dp_out, sol_out, c->st, ev,
+                          dp_ghost_x);
     else
         poisson_dct_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, planes, mean_dev, mean_host,
                           c->work, dp_out, sol_out, c->st, ev);
@@ -388,6 +389,8 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     const Grid &g = c->g;
     const bool cap = c->cap_on && c->cap_count < c->cap_max && ((c->cap_step + 1) % c->cap_every == 0);
     if (ev) (void)hipEventRecord(ev[0], c->st);
+    // the periodic x ghosts of dp are written by the x inverse transform
+    const bool pghost = !c->pp.dct && f.pbc[0] == 1;
     if (c->lazy) {
         const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
         launch_predict_tiled(f, c->first, true, c->us, c->vs, c->ws, c->dp, gauge, c->p, c->hx, c->hy, c->hz, c->us2,
@@ -413,9 +416,9 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     }
     if (ev) (void)hipEventRecord(ev[2], c->st);
     int rc = run_poisson(c, c->us, c->vs, c->ws, nullptr, c->planes, c->planes + c->L.off[24], 0.0, c->dp, nullptr,
-                         ev ? ev + 3 : nullptr);
+                         ev ? ev + 3 : nullptr, pghost);
     if (rc) return rc;
-    launch_pr_bc(g, f.pbc, c->planes, f.dx, f.dy, f.dz, c->dp, c->st);
+    launch_pr_bc(g, f.pbc, c->planes, f.dx, f.dy, f.dz, c->dp, c->st, pghost ? 1 : 0);
     if (ev) (void)hipEventRecord(ev[8], c->st);
     const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
     if (cap) cap_pressure(c, gauge);

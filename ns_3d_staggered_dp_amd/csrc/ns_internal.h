@@ -69,11 +69,13 @@ void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double
 void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff,
                       const double *u, const double *v, const double *w, double *ox, double *oy, double *oz,
                       hipStream_t st);
-void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w, hipStream_t st);
+// fills axes first_axis..2 (the earlier axes already filled by the producer)
+void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w, hipStream_t st,
+                   int first_axis = 0);
 void launch_vel_bc_cn2(const Grid &g, int bc, int dir, const double *planes, double *u, double *v, double *w,
                        hipStream_t st);
 void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,
-                  hipStream_t st);
+                  hipStream_t st, int first_axis = 0);
 void launch_modify_rhs(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz,
                        double *rhs, hipStream_t st);
 void launch_divergence(const Grid &g, double dx, double dy, double dz, double dt, const double *us, const double *vs,
@@ -116,7 +118,8 @@ int spectrum_row(int nx);
 // ev (optional): 5 events recorded after the five passes
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
-                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr);
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr,
+                       bool dp_ghost_x = false);
 void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in,
                        const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,

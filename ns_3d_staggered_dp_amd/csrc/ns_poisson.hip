@@ -362,6 +362,7 @@ struct FFTPass {
     const double *ex, *ey, *ez;   // eigenvalue parts
     double scale;                 // backward scale 1/(nx*ny*nz)
     double *dp_out;               // ghosted dp (interior written) or null
+    int dp_ghost_x;               // also write dp's periodic x ghosts
     double *sol_out;              // interior (nx,ny,nz) or null
 };
 
@@ -519,7 +520,15 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
         const double2 z = A[(rline >> 1) * LS + x];
         const double val = ((rline & 1) ? z.y : z.x) * sc;
         const int k = idiv(rl, P.dNy), j = rl - k * g.ny;
-        if (P.dp_out) P.dp_out[FIX(x + 2, j + 2, k + 2, g.nx + 2, g.ny + 2)] = val;
+        if (P.dp_out) {
+            const size_t q = FIX(x + 2, j + 2, k + 2, g.nx + 2, g.ny + 2);
+            P.dp_out[q] = val;
+            // x part of pr_bc_staggered (pbc 1): dp(1) = dp(nx+1), dp(nx+2) = dp(2)
+            if (P.dp_ghost_x) {
+                if (x == 0) P.dp_out[q + g.nx] = val;
+                if (x == g.nx - 1) P.dp_out[q - g.nx] = val;
+            }
+        }
         if (P.sol_out) P.sol_out[(size_t)x + (size_t)g.nx * rl] = val;
     }
 }
@@ -775,7 +784,7 @@ IDiv make_idiv(int d) { return IDiv{d, 1.0f / (float)d}; }
 
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
-                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev, bool dp_ghost_x) {
     FFTPass P{};
     P.g = g;
     P.nxh = g.nx / 2 + 1;
@@ -791,6 +800,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     P.ex = pl.ex; P.ey = pl.ey; P.ez = pl.ez;
     P.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
     P.dp_out = dp_out;
+    P.dp_ghost_x = dp_ghost_x ? 1 : 0;
     P.sol_out = sol_out;
     const long RL = (long)g.ny * g.nz;
     auto lds_x = [&]() { return ((size_t)(1 << P.lsh) * P.LS + g.nx) * sizeof(double2); };

@@ -590,7 +590,7 @@ static size_t max_plane(const Grid &g, int ax) {
 }
 
 void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w,
-                   hipStream_t st) {
+                   hipStream_t st, int first_axis) {
     BCParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
     P.L = plane_layout(g.nx, g.ny, g.nz);
@@ -598,7 +598,7 @@ void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double 
     P.a[0] = u; P.a[1] = v; P.a[2] = w;
     P.pbc_mode = 0;
     P.d = 0;
-    for (int ax = 0; ax < 3; ax++) {
+    for (int ax = first_axis; ax < 3; ax++) {
         P.ax = ax;
         P.bc = bc[ax];
         if (bc[ax] == 3) continue;
@@ -622,7 +622,7 @@ void launch_vel_bc_cn2(const Grid &g, int bc, int dir, const double *planes, dou
 }
 
 void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,
-                  hipStream_t st) {
+                  hipStream_t st, int first_axis) {
     BCParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
     P.L = plane_layout(g.nx, g.ny, g.nz);
@@ -631,7 +631,7 @@ void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double 
     P.pbc_mode = 2;
     double d[3] = {dx, dy, dz};
     int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
-    for (int ax = 0; ax < 3; ax++) {
+    for (int ax = first_axis; ax < 3; ax++) {
         P.ax = ax;
         P.bc = pbc[ax];
         P.d = d[ax];
