@@ -45,6 +45,10 @@ extern "C" {
  * is mat_CN's (NS_functions.f90:1273-1278), whose row 1 couples to column
  * nx+1 (== u(1)) instead of nx. */
 #define NS_FLAG_CN_CYCLIC 2
+/* ns_step runs every step as individual kernel launches.  Without it, steady
+ * steps (not the first, no plane capture active) are replayed two at a time
+ * from a HIP graph captured once per context; the results are identical. */
+#define NS_FLAG_NO_GRAPH 4
 
 /* Run configuration: the driver parameter block (big_periodic_3D.f90:9-21,
  * 104-108; re_simulation_sub_tstep_interBC_combine.f90:100-112).

@@ -20,6 +20,8 @@ FLAG_GAUGE_CORNER = 1
 # periodic CN rows use the wrap stencil instead of mat_CN's row 1
 # (NS_functions.f90:1273-1278 couples row 1 to column nx+1)
 FLAG_CN_CYCLIC = 2
+# every step as individual launches (no HIP graph replay of steady steps)
+FLAG_NO_GRAPH = 4
 
 PI = 3.1415926535897932  # big_periodic_3D.f90:6
 

@@ -44,6 +44,10 @@ struct ns_ctx {
     double *cap_dst = nullptr;
     double *cap_partial = nullptr;
     int poisson_ok = 0;  // 0: mixed periodic / non-periodic codes (no transform solver)
+    // two steady steps captured as one graph, per parity of the lazy u* buffers
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double *us_base = nullptr;  // u* buffer of parity 0
+    int graph_off = 0;          // capture failed or disabled
     std::vector<void *> allocs;
 };
 
@@ -508,6 +512,7 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
     HIPCHK(hipMemsetAsync(c->us2, 0, g.nu() * 8, c->st));
     HIPCHK(hipMemsetAsync(c->vs2, 0, g.nv() * 8, c->st));
     HIPCHK(hipMemsetAsync(c->ws2, 0, g.nw() * 8, c->st));
+    c->us_base = c->us;
     // lazy projection (dp folded into the next predictor) needs periodic pressure
     // on every periodic velocity axis: the predictor reads dp(n+1) as dp(1)
     c->lazy = (cfg->dp_flag == 0);
@@ -522,6 +527,8 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
 int ns_destroy(ns_ctx *c) {
     if (!c) return NS_ERR_INVALID;
     if (c->st) (void)hipStreamSynchronize(c->st);
+    for (int i = 0; i < 2; i++)
+        if (c->graph[i]) (void)hipGraphExecDestroy(c->graph[i]);
     for (void *q : c->allocs) (void)hipFree(q);
     if (c->snaps) (void)hipFree(c->snaps);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -635,11 +642,49 @@ int ns_bc_interp(ns_ctx *c, const double *a, const double *b, double theta) {
     return NS_OK;
 }
 
+// capture two steady steps on the context stream into graph[parity]; the
+// host-side state (u* buffer swaps, flags) is the same before and after
+int build_graph(ns_ctx *c, int parity) {
+    hipGraph_t gr = nullptr;
+    if (hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return NS_ERR_HIP;
+    }
+    int rc = one_step(c);
+    if (rc == NS_OK) rc = one_step(c);
+    const hipError_t e = hipStreamEndCapture(c->st, &gr);
+    if (rc == NS_OK && (e != hipSuccess || !gr)) rc = NS_ERR_HIP;
+    if (rc == NS_OK && hipGraphInstantiate(&c->graph[parity], gr, nullptr, nullptr, 0) != hipSuccess) {
+        c->graph[parity] = nullptr;
+        rc = NS_ERR_HIP;
+    }
+    if (gr) (void)hipGraphDestroy(gr);
+    (void)hipGetLastError();
+    return rc;
+}
+
 int ns_step(ns_ctx *c, int nsteps) {
     if (!c || nsteps < 0) return NS_ERR_INVALID;
-    for (int s = 0; s < nsteps; s++) {
+    int s = 0;
+    while (s < nsteps) {
+        const bool steady = !c->first && !c->cap_on && !c->graph_off && !(c->cfg.flags & NS_FLAG_NO_GRAPH);
+        if (steady && nsteps - s >= 2) {
+            const int parity = (c->us == c->us_base) ? 0 : 1;
+            if (!c->graph[parity] && build_graph(c, parity) != NS_OK) {
+                c->graph_off = 1;  // run eagerly from here on
+                continue;
+            }
+            HIPCHK(hipGraphLaunch(c->graph[parity], c->st));
+            // host-side effects of the two steps (one_step's, minus the buffer
+            // swaps, which cancel): u, v, w, p must be re-formed on demand
+            if (c->lazy) c->stale = 1;
+            c->first = 0;
+            s += 2;
+            continue;
+        }
         int rc = one_step(c);
         if (rc) return rc;
+        s++;
     }
     if (hipGetLastError() != hipSuccess) return NS_ERR_HIP;
     return NS_OK;

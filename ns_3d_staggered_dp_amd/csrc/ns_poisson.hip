@@ -769,12 +769,15 @@ void launch_lds(K kernel, unsigned nb, size_t lds, hipStream_t st, const FFTPass
     hipLaunchKernelGGL(kernel, dim3(nb), dim3(FFT_NT), lds, st, P);
 }
 
-// log2 of the lines per block: the largest power of two L with L*n <= FFT_ELEMS
-int fft_lsh(int n) {
+// log2 of the lines per block: the largest power of two L with L*n <= elems
+int fft_lsh(int n, int elems = FFT_ELEMS) {
     int s = 0;
-    while ((2 << s) * n <= FFT_ELEMS) s++;
+    while ((2 << s) * n <= elems) s++;
     return s;
 }
+// x passes: contiguous lines need no grouping for coalescing, so they use
+// fewer elements per block (more resident blocks per CU; measured at 256^3)
+constexpr int FFT_XF_ELEMS = FFT_ELEMS / 4, FFT_XI_ELEMS = FFT_ELEMS / 2;
 // line-major row stride: n rounded up to 2 (mod 16) complex
 int fft_row(int n) { return n + ((2 - n) % 16 + 16) % 16; }
 
@@ -808,7 +811,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     auto nb_yz = [&](int others) { return (unsigned)((((P.nxh + (1 << P.lsh) - 1) >> P.lsh)) * others); };
     // x forward (fused with the divergence)
     P.pl = pl.px;
-    P.lsh = fft_lsh(g.nx);
+    P.lsh = fft_lsh(g.nx, FFT_XF_ELEMS);
     P.LS = fft_row(g.nx);
     const unsigned nbx = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
     auto pow2 = [](int n) { return (n & (n - 1)) == 0; };
@@ -835,10 +838,11 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     if (ev) (void)hipEventRecord(ev[3], st);
     // x inverse (c2r) writing dp
     P.pl = pl.px;
-    P.lsh = fft_lsh(g.nx);
+    P.lsh = fft_lsh(g.nx, FFT_XI_ELEMS);
     P.LS = fft_row(g.nx);
-    if (pow2(g.nx)) launch_lds(k_fft_x_inv<false>, nbx, lds_x(), st, P);
-    else launch_lds(k_fft_x_inv<true>, nbx, lds_x(), st, P);
+    const unsigned nbxi = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
+    if (pow2(g.nx)) launch_lds(k_fft_x_inv<false>, nbxi, lds_x(), st, P);
+    else launch_lds(k_fft_x_inv<true>, nbxi, lds_x(), st, P);
     if (ev) (void)hipEventRecord(ev[4], st);
 }
 

@@ -246,3 +246,42 @@ def test_step_walls_lazy_multi_tile(oracle, scheme):
     out, st, div = run_both(oracle, cfg, u, v, w, p, 4, planes=planes)
     for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
         assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+
+
+@pytest.mark.parametrize("case", ["periodic_ab2_lazy", "periodic_ab2cn", "walls_dct_dpflag"])
+def test_graph_replay_matches_eager(case):
+    """ns_step replays steady steps from a HIP graph (two steps per replay, one
+    graph per parity of the lazy u* buffers).  The same call pattern with
+    eager launches (FLAG_NO_GRAPH) must give bitwise the same state after
+    every call; host downloads and odd step counts sit between the replays."""
+    from ns_3d_staggered_dp_amd import FLAG_NO_GRAPH
+
+    if case == "periodic_ab2_lazy":
+        cfg = cfg_periodic(64, 16, 24, scheme=AB2, dp_flag=0)
+        u, v, w, p = fields.decaying_turbulence(cfg, seed=9)
+        planes = None
+    elif case == "periodic_ab2cn":
+        cfg = cfg_periodic(32, 20, 16, scheme=AB2_CN, dp_flag=0)
+        u, v, w, p = fields.taylor_green(cfg, seed=3)
+        planes = None
+    else:
+        cfg = cfg_box(40, 12, 18, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2, dp_flag=1).replace(dt=1e-3)
+        u, v, w, p = rand_fields(cfg, 86)
+        u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
+        planes = rand_planes(cfg, 87, 0.2)
+    pattern = (1, 4, 3, 1, 2, 5)
+    runs = []
+    for flags in (cfg.flags | FLAG_NO_GRAPH, cfg.flags):
+        c = cfg.replace(flags=flags)
+        states = []
+        with NSSolver(c) as s:
+            s.upload(u, v, w, p, first=True)
+            if planes is not None:
+                s.set_bc_planes(planes)
+            for n in pattern:
+                s.step(n)
+                states.append(s.download())
+        runs.append(states)
+    for a, b in zip(*runs):
+        for k in ("u", "v", "w", "p", "hx", "hy", "hz"):
+            assert np.array_equal(a[k], b[k]), k
