@@ -167,17 +167,19 @@ __device__ __forceinline__ void other_axes(int ax, int &o1, int &o2) {
     o2 = (ax == 2) ? 1 : 2;
 }
 
-// velocity ghost/wall fill along one axis (NS_functions.f90:235-308), all three components
+// velocity ghost/wall fill along one axis (NS_functions.f90:235-308), all three
+// components; blockIdx.y selects the component
 __global__ void __launch_bounds__(256) k_vel_bc_axis(BCParams P) {
     Grid g{P.nx, P.ny, P.nz};
     int o1, o2;
     other_axes(P.ax, o1, o2);
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int c = 0; c < 3; c++) {
+    {
+        const int c = blockIdx.y;
         int n[3];
         comp_dims(g, c, n);
         size_t tot = (size_t)n[o1] * n[o2];
-        if (t >= tot) continue;
+        if (t >= tot) return;
         int pp = (int)(t % n[o1]) + 1, qq = (int)(t / n[o1]) + 1;
         int i1[3], i2[3];
         i1[o1] = i2[o1] = pp;
@@ -602,7 +604,7 @@ void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double 
         P.ax = ax;
         P.bc = bc[ax];
         if (bc[ax] == 3) continue;
-        hipLaunchKernelGGL(k_vel_bc_axis, dim3(nblocks(max_plane(g, ax))), dim3(256), 0, st, P);
+        hipLaunchKernelGGL(k_vel_bc_axis, dim3(nblocks(max_plane(g, ax)), 3), dim3(256), 0, st, P);
     }
 }
 
