@@ -34,9 +34,10 @@ ARRAYS_PER_CELL = {"predictor": 13, "vel_bc": 0, "div_fft_x_fwd": 4, "fft_y_fwd"
                    "fft_y_inv": 2, "fft_x_inv_dp": 2, "pr_bc": 0, "project": 0}
 # the reference pipeline's budget (explicit projection pass): 32 arrays = 256 B/cell-step
 REFERENCE_BUDGET_B = 256
-KERNEL_NAMES = {"predictor": "k_predict_tiled<true>", "div_fft_x_fwd": "k_fft_x_fwd", "fft_y_fwd": "k_fft_yz<1,-1,0>",
-                "fft_z_solve": "k_fft_yz<2,-1,1>", "fft_y_inv": "k_fft_yz<1,1,0>", "fft_x_inv_dp": "k_fft_x_inv",
-                "project": "k_project"}
+KERNEL_NAMES = {"predictor": "k_predict_tiled<true>", "div_fft_x_fwd": "k_fft_x_fwd<false>",
+                "fft_y_fwd": "k_fft_yz<1,-1,0,false>", "fft_z_solve": "k_fft_yz<2,-1,1,false>",
+                "fft_y_inv": "k_fft_yz<1,1,0,false>", "fft_x_inv_dp": "k_fft_x_inv<false>", "project": "k_project"}
+COPY_CAL_BYTES = 2 ** 30  # the device-to-device copy used to scale the HBM counters
 
 
 def parse():
@@ -48,7 +49,84 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_256.json"))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--summarize-profile", metavar="DIR",
+                    help="(no GPU) turn rocprofv3 output under DIR (kt/, FETCH_SIZE/, WRITE_SIZE/, cal_FETCH_SIZE/, "
+                         "cal_WRITE_SIZE/) into the per-launch traffic JSON and a markdown table, then exit")
     return ap.parse_args()
+
+
+def short_kernel_name(name):
+    """'void ns::(anonymous namespace)::k_fft_yz<1, -1, 0, false>(ns::...)' -> 'k_fft_yz<1,-1,0,false>'"""
+    s = name.strip('"')
+    if s.startswith("void "):
+        s = s[5:]
+    s = s.replace("ns::(anonymous namespace)::", "")
+    depth, out = 0, []
+    for ch in s:
+        if ch == "(" and depth == 0:
+            break
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        out.append(ch)
+    return "".join(out).replace(" ", "")
+
+
+def summarize_profile(d, traffic_json):
+    """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per
+    dispatch), each scaled by what the same counter reads for the 1 GiB copy."""
+    import csv
+    import glob
+
+    def counter(sub, cname):
+        per = {}
+        for f in glob.glob(os.path.join(d, sub, "*counter_collection.csv")):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == cname:
+                    per.setdefault(short_kernel_name(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in per.items()}
+
+    fetch, write = counter("FETCH_SIZE", "FETCH_SIZE"), counter("WRITE_SIZE", "WRITE_SIZE")
+    cf, cw = counter("cal_FETCH_SIZE", "FETCH_SIZE"), counter("cal_WRITE_SIZE", "WRITE_SIZE")
+    copy = [k for k in cf if "copyBuffer" in k]
+    f_fac = (COPY_CAL_BYTES / 1024) / max(cf[k] for k in copy)
+    w_fac = (COPY_CAL_BYTES / 1024) / max(cw[k] for k in [k for k in cw if "copyBuffer" in k])
+    stats = {}
+    for f in glob.glob(os.path.join(d, "kt", "*kernel_stats.csv")):
+        for r in csv.DictReader(open(f)):
+            stats[short_kernel_name(r["Name"])] = (int(r["Calls"]), float(r["AverageNs"]) / 1e3)
+    n3 = 256 ** 3
+    traffic, rows = {}, []
+    for phase, kname in KERNEL_NAMES.items():
+        if kname not in fetch or kname not in write:
+            continue
+        rd, wr = fetch[kname] * f_fac * 1024, write[kname] * w_fac * 1024
+        traffic[kname] = int(rd + wr)
+        alg = ARRAYS_PER_CELL.get(phase, 0) * 8 * n3 if phase != "project" else 8 * 8 * n3
+        calls, us = stats.get(kname, (0, float("nan")))
+        rows.append(f"| {kname} | {calls} | {us:.1f} | {alg / 1e6:.0f} | {rd / 1e6:.0f} | {wr / 1e6:.0f} | "
+                    f"{(rd + wr) / alg:.2f} | {alg / (us * 1e-6) / 1e9:.0f} | {(rd + wr) / (us * 1e-6) / 1e9:.0f} |")
+    json.dump(traffic, open(traffic_json, "w"), indent=1)
+    print(f"FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} (1 GiB copy)")
+    print("| kernel | calls | avg us | alg MB/launch | HBM read MB | HBM write MB | traffic/alg | alg GB/s | HBM GB/s |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
+    print("\n".join(rows))
+    return 0
+
+
+def copy_calibration():
+    import torch
+
+    a = torch.ones(COPY_CAL_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy calibration done", float(b[0]), flush=True)
+    return 0
 
 
 def cpu_baseline(cfg, u, v, w, p, nsteps):
@@ -73,6 +151,10 @@ def cpu_baseline(cfg, u, v, w, p, nsteps):
 
 def main():
     a = parse()
+    if a.summarize_profile:
+        return summarize_profile(a.summarize_profile, a.traffic_json)
+    if a.copy_calibration:
+        return copy_calibration()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU decomposition is "
               "described in DESIGN.md and is not built.", file=sys.stderr)
