@@ -130,6 +130,16 @@ int ns_capture_end(ns_ctx *ctx, int *nsets_written);
 /* max |div u| over interior cells (big_periodic_3D.f90:576-579) */
 int ns_max_div(ns_ctx *ctx, double *out);
 int ns_check_finite(ns_ctx *ctx);
+/* Re-simulation error metrics of the context's current u, v, w, p against the
+ * parent's sub-box fields (host or device arrays of the ghosted shapes), over
+ * the full ghosted arrays (re_simulation_sub_tstep_interBC_combine.f90:826-833,
+ * other_utility.f90:110-129):
+ *   rms[0..3] = rms(u_sub), rms(v_sub), rms(w_sub), rms(p_sub)
+ *   err[0..3] = max|x - x_sub| / rms(x_sub)   (u, v, w, p)
+ *   err[4..6] = mean|x - x_sub| / rms(x_sub)  (u, v, w)
+ *   err[7]    = max|oz - oz_sub| / rms(oz_sub), oz = diff(v,1,1)/dx - diff(u,1,2)/dy */
+int ns_error_metrics(ns_ctx *ctx, const double *u_sub, const double *v_sub, const double *w_sub,
+                     const double *p_sub, double err[8], double rms[4]);
 
 /* ---- per-subroutine mirrors with the reference's arguments ---- */
 /* cal_conv NS_functions.f90:53-59; conv_* are (nx or nx-1,ny,nz) etc. */

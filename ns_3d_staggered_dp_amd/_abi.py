@@ -57,6 +57,7 @@ SIGNATURES = {
     "ns_capture_end": [P, ctypes.POINTER(I)],
     "ns_max_div": [P, ctypes.POINTER(D)],
     "ns_check_finite": [P],
+    "ns_error_metrics": [P, P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],
     "ns_cal_diff": [P, P, P, P, I, I, I, D, D, D, P, P, P],
     "ns_vel_bc_staggered": [P, P, P, P, P, I, I, I],

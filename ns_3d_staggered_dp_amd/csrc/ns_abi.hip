@@ -48,6 +48,7 @@ struct ns_ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     double *us_base = nullptr;  // u* buffer of parity 0
     int graph_off = 0;          // capture failed or disabled
+    double *mpartial = nullptr; // error-metric block partials
     std::vector<void *> allocs;
 };
 
@@ -788,6 +789,54 @@ int ns_max_div(ns_ctx *c, double *out) {
     double m = 0;
     for (double x : h) m = x > m ? x : m;
     *out = m;
+    return NS_OK;
+}
+
+int ns_error_metrics(ns_ctx *c, const double *u_sub, const double *v_sub, const double *w_sub, const double *p_sub,
+                     double *err, double *rms) {
+    if (!c || !u_sub || !v_sub || !w_sub || !p_sub || !err || !rms) return NS_ERR_INVALID;
+    materialize(c);
+    const Grid &g = c->g;
+    const int MAXB = 2048;
+    if (!c->mpartial) {
+        int rc = dalloc(c, &c->mpartial, (size_t)MAXB * NS_METRIC_PARTS);
+        if (rc) return rc;
+    }
+    ArgList args;
+    Arg &A0 = args.add();
+    Arg &A1 = args.add();
+    Arg &A2 = args.add();
+    Arg &A3 = args.add();
+    int rc;
+    if ((rc = arg_in(c, A0, u_sub, g.nu(), true)) || (rc = arg_in(c, A1, v_sub, g.nv(), true)) ||
+        (rc = arg_in(c, A2, w_sub, g.nw(), true)) || (rc = arg_in(c, A3, p_sub, g.np(), true)))
+        return rc;
+    const double *a[4] = {c->u, c->v, c->w, c->p};
+    const double *b[4] = {A0.dev, A1.dev, A2.dev, A3.dev};
+    const int nb = launch_err_partials(g, c->cfg.dx, c->cfg.dy, a, b, c->mpartial, MAXB, c->st);
+    std::vector<double> h((size_t)nb * NS_METRIC_PARTS);
+    HIPCHK(hipMemcpyAsync(h.data(), c->mpartial, h.size() * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double mx[5] = {0, 0, 0, 0, 0};
+    long double sabs[4] = {0, 0, 0, 0}, ssq[5] = {0, 0, 0, 0, 0};
+    for (int blk = 0; blk < nb; blk++) {
+        const double *q = &h[(size_t)blk * NS_METRIC_PARTS];
+        for (int cc = 0; cc < 4; cc++) {
+            mx[cc] = q[3 * cc] > mx[cc] ? q[3 * cc] : mx[cc];
+            sabs[cc] += q[3 * cc + 1];
+            ssq[cc] += q[3 * cc + 2];
+        }
+        mx[4] = q[12] > mx[4] ? q[12] : mx[4];
+        ssq[4] += q[13];
+    }
+    const double n[4] = {(double)g.nu(), (double)g.nv(), (double)g.nw(), (double)g.np()};
+    const double noz = (double)((size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 2));
+    for (int cc = 0; cc < 4; cc++) {
+        rms[cc] = sqrt((double)(ssq[cc] / n[cc]));
+        err[cc] = mx[cc] / rms[cc];
+    }
+    for (int cc = 0; cc < 3; cc++) err[4 + cc] = (double)(sabs[cc] / n[cc]) / rms[cc];
+    err[7] = mx[4] / sqrt((double)(ssq[4] / noz));
     return NS_OK;
 }
 

@@ -125,6 +125,12 @@ void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
                        const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
                        double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr);
 
+// ---- re-simulation error metrics (ns_metrics.hip) ----
+// partials per block: for u, v, w, p: max|a-b|, sum|a-b|, sum b^2; then max|oz-oz_b|, sum oz_b^2
+#define NS_METRIC_PARTS 14
+int launch_err_partials(const Grid &g, double dx, double dy, const double *const a[4], const double *const b[4],
+                        double *partial, int max_blocks, hipStream_t st);
+
 // ---- ADI (ns_adi.hip) ----
 void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
                       hipStream_t st);

@@ -1,0 +1,95 @@
+// Re-simulation error metrics (re_simulation_sub_tstep_interBC_combine.f90:
+// 826-833, other_utility.f90:110-129) in one pass over the ghosted arrays:
+// per block, for u, v, w, p the max and sum of |x - x_sub| and the sum of
+// x_sub^2, and for the z vorticity oz = diff(v,1,1)/dx - diff(u,1,2)/dy the
+// max of |oz - oz_sub| and the sum of oz_sub^2.  The host combines the block
+// partials (max, and sums in extended precision).
+#include "ns_internal.h"
+
+namespace ns {
+
+namespace {
+
+constexpr int MT = 256;
+
+struct MetricParams {
+    Grid g;
+    double dx, dy;
+    const double *a[4];  // u, v, w, p of the run
+    const double *b[4];  // u_sub, v_sub, w_sub, p_sub of the parent
+    size_t n[4], noz, total;
+    double *partial;     // [block][NS_METRIC_PARTS]
+};
+
+__global__ void __launch_bounds__(MT) k_err_partials(MetricParams P) {
+    double acc[NS_METRIC_PARTS];
+#pragma unroll
+    for (int q = 0; q < NS_METRIC_PARTS; q++) acc[q] = 0.0;
+    const Grid g = P.g;
+    const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1;
+    const int O1 = g.nx + 1, O2 = g.ny + 1;  // oz is (nx+1, ny+1, nz+2)
+    for (size_t e = (size_t)blockIdx.x * MT + threadIdx.x; e < P.total; e += (size_t)gridDim.x * MT) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (e < P.n[c]) {
+                const double x = P.a[c][e], y = P.b[c][e];
+                const double d = fabs(x - y);
+                acc[3 * c] = fmax(acc[3 * c], d);
+                acc[3 * c + 1] += d;
+                acc[3 * c + 2] += y * y;
+            }
+        }
+        if (e < P.noz) {
+            const int i = (int)(e % O1) + 1;
+            const int j = (int)((e / O1) % O2) + 1;
+            const int k = (int)(e / ((size_t)O1 * O2)) + 1;
+            const size_t v0 = FIX(i, j, k, V1, V2), v1 = FIX(i + 1, j, k, V1, V2);
+            const size_t u0 = FIX(i, j, k, U1, U2), u1 = FIX(i, j + 1, k, U1, U2);
+            const double oz = (P.a[1][v1] - P.a[1][v0]) / P.dx - (P.a[0][u1] - P.a[0][u0]) / P.dy;
+            const double ozs = (P.b[1][v1] - P.b[1][v0]) / P.dx - (P.b[0][u1] - P.b[0][u0]) / P.dy;
+            acc[12] = fmax(acc[12], fabs(oz - ozs));
+            acc[13] += ozs * ozs;
+        }
+    }
+    __shared__ double red[MT];
+    for (int q = 0; q < NS_METRIC_PARTS; q++) {
+        const bool is_max = (q == 12) || (q < 12 && q % 3 == 0);
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int s = MT / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s)
+                red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                          : red[threadIdx.x] + red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) P.partial[(size_t)blockIdx.x * NS_METRIC_PARTS + q] = red[0];
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int launch_err_partials(const Grid &g, double dx, double dy, const double *const a[4], const double *const b[4],
+                        double *partial, int max_blocks, hipStream_t st) {
+    MetricParams P;
+    P.g = g;
+    P.dx = dx;
+    P.dy = dy;
+    const size_t n[4] = {g.nu(), g.nv(), g.nw(), g.np()};
+    P.total = 0;
+    for (int c = 0; c < 4; c++) {
+        P.a[c] = a[c];
+        P.b[c] = b[c];
+        P.n[c] = n[c];
+        P.total = n[c] > P.total ? n[c] : P.total;
+    }
+    P.noz = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 2);
+    P.total = P.noz > P.total ? P.noz : P.total;
+    P.partial = partial;
+    size_t nb = (P.total + MT - 1) / MT;
+    if (nb > (size_t)max_blocks) nb = max_blocks;
+    hipLaunchKernelGGL(k_err_partials, dim3((unsigned)nb), dim3(MT), 0, st, P);
+    return (int)nb;
+}
+
+}  // namespace ns

@@ -136,6 +136,14 @@ class NSSolver:
         _abi.call("ns_max_div", self.h, ctypes.byref(out))
         return out.value
 
+    def error_metrics(self, u_sub, v_sub, w_sub, p_sub):
+        """(err[8], rms[4]) of the current state against the parent's sub-box
+        fields, as the reference's per-step error table (see metrics.py)."""
+        err = (ctypes.c_double * 8)()
+        rms = (ctypes.c_double * 4)()
+        _abi.call("ns_error_metrics", self.h, _ptr(u_sub), _ptr(v_sub), _ptr(w_sub), _ptr(p_sub), err, rms)
+        return np.array(err[:]), np.array(rms[:])
+
     def check_finite(self) -> bool:
         rc = _abi.lib().ns_check_finite(self.h)
         if rc == _abi.NS_OK:
