@@ -109,6 +109,29 @@ int ns_step_profiled(ns_ctx *ctx, int nsteps, double *phase_ms);
  * snapshots: snapshot s is at time s*snap_dt, step m uses
  * t = t0 + (m+1)*dt (the boundary data of the step's end time). */
 int ns_resim_advance(ns_ctx *ctx, const double *snapshots, int nsnap, double snap_dt, double t0, int nsteps);
+/* The same with MATLAB interp1(...,'spline') in time, i.e. the not-a-knot
+ * cubic spline through a window of six snapshots, three at or before t,
+ * kept inside the history (sub_tstep_BC_combined_interpolation.m:47-64;
+ * the reference interpolates u*, dp boxes offline and extracts planes, which
+ * is the same linear map applied in the other order). */
+int ns_resim_advance_spline(ns_ctx *ctx, const double *snapshots, int nsnap, double snap_dt, double t0,
+                            int nsteps);
+/* the current plane set := that spline interpolation at time t */
+int ns_bc_interp_spline(ns_ctx *ctx, const double *snapshots, int nsnap, double snap_dt, double t);
+/* Host only (no GPU): weights w[0..m-1] with spline(x) = sum_i w[i]*y[i] for
+ * the not-a-knot cubic spline through (t[i], y[i]), 2 <= m <= 8, t increasing
+ * (m = 3: the parabola, m = 2: the line, as MATLAB's spline). */
+int ns_spline_weights(const double *t, int m, double x, double *w);
+/* Time step of the following steps (the re-simulation's sub-steps use
+ * dt0/sub_tstep, re_simulation_sub_tstep_interBC_combine.f90:446-448).
+ * Rebuilds the CN line systems; a pending lazy projection is applied first
+ * with the old dt. */
+int ns_set_dt(ns_ctx *ctx, double dt);
+/* Save / restore the AB2 history rhs_*_previous: the reference saves it
+ * after the first sub-step and restores it after the last, so the first full
+ * step's AB2 extrapolation spans one dt0 (…interBC_combine.f90:602-612). */
+int ns_history_save(ns_ctx *ctx);
+int ns_history_restore(ns_ctx *ctx);
 /* Parent-run capture of a sub-box's boundary data, the per-step sub-box
  * dump of big_periodic_3D.f90:433-489,664-712 reduced to what a re-simulation
  * reads: after ns_capture_begin, every `every`-th step writes one plane set

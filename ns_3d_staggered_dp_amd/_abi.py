@@ -58,6 +58,12 @@ SIGNATURES = {
     "ns_max_div": [P, ctypes.POINTER(D)],
     "ns_check_finite": [P],
     "ns_error_metrics": [P, P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)],
+    "ns_resim_advance_spline": [P, P, I, D, D, I],
+    "ns_bc_interp_spline": [P, P, I, D, D],
+    "ns_spline_weights": [P, I, D, P],
+    "ns_set_dt": [P, D],
+    "ns_history_save": [P],
+    "ns_history_restore": [P],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],
     "ns_cal_diff": [P, P, P, P, I, I, I, D, D, D, P, P, P],
     "ns_vel_bc_staggered": [P, P, P, P, P, I, I, I],

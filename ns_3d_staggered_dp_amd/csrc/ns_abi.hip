@@ -49,6 +49,7 @@ struct ns_ctx {
     double *us_base = nullptr;  // u* buffer of parity 0
     int graph_off = 0;          // capture failed or disabled
     double *mpartial = nullptr; // error-metric block partials
+    double *hsave[3] = {nullptr, nullptr, nullptr};  // saved AB2 history (sub-step phase)
     std::vector<void *> allocs;
 };
 
@@ -712,22 +713,129 @@ int ns_step_profiled(ns_ctx *c, int nsteps, double *phase_ms) {
     return rc;
 }
 
+// snapshot history on the device: device pointers are used in place, host
+// arrays are copied into the context's staging buffer
+static int stage_snaps(ns_ctx *c, const double *snapshots, int nsnap, const double **base) {
+    const size_t len = (size_t)c->L.total;
+    *base = snapshots;
+    if (is_dev_ptr(snapshots)) return NS_OK;
+    size_t need = len * nsnap;
+    if (c->snaps_len < need) {
+        if (c->snaps) (void)hipFree(c->snaps);
+        c->snaps = nullptr;
+        c->snaps_len = 0;
+        HIPCHK(hipMalloc(&c->snaps, need * 8));
+        c->snaps_len = need;
+    }
+    HIPCHK(hipMemcpyAsync(c->snaps, snapshots, need * 8, hipMemcpyHostToDevice, c->st));
+    *base = c->snaps;
+    return NS_OK;
+}
+
+// Not-a-knot cubic spline through (t[i], y[i]), i < m, evaluated at x, as a
+// linear combination sum_i w[i]*y[i] of the data (MATLAB interp1 'spline').
+// Second-derivative form: interior rows
+//   h[i-1] M[i-1] + 2(h[i-1]+h[i]) M[i] + h[i] M[i+1] = 6((y[i+1]-y[i])/h[i] - (y[i]-y[i-1])/h[i-1]),
+// end rows from the continuity of the third derivative at t[1] and t[m-2].
+// m = 3 gives the interpolating parabola, m = 2 the line; x outside
+// [t[0], t[m-1]] evaluates the end pieces.
+static int spline_weights(const double *t, int m, double x, double *w) {
+    if (m < 2 || m > NS_WSUM_MAX) return NS_ERR_INVALID;
+    for (int i = 0; i + 1 < m; i++)
+        if (!(t[i + 1] > t[i])) return NS_ERR_INVALID;
+    typedef long double R;
+    if (m == 2) {
+        const R th = ((R)x - t[0]) / ((R)t[1] - t[0]);
+        w[0] = (double)(1 - th);
+        w[1] = (double)th;
+        return NS_OK;
+    }
+    if (m == 3) {
+        for (int i = 0; i < 3; i++) {
+            R l = 1;
+            for (int j = 0; j < 3; j++)
+                if (j != i) l *= ((R)x - t[j]) / ((R)t[i] - t[j]);
+            w[i] = (double)l;
+        }
+        return NS_OK;
+    }
+    R h[NS_WSUM_MAX], A[NS_WSUM_MAX][NS_WSUM_MAX], Bm[NS_WSUM_MAX][NS_WSUM_MAX];
+    for (int i = 0; i + 1 < m; i++) h[i] = (R)t[i + 1] - t[i];
+    for (int r = 0; r < m; r++)
+        for (int c = 0; c < m; c++) A[r][c] = Bm[r][c] = 0;
+    // right-hand side matrix: row r of (6 * second differences) applied to unit data
+    A[0][0] = h[1];
+    A[0][1] = -(h[0] + h[1]);
+    A[0][2] = h[0];
+    for (int i = 1; i + 1 < m; i++) {
+        A[i][i - 1] = h[i - 1];
+        A[i][i] = 2 * (h[i - 1] + h[i]);
+        A[i][i + 1] = h[i];
+        Bm[i][i + 1] += 6 / h[i];
+        Bm[i][i] -= 6 / h[i] + 6 / h[i - 1];
+        Bm[i][i - 1] += 6 / h[i - 1];
+    }
+    A[m - 1][m - 3] = h[m - 2];
+    A[m - 1][m - 2] = -(h[m - 3] + h[m - 2]);
+    A[m - 1][m - 1] = h[m - 3];
+    // solve A * Mj = Bm(:, j) for all unit vectors j at once (Gaussian elimination, partial pivoting)
+    for (int k = 0; k < m; k++) {
+        int piv = k;
+        for (int r = k + 1; r < m; r++)
+            if (fabsl(A[r][k]) > fabsl(A[piv][k])) piv = r;
+        if (A[piv][k] == 0) return NS_ERR_INVALID;
+        for (int c = 0; c < m; c++) {
+            R ta = A[k][c]; A[k][c] = A[piv][c]; A[piv][c] = ta;
+            R tb = Bm[k][c]; Bm[k][c] = Bm[piv][c]; Bm[piv][c] = tb;
+        }
+        for (int r = 0; r < m; r++) {
+            if (r == k || A[r][k] == 0) continue;
+            const R f = A[r][k] / A[k][k];
+            for (int c = 0; c < m; c++) {
+                A[r][c] -= f * A[k][c];
+                Bm[r][c] -= f * Bm[k][c];
+            }
+        }
+    }
+    // Bm(r, j) / A(r, r) = second derivative at t[r] for unit data e_j
+    int k = 0;
+    while (k < m - 2 && x >= t[k + 1]) k++;
+    const R hk = h[k], a = (R)t[k + 1] - x, b = (R)x - t[k];
+    for (int j = 0; j < m; j++) {
+        const R Mk = Bm[k][j] / A[k][k], Mk1 = Bm[k + 1][j] / A[k + 1][k + 1];
+        const R yk = (j == k) ? 1 : 0, yk1 = (j == k + 1) ? 1 : 0;
+        const R s = Mk * a * a * a / (6 * hk) + Mk1 * b * b * b / (6 * hk) + (yk / hk - Mk * hk / 6) * a +
+                    (yk1 / hk - Mk1 * hk / 6) * b;
+        w[j] = (double)s;
+    }
+    return NS_OK;
+}
+
+// window of the spline interpolation over snapshots at k*snap_dt: six
+// snapshots, three at or before t (sub_tstep_BC_combined_interpolation.m:56-57),
+// kept inside the history
+static int spline_planes(ns_ctx *c, const double *base, int nsnap, double snap_dt, double t) {
+    const int m = nsnap < 6 ? nsnap : 6;
+    int kle = (int)floor(t / snap_dt);
+    if (kle > nsnap - 1) kle = nsnap - 1;
+    int s0 = kle - 2;
+    if (s0 > nsnap - m) s0 = nsnap - m;
+    if (s0 < 0) s0 = 0;
+    double tk[NS_WSUM_MAX], w[NS_WSUM_MAX];
+    for (int i = 0; i < m; i++) tk[i] = (s0 + i) * snap_dt;
+    int rc = spline_weights(tk, m, t, w);
+    if (rc) return rc;
+    const size_t len = (size_t)c->L.total;
+    launch_wsum(base + len * s0, (long)len, w, m, c->planes, (long)len, c->st);
+    return NS_OK;
+}
+
 int ns_resim_advance(ns_ctx *c, const double *snapshots, int nsnap, double snap_dt, double t0, int nsteps) {
     if (!c || !snapshots || nsnap < 2 || !(snap_dt > 0) || nsteps < 0) return NS_ERR_INVALID;
     const size_t len = (size_t)c->L.total;
-    const double *base = snapshots;
-    if (!is_dev_ptr(snapshots)) {
-        size_t need = len * nsnap;
-        if (c->snaps_len < need) {
-            if (c->snaps) (void)hipFree(c->snaps);
-            c->snaps = nullptr;
-            c->snaps_len = 0;
-            HIPCHK(hipMalloc(&c->snaps, need * 8));
-            c->snaps_len = need;
-        }
-        HIPCHK(hipMemcpyAsync(c->snaps, snapshots, need * 8, hipMemcpyHostToDevice, c->st));
-        base = c->snaps;
-    }
+    const double *base = nullptr;
+    int rc0 = stage_snaps(c, snapshots, nsnap, &base);
+    if (rc0) return rc0;
     for (int m = 0; m < nsteps; m++) {
         double t = t0 + (m + 1) * c->cfg.dt;
         int s = (int)floor(t / snap_dt);
@@ -739,6 +847,84 @@ int ns_resim_advance(ns_ctx *c, const double *snapshots, int nsnap, double snap_
         if (rc) return rc;
     }
     if (hipGetLastError() != hipSuccess) return NS_ERR_HIP;
+    return NS_OK;
+}
+
+int ns_spline_weights(const double *t, int m, double x, double *w) {
+    if (!t || !w) return NS_ERR_INVALID;
+    return spline_weights(t, m, x, w);
+}
+
+int ns_bc_interp_spline(ns_ctx *c, const double *snapshots, int nsnap, double snap_dt, double t) {
+    if (!c || !snapshots || nsnap < 2 || !(snap_dt > 0)) return NS_ERR_INVALID;
+    const double *base = nullptr;
+    int rc = stage_snaps(c, snapshots, nsnap, &base);
+    if (rc) return rc;
+    rc = spline_planes(c, base, nsnap, snap_dt, t);
+    if (rc) return rc;
+    return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
+}
+
+int ns_resim_advance_spline(ns_ctx *c, const double *snapshots, int nsnap, double snap_dt, double t0, int nsteps) {
+    if (!c || !snapshots || nsnap < 2 || !(snap_dt > 0) || nsteps < 0) return NS_ERR_INVALID;
+    const double *base = nullptr;
+    int rc = stage_snaps(c, snapshots, nsnap, &base);
+    if (rc) return rc;
+    for (int m = 0; m < nsteps; m++) {
+        if ((rc = spline_planes(c, base, nsnap, snap_dt, t0 + (m + 1) * c->cfg.dt))) return rc;
+        if ((rc = one_step(c))) return rc;
+    }
+    return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
+}
+
+int ns_set_dt(ns_ctx *c, double dt) {
+    if (!c || !(dt > 0)) return NS_ERR_INVALID;
+    if (dt == c->cfg.dt) return NS_OK;
+    if (c->lazy && c->stale) {
+        // the pending projection belongs to the old dt: form u, v, w, p and
+        // carry u itself as the next predictor's input (dp = 0)
+        materialize(c);
+        const Grid &g = c->g;
+        HIPCHK(hipMemcpyAsync(c->us, c->u, g.nu() * 8, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->vs, c->v, g.nv() * 8, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->ws, c->w, g.nw() * 8, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemsetAsync(c->dp, 0, g.np() * 8, c->st));
+    }
+    c->cfg.dt = dt;
+    if (c->cfg.scheme == NS_AB2_CN)
+        for (int ax = 0; ax < 3; ax++)
+            for (int t = 0; t < 2; t++) {
+                int rc = make_line_sys(c, ax, t, c->cn[ax][t]);
+                if (rc) return rc;
+            }
+    // captured graphs hold the old dt
+    for (int i = 0; i < 2; i++)
+        if (c->graph[i]) {
+            (void)hipGraphExecDestroy(c->graph[i]);
+            c->graph[i] = nullptr;
+        }
+    return NS_OK;
+}
+
+int ns_history_save(ns_ctx *c) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    int rc;
+    if (!c->hsave[0] && ((rc = dalloc(c, &c->hsave[0], g.nu())) || (rc = dalloc(c, &c->hsave[1], g.nv())) ||
+                         (rc = dalloc(c, &c->hsave[2], g.nw()))))
+        return rc;
+    HIPCHK(hipMemcpyAsync(c->hsave[0], c->hx, g.nu() * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->hsave[1], c->hy, g.nv() * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->hsave[2], c->hz, g.nw() * 8, hipMemcpyDeviceToDevice, c->st));
+    return NS_OK;
+}
+
+int ns_history_restore(ns_ctx *c) {
+    if (!c || !c->hsave[0]) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    HIPCHK(hipMemcpyAsync(c->hx, c->hsave[0], g.nu() * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->hy, c->hsave[1], g.nv() * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->hz, c->hsave[2], g.nw() * 8, hipMemcpyDeviceToDevice, c->st));
     return NS_OK;
 }
 

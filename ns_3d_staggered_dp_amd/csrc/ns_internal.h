@@ -90,6 +90,9 @@ void launch_get_velpr_bc(const Grid &g, const int bc[3], const int pbc[3], doubl
 void launch_get_pr_bc(const Grid &g, const int pbc[3], double dx, double dy, double dz, const double *p,
                       double *planes, hipStream_t st);
 void launch_axpby(const double *a, const double *b, double th, double *out, long n, hipStream_t st);
+// out[q] = sum_{i<m} w[i] * a[i*stride + q]
+#define NS_WSUM_MAX 8
+void launch_wsum(const double *a, long stride, const double *w, int m, double *out, long n, hipStream_t st);
 void launch_copy_interior(const Grid &g, const double *sol, double *dp, hipStream_t st);
 void launch_sub_gauge(const double *src, const double *gauge, double *dst, size_t n, hipStream_t st);
 // block-wise partial maxima of |div| into partial[], returns number of partials

@@ -415,6 +415,22 @@ __global__ void __launch_bounds__(256) k_axpby(const double *__restrict__ a, con
     if (q < n) out[q] = (1.0 - th) * a[q] + th * b[q];
 }
 
+// out = sum_i w[i] * a[i*stride + q], i = 0..m-1 (m <= NS_WSUM_MAX), summed in index order
+struct WSumParams {
+    const double *a;
+    long stride, n;
+    int m;
+    double w[NS_WSUM_MAX];
+    double *out;
+};
+__global__ void __launch_bounds__(256) k_wsum(WSumParams P) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P.n) return;
+    double s = P.w[0] * P.a[q];
+    for (int i = 1; i < P.m; i++) s = s + P.w[i] * P.a[(long)i * P.stride + q];
+    P.out[q] = s;
+}
+
 __global__ void __launch_bounds__(256) k_copy_interior(Grid g, const double *__restrict__ sol, double *__restrict__ dp) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.ni()) return;
@@ -721,6 +737,17 @@ void launch_get_pr_bc(const Grid &g, const int pbc[3], double dx, double dy, dou
         int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
         hipLaunchKernelGGL(k_get_pr_bc_axis, dim3(nblocks((size_t)n[o1] * n[o2])), dim3(256), 0, st, P);
     }
+}
+
+void launch_wsum(const double *a, long stride, const double *w, int m, double *out, long n, hipStream_t st) {
+    WSumParams P;
+    P.a = a;
+    P.stride = stride;
+    P.n = n;
+    P.m = m;
+    for (int i = 0; i < NS_WSUM_MAX; i++) P.w[i] = i < m ? w[i] : 0.0;
+    P.out = out;
+    hipLaunchKernelGGL(k_wsum, dim3(nblocks(n)), dim3(256), 0, st, P);
 }
 
 void launch_axpby(const double *a, const double *b, double th, double *out, long n, hipStream_t st) {

@@ -131,6 +131,23 @@ class NSSolver:
     def resim_advance(self, snapshots, nsnap: int, snap_dt: float, t0: float, nsteps: int):
         _abi.call("ns_resim_advance", self.h, _ptr(snapshots), int(nsnap), float(snap_dt), float(t0), int(nsteps))
 
+    def resim_advance_spline(self, snapshots, nsnap: int, snap_dt: float, t0: float, nsteps: int):
+        _abi.call("ns_resim_advance_spline", self.h, _ptr(snapshots), int(nsnap), float(snap_dt), float(t0),
+                  int(nsteps))
+
+    def bc_interp_spline(self, snapshots, nsnap: int, snap_dt: float, t: float):
+        _abi.call("ns_bc_interp_spline", self.h, _ptr(snapshots), int(nsnap), float(snap_dt), float(t))
+
+    def set_dt(self, dt: float):
+        _abi.call("ns_set_dt", self.h, float(dt))
+        self.cfg = self.cfg.replace(dt=float(dt))
+
+    def history_save(self):
+        _abi.call("ns_history_save", self.h)
+
+    def history_restore(self):
+        _abi.call("ns_history_restore", self.h)
+
     def max_div(self) -> float:
         out = ctypes.c_double()
         _abi.call("ns_max_div", self.h, ctypes.byref(out))

@@ -18,3 +18,10 @@ def oracle():
 
     O.lib()
     return O
+
+
+@pytest.fixture(scope="session")
+def golden():
+    import numpy as np
+
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.npz"))

@@ -98,3 +98,102 @@ def test_device_time_interpolation_is_linear():
         s.resim_advance(np.ascontiguousarray(snaps.ravel()), 4, 0.25, 0.2, 1)
         th = (0.2 + 0.25 - 0.25) / 0.25
         assert np.abs(s.get_bc_planes() - ((1.0 - th) * snaps[1] + th * snaps[2])).max() < 1e-15
+
+
+def _smooth_history(L, nsnap, seed):
+    """plane sets varying smoothly in snapshot index (a few low temporal modes)"""
+    rng = np.random.default_rng(seed)
+    k = np.arange(nsnap)[:, None]
+    a, b, c = (0.2 * rng.standard_normal((1, L)) for _ in range(3))
+    return np.ascontiguousarray(a + b * np.sin(0.4 * k) + c * np.cos(0.23 * k + 0.5))
+
+
+def _spline_planes(snaps, snap_dt, t):
+    from scipy.interpolate import CubicSpline
+
+    from ns_3d_staggered_dp_amd.resim import spline_window
+
+    s0, m = spline_window(len(snaps), snap_dt, t)
+    tk = (s0 + np.arange(m)) * snap_dt
+    return CubicSpline(tk, snaps[s0:s0 + m], bc_type="not-a-knot", axis=0)(t)
+
+
+def test_bc_interp_spline_matches_scipy_window():
+    """device spline interpolation of plane sets == scipy's not-a-knot spline
+    on the reference's six-snapshot window (sub_tstep_BC_combined_interpolation.m:56-64)"""
+    cfg = NSConfig(nx=10, ny=8, nz=6, dx=0.3, dy=0.3, dz=0.3, nu=1e-3, dt=0.1, bc=(2, 2, 2), pbc=(3, 3, 3),
+                   dp_flag=1)
+    L = plane_set_length(10, 8, 6)
+    snaps = _smooth_history(L, 11, 21)
+    with NSSolver(cfg) as s:
+        for t in (0.0, 0.13, 0.47, 0.8, 1.0):
+            s.bc_interp_spline(snaps.ravel(), 11, 0.1, t)
+            ref = _spline_planes(snaps, 0.1, t)
+            assert np.abs(s.get_bc_planes() - ref).max() <= 1e-13 * np.abs(ref).max(), t
+
+
+@pytest.mark.parametrize("scheme,interp", [(AB2, "spline"), (AB2_CN, "spline"), (AB2, "linear")])
+def test_sub_tstep_run_matches_oracle(oracle, scheme, interp):
+    """re_simulation_sub_tstep_interBC's schedule: sub-steps of dt0/4 with
+    interpolated boundary planes, AB2 history saved after the first sub-step
+    and restored after the last, then full steps on the snapshot planes"""
+    from ns_3d_staggered_dp_amd import resim
+
+    dt0, sub, nfull, nsnap = 4e-3, 4, 3, 12
+    cfg = NSConfig(nx=16, ny=12, nz=10, dx=0.31, dy=0.27, dz=0.23, nu=0.01, dt=dt0, scheme=scheme, bc=(2, 2, 2),
+                   pbc=(3, 3, 3), dp_flag=1)
+    L = plane_set_length(16, 12, 10)
+    snaps = _smooth_history(L, nsnap, 22)
+    u, v, w, p = fields.random_ghosted(cfg, 23)
+    u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
+    hx, hy, hz = (0.1 * x for x in fields.random_ghosted(cfg, 24)[:3])
+    t_start = 2 * dt0
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, hx, hy, hz, first=False)
+        resim.run_sub_tstep(s, snaps.ravel(), nsnap, dt0, t_start, dt0, sub, nfull, interp=interp)
+        got = s.download()
+    st = oracle.OracleState(cfg.replace(dt=dt0 / sub), u, v, w, p, hx, hy, hz, first=False)
+    for i in range(1, sub + 1):
+        t = t_start + dt0 / sub * i
+        if interp == "spline":
+            planes = _spline_planes(snaps, dt0, t)
+        else:
+            k = int(np.floor(t / dt0))
+            th = (t - k * dt0) / dt0
+            planes = (1.0 - th) * snaps[k] + th * snaps[k + 1]
+        st.step(np.ascontiguousarray(planes))
+        if i == 1:
+            saved = (st.hx.copy(order="F"), st.hy.copy(order="F"), st.hz.copy(order="F"))
+        if i == sub:
+            st.hx, st.hy, st.hz = (x.copy(order="F") for x in saved)
+    st.cfg = cfg
+    for m in range(nfull):
+        t = t_start + dt0 + (m + 1) * dt0
+        st.step(np.ascontiguousarray(snaps[int(round(t / dt0))]))
+    for name in ("u", "v", "w", "p"):
+        ref = getattr(st, name)
+        assert np.abs(got[name] - ref).max() <= 1e-11 * np.abs(ref).max(), name
+
+
+def test_set_dt_in_lazy_periodic_run(oracle):
+    """dt change between steps of a lazily projected run: the pending
+    projection keeps the old dt, and replayed graphs are rebuilt"""
+    cfg = NSConfig(nx=32, ny=16, nz=12, dx=2 * np.pi / 32, dy=0.4, dz=0.5, nu=2e-3, dt=2e-3, scheme=AB2,
+                   flags=FLAG_GAUGE_CORNER)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=25)
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, first=True)
+        s.step(3)
+        s.set_dt(1e-3)
+        s.step(4)
+        s.set_dt(2e-3)
+        s.step(2)
+        got = s.download()
+    st = oracle.OracleState(cfg, u, v, w, p, first=True)
+    for dt, n in ((2e-3, 3), (1e-3, 4), (2e-3, 2)):
+        st.cfg = cfg.replace(dt=dt)
+        for _ in range(n):
+            st.step()
+    for name in ("u", "v", "w", "p"):
+        ref = getattr(st, name)
+        assert np.abs(got[name] - ref).max() <= 1e-11 * np.abs(ref).max(), name
