@@ -72,6 +72,30 @@ module ns_hip_mod
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: out
         end function
+        integer(c_int) function c_ns_set_dt(ctx, dt) bind(C, name='ns_set_dt')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: dt
+        end function
+        integer(c_int) function c_ns_history_save(ctx) bind(C, name='ns_history_save')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function c_ns_history_restore(ctx) bind(C, name='ns_history_restore')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function c_ns_bc_interp_spline(ctx, snaps, nsnap, snap_dt, t) bind(C, name='ns_bc_interp_spline')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, snaps
+            integer(c_int), value :: nsnap
+            real(c_double), value :: snap_dt, t
+        end function
+        integer(c_int) function c_ns_error_metrics(ctx, u, v, w, p, err, rms) bind(C, name='ns_error_metrics')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, u, v, w, p
+            real(c_double), intent(out) :: err(8), rms(4)
+        end function
         integer(c_int) function c_ns_cal_conv(ctx, u, v, w, bx, by, bz, dx, dy, dz, cx, cy, cz) bind(C, name='ns_cal_conv')
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx, u, v, w, cx, cy, cz
@@ -145,7 +169,8 @@ module ns_hip_mod
     real(c_double), allocatable, target, save :: pset(:)
 
     public :: ns_hip_init, ns_hip_finalize, ns_hip_upload, ns_hip_download, ns_hip_download_work, ns_hip_step
-    public :: ns_hip_set_planes, ns_hip_max_div
+    public :: ns_hip_set_planes, ns_hip_max_div, ns_hip_set_dt, ns_hip_history_save, ns_hip_history_restore
+    public :: ns_hip_bc_interp_spline, ns_hip_error_metrics
     public :: cal_conv, cal_diff, vel_bc_staggered, vel_bc_staggered_CN2, get_velpr_bc, get_pr_bc
     public :: pr_bc_staggered_modify_rhs, pr_bc_staggered, DCT_poisson_solver, poisson_fft, divergence, project
     public :: adi_sweep
@@ -234,6 +259,36 @@ contains
         call check(c_ns_max_div(ctx, m), "ns_max_div")
         ns_hip_max_div = m
     end function ns_hip_max_div
+
+    ! sub-step phase of re_simulation_sub_tstep_interBC_combine.f90:446-448,602-612
+    subroutine ns_hip_set_dt(dt)
+        real(8), intent(in) :: dt
+        call check(c_ns_set_dt(ctx, dt), "ns_set_dt")
+    end subroutine ns_hip_set_dt
+
+    subroutine ns_hip_history_save()
+        call check(c_ns_history_save(ctx), "ns_history_save")
+    end subroutine ns_hip_history_save
+
+    subroutine ns_hip_history_restore()
+        call check(c_ns_history_restore(ctx), "ns_history_restore")
+    end subroutine ns_hip_history_restore
+
+    ! plane set := interp1(..., 'spline') of the snapshot history planes(:, 1:nsnap) at time t
+    subroutine ns_hip_bc_interp_spline(snaps, nsnap, snap_dt, t)
+        real(8), intent(in), target :: snaps(:,:)
+        integer, intent(in) :: nsnap
+        real(8), intent(in) :: snap_dt, t
+        call check(c_ns_bc_interp_spline(ctx, c_loc(snaps), int(nsnap, c_int), snap_dt, t), "ns_bc_interp_spline")
+    end subroutine ns_hip_bc_interp_spline
+
+    ! err_vel(1:8, t_step) and rms_vel(1:4, t_step) of …interBC_combine.f90:826-833
+    subroutine ns_hip_error_metrics(u_sub, v_sub, w_sub, p_sub, err_vel, rms_vel)
+        real(8), intent(in), target :: u_sub(:,:,:), v_sub(:,:,:), w_sub(:,:,:), p_sub(:,:,:)
+        real(8), intent(out) :: err_vel(8), rms_vel(4)
+        call check(c_ns_error_metrics(ctx, c_loc(u_sub), c_loc(v_sub), c_loc(w_sub), c_loc(p_sub), err_vel, rms_vel), &
+            "ns_error_metrics")
+    end subroutine ns_hip_error_metrics
 
     ! ---- mirrors with the reference's argument lists ----
 
