@@ -149,6 +149,22 @@ typedef struct ns_subbox {
     int nx, ny, nz;
 } ns_subbox;
 int ns_capture_begin(ns_ctx *ctx, const ns_subbox *sb, double *dst, int max_sets, int every);
+/* Sub-box of one resident field, ghosts included, as the parent's idx_*
+ * vectors select it (big_periodic_3D.f90:21-25,652-657): face axes x0..x0+n,
+ * centred axes x0..x0+n+1.  dst (host or device) gets the Fortran-ordered
+ * box, e.g. u_sub(n+1,ny+2,nz+2).  dp is gauged when NS_FLAG_GAUGE_CORNER. */
+#define NS_FIELD_U 0
+#define NS_FIELD_V 1
+#define NS_FIELD_W 2
+#define NS_FIELD_P 3
+#define NS_FIELD_USTAR 4
+#define NS_FIELD_VSTAR 5
+#define NS_FIELD_WSTAR 6
+#define NS_FIELD_DP 7
+#define NS_FIELD_HX 8
+#define NS_FIELD_HY 9
+#define NS_FIELD_HZ 10
+int ns_extract_subbox(ns_ctx *ctx, const ns_subbox *sb, int field, double *dst);
 int ns_capture_end(ns_ctx *ctx, int *nsets_written);
 /* max |div u| over interior cells (big_periodic_3D.f90:576-579) */
 int ns_max_div(ns_ctx *ctx, double *out);

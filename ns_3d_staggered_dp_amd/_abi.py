@@ -37,6 +37,15 @@ P = ctypes.c_void_p
 D = ctypes.c_double
 I = ctypes.c_int
 
+
+class SubBox(ctypes.Structure):
+    """ns_subbox: 1-based origin and interior size of a re-simulation box"""
+    _fields_ = [("x0", ctypes.c_int), ("y0", ctypes.c_int), ("z0", ctypes.c_int),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)]
+
+
+FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "u_star": 4, "v_star": 5, "w_star": 6, "dp": 7, "hx": 8, "hy": 9, "hz": 10}
+
 # name -> argtypes (restype int unless listed in _RESTYPE)
 SIGNATURES = {
     "ns_create": [P, ctypes.POINTER(P)],
@@ -62,6 +71,7 @@ SIGNATURES = {
     "ns_bc_interp_spline": [P, P, I, D, D],
     "ns_spline_weights": [P, I, D, P],
     "ns_set_dt": [P, D],
+    "ns_extract_subbox": [P, P, I, P],
     "ns_history_save": [P],
     "ns_history_restore": [P],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],

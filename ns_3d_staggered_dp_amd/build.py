@@ -46,6 +46,31 @@ def _compile(src: str) -> str:
     return obj
 
 
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+LIB_H5 = os.path.join(PKG, "libns_h5.so")
+
+
+def build_h5(verbose: bool = False) -> str | None:
+    """libns_h5.so: the reference's HDF5 files from C (host only, g++ against
+    the HDF5 C + high-level libraries).  Skipped when HDF5 is not installed."""
+    inc = os.path.join(HDF5_DIR, "include")
+    libdir = os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5_hl.h")):
+        if verbose:
+            print(f"HDF5 not found under {HDF5_DIR}: libns_h5.so not built")
+        return None
+    tmp = LIB_H5 + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"), "-I", inc,
+           os.path.join(CSRC, "ns_h5.cpp"), "-L", libdir, "-lhdf5_hl", "-lhdf5", f"-Wl,-rpath,{libdir}", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libns_h5 build failed:\n{r.stderr[-4000:]}")
+    shutil.move(tmp, LIB_H5)
+    if verbose:
+        print(f"built {LIB_H5}")
+    return LIB_H5
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = SOURCES + EXTRA_SOURCES
@@ -59,6 +84,7 @@ def build(verbose: bool = False) -> str:
     shutil.move(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_h5(verbose)
     return LIB
 
 

@@ -928,6 +928,33 @@ int ns_history_restore(ns_ctx *c) {
     return NS_OK;
 }
 
+int ns_extract_subbox(ns_ctx *c, const ns_subbox *sb, int field, double *dst) {
+    if (!c || !sb || !dst || field < NS_FIELD_U || field > NS_FIELD_HZ) return NS_ERR_INVALID;
+    const ns_config &f = c->cfg;
+    // component (0 u, 1 v, 2 w, 3 p) and array of the field
+    const int comp_of[11] = {0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2};
+    const int comp = comp_of[field];
+    if (field <= NS_FIELD_P) materialize(c);
+    double *src[11] = {c->u, c->v, c->w, c->p, c->us, c->vs, c->ws, c->dp, c->hx, c->hy, c->hz};
+    int n[3];
+    comp_dims(c->g, comp, n);
+    const int nb[3] = {sb->nx, sb->ny, sb->nz};
+    int e[3];
+    for (int a = 0; a < 3; a++) e[a] = nb[a] + ((comp == a) ? 1 : 2);
+    const int o[3] = {sb->x0 - 1, sb->y0 - 1, sb->z0 - 1};
+    for (int a = 0; a < 3; a++)
+        if (nb[a] < 1 || o[a] < 0 || o[a] + e[a] > n[a]) return NS_ERR_INVALID;
+    const double *gauge = (field == NS_FIELD_DP && (f.flags & NS_FLAG_GAUGE_CORNER)) ? c->dp : nullptr;
+    ArgList args;
+    Arg &D = args.add();
+    int rc = arg_in(c, D, dst, (size_t)e[0] * e[1] * e[2], false);
+    if (rc) return rc;
+    launch_extract_box(src[field], n, o, e, gauge, D.dev, c->st);
+    if ((rc = arg_out(c, D))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
 int ns_capture_begin(ns_ctx *c, const ns_subbox *sb, double *dst, int max_sets, int every) {
     if (!c || !sb || !dst || max_sets < 1 || every < 1) return NS_ERR_INVALID;
     const Grid &g = c->g;

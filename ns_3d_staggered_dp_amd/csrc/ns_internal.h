@@ -95,6 +95,9 @@ void launch_axpby(const double *a, const double *b, double th, double *out, long
 void launch_wsum(const double *a, long stride, const double *w, int m, double *out, long n, hipStream_t st);
 void launch_copy_interior(const Grid &g, const double *sol, double *dp, hipStream_t st);
 void launch_sub_gauge(const double *src, const double *gauge, double *dst, size_t n, hipStream_t st);
+// box of extent e at 0-based origin o of a ghosted array with leading dims n[0], n[1]
+void launch_extract_box(const double *src, const int n[3], const int o[3], const int e[3], const double *gauge,
+                        double *dst, hipStream_t st);
 // block-wise partial maxima of |div| into partial[], returns number of partials
 int launch_max_div(const Grid &g, double dx, double dy, double dz, const double *u, const double *v, const double *w,
                    double *partial, hipStream_t st);

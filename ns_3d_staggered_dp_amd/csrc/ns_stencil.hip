@@ -446,6 +446,22 @@ __global__ void __launch_bounds__(256) k_sub_gauge(const double *__restrict__ sr
     if (q < n) dst[q] = src[q] - *gauge;
 }
 
+// dst(e0,e1,e2) = src(o0+1.., o1+1.., o2+1..) [- *gauge]: a box of a ghosted
+// (n0,n1,*) array, 0-based origin o
+__global__ void __launch_bounds__(256) k_extract_box(const double *__restrict__ src, int n0, int n1, int o0, int o1,
+                                                     int o2, int e0, int e1, int e2, const double *gauge,
+                                                     double *__restrict__ dst) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t tot = (size_t)e0 * e1 * e2;
+    if (q >= tot) return;
+    const int i = (int)(q % e0);
+    const size_t r = q / e0;
+    const int j = (int)(r % e1), k = (int)(r / e1);
+    double x = src[FIX(o0 + i + 1, o1 + j + 1, o2 + k + 1, n0, n1)];
+    if (gauge) x = x - *gauge;
+    dst[q] = x;
+}
+
 __global__ void __launch_bounds__(256) k_max_div(Grid g, double dx, double dy, double dz, const double *__restrict__ u,
                                                  const double *__restrict__ v, const double *__restrict__ w,
                                                  double *partial) {
@@ -756,6 +772,13 @@ void launch_axpby(const double *a, const double *b, double th, double *out, long
 
 void launch_copy_interior(const Grid &g, const double *sol, double *dp, hipStream_t st) {
     hipLaunchKernelGGL(k_copy_interior, dim3(nblocks(g.ni())), dim3(256), 0, st, g, sol, dp);
+}
+
+void launch_extract_box(const double *src, const int n[3], const int o[3], const int e[3], const double *gauge,
+                        double *dst, hipStream_t st) {
+    const size_t tot = (size_t)e[0] * e[1] * e[2];
+    hipLaunchKernelGGL(k_extract_box, dim3(nblocks(tot)), dim3(256), 0, st, src, n[0], n[1], o[0], o[1], o[2], e[0],
+                       e[1], e[2], gauge, dst);
 }
 
 void launch_sub_gauge(const double *src, const double *gauge, double *dst, size_t n, hipStream_t st) {

@@ -148,6 +148,16 @@ class NSSolver:
     def history_restore(self):
         _abi.call("ns_history_restore", self.h)
 
+    def extract_subbox(self, x0: int, y0: int, z0: int, nx0: int, ny0: int, nz0: int, field: str) -> np.ndarray:
+        """sub-box of a resident field (ghosts included), e.g. u_sub of the parent"""
+        comp = {"u": "u", "v": "v", "w": "w", "p": "p", "u_star": "u", "v_star": "v", "w_star": "w", "dp": "p",
+                "hx": "u", "hy": "v", "hz": "w"}[field]
+        shape = field_shapes(nx0, ny0, nz0)[comp]
+        out = fzeros(shape)
+        sb = _abi.SubBox(x0, y0, z0, nx0, ny0, nz0)
+        _abi.call("ns_extract_subbox", self.h, ctypes.byref(sb), _abi.FIELDS[field], _ptr(out))
+        return out
+
     def max_div(self) -> float:
         out = ctypes.c_double()
         _abi.call("ns_max_div", self.h, ctypes.byref(out))

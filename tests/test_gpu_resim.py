@@ -197,3 +197,36 @@ def test_set_dt_in_lazy_periodic_run(oracle):
     for name in ("u", "v", "w", "p"):
         ref = getattr(st, name)
         assert np.abs(got[name] - ref).max() <= 1e-11 * np.abs(ref).max(), name
+
+
+def test_extract_subbox_and_sub_file(tmp_path):
+    """device sub-box extraction == the idx_* slices of the downloaded fields
+    (bitwise, dp gauged), written as a reference sub-box group and read back"""
+    from ns_3d_staggered_dp_amd import h5io
+
+    (x0, y0, z0), (a, b, c) = BOX
+    cfg = NSConfig.cube(20, nu=2e-3, dt=4e-3, scheme=AB2, dp_flag=0, flags=FLAG_GAUGE_CORNER)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=4)
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, first=True)
+        s.step(3)
+        st = s.download()
+        wk = s.download_work()
+        src = {"u": st["u"], "v": st["v"], "w": st["w"], "p": st["p"], "u_star": wk["u_star"],
+               "v_star": wk["v_star"], "w_star": wk["w_star"], "dp": wk["dp"], "hx": st["hx"], "hy": st["hy"],
+               "hz": st["hz"]}
+        comp = {"u": "u", "v": "v", "w": "w", "p": "p", "u_star": "u", "v_star": "v", "w_star": "w", "dp": "p",
+                "hx": "u", "hy": "v", "hz": "w"}
+        boxes = {}
+        for k, arr in src.items():
+            got = s.extract_subbox(x0, y0, z0, a, b, c, k)
+            assert np.array_equal(got, subbox(arr, comp[k], x0, y0, z0, a, b, c)), k
+            boxes[k] = got
+    path = str(tmp_path / h5io.sub_file_name("AB2", 20, 4e-3, 0, x0, a))
+    with h5io.H5File(path, "x") as f:
+        h5io.write_run_attrs(f, 20, 4e-3, 2e-3, "AB2", 3, 0.012, x0=x0, nx0=a)
+        h5io.write_sub_group(f, 0.012, {n + "_sub": boxes[n] for n in ("u", "v", "w", "p", "u_star", "v_star",
+                                                                        "w_star", "dp")})
+    with h5io.H5File(path) as f:
+        back = h5io.read_sub_group(f, 0.012, a, b, c)
+    assert all(np.array_equal(back[n + "_sub"], boxes[n]) for n in ("u", "w_star", "dp"))
