@@ -50,3 +50,24 @@ def test_invalid_config_rejected_without_gpu_work():
     c.scheme = 1
     c.bc[0] = 3  # velocity bc 3 has no fill in the reference (NS_functions.f90:249)
     assert lib.ns_create(ctypes.byref(c), ctypes.byref(h)) == _abi.NS_ERR_INVALID
+
+
+def test_h5_library_exports_every_declared_symbol():
+    from ns_3d_staggered_dp_amd import h5io
+
+    if not os.path.exists(h5io.LIB_PATH):
+        import pytest
+
+        pytest.skip("HDF5 not installed: libns_h5.so not built")
+    lib = ctypes.CDLL(h5io.LIB_PATH)
+    names = _abi.declared_symbols(os.path.join(ROOT, "include", "ns_h5.h"))
+    assert len(names) >= 14
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    h5io.lib()  # every declared symbol gets its ctypes signature
+
+
+def test_spline_weights_host_entry_needs_no_gpu():
+    from ns_3d_staggered_dp_amd.resim import spline_weights
+
+    assert abs(spline_weights([0.0, 1.0, 2.0, 3.0], 1.5).sum() - 1.0) < 1e-15
