@@ -130,4 +130,4 @@ def declared_symbols(header: str) -> list[str]:
     import re
 
     txt = open(header).read()
-    return sorted(set(re.findall(r"^\s*(?:int|long|void)\s+\**\s*(ns_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long|void)\s+\**\s*(nsh?5?_\w+)\s*\(", txt, flags=re.M)))
