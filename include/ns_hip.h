@@ -49,6 +49,10 @@ extern "C" {
  * steps (not the first, no plane capture active) are replayed two at a time
  * from a HIP graph captured once per context; the results are identical. */
 #define NS_FLAG_NO_GRAPH 4
+/* Project every step (u, v, w, p formed each step) even with dp_flag = 0,
+ * instead of folding the projection into the next predictor; needed by the
+ * discrete adjoint. */
+#define NS_FLAG_EAGER_PROJECTION 8
 
 /* Run configuration: the driver parameter block (big_periodic_3D.f90:9-21,
  * 104-108; re_simulation_sub_tstep_interBC_combine.f90:100-112).
@@ -165,6 +169,24 @@ int ns_capture_begin(ns_ctx *ctx, const ns_subbox *sb, double *dst, int max_sets
 #define NS_FIELD_HY 9
 #define NS_FIELD_HZ 10
 int ns_extract_subbox(ns_ctx *ctx, const ns_subbox *sb, int field, double *dst);
+
+/* ---- discrete adjoint of ns_step (no counterpart in the reference: the
+ * gradient side of the re-simulation) ----
+ * Explicit schemes (Euler, AB2) on contexts that project every step (dp_flag
+ * = 1, or NS_FLAG_EAGER_PROJECTION); NS_ERR_UNSUPPORTED otherwise.  The
+ * adjoint state is (u_b, v_b, w_b, p_b, hx_b, hy_b, hz_b), the gradients of a
+ * scalar objective with respect to the state (u, v, w, p, rhs_*_previous)
+ * at the current point of the reverse sweep, all in the ghosted shapes. */
+int ns_adjoint_set_state(ns_ctx *ctx, const double *u_b, const double *v_b, const double *w_b, const double *p_b,
+                         const double *hx_b, const double *hy_b, const double *hz_b); /* null = zero */
+int ns_adjoint_get_state(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, double *p_b, double *hx_b,
+                         double *hy_b, double *hz_b);                                  /* null = skip */
+/* One reverse step: the adjoint state of the end of a forward step becomes
+ * that of its start.  u, v, w: the forward velocity at the start of that step
+ * (host or device); first: the step was the Euler first step; planes_b
+ * (host or device, plane-set length, may be null) receives the gradient with
+ * respect to that step's plane set (mean_dp included). */
+int ns_adjoint_step(ns_ctx *ctx, const double *u, const double *v, const double *w, int first, double *planes_b);
 int ns_capture_end(ns_ctx *ctx, int *nsets_written);
 /* max |div u| over interior cells (big_periodic_3D.f90:576-579) */
 int ns_max_div(ns_ctx *ctx, double *out);

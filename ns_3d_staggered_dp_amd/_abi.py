@@ -72,6 +72,9 @@ SIGNATURES = {
     "ns_spline_weights": [P, I, D, P],
     "ns_set_dt": [P, D],
     "ns_extract_subbox": [P, P, I, P],
+    "ns_adjoint_set_state": [P, P, P, P, P, P, P, P],
+    "ns_adjoint_get_state": [P, P, P, P, P, P, P, P],
+    "ns_adjoint_step": [P, P, P, P, I, P],
     "ns_history_save": [P],
     "ns_history_restore": [P],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],

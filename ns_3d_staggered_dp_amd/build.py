@@ -17,7 +17,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libns_hip.so")
-SOURCES = ["ns_stencil.hip", "ns_predict.hip", "ns_poisson.hip", "ns_adi.hip", "ns_metrics.hip", "ns_abi.hip"]
+SOURCES = ["ns_stencil.hip", "ns_predict.hip", "ns_poisson.hip", "ns_adi.hip", "ns_metrics.hip", "ns_adjoint.hip",
+           "ns_abi.hip"]
 EXTRA_SOURCES: list[str] = []
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

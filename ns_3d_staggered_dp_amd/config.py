@@ -22,6 +22,8 @@ FLAG_GAUGE_CORNER = 1
 FLAG_CN_CYCLIC = 2
 # every step as individual launches (no HIP graph replay of steady steps)
 FLAG_NO_GRAPH = 4
+# project every step even with dp_flag = 0 (needed by the discrete adjoint)
+FLAG_EAGER_PROJECTION = 8
 
 PI = 3.1415926535897932  # big_periodic_3D.f90:6
 

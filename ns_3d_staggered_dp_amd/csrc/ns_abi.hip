@@ -50,6 +50,10 @@ struct ns_ctx {
     int graph_off = 0;          // capture failed or disabled
     double *mpartial = nullptr; // error-metric block partials
     double *hsave[3] = {nullptr, nullptr, nullptr};  // saved AB2 history (sub-step phase)
+    // discrete adjoint: adjoint state (u, v, w, p, H) and work arrays
+    double *ab[3] = {nullptr, nullptr, nullptr}, *apb = nullptr, *ahb[3] = {nullptr, nullptr, nullptr};
+    double *ausb[3] = {nullptr, nullptr, nullptr}, *adpb = nullptr, *agb[3] = {nullptr, nullptr, nullptr};
+    double *ar = nullptr, *arsol = nullptr, *apbar = nullptr;
     std::vector<void *> allocs;
 };
 
@@ -520,6 +524,7 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
     c->lazy = (cfg->dp_flag == 0);
     for (int a = 0; a < 3; a++)
         if (cfg->bc[a] == 1 && cfg->pbc[a] != 1) c->lazy = 0;
+    if (cfg->flags & NS_FLAG_EAGER_PROJECTION) c->lazy = 0;
     HIPCHK(hipMemsetAsync(c->planes, 0, (size_t)c->L.total * 8, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     *out = c;
@@ -926,6 +931,117 @@ int ns_history_restore(ns_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->hy, c->hsave[1], g.nv() * 8, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->hz, c->hsave[2], g.nw() * 8, hipMemcpyDeviceToDevice, c->st));
     return NS_OK;
+}
+
+static int adj_alloc(ns_ctx *c) {
+    if (c->ab[0]) return NS_OK;
+    const Grid &g = c->g;
+    const size_t n[3] = {g.nu(), g.nv(), g.nw()};
+    int rc;
+    for (int a = 0; a < 3; a++)
+        if ((rc = dalloc(c, &c->ab[a], n[a])) || (rc = dalloc(c, &c->ahb[a], n[a])) ||
+            (rc = dalloc(c, &c->ausb[a], n[a])) || (rc = dalloc(c, &c->agb[a], n[a])))
+            return rc;
+    if ((rc = dalloc(c, &c->apb, g.np())) || (rc = dalloc(c, &c->adpb, g.np())) || (rc = dalloc(c, &c->ar, g.ni())) ||
+        (rc = dalloc(c, &c->arsol, g.ni())) || (rc = dalloc(c, &c->apbar, (size_t)c->L.total)))
+        return rc;
+    if (!c->mpartial && (rc = dalloc(c, &c->mpartial, (size_t)2048 * NS_METRIC_PARTS))) return rc;
+    return NS_OK;
+}
+
+// sum of a device array (block partials on the device, combined on the host)
+static int dev_sum(ns_ctx *c, const double *a, size_t n, double *out) {
+    const int nb = launch_block_sums(a, n, c->mpartial, 2048, c->st);
+    std::vector<double> h(nb);
+    HIPCHK(hipMemcpyAsync(h.data(), c->mpartial, nb * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    long double s = 0;
+    for (double x : h) s += x;
+    *out = (double)s;
+    return NS_OK;
+}
+
+static bool adj_supported(const ns_ctx *c) {
+    return !c->lazy && c->cfg.scheme != NS_AB2_CN && c->poisson_ok;
+}
+
+int ns_adjoint_set_state(ns_ctx *c, const double *ub, const double *vb, const double *wb, const double *pb,
+                         const double *hxb, const double *hyb, const double *hzb) {
+    if (!c) return NS_ERR_INVALID;
+    if (!adj_supported(c)) return NS_ERR_UNSUPPORTED;
+    int rc = adj_alloc(c);
+    if (rc) return rc;
+    const Grid &g = c->g;
+    const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
+    const double *src[7] = {ub, vb, wb, pb, hxb, hyb, hzb};
+    double *dst[7] = {c->ab[0], c->ab[1], c->ab[2], c->apb, c->ahb[0], c->ahb[1], c->ahb[2]};
+    for (int i = 0; i < 7; i++) {
+        if (src[i]) HIPCHK(hipMemcpyAsync(dst[i], src[i], n[i] * 8, hipMemcpyDefault, c->st));
+        else HIPCHK(hipMemsetAsync(dst[i], 0, n[i] * 8, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_adjoint_get_state(ns_ctx *c, double *ub, double *vb, double *wb, double *pb, double *hxb, double *hyb,
+                         double *hzb) {
+    if (!c) return NS_ERR_INVALID;
+    if (!c->ab[0]) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
+    double *dst[7] = {ub, vb, wb, pb, hxb, hyb, hzb};
+    const double *src[7] = {c->ab[0], c->ab[1], c->ab[2], c->apb, c->ahb[0], c->ahb[1], c->ahb[2]};
+    for (int i = 0; i < 7; i++)
+        if (dst[i]) HIPCHK(hipMemcpyAsync(dst[i], src[i], n[i] * 8, hipMemcpyDefault, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_adjoint_step(ns_ctx *c, const double *u, const double *v, const double *w, int first, double *planes_bar) {
+    if (!c || !u || !v || !w) return NS_ERR_INVALID;
+    if (!adj_supported(c)) return NS_ERR_UNSUPPORTED;
+    if (!c->ab[0]) return NS_ERR_INVALID;  // ns_adjoint_set_state first
+    const ns_config &f = c->cfg;
+    const Grid &g = c->g;
+    const double h[3] = {f.dx, f.dy, f.dz};
+    int per[3];
+    for (int a = 0; a < 3; a++) per[a] = (f.bc[a] == 1);
+    ArgList args;
+    Arg &AU = args.add();
+    Arg &AV = args.add();
+    Arg &AW = args.add();
+    int rc;
+    if ((rc = arg_in(c, AU, u, g.nu(), true)) || (rc = arg_in(c, AV, v, g.nv(), true)) ||
+        (rc = arg_in(c, AW, w, g.nw(), true)))
+        return rc;
+    HIPCHK(hipMemsetAsync(c->apbar, 0, (size_t)c->L.total * 8, c->st));
+    // S5^T: projection (and the corner gauge: p = dp_flag p + dp - dp(1,1,1))
+    double psum = 0.0;
+    if (f.flags & NS_FLAG_GAUGE_CORNER)
+        if ((rc = dev_sum(c, c->apb, g.np(), &psum))) return rc;
+    launch_adj_project(g, f.dt, h, f.dp_flag, c->ab, c->ausb, c->apb, c->adpb, c->st);
+    if (f.flags & NS_FLAG_GAUGE_CORNER) launch_add_at(c->adpb, 0, -psum, c->st);
+    // S4^T: pressure ghost fill
+    launch_adj_fill(g, 3, f.pbc, h, c->adpb, c->apbar, c->st);
+    // S3^T: zero mode (mean_dp of the DCT solve), symmetric solve, boundary terms, divergence
+    launch_adj_take_interior(g, c->adpb, c->ar, c->st);
+    // mean_dp sets the zero mode, which exists only when every axis is Neumann
+    if (c->pp.dct && f.pbc[0] == 3 && f.pbc[1] == 3 && f.pbc[2] == 3) {
+        double msum = 0.0;
+        if ((rc = dev_sum(c, c->ar, g.ni(), &msum))) return rc;
+        launch_add_at(c->apbar, (size_t)c->L.off[24], msum, c->st);
+    }
+    if ((rc = run_poisson(c, nullptr, nullptr, nullptr, c->ar, nullptr, nullptr, 0.0, nullptr, c->arsol))) return rc;
+    launch_adj_rhs_div(g, f.pbc, h, f.dt, c->arsol, c->ausb, c->apbar, c->st);
+    // S2^T: velocity ghost fill
+    for (int cc = 0; cc < 3; cc++) launch_adj_fill(g, cc, f.bc, h, c->ausb[cc], c->apbar, c->st);
+    // S1^T: predictor
+    const double *uu[3] = {AU.dev, AV.dev, AW.dev};
+    launch_adj_predict(g, per, f.dt, f.nu, f.dp_flag, h, f.scheme, first, uu, c->ausb, c->ahb, c->agb, c->ab, c->apb,
+                       c->st);
+    if (planes_bar) HIPCHK(hipMemcpyAsync(planes_bar, c->apbar, (size_t)c->L.total * 8, hipMemcpyDefault, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
 }
 
 int ns_extract_subbox(ns_ctx *c, const ns_subbox *sb, int field, double *dst) {

@@ -1,0 +1,498 @@
+// Discrete adjoint of one explicit time step (Euler / AB2 predictor, velocity
+// fill, divergence + boundary RHS terms, Poisson solve, pressure fill,
+// projection), FP64, for gradients of a function of the final state with
+// respect to the initial state and to every step's boundary planes.
+//
+// Forward step (as ns_step runs it without lazy projection):
+//   S1 u* = u + dt(c1 rhs(u) - c2 H - dp_flag grad p),  H' = rhs(u) on the
+//      computed faces, rhs = nu diff(u) - conv(u)       (big_periodic_3D.f90:350-396)
+//   S2 ghost fill of u* from the planes, x then y then z   (NS_functions.f90:213-310)
+//   S3 r = div(u*)/dt + boundary terms; dp = P r + mean_dp  (…interBC:771, NS_functions.f90:1345-1400)
+//   S4 ghost fill of dp, x then y then z                  (NS_functions.f90:1403-1464)
+//   S5 u = u* - dt grad dp (all faces), p = dp_flag p + dp (big_periodic_3D.f90:564-567)
+// The adjoint runs S5..S1 transposed.  P is symmetric (an orthogonal
+// transform, a diagonal, its inverse) apart from the zero mode, which is set
+// to mean_dp: the adjoint solve is the same solve with the zero mode dropped
+// and d mean_dp = sum of the adjoint dp.  conv(u) is written through its cell
+// and edge fluxes (u^2 at x-cells, u_avg*v_avg at x-y edges, ...): each flux
+// adjoint is formed from the output adjoints around it and spread back to the
+// faces it averages, so every adjoint value is gathered (no atomics, the
+// result does not depend on thread order).
+#include "ns_internal.h"
+
+namespace ns {
+
+namespace {
+
+constexpr int AT = 256;
+
+inline unsigned int ablocks(size_t n) { return (unsigned int)((n + AT - 1) / AT); }
+
+// 1-based read of a ghosted array, 0 outside it
+__device__ __forceinline__ double rd(const double *a, const int n[3], int i, int j, int k) {
+    if (i < 1 || i > n[0] || j < 1 || j > n[1] || k < 1 || k > n[2]) return 0.0;
+    return a[FIX(i, j, k, n[0], n[1])];
+}
+
+__device__ __forceinline__ void dims_of(const Grid &g, int c, int n[3]) {
+    n[0] = g.nx + 2; n[1] = g.ny + 2; n[2] = g.nz + 2;
+    if (c < 3) n[c] -= 1;
+}
+
+__device__ __forceinline__ void unflat(size_t q, const int n[3], int &i, int &j, int &k) {
+    i = (int)(q % n[0]) + 1;
+    const size_t r = q / n[0];
+    j = (int)(r % n[1]) + 1;
+    k = (int)(r / n[1]) + 1;
+}
+
+// computed faces of component c (the reference's x_range0/x_range1 etc.)
+__device__ __forceinline__ bool computed(const Grid &g, const int per[3], int c, int i, int j, int k) {
+    const int id[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; a++) {
+        const int lo = (a == c) ? (per[a] ? 1 : 2) : 2;
+        const int hi = (a == c) ? n[a] : n[a] + 1;
+        if (id[a] < lo || id[a] > hi) return false;
+    }
+    return true;
+}
+
+// ---- S5: projection --------------------------------------------------------
+struct AdjProj {
+    Grid g;
+    double dt, rd3[3];  // dt and 1/dx, 1/dy, 1/dz
+    int dp_flag;
+    const double *ub[3];
+    double *usb[3];
+    double *pb;   // in: adjoint p; out: dp_flag * it
+    double *dpb;  // out
+};
+
+// u* adjoint = u adjoint (all faces)
+__global__ void __launch_bounds__(AT) k_adj_proj_vel(AdjProj P, int c) {
+    const size_t n = (c == 0) ? P.g.nu() : (c == 1) ? P.g.nv() : P.g.nw();
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q < n) P.usb[c][q] = P.ub[c][q];
+}
+
+// dp adjoint = p adjoint + grad^T(u adjoint) with u(m) = u*(m) - dt (dp(m+1) - dp(m))/dx
+__global__ void __launch_bounds__(AT) k_adj_proj_dp(AdjProj P) {
+    const Grid g = P.g;
+    const int np_[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q >= g.np()) return;
+    int i, j, k;
+    unflat(q, np_, i, j, k);
+    double s = 0.0;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        dims_of(g, c, n);
+        int m1[3] = {i, j, k};
+        m1[c] -= 1;
+        const double hi = rd(P.ub[c], n, i, j, k), lo = rd(P.ub[c], n, m1[0], m1[1], m1[2]);
+        s += P.dt * P.rd3[c] * (hi - lo);
+    }
+    const double pb = P.pb[q];
+    P.dpb[q] = pb + s;
+    P.pb[q] = P.dp_flag ? pb : 0.0;
+}
+
+// ---- S4 / S2: ghost fills, transposed (one axis, one component) ------------
+struct AdjFill {
+    Grid g;
+    int ax, bc, comp;   // comp 0..2 velocity, 3 pressure
+    double d;           // spacing (pressure code 3)
+    PlaneLayout L;
+    double *a;          // adjoint array (in place)
+    double *pbar;       // plane-set adjoint (accumulated)
+};
+
+// forward rules (velocity, NS_functions.f90:235-308; pressure :1421-1462):
+//   bc 1 normal: a(N) = a(1);  other: a(1) = a(N-1), a(N) = a(2)
+//   bc 2 velocity normal: a(1) = b1, a(N) = bn;  tangential: a(1) = 2 b1 - a(2), a(N) = 2 bn - a(N-1)
+//   bc 2 pressure: a(1) = 2 b1 - a(2), a(N) = 2 bn - a(N-1)
+//   bc 3 pressure: a(1) = -d b1 + a(2), a(N) = d bn + a(N-1)
+//   bc 4: a(1) = b1, a(N) = bn
+// transposed in reverse order of the assignments (a(N) first)
+__global__ void __launch_bounds__(AT) k_adj_fill(AdjFill P) {
+    int n[3];
+    dims_of(P.g, P.comp, n);
+    const int ax = P.ax;
+    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    const size_t t = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (t >= (size_t)n[o1] * n[o2]) return;
+    int id[3];
+    id[o1] = (int)(t % n[o1]) + 1;
+    id[o2] = (int)(t / n[o1]) + 1;
+    const int N = n[ax];
+    auto at = [&](int pos) -> double & {
+        int q[3] = {id[0], id[1], id[2]};
+        q[ax] = pos;
+        return P.a[FIX(q[0], q[1], q[2], n[0], n[1])];
+    };
+    const bool pres = P.comp == 3;
+    const long o1p = pres ? P.L.off[18 + 2 * ax] : P.L.off[6 * P.comp + 2 * ax];
+    const long onp = o1p + (pres ? (P.L.off[19 + 2 * ax] - P.L.off[18 + 2 * ax])
+                                 : (P.L.off[6 * P.comp + 2 * ax + 1] - P.L.off[6 * P.comp + 2 * ax]));
+    double *b1 = P.pbar ? P.pbar + o1p : nullptr, *bn = P.pbar ? P.pbar + onp : nullptr;
+    const size_t pl = t;
+    const bool normal = (P.comp == ax);
+    if (P.bc == 1) {
+        if (!pres && normal) {
+            at(1) += at(N);
+            at(N) = 0.0;
+        } else {
+            at(2) += at(N);
+            at(N) = 0.0;
+            at(N - 1) += at(1);
+            at(1) = 0.0;
+        }
+    } else if (P.bc == 2 && !pres && normal) {
+        bn[pl] += at(N);
+        at(N) = 0.0;
+        b1[pl] += at(1);
+        at(1) = 0.0;
+    } else if (P.bc == 2) {
+        bn[pl] += 2.0 * at(N);
+        at(N - 1) -= at(N);
+        at(N) = 0.0;
+        b1[pl] += 2.0 * at(1);
+        at(2) -= at(1);
+        at(1) = 0.0;
+    } else if (P.bc == 3) {
+        bn[pl] += P.d * at(N);
+        at(N - 1) += at(N);
+        at(N) = 0.0;
+        b1[pl] += -P.d * at(1);
+        at(2) += at(1);
+        at(1) = 0.0;
+    } else if (P.bc == 4) {
+        bn[pl] += at(N);
+        at(N) = 0.0;
+        b1[pl] += at(1);
+        at(1) = 0.0;
+    }
+}
+
+// ---- S3: interior gather, boundary RHS terms and divergence, transposed -----
+__global__ void __launch_bounds__(AT) k_adj_take_interior(Grid g, const double *dpb, double *r) {
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q >= g.ni()) return;
+    const int a = (int)(q % g.nx) + 1;
+    const size_t s = q / g.nx;
+    const int b = (int)(s % g.ny) + 1, c = (int)(s / g.ny) + 1;
+    r[q] = dpb[FIX(a + 1, b + 1, c + 1, g.nx + 2, g.ny + 2)];
+}
+
+// pr_bc_staggered_modify_rhs transposed (NS_functions.f90:1362-1398): one axis
+struct AdjRhs {
+    Grid g;
+    int ax, pbc;
+    double d;
+    PlaneLayout L;
+    const double *rb;  // adjoint RHS (interior)
+    double *pbar;
+};
+__global__ void __launch_bounds__(AT) k_adj_modify_rhs(AdjRhs P) {
+    const int n[3] = {P.g.nx, P.g.ny, P.g.nz}, np_[3] = {P.g.nx + 2, P.g.ny + 2, P.g.nz + 2};
+    const int ax = P.ax, o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    const size_t t = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (t >= (size_t)n[o1] * n[o2]) return;
+    const int pp = (int)(t % n[o1]) + 1, qq = (int)(t / n[o1]) + 1;
+    int i1[3], iN[3];
+    i1[o1] = iN[o1] = pp;
+    i1[o2] = iN[o2] = qq;
+    i1[ax] = 1;
+    iN[ax] = n[ax];
+    const size_t pl = (size_t)pp + (size_t)np_[o1] * (size_t)qq;
+    const double r1 = P.rb[FIX(i1[0], i1[1], i1[2], P.g.nx, P.g.ny)], rn = P.rb[FIX(iN[0], iN[1], iN[2], P.g.nx, P.g.ny)];
+    double *b1 = P.pbar + P.L.off[18 + 2 * ax], *bn = P.pbar + P.L.off[19 + 2 * ax];
+    const double d = P.d, d2 = d * d;
+    if (P.pbc == 2) { b1[pl] += -2.0 * r1 / d2; bn[pl] += -2.0 * rn / d2; }
+    else if (P.pbc == 3) { b1[pl] += r1 / d; bn[pl] += -rn / d; }
+    else if (P.pbc == 4) { b1[pl] += -r1 / d2; bn[pl] += -rn / d2; }
+}
+
+// u* adjoint += div^T(r adjoint): r(a,b,c) = sum_c' (s(face+1) - s(face))/h / dt
+__global__ void __launch_bounds__(AT) k_adj_div(Grid g, int c, double rh_dt, const double *rb, double *usb) {
+    int n[3];
+    dims_of(g, c, n);
+    const size_t tot = (size_t)n[0] * n[1] * n[2];
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q >= tot) return;
+    int i, j, k;
+    unflat(q, n, i, j, k);
+    // interior cell (0-based) left of the face along c is (face index - 2), right is (face - 1); others: index - 2
+    int cl[3] = {i - 2, j - 2, k - 2}, cr[3] = {i - 2, j - 2, k - 2};
+    cr[c] += 1;
+    const int m[3] = {g.nx, g.ny, g.nz};
+    auto R = [&](const int *a) -> double {
+        for (int x = 0; x < 3; x++)
+            if (a[x] < 0 || a[x] >= m[x]) return 0.0;
+        return rb[(size_t)a[0] + (size_t)g.nx * ((size_t)a[1] + (size_t)g.ny * a[2])];
+    };
+    usb[q] += (R(cl) - R(cr)) * rh_dt;
+}
+
+// ---- S1: predictor ---------------------------------------------------------
+struct AdjPred {
+    Grid g;
+    int per[3];
+    double dt, nu, dpf, rd3[3], rd2[3];  // 1/h and 1/h^2
+    double c_rhs, c_hist;                // u* = ... + dt*(c_rhs*rhs - c_hist*H ...); H' = rhs if keep_h
+    int keep_h;                          // H' = rhs (AB2) or 0 (Euler)
+    const double *u[3];                  // forward state at the step start
+    const double *usb[3];                // adjoint of u* (after the fill transpose)
+    double *hb[3];                       // in: adjoint of H'; out: adjoint of H
+    double *gb[3];                       // out: adjoint of rhs on the computed faces (0 elsewhere)
+    double *ub[3];                       // out: adjoint of u
+    double *pb;                          // adjoint of p (accumulated)
+};
+
+__global__ void __launch_bounds__(AT) k_adj_pred_prep(AdjPred P, int c) {
+    int n[3];
+    dims_of(P.g, c, n);
+    const size_t tot = (size_t)n[0] * n[1] * n[2];
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q >= tot) return;
+    int i, j, k;
+    unflat(q, n, i, j, k);
+    const bool on = computed(P.g, P.per, c, i, j, k);
+    const double us = on ? P.usb[c][q] : 0.0;
+    const double hb = P.hb[c][q];
+    P.gb[c][q] = on ? (P.dt * P.c_rhs * us + (P.keep_h ? hb : 0.0)) : 0.0;
+    P.hb[c][q] = on ? -P.dt * P.c_hist * us : hb;
+    P.ub[c][q] = us;  // u* = u + ...
+}
+
+// p adjoint += (grad p)^T: u*(q) -= dt dp_flag (p(q+e) - p(q))/h on computed faces
+__global__ void __launch_bounds__(AT) k_adj_pred_p(AdjPred P) {
+    const Grid g = P.g;
+    const int np_[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q >= g.np()) return;
+    int i, j, k;
+    unflat(q, np_, i, j, k);
+    double s = 0.0;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        dims_of(g, c, n);
+        int m1[3] = {i, j, k};
+        m1[c] -= 1;
+        // face m1 reads p(m1+e) = p(i,j,k) with weight -1; face (i,j,k) reads p(i,j,k) with +1
+        double a = 0.0, b = 0.0;
+        if (m1[c] >= 1 && computed(g, P.per, c, m1[0], m1[1], m1[2])) a = rd(P.usb[c], n, m1[0], m1[1], m1[2]);
+        if (i <= n[0] && j <= n[1] && k <= n[2] && computed(g, P.per, c, i, j, k)) b = rd(P.usb[c], n, i, j, k);
+        s += P.rd3[c] * (b - a);
+    }
+    P.pb[q] += P.dt * P.dpf * s;
+}
+
+// u adjoint += nu diff^T(g) - conv^T(g; u): gathered at each face of component c
+__global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
+    const Grid g = P.g;
+    int nu_[3], nv_[3], nw_[3];
+    dims_of(g, 0, nu_);
+    dims_of(g, 1, nv_);
+    dims_of(g, 2, nw_);
+    const int *nc = (c == 0) ? nu_ : (c == 1) ? nv_ : nw_;
+    const size_t tot = (size_t)nc[0] * nc[1] * nc[2];
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q >= tot) return;
+    int i, j, k;
+    unflat(q, nc, i, j, k);
+    const double *U = P.u[0], *V = P.u[1], *W = P.u[2];
+    const double *GU = P.gb[0], *GV = P.gb[1], *GW = P.gb[2];
+    const double rx = P.rd3[0], ry = P.rd3[1], rz = P.rd3[2];
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    const double *Gc = P.gb[c];
+    const double *Xc = P.u[c];
+    int id[3] = {i, j, k};
+    // diffusion, diff2 along each axis (own axis wraps output 1's left neighbour to n when periodic)
+    double dif = 0.0;
+    for (int a = 0; a < 3; a++) {
+        int m[3] = {i, j, k}, p1[3] = {i, j, k};
+        m[a] -= 1;
+        p1[a] += 1;
+        double s = rd(Gc, nc, m[0], m[1], m[2]) - 2.0 * rd(Gc, nc, i, j, k) + rd(Gc, nc, p1[0], p1[1], p1[2]);
+        if (a == c && P.per[c] && id[c] == nn[c]) {
+            int o[3] = {i, j, k};
+            o[c] = 1;
+            s += rd(Gc, nc, o[0], o[1], o[2]);
+        }
+        dif += s * P.rd2[a];
+    }
+    // convection: own-axis squares at the cells m = id-1 and m = id along c
+    double conv = 0.0;
+    {
+        const double rh = P.rd3[c];
+        auto cellbar = [&](int m) -> double {  // adjoint of a_m^2 times 2 a_m, cell between faces m and m+1
+            if (m < 1 || m > nn[c]) return 0.0;
+            int f0[3] = {i, j, k}, f1[3] = {i, j, k};
+            f0[c] = m;
+            f1[c] = m + 1;
+            double X = (rd(Gc, nc, f0[0], f0[1], f0[2]) - rd(Gc, nc, f1[0], f1[1], f1[2])) * rh;
+            if (P.per[c] && m == nn[c]) {  // a_n is also output 1's left average
+                int o[3] = {i, j, k};
+                o[c] = 1;
+                X -= rd(Gc, nc, o[0], o[1], o[2]) * rh;
+            }
+            const double am = 0.5 * (rd(Xc, nc, f1[0], f1[1], f1[2]) + rd(Xc, nc, f0[0], f0[1], f0[2]));
+            return 2.0 * am * X;
+        };
+        conv += 0.5 * (cellbar(id[c]) + cellbar(id[c] - 1));
+    }
+    // edge fluxes: EF (u,v) in x-y, FF (u,w) in x-z, GF (v,w) in y-z
+    // EF(i,j,k) = A*B, A = (u(i,j+1)+u(i,j))/2, B = (v(i+1,j)+v(i,j))/2
+    auto EFbar = [&](int a, int b, int kk) -> double {
+        return (rd(GU, nu_, a, b, kk) - rd(GU, nu_, a, b + 1, kk)) * ry + (rd(GV, nv_, a, b, kk) - rd(GV, nv_, a + 1, b, kk)) * rx;
+    };
+    auto FFbar = [&](int a, int b, int kk) -> double {
+        return (rd(GU, nu_, a, b, kk) - rd(GU, nu_, a, b, kk + 1)) * rz + (rd(GW, nw_, a, b, kk) - rd(GW, nw_, a + 1, b, kk)) * rx;
+    };
+    auto GFbar = [&](int a, int b, int kk) -> double {
+        return (rd(GV, nv_, a, b, kk) - rd(GV, nv_, a, b, kk + 1)) * rz + (rd(GW, nw_, a, b, kk) - rd(GW, nw_, a, b + 1, kk)) * ry;
+    };
+    auto avg = [&](const double *A, const int *n, int a0, int b0, int c0, int a1, int b1, int c1) {
+        return 0.5 * (rd(A, n, a1, b1, c1) + rd(A, n, a0, b0, c0));
+    };
+    if (c == 0) {
+        // u(i,j) in A(i,j) and A(i,j-1) of EF; in A(i,j,k), A(i,j,k-1) of FF
+        conv += 0.5 * (EFbar(i, j, k) * avg(V, nv_, i, j, k, i + 1, j, k) + EFbar(i, j - 1, k) * avg(V, nv_, i, j - 1, k, i + 1, j - 1, k));
+        conv += 0.5 * (FFbar(i, j, k) * avg(W, nw_, i, j, k, i + 1, j, k) + FFbar(i, j, k - 1) * avg(W, nw_, i, j, k - 1, i + 1, j, k - 1));
+    } else if (c == 1) {
+        // v(i,j) in B(i,j) and B(i-1,j) of EF; in A(i,j,k), A(i,j,k-1) of GF
+        conv += 0.5 * (EFbar(i, j, k) * avg(U, nu_, i, j, k, i, j + 1, k) + EFbar(i - 1, j, k) * avg(U, nu_, i - 1, j, k, i - 1, j + 1, k));
+        conv += 0.5 * (GFbar(i, j, k) * avg(W, nw_, i, j, k, i, j + 1, k) + GFbar(i, j, k - 1) * avg(W, nw_, i, j, k - 1, i, j + 1, k - 1));
+    } else {
+        // w(i,j,k) in B(i,j,k), B(i-1,j,k) of FF; in B(i,j,k), B(i,j-1,k) of GF
+        conv += 0.5 * (FFbar(i, j, k) * avg(U, nu_, i, j, k, i, j, k + 1) + FFbar(i - 1, j, k) * avg(U, nu_, i - 1, j, k, i - 1, j, k + 1));
+        conv += 0.5 * (GFbar(i, j, k) * avg(V, nv_, i, j, k, i, j, k + 1) + GFbar(i, j - 1, k) * avg(V, nv_, i, j - 1, k, i, j - 1, k + 1));
+    }
+    P.ub[c][q] += P.nu * dif - conv;
+}
+
+__global__ void __launch_bounds__(AT) k_block_sum(const double *a, size_t n, double *partial) {
+    double s = 0.0;
+    for (size_t q = (size_t)blockIdx.x * AT + threadIdx.x; q < n; q += (size_t)gridDim.x * AT) s += a[q];
+    __shared__ double red[AT];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = AT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_add_at(double *a, size_t q, double v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) a[q] += v;
+}
+
+}  // namespace
+
+int launch_block_sums(const double *a, size_t n, double *partial, int max_blocks, hipStream_t st) {
+    size_t nb = (n + AT - 1) / AT;
+    if (nb > (size_t)max_blocks) nb = max_blocks;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_block_sum, dim3((unsigned)nb), dim3(AT), 0, st, a, n, partial);
+    return (int)nb;
+}
+
+void launch_add_at(double *a, size_t q, double v, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_at, dim3(1), dim3(64), 0, st, a, q, v);
+}
+
+void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag, const double *const ub[3],
+                        double *const usb[3], double *pb, double *dpb, hipStream_t st) {
+    AdjProj P;
+    P.g = g;
+    P.dt = dt;
+    for (int a = 0; a < 3; a++) {
+        P.rd3[a] = 1.0 / h[a];
+        P.ub[a] = ub[a];
+        P.usb[a] = usb[a];
+    }
+    P.dp_flag = dp_flag;
+    P.pb = pb;
+    P.dpb = dpb;
+    const size_t n[3] = {g.nu(), g.nv(), g.nw()};
+    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_proj_vel, dim3(ablocks(n[c])), dim3(AT), 0, st, P, c);
+    hipLaunchKernelGGL(k_adj_proj_dp, dim3(ablocks(g.np())), dim3(AT), 0, st, P);
+}
+
+void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3], double *a, double *pbar,
+                     hipStream_t st) {
+    AdjFill P;
+    P.g = g;
+    P.comp = comp;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.a = a;
+    P.pbar = pbar;
+    int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+    if (comp < 3) n[comp] -= 1;
+    for (int ax = 2; ax >= 0; ax--) {  // forward order x, y, z
+        P.ax = ax;
+        P.bc = bc[ax];
+        P.d = h[ax];
+        if (bc[ax] < 1 || bc[ax] > 4) continue;
+        const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        hipLaunchKernelGGL(k_adj_fill, dim3(ablocks((size_t)n[o1] * n[o2])), dim3(AT), 0, st, P);
+    }
+}
+
+void launch_adj_take_interior(const Grid &g, const double *dpb, double *r, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_take_interior, dim3(ablocks(g.ni())), dim3(AT), 0, st, g, dpb, r);
+}
+
+void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
+                        double *const usb[3], double *pbar, hipStream_t st) {
+    AdjRhs R;
+    R.g = g;
+    R.L = plane_layout(g.nx, g.ny, g.nz);
+    R.rb = rb;
+    R.pbar = pbar;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    for (int ax = 0; ax < 3; ax++) {
+        if (pbc[ax] == 1 || !pbar) continue;
+        R.ax = ax;
+        R.pbc = pbc[ax];
+        R.d = h[ax];
+        const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        hipLaunchKernelGGL(k_adj_modify_rhs, dim3(ablocks((size_t)n[o1] * n[o2])), dim3(AT), 0, st, R);
+    }
+    const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
+    for (int c = 0; c < 3; c++)
+        hipLaunchKernelGGL(k_adj_div, dim3(ablocks(tot[c])), dim3(AT), 0, st, g, c, 1.0 / (h[c] * dt), rb, usb[c]);
+}
+
+void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, int dp_flag, const double h[3],
+                        int scheme, int first, const double *const u[3], const double *const usb[3],
+                        double *const hb[3], double *const gb[3], double *const ub[3], double *pb, hipStream_t st) {
+    AdjPred P;
+    P.g = g;
+    for (int a = 0; a < 3; a++) {
+        P.per[a] = per[a];
+        P.rd3[a] = 1.0 / h[a];
+        P.rd2[a] = 1.0 / (h[a] * h[a]);
+        P.u[a] = u[a];
+        P.usb[a] = usb[a];
+        P.hb[a] = hb[a];
+        P.gb[a] = gb[a];
+        P.ub[a] = ub[a];
+    }
+    P.dt = dt;
+    P.nu = nu;
+    P.dpf = (double)dp_flag;
+    P.pb = pb;
+    const bool ab2 = (scheme == NS_AB2);
+    P.c_rhs = (ab2 && !first) ? 1.5 : 1.0;
+    P.c_hist = (ab2 && !first) ? 0.5 : 0.0;
+    P.keep_h = ab2 ? 1 : 0;
+    const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
+    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_pred_prep, dim3(ablocks(tot[c])), dim3(AT), 0, st, P, c);
+    if (dp_flag) hipLaunchKernelGGL(k_adj_pred_p, dim3(ablocks(g.np())), dim3(AT), 0, st, P);
+    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_convdiff, dim3(ablocks(tot[c])), dim3(AT), 0, st, P, c);
+}
+
+}  // namespace ns

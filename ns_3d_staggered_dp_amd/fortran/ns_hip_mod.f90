@@ -19,7 +19,8 @@ module ns_hip_mod
     private
 
     integer, parameter, public :: NS_EULER = 0, NS_AB2 = 1, NS_AB2_CN = 2
-    integer, parameter, public :: NS_FLAG_GAUGE_CORNER = 1, NS_FLAG_CN_CYCLIC = 2, NS_FLAG_NO_GRAPH = 4
+    integer, parameter, public :: NS_FLAG_GAUGE_CORNER = 1, NS_FLAG_CN_CYCLIC = 2, NS_FLAG_NO_GRAPH = 4, &
+        NS_FLAG_EAGER_PROJECTION = 8
 
     type, bind(C), public :: ns_config
         integer(c_int) :: nx, ny, nz

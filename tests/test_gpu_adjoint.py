@@ -1,0 +1,134 @@
+"""GPU: the discrete adjoint (ns_adjoint_step) against finite differences.
+
+One forward step is a polynomial of degree two in the state and the plane
+set (the convection is quadratic, everything else affine), so the central
+difference (J(x+ed) - J(x-ed))/2e of J = <y, step(x)> is the exact
+directional derivative up to rounding: the adjoint gradient must match it to
+~1e-10 relative for random x, y, d that touch every input (u, v, w, p, the
+AB2 history, all planes and mean_dp).  Over several steps (checkpoint and
+recompute) the bar is the 1e-6 of central differences with a small step."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from ns_3d_staggered_dp_amd import (AB2, EULER, FLAG_EAGER_PROJECTION, FLAG_GAUGE_CORNER, NSConfig, _abi, fields)
+from ns_3d_staggered_dp_amd.grid import plane_set_length
+from ns_3d_staggered_dp_amd.solver import NSSolver, _ptr
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("u", "v", "w", "p", "hx", "hy", "hz")
+
+
+def box(nx, ny, nz, **kw):
+    return NSConfig(nx=nx, ny=ny, nz=nz, dx=0.31, dy=0.27, dz=0.23, nu=0.05, dt=2e-3, **kw)
+
+
+def periodic(nx, ny, nz, **kw):
+    return NSConfig(nx=nx, ny=ny, nz=nz, dx=2 * np.pi / nx, dy=0.4, dz=0.5, nu=0.02, dt=2e-3, bc=(1, 1, 1),
+                    pbc=(1, 1, 1), **kw)
+
+
+def rand_state(cfg, seed, amp=0.5):
+    rng = np.random.default_rng(seed)
+    u, v, w, p = fields.random_ghosted(cfg, seed)
+    st = {"u": amp * u, "v": amp * v, "w": amp * w, "p": p}
+    for k, a in zip(("hx", "hy", "hz"), fields.random_ghosted(cfg, seed + 1)[:3]):
+        st[k] = 0.3 * a
+    return st, rng
+
+
+def step_once(s, st, planes, first):
+    s.upload(*(st[k] for k in KEYS), first=first)
+    s.set_bc_planes(planes)
+    s.step(1)
+    return s.download()
+
+
+CASES = [
+    ("walls_ab2", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2, dp_flag=1), False),
+    ("walls_ab2_first", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2, dp_flag=1), True),
+    ("walls_euler_dst", lambda: box(10, 9, 7, bc=(4, 4, 4), pbc=(2, 2, 2), scheme=EULER, dp_flag=1), False),
+    ("walls_mixed", lambda: box(10, 8, 9, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2, dp_flag=1), False),
+    ("periodic_fft", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=1), False),
+    ("periodic_gauge_eager", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=0,
+                                              flags=FLAG_EAGER_PROJECTION | FLAG_GAUGE_CORNER), False),
+]
+
+
+@pytest.mark.parametrize("name,mk,first", CASES, ids=[c[0] for c in CASES])
+def test_adjoint_step_exact_on_quadratic_map(name, mk, first):
+    cfg = mk()
+    st, rng = rand_state(cfg, 31)
+    L = plane_set_length(cfg.nx, cfg.ny, cfg.nz)
+    planes = 0.2 * rng.standard_normal(L)
+    y = {k: np.asfortranarray(rng.standard_normal(st[k].shape)) for k in KEYS}
+    d = {k: np.asfortranarray(rng.standard_normal(st[k].shape)) for k in KEYS}
+    dpl = rng.standard_normal(L)
+    with NSSolver(cfg) as s:
+        def J(eps):
+            xs = {k: np.asfortranarray(st[k] + eps * d[k]) for k in KEYS}
+            out = step_once(s, xs, planes + eps * dpl, first)
+            return sum(float(np.sum(y[k] * out[k])) for k in KEYS)
+
+        eps = 1e-2
+        fd = (J(eps) - J(-eps)) / (2 * eps)
+        s.upload(*(st[k] for k in KEYS), first=first)
+        _abi.call("ns_adjoint_set_state", s.h, *(_ptr(y[k]) for k in KEYS))
+        pbar = np.zeros(L)
+        _abi.call("ns_adjoint_step", s.h, _ptr(st["u"]), _ptr(st["v"]), _ptr(st["w"]), int(first), _ptr(pbar))
+        g = {k: np.zeros(st[k].shape, order="F") for k in KEYS}
+        _abi.call("ns_adjoint_get_state", s.h, *(_ptr(g[k]) for k in KEYS))
+    ad = sum(float(np.sum(g[k] * d[k])) for k in KEYS) + float(pbar @ dpl)
+    scale = sum(float(np.sum(np.abs(g[k] * d[k]))) for k in KEYS) + float(np.abs(pbar * dpl).sum())
+    assert abs(fd - ad) <= 1e-10 * scale, (fd, ad, scale)
+
+
+def test_adjoint_unsupported_for_lazy_and_cn():
+    from ns_3d_staggered_dp_amd import AB2_CN
+
+    for cfg in (periodic(8, 8, 8, scheme=AB2, dp_flag=0), box(8, 8, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN,
+                                                                 dp_flag=1)):
+        with NSSolver(cfg) as s:
+            rc = _abi.lib().ns_adjoint_set_state(s.h, None, None, None, None, None, None, None)
+            assert rc == _abi.NS_ERR_UNSUPPORTED
+
+
+def test_rollout_gradient_matches_central_differences():
+    """torch.autograd through 6 steps (checkpoint every 2, recompute), walls
+    with per-step planes: d/d(u0, v0, w0, planes) of a sub-domain mismatch,
+    checked in random directions against central differences (1e-6)."""
+    import torch
+
+    from ns_3d_staggered_dp_amd.adjoint import Rollout, make_rollout_function
+
+    cfg = box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2, dp_flag=1)
+    nsteps = 6
+    st, rng = rand_state(cfg, 41, amp=0.3)
+    L = plane_set_length(cfg.nx, cfg.ny, cfg.nz)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).cuda()
+    u0, v0, w0 = dev(st["u"]), dev(st["v"]), dev(st["w"])
+    planes = dev(0.1 * rng.standard_normal(nsteps * L))
+    target = {k: dev(0.3 * rng.standard_normal(st[k].shape)) for k in ("u", "v", "w")}
+    mask = {k: dev((rng.uniform(size=st[k].shape) < 0.5).astype(np.float64)) for k in ("u", "v", "w")}
+    ro = Rollout(cfg, nsteps, checkpoint_every=2)
+    F = make_rollout_function()
+
+    def objective(a, b, c, pl):
+        uN, vN, wN = F.apply(a, b, c, pl, ro)
+        return 0.5 * sum(((x - target[k]) ** 2 * mask[k]).sum() for k, x in (("u", uN), ("v", vN), ("w", wN)))
+
+    xs = [t.clone().requires_grad_(True) for t in (u0, v0, w0, planes)]
+    J0 = objective(*xs)
+    grads = torch.autograd.grad(J0, xs)
+    for trial in range(3):
+        d = [torch.from_numpy(np.random.default_rng(50 + trial).standard_normal(t.numel())).cuda() for t in xs]
+        eps = 1e-6
+        with torch.no_grad():
+            jp = objective(*(t + eps * dd for t, dd in zip(xs, d))).item()
+            jm = objective(*(t - eps * dd for t, dd in zip(xs, d))).item()
+        fd = (jp - jm) / (2 * eps)
+        ad = sum(float((g * dd).sum()) for g, dd in zip(grads, d))
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (trial, fd, ad)
+    ro.close()
