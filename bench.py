@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_256.json"))
+    ap.add_argument("--adjoint-n", type=int, default=128,
+                    help="grid of the adjoint/forward cost measurement (walls, AB2, DCT, dp_flag=1); 0 skips it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -115,6 +117,55 @@ def summarize_profile(d, traffic_json):
     print("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
     print("\n".join(rows))
     return 0
+
+
+def adjoint_cost(n, steps=10):
+    """Adjoint/forward cost ratio on the re-simulation setting (walls bc=2,
+    Neumann pressure pbc=3, AB2, dp_flag=1, n^3): mean ms of one forward step
+    (ns_step) and of one reverse step (ns_adjoint_step, forward velocity
+    already on the device); the checkpointed reverse sweep adds one forward
+    recompute per step on top of that."""
+    import ctypes
+
+    import torch
+
+    from ns_3d_staggered_dp_amd import AB2, NSConfig, fields
+    from ns_3d_staggered_dp_amd import _abi
+    from ns_3d_staggered_dp_amd.grid import plane_set_length
+    from ns_3d_staggered_dp_amd.solver import NSSolver
+
+    cfg = NSConfig.cube(n, nu=1e-3, dt=5e-4, scheme=AB2, bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=5)
+    rng = np.random.default_rng(6)
+    L = plane_set_length(n, n, n)
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, first=True)
+        s.set_bc_planes(0.01 * rng.standard_normal(L))
+        s.step(3)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        fwd = (time.perf_counter() - t0) / steps * 1e3
+        ptr = s.state_pointers()
+        us, vs, ws = (torch.empty(int(x), dtype=torch.float64, device="cuda") for x in
+                      ((n + 1) * (n + 2) ** 2, (n + 2) * (n + 1) * (n + 2), (n + 2) ** 2 * (n + 1)))
+        _abi.call("ns_download_state", s.h, *(ctypes.c_void_p(t.data_ptr()) for t in (us, vs, ws)), None, None, None,
+                  None)
+        ub = torch.from_numpy(np.asfortranarray(u).ravel(order="F")).cuda()
+        _abi.call("ns_adjoint_set_state", s.h, ctypes.c_void_p(ub.data_ptr()), None, None, None, None, None, None)
+        pb = torch.zeros(L, dtype=torch.float64, device="cuda")
+        args = [ctypes.c_void_p(t.data_ptr()) for t in (us, vs, ws)]
+        _abi.call("ns_adjoint_step", s.h, *args, 0, ctypes.c_void_p(pb.data_ptr()))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _abi.call("ns_adjoint_step", s.h, *args, 0, ctypes.c_void_p(pb.data_ptr()))
+        torch.cuda.synchronize()
+        adj = (time.perf_counter() - t0) / steps * 1e3
+    return {"config": f"{n}^3 walls bc=2 pbc=3, AB2, DCT Poisson, dp_flag=1", "forward_ms": round(fwd, 4),
+            "adjoint_ms": round(adj, 4), "ratio": round(adj / fwd, 3),
+            "ratio_with_recompute": round((adj + fwd) / fwd, 3)}
 
 
 def copy_calibration():
@@ -229,6 +280,8 @@ def main():
         "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
         "checks": {"finite": bool(finite), "max_div": maxdiv},
     }
+    if a.adjoint_n > 0:
+        line["adjoint"] = adjoint_cost(a.adjoint_n)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, u, v, w, p, a.cpu_steps)
     print(json.dumps(line), flush=True)

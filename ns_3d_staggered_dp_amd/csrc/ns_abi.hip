@@ -938,26 +938,17 @@ static int adj_alloc(ns_ctx *c) {
     const Grid &g = c->g;
     const size_t n[3] = {g.nu(), g.nv(), g.nw()};
     int rc;
-    for (int a = 0; a < 3; a++)
-        if ((rc = dalloc(c, &c->ab[a], n[a])) || (rc = dalloc(c, &c->ahb[a], n[a])) ||
-            (rc = dalloc(c, &c->ausb[a], n[a])) || (rc = dalloc(c, &c->agb[a], n[a])))
+    // the u* adjoint lives in the u adjoint's arrays: S5^T reads the u adjoint
+    // once (into the dp adjoint) and the u* adjoint then starts equal to it
+    for (int a = 0; a < 3; a++) {
+        if ((rc = dalloc(c, &c->ab[a], n[a])) || (rc = dalloc(c, &c->ahb[a], n[a])) || (rc = dalloc(c, &c->agb[a], n[a])))
             return rc;
+        c->ausb[a] = c->ab[a];
+    }
     if ((rc = dalloc(c, &c->apb, g.np())) || (rc = dalloc(c, &c->adpb, g.np())) || (rc = dalloc(c, &c->ar, g.ni())) ||
         (rc = dalloc(c, &c->arsol, g.ni())) || (rc = dalloc(c, &c->apbar, (size_t)c->L.total)))
         return rc;
     if (!c->mpartial && (rc = dalloc(c, &c->mpartial, (size_t)2048 * NS_METRIC_PARTS))) return rc;
-    return NS_OK;
-}
-
-// sum of a device array (block partials on the device, combined on the host)
-static int dev_sum(ns_ctx *c, const double *a, size_t n, double *out) {
-    const int nb = launch_block_sums(a, n, c->mpartial, 2048, c->st);
-    std::vector<double> h(nb);
-    HIPCHK(hipMemcpyAsync(h.data(), c->mpartial, nb * 8, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    long double s = 0;
-    for (double x : h) s += x;
-    *out = (double)s;
     return NS_OK;
 }
 
@@ -1015,32 +1006,34 @@ int ns_adjoint_step(ns_ctx *c, const double *u, const double *v, const double *w
         (rc = arg_in(c, AW, w, g.nw(), true)))
         return rc;
     HIPCHK(hipMemsetAsync(c->apbar, 0, (size_t)c->L.total * 8, c->st));
-    // S5^T: projection (and the corner gauge: p = dp_flag p + dp - dp(1,1,1))
-    double psum = 0.0;
-    if (f.flags & NS_FLAG_GAUGE_CORNER)
-        if ((rc = dev_sum(c, c->apb, g.np(), &psum))) return rc;
+    // S5^T: projection (and the corner gauge: p = dp_flag p + dp - dp(1,1,1)); the
+    // sums stay on the device (block partials, then one block adds them)
+    int nbs = 0;
+    if (f.flags & NS_FLAG_GAUGE_CORNER) nbs = launch_block_sums(c->apb, g.np(), c->mpartial, 2048, c->st);
     launch_adj_project(g, f.dt, h, f.dp_flag, c->ab, c->ausb, c->apb, c->adpb, c->st);
-    if (f.flags & NS_FLAG_GAUGE_CORNER) launch_add_at(c->adpb, 0, -psum, c->st);
+    if (f.flags & NS_FLAG_GAUGE_CORNER) launch_add_partials(c->adpb, 0, c->mpartial, nbs, -1.0, c->st);
     // S4^T: pressure ghost fill
-    launch_adj_fill(g, 3, f.pbc, h, c->adpb, c->apbar, c->st);
+    double *const dpa[3] = {c->adpb, c->adpb, c->adpb};
+    launch_adj_fill(g, 3, f.pbc, h, dpa, c->apbar, c->st);
     // S3^T: zero mode (mean_dp of the DCT solve), symmetric solve, boundary terms, divergence
     launch_adj_take_interior(g, c->adpb, c->ar, c->st);
     // mean_dp sets the zero mode, which exists only when every axis is Neumann
     if (c->pp.dct && f.pbc[0] == 3 && f.pbc[1] == 3 && f.pbc[2] == 3) {
-        double msum = 0.0;
-        if ((rc = dev_sum(c, c->ar, g.ni(), &msum))) return rc;
-        launch_add_at(c->apbar, (size_t)c->L.off[24], msum, c->st);
+        nbs = launch_block_sums(c->ar, g.ni(), c->mpartial, 2048, c->st);
+        launch_add_partials(c->apbar, (size_t)c->L.off[24], c->mpartial, nbs, 1.0, c->st);
     }
     if ((rc = run_poisson(c, nullptr, nullptr, nullptr, c->ar, nullptr, nullptr, 0.0, nullptr, c->arsol))) return rc;
     launch_adj_rhs_div(g, f.pbc, h, f.dt, c->arsol, c->ausb, c->apbar, c->st);
     // S2^T: velocity ghost fill
-    for (int cc = 0; cc < 3; cc++) launch_adj_fill(g, cc, f.bc, h, c->ausb[cc], c->apbar, c->st);
+    launch_adj_fill(g, 0, f.bc, h, c->ausb, c->apbar, c->st);
     // S1^T: predictor
     const double *uu[3] = {AU.dev, AV.dev, AW.dev};
     launch_adj_predict(g, per, f.dt, f.nu, f.dp_flag, h, f.scheme, first, uu, c->ausb, c->ahb, c->agb, c->ab, c->apb,
                        c->st);
     if (planes_bar) HIPCHK(hipMemcpyAsync(planes_bar, c->apbar, (size_t)c->L.total * 8, hipMemcpyDefault, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    // host outputs and staged host inputs need the stream drained; device ones stay stream-ordered
+    if ((planes_bar && !is_dev_ptr(planes_bar)) || AU.owned || AV.owned || AW.owned)
+        HIPCHK(hipStreamSynchronize(c->st));
     return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
 }
 

@@ -34,6 +34,13 @@ __device__ __forceinline__ double rd(const double *a, const int n[3], int i, int
     return a[FIX(i, j, k, n[0], n[1])];
 }
 
+// the same without the range test, for points whose every access is in range
+template <bool SAFE>
+__device__ __forceinline__ double rdt(const double *a, const int n[3], int i, int j, int k) {
+    if (SAFE) return rd(a, n, i, j, k);
+    return a[(size_t)(i - 1) + (size_t)n[0] * ((size_t)(j - 1) + (size_t)n[1] * (size_t)(k - 1))];
+}
+
 __device__ __forceinline__ void dims_of(const Grid &g, int c, int n[3]) {
     n[0] = g.nx + 2; n[1] = g.ny + 2; n[2] = g.nz + 2;
     if (c < 3) n[c] -= 1;
@@ -104,6 +111,7 @@ struct AdjFill {
     double d;           // spacing (pressure code 3)
     PlaneLayout L;
     double *a;          // adjoint array (in place)
+    double *av[3];      // the three velocity adjoints (velocity fills)
     double *pbar;       // plane-set adjoint (accumulated)
 };
 
@@ -115,6 +123,8 @@ struct AdjFill {
 //   bc 4: a(1) = b1, a(N) = bn
 // transposed in reverse order of the assignments (a(N) first)
 __global__ void __launch_bounds__(AT) k_adj_fill(AdjFill P) {
+    if (P.comp < 3) P.comp = blockIdx.y;  // velocity: one component per blockIdx.y
+    if (P.comp < 3) P.a = P.av[P.comp];
     int n[3];
     dims_of(P.g, P.comp, n);
     const int ax = P.ax;
@@ -289,36 +299,32 @@ __global__ void __launch_bounds__(AT) k_adj_pred_p(AdjPred P) {
 }
 
 // u adjoint += nu diff^T(g) - conv^T(g; u): gathered at each face of component c
-__global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
-    const Grid g = P.g;
-    int nu_[3], nv_[3], nw_[3];
-    dims_of(g, 0, nu_);
-    dims_of(g, 1, nv_);
-    dims_of(g, 2, nw_);
+// (range-checked reads; the ghost faces use this form, the others convdiff_inner).
+template <bool SAFE>
+__device__ __forceinline__ double convdiff_at(const AdjPred &P, int c, int i, int j, int k, const int nu_[3],
+                                              const int nv_[3], const int nw_[3]) {
+    const Grid &g = P.g;
     const int *nc = (c == 0) ? nu_ : (c == 1) ? nv_ : nw_;
-    const size_t tot = (size_t)nc[0] * nc[1] * nc[2];
-    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
-    if (q >= tot) return;
-    int i, j, k;
-    unflat(q, nc, i, j, k);
     const double *U = P.u[0], *V = P.u[1], *W = P.u[2];
     const double *GU = P.gb[0], *GV = P.gb[1], *GW = P.gb[2];
     const double rx = P.rd3[0], ry = P.rd3[1], rz = P.rd3[2];
     const int nn[3] = {g.nx, g.ny, g.nz};
     const double *Gc = P.gb[c];
     const double *Xc = P.u[c];
-    int id[3] = {i, j, k};
+    const int id[3] = {i, j, k};
     // diffusion, diff2 along each axis (own axis wraps output 1's left neighbour to n when periodic)
     double dif = 0.0;
+    const double gc0 = rdt<SAFE>(Gc, nc, i, j, k);
+#pragma unroll
     for (int a = 0; a < 3; a++) {
         int m[3] = {i, j, k}, p1[3] = {i, j, k};
         m[a] -= 1;
         p1[a] += 1;
-        double s = rd(Gc, nc, m[0], m[1], m[2]) - 2.0 * rd(Gc, nc, i, j, k) + rd(Gc, nc, p1[0], p1[1], p1[2]);
+        double s = rdt<SAFE>(Gc, nc, m[0], m[1], m[2]) - 2.0 * gc0 + rdt<SAFE>(Gc, nc, p1[0], p1[1], p1[2]);
         if (a == c && P.per[c] && id[c] == nn[c]) {
             int o[3] = {i, j, k};
             o[c] = 1;
-            s += rd(Gc, nc, o[0], o[1], o[2]);
+            s += rdt<SAFE>(Gc, nc, o[0], o[1], o[2]);
         }
         dif += s * P.rd2[a];
     }
@@ -326,18 +332,18 @@ __global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
     double conv = 0.0;
     {
         const double rh = P.rd3[c];
-        auto cellbar = [&](int m) -> double {  // adjoint of a_m^2 times 2 a_m, cell between faces m and m+1
+        auto cellbar = [&](int m) -> double {  // 2 a_m times the adjoint of a_m^2 (cell between faces m, m+1)
             if (m < 1 || m > nn[c]) return 0.0;
             int f0[3] = {i, j, k}, f1[3] = {i, j, k};
             f0[c] = m;
             f1[c] = m + 1;
-            double X = (rd(Gc, nc, f0[0], f0[1], f0[2]) - rd(Gc, nc, f1[0], f1[1], f1[2])) * rh;
+            double X = (rdt<SAFE>(Gc, nc, f0[0], f0[1], f0[2]) - rdt<SAFE>(Gc, nc, f1[0], f1[1], f1[2])) * rh;
             if (P.per[c] && m == nn[c]) {  // a_n is also output 1's left average
                 int o[3] = {i, j, k};
                 o[c] = 1;
-                X -= rd(Gc, nc, o[0], o[1], o[2]) * rh;
+                X -= rdt<SAFE>(Gc, nc, o[0], o[1], o[2]) * rh;
             }
-            const double am = 0.5 * (rd(Xc, nc, f1[0], f1[1], f1[2]) + rd(Xc, nc, f0[0], f0[1], f0[2]));
+            const double am = 0.5 * (rdt<SAFE>(Xc, nc, f1[0], f1[1], f1[2]) + rdt<SAFE>(Xc, nc, f0[0], f0[1], f0[2]));
             return 2.0 * am * X;
         };
         conv += 0.5 * (cellbar(id[c]) + cellbar(id[c] - 1));
@@ -345,16 +351,19 @@ __global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
     // edge fluxes: EF (u,v) in x-y, FF (u,w) in x-z, GF (v,w) in y-z
     // EF(i,j,k) = A*B, A = (u(i,j+1)+u(i,j))/2, B = (v(i+1,j)+v(i,j))/2
     auto EFbar = [&](int a, int b, int kk) -> double {
-        return (rd(GU, nu_, a, b, kk) - rd(GU, nu_, a, b + 1, kk)) * ry + (rd(GV, nv_, a, b, kk) - rd(GV, nv_, a + 1, b, kk)) * rx;
+        return (rdt<SAFE>(GU, nu_, a, b, kk) - rdt<SAFE>(GU, nu_, a, b + 1, kk)) * ry +
+               (rdt<SAFE>(GV, nv_, a, b, kk) - rdt<SAFE>(GV, nv_, a + 1, b, kk)) * rx;
     };
     auto FFbar = [&](int a, int b, int kk) -> double {
-        return (rd(GU, nu_, a, b, kk) - rd(GU, nu_, a, b, kk + 1)) * rz + (rd(GW, nw_, a, b, kk) - rd(GW, nw_, a + 1, b, kk)) * rx;
+        return (rdt<SAFE>(GU, nu_, a, b, kk) - rdt<SAFE>(GU, nu_, a, b, kk + 1)) * rz +
+               (rdt<SAFE>(GW, nw_, a, b, kk) - rdt<SAFE>(GW, nw_, a + 1, b, kk)) * rx;
     };
     auto GFbar = [&](int a, int b, int kk) -> double {
-        return (rd(GV, nv_, a, b, kk) - rd(GV, nv_, a, b, kk + 1)) * rz + (rd(GW, nw_, a, b, kk) - rd(GW, nw_, a, b + 1, kk)) * ry;
+        return (rdt<SAFE>(GV, nv_, a, b, kk) - rdt<SAFE>(GV, nv_, a, b, kk + 1)) * rz +
+               (rdt<SAFE>(GW, nw_, a, b, kk) - rdt<SAFE>(GW, nw_, a, b + 1, kk)) * ry;
     };
     auto avg = [&](const double *A, const int *n, int a0, int b0, int c0, int a1, int b1, int c1) {
-        return 0.5 * (rd(A, n, a1, b1, c1) + rd(A, n, a0, b0, c0));
+        return 0.5 * (rdt<SAFE>(A, n, a1, b1, c1) + rdt<SAFE>(A, n, a0, b0, c0));
     };
     if (c == 0) {
         // u(i,j) in A(i,j) and A(i,j-1) of EF; in A(i,j,k), A(i,j,k-1) of FF
@@ -369,7 +378,93 @@ __global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
         conv += 0.5 * (FFbar(i, j, k) * avg(U, nu_, i, j, k, i, j, k + 1) + FFbar(i - 1, j, k) * avg(U, nu_, i - 1, j, k, i - 1, j, k + 1));
         conv += 0.5 * (GFbar(i, j, k) * avg(V, nv_, i, j, k, i, j, k + 1) + GFbar(i, j - 1, k) * avg(V, nv_, i, j - 1, k, i, j - 1, k + 1));
     }
-    P.ub[c][q] += P.nu * dif - conv;
+    return P.nu * dif - conv;
+}
+
+// The same sum for a non-ghost face, with one linear base index per array and
+// constant neighbour strides (every access is inside the arrays there).
+__device__ __forceinline__ double convdiff_inner(const AdjPred &P, int c, int i, int j, int k, const int nu_[3],
+                                                 const int nv_[3], const int nw_[3]) {
+    const double *U = P.u[0], *V = P.u[1], *W = P.u[2];
+    const double *GU = P.gb[0], *GV = P.gb[1], *GW = P.gb[2];
+    const double rx = P.rd3[0], ry = P.rd3[1], rz = P.rd3[2];
+    const long su1 = nu_[0], su2 = (long)nu_[0] * nu_[1];
+    const long sv1 = nv_[0], sv2 = (long)nv_[0] * nv_[1];
+    const long sw1 = nw_[0], sw2 = (long)nw_[0] * nw_[1];
+    const long qu = (i - 1) + su1 * (j - 1) + su2 * (k - 1);
+    const long qv = (i - 1) + sv1 * (j - 1) + sv2 * (k - 1);
+    const long qw = (i - 1) + sw1 * (j - 1) + sw2 * (k - 1);
+    double dif, conv;
+    if (c == 0) {
+        dif = (GU[qu - 1] - 2.0 * GU[qu] + GU[qu + 1]) * P.rd2[0] + (GU[qu - su1] - 2.0 * GU[qu] + GU[qu + su1]) * P.rd2[1] +
+              (GU[qu - su2] - 2.0 * GU[qu] + GU[qu + su2]) * P.rd2[2];
+        double X1 = (GU[qu] - GU[qu + 1]) * rx;
+        if (P.per[0] && i == P.g.nx) {
+            const double g1 = GU[qu - (i - 1)];
+            dif += g1 * P.rd2[0];
+            X1 -= g1 * rx;
+        }
+        const double X0 = (GU[qu - 1] - GU[qu]) * rx;
+        conv = 0.5 * (2.0 * (0.5 * (U[qu + 1] + U[qu])) * X1 + 2.0 * (0.5 * (U[qu] + U[qu - 1])) * X0);
+        const double ef1 = (GU[qu] - GU[qu + su1]) * ry + (GV[qv] - GV[qv + 1]) * rx;
+        const double ef0 = (GU[qu - su1] - GU[qu]) * ry + (GV[qv - sv1] - GV[qv - sv1 + 1]) * rx;
+        conv += 0.5 * (ef1 * (0.5 * (V[qv + 1] + V[qv])) + ef0 * (0.5 * (V[qv - sv1 + 1] + V[qv - sv1])));
+        const double ff1 = (GU[qu] - GU[qu + su2]) * rz + (GW[qw] - GW[qw + 1]) * rx;
+        const double ff0 = (GU[qu - su2] - GU[qu]) * rz + (GW[qw - sw2] - GW[qw - sw2 + 1]) * rx;
+        conv += 0.5 * (ff1 * (0.5 * (W[qw + 1] + W[qw])) + ff0 * (0.5 * (W[qw - sw2 + 1] + W[qw - sw2])));
+    } else if (c == 1) {
+        dif = (GV[qv - 1] - 2.0 * GV[qv] + GV[qv + 1]) * P.rd2[0] + (GV[qv - sv1] - 2.0 * GV[qv] + GV[qv + sv1]) * P.rd2[1] +
+              (GV[qv - sv2] - 2.0 * GV[qv] + GV[qv + sv2]) * P.rd2[2];
+        double X1 = (GV[qv] - GV[qv + sv1]) * ry;
+        if (P.per[1] && j == P.g.ny) {
+            const double g1 = GV[qv - (long)(j - 1) * sv1];
+            dif += g1 * P.rd2[1];
+            X1 -= g1 * ry;
+        }
+        const double X0 = (GV[qv - sv1] - GV[qv]) * ry;
+        conv = 0.5 * (2.0 * (0.5 * (V[qv + sv1] + V[qv])) * X1 + 2.0 * (0.5 * (V[qv] + V[qv - sv1])) * X0);
+        const double ef1 = (GU[qu] - GU[qu + su1]) * ry + (GV[qv] - GV[qv + 1]) * rx;
+        const double ef0 = (GU[qu - 1] - GU[qu - 1 + su1]) * ry + (GV[qv - 1] - GV[qv]) * rx;
+        conv += 0.5 * (ef1 * (0.5 * (U[qu + su1] + U[qu])) + ef0 * (0.5 * (U[qu - 1 + su1] + U[qu - 1])));
+        const double gf1 = (GV[qv] - GV[qv + sv2]) * rz + (GW[qw] - GW[qw + sw1]) * ry;
+        const double gf0 = (GV[qv - sv2] - GV[qv]) * rz + (GW[qw - sw2] - GW[qw - sw2 + sw1]) * ry;
+        conv += 0.5 * (gf1 * (0.5 * (W[qw + sw1] + W[qw])) + gf0 * (0.5 * (W[qw - sw2 + sw1] + W[qw - sw2])));
+    } else {
+        dif = (GW[qw - 1] - 2.0 * GW[qw] + GW[qw + 1]) * P.rd2[0] + (GW[qw - sw1] - 2.0 * GW[qw] + GW[qw + sw1]) * P.rd2[1] +
+              (GW[qw - sw2] - 2.0 * GW[qw] + GW[qw + sw2]) * P.rd2[2];
+        double X1 = (GW[qw] - GW[qw + sw2]) * rz;
+        if (P.per[2] && k == P.g.nz) {
+            const double g1 = GW[qw - (long)(k - 1) * sw2];
+            dif += g1 * P.rd2[2];
+            X1 -= g1 * rz;
+        }
+        const double X0 = (GW[qw - sw2] - GW[qw]) * rz;
+        conv = 0.5 * (2.0 * (0.5 * (W[qw + sw2] + W[qw])) * X1 + 2.0 * (0.5 * (W[qw] + W[qw - sw2])) * X0);
+        const double ff1 = (GU[qu] - GU[qu + su2]) * rz + (GW[qw] - GW[qw + 1]) * rx;
+        const double ff0 = (GU[qu - 1] - GU[qu - 1 + su2]) * rz + (GW[qw - 1] - GW[qw]) * rx;
+        conv += 0.5 * (ff1 * (0.5 * (U[qu + su2] + U[qu])) + ff0 * (0.5 * (U[qu - 1 + su2] + U[qu - 1])));
+        const double gf1 = (GV[qv] - GV[qv + sv2]) * rz + (GW[qw] - GW[qw + sw1]) * ry;
+        const double gf0 = (GV[qv - sv1] - GV[qv - sv1 + sv2]) * rz + (GW[qw - sw1] - GW[qw]) * ry;
+        conv += 0.5 * (gf1 * (0.5 * (V[qv + sv2] + V[qv])) + gf0 * (0.5 * (V[qv - sv1 + sv2] + V[qv - sv1])));
+    }
+    return P.nu * dif - conv;
+}
+
+// one launch per component (a launch for all three at once measured slower)
+__global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
+    const Grid g = P.g;
+    int nu_[3], nv_[3], nw_[3];
+    dims_of(g, 0, nu_);
+    dims_of(g, 1, nv_);
+    dims_of(g, 2, nw_);
+    const int *nc = (c == 0) ? nu_ : (c == 1) ? nv_ : nw_;
+    const size_t tot = (size_t)nc[0] * nc[1] * nc[2];
+    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (q >= tot) return;
+    int i, j, k;
+    unflat(q, nc, i, j, k);
+    const bool inner = i >= 2 && i <= nc[0] - 1 && j >= 2 && j <= nc[1] - 1 && k >= 2 && k <= nc[2] - 1;
+    P.ub[c][q] += inner ? convdiff_inner(P, c, i, j, k, nu_, nv_, nw_) : convdiff_at<true>(P, c, i, j, k, nu_, nv_, nw_);
 }
 
 __global__ void __launch_bounds__(AT) k_block_sum(const double *a, size_t n, double *partial) {
@@ -389,6 +484,21 @@ __global__ void k_add_at(double *a, size_t q, double v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) a[q] += v;
 }
 
+// a[q] += scale * (sum of the nb block partials), summed in a fixed order
+__global__ void __launch_bounds__(AT) k_add_partials(double *a, size_t q, const double *partial, int nb,
+                                                     double scale) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += AT) s += partial[i];
+    __shared__ double red[AT];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = AT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a[q] += scale * red[0];
+}
+
 }  // namespace
 
 int launch_block_sums(const double *a, size_t n, double *partial, int max_blocks, hipStream_t st) {
@@ -397,6 +507,10 @@ int launch_block_sums(const double *a, size_t n, double *partial, int max_blocks
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_block_sum, dim3((unsigned)nb), dim3(AT), 0, st, a, n, partial);
     return (int)nb;
+}
+
+void launch_add_partials(double *a, size_t q, const double *partial, int nb, double scale, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_partials, dim3(1), dim3(AT), 0, st, a, q, partial, nb, scale);
 }
 
 void launch_add_at(double *a, size_t q, double v, hipStream_t st) {
@@ -417,27 +531,28 @@ void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag
     P.pb = pb;
     P.dpb = dpb;
     const size_t n[3] = {g.nu(), g.nv(), g.nw()};
-    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_proj_vel, dim3(ablocks(n[c])), dim3(AT), 0, st, P, c);
+    for (int c = 0; c < 3; c++)
+        if (usb[c] != ub[c]) hipLaunchKernelGGL(k_adj_proj_vel, dim3(ablocks(n[c])), dim3(AT), 0, st, P, c);
     hipLaunchKernelGGL(k_adj_proj_dp, dim3(ablocks(g.np())), dim3(AT), 0, st, P);
 }
 
-void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3], double *a, double *pbar,
+void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3], double *const a[3], double *pbar,
                      hipStream_t st) {
     AdjFill P;
     P.g = g;
     P.comp = comp;
     P.L = plane_layout(g.nx, g.ny, g.nz);
-    P.a = a;
+    P.a = a[0];
+    for (int c = 0; c < 3; c++) P.av[c] = a[c];
     P.pbar = pbar;
-    int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
-    if (comp < 3) n[comp] -= 1;
+    const int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};  // the largest plane of any component
     for (int ax = 2; ax >= 0; ax--) {  // forward order x, y, z
         P.ax = ax;
         P.bc = bc[ax];
         P.d = h[ax];
         if (bc[ax] < 1 || bc[ax] > 4) continue;
         const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
-        hipLaunchKernelGGL(k_adj_fill, dim3(ablocks((size_t)n[o1] * n[o2])), dim3(AT), 0, st, P);
+        hipLaunchKernelGGL(k_adj_fill, dim3(ablocks((size_t)n[o1] * n[o2]), comp < 3 ? 3 : 1), dim3(AT), 0, st, P);
     }
 }
 

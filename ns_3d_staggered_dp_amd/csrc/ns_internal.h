@@ -140,8 +140,9 @@ int launch_err_partials(const Grid &g, double dx, double dy, const double *const
 // ---- discrete adjoint of an explicit step (ns_adjoint.hip) ----
 void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag, const double *const ub[3],
                         double *const usb[3], double *pb, double *dpb, hipStream_t st);
-// ghost-fill transpose of one component (0..2 velocity with bc, 3 pressure with pbc), axes z, y, x
-void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3], double *a, double *pbar,
+// ghost-fill transpose, axes z, y, x: comp 3 = the pressure array a[0] (codes pbc),
+// comp 0 = the three velocity arrays a[0..2] (codes bc), one launch per axis
+void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3], double *const a[3], double *pbar,
                      hipStream_t st);
 void launch_adj_take_interior(const Grid &g, const double *dpb, double *r, hipStream_t st);
 void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
@@ -151,6 +152,8 @@ void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, i
                         double *const hb[3], double *const gb[3], double *const ub[3], double *pb, hipStream_t st);
 int launch_block_sums(const double *a, size_t n, double *partial, int max_blocks, hipStream_t st);
 void launch_add_at(double *a, size_t q, double v, hipStream_t st);
+// a[q] += scale * sum(partial[0..nb)), on the device
+void launch_add_partials(double *a, size_t q, const double *partial, int nb, double scale, hipStream_t st);
 
 // ---- ADI (ns_adi.hip) ----
 void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,

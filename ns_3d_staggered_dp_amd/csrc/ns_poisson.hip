@@ -748,9 +748,10 @@ __global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
     }
 }
 
-// lines per block so that the two LDS line buffers stay within 64 KiB
+// lines per block of the DCT passes: two LDS line buffers of ~16 KiB each
+// (4 resident blocks per CU)
 inline int lines_per_block(int LS) {
-    int L = 2048 / LS;
+    int L = 1024 / LS;
     return L < 1 ? 1 : L;
 }
 inline size_t lds_bytes(int L, int LS) { return (size_t)2 * L * LS * sizeof(double2); }
