@@ -187,6 +187,21 @@ int ns_adjoint_get_state(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, dou
  * (host or device, plane-set length, may be null) receives the gradient with
  * respect to that step's plane set (mean_dp included). */
 int ns_adjoint_step(ns_ctx *ctx, const double *u, const double *v, const double *w, int first, double *planes_b);
+
+/* ---- batched re-simulation (no counterpart in the reference, which runs one
+ * sub-box per process): many contexts stepped together, each kernel of the
+ * step launched once for all boxes (a block finds its box from a prefix of
+ * per-box block counts), steady steps replayed from one HIP graph ----
+ * The contexts must share scheme (Euler or AB2), bc/pbc codes, dp_flag and
+ * flags, project every step (dp_flag = 1 or NS_FLAG_EAGER_PROJECTION) and use
+ * the DCT Poisson solver (walls); sizes may differ.  Each context keeps its
+ * own state and current plane set; ns_batch_step(b, n) leaves every context
+ * as ns_step(ctx, n) would.  A batch holds its contexts' device pointers:
+ * destroy it before any of them. */
+typedef struct ns_batch ns_batch;
+int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out);
+int ns_batch_destroy(ns_batch *b);
+int ns_batch_step(ns_batch *b, int nsteps);
 int ns_capture_end(ns_ctx *ctx, int *nsets_written);
 /* max |div u| over interior cells (big_periodic_3D.f90:576-579) */
 int ns_max_div(ns_ctx *ctx, double *out);

@@ -75,6 +75,9 @@ SIGNATURES = {
     "ns_adjoint_set_state": [P, P, P, P, P, P, P, P],
     "ns_adjoint_get_state": [P, P, P, P, P, P, P, P],
     "ns_adjoint_step": [P, P, P, P, I, P],
+    "ns_batch_create": [P, I, P],
+    "ns_batch_destroy": [P],
+    "ns_batch_step": [P, I],
     "ns_history_save": [P],
     "ns_history_restore": [P],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],

@@ -1440,3 +1440,194 @@ int ns_project(ns_ctx *c, const double *u_star, const double *v_star, const doub
 }
 
 }  // extern "C"
+
+// ---- batched steps of many contexts: one launch per kernel for all boxes ----
+
+struct ns_batch {
+    std::vector<ns_ctx *> ctx;
+    hipStream_t st = nullptr;
+    // one launch = per-box parameter array + block prefix (device), block total
+    struct Phase {
+        int kind = 0;    // 0 predictor, 1 velocity fill, 2 DCT pass, 3 pressure fill, 4 projection
+        int arg = 0;     // axis or pass
+        void *params = nullptr;
+        int *prefix = nullptr;
+        int nblk = 0;
+        size_t lds = 0;
+    };
+    std::vector<Phase> phases;  // one steady (non-first) step
+    hipGraphExec_t graph = nullptr;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+template <typename T>
+int balloc(ns_batch *b, T **p, size_t bytes) {
+    void *q = nullptr;
+    if (hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return NS_ERR_NOMEM;
+    }
+    b->allocs.push_back(q);
+    *p = (T *)q;
+    return NS_OK;
+}
+
+// parameters of every box for one launch, uploaded with their block prefix
+int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_batch::Phase> &out) {
+    const int n = (int)b->ctx.size();
+    size_t psz = kind == 0 ? predict_params_size() : kind == 2 ? pass_params_size()
+                           : kind == 4 ? proj_params_size() : bc_params_size();
+    std::vector<char> host((size_t)n * psz);
+    std::vector<int> prefix(n + 1, 0);
+    size_t lds = 0;
+    for (int i = 0; i < n; i++) {
+        ns_ctx *c = b->ctx[i];
+        const ns_config &f = c->cfg;
+        const Grid &g = c->g;
+        const double d[3] = {f.dx, f.dy, f.dz};
+        void *dst = host.data() + (size_t)i * psz;
+        int nb = 0;
+        if (kind == 0) {
+            nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr, c->p,
+                                     c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst);
+        } else if (kind == 1) {
+            nb = make_vel_bc_params(g, f.bc, c->planes, c->us, c->vs, c->ws, arg, dst);
+        } else if (kind == 2) {
+            size_t l = 0;
+            nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, c->us, c->vs, c->ws, nullptr, c->planes,
+                                      c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l);
+            lds = l > lds ? l : lds;
+        } else if (kind == 3) {
+            nb = make_pr_bc_params(g, f.pbc, c->planes, d, c->dp, arg, dst);
+        } else {
+            const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
+            nb = make_project_params(g, d, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
+                                     dst);
+        }
+        prefix[i + 1] = prefix[i] + nb;
+    }
+    if (prefix[n] == 0) return NS_OK;
+    ns_batch::Phase ph;
+    ph.kind = kind;
+    ph.arg = arg;
+    ph.nblk = prefix[n];
+    ph.lds = lds;
+    int rc;
+    if ((rc = balloc(b, &ph.params, host.size())) || (rc = balloc(b, &ph.prefix, prefix.size() * sizeof(int))))
+        return rc;
+    HIPCHK(hipMemcpy(ph.params, host.data(), host.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ph.prefix, prefix.data(), prefix.size() * sizeof(int), hipMemcpyHostToDevice));
+    out.push_back(ph);
+    return NS_OK;
+}
+
+int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out) {
+    int rc;
+    if ((rc = add_phase(b, 0, 0, first_step, out))) return rc;
+    for (int ax = 0; ax < 3; ax++)
+        if ((rc = add_phase(b, 1, ax, first_step, out))) return rc;
+    for (int pass = 0; pass < 5; pass++)
+        if ((rc = add_phase(b, 2, pass, first_step, out))) return rc;
+    for (int ax = 0; ax < 3; ax++)
+        if ((rc = add_phase(b, 3, ax, first_step, out))) return rc;
+    return add_phase(b, 4, 0, first_step, out);
+}
+
+void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
+    const int n = (int)b->ctx.size();
+    for (const auto &ph : phases) {
+        switch (ph.kind) {
+            case 0: launch_predict_batch(false, ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 1: launch_vel_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 2: launch_dct_batch(ph.arg, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
+            case 3: launch_pr_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
+    if (!ctxs || n < 1 || !out) return NS_ERR_INVALID;
+    const ns_config &f0 = ctxs[0]->cfg;
+    for (int i = 0; i < n; i++) {
+        const ns_ctx *c = ctxs[i];
+        if (!c) return NS_ERR_INVALID;
+        const ns_config &f = c->cfg;
+        // one kernel family per launch: same scheme and codes, projected every step, DCT Poisson
+        if (c->lazy || f.scheme == NS_AB2_CN || !c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
+        if (f.scheme != f0.scheme || f.dp_flag != f0.dp_flag || (f.flags & ~NS_FLAG_NO_GRAPH) !=
+            (f0.flags & ~NS_FLAG_NO_GRAPH))
+            return NS_ERR_UNSUPPORTED;
+        for (int a = 0; a < 3; a++)
+            if (f.bc[a] != f0.bc[a] || f.pbc[a] != f0.pbc[a]) return NS_ERR_UNSUPPORTED;
+    }
+    ns_batch *b = new ns_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return NS_ERR_HIP;
+    }
+    int rc = build_phases(b, false, b->phases);
+    if (rc) {
+        ns_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return NS_OK;
+}
+
+int ns_batch_destroy(ns_batch *b) {
+    if (!b) return NS_ERR_INVALID;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (void *q : b->allocs) (void)hipFree(q);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+    return NS_OK;
+}
+
+int ns_batch_step(ns_batch *b, int nsteps) {
+    if (!b || nsteps < 0) return NS_ERR_INVALID;
+    // work queued on the contexts' own streams (uploads, plane sets) comes first
+    for (ns_ctx *c : b->ctx) HIPCHK(hipStreamSynchronize(c->st));
+    int s = 0;
+    bool any_first = false;
+    for (ns_ctx *c : b->ctx) any_first = any_first || c->first;
+    if (any_first && nsteps > 0) {
+        // the first step (Euler for the boxes that start fresh) runs from its own parameters
+        std::vector<ns_batch::Phase> ph;
+        int rc = build_phases(b, true, ph);
+        if (rc) return rc;
+        run_phases(b, ph);
+        HIPCHK(hipStreamSynchronize(b->st));
+        for (ns_ctx *c : b->ctx) c->first = 0;
+        s = 1;
+    }
+    const bool use_graph = !(b->ctx[0]->cfg.flags & NS_FLAG_NO_GRAPH);
+    if (use_graph && !b->graph && s < nsteps) {
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            run_phases(b, b->phases);
+            if (hipStreamEndCapture(b->st, &gr) == hipSuccess && gr) {
+                if (hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0) != hipSuccess) b->graph = nullptr;
+            }
+            if (gr) (void)hipGraphDestroy(gr);
+        }
+        (void)hipGetLastError();
+    }
+    for (; s < nsteps; s++) {
+        if (use_graph && b->graph) HIPCHK(hipGraphLaunch(b->graph, b->st));
+        else run_phases(b, b->phases);
+    }
+    HIPCHK(hipStreamSynchronize(b->st));
+    return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
+}
+
+}  // extern "C"

@@ -66,6 +66,44 @@ struct Ctx;
 void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                           const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
                           double *us, double *vs, double *ws, hipStream_t st);
+
+// ---- batched launches: many boxes, one launch per kernel ----
+// box of block blk: the last i with prefix[i] <= blk (prefix ascending, prefix[0] = 0)
+__device__ __forceinline__ int batch_box(const int *prefix, int nbox, int blk) {
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// predictor parameters of one box written to out (predict_params_size() bytes); returns its block count
+size_t predict_params_size();
+int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+                        const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
+                        double *us, double *vs, double *ws, void *out);
+void launch_predict_batch(bool lazy, const void *params, const int *prefix, int nbox, int nblocks, hipStream_t st);
+size_t bc_params_size();
+size_t proj_params_size();
+// fill / projection parameters of one box; return the block count (0: nothing to launch)
+int make_vel_bc_params(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w, int ax,
+                       void *out);
+int make_pr_bc_params(const Grid &g, const int pbc[3], const double *planes, const double d[3], double *dp, int ax,
+                      void *out);
+int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag, const double *us, const double *vs,
+                        const double *ws, const double *dp, const double *gauge, double *u, double *v, double *w,
+                        double *p, void *out);
+void launch_vel_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
+void launch_pr_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
+void launch_project_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
+struct PoissonPlans;
+size_t pass_params_size();
+int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
+                         const double *us, const double *vs, const double *ws, const double *rhs_in,
+                         const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
+                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds);
+void launch_dct_batch(int pass, const void *params, const int *prefix, int nbox, int nb, size_t lds, hipStream_t st);
 void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff,
                       const double *u, const double *v, const double *w, double *ox, double *oy, double *oz,
                       hipStream_t st);

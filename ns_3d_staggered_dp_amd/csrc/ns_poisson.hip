@@ -586,12 +586,11 @@ __device__ __forceinline__ int dct_n(const Grid &g) { return AXIS == 0 ? g.nx : 
 
 // forward DCT along AXIS; AXIS 0 reads the divergence (+ boundary RHS terms)
 template <int AXIS, int SOLVE>
-__global__ void __launch_bounds__(256) k_dct_fwd(PassParams P) {
+__device__ __forceinline__ void dct_fwd_body(const PassParams P, const int blk) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
     const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;
     double2 *A = smem, *B = smem + (size_t)L * LS;
-    const int blk = blockIdx.x;
     const bool dst = (P.pbc == 2);
     // load 2L real lines, reorder into v, pack pairs as complex
     for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
@@ -692,12 +691,11 @@ __global__ void __launch_bounds__(256) k_dct_fwd(PassParams P) {
 }
 
 template <int AXIS>
-__global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
+__device__ __forceinline__ void dct_inv_body(const PassParams P, const int blk) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
     const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;
     double2 *A = smem, *B = smem + (size_t)L * LS;
-    const int blk = blockIdx.x;
     const bool dst = (P.pbc == 2);
     for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
         int rline, pos;
@@ -746,6 +744,27 @@ __global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
             P.work[base + stride * pos] = val;
         }
     }
+}
+
+template <int AXIS, int SOLVE>
+__global__ void __launch_bounds__(256) k_dct_fwd(PassParams P) {
+    dct_fwd_body<AXIS, SOLVE>(P, blockIdx.x);
+}
+template <int AXIS>
+__global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
+    dct_inv_body<AXIS>(P, blockIdx.x);
+}
+template <int AXIS, int SOLVE>
+__global__ void __launch_bounds__(256) k_dct_fwd_batch(const PassParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                       int nbox) {
+    const int box = batch_box(prefix, nbox, blockIdx.x);
+    dct_fwd_body<AXIS, SOLVE>(Ps[box], blockIdx.x - prefix[box]);
+}
+template <int AXIS>
+__global__ void __launch_bounds__(256) k_dct_inv_batch(const PassParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                       int nbox) {
+    const int box = batch_box(prefix, nbox, blockIdx.x);
+    dct_inv_body<AXIS>(Ps[box], blockIdx.x - prefix[box]);
 }
 
 // lines per block of the DCT passes: two LDS line buffers of ~16 KiB each
@@ -847,11 +866,15 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     if (ev) (void)hipEventRecord(ev[4], st);
 }
 
-void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
-                       const double *us, const double *vs, const double *ws, const double *rhs_in,
-                       const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
-                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
-    PassParams P{};
+// parameters of DCT pass `pass` (0 x forward + divergence, 1 y forward, 2 z
+// forward + modal solve + z inverse, 3 y inverse, 4 x inverse); returns the
+// block count and the dynamic LDS size
+int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
+                         const double *us, const double *vs, const double *ws, const double *rhs_in,
+                         const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
+                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds) {
+    PassParams &P = *(PassParams *)out;
+    P = PassParams{};
     P.g = g;
     P.src.rhs = rhs_in;
     P.src.us = us; P.src.vs = vs; P.src.ws = ws;
@@ -866,38 +889,57 @@ void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     P.dp_out = dp_out;
     P.sol_out = sol_out;
     const long RL = (long)g.ny * g.nz;
-    auto nb_x = [&](int L) { return (unsigned)((((RL + 1) / 2) + L - 1) / L); };
-    auto nb_yz = [&](int L, int others) { return (unsigned)(((((g.nx + 1) / 2) + L - 1) / L) * others); };
-    // x forward
-    P.pl = pl.px; P.wq = pl.wq_x; P.pbc = pl.pbc[0];
-    P.LS = g.nx + 1;
+    const int ax = (pass == 0 || pass == 4) ? 0 : (pass == 2) ? 2 : 1;
+    const FFTPlan plans[3] = {pl.px, pl.py, pl.pz};
+    const double2 *wq[3] = {pl.wq_x, pl.wq_y, pl.wq_z};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    P.pl = plans[ax];
+    P.wq = wq[ax];
+    P.pbc = pl.pbc[ax];
+    P.LS = n[ax] + 1;
     P.L = lines_per_block(P.LS);
-    hipLaunchKernelGGL((k_dct_fwd<0, 0>), dim3(nb_x(P.L)), dim3(256), lds_bytes(P.L, P.LS), st, P);
-    if (ev) (void)hipEventRecord(ev[0], st);
-    // y forward
-    P.pl = pl.py; P.wq = pl.wq_y; P.pbc = pl.pbc[1];
-    P.LS = g.ny + 1;
-    P.L = lines_per_block(P.LS);
-    hipLaunchKernelGGL((k_dct_fwd<1, 0>), dim3(nb_yz(P.L, g.nz)), dim3(256), lds_bytes(P.L, P.LS), st, P);
-    if (ev) (void)hipEventRecord(ev[1], st);
-    // z forward + divide + inverse
-    P.pl = pl.pz; P.wq = pl.wq_z; P.pbc = pl.pbc[2];
-    P.LS = g.nz + 1;
-    P.L = lines_per_block(P.LS);
-    hipLaunchKernelGGL((k_dct_fwd<2, 1>), dim3(nb_yz(P.L, g.ny)), dim3(256), lds_bytes(P.L, P.LS), st, P);
-    if (ev) (void)hipEventRecord(ev[2], st);
-    // y inverse
-    P.pl = pl.py; P.wq = pl.wq_y; P.pbc = pl.pbc[1];
-    P.LS = g.ny + 1;
-    P.L = lines_per_block(P.LS);
-    hipLaunchKernelGGL((k_dct_inv<1>), dim3(nb_yz(P.L, g.nz)), dim3(256), lds_bytes(P.L, P.LS), st, P);
-    if (ev) (void)hipEventRecord(ev[3], st);
-    // x inverse
-    P.pl = pl.px; P.wq = pl.wq_x; P.pbc = pl.pbc[0];
-    P.LS = g.nx + 1;
-    P.L = lines_per_block(P.LS);
-    hipLaunchKernelGGL((k_dct_inv<0>), dim3(nb_x(P.L)), dim3(256), lds_bytes(P.L, P.LS), st, P);
-    if (ev) (void)hipEventRecord(ev[4], st);
+    *lds = lds_bytes(P.L, P.LS);
+    if (ax == 0) return (int)((((RL + 1) / 2) + P.L - 1) / P.L);
+    const int others = (ax == 1) ? g.nz : g.ny;
+    return (int)(((((g.nx + 1) / 2) + P.L - 1) / P.L) * others);
+}
+
+size_t pass_params_size() { return sizeof(PassParams); }
+
+void launch_dct_pass(int pass, const void *params, int nb, size_t lds, hipStream_t st) {
+    const PassParams &P = *(const PassParams *)params;
+    switch (pass) {
+        case 0: hipLaunchKernelGGL((k_dct_fwd<0, 0>), dim3(nb), dim3(256), lds, st, P); break;
+        case 1: hipLaunchKernelGGL((k_dct_fwd<1, 0>), dim3(nb), dim3(256), lds, st, P); break;
+        case 2: hipLaunchKernelGGL((k_dct_fwd<2, 1>), dim3(nb), dim3(256), lds, st, P); break;
+        case 3: hipLaunchKernelGGL((k_dct_inv<1>), dim3(nb), dim3(256), lds, st, P); break;
+        case 4: hipLaunchKernelGGL((k_dct_inv<0>), dim3(nb), dim3(256), lds, st, P); break;
+    }
+}
+
+void launch_dct_batch(int pass, const void *params, const int *prefix, int nbox, int nb, size_t lds, hipStream_t st) {
+    const PassParams *Ps = (const PassParams *)params;
+    switch (pass) {
+        case 0: hipLaunchKernelGGL((k_dct_fwd_batch<0, 0>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
+        case 1: hipLaunchKernelGGL((k_dct_fwd_batch<1, 0>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
+        case 2: hipLaunchKernelGGL((k_dct_fwd_batch<2, 1>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
+        case 3: hipLaunchKernelGGL((k_dct_inv_batch<1>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
+        case 4: hipLaunchKernelGGL((k_dct_inv_batch<0>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
+    }
+}
+
+void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
+                       const double *us, const double *vs, const double *ws, const double *rhs_in,
+                       const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
+    for (int pass = 0; pass < 5; pass++) {
+        PassParams P;
+        size_t lds = 0;
+        const int nb = make_dct_pass_params(g, pl, dx, dy, dz, dt, us, vs, ws, rhs_in, planes, mean_dp_dev,
+                                            mean_dp_host, work, dp_out, sol_out, pass, &P, &lds);
+        launch_dct_pass(pass, &P, nb, lds, st);
+        if (ev) (void)hipEventRecord(ev[pass], st);
+    }
 }
 
 }  // namespace ns

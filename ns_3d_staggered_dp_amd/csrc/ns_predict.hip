@@ -118,12 +118,12 @@ struct Tiles {
     double d[LAZY ? NDS : 1][DYT][DXT];
 };
 
+// one block's column of tiles; b is the block's index within its box
 template <bool LAZY>
-__global__ void __launch_bounds__(NT, 3) k_predict_tiled(TileParams P) {
+__device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
     __shared__ Tiles<LAZY> T;
     // XCD-aware tile order: consecutive tiles of one XCD are x/y neighbours
     const int nb = P.ntiles;
-    const int b = blockIdx.x;
     const int per_xcd = (nb + 7) / 8;
     const int t = (b % 8) * per_xcd + b / 8;
     if (t >= nb) return;  // only when nb is not a multiple of 8: padding blocks exit at once
@@ -454,6 +454,19 @@ __global__ void __launch_bounds__(NT, 3) k_predict_tiled(TileParams P) {
     }
 }
 
+template <bool LAZY>
+__global__ void __launch_bounds__(NT, 3) k_predict_tiled(TileParams P) {
+    predict_tile<LAZY>(P, blockIdx.x);
+}
+
+// many boxes in one launch: prefix[i] is the first block of box i
+template <bool LAZY>
+__global__ void __launch_bounds__(NT, 3) k_predict_tiled_batch(const TileParams *__restrict__ Ps,
+                                                               const int *__restrict__ prefix, int nbox) {
+    const int box = batch_box(prefix, nbox, blockIdx.x);
+    predict_tile<LAZY>(Ps[box], blockIdx.x - prefix[box]);
+}
+
 }  // namespace
 
 namespace {
@@ -495,10 +508,12 @@ int choose_zc(int nz, long txy, int slots) {
 
 }  // namespace
 
-void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
-                          const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                          double *us, double *vs, double *ws, hipStream_t st) {
-    TileParams P;
+size_t predict_params_size() { return sizeof(TileParams); }
+
+int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+                        const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
+                        double *us, double *vs, double *ws, void *out) {
+    TileParams &P = *(TileParams *)out;
     P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
     P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
     const double d[3] = {c.dx, c.dy, c.dz};
@@ -518,7 +533,22 @@ void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double
     P.p = p;
     P.h[0] = hx; P.h[1] = hy; P.h[2] = hz;
     P.out[0] = us; P.out[1] = vs; P.out[2] = ws;
-    const int nblk = ((P.ntiles + 7) / 8) * 8;
+    return ((P.ntiles + 7) / 8) * 8;
+}
+
+void launch_predict_batch(bool lazy, const void *params, const int *prefix, int nbox, int nblocks, hipStream_t st) {
+    const TileParams *Ps = (const TileParams *)params;
+    if (lazy)
+        hipLaunchKernelGGL(k_predict_tiled_batch<true>, dim3(nblocks), dim3(TX * TY), 0, st, Ps, prefix, nbox);
+    else
+        hipLaunchKernelGGL(k_predict_tiled_batch<false>, dim3(nblocks), dim3(TX * TY), 0, st, Ps, prefix, nbox);
+}
+
+void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+                          const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
+                          double *us, double *vs, double *ws, hipStream_t st) {
+    TileParams P;
+    const int nblk = make_predict_params(c, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P);
     if (lazy)
         hipLaunchKernelGGL(k_predict_tiled<true>, dim3(nblk), dim3(TX * TY), 0, st, P);
     else

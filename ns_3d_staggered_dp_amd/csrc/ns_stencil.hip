@@ -169,13 +169,11 @@ __device__ __forceinline__ void other_axes(int ax, int &o1, int &o2) {
 
 // velocity ghost/wall fill along one axis (NS_functions.f90:235-308), all three
 // components; blockIdx.y selects the component
-__global__ void __launch_bounds__(256) k_vel_bc_axis(BCParams P) {
+__device__ __forceinline__ void vel_bc_body(const BCParams P, const size_t t, const int c) {
     Grid g{P.nx, P.ny, P.nz};
     int o1, o2;
     other_axes(P.ax, o1, o2);
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     {
-        const int c = blockIdx.y;
         int n[3];
         comp_dims(g, c, n);
         size_t tot = (size_t)n[o1] * n[o2];
@@ -212,6 +210,16 @@ __global__ void __launch_bounds__(256) k_vel_bc_axis(BCParams P) {
     }
 }
 
+__global__ void __launch_bounds__(256) k_vel_bc_axis(BCParams P) {
+    vel_bc_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) k_vel_bc_axis_batch(const BCParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                           int nbox) {
+    const int box = batch_box(prefix, nbox, blockIdx.x);
+    vel_bc_body(Ps[box], (size_t)(blockIdx.x - prefix[box]) * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
 // boundary rows before an ADI sweep (vel_bc_staggered_CN2, NS_functions.f90:313-410)
 __global__ void __launch_bounds__(256) k_cn2_axis(BCParams P) {
     Grid g{P.nx, P.ny, P.nz};
@@ -241,11 +249,10 @@ __global__ void __launch_bounds__(256) k_cn2_axis(BCParams P) {
 }
 
 // pressure ghost fill along one axis (pr_bc_staggered, NS_functions.f90:1416-1462)
-__global__ void __launch_bounds__(256) k_pr_bc_axis(BCParams P) {
+__device__ __forceinline__ void pr_bc_body(const BCParams P, const size_t t) {
     int n[3] = {P.nx + 2, P.ny + 2, P.nz + 2};
     int o1, o2;
     other_axes(P.ax, o1, o2);
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)n[o1] * n[o2]) return;
     int i1[3], i2[3];
     i1[o1] = i2[o1] = (int)(t % n[o1]) + 1;
@@ -268,6 +275,16 @@ __global__ void __launch_bounds__(256) k_pr_bc_axis(BCParams P) {
         i1[P.ax] = N; AT(i1) = bn[t];
     }
 #undef AT
+}
+
+__global__ void __launch_bounds__(256) k_pr_bc_axis(BCParams P) {
+    pr_bc_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_pr_bc_axis_batch(const BCParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                          int nbox) {
+    const int box = batch_box(prefix, nbox, blockIdx.x);
+    pr_bc_body(Ps[box], (size_t)(blockIdx.x - prefix[box]) * blockDim.x + threadIdx.x);
 }
 
 // Poisson RHS boundary terms along one axis (NS_functions.f90:1362-1398)
@@ -317,12 +334,10 @@ struct ProjParams {
     double *o[4];
 };
 
-__global__ void __launch_bounds__(256) k_project(ProjParams P) {
-    const int c = blockIdx.y;
+__device__ __forceinline__ void project_body(const ProjParams P, const size_t q, const int c) {
     int n[3];
     comp_dims(P.g, c, n);
     size_t tot = (size_t)n[0] * n[1] * n[2];
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= tot) return;
     const double gc = P.gauge ? *P.gauge : 0.0;
     if (c == 3) {
@@ -340,6 +355,16 @@ __global__ void __launch_bounds__(256) k_project(ProjParams P) {
     if (P.gauge) { hi = hi - gc; lo = lo - gc; }
     double gr = hi - lo;
     P.o[c][q] = P.s[c][q] - P.dt * gr / P.d[c];
+}
+
+__global__ void __launch_bounds__(256) k_project(ProjParams P) {
+    project_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) k_project_batch(const ProjParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                       int nbox) {
+    const int box = batch_box(prefix, nbox, blockIdx.x);
+    project_body(Ps[box], (size_t)(blockIdx.x - prefix[box]) * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 struct GetBCParams {
@@ -835,6 +860,65 @@ void launch_subbox_mean(const Grid &gp, const Grid &gs, const int o[3], const do
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(k_subbox_partial, dim3(nb), dim3(256), 0, st, P, partial);
     hipLaunchKernelGGL(k_subbox_mean, dim3(1), dim3(64), 0, st, partial, nb, 1.0 / (double)gs.ni(), out);
+}
+
+// ---- per-box parameters and batched launches (ns_batch) ----
+size_t bc_params_size() { return sizeof(BCParams); }
+size_t proj_params_size() { return sizeof(ProjParams); }
+
+int make_vel_bc_params(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w, int ax,
+                       void *out) {
+    BCParams &P = *(BCParams *)out;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.planes = planes;
+    P.a[0] = u; P.a[1] = v; P.a[2] = w;
+    P.pbc_mode = 0;
+    P.d = 0;
+    P.ax = ax;
+    P.bc = bc[ax];
+    return bc[ax] == 3 ? 0 : (int)nblocks(max_plane(g, ax));
+}
+
+int make_pr_bc_params(const Grid &g, const int pbc[3], const double *planes, const double d[3], double *dp, int ax,
+                      void *out) {
+    BCParams &P = *(BCParams *)out;
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.planes = planes;
+    P.a[0] = dp; P.a[1] = nullptr; P.a[2] = nullptr;
+    P.pbc_mode = 2;
+    P.ax = ax;
+    P.bc = pbc[ax];
+    P.d = d[ax];
+    const int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    return (int)nblocks((size_t)n[o1] * n[o2]);
+}
+
+int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag, const double *us, const double *vs,
+                        const double *ws, const double *dp, const double *gauge, double *u, double *v, double *w,
+                        double *p, void *out) {
+    ProjParams &P = *(ProjParams *)out;
+    P.g = g;
+    P.d[0] = d[0]; P.d[1] = d[1]; P.d[2] = d[2];
+    P.dt = dt;
+    P.dp_flag = dp_flag;
+    P.s[0] = us; P.s[1] = vs; P.s[2] = ws;
+    P.dp = dp;
+    P.gauge = gauge;
+    P.o[0] = u; P.o[1] = v; P.o[2] = w; P.o[3] = p;
+    return (int)nblocks(g.np());
+}
+
+void launch_vel_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(k_vel_bc_axis_batch, dim3(nblk, 3), dim3(256), 0, st, (const BCParams *)params, prefix, nbox);
+}
+void launch_pr_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(k_pr_bc_axis_batch, dim3(nblk), dim3(256), 0, st, (const BCParams *)params, prefix, nbox);
+}
+void launch_project_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(k_project_batch, dim3(nblk, 4), dim3(256), 0, st, (const ProjParams *)params, prefix, nbox);
 }
 
 }  // namespace ns

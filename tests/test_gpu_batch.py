@@ -1,0 +1,79 @@
+"""GPU: batched steps of boxes of different sizes (ns_batch) are bitwise the
+steps of each box on its own (the same kernel bodies and parameters, one
+launch for all boxes), including the Euler first step of fresh boxes next to
+restarted ones, and graph replay of the steady steps."""
+import numpy as np
+import pytest
+
+from ns_3d_staggered_dp_amd import AB2, EULER, FLAG_NO_GRAPH, NSConfig, _abi, fields
+from ns_3d_staggered_dp_amd.batch import Batch, box_sizes
+from ns_3d_staggered_dp_amd.grid import plane_set_length
+from ns_3d_staggered_dp_amd.solver import NSSolver
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(12, 10, 8), (16, 16, 16), (20, 12, 14), (9, 10, 12), (32, 8, 10), (70, 9, 18)]
+
+
+def make(nx, ny, nz, scheme, flags=0, pbc=(3, 3, 3), bc=(2, 2, 2)):
+    return NSConfig(nx=nx, ny=ny, nz=nz, dx=0.31, dy=0.27, dz=0.23, nu=0.01, dt=1e-3, scheme=scheme, bc=bc, pbc=pbc,
+                    dp_flag=1, flags=flags)
+
+
+def inputs(cfg, seed):
+    u, v, w, p = fields.random_ghosted(cfg, seed)
+    rng = np.random.default_rng(seed + 100)
+    planes = 0.2 * rng.standard_normal(plane_set_length(cfg.nx, cfg.ny, cfg.nz))
+    return (0.3 * u, 0.3 * v, 0.3 * w, p), planes
+
+
+@pytest.mark.parametrize("scheme,flags,bc,pbc", [(AB2, 0, (2, 2, 2), (3, 3, 3)), (EULER, 0, (4, 4, 4), (2, 2, 2)),
+                                                (AB2, FLAG_NO_GRAPH, (2, 4, 2), (3, 2, 3))])
+def test_batch_equals_individual_steps(scheme, flags, bc, pbc):
+    cfgs = [make(*s, scheme, flags, pbc, bc) for s in SIZES]
+    data = [inputs(c, 60 + i) for i, c in enumerate(cfgs)]
+    nsteps = 5
+    ref = []
+    for i, (c, (st, pl)) in enumerate(zip(cfgs, data)):
+        with NSSolver(c) as s:
+            s.upload(*st, first=(i % 2 == 0))
+            s.set_bc_planes(pl)
+            s.step(nsteps)
+            ref.append(s.download())
+    solvers = [NSSolver(c) for c in cfgs]
+    try:
+        for i, (s, (st, pl)) in enumerate(zip(solvers, data)):
+            s.upload(*st, first=(i % 2 == 0))
+            s.set_bc_planes(pl)
+        with Batch(solvers) as b:
+            b.step(2)
+            b.step(nsteps - 2)
+        for i, s in enumerate(solvers):
+            got = s.download()
+            for k in ("u", "v", "w", "p", "hx", "hy", "hz"):
+                assert np.array_equal(got[k], ref[i][k]), (SIZES[i], k)
+    finally:
+        for s in solvers:
+            s.close()
+
+
+def test_batch_rejects_mixed_or_unsupported_boxes():
+    a = NSSolver(make(8, 8, 8, AB2))
+    b = NSSolver(make(8, 8, 8, EULER))
+    per = NSSolver(NSConfig.cube(8, scheme=AB2, dp_flag=1))  # periodic: FFT Poisson, not batched
+    try:
+        import ctypes
+
+        for pair in ((a, b), (a, per)):
+            arr = (ctypes.c_void_p * 2)(pair[0].h, pair[1].h)
+            h = ctypes.c_void_p()
+            assert _abi.lib().ns_batch_create(arr, 2, ctypes.byref(h)) == _abi.NS_ERR_UNSUPPORTED
+    finally:
+        for s in (a, b, per):
+            s.close()
+
+
+def test_box_sizes_are_transform_sizes():
+    sizes = box_sizes(64, 32, 128, seed=4)
+    assert len(sizes) == 64 and all(32 <= n <= 128 for s in sizes for n in s)
+    assert box_sizes(64, seed=4) == sizes
