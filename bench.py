@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_256.json"))
     ap.add_argument("--adjoint-n", type=int, default=128,
                     help="grid of the adjoint/forward cost measurement (walls, AB2, DCT, dp_flag=1); 0 skips it")
+    ap.add_argument("--batch-boxes", type=int, default=64,
+                    help="sub-boxes of the batched re-simulation measurement (sizes 32..128, seed 4); 0 skips it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -168,6 +170,52 @@ def adjoint_cost(n, steps=10):
             "ratio_with_recompute": round((adj + fwd) / fwd, 3)}
 
 
+def batch_throughput(nbox, steps=20):
+    """Batched re-simulation of nbox sub-boxes of uneven sizes (walls bc=2,
+    pbc=3, AB2, DCT, dp_flag=1; sizes drawn per axis from the 7-smooth lengths
+    in [32, 128], seed 4): Mcell-steps/s of ns_batch_step over all boxes, and
+    of the same boxes stepped one after another (each with ns_step and its own
+    graph replay)."""
+    from ns_3d_staggered_dp_amd import AB2, NSConfig, fields
+    from ns_3d_staggered_dp_amd.batch import Batch, box_sizes
+    from ns_3d_staggered_dp_amd.grid import plane_set_length
+    from ns_3d_staggered_dp_amd.solver import NSSolver
+
+    sizes = box_sizes(nbox, 32, 128, seed=4)
+    solvers = []
+    cells = 0
+    try:
+        for i, (nx, ny, nz) in enumerate(sizes):
+            cfg = NSConfig(nx=nx, ny=ny, nz=nz, dx=2 * np.pi / 256, dy=2 * np.pi / 256, dz=2 * np.pi / 256, nu=1e-3,
+                           dt=5e-4, scheme=AB2, bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
+            s = NSSolver(cfg)
+            u, v, w, p = fields.random_ghosted(cfg, 400 + i)
+            s.upload(0.1 * u, 0.1 * v, 0.1 * w, p, first=True)
+            s.set_bc_planes(0.01 * np.random.default_rng(500 + i).standard_normal(plane_set_length(nx, ny, nz)))
+            solvers.append(s)
+            cells += nx * ny * nz
+        for s in solvers:
+            s.step(3)
+            s.synchronize()
+        t0 = time.perf_counter()
+        for s in solvers:
+            s.step(steps)
+            s.synchronize()
+        seq = time.perf_counter() - t0
+        with Batch(solvers) as b:
+            b.step(3)
+            t0 = time.perf_counter()
+            b.step(steps)
+            bat = time.perf_counter() - t0
+    finally:
+        for s in solvers:
+            s.close()
+    return {"config": f"{nbox} sub-boxes, sizes 32..128 per axis (7-smooth, seed 4), walls bc=2 pbc=3, AB2, DCT, "
+                      f"dp_flag=1", "cells": cells, "steps": steps,
+            "batched_mcell_steps_s": round(cells * steps / bat / 1e6, 1),
+            "sequential_mcell_steps_s": round(cells * steps / seq / 1e6, 1), "speedup": round(seq / bat, 3)}
+
+
 def copy_calibration():
     import torch
 
@@ -282,6 +330,8 @@ def main():
     }
     if a.adjoint_n > 0:
         line["adjoint"] = adjoint_cost(a.adjoint_n)
+    if a.batch_boxes > 0:
+        line["batch"] = batch_throughput(a.batch_boxes)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, u, v, w, p, a.cpu_steps)
     print(json.dumps(line), flush=True)

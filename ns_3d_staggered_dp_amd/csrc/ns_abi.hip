@@ -1482,6 +1482,8 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     std::vector<char> host((size_t)n * psz);
     std::vector<int> prefix(n + 1, 0);
     size_t lds = 0;
+    double cells = 0;
+    for (ns_ctx *c : b->ctx) cells += (double)c->g.ni();
     for (int i = 0; i < n; i++) {
         ns_ctx *c = b->ctx[i];
         const ns_config &f = c->cfg;
@@ -1491,7 +1493,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         int nb = 0;
         if (kind == 0) {
             nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr, c->p,
-                                     c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst);
+                                     c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst, (double)g.ni() / cells);
         } else if (kind == 1) {
             nb = make_vel_bc_params(g, f.bc, c->planes, c->us, c->vs, c->ws, arg, dst);
         } else if (kind == 2) {

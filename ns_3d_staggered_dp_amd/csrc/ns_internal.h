@@ -78,11 +78,12 @@ __device__ __forceinline__ int batch_box(const int *prefix, int nbox, int blk) {
     }
     return lo;
 }
-// predictor parameters of one box written to out (predict_params_size() bytes); returns its block count
+// predictor parameters of one box written to out (predict_params_size() bytes); returns its
+// block count.  share: the box's fraction of the device (1 alone, its cell share in a batch)
 size_t predict_params_size();
 int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                        double *us, double *vs, double *ws, void *out);
+                        double *us, double *vs, double *ws, void *out, double share);
 void launch_predict_batch(bool lazy, const void *params, const int *prefix, int nbox, int nblocks, hipStream_t st);
 size_t bc_params_size();
 size_t proj_params_size();

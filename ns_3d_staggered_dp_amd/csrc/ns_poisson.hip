@@ -586,7 +586,7 @@ __device__ __forceinline__ int dct_n(const Grid &g) { return AXIS == 0 ? g.nx : 
 
 // forward DCT along AXIS; AXIS 0 reads the divergence (+ boundary RHS terms)
 template <int AXIS, int SOLVE>
-__device__ __forceinline__ void dct_fwd_body(const PassParams P, const int blk) {
+__device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
     const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;
@@ -691,7 +691,7 @@ __device__ __forceinline__ void dct_fwd_body(const PassParams P, const int blk) 
 }
 
 template <int AXIS>
-__device__ __forceinline__ void dct_inv_body(const PassParams P, const int blk) {
+__device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
     const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;

@@ -169,7 +169,7 @@ __device__ __forceinline__ void other_axes(int ax, int &o1, int &o2) {
 
 // velocity ghost/wall fill along one axis (NS_functions.f90:235-308), all three
 // components; blockIdx.y selects the component
-__device__ __forceinline__ void vel_bc_body(const BCParams P, const size_t t, const int c) {
+__device__ __forceinline__ void vel_bc_body(const BCParams &P, const size_t t, const int c) {
     Grid g{P.nx, P.ny, P.nz};
     int o1, o2;
     other_axes(P.ax, o1, o2);
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_cn2_axis(BCParams P) {
 }
 
 // pressure ghost fill along one axis (pr_bc_staggered, NS_functions.f90:1416-1462)
-__device__ __forceinline__ void pr_bc_body(const BCParams P, const size_t t) {
+__device__ __forceinline__ void pr_bc_body(const BCParams &P, const size_t t) {
     int n[3] = {P.nx + 2, P.ny + 2, P.nz + 2};
     int o1, o2;
     other_axes(P.ax, o1, o2);
@@ -334,7 +334,7 @@ struct ProjParams {
     double *o[4];
 };
 
-__device__ __forceinline__ void project_body(const ProjParams P, const size_t q, const int c) {
+__device__ __forceinline__ void project_body(const ProjParams &P, const size_t q, const int c) {
     int n[3];
     comp_dims(P.g, c, n);
     size_t tot = (size_t)n[0] * n[1] * n[2];
