@@ -109,8 +109,10 @@ bool factor_radices(int n, FFTPlan &pl) {
     return m == 1 && pl.nst > 0 && pl.nst <= 10;
 }
 
+// lengths up to 2048: a transform pass holds its lines (at least one) in
+// one block's LDS and registers
 int make_fft_plan(ns_ctx *c, int n, FFTPlan &pl) {
-    if (n < 2 || !factor_radices(n, pl)) return NS_ERR_UNSUPPORTED;
+    if (n < 2 || n > 2048 || !factor_radices(n, pl)) return NS_ERR_UNSUPPORTED;
     std::vector<double2> tw(n);
     const long double tp = 2.0L * acosl(-1.0L);
     for (int m = 0; m < n; m++) {
@@ -1454,6 +1456,7 @@ struct ns_batch {
         int *prefix = nullptr;
         int nblk = 0;
         size_t lds = 0;
+        int odd = 0;     // DCT pass: some box's transform has odd radices
     };
     std::vector<Phase> phases;  // one steady (non-first) step
     hipGraphExec_t graph = nullptr;
@@ -1482,6 +1485,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     std::vector<char> host((size_t)n * psz);
     std::vector<int> prefix(n + 1, 0);
     size_t lds = 0;
+    int odd = 0;
     double cells = 0;
     for (ns_ctx *c : b->ctx) cells += (double)c->g.ni();
     for (int i = 0; i < n; i++) {
@@ -1501,6 +1505,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
             nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, c->us, c->vs, c->ws, nullptr, c->planes,
                                       c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l);
             lds = l > lds ? l : lds;
+            odd |= dct_params_odd(dst);
         } else if (kind == 3) {
             nb = make_pr_bc_params(g, f.pbc, c->planes, d, c->dp, arg, dst);
         } else {
@@ -1516,6 +1521,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     ph.arg = arg;
     ph.nblk = prefix[n];
     ph.lds = lds;
+    ph.odd = odd;
     int rc;
     if ((rc = balloc(b, &ph.params, host.size())) || (rc = balloc(b, &ph.prefix, prefix.size() * sizeof(int))))
         return rc;
@@ -1543,7 +1549,7 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
         switch (ph.kind) {
             case 0: launch_predict_batch(false, ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 1: launch_vel_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
-            case 2: launch_dct_batch(ph.arg, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
+            case 2: launch_dct_batch(ph.arg, ph.odd, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             case 3: launch_pr_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
         }

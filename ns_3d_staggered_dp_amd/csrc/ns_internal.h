@@ -69,14 +69,27 @@ void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double
 
 // ---- batched launches: many boxes, one launch per kernel ----
 // box of block blk: the last i with prefix[i] <= blk (prefix ascending, prefix[0] = 0)
+// The prefix and parameter tables are not written while a batch kernel runs,
+// so they are read through the constant address space: the loads are
+// uniform within a block and become scalar loads, as for kernel arguments.
+typedef const __attribute__((address_space(4))) int const_int;
 __device__ __forceinline__ int batch_box(const int *prefix, int nbox, int blk) {
+    const_int *pc = (const_int *)prefix;
     int lo = 0, hi = nbox - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= blk) lo = mid;
+        if (pc[mid] <= blk) lo = mid;
         else hi = mid - 1;
     }
     return lo;
+}
+// parameters of block blockIdx.x's box; blk = the block's index within the box
+template <typename T>
+__device__ __forceinline__ const T &batch_params(const T *params, const int *prefix, int nbox, int &blk) {
+    typedef const __attribute__((address_space(4))) T const_T;
+    const int box = batch_box(prefix, nbox, (int)blockIdx.x);
+    blk = (int)blockIdx.x - ((const_int *)prefix)[box];
+    return *(const T *)((const_T *)params + box);
 }
 // predictor parameters of one box written to out (predict_params_size() bytes); returns its
 // block count.  share: the box's fraction of the device (1 alone, its cell share in a batch)
@@ -104,7 +117,10 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
                          const double *us, const double *vs, const double *ws, const double *rhs_in,
                          const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
                          double *dp_out, double *sol_out, int pass, void *out, size_t *lds);
-void launch_dct_batch(int pass, const void *params, const int *prefix, int nbox, int nb, size_t lds, hipStream_t st);
+// odd: some box's plan for this pass's axis has radices 3, 5 or 7
+int dct_params_odd(const void *params);
+void launch_dct_batch(int pass, int odd, const void *params, const int *prefix, int nbox, int nb, size_t lds,
+                      hipStream_t st);
 void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff,
                       const double *u, const double *v, const double *w, double *ox, double *oy, double *oz,
                       hipStream_t st);

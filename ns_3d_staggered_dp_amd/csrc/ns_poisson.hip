@@ -128,132 +128,6 @@ struct Lay {
     int SL, SP;
 };
 
-// one Stockham stage over L lines of length N
-template <int R>
-__device__ __forceinline__ void stage(const double2 *__restrict__ A, double2 *__restrict__ B, int N, Lay ly, int L,
-                                      int Ns, const double2 *__restrict__ tw, int sign) {
-    const int NR = N / R;
-    const int nb = L * NR;
-    const int tstep = N / (Ns * R);
-    const bool line_fast = (ly.SL == 1);
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-        int line, j;
-        if (line_fast) { line = t % L; j = t / L; }
-        else { line = t / NR; j = t - line * NR; }
-        const double2 *in = A + line * ly.SL;
-        double2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) v[r] = in[(j + r * NR) * ly.SP];
-        int jm = j % Ns;
-        if (jm) {
-#pragma unroll
-            for (int r = 1; r < R; r++) {
-                double2 w = tw[jm * r * tstep];
-                if (sign > 0) w.y = -w.y;
-                v[r] = cmul(v[r], w);
-            }
-        }
-        SmallDFT<R>::run(v, sign);
-        int idxD = (j - jm) * R + jm;
-        double2 *out = B + line * ly.SL;
-#pragma unroll
-        for (int r = 0; r < R; r++) out[(idxD + r * Ns) * ly.SP] = v[r];
-    }
-}
-
-// full transform; data starts in A, result pointer returned (A or B)
-__device__ double2 *fft_lds(double2 *A, double2 *B, const FFTPlan &pl, Lay ly, int L, int sign) {
-    int Ns = 1;
-    for (int s = 0; s < pl.nst; s++) {
-        switch (pl.radix[s]) {
-            case 8: stage<8>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
-            case 4: stage<4>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
-            case 2: stage<2>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
-            case 3: stage<3>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
-            case 5: stage<5>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
-            case 7: stage<7>(A, B, pl.n, ly, L, Ns, pl.tw, sign); break;
-        }
-        __syncthreads();
-        Ns *= pl.radix[s];
-        double2 *t = A;
-        A = B;
-        B = t;
-    }
-    return A;
-}
-
-struct DivSrc {
-    // divergence source: either rhs (interior array) or the staggered u*, v*, w*
-    const double *rhs;
-    const double *us, *vs, *ws;
-    double dx, dy, dz, dt;
-    // boundary RHS terms (pr_bc_staggered_modify_rhs), DCT family only
-    const double *planes;
-    PlaneLayout L;
-    int pbc[3];
-};
-
-// divergence RHS at interior cell (a,b,c), 0-based (big_periodic_3D.f90:498-501)
-__device__ __forceinline__ double div_at(const Grid &g, const DivSrc &S, int a, int b, int c) {
-    double s;
-    if (S.rhs) {
-        s = S.rhs[(size_t)a + (size_t)g.nx * ((size_t)b + (size_t)g.ny * c)];
-    } else {
-        const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
-        const int A = a + 1, B = b + 1, C = c + 1;
-        s = (S.us[FIX(A + 1, B + 1, C + 1, U1, U2)] - S.us[FIX(A, B + 1, C + 1, U1, U2)]) / S.dx;
-        s = s + (S.vs[FIX(A + 1, B + 1, C + 1, V1, V2)] - S.vs[FIX(A + 1, B, C + 1, V1, V2)]) / S.dy;
-        s = s + (S.ws[FIX(A + 1, B + 1, C + 1, W1, W2)] - S.ws[FIX(A + 1, B + 1, C, W1, W2)]) / S.dz;
-        s = s / S.dt;
-    }
-    if (S.planes) {
-        // x, then y, then z terms, in the reference's order (NS_functions.f90:1362-1398)
-        const int id[3] = {a, b, c}, n[3] = {g.nx, g.ny, g.nz}, np_[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
-        const double d[3] = {S.dx, S.dy, S.dz};
-#pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-            int pbc = S.pbc[ax];
-            if (pbc == 1) continue;
-            int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
-            size_t pl = (size_t)(id[o1] + 1) + (size_t)np_[o1] * (size_t)(id[o2] + 1);
-            double dd = d[ax], d2 = d[ax] * d[ax];
-            if (id[ax] == 0) {
-                double b1 = S.planes[S.L.off[18 + 2 * ax] + pl];
-                if (pbc == 2) s = s - 2 * b1 / d2;
-                else if (pbc == 3) s = s + b1 / dd;
-                else if (pbc == 4) s = s - b1 / d2;
-            }
-            if (id[ax] == n[ax] - 1) {
-                double bn = S.planes[S.L.off[19 + 2 * ax] + pl];
-                if (pbc == 2) s = s - 2 * bn / d2;
-                else if (pbc == 3) s = s - bn / dd;
-                else if (pbc == 4) s = s - bn / d2;
-            }
-        }
-    }
-    return s;
-}
-
-struct PassParams {
-    Grid g;
-    FFTPlan pl;  // plan of this pass's axis
-    int L;       // complex lines per block
-    int LS;      // LDS line stride (complex)
-    int nxh;     // nx/2+1 (periodic)
-    int nxhp;    // spectrum row stride: nxh rounded up to a multiple of 8
-    DivSrc src;
-    double2 *spec;                          // periodic spectrum [z][y][kx]
-    double *work;                           // DCT work array [z][y][x]
-    const double *ex, *ey, *ez;             // eigenvalue parts
-    const double2 *wq;                      // Makhoul twiddles of this axis
-    int pbc;                                // this axis' pressure code (DCT)
-    const double *mean_dp_dev;              // DCT zero mode source (device) or null
-    double mean_dp;                         // host value if mean_dp_dev is null
-    double scale;                           // backward scale (periodic)
-    double *dp_out;                         // ghosted dp (interior written) or null
-    double *sol_out;                        // interior (nx,ny,nz) or null
-};
-
 // ---------------- periodic passes ----------------
 //
 // In-place Stockham: each stage reads its butterflies into registers, waits
@@ -329,7 +203,7 @@ __device__ __forceinline__ void stage_ip(double2 *A, int N, Lay ly, int lsh, int
 // ODD: the plan may contain radices 3, 5, 7; power-of-two lengths use the
 // variant without them, whose register footprint is a third.
 template <bool ODD>
-__device__ void fft_ip(double2 *A, const FFTPlan &pl, Lay ly, int lsh, const double2 *tw, int sign) {
+__device__ __forceinline__ void fft_ip(double2 *A, const FFTPlan &pl, Lay ly, int lsh, const double2 *tw, int sign) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; s++) {
         const int R = pl.radix[s];
@@ -534,246 +408,377 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
 }
 
 // ---------------- DCT family passes ----------------
-// Makhoul reorder: v[m] = x[src(m)]
-__device__ __forceinline__ int mk_src(int m, int N) {
-    int h = (N + 1) / 2;
-    return (m < h) ? 2 * m : 2 * (N - 1 - m) + 1;
-}
+//
+// Same in-place machinery as the periodic passes.  A block holds L = 2^lsh
+// complex lines; complex line l is the pair of real lines (2l, 2l+1) as real
+// and imaginary parts.  x lines are line-major (row stride LS); y and z lines
+// are 2L consecutive x values at one z (y pass) or one y (z pass), held
+// line-interleaved, so a row of 2L doubles in memory is a row of 2L doubles
+// in LDS.  Makhoul's reorder puts x[pos] at m(pos) = pos/2 (even pos) or
+// N-1-pos/2 (odd pos).  DST-II lines (pbc 2) flip the sign of the even
+// positions and reverse the coefficient order (NS_functions.f90:1522-1525).
 
-// forward DCT-II (pbc 3) or DST-II (pbc 2) of the pair of real lines held as
-// Z (after the forward FFT of the reordered data); writes X_a, X_b into out
-__device__ __forceinline__ void makhoul_post(const double2 *Z, int N, int k, const double2 *wq, double &xa, double &xb) {
-    double2 a = Z[k], b = Z[(N - k) % N];
-    // Va = (Z[k] + conj(Z[N-k]))/2, Vb = -i (Z[k] - conj(Z[N-k]))/2
-    double2 va = make_double2((a.x + b.x) * 0.5, (a.y - b.y) * 0.5);
-    double2 vb = make_double2((a.y + b.y) * 0.5, (b.x - a.x) * 0.5);
-    double c = wq[k].x, s = wq[k].y;  // exp(-i pi k/(2N)) = c - i s
-    xa = c * va.x + s * va.y;
-    xb = c * vb.x + s * vb.y;
-}
+struct PassParams {
+    Grid g;
+    FFTPlan pl;  // plan of this pass's axis
+    int lsh;     // log2 of the complex lines per block
+    int LS;      // row stride (complex) of line-major buffers (x passes)
+    int ngrp;    // y/z passes: groups of 2L consecutive x per row
+    int odd;     // the plan has radices 3, 5 or 7
+    IDiv dN, dNy, dGrp;
+    // x forward: divergence of u*, v*, w* (or the interior array rhs) ...
+    const double *rhs;
+    const double *us, *vs, *ws;
+    double dx, dy, dz, dt, rdx, rdy, rdz, rdt;
+    // ... minus the boundary terms of pr_bc_staggered_modify_rhs (planes
+    // 18..23 of the plane set: low and high face of x, y, z), when planes
+    // is not null
+    const double *planes;
+    long boff[6];
+    int pbcs[3];
+    double *work;                // DCT work array [z][y][x]
+    const double *ex, *ey, *ez;  // eigenvalue parts
+    const double2 *wq;           // exp(-i pi k/(2N)) of this axis
+    int pbc;                     // this axis' pressure code (2 DST-II, 3 DCT-II)
+    const double *mean_dp_dev;   // zero-mode source on the device, or null
+    double mean_dp;              // zero-mode source from the host
+    double *dp_out;              // ghosted dp (interior written) or null
+    double *sol_out;             // interior (nx,ny,nz) or null
+};
 
-// inverse: given X (length N, real) returns V[k] = exp(i pi k/(2N)) (X[k] - i X[N-k])
-__device__ __forceinline__ double2 makhoul_pre_inv(double xk, double xnk, int k, const double2 *wq) {
-    double c = wq[k].x, s = wq[k].y;
-    return make_double2(c * xk + s * xnk, s * xk - c * xnk);
-}
-
-// generic real-line gather/scatter for the DCT passes.
-// AXIS 0: lines along x, pairs of consecutive flattened (y,z) lines.
-// AXIS 1/2: lines along y/z, pairs of consecutive x at fixed z (AXIS 1) or y (AXIS 2).
-struct RealLines {
-    __device__ static void locate(const Grid &g, int AXIS, int blk, int L, int rline, long &base, long &stride,
-                                  bool &valid) {
-        if (AXIS == 0) {
-            long rl = (long)blk * 2 * L + rline;
-            valid = rl < (long)g.ny * g.nz;
-            base = rl * g.nx;
-            stride = 1;
-        } else {
-            int nxp = (g.nx + 1) / 2;               // pairs per row
-            int ngrp = (nxp + L - 1) / L;
-            int other = blk / ngrp;
-            int x = (blk % ngrp) * 2 * L + rline;
-            valid = x < g.nx;
-            if (AXIS == 1) { base = (long)other * g.nx * g.ny + x; stride = g.nx; }
-            else { base = (long)other * g.nx + x; stride = (long)g.nx * g.ny; }
+// divergence RHS at interior cell (a,b,c), 0-based (big_periodic_3D.f90:498-501),
+// then the boundary terms of the non-periodic axes in x, y, z order
+// (NS_functions.f90:1362-1398)
+__device__ __forceinline__ double wall_rhs(const PassParams &P, int a, int b, int c) {
+    const Grid &g = P.g;
+    double s;
+    if (P.rhs) {
+        s = P.rhs[(size_t)a + (size_t)g.nx * ((size_t)b + (size_t)g.ny * c)];
+    } else {
+        const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+        const int A = a + 1, B = b + 1, C = c + 1;
+        s = qdiv(P.us[FIX(A + 1, B + 1, C + 1, U1, U2)] - P.us[FIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
+        s = s + qdiv(P.vs[FIX(A + 1, B + 1, C + 1, V1, V2)] - P.vs[FIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
+        s = s + qdiv(P.ws[FIX(A + 1, B + 1, C + 1, W1, W2)] - P.ws[FIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
+        s = qdiv(s, P.dt, P.rdt);
+    }
+    if (P.planes) {
+        const int id[3] = {a, b, c}, n[3] = {g.nx, g.ny, g.nz}, np_[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
+        const double d[3] = {P.dx, P.dy, P.dz};
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            const int pbc = P.pbcs[ax];
+            if (pbc == 1) continue;
+            const bool lo = id[ax] == 0, hi = id[ax] == n[ax] - 1;
+            if (!lo && !hi) continue;
+            const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+            const size_t pl = (size_t)(id[o1] + 1) + (size_t)np_[o1] * (size_t)(id[o2] + 1);
+            const double dd = d[ax], d2 = d[ax] * d[ax];
+            if (lo) {
+                const double b1 = P.planes[P.boff[2 * ax] + pl];
+                if (pbc == 2) s = s - 2 * b1 / d2;
+                else if (pbc == 3) s = s + b1 / dd;
+                else if (pbc == 4) s = s - b1 / d2;
+            }
+            if (hi) {
+                const double bn = P.planes[P.boff[2 * ax + 1] + pl];
+                if (pbc == 2) s = s - 2 * bn / d2;
+                else if (pbc == 3) s = s - bn / dd;
+                else if (pbc == 4) s = s - bn / d2;
+            }
         }
     }
+    return s;
+}
+
+// Makhoul reorder: x[pos] sits at v[mk_pos(pos)]
+__device__ __forceinline__ int mk_pos(int pos, int N) { return (pos & 1) ? (N - 1 - (pos >> 1)) : (pos >> 1); }
+
+// DCT-II coefficient k of both real lines of a pair from their paired
+// transform (a = Z[k], b = Z[(N-k) mod N]):
+// Va = (a + conj b)/2, Vb = -i (a - conj b)/2, X = Re(exp(-i pi k/(2N)) V)
+__device__ __forceinline__ void mk_post(double2 a, double2 b, double2 q, double &xa, double &xb) {
+    const double2 va = make_double2((a.x + b.x) * 0.5, (a.y - b.y) * 0.5);
+    const double2 vb = make_double2((a.y + b.y) * 0.5, (b.x - a.x) * 0.5);
+    xa = q.x * va.x + q.y * va.y;  // q = (c, s): exp(-i pi k/(2N)) = c - i s
+    xb = q.x * vb.x + q.y * vb.y;
+}
+
+// input of the inverse FFT at k for a pair: Va + i Vb with
+// V[k] = exp(i pi k/(2N)) (X[k] - i X[N-k]), X[N] = 0
+__device__ __forceinline__ double2 mk_pre(double2 xk, double2 xnk, double2 q) {
+    const double2 va = make_double2(q.x * xk.x + q.y * xnk.x, q.y * xk.x - q.x * xnk.x);
+    const double2 vb = make_double2(q.x * xk.y + q.y * xnk.y, q.y * xk.y - q.x * xnk.y);
+    return make_double2(va.x - vb.y, va.y + vb.x);
+}
+
+// in place over the L lines of X (coefficients in DCT order): the inverse
+// FFT input at k and N-k from X[k] and X[N-k], one thread per pair (k, N-k)
+__device__ __forceinline__ void mk_pre_pairs(double2 *A, Lay ly, int lsh, int N, const double2 *__restrict__ wq) {
+    const int cnt = (N / 2 + 1) << lsh, lm = (1 << lsh) - 1;
+    for (int t = threadIdx.x; t < cnt; t += FFT_NT) {
+        const int l = t & lm, h = t >> lsh, h2 = N - h;
+        double2 *X = A + l * ly.SL;
+        const double2 xh = X[h * ly.SP];
+        const double2 xh2 = (h > 0) ? X[h2 * ly.SP] : make_double2(0.0, 0.0);
+        const double2 zh = mk_pre(xh, xh2, wq[h]);
+        if (h > 0 && h2 != h) X[h2 * ly.SP] = mk_pre(xh2, xh, wq[h2]);
+        X[h * ly.SP] = zh;
+    }
+}
+
+// y/z pass block: first x of its 2L real lines, row base, stride along the line
+struct YZGroup {
+    int x0;
+    size_t base, stride;
 };
+template <int AXIS>
+__device__ __forceinline__ YZGroup yz_group(const PassParams &P, int blk) {
+    const Grid &g = P.g;
+    const int other = idiv(blk, P.dGrp), grp = blk - other * P.ngrp;
+    YZGroup r;
+    r.x0 = grp << (P.lsh + 1);
+    if (AXIS == 1) {
+        r.base = (size_t)other * g.nx * g.ny;
+        r.stride = (size_t)g.nx;
+    } else {
+        r.base = (size_t)other * g.nx;
+        r.stride = (size_t)g.nx * g.ny;
+    }
+    return r;
+}
 
 template <int AXIS>
 __device__ __forceinline__ int dct_n(const Grid &g) { return AXIS == 0 ? g.nx : AXIS == 1 ? g.ny : g.nz; }
 
-// forward DCT along AXIS; AXIS 0 reads the divergence (+ boundary RHS terms)
-template <int AXIS, int SOLVE>
+// forward DCT-II / DST-II along x (fused with the RHS) or y, coefficients to work
+template <int AXIS, bool ODD>
 __device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk) {
     extern __shared__ double2 smem[];
-    const Grid g = P.g;
-    const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;
-    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const Grid &g = P.g;
+    const int N = dct_n<AXIS>(g), lsh = P.lsh, L = 1 << lsh;
     const bool dst = (P.pbc == 2);
-    // load 2L real lines, reorder into v, pack pairs as complex
-    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
-        int rline, pos;
-        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
-        else { rline = f % (2 * L); pos = f / (2 * L); }
-        long base, stride;
-        bool valid;
-        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
-        double val = 0.0;
-        if (valid) {
-            if (AXIS == 0) {
-                long rl = base / g.nx;
-                val = div_at(g, P.src, pos, (int)(rl % g.ny), (int)(rl / g.ny));
-            } else {
-                val = P.work[base + stride * pos];
+    const Lay ly = (AXIS == 0) ? Lay{P.LS, 1} : Lay{1, L};
+    double2 *A = smem, *tws = smem + (size_t)L * (AXIS == 0 ? P.LS : N);
+    double *Ad = (double *)A;
+    for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
+    const int cnt = 2 * L * N;
+    if (AXIS == 0) {
+        const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
+#pragma unroll 2
+        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+            const int rline = idiv(f, P.dN), x = f - rline * N;
+            const int rl = rl0 + rline;
+            double val = 0.0;
+            if (rl < RL) {
+                const int k = idiv(rl, P.dNy);
+                val = wall_rhs(P, x, rl - k * g.ny, k);
+                if (dst && (x & 1) == 0) val = -val;
             }
-            if (dst && (pos & 1) == 0) val = -val;
+            Ad[2 * ((rline >> 1) * P.LS + mk_pos(x, N)) + (rline & 1)] = val;
         }
-        // x[pos] goes to v[m] with mk_src(m) == pos
-        int m = (pos & 1) ? (N - 1 - (pos >> 1)) : (pos >> 1);
-        double *slot = (double *)&A[(rline >> 1) * LS + m];
-        slot[rline & 1] = val;
+    } else {
+        const YZGroup G = yz_group<AXIS>(P, blk);
+        const int rm = 2 * L - 1;
+#pragma unroll 4
+        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+            const int rline = f & rm, pos = f >> (lsh + 1);
+            const int x = G.x0 + rline;
+            double val = (x < g.nx) ? P.work[G.base + G.stride * pos + x] : 0.0;
+            if (dst && (pos & 1) == 0) val = -val;
+            Ad[2 * L * mk_pos(pos, N) + rline] = val;
+        }
     }
     __syncthreads();
-    double2 *Z = fft_lds(A, B, P.pl, Lay{LS, 1}, L, -1);
-    double2 *O = (Z == A) ? B : A;
-    // post-twiddle into real coefficients, stored as (xa, xb) pairs in O
-    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-        int l = f / N, k = f - l * N;
-        double xa, xb;
-        makhoul_post(Z + l * LS, N, k, P.wq, xa, xb);
-        int kk = dst ? (N - 1 - k) : k;  // DST-II reverses the output order
-        O[l * LS + kk] = make_double2(xa, xb);
-    }
-    __syncthreads();
-    if (SOLVE) {
-        // modal division; the zero mode carries mean_dp (NS_functions.f90:1576-1577)
-        for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-            int l = f / N, kz = f - l * N;
-            for (int h = 0; h < 2; h++) {
-                int rline = 2 * l + h;
-                long base, stride;
-                bool valid;
-                RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
-                if (!valid) continue;
-                int xi = (int)(base % g.nx), yj = (int)((base / g.nx) % g.ny);
-                double e = P.ex[xi];
-                e = e + P.ey[yj];
-                e = e + P.ez[kz];
-                e = -4.0 * e;
-                double *slot = (double *)&O[l * LS + kz];
-                if (e == 0.0) slot[h] = P.mean_dp_dev ? (*P.mean_dp_dev) * g.nx * g.ny * g.nz : P.mean_dp * g.nx * g.ny * g.nz;
-                else slot[h] = slot[h] / e;
-            }
+    fft_ip<ODD>(A, P.pl, ly, lsh, tws, -1);
+    const int cnto = L * N;
+    if (AXIS == 0) {
+        const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
+        for (int f = threadIdx.x; f < cnto; f += FFT_NT) {
+            const int l = idiv(f, P.dN), k = f - l * N;
+            const int ra = rl0 + 2 * l;
+            if (ra >= RL) continue;
+            const double2 *Z = A + l * P.LS;
+            double xa, xb;
+            mk_post(Z[k], Z[k == 0 ? 0 : N - k], P.wq[k], xa, xb);
+            const int kk = dst ? (N - 1 - k) : k;
+            P.work[(size_t)ra * N + kk] = xa;
+            if (ra + 1 < RL) P.work[(size_t)(ra + 1) * N + kk] = xb;
         }
-        __syncthreads();
-        // inverse along z: (DST: reverse input) -> V -> inverse FFT -> un-reorder (-> DST sign flip)
-        for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-            int l = f / N, k = f - l * N;
-            const double2 *X = O + l * LS;
-            int k0 = dst ? (N - 1 - k) : k;
-            double2 xk = X[k0];
-            double2 xnk = make_double2(0.0, 0.0);
-            if (k > 0) xnk = X[dst ? (k - 1) : (N - k)];
-            double2 va = makhoul_pre_inv(xk.x, xnk.x, k, P.wq);
-            double2 vb = makhoul_pre_inv(xk.y, xnk.y, k, P.wq);
-            Z[l * LS + k] = make_double2(va.x - vb.y, va.y + vb.x);
+    } else {
+        const YZGroup G = yz_group<AXIS>(P, blk);
+        for (int f = threadIdx.x; f < cnto; f += FFT_NT) {
+            const int l = f & (L - 1), k = f >> lsh;
+            const int x = G.x0 + 2 * l;
+            if (x >= g.nx) continue;
+            double xa, xb;
+            mk_post(A[k * L + l], A[(k == 0 ? 0 : N - k) * L + l], P.wq[k], xa, xb);
+            const int kk = dst ? (N - 1 - k) : k;
+            double *o = P.work + G.base + G.stride * kk + x;
+            o[0] = xa;
+            if (x + 1 < g.nx) o[1] = xb;
         }
-        __syncthreads();
-        double2 *R = fft_lds(Z, O, P.pl, Lay{LS, 1}, L, +1);
-        const double inv = 1.0 / N;
-        for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
-            int rline = f % (2 * L), pos = f / (2 * L);
-            long base, stride;
-            bool valid;
-            RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
-            if (!valid) continue;
-            int m = (pos & 1) ? (N - 1 - (pos >> 1)) : (pos >> 1);
-            double2 z = R[(rline >> 1) * LS + m];
-            double val = ((rline & 1) ? z.y : z.x) * inv;
-            if (dst && (pos & 1) == 0) val = -val;
-            P.work[base + stride * pos] = val;
-        }
-        return;
-    }
-    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
-        int rline, pos;
-        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
-        else { rline = f % (2 * L); pos = f / (2 * L); }
-        long base, stride;
-        bool valid;
-        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
-        if (!valid) continue;
-        double2 z = O[(rline >> 1) * LS + pos];
-        P.work[base + stride * pos] = (rline & 1) ? z.y : z.x;
     }
 }
 
-template <int AXIS>
+// z pass: forward transform, modal division (zero mode := mean_dp*nx*ny*nz,
+// NS_functions.f90:1576-1577), inverse transform, back to work
+template <bool ODD>
+__device__ __forceinline__ void dct_solve_body(const PassParams &P, const int blk) {
+    extern __shared__ double2 smem[];
+    const Grid &g = P.g;
+    const int N = g.nz, lsh = P.lsh, L = 1 << lsh;
+    const bool dst = (P.pbc == 2);
+    const Lay ly{1, L};
+    double2 *A = smem, *tws = smem + (size_t)L * N;
+    double *Ad = (double *)A;
+    for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
+    const YZGroup G = yz_group<2>(P, blk);
+    const int yj = idiv(blk, P.dGrp);
+    const int cnt = 2 * L * N, rm = 2 * L - 1;
+#pragma unroll 4
+    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+        const int rline = f & rm, pos = f >> (lsh + 1);
+        const int x = G.x0 + rline;
+        double val = (x < g.nx) ? P.work[G.base + G.stride * pos + x] : 0.0;
+        if (dst && (pos & 1) == 0) val = -val;
+        Ad[2 * L * mk_pos(pos, N) + rline] = val;
+    }
+    __syncthreads();
+    fft_ip<ODD>(A, P.pl, ly, lsh, tws, -1);
+    // per pair (k, N-k): coefficients, division by the eigenvalue at the
+    // stored position (reversed for DST-II), inverse pre-twiddle, in place
+    const double zero_mode = (P.mean_dp_dev ? *P.mean_dp_dev : P.mean_dp) * g.nx * g.ny * g.nz;
+    const int cntp = (N / 2 + 1) << lsh;
+    for (int t = threadIdx.x; t < cntp; t += FFT_NT) {
+        const int l = t & (L - 1), h = t >> lsh, h2 = N - h;
+        const int xa_ = G.x0 + 2 * l, xb_ = xa_ + 1;
+        // eigenvalue parts of both real lines (lines past nx hold zeros)
+        const double ea = (xa_ < g.nx) ? P.ex[xa_] + P.ey[yj] : 1.0;
+        const double eb = (xb_ < g.nx) ? P.ex[xb_] + P.ey[yj] : 1.0;
+        const double2 zh = A[h * L + l];
+        const double2 zh2 = A[(h == 0 ? 0 : h2) * L + l];
+        double2 X[2];
+        int kz[2];
+        const int np = (h > 0 && h2 != h) ? 2 : 1;
+        mk_post(zh, zh2, P.wq[h], X[0].x, X[0].y);
+        kz[0] = dst ? (N - 1 - h) : h;
+        if (np == 2) {
+            mk_post(zh2, zh, P.wq[h2], X[1].x, X[1].y);
+            kz[1] = dst ? (N - 1 - h2) : h2;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            if (i >= np) break;
+            const double ez = P.ez[kz[i]];
+            const double e0 = -4.0 * (ea + ez), e1 = -4.0 * (eb + ez);
+            X[i].x = (e0 == 0.0) ? zero_mode : X[i].x / e0;
+            X[i].y = (e1 == 0.0) ? zero_mode : X[i].y / e1;
+        }
+        if (np == 2) {
+            A[h2 * L + l] = mk_pre(X[1], X[0], P.wq[h2]);
+            A[h * L + l] = mk_pre(X[0], X[1], P.wq[h]);
+        } else {
+            // h == 0 (X[N] = 0) or h == N/2 (X[N-h] = X[h])
+            A[h * L + l] = mk_pre(X[0], h == 0 ? make_double2(0.0, 0.0) : X[0], P.wq[h]);
+        }
+    }
+    __syncthreads();
+    fft_ip<ODD>(A, P.pl, ly, lsh, tws, +1);
+    const double inv = 1.0 / N;
+    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+        const int rline = f & rm, pos = f >> (lsh + 1);
+        const int x = G.x0 + rline;
+        if (x >= g.nx) continue;
+        double val = Ad[2 * L * mk_pos(pos, N) + rline] * inv;
+        if (dst && (pos & 1) == 0) val = -val;
+        P.work[G.base + G.stride * pos + x] = val;
+    }
+}
+
+// inverse DCT-II / DST-II along y (work -> work) or x (work -> dp, sol)
+template <int AXIS, bool ODD>
 __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk) {
     extern __shared__ double2 smem[];
-    const Grid g = P.g;
-    const int N = dct_n<AXIS>(g), L = P.L, LS = P.LS;
-    double2 *A = smem, *B = smem + (size_t)L * LS;
+    const Grid &g = P.g;
+    const int N = dct_n<AXIS>(g), lsh = P.lsh, L = 1 << lsh;
     const bool dst = (P.pbc == 2);
-    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
-        int rline, pos;
-        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
-        else { rline = f % (2 * L); pos = f / (2 * L); }
-        long base, stride;
-        bool valid;
-        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
-        double val = valid ? P.work[base + stride * pos] : 0.0;
-        double *slot = (double *)&B[(rline >> 1) * LS + pos];
-        slot[rline & 1] = val;
+    const Lay ly = (AXIS == 0) ? Lay{P.LS, 1} : Lay{1, L};
+    double2 *A = smem, *tws = smem + (size_t)L * (AXIS == 0 ? P.LS : N);
+    double *Ad = (double *)A;
+    for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
+    const int cnt = 2 * L * N, rm = 2 * L - 1;
+    // coefficients in DCT order: a DST-II line is stored reversed
+    if (AXIS == 0) {
+        const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
+#pragma unroll 4
+        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+            const int rline = idiv(f, P.dN), pos = f - rline * N;
+            const int rl = rl0 + rline;
+            const double val = (rl < RL) ? P.work[(size_t)rl * N + pos] : 0.0;
+            Ad[2 * ((rline >> 1) * P.LS + (dst ? N - 1 - pos : pos)) + (rline & 1)] = val;
+        }
+    } else {
+        const YZGroup G = yz_group<AXIS>(P, blk);
+#pragma unroll 4
+        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+            const int rline = f & rm, pos = f >> (lsh + 1);
+            const int x = G.x0 + rline;
+            const double val = (x < g.nx) ? P.work[G.base + G.stride * pos + x] : 0.0;
+            Ad[2 * L * (dst ? N - 1 - pos : pos) + rline] = val;
+        }
     }
     __syncthreads();
-    for (int f = threadIdx.x; f < L * N; f += blockDim.x) {
-        int l = f / N, k = f - l * N;
-        const double2 *X = B + l * LS;
-        int k0 = dst ? (N - 1 - k) : k;
-        double2 xk = X[k0];
-        double2 xnk = make_double2(0.0, 0.0);
-        if (k > 0) xnk = X[dst ? (k - 1) : (N - k)];
-        double2 va = makhoul_pre_inv(xk.x, xnk.x, k, P.wq);
-        double2 vb = makhoul_pre_inv(xk.y, xnk.y, k, P.wq);
-        A[l * LS + k] = make_double2(va.x - vb.y, va.y + vb.x);
-    }
+    mk_pre_pairs(A, ly, lsh, N, P.wq);
     __syncthreads();
-    double2 *R = fft_lds(A, B, P.pl, Lay{LS, 1}, L, +1);
+    fft_ip<ODD>(A, P.pl, ly, lsh, tws, +1);
     const double inv = 1.0 / N;
-    for (int f = threadIdx.x; f < 2 * L * N; f += blockDim.x) {
-        int rline, pos;
-        if (AXIS == 0) { rline = f / N; pos = f - rline * N; }
-        else { rline = f % (2 * L); pos = f / (2 * L); }
-        long base, stride;
-        bool valid;
-        RealLines::locate(g, AXIS, blk, L, rline, base, stride, valid);
-        if (!valid) continue;
-        int m = (pos & 1) ? (N - 1 - (pos >> 1)) : (pos >> 1);
-        double2 z = R[(rline >> 1) * LS + m];
-        double val = ((rline & 1) ? z.y : z.x) * inv;
-        if (dst && (pos & 1) == 0) val = -val;
-        if (AXIS == 0) {
-            long rl = base / g.nx;
-            int j = (int)(rl % g.ny), k = (int)(rl / g.ny);
+    if (AXIS == 0) {
+        const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
+        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+            const int rline = idiv(f, P.dN), pos = f - rline * N;
+            const int rl = rl0 + rline;
+            if (rl >= RL) continue;
+            double val = Ad[2 * ((rline >> 1) * P.LS + mk_pos(pos, N)) + (rline & 1)] * inv;
+            if (dst && (pos & 1) == 0) val = -val;
+            const int k = idiv(rl, P.dNy), j = rl - k * g.ny;
             if (P.dp_out) P.dp_out[FIX(pos + 2, j + 2, k + 2, g.nx + 2, g.ny + 2)] = val;
-            if (P.sol_out) P.sol_out[base + pos] = val;
-        } else {
-            P.work[base + stride * pos] = val;
+            if (P.sol_out) P.sol_out[(size_t)rl * N + pos] = val;
+        }
+    } else {
+        const YZGroup G = yz_group<AXIS>(P, blk);
+        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+            const int rline = f & rm, pos = f >> (lsh + 1);
+            const int x = G.x0 + rline;
+            if (x >= g.nx) continue;
+            double val = Ad[2 * L * mk_pos(pos, N) + rline] * inv;
+            if (dst && (pos & 1) == 0) val = -val;
+            P.work[G.base + G.stride * pos + x] = val;
         }
     }
 }
 
-template <int AXIS, int SOLVE>
-__global__ void __launch_bounds__(256) k_dct_fwd(PassParams P) {
-    dct_fwd_body<AXIS, SOLVE>(P, blockIdx.x);
-}
-template <int AXIS>
-__global__ void __launch_bounds__(256) k_dct_inv(PassParams P) {
-    dct_inv_body<AXIS>(P, blockIdx.x);
-}
-template <int AXIS, int SOLVE>
-__global__ void __launch_bounds__(256) k_dct_fwd_batch(const PassParams *__restrict__ Ps, const int *__restrict__ prefix,
-                                                       int nbox) {
-    const int box = batch_box(prefix, nbox, blockIdx.x);
-    dct_fwd_body<AXIS, SOLVE>(Ps[box], blockIdx.x - prefix[box]);
-}
-template <int AXIS>
-__global__ void __launch_bounds__(256) k_dct_inv_batch(const PassParams *__restrict__ Ps, const int *__restrict__ prefix,
-                                                       int nbox) {
-    const int box = batch_box(prefix, nbox, blockIdx.x);
-    dct_inv_body<AXIS>(Ps[box], blockIdx.x - prefix[box]);
+// pass 0 x forward + RHS, 1 y forward, 2 z solve, 3 y inverse, 4 x inverse
+template <int PASS, bool ODD>
+__device__ __forceinline__ void dct_pass_body(const PassParams &P, const int blk) {
+    if (PASS == 0) dct_fwd_body<0, ODD>(P, blk);
+    else if (PASS == 1) dct_fwd_body<1, ODD>(P, blk);
+    else if (PASS == 2) dct_solve_body<ODD>(P, blk);
+    else if (PASS == 3) dct_inv_body<1, ODD>(P, blk);
+    else dct_inv_body<0, ODD>(P, blk);
 }
 
-// lines per block of the DCT passes: two LDS line buffers of ~16 KiB each
-// (4 resident blocks per CU)
-inline int lines_per_block(int LS) {
-    int L = 1024 / LS;
-    return L < 1 ? 1 : L;
+template <int PASS, bool ODD>
+__global__ void __launch_bounds__(FFT_NT) k_dct(PassParams P) {
+    dct_pass_body<PASS, ODD>(P, blockIdx.x);
 }
-inline size_t lds_bytes(int L, int LS) { return (size_t)2 * L * LS * sizeof(double2); }
+template <int PASS, bool ODD>
+__global__ void __launch_bounds__(FFT_NT) k_dct_batch(const PassParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                      int nbox) {
+    int blk;
+    const PassParams &P = batch_params(Ps, prefix, nbox, blk);
+    dct_pass_body<PASS, ODD>(P, blk);
+}
 
 }  // namespace
 
@@ -866,6 +871,53 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     if (ev) (void)hipEventRecord(ev[4], st);
 }
 
+namespace {
+
+// x passes hold fewer complex elements per block than the y/z passes
+// (contiguous lines need no grouping for coalescing)
+constexpr int DCT_X_ELEMS = 1024;
+
+// log2 of the lines per block of a y/z pass over rows of nx values: the
+// largest L = 2^s with L*N <= FFT_ELEMS that leaves the fewest idle lines
+// past nx, with 2L >= 16 (a 128-byte row segment) where that fits
+int dct_yz_lsh(int nx, int N) {
+    const int smax = fft_lsh(N);
+    const int smin = smax < 3 ? smax : 3;
+    int best = smax;
+    long bw = -1;
+    for (int s = smax; s >= smin; s--) {
+        const long w2 = 2L << s;
+        const long w = (nx + w2 - 1) / w2 * w2;
+        if (bw < 0 || w < bw) {
+            bw = w;
+            best = s;
+        }
+    }
+    return best;
+}
+
+bool plan_has_odd(const FFTPlan &pl) {
+    for (int s = 0; s < pl.nst; s++)
+        if (pl.radix[s] & 1) return true;
+    return false;
+}
+
+template <int PASS, bool ODD>
+void dct_launch(const PassParams &P, int nb, size_t lds, hipStream_t st) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)k_dct<PASS, ODD>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_dct<PASS, ODD>), dim3(nb), dim3(FFT_NT), lds, st, P);
+}
+template <int PASS, bool ODD>
+void dct_launch_batch(const PassParams *Ps, const int *prefix, int nbox, int nb, size_t lds, hipStream_t st) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)k_dct_batch<PASS, ODD>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+    hipLaunchKernelGGL((k_dct_batch<PASS, ODD>), dim3(nb), dim3(FFT_NT), lds, st, Ps, prefix, nbox);
+}
+
+}  // namespace
+
 // parameters of DCT pass `pass` (0 x forward + divergence, 1 y forward, 2 z
 // forward + modal solve + z inverse, 3 y inverse, 4 x inverse); returns the
 // block count and the dynamic LDS size
@@ -876,19 +928,22 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
     PassParams &P = *(PassParams *)out;
     P = PassParams{};
     P.g = g;
-    P.src.rhs = rhs_in;
-    P.src.us = us; P.src.vs = vs; P.src.ws = ws;
-    P.src.dx = dx; P.src.dy = dy; P.src.dz = dz; P.src.dt = dt;
-    P.src.planes = planes;
-    P.src.L = plane_layout(g.nx, g.ny, g.nz);
-    for (int a = 0; a < 3; a++) P.src.pbc[a] = pl.pbc[a];
+    P.rhs = rhs_in;
+    P.us = us; P.vs = vs; P.ws = ws;
+    P.dx = dx; P.dy = dy; P.dz = dz; P.dt = dt;
+    P.rdx = 1.0 / dx; P.rdy = 1.0 / dy; P.rdz = 1.0 / dz; P.rdt = 1.0 / dt;
+    P.planes = planes;
+    if (planes) {
+        const PlaneLayout L = plane_layout(g.nx, g.ny, g.nz);
+        for (int i = 0; i < 6; i++) P.boff[i] = L.off[18 + i];
+    }
+    for (int a = 0; a < 3; a++) P.pbcs[a] = pl.pbc[a];
     P.work = work;
     P.ex = pl.ex; P.ey = pl.ey; P.ez = pl.ez;
     P.mean_dp_dev = mean_dp_dev;
     P.mean_dp = mean_dp_host;
     P.dp_out = dp_out;
     P.sol_out = sol_out;
-    const long RL = (long)g.ny * g.nz;
     const int ax = (pass == 0 || pass == 4) ? 0 : (pass == 2) ? 2 : 1;
     const FFTPlan plans[3] = {pl.px, pl.py, pl.pz};
     const double2 *wq[3] = {pl.wq_x, pl.wq_y, pl.wq_z};
@@ -896,35 +951,58 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
     P.pl = plans[ax];
     P.wq = wq[ax];
     P.pbc = pl.pbc[ax];
-    P.LS = n[ax] + 1;
-    P.L = lines_per_block(P.LS);
-    *lds = lds_bytes(P.L, P.LS);
-    if (ax == 0) return (int)((((RL + 1) / 2) + P.L - 1) / P.L);
-    const int others = (ax == 1) ? g.nz : g.ny;
-    return (int)(((((g.nx + 1) / 2) + P.L - 1) / P.L) * others);
+    P.odd = plan_has_odd(P.pl) ? 1 : 0;
+    P.dN = make_idiv(g.nx);
+    P.dNy = make_idiv(g.ny);
+    if (ax == 0) {
+        P.lsh = fft_lsh(g.nx, DCT_X_ELEMS);
+        P.LS = fft_row(g.nx);
+        *lds = ((size_t)(1 << P.lsh) * P.LS + g.nx) * sizeof(double2);
+        const long RL = (long)g.ny * g.nz, per = 2L << P.lsh;
+        return (int)((RL + per - 1) / per);
+    }
+    const int N = n[ax];
+    P.lsh = dct_yz_lsh(g.nx, N);
+    P.ngrp = (int)((g.nx + (2L << P.lsh) - 1) / (2L << P.lsh));
+    P.dGrp = make_idiv(P.ngrp);
+    *lds = ((size_t)(1 << P.lsh) * N + N) * sizeof(double2);
+    return P.ngrp * (ax == 1 ? g.nz : g.ny);
 }
 
 size_t pass_params_size() { return sizeof(PassParams); }
 
+int dct_params_odd(const void *params) { return ((const PassParams *)params)->odd; }
+
 void launch_dct_pass(int pass, const void *params, int nb, size_t lds, hipStream_t st) {
     const PassParams &P = *(const PassParams *)params;
-    switch (pass) {
-        case 0: hipLaunchKernelGGL((k_dct_fwd<0, 0>), dim3(nb), dim3(256), lds, st, P); break;
-        case 1: hipLaunchKernelGGL((k_dct_fwd<1, 0>), dim3(nb), dim3(256), lds, st, P); break;
-        case 2: hipLaunchKernelGGL((k_dct_fwd<2, 1>), dim3(nb), dim3(256), lds, st, P); break;
-        case 3: hipLaunchKernelGGL((k_dct_inv<1>), dim3(nb), dim3(256), lds, st, P); break;
-        case 4: hipLaunchKernelGGL((k_dct_inv<0>), dim3(nb), dim3(256), lds, st, P); break;
+    switch (pass * 2 + (P.odd ? 1 : 0)) {
+        case 0: dct_launch<0, false>(P, nb, lds, st); break;
+        case 1: dct_launch<0, true>(P, nb, lds, st); break;
+        case 2: dct_launch<1, false>(P, nb, lds, st); break;
+        case 3: dct_launch<1, true>(P, nb, lds, st); break;
+        case 4: dct_launch<2, false>(P, nb, lds, st); break;
+        case 5: dct_launch<2, true>(P, nb, lds, st); break;
+        case 6: dct_launch<3, false>(P, nb, lds, st); break;
+        case 7: dct_launch<3, true>(P, nb, lds, st); break;
+        case 8: dct_launch<4, false>(P, nb, lds, st); break;
+        case 9: dct_launch<4, true>(P, nb, lds, st); break;
     }
 }
 
-void launch_dct_batch(int pass, const void *params, const int *prefix, int nbox, int nb, size_t lds, hipStream_t st) {
+void launch_dct_batch(int pass, int odd, const void *params, const int *prefix, int nbox, int nb, size_t lds,
+                      hipStream_t st) {
     const PassParams *Ps = (const PassParams *)params;
-    switch (pass) {
-        case 0: hipLaunchKernelGGL((k_dct_fwd_batch<0, 0>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
-        case 1: hipLaunchKernelGGL((k_dct_fwd_batch<1, 0>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
-        case 2: hipLaunchKernelGGL((k_dct_fwd_batch<2, 1>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
-        case 3: hipLaunchKernelGGL((k_dct_inv_batch<1>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
-        case 4: hipLaunchKernelGGL((k_dct_inv_batch<0>), dim3(nb), dim3(256), lds, st, Ps, prefix, nbox); break;
+    switch (pass * 2 + (odd ? 1 : 0)) {
+        case 0: dct_launch_batch<0, false>(Ps, prefix, nbox, nb, lds, st); break;
+        case 1: dct_launch_batch<0, true>(Ps, prefix, nbox, nb, lds, st); break;
+        case 2: dct_launch_batch<1, false>(Ps, prefix, nbox, nb, lds, st); break;
+        case 3: dct_launch_batch<1, true>(Ps, prefix, nbox, nb, lds, st); break;
+        case 4: dct_launch_batch<2, false>(Ps, prefix, nbox, nb, lds, st); break;
+        case 5: dct_launch_batch<2, true>(Ps, prefix, nbox, nb, lds, st); break;
+        case 6: dct_launch_batch<3, false>(Ps, prefix, nbox, nb, lds, st); break;
+        case 7: dct_launch_batch<3, true>(Ps, prefix, nbox, nb, lds, st); break;
+        case 8: dct_launch_batch<4, false>(Ps, prefix, nbox, nb, lds, st); break;
+        case 9: dct_launch_batch<4, true>(Ps, prefix, nbox, nb, lds, st); break;
     }
 }
 

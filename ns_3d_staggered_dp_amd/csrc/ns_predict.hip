@@ -463,8 +463,9 @@ __global__ void __launch_bounds__(NT, 3) k_predict_tiled(TileParams P) {
 template <bool LAZY>
 __global__ void __launch_bounds__(NT, 3) k_predict_tiled_batch(const TileParams *__restrict__ Ps,
                                                                const int *__restrict__ prefix, int nbox) {
-    const int box = batch_box(prefix, nbox, blockIdx.x);
-    predict_tile<LAZY>(Ps[box], blockIdx.x - prefix[box]);
+    int blk;
+    const TileParams &P = batch_params(Ps, prefix, nbox, blk);
+    predict_tile<LAZY>(P, blk);
 }
 
 }  // namespace

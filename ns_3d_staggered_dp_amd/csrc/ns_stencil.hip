@@ -216,8 +216,9 @@ __global__ void __launch_bounds__(256) k_vel_bc_axis(BCParams P) {
 
 __global__ void __launch_bounds__(256) k_vel_bc_axis_batch(const BCParams *__restrict__ Ps, const int *__restrict__ prefix,
                                                            int nbox) {
-    const int box = batch_box(prefix, nbox, blockIdx.x);
-    vel_bc_body(Ps[box], (size_t)(blockIdx.x - prefix[box]) * blockDim.x + threadIdx.x, blockIdx.y);
+    int blk;
+    const BCParams &P = batch_params(Ps, prefix, nbox, blk);
+    vel_bc_body(P, (size_t)blk * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 // boundary rows before an ADI sweep (vel_bc_staggered_CN2, NS_functions.f90:313-410)
@@ -283,8 +284,9 @@ __global__ void __launch_bounds__(256) k_pr_bc_axis(BCParams P) {
 
 __global__ void __launch_bounds__(256) k_pr_bc_axis_batch(const BCParams *__restrict__ Ps, const int *__restrict__ prefix,
                                                           int nbox) {
-    const int box = batch_box(prefix, nbox, blockIdx.x);
-    pr_bc_body(Ps[box], (size_t)(blockIdx.x - prefix[box]) * blockDim.x + threadIdx.x);
+    int blk;
+    const BCParams &P = batch_params(Ps, prefix, nbox, blk);
+    pr_bc_body(P, (size_t)blk * blockDim.x + threadIdx.x);
 }
 
 // Poisson RHS boundary terms along one axis (NS_functions.f90:1362-1398)
@@ -363,8 +365,9 @@ __global__ void __launch_bounds__(256) k_project(ProjParams P) {
 
 __global__ void __launch_bounds__(256) k_project_batch(const ProjParams *__restrict__ Ps, const int *__restrict__ prefix,
                                                        int nbox) {
-    const int box = batch_box(prefix, nbox, blockIdx.x);
-    project_body(Ps[box], (size_t)(blockIdx.x - prefix[box]) * blockDim.x + threadIdx.x, blockIdx.y);
+    int blk;
+    const ProjParams &P = batch_params(Ps, prefix, nbox, blk);
+    project_body(P, (size_t)blk * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 struct GetBCParams {

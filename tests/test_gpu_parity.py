@@ -132,7 +132,7 @@ def test_poisson_fft(oracle, n):
 
 
 @pytest.mark.parametrize("pbc", [(3, 3, 3), (2, 2, 2), (2, 3, 2)])
-@pytest.mark.parametrize("n", [(16, 16, 16), (12, 10, 14), (48, 20, 28)])
+@pytest.mark.parametrize("n", [(16, 16, 16), (12, 10, 14), (48, 20, 28), (70, 36, 20), (128, 24, 40)])
 def test_dct_poisson(oracle, pbc, n):
     cfg = cfg_box(*n, bc=(2, 2, 2), pbc=pbc)
     rhs = np.asfortranarray(np.random.default_rng(61).standard_normal(n))
