@@ -472,6 +472,8 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
         return fail(NS_ERR_HIP);
     }
     const Grid &g = c->g;
+    // element indices of the ghosted arrays are 32-bit in some kernels
+    if (g.np() >= ((size_t)1 << 32)) return fail(NS_ERR_UNSUPPORTED);
     if ((rc = dalloc(c, &c->u, g.nu())) || (rc = dalloc(c, &c->v, g.nv())) || (rc = dalloc(c, &c->w, g.nw())) ||
         (rc = dalloc(c, &c->p, g.np())) || (rc = dalloc(c, &c->hx, g.nu())) || (rc = dalloc(c, &c->hy, g.nv())) ||
         (rc = dalloc(c, &c->hz, g.nw())) || (rc = dalloc(c, &c->us, g.nu())) || (rc = dalloc(c, &c->vs, g.nv())) ||
@@ -1456,7 +1458,7 @@ struct ns_batch {
         int *prefix = nullptr;
         int nblk = 0;
         size_t lds = 0;
-        int odd = 0;     // DCT pass: some box's transform has odd radices
+        int variant = 0;  // DCT pass: kernel variant (OR over the boxes)
     };
     std::vector<Phase> phases;  // one steady (non-first) step
     hipGraphExec_t graph = nullptr;
@@ -1477,6 +1479,8 @@ int balloc(ns_batch *b, T **p, size_t bytes) {
     return NS_OK;
 }
 
+constexpr long BATCH_ROUNDS = 8;
+
 // parameters of every box for one launch, uploaded with their block prefix
 int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_batch::Phase> &out) {
     const int n = (int)b->ctx.size();
@@ -1485,9 +1489,18 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     std::vector<char> host((size_t)n * psz);
     std::vector<int> prefix(n + 1, 0);
     size_t lds = 0;
-    int odd = 0;
-    double cells = 0;
-    for (ns_ctx *c : b->ctx) cells += (double)c->g.ni();
+    int variant = 0;
+    // predictor: blocks stream z chunks serially, so chunks of one length for
+    // all boxes, short enough that the launch spans about BATCH_ROUNDS rounds
+    // of resident blocks (a short tail), and at least 8 cells (the prologue
+    // of a chunk re-reads 3 slices)
+    int zc_hint = 0;
+    if (kind == 0) {
+        long cols = 0;
+        for (ns_ctx *c : b->ctx) cols += predict_tile_columns(c->cfg);
+        const long z = cols / ((long)predict_resident_blocks(false) * BATCH_ROUNDS);
+        zc_hint = (int)(z > 8 ? z : 8);
+    }
     for (int i = 0; i < n; i++) {
         ns_ctx *c = b->ctx[i];
         const ns_config &f = c->cfg;
@@ -1497,7 +1510,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         int nb = 0;
         if (kind == 0) {
             nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr, c->p,
-                                     c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst, (double)g.ni() / cells);
+                                     c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst, zc_hint);
         } else if (kind == 1) {
             nb = make_vel_bc_params(g, f.bc, c->planes, c->us, c->vs, c->ws, arg, dst);
         } else if (kind == 2) {
@@ -1505,7 +1518,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
             nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, c->us, c->vs, c->ws, nullptr, c->planes,
                                       c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l);
             lds = l > lds ? l : lds;
-            odd |= dct_params_odd(dst);
+            variant |= dct_params_variant(dst);
         } else if (kind == 3) {
             nb = make_pr_bc_params(g, f.pbc, c->planes, d, c->dp, arg, dst);
         } else {
@@ -1521,7 +1534,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     ph.arg = arg;
     ph.nblk = prefix[n];
     ph.lds = lds;
-    ph.odd = odd;
+    ph.variant = variant;
     int rc;
     if ((rc = balloc(b, &ph.params, host.size())) || (rc = balloc(b, &ph.prefix, prefix.size() * sizeof(int))))
         return rc;
@@ -1549,7 +1562,7 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
         switch (ph.kind) {
             case 0: launch_predict_batch(false, ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 1: launch_vel_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
-            case 2: launch_dct_batch(ph.arg, ph.odd, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
+            case 2: launch_dct_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             case 3: launch_pr_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
         }

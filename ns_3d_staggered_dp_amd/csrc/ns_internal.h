@@ -20,6 +20,17 @@ struct Grid {
     __host__ __device__ size_t ni() const { return (size_t)nx * ny * nz; }
 };
 
+// x / d for x < 2^32 and 1 <= d < 2^20, with rd = 1.0/d: the double product
+// is within 2^-20 of x/d, so truncation gives the quotient or, when d divides
+// x, one less; one correction step fixes that
+__device__ __forceinline__ unsigned udiv_rd(unsigned x, unsigned d, double rd) {
+    unsigned q = (unsigned)((double)x * rd);
+    const long long rem = (long long)x - (long long)q * d;
+    if (rem < 0) q--;
+    else if (rem >= (long long)d) q++;
+    return q;
+}
+
 // component dims: c=0 u, 1 v, 2 w, 3 p
 __host__ __device__ inline void comp_dims(const Grid &g, int c, int n[3]) {
     n[0] = g.nx + 2; n[1] = g.ny + 2; n[2] = g.nz + 2;
@@ -92,11 +103,14 @@ __device__ __forceinline__ const T &batch_params(const T *params, const int *pre
     return *(const T *)((const_T *)params + box);
 }
 // predictor parameters of one box written to out (predict_params_size() bytes); returns its
-// block count.  share: the box's fraction of the device (1 alone, its cell share in a batch)
+// block count.  zc_hint: z cells per block (0: chosen for the box alone on the device)
 size_t predict_params_size();
 int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                        double *us, double *vs, double *ws, void *out, double share);
+                        double *us, double *vs, double *ws, void *out, int zc_hint);
+// tile columns x z cells of a box (the predictor's serial work units) and resident predictor blocks
+long predict_tile_columns(const ns_config &c);
+int predict_resident_blocks(bool lazy);
 void launch_predict_batch(bool lazy, const void *params, const int *prefix, int nbox, int nblocks, hipStream_t st);
 size_t bc_params_size();
 size_t proj_params_size();
@@ -117,9 +131,10 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
                          const double *us, const double *vs, const double *ws, const double *rhs_in,
                          const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
                          double *dp_out, double *sol_out, int pass, void *out, size_t *lds);
-// odd: some box's plan for this pass's axis has radices 3, 5 or 7
-int dct_params_odd(const void *params);
-void launch_dct_batch(int pass, int odd, const void *params, const int *prefix, int nbox, int nb, size_t lds,
+// kernel variant of a box's pass (bit 1: odd radices, bit 0: lines over 1024);
+// a batch launch uses the bitwise OR over its boxes
+int dct_params_variant(const void *params);
+void launch_dct_batch(int pass, int variant, const void *params, const int *prefix, int nbox, int nb, size_t lds,
                       hipStream_t st);
 void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, double h2, int which_diff,
                       const double *u, const double *v, const double *w, double *ox, double *oy, double *oz,

@@ -139,7 +139,12 @@ struct Lay {
 // staged in LDS.
 
 constexpr int FFT_NT = 256;      // threads per block
-constexpr int FFT_ELEMS = 2048;  // complex elements per block (L*N)
+constexpr int FFT_ELEMS = 2048;  // complex elements per block (L*N), at most
+// x passes: contiguous lines need no grouping for coalescing, so they use
+// fewer elements per block (more resident blocks per CU; measured at 256^3)
+constexpr int FFT_XF_ELEMS = FFT_ELEMS / 4, FFT_XI_ELEMS = FFT_ELEMS / 2;
+// DCT passes: 1024 elements per block
+constexpr int DCT_ELEMS = 1024;
 
 // x / d for 0 <= x < 2^24 with r = 1/d in single precision (one correction)
 struct IDiv {
@@ -154,10 +159,11 @@ __device__ __forceinline__ int idiv(int x, IDiv D) {
     return q;
 }
 
-template <int R>
+// EL: the block's element cap (L*N <= EL), which sizes the register sets
+template <int R, int EL>
 __device__ __forceinline__ void stage_ip(double2 *A, int N, Lay ly, int lsh, int Ns, int tstep,
                                          const double2 *__restrict__ tw, int sign) {
-    constexpr int MB = (FFT_ELEMS / R + FFT_NT - 1) / FFT_NT;
+    constexpr int MB = (EL / R + FFT_NT - 1) / FFT_NT;
     const int NR = N / R;
     const int nb = NR << lsh;
     const int lm = (1 << lsh) - 1;
@@ -202,19 +208,19 @@ __device__ __forceinline__ void stage_ip(double2 *A, int N, Lay ly, int lsh, int
 // full in-place transform of the 2^lsh lines in A (callers synchronise before).
 // ODD: the plan may contain radices 3, 5, 7; power-of-two lengths use the
 // variant without them, whose register footprint is a third.
-template <bool ODD>
+template <bool ODD, int EL = FFT_ELEMS>
 __device__ __forceinline__ void fft_ip(double2 *A, const FFTPlan &pl, Lay ly, int lsh, const double2 *tw, int sign) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; s++) {
         const int R = pl.radix[s];
         const int tstep = pl.n / (Ns * R);
         switch (R) {
-            case 8: stage_ip<8>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
-            case 4: stage_ip<4>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
-            case 2: stage_ip<2>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
-            case 3: if (ODD) stage_ip<3>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
-            case 5: if (ODD) stage_ip<5>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
-            case 7: if (ODD) stage_ip<7>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 8: stage_ip<8, EL>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 4: stage_ip<4, EL>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 2: stage_ip<2, EL>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 3: if (ODD) stage_ip<3, EL>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 5: if (ODD) stage_ip<5, EL>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
+            case 7: if (ODD) stage_ip<7, EL>(A, pl.n, ly, lsh, Ns, tstep, tw, sign); break;
         }
         Ns *= R;
     }
@@ -260,7 +266,7 @@ __device__ __forceinline__ double div_rhs(const FFTPass &P, int a, int b, int c)
     return qdiv(s, P.dt, P.rdt);
 }
 
-template <bool ODD>
+template <bool ODD, int EL>
 __global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd(FFTPass P) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
@@ -282,7 +288,7 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd(FFTPass P) {
         ((double *)&A[(rline >> 1) * LS + x])[rline & 1] = val;
     }
     __syncthreads();
-    fft_ip<ODD>(A, P.pl, Lay{LS, 1}, P.lsh, tws, -1);
+    fft_ip<ODD, EL>(A, P.pl, Lay{LS, 1}, P.lsh, tws, -1);
     // split the paired transform into the two half spectra
     const int cnto = 2 * L * nxh;
     for (int f = threadIdx.x; f < cnto; f += FFT_NT) {
@@ -353,7 +359,7 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
     }
 }
 
-template <bool ODD>
+template <bool ODD, int EL>
 __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
@@ -384,7 +390,7 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
         A[l * LS + m] = make_double2(a.x - b.y, a.y + b.x);
     }
     __syncthreads();
-    fft_ip<ODD>(A, P.pl, Lay{LS, 1}, P.lsh, tws, +1);
+    fft_ip<ODD, EL>(A, P.pl, Lay{LS, 1}, P.lsh, tws, +1);
     const double sc = P.scale;
     const int cnto = 2 * L * N;
     for (int f = threadIdx.x; f < cnto; f += FFT_NT) {
@@ -425,6 +431,7 @@ struct PassParams {
     int LS;      // row stride (complex) of line-major buffers (x passes)
     int ngrp;    // y/z passes: groups of 2L consecutive x per row
     int odd;     // the plan has radices 3, 5 or 7
+    int big;     // line longer than DCT_ELEMS: full-size register sets
     IDiv dN, dNy, dGrp;
     // x forward: divergence of u*, v*, w* (or the interior array rhs) ...
     const double *rhs;
@@ -552,7 +559,7 @@ template <int AXIS>
 __device__ __forceinline__ int dct_n(const Grid &g) { return AXIS == 0 ? g.nx : AXIS == 1 ? g.ny : g.nz; }
 
 // forward DCT-II / DST-II along x (fused with the RHS) or y, coefficients to work
-template <int AXIS, bool ODD>
+template <int AXIS, bool ODD, int EL>
 __device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk) {
     extern __shared__ double2 smem[];
     const Grid &g = P.g;
@@ -590,7 +597,7 @@ __device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk)
         }
     }
     __syncthreads();
-    fft_ip<ODD>(A, P.pl, ly, lsh, tws, -1);
+    fft_ip<ODD, EL>(A, P.pl, ly, lsh, tws, -1);
     const int cnto = L * N;
     if (AXIS == 0) {
         const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
@@ -623,7 +630,7 @@ __device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk)
 
 // z pass: forward transform, modal division (zero mode := mean_dp*nx*ny*nz,
 // NS_functions.f90:1576-1577), inverse transform, back to work
-template <bool ODD>
+template <bool ODD, int EL>
 __device__ __forceinline__ void dct_solve_body(const PassParams &P, const int blk) {
     extern __shared__ double2 smem[];
     const Grid &g = P.g;
@@ -645,7 +652,7 @@ __device__ __forceinline__ void dct_solve_body(const PassParams &P, const int bl
         Ad[2 * L * mk_pos(pos, N) + rline] = val;
     }
     __syncthreads();
-    fft_ip<ODD>(A, P.pl, ly, lsh, tws, -1);
+    fft_ip<ODD, EL>(A, P.pl, ly, lsh, tws, -1);
     // per pair (k, N-k): coefficients, division by the eigenvalue at the
     // stored position (reversed for DST-II), inverse pre-twiddle, in place
     const double zero_mode = (P.mean_dp_dev ? *P.mean_dp_dev : P.mean_dp) * g.nx * g.ny * g.nz;
@@ -684,7 +691,7 @@ __device__ __forceinline__ void dct_solve_body(const PassParams &P, const int bl
         }
     }
     __syncthreads();
-    fft_ip<ODD>(A, P.pl, ly, lsh, tws, +1);
+    fft_ip<ODD, EL>(A, P.pl, ly, lsh, tws, +1);
     const double inv = 1.0 / N;
     for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
         const int rline = f & rm, pos = f >> (lsh + 1);
@@ -697,7 +704,7 @@ __device__ __forceinline__ void dct_solve_body(const PassParams &P, const int bl
 }
 
 // inverse DCT-II / DST-II along y (work -> work) or x (work -> dp, sol)
-template <int AXIS, bool ODD>
+template <int AXIS, bool ODD, int EL>
 __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk) {
     extern __shared__ double2 smem[];
     const Grid &g = P.g;
@@ -731,7 +738,7 @@ __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk)
     __syncthreads();
     mk_pre_pairs(A, ly, lsh, N, P.wq);
     __syncthreads();
-    fft_ip<ODD>(A, P.pl, ly, lsh, tws, +1);
+    fft_ip<ODD, EL>(A, P.pl, ly, lsh, tws, +1);
     const double inv = 1.0 / N;
     if (AXIS == 0) {
         const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
@@ -759,25 +766,25 @@ __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk)
 }
 
 // pass 0 x forward + RHS, 1 y forward, 2 z solve, 3 y inverse, 4 x inverse
-template <int PASS, bool ODD>
+template <int PASS, bool ODD, int EL>
 __device__ __forceinline__ void dct_pass_body(const PassParams &P, const int blk) {
-    if (PASS == 0) dct_fwd_body<0, ODD>(P, blk);
-    else if (PASS == 1) dct_fwd_body<1, ODD>(P, blk);
-    else if (PASS == 2) dct_solve_body<ODD>(P, blk);
-    else if (PASS == 3) dct_inv_body<1, ODD>(P, blk);
-    else dct_inv_body<0, ODD>(P, blk);
+    if (PASS == 0) dct_fwd_body<0, ODD, EL>(P, blk);
+    else if (PASS == 1) dct_fwd_body<1, ODD, EL>(P, blk);
+    else if (PASS == 2) dct_solve_body<ODD, EL>(P, blk);
+    else if (PASS == 3) dct_inv_body<1, ODD, EL>(P, blk);
+    else dct_inv_body<0, ODD, EL>(P, blk);
 }
 
-template <int PASS, bool ODD>
+template <int PASS, bool ODD, int EL>
 __global__ void __launch_bounds__(FFT_NT) k_dct(PassParams P) {
-    dct_pass_body<PASS, ODD>(P, blockIdx.x);
+    dct_pass_body<PASS, ODD, EL>(P, blockIdx.x);
 }
-template <int PASS, bool ODD>
+template <int PASS, bool ODD, int EL>
 __global__ void __launch_bounds__(FFT_NT) k_dct_batch(const PassParams *__restrict__ Ps, const int *__restrict__ prefix,
                                                       int nbox) {
     int blk;
     const PassParams &P = batch_params(Ps, prefix, nbox, blk);
-    dct_pass_body<PASS, ODD>(P, blk);
+    dct_pass_body<PASS, ODD, EL>(P, blk);
 }
 
 }  // namespace
@@ -800,9 +807,6 @@ int fft_lsh(int n, int elems = FFT_ELEMS) {
     while ((2 << s) * n <= elems) s++;
     return s;
 }
-// x passes: contiguous lines need no grouping for coalescing, so they use
-// fewer elements per block (more resident blocks per CU; measured at 256^3)
-constexpr int FFT_XF_ELEMS = FFT_ELEMS / 4, FFT_XI_ELEMS = FFT_ELEMS / 2;
 // line-major row stride: n rounded up to 2 (mod 16) complex
 int fft_row(int n) { return n + ((2 - n) % 16 + 16) % 16; }
 
@@ -836,12 +840,19 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     auto nb_yz = [&](int others) { return (unsigned)((((P.nxh + (1 << P.lsh) - 1) >> P.lsh)) * others); };
     // x forward (fused with the divergence)
     P.pl = pl.px;
-    P.lsh = fft_lsh(g.nx, FFT_XF_ELEMS);
+    // lines longer than the x-pass cap use the full-size register sets
+    const bool xf_small = g.nx <= FFT_XF_ELEMS, xi_small = g.nx <= FFT_XI_ELEMS;
+    P.lsh = fft_lsh(g.nx, xf_small ? FFT_XF_ELEMS : FFT_ELEMS);
     P.LS = fft_row(g.nx);
     const unsigned nbx = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
     auto pow2 = [](int n) { return (n & (n - 1)) == 0; };
-    if (pow2(g.nx)) launch_lds(k_fft_x_fwd<false>, nbx, lds_x(), st, P);
-    else launch_lds(k_fft_x_fwd<true>, nbx, lds_x(), st, P);
+    if (xf_small) {
+        if (pow2(g.nx)) launch_lds(k_fft_x_fwd<false, FFT_XF_ELEMS>, nbx, lds_x(), st, P);
+        else launch_lds(k_fft_x_fwd<true, FFT_XF_ELEMS>, nbx, lds_x(), st, P);
+    } else {
+        if (pow2(g.nx)) launch_lds(k_fft_x_fwd<false, FFT_ELEMS>, nbx, lds_x(), st, P);
+        else launch_lds(k_fft_x_fwd<true, FFT_ELEMS>, nbx, lds_x(), st, P);
+    }
     if (ev) (void)hipEventRecord(ev[0], st);
     // y forward
     P.pl = pl.py;
@@ -863,25 +874,26 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     if (ev) (void)hipEventRecord(ev[3], st);
     // x inverse (c2r) writing dp
     P.pl = pl.px;
-    P.lsh = fft_lsh(g.nx, FFT_XI_ELEMS);
+    P.lsh = fft_lsh(g.nx, xi_small ? FFT_XI_ELEMS : FFT_ELEMS);
     P.LS = fft_row(g.nx);
     const unsigned nbxi = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
-    if (pow2(g.nx)) launch_lds(k_fft_x_inv<false>, nbxi, lds_x(), st, P);
-    else launch_lds(k_fft_x_inv<true>, nbxi, lds_x(), st, P);
+    if (xi_small) {
+        if (pow2(g.nx)) launch_lds(k_fft_x_inv<false, FFT_XI_ELEMS>, nbxi, lds_x(), st, P);
+        else launch_lds(k_fft_x_inv<true, FFT_XI_ELEMS>, nbxi, lds_x(), st, P);
+    } else {
+        if (pow2(g.nx)) launch_lds(k_fft_x_inv<false, FFT_ELEMS>, nbxi, lds_x(), st, P);
+        else launch_lds(k_fft_x_inv<true, FFT_ELEMS>, nbxi, lds_x(), st, P);
+    }
     if (ev) (void)hipEventRecord(ev[4], st);
 }
 
 namespace {
 
-// x passes hold fewer complex elements per block than the y/z passes
-// (contiguous lines need no grouping for coalescing)
-constexpr int DCT_X_ELEMS = 1024;
-
 // log2 of the lines per block of a y/z pass over rows of nx values: the
-// largest L = 2^s with L*N <= FFT_ELEMS that leaves the fewest idle lines
-// past nx, with 2L >= 16 (a 128-byte row segment) where that fits
-int dct_yz_lsh(int nx, int N) {
-    const int smax = fft_lsh(N);
+// largest L = 2^s with L*N <= el that leaves the fewest idle lines past nx,
+// with 2L >= 16 (a 128-byte row segment) where that fits
+int dct_yz_lsh(int nx, int N, int el) {
+    const int smax = fft_lsh(N, el);
     const int smin = smax < 3 ? smax : 3;
     int best = smax;
     long bw = -1;
@@ -902,18 +914,38 @@ bool plan_has_odd(const FFTPlan &pl) {
     return false;
 }
 
-template <int PASS, bool ODD>
+template <int PASS, bool ODD, int EL>
 void dct_launch(const PassParams &P, int nb, size_t lds, hipStream_t st) {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)k_dct<PASS, ODD>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_dct<PASS, ODD>), dim3(nb), dim3(FFT_NT), lds, st, P);
+        (void)hipFuncSetAttribute((const void *)k_dct<PASS, ODD, EL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+    hipLaunchKernelGGL((k_dct<PASS, ODD, EL>), dim3(nb), dim3(FFT_NT), lds, st, P);
 }
-template <int PASS, bool ODD>
+template <int PASS, bool ODD, int EL>
 void dct_launch_batch(const PassParams *Ps, const int *prefix, int nbox, int nb, size_t lds, hipStream_t st) {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)k_dct_batch<PASS, ODD>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
-    hipLaunchKernelGGL((k_dct_batch<PASS, ODD>), dim3(nb), dim3(FFT_NT), lds, st, Ps, prefix, nbox);
+        (void)hipFuncSetAttribute((const void *)k_dct_batch<PASS, ODD, EL>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_dct_batch<PASS, ODD, EL>), dim3(nb), dim3(FFT_NT), lds, st, Ps, prefix, nbox);
+}
+// variant index: odd radices, long lines
+template <int PASS>
+void dct_launch_v(int v, const PassParams &P, int nb, size_t lds, hipStream_t st) {
+    switch (v) {
+        case 0: dct_launch<PASS, false, DCT_ELEMS>(P, nb, lds, st); break;
+        case 1: dct_launch<PASS, false, FFT_ELEMS>(P, nb, lds, st); break;
+        case 2: dct_launch<PASS, true, DCT_ELEMS>(P, nb, lds, st); break;
+        default: dct_launch<PASS, true, FFT_ELEMS>(P, nb, lds, st); break;
+    }
+}
+template <int PASS>
+void dct_launch_batch_v(int v, const PassParams *Ps, const int *prefix, int nbox, int nb, size_t lds, hipStream_t st) {
+    switch (v) {
+        case 0: dct_launch_batch<PASS, false, DCT_ELEMS>(Ps, prefix, nbox, nb, lds, st); break;
+        case 1: dct_launch_batch<PASS, false, FFT_ELEMS>(Ps, prefix, nbox, nb, lds, st); break;
+        case 2: dct_launch_batch<PASS, true, DCT_ELEMS>(Ps, prefix, nbox, nb, lds, st); break;
+        default: dct_launch_batch<PASS, true, FFT_ELEMS>(Ps, prefix, nbox, nb, lds, st); break;
+    }
 }
 
 }  // namespace
@@ -952,17 +984,19 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
     P.wq = wq[ax];
     P.pbc = pl.pbc[ax];
     P.odd = plan_has_odd(P.pl) ? 1 : 0;
+    P.big = n[ax] > DCT_ELEMS ? 1 : 0;
+    const int el = P.big ? FFT_ELEMS : DCT_ELEMS;
     P.dN = make_idiv(g.nx);
     P.dNy = make_idiv(g.ny);
     if (ax == 0) {
-        P.lsh = fft_lsh(g.nx, DCT_X_ELEMS);
+        P.lsh = fft_lsh(g.nx, el);
         P.LS = fft_row(g.nx);
         *lds = ((size_t)(1 << P.lsh) * P.LS + g.nx) * sizeof(double2);
         const long RL = (long)g.ny * g.nz, per = 2L << P.lsh;
         return (int)((RL + per - 1) / per);
     }
     const int N = n[ax];
-    P.lsh = dct_yz_lsh(g.nx, N);
+    P.lsh = dct_yz_lsh(g.nx, N, el);
     P.ngrp = (int)((g.nx + (2L << P.lsh) - 1) / (2L << P.lsh));
     P.dGrp = make_idiv(P.ngrp);
     *lds = ((size_t)(1 << P.lsh) * N + N) * sizeof(double2);
@@ -971,38 +1005,32 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
 
 size_t pass_params_size() { return sizeof(PassParams); }
 
-int dct_params_odd(const void *params) { return ((const PassParams *)params)->odd; }
+int dct_params_variant(const void *params) {
+    const PassParams &P = *(const PassParams *)params;
+    return (P.odd ? 2 : 0) | (P.big ? 1 : 0);
+}
 
 void launch_dct_pass(int pass, const void *params, int nb, size_t lds, hipStream_t st) {
     const PassParams &P = *(const PassParams *)params;
-    switch (pass * 2 + (P.odd ? 1 : 0)) {
-        case 0: dct_launch<0, false>(P, nb, lds, st); break;
-        case 1: dct_launch<0, true>(P, nb, lds, st); break;
-        case 2: dct_launch<1, false>(P, nb, lds, st); break;
-        case 3: dct_launch<1, true>(P, nb, lds, st); break;
-        case 4: dct_launch<2, false>(P, nb, lds, st); break;
-        case 5: dct_launch<2, true>(P, nb, lds, st); break;
-        case 6: dct_launch<3, false>(P, nb, lds, st); break;
-        case 7: dct_launch<3, true>(P, nb, lds, st); break;
-        case 8: dct_launch<4, false>(P, nb, lds, st); break;
-        case 9: dct_launch<4, true>(P, nb, lds, st); break;
+    const int v = dct_params_variant(params);
+    switch (pass) {
+        case 0: dct_launch_v<0>(v, P, nb, lds, st); break;
+        case 1: dct_launch_v<1>(v, P, nb, lds, st); break;
+        case 2: dct_launch_v<2>(v, P, nb, lds, st); break;
+        case 3: dct_launch_v<3>(v, P, nb, lds, st); break;
+        default: dct_launch_v<4>(v, P, nb, lds, st); break;
     }
 }
 
-void launch_dct_batch(int pass, int odd, const void *params, const int *prefix, int nbox, int nb, size_t lds,
+void launch_dct_batch(int pass, int variant, const void *params, const int *prefix, int nbox, int nb, size_t lds,
                       hipStream_t st) {
     const PassParams *Ps = (const PassParams *)params;
-    switch (pass * 2 + (odd ? 1 : 0)) {
-        case 0: dct_launch_batch<0, false>(Ps, prefix, nbox, nb, lds, st); break;
-        case 1: dct_launch_batch<0, true>(Ps, prefix, nbox, nb, lds, st); break;
-        case 2: dct_launch_batch<1, false>(Ps, prefix, nbox, nb, lds, st); break;
-        case 3: dct_launch_batch<1, true>(Ps, prefix, nbox, nb, lds, st); break;
-        case 4: dct_launch_batch<2, false>(Ps, prefix, nbox, nb, lds, st); break;
-        case 5: dct_launch_batch<2, true>(Ps, prefix, nbox, nb, lds, st); break;
-        case 6: dct_launch_batch<3, false>(Ps, prefix, nbox, nb, lds, st); break;
-        case 7: dct_launch_batch<3, true>(Ps, prefix, nbox, nb, lds, st); break;
-        case 8: dct_launch_batch<4, false>(Ps, prefix, nbox, nb, lds, st); break;
-        case 9: dct_launch_batch<4, true>(Ps, prefix, nbox, nb, lds, st); break;
+    switch (pass) {
+        case 0: dct_launch_batch_v<0>(variant, Ps, prefix, nbox, nb, lds, st); break;
+        case 1: dct_launch_batch_v<1>(variant, Ps, prefix, nbox, nb, lds, st); break;
+        case 2: dct_launch_batch_v<2>(variant, Ps, prefix, nbox, nb, lds, st); break;
+        case 3: dct_launch_batch_v<3>(variant, Ps, prefix, nbox, nb, lds, st); break;
+        default: dct_launch_batch_v<4>(variant, Ps, prefix, nbox, nb, lds, st); break;
     }
 }
 

@@ -511,9 +511,15 @@ int choose_zc(int nz, long txy, int slots) {
 
 size_t predict_params_size() { return sizeof(TileParams); }
 
+long predict_tile_columns(const ns_config &c) {
+    return (long)((c.nx + TX - 1) / TX) * ((c.ny + TY - 1) / TY) * c.nz;
+}
+
+int predict_resident_blocks(bool lazy) { return resident_blocks(lazy); }
+
 int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                        double *us, double *vs, double *ws, void *out, double share) {
+                        double *us, double *vs, double *ws, void *out, int zc_hint) {
     TileParams &P = *(TileParams *)out;
     P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
     P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
@@ -526,10 +532,14 @@ int make_predict_params(const ns_config &c, int first, bool lazy, const double *
     for (int a = 0; a < 3; a++) P.per[a] = (c.bc[a] == 1);
     P.tiles_x = (c.nx + TX - 1) / TX;
     P.tiles_y = (c.ny + TY - 1) / TY;
-    // a box alone fills the device; in a batch it gets its share of the resident blocks
-    int slots = (int)(resident_blocks(lazy) * share);
-    if (slots < 1) slots = 1;
-    P.zc = choose_zc(c.nz, (long)P.tiles_x * P.tiles_y, slots);
+    if (zc_hint > 0) {
+        // equal chunks of about zc_hint cells
+        const int tz = (c.nz + zc_hint - 1) / zc_hint;
+        P.zc = (c.nz + tz - 1) / tz;
+    } else {
+        // a box alone: chunks that fill whole rounds of the resident blocks
+        P.zc = choose_zc(c.nz, (long)P.tiles_x * P.tiles_y, resident_blocks(lazy));
+    }
     P.tiles_z = (c.nz + P.zc - 1) / P.zc;
     P.ntiles = P.tiles_x * P.tiles_y * P.tiles_z;
     P.src[0] = u; P.src[1] = v; P.src[2] = w;
@@ -552,7 +562,7 @@ void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double
                           const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
                           double *us, double *vs, double *ws, hipStream_t st) {
     TileParams P;
-    const int nblk = make_predict_params(c, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P, 1.0);
+    const int nblk = make_predict_params(c, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P, 0);
     if (lazy)
         hipLaunchKernelGGL(k_predict_tiled<true>, dim3(nblk), dim3(TX * TY), 0, st, P);
     else

@@ -334,12 +334,15 @@ struct ProjParams {
     const double *s[3];
     const double *dp, *gauge;
     double *o[4];
+    double r0[3], r1[3];  // 1/n0 and 1/n1 of the velocity components' dims
 };
 
+// one element q of component c (c = 3: p) of the projection
+// (big_periodic_3D.f90:562-567): u = u* - dt*(dp(i+1)-dp(i))/dx, p = dp_flag*p + dp
 __device__ __forceinline__ void project_body(const ProjParams &P, const size_t q, const int c) {
     int n[3];
     comp_dims(P.g, c, n);
-    size_t tot = (size_t)n[0] * n[1] * n[2];
+    const size_t tot = (size_t)n[0] * n[1] * n[2];
     if (q >= tot) return;
     const double gc = P.gauge ? *P.gauge : 0.0;
     if (c == 3) {
@@ -347,9 +350,11 @@ __device__ __forceinline__ void project_body(const ProjParams &P, const size_t q
         P.o[3][q] = (double)P.dp_flag * P.o[3][q] + dpv;
         return;
     }
-    int i = (int)(q % n[0]) + 1;
-    size_t r = q / n[0];
-    int j = (int)(r % n[1]) + 1, k = (int)(r / n[1]) + 1;
+    // arrays hold fewer than 2^32 elements (checked at context creation)
+    const unsigned qq = (unsigned)q;
+    const unsigned r = udiv_rd(qq, (unsigned)n[0], P.r0[c]);
+    const unsigned kk = udiv_rd(r, (unsigned)n[1], P.r1[c]);
+    const int i = (int)(qq - r * (unsigned)n[0]) + 1, j = (int)(r - kk * (unsigned)n[1]) + 1, k = (int)kk + 1;
     int b[3] = {i, j, k};
     b[c]++;
     const int P1 = P.g.nx + 2, P2 = P.g.ny + 2;
@@ -359,15 +364,23 @@ __device__ __forceinline__ void project_body(const ProjParams &P, const size_t q
     P.o[c][q] = P.s[c][q] - P.dt * gr / P.d[c];
 }
 
+// a block projects PROJ_PER * 256 consecutive elements of one component,
+// 256 apart per thread (coalesced, independent loads in flight)
+constexpr int PROJ_PER = 4;
+
 __global__ void __launch_bounds__(256) k_project(ProjParams P) {
-    project_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+    const size_t q0 = (size_t)blockIdx.x * (256 * PROJ_PER) + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < PROJ_PER; m++) project_body(P, q0 + (size_t)m * 256, blockIdx.y);
 }
 
 __global__ void __launch_bounds__(256) k_project_batch(const ProjParams *__restrict__ Ps, const int *__restrict__ prefix,
                                                        int nbox) {
     int blk;
     const ProjParams &P = batch_params(Ps, prefix, nbox, blk);
-    project_body(P, (size_t)blk * blockDim.x + threadIdx.x, blockIdx.y);
+    const size_t q0 = (size_t)blk * (256 * PROJ_PER) + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < PROJ_PER; m++) project_body(P, q0 + (size_t)m * 256, blockIdx.y);
 }
 
 struct GetBCParams {
@@ -730,15 +743,9 @@ void launch_project(const Grid &g, double dx, double dy, double dz, double dt, i
                     const double *vs, const double *ws, const double *dp, const double *gauge, double *u, double *v,
                     double *w, double *p, hipStream_t st) {
     ProjParams P;
-    P.g = g;
-    P.d[0] = dx; P.d[1] = dy; P.d[2] = dz;
-    P.dt = dt;
-    P.dp_flag = dp_flag;
-    P.s[0] = us; P.s[1] = vs; P.s[2] = ws;
-    P.dp = dp;
-    P.gauge = gauge;
-    P.o[0] = u; P.o[1] = v; P.o[2] = w; P.o[3] = p;
-    hipLaunchKernelGGL(k_project, dim3(nblocks(g.np()), 4), dim3(256), 0, st, P);
+    const double d[3] = {dx, dy, dz};
+    const int nb = make_project_params(g, d, dt, dp_flag, us, vs, ws, dp, gauge, u, v, w, p, &P);
+    hipLaunchKernelGGL(k_project, dim3(nb, 4), dim3(256), 0, st, P);
 }
 
 void launch_get_velpr_bc(const Grid &g, const int bc[3], const int pbc[3], double dx, double dy, double dz, double dt,
@@ -911,7 +918,13 @@ int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag
     P.dp = dp;
     P.gauge = gauge;
     P.o[0] = u; P.o[1] = v; P.o[2] = w; P.o[3] = p;
-    return (int)nblocks(g.np());
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        P.r0[c] = 1.0 / n[0];
+        P.r1[c] = 1.0 / n[1];
+    }
+    return (int)nblocks(g.np(), 256 * PROJ_PER);
 }
 
 void launch_vel_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
