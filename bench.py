@@ -34,9 +34,9 @@ ARRAYS_PER_CELL = {"predictor": 13, "vel_bc": 0, "div_fft_x_fwd": 4, "fft_y_fwd"
                    "fft_y_inv": 2, "fft_x_inv_dp": 2, "pr_bc": 0, "project": 0}
 # the reference pipeline's budget (explicit projection pass): 32 arrays = 256 B/cell-step
 REFERENCE_BUDGET_B = 256
-KERNEL_NAMES = {"predictor": "k_predict_tiled<true>", "div_fft_x_fwd": "k_fft_x_fwd<false>",
+KERNEL_NAMES = {"predictor": "k_predict_tiled<true>", "div_fft_x_fwd": "k_fft_x_fwd<false,512>",
                 "fft_y_fwd": "k_fft_yz<1,-1,0,false>", "fft_z_solve": "k_fft_yz<2,-1,1,false>",
-                "fft_y_inv": "k_fft_yz<1,1,0,false>", "fft_x_inv_dp": "k_fft_x_inv<false>", "project": "k_project"}
+                "fft_y_inv": "k_fft_yz<1,1,0,false>", "fft_x_inv_dp": "k_fft_x_inv<false,1024>", "project": "k_project"}
 COPY_CAL_BYTES = 2 ** 30  # the device-to-device copy used to scale the HBM counters
 
 
@@ -174,26 +174,31 @@ def batch_throughput(nbox, steps=20):
     """Batched re-simulation of nbox sub-boxes of uneven sizes (walls bc=2,
     pbc=3, AB2, DCT, dp_flag=1; sizes drawn per axis from the 7-smooth lengths
     in [32, 128], seed 4): Mcell-steps/s of ns_batch_step over all boxes, and
-    of the same boxes stepped one after another (each with ns_step and its own
-    graph replay)."""
-    from ns_3d_staggered_dp_amd import AB2, NSConfig, fields
+    of the same boxes stepped one after another, each with its own launches
+    (eager: every kernel launched per step; graph: steady steps replayed from
+    the box's own HIP graph)."""
+    from ns_3d_staggered_dp_amd import AB2, FLAG_NO_GRAPH, NSConfig, fields
     from ns_3d_staggered_dp_amd.batch import Batch, box_sizes
     from ns_3d_staggered_dp_amd.grid import plane_set_length
     from ns_3d_staggered_dp_amd.solver import NSSolver
 
     sizes = box_sizes(nbox, 32, 128, seed=4)
-    solvers = []
-    cells = 0
-    try:
+    cells = sum(nx * ny * nz for nx, ny, nz in sizes)
+
+    def make(flags):
+        out = []
         for i, (nx, ny, nz) in enumerate(sizes):
-            cfg = NSConfig(nx=nx, ny=ny, nz=nz, dx=2 * np.pi / 256, dy=2 * np.pi / 256, dz=2 * np.pi / 256, nu=1e-3,
-                           dt=5e-4, scheme=AB2, bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
+            h = 2 * np.pi / 256
+            cfg = NSConfig(nx=nx, ny=ny, nz=nz, dx=h, dy=h, dz=h, nu=1e-3, dt=5e-4, scheme=AB2, bc=(2, 2, 2),
+                           pbc=(3, 3, 3), dp_flag=1, flags=flags)
             s = NSSolver(cfg)
+            out.append(s)
             u, v, w, p = fields.random_ghosted(cfg, 400 + i)
             s.upload(0.1 * u, 0.1 * v, 0.1 * w, p, first=True)
             s.set_bc_planes(0.01 * np.random.default_rng(500 + i).standard_normal(plane_set_length(nx, ny, nz)))
-            solvers.append(s)
-            cells += nx * ny * nz
+        return out
+
+    def sequential(solvers):
         for s in solvers:
             s.step(3)
             s.synchronize()
@@ -201,19 +206,29 @@ def batch_throughput(nbox, steps=20):
         for s in solvers:
             s.step(steps)
             s.synchronize()
-        seq = time.perf_counter() - t0
-        with Batch(solvers) as b:
-            b.step(3)
-            t0 = time.perf_counter()
-            b.step(steps)
-            bat = time.perf_counter() - t0
-    finally:
-        for s in solvers:
-            s.close()
+        return time.perf_counter() - t0
+
+    res = {}
+    for name, flags in (("eager", FLAG_NO_GRAPH), ("graph", 0)):
+        solvers = []
+        try:
+            solvers = make(flags)
+            res[name] = sequential(solvers)
+            if flags == 0:
+                with Batch(solvers) as b:
+                    b.step(3)
+                    t0 = time.perf_counter()
+                    b.step(steps)
+                    res["batch"] = time.perf_counter() - t0
+        finally:
+            for s in solvers:
+                s.close()
+    rate = {k: round(cells * steps / v / 1e6, 1) for k, v in res.items()}
     return {"config": f"{nbox} sub-boxes, sizes 32..128 per axis (7-smooth, seed 4), walls bc=2 pbc=3, AB2, DCT, "
                       f"dp_flag=1", "cells": cells, "steps": steps,
-            "batched_mcell_steps_s": round(cells * steps / bat / 1e6, 1),
-            "sequential_mcell_steps_s": round(cells * steps / seq / 1e6, 1), "speedup": round(seq / bat, 3)}
+            "batched_mcell_steps_s": rate["batch"], "sequential_graph_mcell_steps_s": rate["graph"],
+            "sequential_eager_mcell_steps_s": rate["eager"], "speedup_vs_graph": round(res["graph"] / res["batch"], 3),
+            "speedup_vs_eager": round(res["eager"] / res["batch"], 3)}
 
 
 def copy_calibration():

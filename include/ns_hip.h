@@ -172,8 +172,10 @@ int ns_extract_subbox(ns_ctx *ctx, const ns_subbox *sb, int field, double *dst);
 
 /* ---- discrete adjoint of ns_step (no counterpart in the reference: the
  * gradient side of the re-simulation) ----
- * Explicit schemes (Euler, AB2) on contexts that project every step (dp_flag
- * = 1, or NS_FLAG_EAGER_PROJECTION); NS_ERR_UNSUPPORTED otherwise.  The
+ * Euler and AB2 on any codes, and AB2-CN when every velocity code is a wall
+ * (2 or 4: the ADI sweeps' adjoints are solves with the transposed
+ * tridiagonal systems), on contexts that project every step (dp_flag = 1,
+ * or NS_FLAG_EAGER_PROJECTION); NS_ERR_UNSUPPORTED otherwise.  The
  * adjoint state is (u_b, v_b, w_b, p_b, hx_b, hy_b, hz_b), the gradients of a
  * scalar objective with respect to the state (u, v, w, p, rhs_*_previous)
  * at the current point of the reverse sweep, all in the ghosted shapes. */

@@ -35,6 +35,8 @@ struct ns_ctx {
     unsigned int *counter = nullptr;
     PoissonPlans pp{};
     LineSys cn[3][2]{};
+    LineSys cnT[3][2]{};  // transposed CN systems (adjoint of the ADI sweeps, walls)
+    int cnT_ok = 0;       // cnT matches the current dt
     int first = 1;
     // parent-run capture of a sub-box's boundary plane sets
     int cap_on = 0, cap_every = 1, cap_max = 0, cap_count = 0, cap_step = 0;
@@ -54,6 +56,10 @@ struct ns_ctx {
     double *ab[3] = {nullptr, nullptr, nullptr}, *apb = nullptr, *ahb[3] = {nullptr, nullptr, nullptr};
     double *ausb[3] = {nullptr, nullptr, nullptr}, *adpb = nullptr, *agb[3] = {nullptr, nullptr, nullptr};
     double *ar = nullptr, *arsol = nullptr, *apbar = nullptr;
+    // adjoint of the convective part of the right-hand side (AB2-CN: the
+    // diffusive and convective terms enter u* and H with different weights;
+    // explicit schemes alias agb)
+    double *agcb[3] = {nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
 };
 
@@ -197,8 +203,9 @@ std::vector<double> thomas_solve(const std::vector<double> &lo, const std::vecto
     return r;
 }
 
-// CN line system for direction ax (0..2) and type A (face-normal, n+1) or B (n+2)
-int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S) {
+// CN line system for direction ax (0..2) and type A (face-normal, n+1) or B (n+2);
+// transpose: the system of A^T (walls only), whose solve is the adjoint of the sweep
+int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = false) {
     const ns_config &f = c->cfg;
     const int n = (ax == 0) ? f.nx : (ax == 1) ? f.ny : f.nz;
     const double h = (ax == 0) ? f.dx : (ax == 1) ? f.dy : f.dz;
@@ -211,6 +218,7 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S) {
     S.a = a;
     std::vector<double> lo, d, up, inv, cp;
     if (bc == 1) {
+        if (transpose) return NS_ERR_UNSUPPORTED;
         const int m = n;
         S.periodic = 1;
         S.m = m;
@@ -259,6 +267,14 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S) {
         } else {
             d[0] = 0.5; up[0] = 0.5;
             d[N - 1] = 0.5; lo[N - 1] = 0.5;
+        }
+        if (transpose) {
+            // (A^T)_{i,i-1} = A_{i-1,i}, (A^T)_{i,i+1} = A_{i+1,i}
+            std::vector<double> lt(N, 0.0), ut(N, 0.0);
+            for (int i = 1; i < N; i++) lt[i] = up[i - 1];
+            for (int i = 0; i + 1 < N; i++) ut[i] = lo[i + 1];
+            lo.swap(lt);
+            up.swap(ut);
         }
         thomas_factor(lo, d, up, inv, cp);
         S.z = nullptr;
@@ -900,6 +916,7 @@ int ns_set_dt(ns_ctx *c, double dt) {
         HIPCHK(hipMemsetAsync(c->dp, 0, g.np() * 8, c->st));
     }
     c->cfg.dt = dt;
+    c->cnT_ok = 0;
     if (c->cfg.scheme == NS_AB2_CN)
         for (int ax = 0; ax < 3; ax++)
             for (int t = 0; t < 2; t++) {
@@ -941,6 +958,7 @@ static int adj_alloc(ns_ctx *c) {
     if (c->ab[0]) return NS_OK;
     const Grid &g = c->g;
     const size_t n[3] = {g.nu(), g.nv(), g.nw()};
+    const bool cn = c->cfg.scheme == NS_AB2_CN;
     int rc;
     // the u* adjoint lives in the u adjoint's arrays: S5^T reads the u adjoint
     // once (into the dp adjoint) and the u* adjoint then starts equal to it
@@ -948,6 +966,11 @@ static int adj_alloc(ns_ctx *c) {
         if ((rc = dalloc(c, &c->ab[a], n[a])) || (rc = dalloc(c, &c->ahb[a], n[a])) || (rc = dalloc(c, &c->agb[a], n[a])))
             return rc;
         c->ausb[a] = c->ab[a];
+        if (cn) {
+            if ((rc = dalloc(c, &c->agcb[a], n[a]))) return rc;
+        } else {
+            c->agcb[a] = c->agb[a];
+        }
     }
     if ((rc = dalloc(c, &c->apb, g.np())) || (rc = dalloc(c, &c->adpb, g.np())) || (rc = dalloc(c, &c->ar, g.ni())) ||
         (rc = dalloc(c, &c->arsol, g.ni())) || (rc = dalloc(c, &c->apbar, (size_t)c->L.total)))
@@ -956,8 +979,26 @@ static int adj_alloc(ns_ctx *c) {
     return NS_OK;
 }
 
+// explicit schemes on any codes; AB2-CN with wall velocity codes (2, 4) on
+// every axis, whose ADI systems are plain tridiagonal
 static bool adj_supported(const ns_ctx *c) {
-    return !c->lazy && c->cfg.scheme != NS_AB2_CN && c->poisson_ok;
+    if (c->lazy || !c->poisson_ok) return false;
+    if (c->cfg.scheme == NS_AB2_CN)
+        for (int a = 0; a < 3; a++)
+            if (c->cfg.bc[a] != 2 && c->cfg.bc[a] != 4) return false;
+    return true;
+}
+
+// transposed CN systems for the current dt
+static int adj_cn_systems(ns_ctx *c) {
+    if (c->cfg.scheme != NS_AB2_CN || c->cnT_ok) return NS_OK;
+    for (int ax = 0; ax < 3; ax++)
+        for (int t = 0; t < 2; t++) {
+            const int rc = make_line_sys(c, ax, t, c->cnT[ax][t], true);
+            if (rc) return rc;
+        }
+    c->cnT_ok = 1;
+    return NS_OK;
 }
 
 int ns_adjoint_set_state(ns_ctx *c, const double *ub, const double *vb, const double *wb, const double *pb,
@@ -1028,12 +1069,23 @@ int ns_adjoint_step(ns_ctx *c, const double *u, const double *v, const double *w
     }
     if ((rc = run_poisson(c, nullptr, nullptr, nullptr, c->ar, nullptr, nullptr, 0.0, nullptr, c->arsol))) return rc;
     launch_adj_rhs_div(g, f.pbc, h, f.dt, c->arsol, c->ausb, c->apbar, c->st);
-    // S2^T: velocity ghost fill
-    launch_adj_fill(g, 0, f.bc, h, c->ausb, c->apbar, c->st);
+    if (f.scheme != NS_AB2_CN) {
+        // S2^T: velocity ghost fill
+        launch_adj_fill(g, 0, f.bc, h, c->ausb, c->apbar, c->st);
+    } else {
+        // S2^T for AB2-CN: the sweeps in reverse, each a solve with A^T, then
+        // the transpose of its boundary rows (vel_bc_staggered_CN2)
+        if ((rc = adj_cn_systems(c))) return rc;
+        for (int dir = 3; dir >= 1; dir--) {
+            launch_adi_sweep(g, dir, c->cnT[dir - 1][0], c->cnT[dir - 1][1], c->ausb[0], c->ausb[1], c->ausb[2],
+                             c->st);
+            launch_adj_cn2(g, dir - 1, c->ausb, c->apbar, c->st);
+        }
+    }
     // S1^T: predictor
     const double *uu[3] = {AU.dev, AV.dev, AW.dev};
-    launch_adj_predict(g, per, f.dt, f.nu, f.dp_flag, h, f.scheme, first, uu, c->ausb, c->ahb, c->agb, c->ab, c->apb,
-                       c->st);
+    launch_adj_predict(g, per, f.dt, f.nu, f.dp_flag, h, f.scheme, first, uu, c->ausb, c->ahb, c->agb, c->agcb, c->ab,
+                       c->apb, c->st);
     if (planes_bar) HIPCHK(hipMemcpyAsync(planes_bar, c->apbar, (size_t)c->L.total * 8, hipMemcpyDefault, c->st));
     // host outputs and staged host inputs need the stream drained; device ones stay stream-ordered
     if ((planes_bar && !is_dev_ptr(planes_bar)) || AU.owned || AV.owned || AW.owned)

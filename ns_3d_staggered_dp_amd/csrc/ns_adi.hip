@@ -16,21 +16,77 @@ namespace ns {
 
 namespace {
 
+// The sweep is one thread per line, so each elimination step would wait for
+// its own load: lines are walked in register chunks of CHK elements, the next
+// chunk's loads issued before the current chunk's dependent arithmetic.  The
+// coefficients are line-independent and read at a wave-uniform index through
+// the constant address space (scalar loads).
+constexpr int CHK = 16;
+typedef const __attribute__((address_space(4))) double cdouble;
+
+// forward elimination over y[0..m) (stride s): d_i = (r_i - lo_i d_{i-1}) inv_i,
+// stored in place; r_0 and (m > 1) r_{m-1} are replaced by r_first, r_last when ovr.
+// Returns d_{m-1}.
+__device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const double *lo_, const double *inv_,
+                                             bool ovr, double r_first, double r_last) {
+    cdouble *lo = (cdouble *)lo_, *inv = (cdouble *)inv_;
+    double cur[CHK], nxt[CHK];
+#pragma unroll
+    for (int j = 0; j < CHK; j++) cur[j] = (j < m) ? y[s * j] : 0.0;
+    double d = 0.0;
+    for (int i0 = 0; i0 < m; i0 += CHK) {
+#pragma unroll
+        for (int j = 0; j < CHK; j++) nxt[j] = (i0 + CHK + j < m) ? y[s * (i0 + CHK + j)] : 0.0;
+#pragma unroll
+        for (int j = 0; j < CHK; j++) {
+            const int i = i0 + j;
+            if (i < m) {
+                double r = cur[j];
+                if (ovr) {
+                    if (i == 0) r = r_first;
+                    else if (i == m - 1) r = r_last;
+                }
+                d = (i == 0) ? r * inv[0] : (r - lo[i] * d) * inv[i];
+                y[s * i] = d;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < CHK; j++) cur[j] = nxt[j];
+    }
+    return d;
+}
+
+// back substitution x_i = d_i - cp_i x_{i+1}, i = m-2 .. 0, with x_{m-1} = last
+// already in place; returns x_0
+__device__ __forceinline__ double thomas_bwd(double *y, size_t s, int m, const double *cp_, double last) {
+    cdouble *cp = (cdouble *)cp_;
+    double cur[CHK], nxt[CHK];
+    const int top = m - 2;
+#pragma unroll
+    for (int j = 0; j < CHK; j++) cur[j] = (top - j >= 0) ? y[s * (top - j)] : 0.0;
+    double x = last;
+    for (int i0 = top; i0 >= 0; i0 -= CHK) {
+#pragma unroll
+        for (int j = 0; j < CHK; j++) nxt[j] = (i0 - CHK - j >= 0) ? y[s * (i0 - CHK - j)] : 0.0;
+#pragma unroll
+        for (int j = 0; j < CHK; j++) {
+            const int i = i0 - j;
+            if (i >= 0) {
+                x = cur[j] - cp[i] * x;
+                y[s * i] = x;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < CHK; j++) cur[j] = nxt[j];
+    }
+    return x;
+}
+
 // solve one line in place (global memory, stride s)
 __device__ __forceinline__ void solve_line(double *x, size_t s, const LineSys &S) {
     if (!S.periodic) {
-        const int N = S.N;
-        double d = x[0] * S.inv[0];
-        x[0] = d;
-        for (int i = 1; i < N; i++) {
-            d = (x[s * i] - S.lo[i] * d) * S.inv[i];
-            x[s * i] = d;
-        }
-        double nxt = d;
-        for (int i = N - 2; i >= 0; i--) {
-            nxt = x[s * i] - S.cp[i] * nxt;
-            x[s * i] = nxt;
-        }
+        const double last = thomas_fwd(x, s, S.N, S.lo, S.inv, false, 0.0, 0.0);
+        thomas_bwd(x, s, S.N, S.cp, last);
         return;
     }
     const int m = S.m;
@@ -48,28 +104,27 @@ __device__ __forceinline__ void solve_line(double *x, size_t s, const LineSys &S
         r_first = y[0] + S.a * g_lo;
         r_last = y[s * (m - 1)] + S.a * g_hi;
     }
-    // Thomas forward on the reduced system
-    double d = r_first * S.inv[0];
-    y[0] = d;
-    for (int i = 1; i < m; i++) {
-        double r = (i == m - 1) ? r_last : y[s * i];
-        d = (r - S.lo[i] * d) * S.inv[i];
-        y[s * i] = d;
-    }
-    double nxt = d;
-    const double ylast = d;
-    for (int i = m - 2; i >= 0; i--) {
-        nxt = y[s * i] - S.cp[i] * nxt;
-        y[s * i] = nxt;
-    }
+    // Thomas on the reduced system (rows 0 and m-1 carry the folded ghosts)
+    const double ylast = thomas_fwd(y, s, m, S.lo, S.inv, true, r_first, r_last);
+    const double y0 = thomas_bwd(y, s, m, S.cp, ylast);
     // Sherman-Morrison correction
-    const double f = (nxt + S.s * ylast) / S.den;
+    const double f = (y0 + S.s * ylast) / S.den;
+    cdouble *z = (cdouble *)S.z;
     double x0 = 0.0, xl = 0.0;
-    for (int i = 0; i < m; i++) {
-        double v = y[s * i] - f * S.z[i];
-        y[s * i] = v;
-        if (i == 0) x0 = v;
-        if (i == m - 1) xl = v;
+    for (int i0 = 0; i0 < m; i0 += CHK) {
+        double v[CHK];
+#pragma unroll
+        for (int j = 0; j < CHK; j++) v[j] = (i0 + j < m) ? y[s * (i0 + j)] : 0.0;
+#pragma unroll
+        for (int j = 0; j < CHK; j++) {
+            const int i = i0 + j;
+            if (i < m) {
+                const double w = v[j] - f * z[i];
+                y[s * i] = w;
+                if (i == 0) x0 = w;
+                if (i == m - 1) xl = w;
+            }
+        }
     }
     if (S.off == 0) {
         x[s * m] = x0 + g_hi;

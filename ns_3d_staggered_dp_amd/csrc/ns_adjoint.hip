@@ -1,12 +1,16 @@
-// Discrete adjoint of one explicit time step (Euler / AB2 predictor, velocity
-// fill, divergence + boundary RHS terms, Poisson solve, pressure fill,
-// projection), FP64, for gradients of a function of the final state with
-// respect to the initial state and to every step's boundary planes.
+// Discrete adjoint of one time step (Euler / AB2 / AB2-CN predictor, velocity
+// fill or, for AB2-CN, the boundary rows and ADI sweeps, divergence +
+// boundary RHS terms, Poisson solve, pressure fill, projection), FP64, for
+// gradients of a function of the final state with respect to the initial
+// state and to every step's boundary planes.
 //
 // Forward step (as ns_step runs it without lazy projection):
 //   S1 u* = u + dt(c1 rhs(u) - c2 H - dp_flag grad p),  H' = rhs(u) on the
 //      computed faces, rhs = nu diff(u) - conv(u)       (big_periodic_3D.f90:350-396)
-//   S2 ghost fill of u* from the planes, x then y then z   (NS_functions.f90:213-310)
+//   S2 ghost fill of u* from the planes, x then y then z   (NS_functions.f90:213-310);
+//      AB2-CN instead, per axis x, y, z: line ends of u* := planes
+//      (NS_functions.f90:313-410), then u* := A^{-1} u* along that axis
+//      (big_periodic_3D.f90:425-483); the adjoint solves with A^T
 //   S3 r = div(u*)/dt + boundary terms; dp = P r + mean_dp  (…interBC:771, NS_functions.f90:1345-1400)
 //   S4 ghost fill of dp, x then y then z                  (NS_functions.f90:1403-1464)
 //   S5 u = u* - dt grad dp (all faces), p = dp_flag p + dp (big_periodic_3D.f90:564-567)
@@ -249,12 +253,15 @@ struct AdjPred {
     Grid g;
     int per[3];
     double dt, nu, dpf, rd3[3], rd2[3];  // 1/h and 1/h^2
-    double c_rhs, c_hist;                // u* = ... + dt*(c_rhs*rhs - c_hist*H ...); H' = rhs if keep_h
-    int keep_h;                          // H' = rhs (AB2) or 0 (Euler)
+    // u* = u + dt*(a_df*nu*diff - a_cv*conv - c_hist*H - dp_flag*grad p),
+    // H' = h_df*nu*diff - h_cv*conv on the computed faces
+    double a_df, a_cv, c_hist, h_df, h_cv;
+    int split;                           // gcb is an array of its own (a_df != a_cv or h_df != h_cv)
     const double *u[3];                  // forward state at the step start
     const double *usb[3];                // adjoint of u* (after the fill transpose)
     double *hb[3];                       // in: adjoint of H'; out: adjoint of H
-    double *gb[3];                       // out: adjoint of rhs on the computed faces (0 elsewhere)
+    double *gb[3];                       // out: adjoint of nu*diff on the computed faces (0 elsewhere)
+    double *gcb[3];                      // out: adjoint of -conv (gb itself unless split)
     double *ub[3];                       // out: adjoint of u
     double *pb;                          // adjoint of p (accumulated)
 };
@@ -270,7 +277,8 @@ __global__ void __launch_bounds__(AT) k_adj_pred_prep(AdjPred P, int c) {
     const bool on = computed(P.g, P.per, c, i, j, k);
     const double us = on ? P.usb[c][q] : 0.0;
     const double hb = P.hb[c][q];
-    P.gb[c][q] = on ? (P.dt * P.c_rhs * us + (P.keep_h ? hb : 0.0)) : 0.0;
+    P.gb[c][q] = on ? (P.dt * P.a_df * us + P.h_df * hb) : 0.0;
+    if (P.split) P.gcb[c][q] = on ? (P.dt * P.a_cv * us + P.h_cv * hb) : 0.0;
     P.hb[c][q] = on ? -P.dt * P.c_hist * us : hb;
     P.ub[c][q] = us;  // u* = u + ...
 }
@@ -306,25 +314,26 @@ __device__ __forceinline__ double convdiff_at(const AdjPred &P, int c, int i, in
     const Grid &g = P.g;
     const int *nc = (c == 0) ? nu_ : (c == 1) ? nv_ : nw_;
     const double *U = P.u[0], *V = P.u[1], *W = P.u[2];
-    const double *GU = P.gb[0], *GV = P.gb[1], *GW = P.gb[2];
+    const double *GU = P.gcb[0], *GV = P.gcb[1], *GW = P.gcb[2];  // convective part
     const double rx = P.rd3[0], ry = P.rd3[1], rz = P.rd3[2];
     const int nn[3] = {g.nx, g.ny, g.nz};
-    const double *Gc = P.gb[c];
+    const double *Gd = P.gb[c];   // diffusive part
+    const double *Gc = P.gcb[c];  // convective part
     const double *Xc = P.u[c];
     const int id[3] = {i, j, k};
     // diffusion, diff2 along each axis (own axis wraps output 1's left neighbour to n when periodic)
     double dif = 0.0;
-    const double gc0 = rdt<SAFE>(Gc, nc, i, j, k);
+    const double gc0 = rdt<SAFE>(Gd, nc, i, j, k);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         int m[3] = {i, j, k}, p1[3] = {i, j, k};
         m[a] -= 1;
         p1[a] += 1;
-        double s = rdt<SAFE>(Gc, nc, m[0], m[1], m[2]) - 2.0 * gc0 + rdt<SAFE>(Gc, nc, p1[0], p1[1], p1[2]);
+        double s = rdt<SAFE>(Gd, nc, m[0], m[1], m[2]) - 2.0 * gc0 + rdt<SAFE>(Gd, nc, p1[0], p1[1], p1[2]);
         if (a == c && P.per[c] && id[c] == nn[c]) {
             int o[3] = {i, j, k};
             o[c] = 1;
-            s += rdt<SAFE>(Gc, nc, o[0], o[1], o[2]);
+            s += rdt<SAFE>(Gd, nc, o[0], o[1], o[2]);
         }
         dif += s * P.rd2[a];
     }
@@ -386,7 +395,8 @@ __device__ __forceinline__ double convdiff_at(const AdjPred &P, int c, int i, in
 __device__ __forceinline__ double convdiff_inner(const AdjPred &P, int c, int i, int j, int k, const int nu_[3],
                                                  const int nv_[3], const int nw_[3]) {
     const double *U = P.u[0], *V = P.u[1], *W = P.u[2];
-    const double *GU = P.gb[0], *GV = P.gb[1], *GW = P.gb[2];
+    const double *GU = P.gcb[0], *GV = P.gcb[1], *GW = P.gcb[2];  // convective part
+    const double *GD = P.gb[c];                                    // diffusive part, component c
     const double rx = P.rd3[0], ry = P.rd3[1], rz = P.rd3[2];
     const long su1 = nu_[0], su2 = (long)nu_[0] * nu_[1];
     const long sv1 = nv_[0], sv2 = (long)nv_[0] * nv_[1];
@@ -396,13 +406,12 @@ __device__ __forceinline__ double convdiff_inner(const AdjPred &P, int c, int i,
     const long qw = (i - 1) + sw1 * (j - 1) + sw2 * (k - 1);
     double dif, conv;
     if (c == 0) {
-        dif = (GU[qu - 1] - 2.0 * GU[qu] + GU[qu + 1]) * P.rd2[0] + (GU[qu - su1] - 2.0 * GU[qu] + GU[qu + su1]) * P.rd2[1] +
-              (GU[qu - su2] - 2.0 * GU[qu] + GU[qu + su2]) * P.rd2[2];
+        dif = (GD[qu - 1] - 2.0 * GD[qu] + GD[qu + 1]) * P.rd2[0] + (GD[qu - su1] - 2.0 * GD[qu] + GD[qu + su1]) * P.rd2[1] +
+              (GD[qu - su2] - 2.0 * GD[qu] + GD[qu + su2]) * P.rd2[2];
         double X1 = (GU[qu] - GU[qu + 1]) * rx;
         if (P.per[0] && i == P.g.nx) {
-            const double g1 = GU[qu - (i - 1)];
-            dif += g1 * P.rd2[0];
-            X1 -= g1 * rx;
+            dif += GD[qu - (i - 1)] * P.rd2[0];
+            X1 -= GU[qu - (i - 1)] * rx;
         }
         const double X0 = (GU[qu - 1] - GU[qu]) * rx;
         conv = 0.5 * (2.0 * (0.5 * (U[qu + 1] + U[qu])) * X1 + 2.0 * (0.5 * (U[qu] + U[qu - 1])) * X0);
@@ -413,13 +422,12 @@ __device__ __forceinline__ double convdiff_inner(const AdjPred &P, int c, int i,
         const double ff0 = (GU[qu - su2] - GU[qu]) * rz + (GW[qw - sw2] - GW[qw - sw2 + 1]) * rx;
         conv += 0.5 * (ff1 * (0.5 * (W[qw + 1] + W[qw])) + ff0 * (0.5 * (W[qw - sw2 + 1] + W[qw - sw2])));
     } else if (c == 1) {
-        dif = (GV[qv - 1] - 2.0 * GV[qv] + GV[qv + 1]) * P.rd2[0] + (GV[qv - sv1] - 2.0 * GV[qv] + GV[qv + sv1]) * P.rd2[1] +
-              (GV[qv - sv2] - 2.0 * GV[qv] + GV[qv + sv2]) * P.rd2[2];
+        dif = (GD[qv - 1] - 2.0 * GD[qv] + GD[qv + 1]) * P.rd2[0] + (GD[qv - sv1] - 2.0 * GD[qv] + GD[qv + sv1]) * P.rd2[1] +
+              (GD[qv - sv2] - 2.0 * GD[qv] + GD[qv + sv2]) * P.rd2[2];
         double X1 = (GV[qv] - GV[qv + sv1]) * ry;
         if (P.per[1] && j == P.g.ny) {
-            const double g1 = GV[qv - (long)(j - 1) * sv1];
-            dif += g1 * P.rd2[1];
-            X1 -= g1 * ry;
+            dif += GD[qv - (long)(j - 1) * sv1] * P.rd2[1];
+            X1 -= GV[qv - (long)(j - 1) * sv1] * ry;
         }
         const double X0 = (GV[qv - sv1] - GV[qv]) * ry;
         conv = 0.5 * (2.0 * (0.5 * (V[qv + sv1] + V[qv])) * X1 + 2.0 * (0.5 * (V[qv] + V[qv - sv1])) * X0);
@@ -430,13 +438,12 @@ __device__ __forceinline__ double convdiff_inner(const AdjPred &P, int c, int i,
         const double gf0 = (GV[qv - sv2] - GV[qv]) * rz + (GW[qw - sw2] - GW[qw - sw2 + sw1]) * ry;
         conv += 0.5 * (gf1 * (0.5 * (W[qw + sw1] + W[qw])) + gf0 * (0.5 * (W[qw - sw2 + sw1] + W[qw - sw2])));
     } else {
-        dif = (GW[qw - 1] - 2.0 * GW[qw] + GW[qw + 1]) * P.rd2[0] + (GW[qw - sw1] - 2.0 * GW[qw] + GW[qw + sw1]) * P.rd2[1] +
-              (GW[qw - sw2] - 2.0 * GW[qw] + GW[qw + sw2]) * P.rd2[2];
+        dif = (GD[qw - 1] - 2.0 * GD[qw] + GD[qw + 1]) * P.rd2[0] + (GD[qw - sw1] - 2.0 * GD[qw] + GD[qw + sw1]) * P.rd2[1] +
+              (GD[qw - sw2] - 2.0 * GD[qw] + GD[qw + sw2]) * P.rd2[2];
         double X1 = (GW[qw] - GW[qw + sw2]) * rz;
         if (P.per[2] && k == P.g.nz) {
-            const double g1 = GW[qw - (long)(k - 1) * sw2];
-            dif += g1 * P.rd2[2];
-            X1 -= g1 * rz;
+            dif += GD[qw - (long)(k - 1) * sw2] * P.rd2[2];
+            X1 -= GW[qw - (long)(k - 1) * sw2] * rz;
         }
         const double X0 = (GW[qw - sw2] - GW[qw]) * rz;
         conv = 0.5 * (2.0 * (0.5 * (W[qw + sw2] + W[qw])) * X1 + 2.0 * (0.5 * (W[qw] + W[qw - sw2])) * X0);
@@ -581,9 +588,53 @@ void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], doub
         hipLaunchKernelGGL(k_adj_div, dim3(ablocks(tot[c])), dim3(AT), 0, st, g, c, 1.0 / (h[c] * dt), rb, usb[c]);
 }
 
+namespace {
+
+// transpose of vel_bc_staggered_CN2 for walls along axis ax
+// (NS_functions.f90:313-410): the forward sets both line ends of every
+// component to its plane values, so their adjoints move to the planes and the
+// ends' adjoints become zero (one thread per line and component)
+__global__ void __launch_bounds__(AT) k_adj_cn2(Grid g, int ax, PlaneLayout L, double *a0, double *a1, double *a2,
+                                              double *pbar) {
+    const int c = blockIdx.y;
+    int n[3];
+    dims_of(g, c, n);
+    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    const size_t t = (size_t)blockIdx.x * AT + threadIdx.x;
+    if (t >= (size_t)n[o1] * n[o2]) return;
+    int id[3];
+    id[o1] = (int)(t % n[o1]) + 1;
+    id[o2] = (int)(t / n[o1]) + 1;
+    double *A = (c == 0) ? a0 : (c == 1) ? a1 : a2;
+    id[ax] = 1;
+    const size_t q1 = FIX(id[0], id[1], id[2], n[0], n[1]);
+    id[ax] = n[ax];
+    const size_t qn = FIX(id[0], id[1], id[2], n[0], n[1]);
+    pbar[L.off[6 * c + 2 * ax] + t] += A[q1];
+    pbar[L.off[6 * c + 2 * ax + 1] + t] += A[qn];
+    A[q1] = 0.0;
+    A[qn] = 0.0;
+}
+
+}  // namespace
+
+void launch_adj_cn2(const Grid &g, int ax, double *const a[3], double *pbar, hipStream_t st) {
+    size_t mx = 0;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+        const size_t m = (size_t)n[o1] * n[o2];
+        mx = m > mx ? m : mx;
+    }
+    hipLaunchKernelGGL(k_adj_cn2, dim3(ablocks(mx), 3), dim3(AT), 0, st, g, ax, plane_layout(g.nx, g.ny, g.nz), a[0],
+                       a[1], a[2], pbar);
+}
+
 void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, int dp_flag, const double h[3],
                         int scheme, int first, const double *const u[3], const double *const usb[3],
-                        double *const hb[3], double *const gb[3], double *const ub[3], double *pb, hipStream_t st) {
+                        double *const hb[3], double *const gb[3], double *const gcb[3], double *const ub[3],
+                        double *pb, hipStream_t st) {
     AdjPred P;
     P.g = g;
     for (int a = 0; a < 3; a++) {
@@ -594,16 +645,30 @@ void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, i
         P.usb[a] = usb[a];
         P.hb[a] = hb[a];
         P.gb[a] = gb[a];
+        P.gcb[a] = gcb[a];
         P.ub[a] = ub[a];
     }
     P.dt = dt;
     P.nu = nu;
     P.dpf = (double)dp_flag;
     P.pb = pb;
-    const bool ab2 = (scheme == NS_AB2);
-    P.c_rhs = (ab2 && !first) ? 1.5 : 1.0;
-    P.c_hist = (ab2 && !first) ? 0.5 : 0.0;
-    P.keep_h = ab2 ? 1 : 0;
+    // weights of the predictor (big_periodic_3D.f90:350-396): Euler
+    // u* = u + dt*rhs, H' = 0; AB2 u* = u + dt*(1.5 rhs - 0.5 H), H' = rhs
+    // (first step: Euler weights, H' = rhs); AB2-CN u* = u + dt*(nu/2 diff
+    // - 1.5 conv - 0.5 H), H' = -conv (first step: - conv)
+    if (scheme == NS_AB2_CN) {
+        P.a_df = 0.5;
+        P.a_cv = first ? 1.0 : 1.5;
+        P.c_hist = first ? 0.0 : 0.5;
+        P.h_df = 0.0;
+        P.h_cv = 1.0;
+    } else {
+        const bool ab2 = (scheme == NS_AB2);
+        P.a_df = P.a_cv = (ab2 && !first) ? 1.5 : 1.0;
+        P.c_hist = (ab2 && !first) ? 0.5 : 0.0;
+        P.h_df = P.h_cv = ab2 ? 1.0 : 0.0;
+    }
+    P.split = (gcb[0] != gb[0]) ? 1 : 0;
     const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
     for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_pred_prep, dim3(ablocks(tot[c])), dim3(AT), 0, st, P, c);
     if (dp_flag) hipLaunchKernelGGL(k_adj_pred_p, dim3(ablocks(g.np())), dim3(AT), 0, st, P);

@@ -207,7 +207,7 @@ void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
 int launch_err_partials(const Grid &g, double dx, double dy, const double *const a[4], const double *const b[4],
                         double *partial, int max_blocks, hipStream_t st);
 
-// ---- discrete adjoint of an explicit step (ns_adjoint.hip) ----
+// ---- discrete adjoint of a step (ns_adjoint.hip) ----
 void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag, const double *const ub[3],
                         double *const usb[3], double *pb, double *dpb, hipStream_t st);
 // ghost-fill transpose, axes z, y, x: comp 3 = the pressure array a[0] (codes pbc),
@@ -217,9 +217,15 @@ void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3]
 void launch_adj_take_interior(const Grid &g, const double *dpb, double *r, hipStream_t st);
 void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
                         double *const usb[3], double *pbar, hipStream_t st);
+// gb / gcb: work arrays for the adjoints of the diffusive and convective parts
+// of the right-hand side (the same arrays for the explicit schemes)
 void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, int dp_flag, const double h[3],
                         int scheme, int first, const double *const u[3], const double *const usb[3],
-                        double *const hb[3], double *const gb[3], double *const ub[3], double *pb, hipStream_t st);
+                        double *const hb[3], double *const gb[3], double *const gcb[3], double *const ub[3],
+                        double *pb, hipStream_t st);
+// transpose of the AB2-CN boundary rows before the sweep along axis ax
+// (vel_bc_staggered_CN2, walls): the line ends' adjoints move to the planes
+void launch_adj_cn2(const Grid &g, int ax, double *const a[3], double *pbar, hipStream_t st);
 int launch_block_sums(const double *a, size_t n, double *partial, int max_blocks, hipStream_t st);
 void launch_add_at(double *a, size_t q, double v, hipStream_t st);
 // a[q] += scale * sum(partial[0..nb)), on the device

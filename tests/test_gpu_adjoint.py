@@ -12,7 +12,8 @@ import ctypes
 import numpy as np
 import pytest
 
-from ns_3d_staggered_dp_amd import (AB2, EULER, FLAG_EAGER_PROJECTION, FLAG_GAUGE_CORNER, NSConfig, _abi, fields)
+from ns_3d_staggered_dp_amd import (AB2, AB2_CN, EULER, FLAG_EAGER_PROJECTION, FLAG_GAUGE_CORNER, NSConfig, _abi,
+                                    fields)
 from ns_3d_staggered_dp_amd.grid import plane_set_length
 from ns_3d_staggered_dp_amd.solver import NSSolver, _ptr
 
@@ -54,6 +55,10 @@ CASES = [
     ("periodic_fft", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=1), False),
     ("periodic_gauge_eager", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=0,
                                               flags=FLAG_EAGER_PROJECTION | FLAG_GAUGE_CORNER), False),
+    ("walls_cn", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), False),
+    ("walls_cn_first", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), True),
+    ("walls_cn_mixed_dst", lambda: box(10, 8, 9, bc=(2, 4, 4), pbc=(3, 2, 2), scheme=AB2_CN, dp_flag=1), False),
+    ("walls_cn_long_lines", lambda: box(40, 6, 36, bc=(4, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), False),
 ]
 
 
@@ -85,17 +90,16 @@ def test_adjoint_step_exact_on_quadratic_map(name, mk, first):
     assert abs(fd - ad) <= 1e-10 * scale, (fd, ad, scale)
 
 
-def test_adjoint_unsupported_for_lazy_and_cn():
-    from ns_3d_staggered_dp_amd import AB2_CN
-
-    for cfg in (periodic(8, 8, 8, scheme=AB2, dp_flag=0), box(8, 8, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN,
-                                                                 dp_flag=1)):
+def test_adjoint_unsupported_for_lazy_and_periodic_cn():
+    for cfg in (periodic(8, 8, 8, scheme=AB2, dp_flag=0), periodic(8, 8, 8, scheme=AB2_CN, dp_flag=1),
+                box(8, 8, 8, bc=(1, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1)):
         with NSSolver(cfg) as s:
             rc = _abi.lib().ns_adjoint_set_state(s.h, None, None, None, None, None, None, None)
             assert rc == _abi.NS_ERR_UNSUPPORTED
 
 
-def test_rollout_gradient_matches_central_differences():
+@pytest.mark.parametrize("scheme", [AB2, AB2_CN], ids=["ab2", "ab2_cn"])
+def test_rollout_gradient_matches_central_differences(scheme):
     """torch.autograd through 6 steps (checkpoint every 2, recompute), walls
     with per-step planes: d/d(u0, v0, w0, planes) of a sub-domain mismatch,
     checked in random directions against central differences (1e-6)."""
@@ -103,7 +107,7 @@ def test_rollout_gradient_matches_central_differences():
 
     from ns_3d_staggered_dp_amd.adjoint import Rollout, make_rollout_function
 
-    cfg = box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2, dp_flag=1)
+    cfg = box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=scheme, dp_flag=1)
     nsteps = 6
     st, rng = rand_state(cfg, 41, amp=0.3)
     L = plane_set_length(cfg.nx, cfg.ny, cfg.nz)
