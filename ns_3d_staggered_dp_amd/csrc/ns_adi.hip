@@ -16,25 +16,30 @@ namespace ns {
 
 namespace {
 
-// The sweep is one thread per line, so each elimination step would wait for
-// its own load: lines are walked in register chunks of CHK elements, the next
-// chunk's loads issued before the current chunk's dependent arithmetic.  The
-// coefficients are line-independent and read at a wave-uniform index through
-// the constant address space (scalar loads).
-constexpr int CHK = 16;
-typedef const __attribute__((address_space(4))) double cdouble;
+// One thread solves one line, so each elimination step would wait for its
+// own loads: lines and coefficients are read in register chunks of CHK
+// elements (one wait per chunk), the next chunk's data issued before the
+// current chunk's dependent arithmetic.
+constexpr int SB = 16;  // staging copies in flight per thread
 
 // forward elimination over y[0..m) (stride s): d_i = (r_i - lo_i d_{i-1}) inv_i,
 // stored in place; r_0 and (m > 1) r_{m-1} are replaced by r_first, r_last when ovr.
 // Returns d_{m-1}.
-__device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const double *lo_, const double *inv_,
+template <int CHK>
+__device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const double *lo, const double *inv,
                                              bool ovr, double r_first, double r_last) {
-    cdouble *lo = (cdouble *)lo_, *inv = (cdouble *)inv_;
     double cur[CHK], nxt[CHK];
 #pragma unroll
     for (int j = 0; j < CHK; j++) cur[j] = (j < m) ? y[s * j] : 0.0;
     double d = 0.0;
     for (int i0 = 0; i0 < m; i0 += CHK) {
+        double clo[CHK], cinv[CHK];
+#pragma unroll
+        for (int j = 0; j < CHK; j++) {
+            const int i = (i0 + j < m) ? i0 + j : m - 1;
+            clo[j] = lo[i];
+            cinv[j] = inv[i];
+        }
 #pragma unroll
         for (int j = 0; j < CHK; j++) nxt[j] = (i0 + CHK + j < m) ? y[s * (i0 + CHK + j)] : 0.0;
 #pragma unroll
@@ -46,7 +51,7 @@ __device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const d
                     if (i == 0) r = r_first;
                     else if (i == m - 1) r = r_last;
                 }
-                d = (i == 0) ? r * inv[0] : (r - lo[i] * d) * inv[i];
+                d = (i == 0) ? r * cinv[j] : (r - clo[j] * d) * cinv[j];
                 y[s * i] = d;
             }
         }
@@ -58,21 +63,24 @@ __device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const d
 
 // back substitution x_i = d_i - cp_i x_{i+1}, i = m-2 .. 0, with x_{m-1} = last
 // already in place; returns x_0
-__device__ __forceinline__ double thomas_bwd(double *y, size_t s, int m, const double *cp_, double last) {
-    cdouble *cp = (cdouble *)cp_;
+template <int CHK>
+__device__ __forceinline__ double thomas_bwd(double *y, size_t s, int m, const double *cp, double last) {
     double cur[CHK], nxt[CHK];
     const int top = m - 2;
 #pragma unroll
     for (int j = 0; j < CHK; j++) cur[j] = (top - j >= 0) ? y[s * (top - j)] : 0.0;
     double x = last;
     for (int i0 = top; i0 >= 0; i0 -= CHK) {
+        double ccp[CHK];
+#pragma unroll
+        for (int j = 0; j < CHK; j++) ccp[j] = cp[(i0 - j >= 0) ? i0 - j : 0];
 #pragma unroll
         for (int j = 0; j < CHK; j++) nxt[j] = (i0 - CHK - j >= 0) ? y[s * (i0 - CHK - j)] : 0.0;
 #pragma unroll
         for (int j = 0; j < CHK; j++) {
             const int i = i0 - j;
             if (i >= 0) {
-                x = cur[j] - cp[i] * x;
+                x = cur[j] - ccp[j] * x;
                 y[s * i] = x;
             }
         }
@@ -82,11 +90,13 @@ __device__ __forceinline__ double thomas_bwd(double *y, size_t s, int m, const d
     return x;
 }
 
-// solve one line in place (global memory, stride s)
+// solve one line in place (stride s); CHK: elements per register chunk
+// (16 for lines in global memory, 8 for lines in LDS)
+template <int CHK>
 __device__ __forceinline__ void solve_line(double *x, size_t s, const LineSys &S) {
     if (!S.periodic) {
-        const double last = thomas_fwd(x, s, S.N, S.lo, S.inv, false, 0.0, 0.0);
-        thomas_bwd(x, s, S.N, S.cp, last);
+        const double last = thomas_fwd<CHK>(x, s, S.N, S.lo, S.inv, false, 0.0, 0.0);
+        thomas_bwd<CHK>(x, s, S.N, S.cp, last);
         return;
     }
     const int m = S.m;
@@ -105,11 +115,11 @@ __device__ __forceinline__ void solve_line(double *x, size_t s, const LineSys &S
         r_last = y[s * (m - 1)] + S.a * g_hi;
     }
     // Thomas on the reduced system (rows 0 and m-1 carry the folded ghosts)
-    const double ylast = thomas_fwd(y, s, m, S.lo, S.inv, true, r_first, r_last);
-    const double y0 = thomas_bwd(y, s, m, S.cp, ylast);
+    const double ylast = thomas_fwd<CHK>(y, s, m, S.lo, S.inv, true, r_first, r_last);
+    const double y0 = thomas_bwd<CHK>(y, s, m, S.cp, ylast);
     // Sherman-Morrison correction
     const double f = (y0 + S.s * ylast) / S.den;
-    cdouble *z = (cdouble *)S.z;
+    const double *z = S.z;
     double x0 = 0.0, xl = 0.0;
     for (int i0 = 0; i0 < m; i0 += CHK) {
         double v[CHK];
@@ -141,21 +151,83 @@ struct SweepParams {
     double *a[3];
 };
 
-__global__ void __launch_bounds__(64) k_adi(SweepParams P) {
+// y / z sweeps: one thread per line, lines walked in global memory (the
+// threads of a wave hold consecutive x, so every chunk load is coalesced and
+// the loads stream while the chain computes)
+template <int AX>
+__global__ void __launch_bounds__(64) k_adi_strided(SweepParams P) {
     const int c = blockIdx.y;
     int n[3];
     comp_dims(P.g, c, n);
-    const int ax = P.ax;
-    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)n[o1] * n[o2]) return;
+    const int o2 = (AX == 2) ? 1 : 2;
+    const size_t t = (size_t)blockIdx.x * 64 + threadIdx.x;
+    if (t >= (size_t)n[0] * n[o2]) return;
     int id[3];
-    id[o1] = (int)(t % n[o1]) + 1;
-    id[o2] = (int)(t / n[o1]) + 1;
-    id[ax] = 1;
-    size_t stride = (ax == 0) ? 1 : (ax == 1) ? (size_t)n[0] : (size_t)n[0] * n[1];
-    double *line = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]);
-    solve_line(line, stride, P.sys[c]);
+    id[0] = (int)(t % n[0]) + 1;
+    id[o2] = (int)(t / n[0]) + 1;
+    id[AX] = 1;
+    const size_t stride = (AX == 1) ? (size_t)n[0] : (size_t)n[0] * n[1];
+    solve_line<16>(P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]), stride, P.sys[c]);
+}
+
+// x sweeps: a block stages L = 2^lsh lines of one component in LDS (x lines
+// are contiguous rows, so the block's lines are one contiguous span of
+// memory: coalesced loads and stores), with the component's coefficients
+// after them; each of its first L threads solves one line there, and the
+// block writes the lines back: one read and one write of every element.
+__global__ void __launch_bounds__(64) k_adi_x(SweepParams P, int lsh, int coff) {
+    extern __shared__ double sm[];
+    const int c = blockIdx.y;
+    int n[3];
+    comp_dims(P.g, c, n);
+    const int N = n[0], L = 1 << lsh;
+    double *a = P.a[c];
+    const LineSys &S0 = P.sys[c];
+    const int M = S0.m;
+    double *cf = sm + coff;
+    for (int f = threadIdx.x; f < M; f += 64) {
+        cf[f] = S0.lo[f];
+        cf[M + f] = S0.inv[f];
+        cf[2 * M + f] = S0.cp[f];
+        if (S0.periodic) cf[3 * M + f] = S0.z[f];
+    }
+    LineSys S = S0;
+    S.lo = cf;
+    S.inv = cf + M;
+    S.cp = cf + 2 * M;
+    if (S0.periodic) S.z = cf + 3 * M;
+    const long nl = (long)n[1] * n[2];
+    const long l0 = (long)blockIdx.x << lsh;
+    if (l0 >= nl) return;  // this component has fewer lines (uniform per block)
+    const int cnt = (int)(nl - l0 < L ? nl - l0 : L);
+    const int LS = N + 1, tot = cnt * N;
+    const size_t base = (size_t)l0 * N;
+    const float rN = 1.0f / (float)N;  // f < 2^24: float quotient within one
+    auto slot = [&](int f) {
+        int l = (int)((float)f * rN);
+        if (l * N > f) l--;
+        else if ((l + 1) * N <= f) l++;
+        return l * LS + (f - l * N);
+    };
+    for (int f0 = threadIdx.x; f0 < tot; f0 += 64 * SB) {
+        double v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; u++) v[u] = (f0 + 64 * u < tot) ? a[base + f0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < SB; u++)
+            if (f0 + 64 * u < tot) sm[slot(f0 + 64 * u)] = v[u];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) solve_line<8>(sm + threadIdx.x * LS, 1, S);
+    __syncthreads();
+    for (int f0 = threadIdx.x; f0 < tot; f0 += 64 * SB) {
+        double v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; u++) v[u] = (f0 + 64 * u < tot) ? sm[slot(f0 + 64 * u)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < SB; u++)
+            if (f0 + 64 * u < tot) a[base + f0 + 64 * u] = v[u];
+    }
 }
 
 }  // namespace
@@ -167,15 +239,31 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
     P.ax = dir - 1;
     for (int c = 0; c < 3; c++) P.sys[c] = (c == P.ax) ? sysA : sysB;
     P.a[0] = u; P.a[1] = v; P.a[2] = w;
-    size_t mx = 0;
+    const int ax = P.ax;
+    if (ax == 0) {
+        // longest x line, lines per block (up to 64 within 64 KiB), blocks of the largest component
+        const int nmax = g.nx + 2;
+        const size_t row = (size_t)nmax + 1;
+        int lsh = 6;
+        while (lsh > 0 && ((size_t)1 << lsh) * row * sizeof(double) > 65536) lsh--;
+        const size_t L = (size_t)1 << lsh;
+        const size_t nb = ((size_t)(g.ny + 2) * (g.nz + 2) + L - 1) / L;
+        const int coff = (int)(L * row);
+        const size_t lds = ((size_t)coff + 4 * (size_t)nmax) * sizeof(double);
+        if (lds > 65536)
+            (void)hipFuncSetAttribute((const void *)k_adi_x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_adi_x, dim3((unsigned)nb, 3), dim3(64), lds, st, P, lsh, coff);
+        return;
+    }
+    size_t nl = 0;
     for (int c = 0; c < 3; c++) {
         int n[3];
         comp_dims(g, c, n);
-        int o1 = (P.ax == 0) ? 1 : 0, o2 = (P.ax == 2) ? 1 : 2;
-        size_t s = (size_t)n[o1] * n[o2];
-        mx = s > mx ? s : mx;
+        const size_t m = (size_t)n[0] * ((ax == 1) ? n[2] : n[1]);
+        nl = m > nl ? m : nl;
     }
-    hipLaunchKernelGGL(k_adi, dim3((unsigned)((mx + 63) / 64), 3), dim3(64), 0, st, P);
+    if (ax == 1) hipLaunchKernelGGL(k_adi_strided<1>, dim3((unsigned)((nl + 63) / 64), 3), dim3(64), 0, st, P);
+    else hipLaunchKernelGGL(k_adi_strided<2>, dim3((unsigned)((nl + 63) / 64), 3), dim3(64), 0, st, P);
 }
 
 }  // namespace ns
