@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_256.json"))
     ap.add_argument("--adjoint-n", type=int, default=128,
                     help="grid of the adjoint/forward cost measurement (walls, AB2, DCT, dp_flag=1); 0 skips it")
+    ap.add_argument("--resim-scheme", choices=["AB2-CN", "AB2"], default="AB2-CN",
+                    help="time scheme of the adjoint and batched re-simulation measurements (walls)")
     ap.add_argument("--batch-boxes", type=int, default=64,
                     help="sub-boxes of the batched re-simulation measurement (sizes 32..128, seed 4); 0 skips it")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -121,9 +123,9 @@ def summarize_profile(d, traffic_json):
     return 0
 
 
-def adjoint_cost(n, steps=10):
+def adjoint_cost(n, scheme_name="AB2-CN", steps=10):
     """Adjoint/forward cost ratio on the re-simulation setting (walls bc=2,
-    Neumann pressure pbc=3, AB2, dp_flag=1, n^3): mean ms of one forward step
+    Neumann pressure pbc=3, AB2-CN or AB2, dp_flag=1, n^3): mean ms of one forward step
     (ns_step) and of one reverse step (ns_adjoint_step, forward velocity
     already on the device); the checkpointed reverse sweep adds one forward
     recompute per step on top of that."""
@@ -131,12 +133,12 @@ def adjoint_cost(n, steps=10):
 
     import torch
 
-    from ns_3d_staggered_dp_amd import AB2, NSConfig, fields
+    from ns_3d_staggered_dp_amd import SCHEMES, NSConfig, fields
     from ns_3d_staggered_dp_amd import _abi
     from ns_3d_staggered_dp_amd.grid import plane_set_length
     from ns_3d_staggered_dp_amd.solver import NSSolver
 
-    cfg = NSConfig.cube(n, nu=1e-3, dt=5e-4, scheme=AB2, bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
+    cfg = NSConfig.cube(n, nu=1e-3, dt=5e-4, scheme=SCHEMES[scheme_name], bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
     u, v, w, p = fields.decaying_turbulence(cfg, seed=5)
     rng = np.random.default_rng(6)
     L = plane_set_length(n, n, n)
@@ -165,19 +167,19 @@ def adjoint_cost(n, steps=10):
             _abi.call("ns_adjoint_step", s.h, *args, 0, ctypes.c_void_p(pb.data_ptr()))
         torch.cuda.synchronize()
         adj = (time.perf_counter() - t0) / steps * 1e3
-    return {"config": f"{n}^3 walls bc=2 pbc=3, AB2, DCT Poisson, dp_flag=1", "forward_ms": round(fwd, 4),
+    return {"config": f"{n}^3 walls bc=2 pbc=3, {scheme_name}, DCT Poisson, dp_flag=1", "forward_ms": round(fwd, 4),
             "adjoint_ms": round(adj, 4), "ratio": round(adj / fwd, 3),
             "ratio_with_recompute": round((adj + fwd) / fwd, 3)}
 
 
-def batch_throughput(nbox, steps=20):
+def batch_throughput(nbox, scheme_name="AB2-CN", steps=20):
     """Batched re-simulation of nbox sub-boxes of uneven sizes (walls bc=2,
-    pbc=3, AB2, DCT, dp_flag=1; sizes drawn per axis from the 7-smooth lengths
+    pbc=3, AB2-CN or AB2, DCT, dp_flag=1; sizes drawn per axis from the 7-smooth lengths
     in [32, 128], seed 4): Mcell-steps/s of ns_batch_step over all boxes, and
     of the same boxes stepped one after another, each with its own launches
     (eager: every kernel launched per step; graph: steady steps replayed from
     the box's own HIP graph)."""
-    from ns_3d_staggered_dp_amd import AB2, FLAG_NO_GRAPH, NSConfig, fields
+    from ns_3d_staggered_dp_amd import FLAG_NO_GRAPH, SCHEMES, NSConfig, fields
     from ns_3d_staggered_dp_amd.batch import Batch, box_sizes
     from ns_3d_staggered_dp_amd.grid import plane_set_length
     from ns_3d_staggered_dp_amd.solver import NSSolver
@@ -189,7 +191,8 @@ def batch_throughput(nbox, steps=20):
         out = []
         for i, (nx, ny, nz) in enumerate(sizes):
             h = 2 * np.pi / 256
-            cfg = NSConfig(nx=nx, ny=ny, nz=nz, dx=h, dy=h, dz=h, nu=1e-3, dt=5e-4, scheme=AB2, bc=(2, 2, 2),
+            cfg = NSConfig(nx=nx, ny=ny, nz=nz, dx=h, dy=h, dz=h, nu=1e-3, dt=5e-4, scheme=SCHEMES[scheme_name],
+                           bc=(2, 2, 2),
                            pbc=(3, 3, 3), dp_flag=1, flags=flags)
             s = NSSolver(cfg)
             out.append(s)
@@ -224,8 +227,8 @@ def batch_throughput(nbox, steps=20):
             for s in solvers:
                 s.close()
     rate = {k: round(cells * steps / v / 1e6, 1) for k, v in res.items()}
-    return {"config": f"{nbox} sub-boxes, sizes 32..128 per axis (7-smooth, seed 4), walls bc=2 pbc=3, AB2, DCT, "
-                      f"dp_flag=1", "cells": cells, "steps": steps,
+    return {"config": f"{nbox} sub-boxes, sizes 32..128 per axis (7-smooth, seed 4), walls bc=2 pbc=3, "
+                      f"{scheme_name}, DCT, dp_flag=1", "cells": cells, "steps": steps,
             "batched_mcell_steps_s": rate["batch"], "sequential_graph_mcell_steps_s": rate["graph"],
             "sequential_eager_mcell_steps_s": rate["eager"], "speedup_vs_graph": round(res["graph"] / res["batch"], 3),
             "speedup_vs_eager": round(res["eager"] / res["batch"], 3)}
@@ -344,9 +347,9 @@ def main():
         "checks": {"finite": bool(finite), "max_div": maxdiv},
     }
     if a.adjoint_n > 0:
-        line["adjoint"] = adjoint_cost(a.adjoint_n)
+        line["adjoint"] = adjoint_cost(a.adjoint_n, a.resim_scheme)
     if a.batch_boxes > 0:
-        line["batch"] = batch_throughput(a.batch_boxes)
+        line["batch"] = batch_throughput(a.batch_boxes, a.resim_scheme)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, u, v, w, p, a.cpu_steps)
     print(json.dumps(line), flush=True)

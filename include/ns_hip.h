@@ -194,12 +194,13 @@ int ns_adjoint_step(ns_ctx *ctx, const double *u, const double *v, const double 
  * sub-box per process): many contexts stepped together, each kernel of the
  * step launched once for all boxes (a block finds its box from a prefix of
  * per-box block counts), steady steps replayed from one HIP graph ----
- * The contexts must share scheme (Euler or AB2), bc/pbc codes, dp_flag and
- * flags, project every step (dp_flag = 1 or NS_FLAG_EAGER_PROJECTION) and use
- * the DCT Poisson solver (walls); sizes may differ.  Each context keeps its
- * own state and current plane set; ns_batch_step(b, n) leaves every context
- * as ns_step(ctx, n) would.  A batch holds its contexts' device pointers:
- * destroy it before any of them. */
+ * The contexts must share scheme (Euler, AB2 or AB2-CN), bc/pbc codes,
+ * dp_flag and flags, project every step (dp_flag = 1 or
+ * NS_FLAG_EAGER_PROJECTION) and use the DCT Poisson solver (walls); sizes
+ * may differ.  Each context keeps its own state and current plane set;
+ * ns_batch_step(b, n) leaves every context as ns_step(ctx, n) would.  A batch
+ * holds its contexts' device pointers and time steps: destroy it before any
+ * of them, and re-create it after ns_set_dt on any of them. */
 typedef struct ns_batch ns_batch;
 int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out);
 int ns_batch_destroy(ns_batch *b);

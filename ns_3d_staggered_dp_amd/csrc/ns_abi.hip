@@ -1504,7 +1504,8 @@ struct ns_batch {
     hipStream_t st = nullptr;
     // one launch = per-box parameter array + block prefix (device), block total
     struct Phase {
-        int kind = 0;    // 0 predictor, 1 velocity fill, 2 DCT pass, 3 pressure fill, 4 projection
+        int kind = 0;    // 0 predictor, 1 velocity fill, 2 DCT pass, 3 pressure fill, 4 projection,
+                         // 5 AB2-CN boundary rows, 6 ADI sweep
         int arg = 0;     // axis or pass
         void *params = nullptr;
         int *prefix = nullptr;
@@ -1537,7 +1538,7 @@ constexpr long BATCH_ROUNDS = 8;
 int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_batch::Phase> &out) {
     const int n = (int)b->ctx.size();
     size_t psz = kind == 0 ? predict_params_size() : kind == 2 ? pass_params_size()
-                           : kind == 4 ? proj_params_size() : bc_params_size();
+                           : kind == 4 ? proj_params_size() : kind == 6 ? sweep_params_size() : bc_params_size();
     std::vector<char> host((size_t)n * psz);
     std::vector<int> prefix(n + 1, 0);
     size_t lds = 0;
@@ -1571,6 +1572,12 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
                                       c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l);
             lds = l > lds ? l : lds;
             variant |= dct_params_variant(dst);
+        } else if (kind == 5) {
+            nb = make_cn2_params(g, f.bc[arg], arg, c->planes, c->us, c->vs, c->ws, dst);
+        } else if (kind == 6) {
+            size_t l = 0;
+            nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l);
+            lds = l > lds ? l : lds;
         } else if (kind == 3) {
             nb = make_pr_bc_params(g, f.pbc, c->planes, d, c->dp, arg, dst);
         } else {
@@ -1599,8 +1606,14 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
 int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out) {
     int rc;
     if ((rc = add_phase(b, 0, 0, first_step, out))) return rc;
-    for (int ax = 0; ax < 3; ax++)
-        if ((rc = add_phase(b, 1, ax, first_step, out))) return rc;
+    if (b->ctx[0]->cfg.scheme == NS_AB2_CN) {
+        // per axis: boundary rows, then the sweep (x, y, z)
+        for (int ax = 0; ax < 3; ax++)
+            if ((rc = add_phase(b, 5, ax, first_step, out)) || (rc = add_phase(b, 6, ax, first_step, out))) return rc;
+    } else {
+        for (int ax = 0; ax < 3; ax++)
+            if ((rc = add_phase(b, 1, ax, first_step, out))) return rc;
+    }
     for (int pass = 0; pass < 5; pass++)
         if ((rc = add_phase(b, 2, pass, first_step, out))) return rc;
     for (int ax = 0; ax < 3; ax++)
@@ -1616,6 +1629,8 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
             case 1: launch_vel_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 2: launch_dct_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             case 3: launch_pr_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 5: launch_cn2_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 6: launch_adi_batch(ph.arg, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
         }
     }
@@ -1633,7 +1648,7 @@ int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
         if (!c) return NS_ERR_INVALID;
         const ns_config &f = c->cfg;
         // one kernel family per launch: same scheme and codes, projected every step, DCT Poisson
-        if (c->lazy || f.scheme == NS_AB2_CN || !c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
+        if (c->lazy || !c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
         if (f.scheme != f0.scheme || f.dp_flag != f0.dp_flag || (f.flags & ~NS_FLAG_NO_GRAPH) !=
             (f0.flags & ~NS_FLAG_NO_GRAPH))
             return NS_ERR_UNSUPPORTED;

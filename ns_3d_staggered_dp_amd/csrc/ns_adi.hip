@@ -149,18 +149,19 @@ struct SweepParams {
     int ax;
     LineSys sys[3];  // per component (A for the normal one, B otherwise)
     double *a[3];
+    int lsh, coff;   // x sweep: log2 lines per block, LDS offset of the coefficients
 };
 
 // y / z sweeps: one thread per line, lines walked in global memory (the
 // threads of a wave hold consecutive x, so every chunk load is coalesced and
 // the loads stream while the chain computes)
 template <int AX>
-__global__ void __launch_bounds__(64) k_adi_strided(SweepParams P) {
+__device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int blk) {
     const int c = blockIdx.y;
     int n[3];
     comp_dims(P.g, c, n);
     const int o2 = (AX == 2) ? 1 : 2;
-    const size_t t = (size_t)blockIdx.x * 64 + threadIdx.x;
+    const size_t t = (size_t)blk * 64 + threadIdx.x;
     if (t >= (size_t)n[0] * n[o2]) return;
     int id[3];
     id[0] = (int)(t % n[0]) + 1;
@@ -175,8 +176,9 @@ __global__ void __launch_bounds__(64) k_adi_strided(SweepParams P) {
 // memory: coalesced loads and stores), with the component's coefficients
 // after them; each of its first L threads solves one line there, and the
 // block writes the lines back: one read and one write of every element.
-__global__ void __launch_bounds__(64) k_adi_x(SweepParams P, int lsh, int coff) {
+__device__ __forceinline__ void adi_x_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
+    const int lsh = P.lsh, coff = P.coff;
     const int c = blockIdx.y;
     int n[3];
     comp_dims(P.g, c, n);
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(64) k_adi_x(SweepParams P, int lsh, int coff) 
     S.cp = cf + 2 * M;
     if (S0.periodic) S.z = cf + 3 * M;
     const long nl = (long)n[1] * n[2];
-    const long l0 = (long)blockIdx.x << lsh;
+    const long l0 = (long)blk << lsh;
     if (l0 >= nl) return;  // this component has fewer lines (uniform per block)
     const int cnt = (int)(nl - l0 < L ? nl - l0 : L);
     const int LS = N + 1, tot = cnt * N;
@@ -230,11 +232,41 @@ __global__ void __launch_bounds__(64) k_adi_x(SweepParams P, int lsh, int coff) 
     }
 }
 
+template <int AX>
+__global__ void __launch_bounds__(64) k_adi_strided(SweepParams P) {
+    adi_strided_body<AX>(P, blockIdx.x);
+}
+__global__ void __launch_bounds__(64) k_adi_x(SweepParams P) { adi_x_body(P, blockIdx.x); }
+
+template <int AX>
+__global__ void __launch_bounds__(64) k_adi_strided_batch(const SweepParams *__restrict__ Ps,
+                                                          const int *__restrict__ prefix, int nbox) {
+    int blk;
+    const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
+    adi_strided_body<AX>(P, blk);
+}
+__global__ void __launch_bounds__(64) k_adi_x_batch(const SweepParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                    int nbox) {
+    int blk;
+    const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
+    adi_x_body(P, blk);
+}
+
+template <typename K>
+void set_lds(K kern, size_t lds) {
+    if (lds > 65536) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 }  // namespace
 
-void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      hipStream_t st) {
-    SweepParams P;
+size_t sweep_params_size() { return sizeof(SweepParams); }
+
+// parameters of the sweep along axis dir-1; returns the block count (grid.y = 3
+// components) and the dynamic LDS size
+int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
+                      void *out, size_t *lds) {
+    SweepParams &P = *(SweepParams *)out;
+    P = SweepParams{};
     P.g = g;
     P.ax = dir - 1;
     for (int c = 0; c < 3; c++) P.sys[c] = (c == P.ax) ? sysA : sysB;
@@ -247,13 +279,10 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
         int lsh = 6;
         while (lsh > 0 && ((size_t)1 << lsh) * row * sizeof(double) > 65536) lsh--;
         const size_t L = (size_t)1 << lsh;
-        const size_t nb = ((size_t)(g.ny + 2) * (g.nz + 2) + L - 1) / L;
-        const int coff = (int)(L * row);
-        const size_t lds = ((size_t)coff + 4 * (size_t)nmax) * sizeof(double);
-        if (lds > 65536)
-            (void)hipFuncSetAttribute((const void *)k_adi_x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_adi_x, dim3((unsigned)nb, 3), dim3(64), lds, st, P, lsh, coff);
-        return;
+        P.lsh = lsh;
+        P.coff = (int)(L * row);
+        *lds = ((size_t)P.coff + 4 * (size_t)nmax) * sizeof(double);
+        return (int)(((size_t)(g.ny + 2) * (g.nz + 2) + L - 1) / L);
     }
     size_t nl = 0;
     for (int c = 0; c < 3; c++) {
@@ -262,8 +291,35 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
         const size_t m = (size_t)n[0] * ((ax == 1) ? n[2] : n[1]);
         nl = m > nl ? m : nl;
     }
-    if (ax == 1) hipLaunchKernelGGL(k_adi_strided<1>, dim3((unsigned)((nl + 63) / 64), 3), dim3(64), 0, st, P);
-    else hipLaunchKernelGGL(k_adi_strided<2>, dim3((unsigned)((nl + 63) / 64), 3), dim3(64), 0, st, P);
+    *lds = 0;
+    return (int)((nl + 63) / 64);
+}
+
+void launch_adi_batch(int ax, const void *params, const int *prefix, int nbox, int nblk, size_t lds, hipStream_t st) {
+    const SweepParams *Ps = (const SweepParams *)params;
+    if (ax == 0) {
+        set_lds(k_adi_x_batch, lds);
+        hipLaunchKernelGGL(k_adi_x_batch, dim3(nblk, 3), dim3(64), lds, st, Ps, prefix, nbox);
+    } else if (ax == 1) {
+        hipLaunchKernelGGL(k_adi_strided_batch<1>, dim3(nblk, 3), dim3(64), 0, st, Ps, prefix, nbox);
+    } else {
+        hipLaunchKernelGGL(k_adi_strided_batch<2>, dim3(nblk, 3), dim3(64), 0, st, Ps, prefix, nbox);
+    }
+}
+
+void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
+                      hipStream_t st) {
+    SweepParams P;
+    size_t lds = 0;
+    const int nb = make_sweep_params(g, dir, sysA, sysB, u, v, w, &P, &lds);
+    if (dir == 1) {
+        set_lds(k_adi_x, lds);
+        hipLaunchKernelGGL(k_adi_x, dim3(nb, 3), dim3(64), lds, st, P);
+    } else if (dir == 2) {
+        hipLaunchKernelGGL(k_adi_strided<1>, dim3(nb, 3), dim3(64), 0, st, P);
+    } else {
+        hipLaunchKernelGGL(k_adi_strided<2>, dim3(nb, 3), dim3(64), 0, st, P);
+    }
 }
 
 }  // namespace ns

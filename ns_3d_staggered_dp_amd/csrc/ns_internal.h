@@ -125,6 +125,14 @@ int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag
 void launch_vel_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
 void launch_pr_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
 void launch_project_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
+// AB2-CN boundary rows (vel_bc_staggered_CN2) along axis ax; 0 blocks: nothing to launch
+int make_cn2_params(const Grid &g, int bc, int ax, const double *planes, double *u, double *v, double *w, void *out);
+void launch_cn2_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
+// ADI sweep along axis dir-1 (ns_adi.hip): per-box parameters, block count (grid.y = 3), LDS size
+size_t sweep_params_size();
+int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
+                      void *out, size_t *lds);
+void launch_adi_batch(int ax, const void *params, const int *prefix, int nbox, int nblk, size_t lds, hipStream_t st);
 struct PoissonPlans;
 size_t pass_params_size();
 int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,

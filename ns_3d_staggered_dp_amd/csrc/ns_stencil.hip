@@ -222,11 +222,10 @@ __global__ void __launch_bounds__(256) k_vel_bc_axis_batch(const BCParams *__res
 }
 
 // boundary rows before an ADI sweep (vel_bc_staggered_CN2, NS_functions.f90:313-410)
-__global__ void __launch_bounds__(256) k_cn2_axis(BCParams P) {
+__device__ __forceinline__ void cn2_body(const BCParams &P, const size_t t) {
     Grid g{P.nx, P.ny, P.nz};
     int o1, o2;
     other_axes(P.ax, o1, o2);
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int c = 0; c < 3; c++) {
         int n[3];
         comp_dims(g, c, n);
@@ -247,6 +246,17 @@ __global__ void __launch_bounds__(256) k_cn2_axis(BCParams P) {
             id[P.ax] = N; A[FIX(id[0], id[1], id[2], n[0], n[1])] = bn[t];
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_cn2_axis(BCParams P) {
+    cn2_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_cn2_axis_batch(const BCParams *__restrict__ Ps, const int *__restrict__ prefix,
+                                                        int nbox) {
+    int blk;
+    const BCParams &P = batch_params(Ps, prefix, nbox, blk);
+    cn2_body(P, (size_t)blk * blockDim.x + threadIdx.x);
 }
 
 // pressure ghost fill along one axis (pr_bc_staggered, NS_functions.f90:1416-1462)
@@ -681,19 +691,28 @@ void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double 
     }
 }
 
-void launch_vel_bc_cn2(const Grid &g, int bc, int dir, const double *planes, double *u, double *v, double *w,
-                       hipStream_t st) {
-    BCParams P;
+int make_cn2_params(const Grid &g, int bc, int ax, const double *planes, double *u, double *v, double *w, void *out) {
+    BCParams &P = *(BCParams *)out;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
     P.L = plane_layout(g.nx, g.ny, g.nz);
     P.planes = planes;
     P.a[0] = u; P.a[1] = v; P.a[2] = w;
-    P.ax = dir - 1;
+    P.ax = ax;
     P.bc = bc;
     P.pbc_mode = 1;
     P.d = 0;
-    if (bc == 3) return;
-    hipLaunchKernelGGL(k_cn2_axis, dim3(nblocks(max_plane(g, P.ax))), dim3(256), 0, st, P);
+    return bc == 3 ? 0 : (int)nblocks(max_plane(g, ax));
+}
+
+void launch_vel_bc_cn2(const Grid &g, int bc, int dir, const double *planes, double *u, double *v, double *w,
+                       hipStream_t st) {
+    BCParams P;
+    const int nb = make_cn2_params(g, bc, dir - 1, planes, u, v, w, &P);
+    if (nb) hipLaunchKernelGGL(k_cn2_axis, dim3(nb), dim3(256), 0, st, P);
+}
+
+void launch_cn2_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(k_cn2_axis_batch, dim3(nblk), dim3(256), 0, st, (const BCParams *)params, prefix, nbox);
 }
 
 void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,

@@ -1,11 +1,11 @@
-"""GPU: batched steps of boxes of different sizes (ns_batch) are bitwise the
-steps of each box on its own (the same kernel bodies and parameters, one
-launch for all boxes), including the Euler first step of fresh boxes next to
-restarted ones, and graph replay of the steady steps."""
+"""GPU: batched steps of boxes of different sizes (ns_batch), explicit and
+AB2-CN, are bitwise the steps of each box on its own (the same kernel bodies
+and parameters, one launch for all boxes), including the Euler first step of
+fresh boxes next to restarted ones, and graph replay of the steady steps."""
 import numpy as np
 import pytest
 
-from ns_3d_staggered_dp_amd import AB2, EULER, FLAG_NO_GRAPH, NSConfig, _abi, fields
+from ns_3d_staggered_dp_amd import AB2, AB2_CN, EULER, FLAG_NO_GRAPH, NSConfig, _abi, fields
 from ns_3d_staggered_dp_amd.batch import Batch, box_sizes
 from ns_3d_staggered_dp_amd.grid import plane_set_length
 from ns_3d_staggered_dp_amd.solver import NSSolver
@@ -28,7 +28,9 @@ def inputs(cfg, seed):
 
 
 @pytest.mark.parametrize("scheme,flags,bc,pbc", [(AB2, 0, (2, 2, 2), (3, 3, 3)), (EULER, 0, (4, 4, 4), (2, 2, 2)),
-                                                (AB2, FLAG_NO_GRAPH, (2, 4, 2), (3, 2, 3))])
+                                                (AB2, FLAG_NO_GRAPH, (2, 4, 2), (3, 2, 3)),
+                                                (AB2_CN, 0, (2, 2, 2), (3, 3, 3)),
+                                                (AB2_CN, FLAG_NO_GRAPH, (4, 2, 4), (2, 3, 2))])
 def test_batch_equals_individual_steps(scheme, flags, bc, pbc):
     cfgs = [make(*s, scheme, flags, pbc, bc) for s in SIZES]
     data = [inputs(c, 60 + i) for i, c in enumerate(cfgs)]
