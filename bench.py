@@ -55,6 +55,8 @@ def parse():
                     help="time scheme of the adjoint and batched re-simulation measurements (walls)")
     ap.add_argument("--batch-boxes", type=int, default=64,
                     help="sub-boxes of the batched re-simulation measurement (sizes 32..128, seed 4); 0 skips it")
+    ap.add_argument("--resim-only", action="store_true",
+                    help="run only the adjoint / batch measurements (for profiling them) and print them as one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -281,6 +283,15 @@ def main():
     from ns_3d_staggered_dp_amd import AB2, FLAG_GAUGE_CORNER, NSConfig, fields
     from ns_3d_staggered_dp_amd import _abi
     from ns_3d_staggered_dp_amd.solver import NSSolver
+
+    if a.resim_only:
+        out = {}
+        if a.adjoint_n > 0:
+            out["adjoint"] = adjoint_cost(a.adjoint_n, a.resim_scheme)
+        if a.batch_boxes > 0:
+            out["batch"] = batch_throughput(a.batch_boxes, a.resim_scheme)
+        print(json.dumps(out), flush=True)
+        return 0
 
     n = a.n
     cfg = NSConfig.cube(n, nu=1e-3, dt=5e-4, scheme=AB2, flags=FLAG_GAUGE_CORNER)

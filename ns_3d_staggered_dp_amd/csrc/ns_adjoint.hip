@@ -126,11 +126,11 @@ struct AdjFill {
 //   bc 3 pressure: a(1) = -d b1 + a(2), a(N) = d bn + a(N-1)
 //   bc 4: a(1) = b1, a(N) = bn
 // transposed in reverse order of the assignments (a(N) first)
-__global__ void __launch_bounds__(AT) k_adj_fill(AdjFill P) {
-    if (P.comp < 3) P.comp = blockIdx.y;  // velocity: one component per blockIdx.y
-    if (P.comp < 3) P.a = P.av[P.comp];
+__global__ void __launch_bounds__(AT) k_adj_fill(const AdjFill P) {
+    const int comp = (P.comp < 3) ? (int)blockIdx.y : 3;  // velocity: one component per blockIdx.y
+    double *const a = (comp < 3) ? P.av[comp] : P.a;
     int n[3];
-    dims_of(P.g, P.comp, n);
+    dims_of(P.g, comp, n);
     const int ax = P.ax;
     const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
     const size_t t = (size_t)blockIdx.x * AT + threadIdx.x;
@@ -142,15 +142,15 @@ __global__ void __launch_bounds__(AT) k_adj_fill(AdjFill P) {
     auto at = [&](int pos) -> double & {
         int q[3] = {id[0], id[1], id[2]};
         q[ax] = pos;
-        return P.a[FIX(q[0], q[1], q[2], n[0], n[1])];
+        return a[FIX(q[0], q[1], q[2], n[0], n[1])];
     };
-    const bool pres = P.comp == 3;
-    const long o1p = pres ? P.L.off[18 + 2 * ax] : P.L.off[6 * P.comp + 2 * ax];
+    const bool pres = comp == 3;
+    const long o1p = pres ? P.L.off[18 + 2 * ax] : P.L.off[6 * comp + 2 * ax];
     const long onp = o1p + (pres ? (P.L.off[19 + 2 * ax] - P.L.off[18 + 2 * ax])
-                                 : (P.L.off[6 * P.comp + 2 * ax + 1] - P.L.off[6 * P.comp + 2 * ax]));
+                                 : (P.L.off[6 * comp + 2 * ax + 1] - P.L.off[6 * comp + 2 * ax]));
     double *b1 = P.pbar ? P.pbar + o1p : nullptr, *bn = P.pbar ? P.pbar + onp : nullptr;
     const size_t pl = t;
-    const bool normal = (P.comp == ax);
+    const bool normal = (comp == ax);
     if (P.bc == 1) {
         if (!pres && normal) {
             at(1) += at(N);
