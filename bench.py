@@ -35,8 +35,8 @@ ARRAYS_PER_CELL = {"predictor": 13, "vel_bc": 0, "div_fft_x_fwd": 4, "fft_y_fwd"
 # the reference pipeline's budget (explicit projection pass): 32 arrays = 256 B/cell-step
 REFERENCE_BUDGET_B = 256
 KERNEL_NAMES = {"predictor": "k_predict_tiled<true>", "div_fft_x_fwd": "k_fft_x_fwd<false,512>",
-                "fft_y_fwd": "k_fft_yz<1,-1,0,false>", "fft_z_solve": "k_fft_yz<2,-1,1,false>",
-                "fft_y_inv": "k_fft_yz<1,1,0,false>", "fft_x_inv_dp": "k_fft_x_inv<false,1024>", "project": "k_project"}
+                "fft_y_fwd": "k_fft_yz<1,-1,0,false,2048>", "fft_z_solve": "k_fft_yz<2,-1,1,false,1024>",
+                "fft_y_inv": "k_fft_yz<1,1,0,false,2048>", "fft_x_inv_dp": "k_fft_x_inv<false,1024>", "project": "k_project"}
 COPY_CAL_BYTES = 2 ** 30  # the device-to-device copy used to scale the HBM counters
 
 

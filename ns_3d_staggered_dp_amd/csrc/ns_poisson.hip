@@ -145,6 +145,8 @@ constexpr int FFT_ELEMS = 2048;  // complex elements per block (L*N), at most
 constexpr int FFT_XF_ELEMS = FFT_ELEMS / 4, FFT_XI_ELEMS = FFT_ELEMS / 2;
 // DCT passes: 1024 elements per block
 constexpr int DCT_ELEMS = 1024;
+// periodic z pass (two transforms per visit; measured at 256^3)
+constexpr int FFT_ZS_ELEMS = 1024;
 
 // x / d for 0 <= x < 2^24 with r = 1/d in single precision (one correction)
 struct IDiv {
@@ -315,7 +317,7 @@ __device__ __forceinline__ size_t spec_index(const Grid &g, int nxhp, int kx, in
     return ((size_t)pos * g.ny + other) * nxhp + kx;
 }
 
-template <int AXIS, int SIGN, int SOLVE, bool ODD>
+template <int AXIS, int SIGN, int SOLVE, bool ODD, int EL>
 __global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
@@ -334,7 +336,7 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
     }
     __syncthreads();
     const Lay ly{1, L};
-    fft_ip<ODD>(A, P.pl, ly, lsh, tws, SIGN);
+    fft_ip<ODD, EL>(A, P.pl, ly, lsh, tws, SIGN);
     if (SOLVE) {
         // modal division (z pass): eig = -4*((ex + ey) + ez); mode (0,0,0) -> 0
         for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
@@ -350,7 +352,7 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
             else *z = make_double2(z->x / e, z->y / e);
         }
         __syncthreads();
-        fft_ip<ODD>(A, P.pl, ly, lsh, tws, +1);
+        fft_ip<ODD, EL>(A, P.pl, ly, lsh, tws, +1);
     }
     for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
         const int l = f & (L - 1), pos = f >> lsh;
@@ -857,20 +859,26 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     // y forward
     P.pl = pl.py;
     P.lsh = fft_lsh(g.ny);
-    if (pow2(g.ny)) launch_lds(k_fft_yz<1, -1, 0, false>, nb_yz(g.nz), lds_yz(g.ny), st, P);
-    else launch_lds(k_fft_yz<1, -1, 0, true>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    if (pow2(g.ny)) launch_lds(k_fft_yz<1, -1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    else launch_lds(k_fft_yz<1, -1, 0, true, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[1], st);
     // z forward + modal division + z inverse
     P.pl = pl.pz;
-    P.lsh = fft_lsh(g.nz);
-    if (pow2(g.nz)) launch_lds(k_fft_yz<2, -1, 1, false>, nb_yz(g.ny), lds_yz(g.nz), st, P);
-    else launch_lds(k_fft_yz<2, -1, 1, true>, nb_yz(g.ny), lds_yz(g.nz), st, P);
+    const bool zs_small = g.nz <= FFT_ZS_ELEMS;
+    P.lsh = fft_lsh(g.nz, zs_small ? FFT_ZS_ELEMS : FFT_ELEMS);
+    if (zs_small) {
+        if (pow2(g.nz)) launch_lds(k_fft_yz<2, -1, 1, false, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
+        else launch_lds(k_fft_yz<2, -1, 1, true, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
+    } else {
+        if (pow2(g.nz)) launch_lds(k_fft_yz<2, -1, 1, false, FFT_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
+        else launch_lds(k_fft_yz<2, -1, 1, true, FFT_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
+    }
     if (ev) (void)hipEventRecord(ev[2], st);
     // y inverse
     P.pl = pl.py;
     P.lsh = fft_lsh(g.ny);
-    if (pow2(g.ny)) launch_lds(k_fft_yz<1, 1, 0, false>, nb_yz(g.nz), lds_yz(g.ny), st, P);
-    else launch_lds(k_fft_yz<1, 1, 0, true>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    if (pow2(g.ny)) launch_lds(k_fft_yz<1, 1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    else launch_lds(k_fft_yz<1, 1, 0, true, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[3], st);
     // x inverse (c2r) writing dp
     P.pl = pl.px;
