@@ -180,7 +180,8 @@ def batch_throughput(nbox, scheme_name="AB2-CN", steps=20):
     in [32, 128], seed 4): Mcell-steps/s of ns_batch_step over all boxes, and
     of the same boxes stepped one after another, each with its own launches
     (eager: every kernel launched per step; graph: steady steps replayed from
-    the box's own HIP graph)."""
+    the box's own HIP graph).  The batched rate is measured with the default
+    launch shapes and again after ns_batch_tune."""
     from ns_3d_staggered_dp_amd import FLAG_NO_GRAPH, SCHEMES, NSConfig, fields
     from ns_3d_staggered_dp_amd.batch import Batch, box_sizes
     from ns_3d_staggered_dp_amd.grid import plane_set_length
@@ -213,7 +214,7 @@ def batch_throughput(nbox, scheme_name="AB2-CN", steps=20):
             s.synchronize()
         return time.perf_counter() - t0
 
-    res = {}
+    res, knobs = {}, None
     for name, flags in (("eager", FLAG_NO_GRAPH), ("graph", 0)):
         solvers = []
         try:
@@ -224,6 +225,11 @@ def batch_throughput(nbox, scheme_name="AB2-CN", steps=20):
                     b.step(3)
                     t0 = time.perf_counter()
                     b.step(steps)
+                    res["batch_default"] = time.perf_counter() - t0
+                    knobs = b.tune()
+                    b.step(1)
+                    t0 = time.perf_counter()
+                    b.step(steps)
                     res["batch"] = time.perf_counter() - t0
         finally:
             for s in solvers:
@@ -231,7 +237,8 @@ def batch_throughput(nbox, scheme_name="AB2-CN", steps=20):
     rate = {k: round(cells * steps / v / 1e6, 1) for k, v in res.items()}
     return {"config": f"{nbox} sub-boxes, sizes 32..128 per axis (7-smooth, seed 4), walls bc=2 pbc=3, "
                       f"{scheme_name}, DCT, dp_flag=1", "cells": cells, "steps": steps,
-            "batched_mcell_steps_s": rate["batch"], "sequential_graph_mcell_steps_s": rate["graph"],
+            "batched_mcell_steps_s": rate["batch"], "batched_default_shapes_mcell_steps_s": rate["batch_default"],
+            "tuned_shapes": knobs, "sequential_graph_mcell_steps_s": rate["graph"],
             "sequential_eager_mcell_steps_s": rate["eager"], "speedup_vs_graph": round(res["graph"] / res["batch"], 3),
             "speedup_vs_eager": round(res["eager"] / res["batch"], 3)}
 

@@ -78,6 +78,7 @@ SIGNATURES = {
     "ns_batch_create": [P, I, P],
     "ns_batch_destroy": [P],
     "ns_batch_step": [P, I],
+    "ns_batch_tune": [P, I, P],
     "ns_history_save": [P],
     "ns_history_restore": [P],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],

@@ -40,6 +40,14 @@ class Batch:
     def step(self, nsteps: int = 1):
         _abi.call("ns_batch_step", self.h, int(nsteps))
 
+    def tune(self, trial_steps: int = 3) -> dict:
+        """Pick the launch shapes per size class by timing steady batch steps
+        (ns_batch_tune); the boxes' states are unchanged.  Returns the choice
+        per class: {"small"|"medium"|"large": (rounds, dct_drop, x_sweep_lsh)}."""
+        k = (ctypes.c_int * 9)()
+        _abi.call("ns_batch_tune", self.h, int(trial_steps), k)
+        return {name: (k[3 * i], k[3 * i + 1], k[3 * i + 2]) for i, name in enumerate(("small", "medium", "large"))}
+
     def close(self):
         if self.h:
             _abi.call("ns_batch_destroy", self.h)

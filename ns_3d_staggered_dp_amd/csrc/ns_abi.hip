@@ -1515,24 +1515,38 @@ struct ns_batch {
     };
     std::vector<Phase> phases;  // one steady (non-first) step
     hipGraphExec_t graph = nullptr;
-    std::vector<void *> allocs;
+    // launch-shape choices per size class of the boxes (cells < 2^17, < 2^20,
+    // larger): predictor z chunks sized for `rounds` rounds of resident
+    // blocks, DCT lines per block reduced by 2^dct_drop, 2^adix_lsh lines per
+    // x-sweep block.  None of them changes any value computed.
+    struct Knobs {
+        int rounds = 8, dct_drop = 0, adix_lsh = 6;
+    };
+    Knobs knobs[3];
+    std::vector<int> cls;  // size class of each box
 };
 
 namespace {
 
 template <typename T>
-int balloc(ns_batch *b, T **p, size_t bytes) {
+int balloc(T **p, size_t bytes) {
     void *q = nullptr;
     if (hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) {
         (void)hipGetLastError();
         return NS_ERR_NOMEM;
     }
-    b->allocs.push_back(q);
     *p = (T *)q;
     return NS_OK;
 }
 
-constexpr long BATCH_ROUNDS = 8;
+// phases own their parameter tables (callers synchronise the stream before)
+void free_phases(std::vector<ns_batch::Phase> &ph) {
+    for (auto &p : ph) {
+        if (p.params) (void)hipFree(p.params);
+        if (p.prefix) (void)hipFree(p.prefix);
+    }
+    ph.clear();
+}
 
 // parameters of every box for one launch, uploaded with their block prefix
 int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_batch::Phase> &out) {
@@ -1543,17 +1557,13 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     std::vector<int> prefix(n + 1, 0);
     size_t lds = 0;
     int variant = 0;
-    // predictor: blocks stream z chunks serially, so chunks of one length for
-    // all boxes, short enough that the launch spans about BATCH_ROUNDS rounds
-    // of resident blocks (a short tail), and at least 8 cells (the prologue
-    // of a chunk re-reads 3 slices)
-    int zc_hint = 0;
-    if (kind == 0) {
-        long cols = 0;
+    // predictor: blocks stream z chunks serially, so chunks of one length per
+    // size class, short enough that the launch spans about `rounds` rounds of
+    // resident blocks (a short tail), and at least 8 cells (the prologue of a
+    // chunk re-reads 3 slices)
+    long cols = 0;
+    if (kind == 0)
         for (ns_ctx *c : b->ctx) cols += predict_tile_columns(c->cfg);
-        const long z = cols / ((long)predict_resident_blocks(false) * BATCH_ROUNDS);
-        zc_hint = (int)(z > 8 ? z : 8);
-    }
     for (int i = 0; i < n; i++) {
         ns_ctx *c = b->ctx[i];
         const ns_config &f = c->cfg;
@@ -1562,6 +1572,8 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         void *dst = host.data() + (size_t)i * psz;
         int nb = 0;
         if (kind == 0) {
+            const long z = cols / ((long)predict_resident_blocks(false) * b->knobs[b->cls[i]].rounds);
+            const int zc_hint = (int)(z > 8 ? z : 8);
             nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr, c->p,
                                      c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst, zc_hint);
         } else if (kind == 1) {
@@ -1569,14 +1581,16 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         } else if (kind == 2) {
             size_t l = 0;
             nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, c->us, c->vs, c->ws, nullptr, c->planes,
-                                      c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l);
+                                      c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l,
+                                      b->knobs[b->cls[i]].dct_drop);
             lds = l > lds ? l : lds;
             variant |= dct_params_variant(dst);
         } else if (kind == 5) {
             nb = make_cn2_params(g, f.bc[arg], arg, c->planes, c->us, c->vs, c->ws, dst);
         } else if (kind == 6) {
             size_t l = 0;
-            nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l);
+            nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l,
+                                   b->knobs[b->cls[i]].adix_lsh);
             lds = l > lds ? l : lds;
         } else if (kind == 3) {
             nb = make_pr_bc_params(g, f.pbc, c->planes, d, c->dp, arg, dst);
@@ -1595,11 +1609,13 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     ph.lds = lds;
     ph.variant = variant;
     int rc;
-    if ((rc = balloc(b, &ph.params, host.size())) || (rc = balloc(b, &ph.prefix, prefix.size() * sizeof(int))))
+    if ((rc = balloc(&ph.params, host.size())) || (rc = balloc(&ph.prefix, prefix.size() * sizeof(int)))) {
+        if (ph.params) (void)hipFree(ph.params);
         return rc;
+    }
+    out.push_back(ph);  // owned by `out` from here
     HIPCHK(hipMemcpy(ph.params, host.data(), host.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ph.prefix, prefix.data(), prefix.size() * sizeof(int), hipMemcpyHostToDevice));
-    out.push_back(ph);
     return NS_OK;
 }
 
@@ -1657,6 +1673,10 @@ int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
     }
     ns_batch *b = new ns_batch();
     b->ctx.assign(ctxs, ctxs + n);
+    for (ns_ctx *c : b->ctx) {
+        const size_t cells = c->g.ni();
+        b->cls.push_back(cells < ((size_t)1 << 17) ? 0 : cells < ((size_t)1 << 20) ? 1 : 2);
+    }
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
         delete b;
@@ -1675,7 +1695,7 @@ int ns_batch_destroy(ns_batch *b) {
     if (!b) return NS_ERR_INVALID;
     if (b->st) (void)hipStreamSynchronize(b->st);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    for (void *q : b->allocs) (void)hipFree(q);
+    free_phases(b->phases);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
     return NS_OK;
@@ -1692,9 +1712,11 @@ int ns_batch_step(ns_batch *b, int nsteps) {
         // the first step (Euler for the boxes that start fresh) runs from its own parameters
         std::vector<ns_batch::Phase> ph;
         int rc = build_phases(b, true, ph);
+        if (!rc) run_phases(b, ph);
+        const hipError_t e = hipStreamSynchronize(b->st);
+        free_phases(ph);
         if (rc) return rc;
-        run_phases(b, ph);
-        HIPCHK(hipStreamSynchronize(b->st));
+        HIPCHK(e);
         for (ns_ctx *c : b->ctx) c->first = 0;
         s = 1;
     }
@@ -1716,6 +1738,118 @@ int ns_batch_step(ns_batch *b, int nsteps) {
     }
     HIPCHK(hipStreamSynchronize(b->st));
     return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
+}
+
+int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
+    if (!b || trial_steps < 1) return NS_ERR_INVALID;
+    for (ns_ctx *c : b->ctx) HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    // the state every trial starts from: u, v, w, p and the AB2 histories of each box
+    std::vector<double *> saved;
+    auto release = [&]() {
+        for (double *q : saved) (void)hipFree(q);
+        saved.clear();
+    };
+    std::vector<int> first;
+    for (ns_ctx *c : b->ctx) {
+        const Grid &g = c->g;
+        const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
+        double *src[7] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz};
+        for (int i = 0; i < 7; i++) {
+            double *q = nullptr;
+            if (balloc(&q, n[i] * sizeof(double))) {
+                release();
+                return NS_ERR_NOMEM;
+            }
+            saved.push_back(q);
+            (void)hipMemcpyAsync(q, src[i], n[i] * sizeof(double), hipMemcpyDeviceToDevice, b->st);
+        }
+        first.push_back(c->first);
+    }
+    auto restore = [&]() {
+        size_t k = 0;
+        for (size_t j = 0; j < b->ctx.size(); j++) {
+            ns_ctx *c = b->ctx[j];
+            const Grid &g = c->g;
+            const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
+            double *dst[7] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz};
+            for (int i = 0; i < 7; i++, k++)
+                (void)hipMemcpyAsync(dst[i], saved[k], n[i] * sizeof(double), hipMemcpyDeviceToDevice, b->st);
+            c->first = first[j];
+        }
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        return NS_ERR_HIP;
+    }
+    // ms of trial_steps steady steps with the current knobs (best of two runs)
+    auto trial = [&](float &ms) -> int {
+        std::vector<ns_batch::Phase> ph;
+        int rc = build_phases(b, false, ph);
+        ms = 1e30f;
+        for (int rep = 0; rep < 2 && !rc; rep++) {
+            run_phases(b, ph);  // warm-up of this shape
+            (void)hipEventRecord(e0, b->st);
+            for (int s = 0; s < trial_steps; s++) run_phases(b, ph);
+            (void)hipEventRecord(e1, b->st);
+            restore();
+            if (hipEventSynchronize(e1) != hipSuccess) rc = NS_ERR_HIP;
+            float t = 0.0f;
+            if (!rc && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t < ms) ms = t;
+        }
+        (void)hipStreamSynchronize(b->st);
+        free_phases(ph);
+        (void)hipGetLastError();
+        return rc;
+    };
+    float best = 0.0f;
+    int rc = trial(best);
+    bool present[3] = {false, false, false};
+    for (int k : b->cls) present[k] = true;
+    // coordinate descent over (size class, knob); a change is kept when it
+    // saves more than 1% of the step
+    const int cand_rounds[4] = {4, 8, 16, 32}, cand_drop[3] = {0, 1, 2}, cand_lsh[4] = {6, 5, 4, 3};
+    for (int k = 0; k < 3 && !rc; k++) {
+        if (!present[k]) continue;
+        for (int knob = 0; knob < 3 && !rc; knob++) {
+            int *val = knob == 0 ? &b->knobs[k].rounds : knob == 1 ? &b->knobs[k].dct_drop : &b->knobs[k].adix_lsh;
+            const int *cand = knob == 0 ? cand_rounds : knob == 1 ? cand_drop : cand_lsh;
+            const int nc = knob == 1 ? 3 : 4;
+            const int keep = *val;
+            int best_v = keep;
+            for (int i = 0; i < nc && !rc; i++) {
+                if (cand[i] == keep) continue;
+                *val = cand[i];
+                float t = 0.0f;
+                rc = trial(t);
+                if (!rc && t < 0.99f * best) {
+                    best = t;
+                    best_v = cand[i];
+                }
+            }
+            *val = best_v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    release();
+    if (rc) return rc;
+    // the steady-step tables and graph with the chosen shapes
+    free_phases(b->phases);
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    if ((rc = build_phases(b, false, b->phases))) return rc;
+    if (knobs_out)
+        for (int k = 0; k < 3; k++) {
+            knobs_out[3 * k] = b->knobs[k].rounds;
+            knobs_out[3 * k + 1] = b->knobs[k].dct_drop;
+            knobs_out[3 * k + 2] = b->knobs[k].adix_lsh;
+        }
+    return NS_OK;
 }
 
 }  // extern "C"

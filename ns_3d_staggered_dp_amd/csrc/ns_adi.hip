@@ -264,7 +264,7 @@ size_t sweep_params_size() { return sizeof(SweepParams); }
 // parameters of the sweep along axis dir-1; returns the block count (grid.y = 3
 // components) and the dynamic LDS size
 int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      void *out, size_t *lds) {
+                      void *out, size_t *lds, int x_lsh) {
     SweepParams &P = *(SweepParams *)out;
     P = SweepParams{};
     P.g = g;
@@ -276,7 +276,7 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
         // longest x line, lines per block (up to 64 within 64 KiB), blocks of the largest component
         const int nmax = g.nx + 2;
         const size_t row = (size_t)nmax + 1;
-        int lsh = 6;
+        int lsh = (x_lsh >= 0 && x_lsh < 6) ? x_lsh : 6;
         while (lsh > 0 && ((size_t)1 << lsh) * row * sizeof(double) > 65536) lsh--;
         const size_t L = (size_t)1 << lsh;
         P.lsh = lsh;

@@ -130,15 +130,17 @@ int make_cn2_params(const Grid &g, int bc, int ax, const double *planes, double 
 void launch_cn2_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
 // ADI sweep along axis dir-1 (ns_adi.hip): per-box parameters, block count (grid.y = 3), LDS size
 size_t sweep_params_size();
+// x_lsh: at most 2^x_lsh lines per x-sweep block (6: 64)
 int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      void *out, size_t *lds);
+                      void *out, size_t *lds, int x_lsh = 6);
 void launch_adi_batch(int ax, const void *params, const int *prefix, int nbox, int nblk, size_t lds, hipStream_t st);
 struct PoissonPlans;
 size_t pass_params_size();
+// lsh_drop: lines per block reduced by 2^lsh_drop from the default (a launch-shape choice only)
 int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                          const double *us, const double *vs, const double *ws, const double *rhs_in,
                          const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
-                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds);
+                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds, int lsh_drop = 0);
 // kernel variant of a box's pass (bit 1: odd radices, bit 0: lines over 1024);
 // a batch launch uses the bitwise OR over its boxes
 int dct_params_variant(const void *params);

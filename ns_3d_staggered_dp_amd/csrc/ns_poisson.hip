@@ -964,7 +964,7 @@ void dct_launch_batch_v(int v, const PassParams *Ps, const int *prefix, int nbox
 int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                          const double *us, const double *vs, const double *ws, const double *rhs_in,
                          const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
-                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds) {
+                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds, int lsh_drop) {
     PassParams &P = *(PassParams *)out;
     P = PassParams{};
     P.g = g;
@@ -997,14 +997,16 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
     P.dN = make_idiv(g.nx);
     P.dNy = make_idiv(g.ny);
     if (ax == 0) {
-        P.lsh = fft_lsh(g.nx, el);
+        P.lsh = fft_lsh(g.nx, el) - lsh_drop;
+        if (P.lsh < 0) P.lsh = 0;
         P.LS = fft_row(g.nx);
         *lds = ((size_t)(1 << P.lsh) * P.LS + g.nx) * sizeof(double2);
         const long RL = (long)g.ny * g.nz, per = 2L << P.lsh;
         return (int)((RL + per - 1) / per);
     }
     const int N = n[ax];
-    P.lsh = dct_yz_lsh(g.nx, N, el);
+    P.lsh = dct_yz_lsh(g.nx, N, el) - lsh_drop;
+    if (P.lsh < 0) P.lsh = 0;
     P.ngrp = (int)((g.nx + (2L << P.lsh) - 1) / (2L << P.lsh));
     P.dGrp = make_idiv(P.ngrp);
     *lds = ((size_t)(1 << P.lsh) * N + N) * sizeof(double2);

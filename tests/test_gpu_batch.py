@@ -79,3 +79,39 @@ def test_box_sizes_are_transform_sizes():
     sizes = box_sizes(64, 32, 128, seed=4)
     assert len(sizes) == 64 and all(32 <= n <= 128 for s in sizes for n in s)
     assert box_sizes(64, seed=4) == sizes
+
+
+def test_tuned_batch_keeps_state_and_equals_individual_steps():
+    """ns_batch_tune times launch shapes on the boxes' own state and restores
+    it; the tuned batch still steps bitwise like the boxes on their own."""
+    sizes = [(12, 10, 8), (70, 64, 40), (40, 36, 48), (96, 90, 126)]  # small, medium, large classes
+    cfgs = [make(*sz, AB2_CN) for sz in sizes]
+    data = [inputs(c, 80 + i) for i, c in enumerate(cfgs)]
+    ref = []
+    for c, (st, pl) in zip(cfgs, data):
+        with NSSolver(c) as s:
+            s.upload(*st, first=False)
+            s.set_bc_planes(pl)
+            s.step(3)
+            ref.append(s.download())
+    solvers = [NSSolver(c) for c in cfgs]
+    try:
+        for s, (st, pl) in zip(solvers, data):
+            s.upload(*st, first=False)
+            s.set_bc_planes(pl)
+        before = [s.download() for s in solvers]
+        with Batch(solvers) as b:
+            knobs = b.tune(trial_steps=1)
+            assert set(knobs) == {"small", "medium", "large"}
+            for s, st0 in zip(solvers, before):
+                got = s.download()
+                for k in ("u", "v", "w", "p", "hx", "hy", "hz"):
+                    assert np.array_equal(got[k], st0[k]), k
+            b.step(3)
+        for i, s in enumerate(solvers):
+            got = s.download()
+            for k in ("u", "v", "w", "p", "hx", "hy", "hz"):
+                assert np.array_equal(got[k], ref[i][k]), (sizes[i], k)
+    finally:
+        for s in solvers:
+            s.close()
