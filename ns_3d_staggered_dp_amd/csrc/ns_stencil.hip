@@ -674,8 +674,77 @@ static size_t max_plane(const Grid &g, int ax) {
     return m;
 }
 
+namespace {
+
+// All-periodic ghost fill of up to three arrays in one pass.  The x, y, z
+// fills copy interior values and each later axis copies the earlier axes'
+// ghost columns, so every ghost entry ends up as the interior value at its
+// index wrapped along every axis (pure copies: bitwise the sequential fills).
+// Wrap along axis b of length N: the normal axis of a velocity component maps
+// N -> 1 (NS_functions.f90:236), the other axes and pressure map 1 -> N-1,
+// N -> 2 (:237-240, :1420-1422; the y and z blocks alike).  Sources are never
+// ghosts, so no value is read after another thread writes it.
+struct PerFill {
+    Grid g;
+    int comp[3];  // 0..2 velocity component, 3 pressure
+    double *a[3];
+    int first_axis;  // faces of the axes first_axis..2 are written
+};
+
+__global__ void __launch_bounds__(256) k_periodic_fill(const PerFill P) {
+    const int k = blockIdx.y, face = blockIdx.z, ax = face >> 1;
+    if (ax < P.first_axis) return;
+    const int c = P.comp[k];
+    int n[3];
+    comp_dims(P.g, c < 3 ? c : 3, n);
+    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t)n[o1] * n[o2]) return;
+    int id[3];
+    id[o1] = (int)(t % n[o1]) + 1;
+    id[o2] = (int)(t / n[o1]) + 1;
+    id[ax] = (face & 1) ? n[ax] : 1;
+    int src[3];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const int N = n[b], i = id[b];
+        if (b == c) src[b] = (i == N) ? 1 : i;
+        else src[b] = (i == 1) ? N - 1 : (i == N) ? 2 : i;
+    }
+    if (src[0] == id[0] && src[1] == id[1] && src[2] == id[2]) return;
+    double *A = P.a[k];
+    A[FIX(id[0], id[1], id[2], n[0], n[1])] = A[FIX(src[0], src[1], src[2], n[0], n[1])];
+}
+
+void launch_periodic_fill(const Grid &g, int na, const int comp[3], double *const a[3], int first_axis,
+                          hipStream_t st) {
+    PerFill P;
+    P.g = g;
+    for (int k = 0; k < 3; k++) {
+        P.comp[k] = k < na ? comp[k] : 3;
+        P.a[k] = k < na ? a[k] : nullptr;
+    }
+    P.first_axis = first_axis;
+    size_t mx = 0;
+    for (int k = 0; k < na; k++) {
+        int n[3];
+        comp_dims(g, comp[k], n);
+        const size_t f[3] = {(size_t)n[1] * n[2], (size_t)n[0] * n[2], (size_t)n[0] * n[1]};
+        for (int b = 0; b < 3; b++) mx = f[b] > mx ? f[b] : mx;
+    }
+    hipLaunchKernelGGL(k_periodic_fill, dim3(nblocks(mx), na, 6), dim3(256), 0, st, P);
+}
+
+}  // namespace
+
 void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w,
                    hipStream_t st, int first_axis) {
+    if (bc[0] == 1 && bc[1] == 1 && bc[2] == 1) {
+        const int comp[3] = {0, 1, 2};
+        double *const a[3] = {u, v, w};
+        launch_periodic_fill(g, 3, comp, a, first_axis, st);
+        return;
+    }
     BCParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
     P.L = plane_layout(g.nx, g.ny, g.nz);
@@ -717,6 +786,12 @@ void launch_cn2_batch(const void *params, const int *prefix, int nbox, int nblk,
 
 void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,
                   hipStream_t st, int first_axis) {
+    if (pbc[0] == 1 && pbc[1] == 1 && pbc[2] == 1) {
+        const int comp[3] = {3, 3, 3};
+        double *const a[3] = {dp, nullptr, nullptr};
+        launch_periodic_fill(g, 1, comp, a, first_axis, st);
+        return;
+    }
     BCParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
     P.L = plane_layout(g.nx, g.ny, g.nz);
