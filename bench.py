@@ -61,7 +61,8 @@ def parse():
                     help="run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--summarize-profile", metavar="DIR",
                     help="(no GPU) turn rocprofv3 output under DIR (kt/, FETCH_SIZE/, WRITE_SIZE/, cal_FETCH_SIZE/, "
-                         "cal_WRITE_SIZE/) into the per-launch traffic JSON and a markdown table, then exit")
+                         "cal_WRITE_SIZE/) into markdown tables (the 256^3 step's kernels against their algorithmic "
+                         "bytes, written to the traffic JSON when present; every kernel's counter traffic), then exit")
     return ap.parse_args()
 
 
@@ -108,8 +109,10 @@ def summarize_profile(d, traffic_json):
             stats[short_kernel_name(r["Name"])] = (int(r["Calls"]), float(r["AverageNs"]) / 1e3)
     n3 = 256 ** 3
     traffic, rows = {}, []
+    # the algorithmic table is the 256^3 headline step's (its lazy predictor is in the run)
+    headline = KERNEL_NAMES["predictor"] in fetch
     for phase, kname in KERNEL_NAMES.items():
-        if kname not in fetch or kname not in write:
+        if not headline or kname not in fetch or kname not in write:
             continue
         rd, wr = fetch[kname] * f_fac * 1024, write[kname] * w_fac * 1024
         traffic[kname] = int(rd + wr)
@@ -117,11 +120,26 @@ def summarize_profile(d, traffic_json):
         calls, us = stats.get(kname, (0, float("nan")))
         rows.append(f"| {kname} | {calls} | {us:.1f} | {alg / 1e6:.0f} | {rd / 1e6:.0f} | {wr / 1e6:.0f} | "
                     f"{(rd + wr) / alg:.2f} | {alg / (us * 1e-6) / 1e9:.0f} | {(rd + wr) / (us * 1e-6) / 1e9:.0f} |")
-    json.dump(traffic, open(traffic_json, "w"), indent=1)
+    if rows:
+        json.dump(traffic, open(traffic_json, "w"), indent=1)
     print(f"FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} (1 GiB copy)")
-    print("| kernel | calls | avg us | alg MB/launch | HBM read MB | HBM write MB | traffic/alg | alg GB/s | HBM GB/s |")
-    print("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
-    print("\n".join(rows))
+    if rows:
+        print("| kernel | calls | avg us | alg MB/launch | HBM read MB | HBM write MB | traffic/alg | alg GB/s | HBM GB/s |")
+        print("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
+        print("\n".join(rows))
+        print()
+    # every kernel of the run: counter traffic per launch and the rate it implies
+    print("| kernel (all launches of the run) | calls | avg us | HBM read MB | HBM write MB | HBM GB/s |")
+    print("|---|---:|---:|---:|---:|---:|")
+    allk = []
+    for kname in set(fetch) & set(write):
+        if "copyBuffer" in kname or "fillBuffer" in kname or kname not in stats:
+            continue
+        rd, wr = fetch[kname] * f_fac * 1024, write[kname] * w_fac * 1024
+        calls, us = stats[kname]
+        allk.append((calls * us, f"| {kname} | {calls} | {us:.1f} | {rd / 1e6:.1f} | {wr / 1e6:.1f} | "
+                                 f"{(rd + wr) / (us * 1e-6) / 1e9:.0f} |"))
+    print("\n".join(r for _, r in sorted(allk, reverse=True)))
     return 0
 
 

@@ -324,9 +324,19 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
     const int N = (AXIS == 1) ? g.ny : g.nz, lsh = P.lsh, L = 1 << lsh, nxh = P.nxh, nxhp = P.nxhp;
     double2 *A = smem, *tws = smem + (size_t)L * N;
     for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
-    const int ngrp = (nxh + L - 1) >> lsh;
-    const int kx0 = (blockIdx.x % ngrp) << lsh;
-    const int other = blockIdx.x / ngrp;
+    // blocks of fewer than 8 wavenumbers share a 128-byte line of a spectrum
+    // row with G-1 others: those run on one XCD (physical blocks 8 apart), so
+    // the line is fetched into that XCD's L2 once
+    const int G = (L < 8) ? (8 >> lsh) : 1;
+    const int ngrp = ((((nxh + L - 1) >> lsh) + G - 1) / G) * G;
+    int blk = (int)blockIdx.x;
+    if (G > 1) {
+        const int sl = blk >> 3, xcd = blk & 7;
+        blk = ((sl / G) * 8 + xcd) * G + (sl % G);
+    }
+    const int other = blk / ngrp;
+    if (other >= ((AXIS == 1) ? g.nz : g.ny)) return;  // padding of the grid (uniform per block)
+    const int kx0 = (blk - other * ngrp) << lsh;
     const int cnt = L * N;
 #pragma unroll 4
     for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
@@ -839,7 +849,14 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     const long RL = (long)g.ny * g.nz;
     auto lds_x = [&]() { return ((size_t)(1 << P.lsh) * P.LS + g.nx) * sizeof(double2); };
     auto lds_yz = [&](int n) { return ((size_t)(1 << P.lsh) * n + n) * sizeof(double2); };
-    auto nb_yz = [&](int others) { return (unsigned)((((P.nxh + (1 << P.lsh) - 1) >> P.lsh)) * others); };
+    // blocks of a y/z pass: wavenumber groups per row (padded to whole 128-byte
+    // lines when a group is narrower) times rows, a multiple of 8 lines' blocks
+    auto nb_yz = [&](int others) {
+        const int L = 1 << P.lsh, G = (L < 8) ? 8 / L : 1;
+        const long ngrp = (((P.nxh + L - 1) / L + G - 1) / G) * G;
+        const long nb = ngrp * others, q = 8L * G;
+        return (unsigned)((nb + q - 1) / q * q);
+    };
     // x forward (fused with the divergence)
     P.pl = pl.px;
     // lines longer than the x-pass cap use the full-size register sets
