@@ -216,6 +216,7 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
     const double a = 0.5 * dt * nu * 1.0 * idx2;
     S = LineSys{};
     S.a = a;
+    int rc0 = NS_OK;
     std::vector<double> lo, d, up, inv, cp;
     if (bc == 1) {
         if (transpose) return NS_ERR_UNSUPPORTED;
@@ -278,6 +279,27 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         }
         thomas_factor(lo, d, up, inv, cp);
         S.z = nullptr;
+        // twisted factorization at tk = N/2: the bottom-up elimination is Thomas
+        // on the reversed system (sub- and super-diagonal exchanged)
+        const int tk = N / 2;
+        std::vector<double> loR(N, 0.0), dR(N), upR(N, 0.0), invR, cpR;
+        for (int i = 0; i < N; i++) {
+            loR[i] = up[N - 1 - i];
+            dR[i] = d[N - 1 - i];
+            upR[i] = lo[N - 1 - i];
+        }
+        thomas_factor(loR, dR, upR, invR, cpR);
+        S.tk = tk;
+        S.ak = lo[tk];
+        S.ck = up[tk];
+        // cp_{tk-1} (top-down) and cq_{tk+1} = cpR[N-2-tk] (bottom-up)
+        S.gk = 1.0 / (d[tk] - lo[tk] * cp[tk - 1] - up[tk] * cpR[N - 2 - tk]);
+        double *dlr, *dir_, *dcr;
+        if ((rc0 = dupload(c, &dlr, loR)) || (rc0 = dupload(c, &dir_, invR)) || (rc0 = dupload(c, &dcr, cpR)))
+            return rc0;
+        S.loR = dlr;
+        S.invR = dir_;
+        S.cpR = dcr;
     } else {
         return NS_ERR_UNSUPPORTED;
     }

@@ -23,11 +23,12 @@ namespace {
 constexpr int SB = 16;  // staging copies in flight per thread
 
 // forward elimination over y[0..m) (stride s): d_i = (r_i - lo_i d_{i-1}) inv_i,
-// stored in place; r_0 and (m > 1) r_{m-1} are replaced by r_first, r_last when ovr.
-// Returns d_{m-1}.
+// stored to dout (stride so; y itself for an in-place solve); r_0 and (m > 1)
+// r_{m-1} are replaced by r_first, r_last when ovr.  Returns d_{m-1}.
 template <int CHK>
-__device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const double *lo, const double *inv,
-                                             bool ovr, double r_first, double r_last) {
+__device__ __forceinline__ double thomas_fwd(const double *y, long s, double *dout, long so, int m,
+                                             const double *lo, const double *inv, bool ovr, double r_first,
+                                             double r_last) {
     double cur[CHK], nxt[CHK];
 #pragma unroll
     for (int j = 0; j < CHK; j++) cur[j] = (j < m) ? y[s * j] : 0.0;
@@ -52,7 +53,7 @@ __device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const d
                     else if (i == m - 1) r = r_last;
                 }
                 d = (i == 0) ? r * cinv[j] : (r - clo[j] * d) * cinv[j];
-                y[s * i] = d;
+                dout[so * i] = d;
             }
         }
 #pragma unroll
@@ -61,10 +62,12 @@ __device__ __forceinline__ double thomas_fwd(double *y, size_t s, int m, const d
     return d;
 }
 
-// back substitution x_i = d_i - cp_i x_{i+1}, i = m-2 .. 0, with x_{m-1} = last
-// already in place; returns x_0
+// back substitution x_i = d_i - cp_i x_{i+1}, i = m-2 .. 0, reading d from y
+// (stride s) and storing x to xout (stride so), x_{m-1} = last already stored
+// there; returns x_0
 template <int CHK>
-__device__ __forceinline__ double thomas_bwd(double *y, size_t s, int m, const double *cp, double last) {
+__device__ __forceinline__ double thomas_bwd(const double *y, long s, double *xout, long so, int m,
+                                             const double *cp, double last) {
     double cur[CHK], nxt[CHK];
     const int top = m - 2;
 #pragma unroll
@@ -81,7 +84,7 @@ __device__ __forceinline__ double thomas_bwd(double *y, size_t s, int m, const d
             const int i = i0 - j;
             if (i >= 0) {
                 x = cur[j] - ccp[j] * x;
-                y[s * i] = x;
+                xout[so * i] = x;
             }
         }
 #pragma unroll
@@ -93,10 +96,10 @@ __device__ __forceinline__ double thomas_bwd(double *y, size_t s, int m, const d
 // solve one line in place (stride s); CHK: elements per register chunk
 // (16 for lines in global memory, 8 for lines in LDS)
 template <int CHK>
-__device__ __forceinline__ void solve_line(double *x, size_t s, const LineSys &S) {
+__device__ __forceinline__ void solve_line(double *x, long s, const LineSys &S) {
     if (!S.periodic) {
-        const double last = thomas_fwd<CHK>(x, s, S.N, S.lo, S.inv, false, 0.0, 0.0);
-        thomas_bwd<CHK>(x, s, S.N, S.cp, last);
+        const double last = thomas_fwd<CHK>(x, s, x, s, S.N, S.lo, S.inv, false, 0.0, 0.0);
+        thomas_bwd<CHK>(x, s, x, s, S.N, S.cp, last);
         return;
     }
     const int m = S.m;
@@ -115,8 +118,8 @@ __device__ __forceinline__ void solve_line(double *x, size_t s, const LineSys &S
         r_last = y[s * (m - 1)] + S.a * g_hi;
     }
     // Thomas on the reduced system (rows 0 and m-1 carry the folded ghosts)
-    const double ylast = thomas_fwd<CHK>(y, s, m, S.lo, S.inv, true, r_first, r_last);
-    const double y0 = thomas_bwd<CHK>(y, s, m, S.cp, ylast);
+    const double ylast = thomas_fwd<CHK>(y, s, y, s, m, S.lo, S.inv, true, r_first, r_last);
+    const double y0 = thomas_bwd<CHK>(y, s, y, s, m, S.cp, ylast);
     // Sherman-Morrison correction
     const double f = (y0 + S.s * ylast) / S.den;
     const double *z = S.z;
@@ -155,20 +158,56 @@ struct SweepParams {
 // y / z sweeps: one thread per line, lines walked in global memory (the
 // threads of a wave hold consecutive x, so every chunk load is coalesced and
 // the loads stream while the chain computes)
+//
+// Wall lines use the twisted factorization: lane l < 32 eliminates rows
+// 0..tk-1 downward, lane l + 32 rows N-1..tk+1 upward (the same code on a
+// reversed view), they exchange d_{tk-1} and e_{tk+1}, both form x_tk, and
+// each substitutes back over its half: chains of about N/2 steps, two lanes
+// per line.  Periodic lines: one lane per line (reduced cyclic solve).
 template <int AX>
 __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int blk) {
     const int c = blockIdx.y;
     int n[3];
     comp_dims(P.g, c, n);
     const int o2 = (AX == 2) ? 1 : 2;
-    const size_t t = (size_t)blk * 64 + threadIdx.x;
-    if (t >= (size_t)n[0] * n[o2]) return;
-    int id[3];
-    id[0] = (int)(t % n[0]) + 1;
-    id[o2] = (int)(t / n[0]) + 1;
-    id[AX] = 1;
-    const size_t stride = (AX == 1) ? (size_t)n[0] : (size_t)n[0] * n[1];
-    solve_line<16>(P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]), stride, P.sys[c]);
+    const long stride = (AX == 1) ? (long)n[0] : (long)n[0] * n[1];
+    const LineSys &S = P.sys[c];
+    const size_t nl = (size_t)n[0] * n[o2];
+    if (S.periodic) {
+        const size_t t = (size_t)blk * 64 + threadIdx.x;
+        if (t >= nl) return;
+        int id[3];
+        id[0] = (int)(t % n[0]) + 1;
+        id[o2] = (int)(t / n[0]) + 1;
+        id[AX] = 1;
+        solve_line<16>(P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]), stride, S);
+        return;
+    }
+    const bool top = threadIdx.x < 32;
+    const size_t t = (size_t)blk * 32 + (threadIdx.x & 31);
+    const bool valid = t < nl;
+    int id[3] = {1, 1, 1};
+    if (valid) {
+        id[0] = (int)(t % n[0]) + 1;
+        id[o2] = (int)(t / n[0]) + 1;
+    }
+    const int N = S.N, tk = S.tk;
+    double *x = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]);
+    double *base = top ? x : x + stride * (N - 1);
+    const long st = top ? stride : -stride;
+    const int m = valid ? (top ? tk : N - 1 - tk) : 0;
+    // the elimination results of a half line stay in this lane's LDS column
+    extern __shared__ double sm[];
+    double *col = sm + threadIdx.x;
+    const double dk = thomas_fwd<16>(base, st, col, 64, m, top ? S.lo : S.loR, top ? S.inv : S.invR, false, 0.0, 0.0);
+    const double other = __shfl_xor(dk, 32);
+    double xk = 0.0;
+    if (valid) {
+        const double dt = top ? dk : other, eb = top ? other : dk;
+        xk = ((x[stride * tk] - S.ak * dt) - S.ck * eb) * S.gk;
+    }
+    if (valid && top) x[stride * tk] = xk;
+    thomas_bwd<16>(col, 64, base, st, valid ? (top ? tk + 1 : N - tk) : 0, top ? S.cp : S.cpR, xk);
 }
 
 // x sweeps: a block stages L = 2^lsh lines of one component in LDS (x lines
@@ -220,7 +259,31 @@ __device__ __forceinline__ void adi_x_body(const SweepParams &P, const int blk) 
             if (f0 + 64 * u < tot) sm[slot(f0 + 64 * u)] = v[u];
     }
     __syncthreads();
-    if ((int)threadIdx.x < cnt) solve_line<8>(sm + threadIdx.x * LS, 1, S);
+    if (S.periodic) {
+        if ((int)threadIdx.x < cnt) solve_line<8>(sm + threadIdx.x * LS, 1, S);
+    } else {
+        // twisted factorization, two lanes per line (L <= 32): lane l < 32 the
+        // top half of line l, lane l + 32 its bottom half
+        const bool top = threadIdx.x < 32;
+        const int l = threadIdx.x & 31;
+        const bool valid = l < cnt;
+        const int tk = S.tk;
+        double *row = sm + (valid ? l : 0) * LS;
+        double *base = top ? row : row + (N - 1);
+        const long st = top ? 1 : -1;
+        const int m = valid ? (top ? tk : N - 1 - tk) : 0;
+        const double dk = thomas_fwd<8>(base, st, base, st, m, top ? S.lo : S.loR, top ? S.inv : S.invR, false, 0.0,
+                                        0.0);
+        const double other = __shfl_xor(dk, 32);
+        double xk = 0.0;
+        if (valid) {
+            const double dt = top ? dk : other, eb = top ? other : dk;
+            xk = ((row[tk] - S.ak * dt) - S.ck * eb) * S.gk;
+        }
+        __syncthreads();  // both lanes have read row[tk]
+        if (valid && top) row[tk] = xk;
+        thomas_bwd<8>(base, st, base, st, valid ? (top ? tk + 1 : N - tk) : 0, top ? S.cp : S.cpR, xk);
+    }
     __syncthreads();
     for (int f0 = threadIdx.x; f0 < tot; f0 += 64 * SB) {
         double v[SB];
@@ -277,6 +340,7 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
         const int nmax = g.nx + 2;
         const size_t row = (size_t)nmax + 1;
         int lsh = (x_lsh >= 0 && x_lsh < 6) ? x_lsh : 6;
+        if (!sysA.periodic && lsh > 5) lsh = 5;  // wall lines: two lanes per line
         while (lsh > 0 && ((size_t)1 << lsh) * row * sizeof(double) > 65536) lsh--;
         const size_t L = (size_t)1 << lsh;
         P.lsh = lsh;
@@ -291,8 +355,17 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
         const size_t m = (size_t)n[0] * ((ax == 1) ? n[2] : n[1]);
         nl = m > nl ? m : nl;
     }
-    *lds = 0;
-    return (int)((nl + 63) / 64);
+    // wall lines: two lanes per line (twisted factorization), each with an
+    // LDS column for its half line's elimination results
+    int nmax = 0;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        nmax = n[ax] > nmax ? n[ax] : nmax;
+    }
+    *lds = sysA.periodic ? 0 : (size_t)64 * (nmax / 2 + 1) * sizeof(double);
+    const size_t per = sysA.periodic ? 64 : 32;
+    return (int)((nl + per - 1) / per);
 }
 
 void launch_adi_batch(int ax, const void *params, const int *prefix, int nbox, int nblk, size_t lds, hipStream_t st) {
@@ -301,9 +374,11 @@ void launch_adi_batch(int ax, const void *params, const int *prefix, int nbox, i
         set_lds(k_adi_x_batch, lds);
         hipLaunchKernelGGL(k_adi_x_batch, dim3(nblk, 3), dim3(64), lds, st, Ps, prefix, nbox);
     } else if (ax == 1) {
-        hipLaunchKernelGGL(k_adi_strided_batch<1>, dim3(nblk, 3), dim3(64), 0, st, Ps, prefix, nbox);
+        set_lds(k_adi_strided_batch<1>, lds);
+        hipLaunchKernelGGL(k_adi_strided_batch<1>, dim3(nblk, 3), dim3(64), lds, st, Ps, prefix, nbox);
     } else {
-        hipLaunchKernelGGL(k_adi_strided_batch<2>, dim3(nblk, 3), dim3(64), 0, st, Ps, prefix, nbox);
+        set_lds(k_adi_strided_batch<2>, lds);
+        hipLaunchKernelGGL(k_adi_strided_batch<2>, dim3(nblk, 3), dim3(64), lds, st, Ps, prefix, nbox);
     }
 }
 
@@ -316,9 +391,11 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
         set_lds(k_adi_x, lds);
         hipLaunchKernelGGL(k_adi_x, dim3(nb, 3), dim3(64), lds, st, P);
     } else if (dir == 2) {
-        hipLaunchKernelGGL(k_adi_strided<1>, dim3(nb, 3), dim3(64), 0, st, P);
+        set_lds(k_adi_strided<1>, lds);
+        hipLaunchKernelGGL(k_adi_strided<1>, dim3(nb, 3), dim3(64), lds, st, P);
     } else {
-        hipLaunchKernelGGL(k_adi_strided<2>, dim3(nb, 3), dim3(64), 0, st, P);
+        set_lds(k_adi_strided<2>, lds);
+        hipLaunchKernelGGL(k_adi_strided<2>, dim3(nb, 3), dim3(64), lds, st, P);
     }
 }
 

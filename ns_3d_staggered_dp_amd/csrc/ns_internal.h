@@ -67,6 +67,12 @@ struct LineSys {
     // Sherman-Morrison correction (periodic only): x -= f * z, f = (y_0 + s*y_{m-1}) / den
     const double *z;
     double s, den;
+    // twisted factorization (walls only): rows 0..tk-1 eliminated top-down with
+    // lo, inv, cp; rows N-1..tk+1 bottom-up with loR, invR, cpR indexed from the
+    // bottom (loR[i'] = up_{N-1-i'}); row tk: x = (r - ak d_{tk-1} - ck e_{tk+1}) * gk
+    int tk;
+    double ak, ck, gk;
+    const double *loR, *invR, *cpR;
 };
 
 struct Ctx;
