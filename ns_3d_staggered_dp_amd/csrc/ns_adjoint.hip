@@ -50,11 +50,15 @@ __device__ __forceinline__ void dims_of(const Grid &g, int c, int n[3]) {
     if (c < 3) n[c] -= 1;
 }
 
+// 1-based (i, j, k) of flat index q (arrays hold fewer than 2^32 elements,
+// checked at context creation: 32-bit division through a double reciprocal)
 __device__ __forceinline__ void unflat(size_t q, const int n[3], int &i, int &j, int &k) {
-    i = (int)(q % n[0]) + 1;
-    const size_t r = q / n[0];
-    j = (int)(r % n[1]) + 1;
-    k = (int)(r / n[1]) + 1;
+    const unsigned qq = (unsigned)q, n0 = (unsigned)n[0], n1 = (unsigned)n[1];
+    const unsigned r = udiv_rd(qq, n0, 1.0 / n[0]);
+    const unsigned kk = udiv_rd(r, n1, 1.0 / n[1]);
+    i = (int)(qq - r * n0) + 1;
+    j = (int)(r - kk * n1) + 1;
+    k = (int)kk + 1;
 }
 
 // computed faces of component c (the reference's x_range0/x_range1 etc.)
@@ -466,7 +470,7 @@ __global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
     dims_of(g, 2, nw_);
     const int *nc = (c == 0) ? nu_ : (c == 1) ? nv_ : nw_;
     const size_t tot = (size_t)nc[0] * nc[1] * nc[2];
-    const size_t q = (size_t)blockIdx.x * AT + threadIdx.x;
+    const size_t q = (size_t)xcd_slab_block(blockIdx.x, gridDim.x) * AT + threadIdx.x;
     if (q >= tot) return;
     int i, j, k;
     unflat(q, nc, i, j, k);
@@ -672,7 +676,7 @@ void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, i
     const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
     for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_pred_prep, dim3(ablocks(tot[c])), dim3(AT), 0, st, P, c);
     if (dp_flag) hipLaunchKernelGGL(k_adj_pred_p, dim3(ablocks(g.np())), dim3(AT), 0, st, P);
-    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_convdiff, dim3(ablocks(tot[c])), dim3(AT), 0, st, P, c);
+    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_convdiff, dim3(round8(ablocks(tot[c]))), dim3(AT), 0, st, P, c);
 }
 
 }  // namespace ns

@@ -31,6 +31,13 @@ __device__ __forceinline__ unsigned udiv_rd(unsigned x, unsigned d, double rd) {
     return q;
 }
 
+// Blocks are dealt to the 8 XCDs round robin (block b on XCD b mod 8), each
+// with its own L2.  For a flat stencil launch of nb blocks (nb a multiple of
+// 8), this logical order gives every XCD one contiguous slab of the index
+// space, so the neighbour planes a block reads were fetched by the same XCD.
+__device__ __forceinline__ int xcd_slab_block(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
+inline unsigned int round8(unsigned int nb) { return (nb + 7u) & ~7u; }
+
 // component dims: c=0 u, 1 v, 2 w, 3 p
 __host__ __device__ inline void comp_dims(const Grid &g, int c, int n[3]) {
     n[0] = g.nx + 2; n[1] = g.ny + 2; n[2] = g.nz + 2;

@@ -379,7 +379,7 @@ __device__ __forceinline__ void project_body(const ProjParams &P, const size_t q
 constexpr int PROJ_PER = 4;
 
 __global__ void __launch_bounds__(256) k_project(ProjParams P) {
-    const size_t q0 = (size_t)blockIdx.x * (256 * PROJ_PER) + threadIdx.x;
+    const size_t q0 = (size_t)xcd_slab_block(blockIdx.x, gridDim.x) * (256 * PROJ_PER) + threadIdx.x;
 #pragma unroll
     for (int m = 0; m < PROJ_PER; m++) project_body(P, q0 + (size_t)m * 256, blockIdx.y);
 }
@@ -1018,7 +1018,7 @@ int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag
         P.r0[c] = 1.0 / n[0];
         P.r1[c] = 1.0 / n[1];
     }
-    return (int)nblocks(g.np(), 256 * PROJ_PER);
+    return (int)round8(nblocks(g.np(), 256 * PROJ_PER));
 }
 
 void launch_vel_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
