@@ -300,8 +300,8 @@ def main():
     if a.copy_calibration:
         return copy_calibration()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU decomposition is "
-              "described in DESIGN.md and is not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the multi-GPU decomposition (slabs with "
+              "halo exchange and transform all-to-alls, whole sub-boxes per GPU) is not built.", file=sys.stderr)
         return 2
     import torch
 
