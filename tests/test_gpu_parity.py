@@ -205,6 +205,21 @@ def test_step_walls(oracle, scheme, bc, pbc):
         assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
 
 
+@pytest.mark.parametrize("n", [(126, 12, 10), (12, 126, 10), (12, 10, 126)], ids=["long_x", "long_y", "long_z"])
+def test_step_walls_cn_long_lines(oracle, n):
+    """AB2-CN on walls with 126-cell lines along one axis: the x sweep's LDS
+    staging (32 lines per block, batched copies), the y/z sweeps' twisted
+    halves over several register chunks, and the DCT passes on long lines
+    with odd radices, against the oracle after 3 steps."""
+    cfg = cfg_box(*n, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1).replace(dt=1e-3)
+    u, v, w, p = rand_fields(cfg, 91)
+    u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
+    planes = rand_planes(cfg, 92, 0.2)
+    out, st, div = run_both(oracle, cfg, u, v, w, p, 3, planes=planes)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+
+
 def cfg_periodic(nx, ny, nz, **kw):
     return NSConfig(nx=nx, ny=ny, nz=nz, dx=2 * np.pi / nx, dy=0.5, dz=0.35, nu=2e-3, dt=2e-3, bc=(1, 1, 1),
                     pbc=(1, 1, 1), flags=FLAG_GAUGE_CORNER, **kw)
