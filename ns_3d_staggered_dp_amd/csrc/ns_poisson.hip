@@ -444,6 +444,7 @@ struct PassParams {
     int ngrp;    // y/z passes: groups of 2L consecutive x per row
     int odd;     // the plan has radices 3, 5 or 7
     int big;     // line longer than DCT_ELEMS: full-size register sets
+    int nb, nb8; // blocks of the pass, and that count rounded up to a multiple of 8 (launched)
     IDiv dN, dNy, dGrp;
     // x forward: divergence of u*, v*, w* (or the interior array rhs) ...
     const double *rhs;
@@ -779,7 +780,11 @@ __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk)
 
 // pass 0 x forward + RHS, 1 y forward, 2 z solve, 3 y inverse, 4 x inverse
 template <int PASS, bool ODD, int EL>
-__device__ __forceinline__ void dct_pass_body(const PassParams &P, const int blk) {
+__device__ __forceinline__ void dct_pass_body(const PassParams &P, const int b) {
+    // each XCD takes a contiguous range of the pass's blocks, so the blocks
+    // that share a cache line at their row ends share that XCD's L2
+    const int blk = xcd_slab_block(b, P.nb8);
+    if (blk >= P.nb) return;  // padding (uniform per block)
     if (PASS == 0) dct_fwd_body<0, ODD, EL>(P, blk);
     else if (PASS == 1) dct_fwd_body<1, ODD, EL>(P, blk);
     else if (PASS == 2) dct_solve_body<ODD, EL>(P, blk);
@@ -1019,7 +1024,9 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
         P.LS = fft_row(g.nx);
         *lds = ((size_t)(1 << P.lsh) * P.LS + g.nx) * sizeof(double2);
         const long RL = (long)g.ny * g.nz, per = 2L << P.lsh;
-        return (int)((RL + per - 1) / per);
+        P.nb = (int)((RL + per - 1) / per);
+        P.nb8 = (int)round8((unsigned)P.nb);
+        return P.nb8;
     }
     const int N = n[ax];
     P.lsh = dct_yz_lsh(g.nx, N, el) - lsh_drop;
@@ -1027,7 +1034,9 @@ int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, doubl
     P.ngrp = (int)((g.nx + (2L << P.lsh) - 1) / (2L << P.lsh));
     P.dGrp = make_idiv(P.ngrp);
     *lds = ((size_t)(1 << P.lsh) * N + N) * sizeof(double2);
-    return P.ngrp * (ax == 1 ? g.nz : g.ny);
+    P.nb = P.ngrp * (ax == 1 ? g.nz : g.ny);
+    P.nb8 = (int)round8((unsigned)P.nb);
+    return P.nb8;
 }
 
 size_t pass_params_size() { return sizeof(PassParams); }
