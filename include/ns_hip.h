@@ -267,6 +267,37 @@ int ns_pr_bc_staggered(ns_ctx *ctx, double *dp, const double *planes);
 int ns_project(ns_ctx *ctx, const double *u_star, const double *v_star, const double *w_star, const double *dp,
                double dt, int dp_flag, double *u, double *v, double *w, double *p);
 
+/* ---- adjoint mirrors (no counterpart in the reference: the transposed
+ * kernels of ns_adjoint_step, one per linear mirror above) ----
+ * Each applies the transpose of its forward mirror's linear map, with the
+ * same context and codes.  Arrays the forward transforms in place are
+ * transformed in place; the adjoints of the forward's plane-set input are
+ * ADDED to planes_b (packed set, plane-set length); other adjoint outputs
+ * are noted per entry.  Host or device arrays, as for the forward mirrors. */
+/* (u_b, v_b, w_b) := F_u^T (u_b, v_b, w_b), planes_b += F_p^T (...) */
+int ns_vel_bc_staggered_adj(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, double *planes_b, int bc_x,
+                            int bc_y, int bc_z);
+/* walls (bc 2, 4; bc 3 is the identity, bc 1 NS_ERR_UNSUPPORTED) */
+int ns_vel_bc_staggered_cn2_adj(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, double *planes_b, int bc,
+                                int dir);
+/* solve with the transposed CN systems (AB2-CN contexts with wall codes) */
+int ns_adi_sweep_adj(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, int dir);
+/* (u_b, v_b, w_b) += div^T rhs_b */
+int ns_divergence_adj(ns_ctx *ctx, const double *rhs_b, double dt, double *u_b, double *v_b, double *w_b);
+/* planes_b += the boundary terms' transpose applied to rhs_b (rhs itself passes through unchanged) */
+int ns_pr_bc_staggered_modify_rhs_adj(ns_ctx *ctx, const double *rhs_b, double *planes_b);
+/* rhs_b := P^T sol_b (the periodic solve is symmetric) */
+int ns_poisson_fft_adj(ns_ctx *ctx, const double *sol_b, double *rhs_b);
+/* in place: the solution's adjoint becomes the RHS's adjoint (solve with the
+ * zero mode dropped); *mean_dp_b += its sum when every pressure code is 3 */
+int ns_dct_poisson_solver_adj(ns_ctx *ctx, double *rhs_b, double *mean_dp_b);
+/* dp_b := F^T dp_b, planes_b += ... */
+int ns_pr_bc_staggered_adj(ns_ctx *ctx, double *dp_b, double *planes_b);
+/* from the adjoints of (u, v, w, p): us_b.. := u_b.., dp_b := p_b + grad^T(dt u_b),
+ * p_b := dp_flag * p_b (the adjoint of the input p) */
+int ns_project_adj(ns_ctx *ctx, const double *u_b, const double *v_b, const double *w_b, double *p_b, double dt,
+                   int dp_flag, double *us_b, double *vs_b, double *ws_b, double *dp_b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,15 @@ SIGNATURES = {
     "ns_batch_destroy": [P],
     "ns_batch_step": [P, I],
     "ns_batch_tune": [P, I, P],
+    "ns_vel_bc_staggered_adj": [P, P, P, P, P, I, I, I],
+    "ns_vel_bc_staggered_cn2_adj": [P, P, P, P, P, I, I],
+    "ns_adi_sweep_adj": [P, P, P, P, I],
+    "ns_divergence_adj": [P, P, D, P, P, P],
+    "ns_pr_bc_staggered_modify_rhs_adj": [P, P, P],
+    "ns_poisson_fft_adj": [P, P, P],
+    "ns_dct_poisson_solver_adj": [P, P, P],
+    "ns_pr_bc_staggered_adj": [P, P, P],
+    "ns_project_adj": [P, P, P, P, P, D, I, P, P, P, P],
     "ns_history_save": [P],
     "ns_history_restore": [P],
     "ns_cal_conv": [P, P, P, P, I, I, I, D, D, D, P, P, P],

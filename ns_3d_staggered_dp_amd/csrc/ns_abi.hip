@@ -374,6 +374,7 @@ struct ArgList {
         v.emplace_back();
         return v.back();
     }
+    Arg &back() { return v.back(); }
 };
 
 bool valid_cfg(const ns_config *f) {
@@ -1513,6 +1514,199 @@ int ns_project(ns_ctx *c, const double *u_star, const double *v_star, const doub
     launch_project(g, f.dx, f.dy, f.dz, dt, dp_flag, a1.dev, a2.dev, a3.dev, a4.dev, nullptr, o1.dev, o2.dev, o3.dev,
                    o4.dev, c->st);
     if ((rc = arg_out(c, o1)) || (rc = arg_out(c, o2)) || (rc = arg_out(c, o3)) || (rc = arg_out(c, o4))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+// ---- adjoint mirrors: the transpose of each linear mirror above ----
+
+namespace {
+// the plane-set adjoint staged in and out (added to)
+int stage_planes_b(ns_ctx *c, ArgList &A, double *planes_b, double **dev) {
+    Arg &ap = A.add();
+    int rc = arg_in(c, ap, planes_b, (size_t)c->L.total, true);
+    *dev = ap.dev;
+    return rc;
+}
+}  // namespace
+
+int ns_vel_bc_staggered_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, double *planes_b, int bc_x, int bc_y,
+                            int bc_z) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    int bc[3] = {bc_x, bc_y, bc_z};
+    for (int a = 0; a < 3; a++) {
+        if (bc[a] < 1 || bc[a] > 4) return NS_ERR_INVALID;
+        if (bc[a] != 1 && bc[a] != 3 && !planes_b) return NS_ERR_INVALID;
+    }
+    const double h[3] = {f.dx, f.dy, f.dz};
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_b, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_b, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_b, g.nw(), true))) return rc;
+    double *pb = nullptr;
+    if (planes_b && (rc = stage_planes_b(c, A, planes_b, &pb))) return rc;
+    double *const a3[3] = {au.dev, av.dev, aw.dev};
+    launch_adj_fill(g, 0, bc, h, a3, pb, c->st);
+    if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw))) return rc;
+    if (planes_b && (rc = arg_out(c, A.back()))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_vel_bc_staggered_cn2_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, double *planes_b, int bc,
+                                int dir) {
+    if (!c || dir < 1 || dir > 3 || bc < 1 || bc > 4) return NS_ERR_INVALID;
+    if (bc == 1) return NS_ERR_UNSUPPORTED;  // periodic rows: not transposed
+    if (bc == 3) return NS_OK;               // the forward leaves the lines as they are
+    if (!planes_b) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    ArgList A;
+    int rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_b, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_b, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_b, g.nw(), true))) return rc;
+    double *pb = nullptr;
+    if ((rc = stage_planes_b(c, A, planes_b, &pb))) return rc;
+    double *const a3[3] = {au.dev, av.dev, aw.dev};
+    launch_adj_cn2(g, dir - 1, a3, pb, c->st);
+    if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw)) || (rc = arg_out(c, A.back())))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_adi_sweep_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, int dir) {
+    if (!c || dir < 1 || dir > 3) return NS_ERR_INVALID;
+    if (c->cfg.scheme != NS_AB2_CN) return NS_ERR_UNSUPPORTED;
+    for (int a = 0; a < 3; a++)
+        if (c->cfg.bc[a] != 2 && c->cfg.bc[a] != 4) return NS_ERR_UNSUPPORTED;
+    int rc = adj_cn_systems(c);
+    if (rc) return rc;
+    const Grid &g = c->g;
+    ArgList A;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_b, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_b, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_b, g.nw(), true))) return rc;
+    launch_adi_sweep(g, dir, c->cnT[dir - 1][0], c->cnT[dir - 1][1], au.dev, av.dev, aw.dev, c->st);
+    if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_divergence_adj(ns_ctx *c, const double *rhs_b, double dt, double *u_b, double *v_b, double *w_b) {
+    if (!c || !rhs_b) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    const double h[3] = {f.dx, f.dy, f.dz};
+    ArgList A;
+    int rc;
+    Arg &ar = A.add(); if ((rc = arg_in(c, ar, rhs_b, g.ni(), true))) return rc;
+    Arg &au = A.add(); if ((rc = arg_in(c, au, u_b, g.nu(), true))) return rc;
+    Arg &av = A.add(); if ((rc = arg_in(c, av, v_b, g.nv(), true))) return rc;
+    Arg &aw = A.add(); if ((rc = arg_in(c, aw, w_b, g.nw(), true))) return rc;
+    double *const a3[3] = {au.dev, av.dev, aw.dev};
+    launch_adj_div(g, h, dt, ar.dev, a3, c->st);
+    if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_pr_bc_staggered_modify_rhs_adj(ns_ctx *c, const double *rhs_b, double *planes_b) {
+    if (!c || !rhs_b || !planes_b) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    const double h[3] = {f.dx, f.dy, f.dz};
+    ArgList A;
+    int rc;
+    Arg &ar = A.add(); if ((rc = arg_in(c, ar, rhs_b, g.ni(), true))) return rc;
+    double *pb = nullptr;
+    if ((rc = stage_planes_b(c, A, planes_b, &pb))) return rc;
+    launch_adj_modify_rhs(g, f.pbc, h, ar.dev, pb, c->st);
+    if ((rc = arg_out(c, A.back()))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_poisson_fft_adj(ns_ctx *c, const double *sol_b, double *rhs_b) {
+    // the periodic solve (modal division, zero mode to 0) is symmetric
+    return ns_poisson_fft(c, sol_b, rhs_b);
+}
+
+int ns_dct_poisson_solver_adj(ns_ctx *c, double *rhs_b, double *mean_dp_b) {
+    if (!c || !rhs_b) return NS_ERR_INVALID;
+    if (!c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    if (!c->mpartial) {
+        int rc = dalloc(c, &c->mpartial, (size_t)2048 * NS_METRIC_PARTS);
+        if (rc) return rc;
+    }
+    ArgList A;
+    int rc;
+    Arg &ar = A.add(); if ((rc = arg_in(c, ar, rhs_b, g.ni(), true))) return rc;
+    // mean_dp sets the zero mode (every pressure code Neumann): its adjoint is the sum of the input
+    const bool zero_mode = f.pbc[0] == 3 && f.pbc[1] == 3 && f.pbc[2] == 3;
+    double sum = 0.0;
+    if (zero_mode && mean_dp_b) {
+        const int nbs = launch_block_sums(ar.dev, g.ni(), c->mpartial, 2048, c->st);
+        HIPCHK(hipMemsetAsync(c->mpartial + 2048, 0, sizeof(double), c->st));
+        launch_add_partials(c->mpartial + 2048, 0, c->mpartial, nbs, 1.0, c->st);
+        HIPCHK(hipMemcpyAsync(&sum, c->mpartial + 2048, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    }
+    // the solve with the zero mode dropped is symmetric
+    poisson_dct_solve(g, c->pp, f.dx, f.dy, f.dz, f.dt, nullptr, nullptr, nullptr, ar.dev, nullptr, nullptr, 0.0,
+                      c->work, nullptr, ar.dev, c->st);
+    if ((rc = arg_out(c, ar))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (mean_dp_b) *mean_dp_b += sum;
+    return NS_OK;
+}
+
+int ns_pr_bc_staggered_adj(ns_ctx *c, double *dp_b, double *planes_b) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    const bool need = f.pbc[0] != 1 || f.pbc[1] != 1 || f.pbc[2] != 1;
+    if (need && !planes_b) return NS_ERR_INVALID;
+    const double h[3] = {f.dx, f.dy, f.dz};
+    ArgList A;
+    int rc;
+    Arg &ad = A.add(); if ((rc = arg_in(c, ad, dp_b, g.np(), true))) return rc;
+    double *pb = nullptr;
+    if (planes_b && (rc = stage_planes_b(c, A, planes_b, &pb))) return rc;
+    double *const a3[3] = {ad.dev, ad.dev, ad.dev};
+    launch_adj_fill(g, 3, f.pbc, h, a3, pb, c->st);
+    if ((rc = arg_out(c, ad))) return rc;
+    if (planes_b && (rc = arg_out(c, A.back()))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
+int ns_project_adj(ns_ctx *c, const double *u_b, const double *v_b, const double *w_b, double *p_b, double dt,
+                   int dp_flag, double *us_b, double *vs_b, double *ws_b, double *dp_b) {
+    if (!c) return NS_ERR_INVALID;
+    const Grid &g = c->g;
+    const ns_config &f = c->cfg;
+    const double h[3] = {f.dx, f.dy, f.dz};
+    ArgList A;
+    int rc;
+    Arg &a1 = A.add(); if ((rc = arg_in(c, a1, u_b, g.nu(), true))) return rc;
+    Arg &a2 = A.add(); if ((rc = arg_in(c, a2, v_b, g.nv(), true))) return rc;
+    Arg &a3 = A.add(); if ((rc = arg_in(c, a3, w_b, g.nw(), true))) return rc;
+    Arg &a4 = A.add(); if ((rc = arg_in(c, a4, p_b, g.np(), true))) return rc;
+    Arg &o1 = A.add(); if ((rc = arg_in(c, o1, us_b, g.nu(), false))) return rc;
+    Arg &o2 = A.add(); if ((rc = arg_in(c, o2, vs_b, g.nv(), false))) return rc;
+    Arg &o3 = A.add(); if ((rc = arg_in(c, o3, ws_b, g.nw(), false))) return rc;
+    Arg &o4 = A.add(); if ((rc = arg_in(c, o4, dp_b, g.np(), false))) return rc;
+    const double *ub[3] = {a1.dev, a2.dev, a3.dev};
+    double *const usb[3] = {o1.dev, o2.dev, o3.dev};
+    launch_adj_project(g, dt, h, dp_flag, ub, usb, a4.dev, o4.dev, c->st);
+    if ((rc = arg_out(c, a4)) || (rc = arg_out(c, o1)) || (rc = arg_out(c, o2)) || (rc = arg_out(c, o3)) ||
+        (rc = arg_out(c, o4)))
+        return rc;
     HIPCHK(hipStreamSynchronize(c->st));
     return NS_OK;
 }

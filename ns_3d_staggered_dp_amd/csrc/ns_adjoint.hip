@@ -571,8 +571,8 @@ void launch_adj_take_interior(const Grid &g, const double *dpb, double *r, hipSt
     hipLaunchKernelGGL(k_adj_take_interior, dim3(ablocks(g.ni())), dim3(AT), 0, st, g, dpb, r);
 }
 
-void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
-                        double *const usb[3], double *pbar, hipStream_t st) {
+void launch_adj_modify_rhs(const Grid &g, const int pbc[3], const double h[3], const double *rb, double *pbar,
+                           hipStream_t st) {
     AdjRhs R;
     R.g = g;
     R.L = plane_layout(g.nx, g.ny, g.nz);
@@ -587,9 +587,19 @@ void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], doub
         const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
         hipLaunchKernelGGL(k_adj_modify_rhs, dim3(ablocks((size_t)n[o1] * n[o2])), dim3(AT), 0, st, R);
     }
+}
+
+void launch_adj_div(const Grid &g, const double h[3], double dt, const double *rb, double *const usb[3],
+                    hipStream_t st) {
     const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
     for (int c = 0; c < 3; c++)
         hipLaunchKernelGGL(k_adj_div, dim3(ablocks(tot[c])), dim3(AT), 0, st, g, c, 1.0 / (h[c] * dt), rb, usb[c]);
+}
+
+void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
+                        double *const usb[3], double *pbar, hipStream_t st) {
+    launch_adj_modify_rhs(g, pbc, h, rb, pbar, st);
+    launch_adj_div(g, h, dt, rb, usb, st);
 }
 
 namespace {

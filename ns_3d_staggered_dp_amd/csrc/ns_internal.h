@@ -240,6 +240,11 @@ void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3]
 void launch_adj_take_interior(const Grid &g, const double *dpb, double *r, hipStream_t st);
 void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
                         double *const usb[3], double *pbar, hipStream_t st);
+// its two parts: boundary RHS terms (plane adjoints added to pbar) and divergence (added to usb)
+void launch_adj_modify_rhs(const Grid &g, const int pbc[3], const double h[3], const double *rb, double *pbar,
+                           hipStream_t st);
+void launch_adj_div(const Grid &g, const double h[3], double dt, const double *rb, double *const usb[3],
+                    hipStream_t st);
 // gb / gcb: work arrays for the adjoints of the diffusive and convective parts
 // of the right-hand side (the same arrays for the explicit schemes)
 void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, int dp_flag, const double h[3],
