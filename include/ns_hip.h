@@ -172,10 +172,10 @@ int ns_extract_subbox(ns_ctx *ctx, const ns_subbox *sb, int field, double *dst);
 
 /* ---- discrete adjoint of ns_step (no counterpart in the reference: the
  * gradient side of the re-simulation) ----
- * Euler and AB2 on any codes, and AB2-CN when every velocity code is a wall
- * (2 or 4: the ADI sweeps' adjoints are solves with the transposed
- * tridiagonal systems), on contexts that project every step (dp_flag = 1,
- * or NS_FLAG_EAGER_PROJECTION); NS_ERR_UNSUPPORTED otherwise.  The
+ * Euler, AB2 and AB2-CN on every code the forward step takes (the ADI
+ * sweeps' adjoints are solves with the transposed tridiagonal or reduced
+ * cyclic systems), on contexts that project every step (dp_flag = 1, or
+ * NS_FLAG_EAGER_PROJECTION); NS_ERR_UNSUPPORTED otherwise.  The
  * adjoint state is (u_b, v_b, w_b, p_b, hx_b, hy_b, hz_b), the gradients of a
  * scalar objective with respect to the state (u, v, w, p, rhs_*_previous)
  * at the current point of the reverse sweep, all in the ghosted shapes. */
@@ -277,10 +277,10 @@ int ns_project(ns_ctx *ctx, const double *u_star, const double *v_star, const do
 /* (u_b, v_b, w_b) := F_u^T (u_b, v_b, w_b), planes_b += F_p^T (...) */
 int ns_vel_bc_staggered_adj(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, double *planes_b, int bc_x,
                             int bc_y, int bc_z);
-/* walls (bc 2, 4; bc 3 is the identity, bc 1 NS_ERR_UNSUPPORTED) */
+/* bc 2, 4: the ends' adjoints move to planes_b; bc 1: ghost rows' adjoints zeroed; bc 3: identity */
 int ns_vel_bc_staggered_cn2_adj(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, double *planes_b, int bc,
                                 int dir);
-/* solve with the transposed CN systems (AB2-CN contexts with wall codes) */
+/* solve with the transposed CN systems (AB2-CN contexts) */
 int ns_adi_sweep_adj(ns_ctx *ctx, double *u_b, double *v_b, double *w_b, int dir);
 /* (u_b, v_b, w_b) += div^T rhs_b */
 int ns_divergence_adj(ns_ctx *ctx, const double *rhs_b, double dt, double *u_b, double *v_b, double *w_b);

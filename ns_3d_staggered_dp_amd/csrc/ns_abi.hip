@@ -203,8 +203,19 @@ std::vector<double> thomas_solve(const std::vector<double> &lo, const std::vecto
     return r;
 }
 
+// device copy of a coefficient table: into the table the system held before
+// (a rebuild for a new dt: the line lengths of a context never change, and the
+// caller has drained the stream) or into a new one
+template <typename T>
+int dstore(ns_ctx *c, const T *old, T **p, const std::vector<T> &h) {
+    if (!old) return dupload(c, p, h);
+    *p = const_cast<T *>(old);
+    HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return NS_OK;
+}
+
 // CN line system for direction ax (0..2) and type A (face-normal, n+1) or B (n+2);
-// transpose: the system of A^T (walls only), whose solve is the adjoint of the sweep
+// transpose: the system of A^T, whose solve is the adjoint of the sweep
 int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = false) {
     const ns_config &f = c->cfg;
     const int n = (ax == 0) ? f.nx : (ax == 1) ? f.ny : f.nz;
@@ -214,12 +225,12 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
     // mat_CN entries (NS_functions.f90:1261-1269)
     const double dg = 1.0 + 0.5 * dt * nu * 2.0 * idx2;
     const double a = 0.5 * dt * nu * 1.0 * idx2;
+    const LineSys old = S;
     S = LineSys{};
     S.a = a;
     int rc0 = NS_OK;
     std::vector<double> lo, d, up, inv, cp;
     if (bc == 1) {
-        if (transpose) return NS_ERR_UNSUPPORTED;
         const int m = n;
         S.periodic = 1;
         S.m = m;
@@ -236,6 +247,11 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
             d[0] = dg - a;  // (1+2a)u1 - a u_N with u_N = u_1 + r_N
             alpha = 0.0;
         }
+        // the transposed system exchanges the corners (the band is symmetric)
+        if (transpose) {
+            std::swap(alpha, beta);
+            S.adj = 1;
+        }
         double gamma = -d[0];
         std::vector<double> dt_ = d;
         dt_[0] = d[0] - gamma;
@@ -248,7 +264,7 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         S.s = alpha / gamma;
         S.den = 1.0 + z[0] + S.s * z[m - 1];
         double *dz_;
-        int rc = dupload(c, &dz_, z);
+        int rc = dstore(c, old.z, &dz_, z);
         if (rc) return rc;
         S.z = dz_;
     } else if (bc == 2 || bc == 4) {
@@ -295,7 +311,8 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         // cp_{tk-1} (top-down) and cq_{tk+1} = cpR[N-2-tk] (bottom-up)
         S.gk = 1.0 / (d[tk] - lo[tk] * cp[tk - 1] - up[tk] * cpR[N - 2 - tk]);
         double *dlr, *dir_, *dcr;
-        if ((rc0 = dupload(c, &dlr, loR)) || (rc0 = dupload(c, &dir_, invR)) || (rc0 = dupload(c, &dcr, cpR)))
+        if ((rc0 = dstore(c, old.loR, &dlr, loR)) || (rc0 = dstore(c, old.invR, &dir_, invR)) ||
+            (rc0 = dstore(c, old.cpR, &dcr, cpR)))
             return rc0;
         S.loR = dlr;
         S.invR = dir_;
@@ -304,11 +321,11 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         return NS_ERR_UNSUPPORTED;
     }
     double *dl, *di, *dc;
-    int rc = dupload(c, &dl, lo);
+    int rc = dstore(c, old.lo, &dl, lo);
     if (rc) return rc;
-    rc = dupload(c, &di, inv);
+    rc = dstore(c, old.inv, &di, inv);
     if (rc) return rc;
-    rc = dupload(c, &dc, cp);
+    rc = dstore(c, old.cp, &dc, cp);
     if (rc) return rc;
     S.lo = dl;
     S.inv = di;
@@ -940,6 +957,8 @@ int ns_set_dt(ns_ctx *c, double dt) {
     }
     c->cfg.dt = dt;
     c->cnT_ok = 0;
+    // the systems are rebuilt in the tables queued sweeps read
+    HIPCHK(hipStreamSynchronize(c->st));
     if (c->cfg.scheme == NS_AB2_CN)
         for (int ax = 0; ax < 3; ax++)
             for (int t = 0; t < 2; t++) {
@@ -1004,17 +1023,12 @@ static int adj_alloc(ns_ctx *c) {
 
 // explicit schemes on any codes; AB2-CN with wall velocity codes (2, 4) on
 // every axis, whose ADI systems are plain tridiagonal
-static bool adj_supported(const ns_ctx *c) {
-    if (c->lazy || !c->poisson_ok) return false;
-    if (c->cfg.scheme == NS_AB2_CN)
-        for (int a = 0; a < 3; a++)
-            if (c->cfg.bc[a] != 2 && c->cfg.bc[a] != 4) return false;
-    return true;
-}
+static bool adj_supported(const ns_ctx *c) { return !c->lazy && c->poisson_ok; }
 
 // transposed CN systems for the current dt
 static int adj_cn_systems(ns_ctx *c) {
     if (c->cfg.scheme != NS_AB2_CN || c->cnT_ok) return NS_OK;
+    HIPCHK(hipStreamSynchronize(c->st));  // a rebuild rewrites tables earlier sweeps read
     for (int ax = 0; ax < 3; ax++)
         for (int t = 0; t < 2; t++) {
             const int rc = make_line_sys(c, ax, t, c->cnT[ax][t], true);
@@ -1102,7 +1116,7 @@ int ns_adjoint_step(ns_ctx *c, const double *u, const double *v, const double *w
         for (int dir = 3; dir >= 1; dir--) {
             launch_adi_sweep(g, dir, c->cnT[dir - 1][0], c->cnT[dir - 1][1], c->ausb[0], c->ausb[1], c->ausb[2],
                              c->st);
-            launch_adj_cn2(g, dir - 1, c->ausb, c->apbar, c->st);
+            launch_adj_cn2(g, dir - 1, f.bc[dir - 1], c->ausb, c->apbar, c->st);
         }
     }
     // S1^T: predictor
@@ -1559,8 +1573,7 @@ int ns_vel_bc_staggered_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, do
 int ns_vel_bc_staggered_cn2_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, double *planes_b, int bc,
                                 int dir) {
     if (!c || dir < 1 || dir > 3 || bc < 1 || bc > 4) return NS_ERR_INVALID;
-    if (bc == 1) return NS_ERR_UNSUPPORTED;  // periodic rows: not transposed
-    if (bc == 3) return NS_OK;               // the forward leaves the lines as they are
+    if (bc == 3) return NS_OK;  // the forward leaves the lines as they are
     if (!planes_b) return NS_ERR_INVALID;
     const Grid &g = c->g;
     ArgList A;
@@ -1571,7 +1584,7 @@ int ns_vel_bc_staggered_cn2_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b
     double *pb = nullptr;
     if ((rc = stage_planes_b(c, A, planes_b, &pb))) return rc;
     double *const a3[3] = {au.dev, av.dev, aw.dev};
-    launch_adj_cn2(g, dir - 1, a3, pb, c->st);
+    launch_adj_cn2(g, dir - 1, bc, a3, pb, c->st);
     if ((rc = arg_out(c, au)) || (rc = arg_out(c, av)) || (rc = arg_out(c, aw)) || (rc = arg_out(c, A.back())))
         return rc;
     HIPCHK(hipStreamSynchronize(c->st));
@@ -1581,8 +1594,6 @@ int ns_vel_bc_staggered_cn2_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b
 int ns_adi_sweep_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, int dir) {
     if (!c || dir < 1 || dir > 3) return NS_ERR_INVALID;
     if (c->cfg.scheme != NS_AB2_CN) return NS_ERR_UNSUPPORTED;
-    for (int a = 0; a < 3; a++)
-        if (c->cfg.bc[a] != 2 && c->cfg.bc[a] != 4) return NS_ERR_UNSUPPORTED;
     int rc = adj_cn_systems(c);
     if (rc) return rc;
     const Grid &g = c->g;

@@ -109,13 +109,25 @@ __device__ __forceinline__ void solve_line(double *x, long s, const LineSys &S) 
     double g_lo = 0.0, g_hi = 0.0;  // r of the leading / trailing ghost rows
     if (S.off == 0) {
         g_hi = x[s * m];  // A-type: row N (u_N - u_1 = r_N)
-        r_first = y[0] + (S.quirk ? S.a * g_hi : 0.0);
-        r_last = y[s * (m - 1)] + S.a * g_hi;
+        if (S.adj) {
+            // transpose of "u_N = u_1 + r_N": the ghost's adjoint joins row 0's
+            r_first = y[0] + g_hi;
+            r_last = y[s * (m - 1)];
+        } else {
+            r_first = y[0] + (S.quirk ? S.a * g_hi : 0.0);
+            r_last = y[s * (m - 1)] + S.a * g_hi;
+        }
     } else {
         g_lo = x[0];
         g_hi = x[s * (m + 1)];
-        r_first = y[0] + S.a * g_lo;
-        r_last = y[s * (m - 1)] + S.a * g_hi;
+        if (S.adj) {
+            // transpose of "v_1 = v_{n+1} + r_1, v_{n+2} = v_2 + r_{n+2}"
+            r_first = y[0] + g_hi;
+            r_last = y[s * (m - 1)] + g_lo;
+        } else {
+            r_first = y[0] + S.a * g_lo;
+            r_last = y[s * (m - 1)] + S.a * g_hi;
+        }
     }
     // Thomas on the reduced system (rows 0 and m-1 carry the folded ghosts)
     const double ylast = thomas_fwd<CHK>(y, s, y, s, m, S.lo, S.inv, true, r_first, r_last);
@@ -139,7 +151,15 @@ __device__ __forceinline__ void solve_line(double *x, long s, const LineSys &S) 
             }
         }
     }
-    if (S.off == 0) {
+    if (S.adj) {
+        // adjoints of the ghost-row right-hand sides: the end rows they were folded into
+        if (S.off == 0) {
+            x[s * m] = ((S.quirk ? S.a * x0 : 0.0) + S.a * xl) + g_hi;
+        } else {
+            x[0] = S.a * x0 + g_lo;
+            x[s * (m + 1)] = S.a * xl + g_hi;
+        }
+    } else if (S.off == 0) {
         x[s * m] = x0 + g_hi;
     } else {
         x[0] = xl + g_lo;

@@ -604,12 +604,13 @@ void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], doub
 
 namespace {
 
-// transpose of vel_bc_staggered_CN2 for walls along axis ax
-// (NS_functions.f90:313-410): the forward sets both line ends of every
-// component to its plane values, so their adjoints move to the planes and the
-// ends' adjoints become zero (one thread per line and component)
-__global__ void __launch_bounds__(AT) k_adj_cn2(Grid g, int ax, PlaneLayout L, double *a0, double *a1, double *a2,
-                                              double *pbar) {
+// transpose of vel_bc_staggered_CN2 along axis ax (NS_functions.f90:313-410):
+// on walls the forward sets both line ends of every component to its plane
+// values, so their adjoints move to the planes and the ends' adjoints become
+// zero; on periodic lines it sets the ghost rows to zero, so their adjoints
+// become zero (one thread per line and component)
+__global__ void __launch_bounds__(AT) k_adj_cn2(Grid g, int ax, int bc, PlaneLayout L, double *a0, double *a1,
+                                              double *a2, double *pbar) {
     const int c = blockIdx.y;
     int n[3];
     dims_of(g, c, n);
@@ -624,6 +625,12 @@ __global__ void __launch_bounds__(AT) k_adj_cn2(Grid g, int ax, PlaneLayout L, d
     const size_t q1 = FIX(id[0], id[1], id[2], n[0], n[1]);
     id[ax] = n[ax];
     const size_t qn = FIX(id[0], id[1], id[2], n[0], n[1]);
+    if (bc == 1) {
+        // the forward zeroes the ghost rows (row N; row 1 too for the tangential components)
+        if (c != ax) A[q1] = 0.0;
+        A[qn] = 0.0;
+        return;
+    }
     pbar[L.off[6 * c + 2 * ax] + t] += A[q1];
     pbar[L.off[6 * c + 2 * ax + 1] + t] += A[qn];
     A[q1] = 0.0;
@@ -632,7 +639,8 @@ __global__ void __launch_bounds__(AT) k_adj_cn2(Grid g, int ax, PlaneLayout L, d
 
 }  // namespace
 
-void launch_adj_cn2(const Grid &g, int ax, double *const a[3], double *pbar, hipStream_t st) {
+void launch_adj_cn2(const Grid &g, int ax, int bc, double *const a[3], double *pbar, hipStream_t st) {
+    if (bc != 1 && bc != 2 && bc != 4) return;  // code 3: the forward leaves the lines as they are
     size_t mx = 0;
     for (int c = 0; c < 3; c++) {
         int n[3];
@@ -641,8 +649,8 @@ void launch_adj_cn2(const Grid &g, int ax, double *const a[3], double *pbar, hip
         const size_t m = (size_t)n[o1] * n[o2];
         mx = m > mx ? m : mx;
     }
-    hipLaunchKernelGGL(k_adj_cn2, dim3(ablocks(mx), 3), dim3(AT), 0, st, g, ax, plane_layout(g.nx, g.ny, g.nz), a[0],
-                       a[1], a[2], pbar);
+    hipLaunchKernelGGL(k_adj_cn2, dim3(ablocks(mx), 3), dim3(AT), 0, st, g, ax, bc, plane_layout(g.nx, g.ny, g.nz),
+                       a[0], a[1], a[2], pbar);
 }
 
 void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, int dp_flag, const double h[3],

@@ -69,6 +69,8 @@ struct LineSys {
     int off;      // index (0-based) of the first unknown in the line
     double a;     // CN off-diagonal magnitude: 0.5*dt*nu/h^2
     int quirk;    // periodic A-type with mat_CN's row 1
+    int adj;      // periodic: the transposed map (ghost rows added into the end rows
+                  // before the solve, ghost outputs formed from the solved ends after it)
     // Thomas coefficients for the m unknowns: d'_i = (r_i - lo_i d'_{i-1}) * inv_i ; x_i = d'_i - cp_i x_{i+1}
     const double *lo, *inv, *cp;
     // Sherman-Morrison correction (periodic only): x -= f * z, f = (y_0 + s*y_{m-1}) / den
@@ -253,7 +255,7 @@ void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, i
                         double *pb, hipStream_t st);
 // transpose of the AB2-CN boundary rows before the sweep along axis ax
 // (vel_bc_staggered_CN2, walls): the line ends' adjoints move to the planes
-void launch_adj_cn2(const Grid &g, int ax, double *const a[3], double *pbar, hipStream_t st);
+void launch_adj_cn2(const Grid &g, int ax, int bc, double *const a[3], double *pbar, hipStream_t st);
 int launch_block_sums(const double *a, size_t n, double *partial, int max_blocks, hipStream_t st);
 void launch_add_at(double *a, size_t q, double v, hipStream_t st);
 // a[q] += scale * sum(partial[0..nb)), on the device

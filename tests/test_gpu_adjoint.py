@@ -12,8 +12,8 @@ import ctypes
 import numpy as np
 import pytest
 
-from ns_3d_staggered_dp_amd import (AB2, AB2_CN, EULER, FLAG_EAGER_PROJECTION, FLAG_GAUGE_CORNER, NSConfig, _abi,
-                                    fields)
+from ns_3d_staggered_dp_amd import (AB2, AB2_CN, EULER, FLAG_CN_CYCLIC, FLAG_EAGER_PROJECTION, FLAG_GAUGE_CORNER,
+                                    NSConfig, _abi, fields)
 from ns_3d_staggered_dp_amd.grid import plane_set_length
 from ns_3d_staggered_dp_amd.solver import NSSolver, _ptr
 
@@ -59,6 +59,13 @@ CASES = [
     ("walls_cn_first", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), True),
     ("walls_cn_mixed_dst", lambda: box(10, 8, 9, bc=(2, 4, 4), pbc=(3, 2, 2), scheme=AB2_CN, dp_flag=1), False),
     ("walls_cn_long_lines", lambda: box(40, 6, 36, bc=(4, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), False),
+    # periodic CN lines: the reference's row-1 coupling (default) and the cyclic operator
+    ("periodic_cn", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=1), False),
+    ("periodic_cn_first", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=1), True),
+    ("periodic_cn_cyclic_gauge", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=0,
+                                                  flags=FLAG_CN_CYCLIC | FLAG_EAGER_PROJECTION | FLAG_GAUGE_CORNER),
+     False),
+    ("cn_periodic_x_walls_yz", lambda: box(12, 10, 8, bc=(1, 2, 4), pbc=(3, 3, 2), scheme=AB2_CN, dp_flag=1), False),
 ]
 
 
@@ -90,9 +97,8 @@ def test_adjoint_step_exact_on_quadratic_map(name, mk, first):
     assert abs(fd - ad) <= 1e-10 * scale, (fd, ad, scale)
 
 
-def test_adjoint_unsupported_for_lazy_and_periodic_cn():
-    for cfg in (periodic(8, 8, 8, scheme=AB2, dp_flag=0), periodic(8, 8, 8, scheme=AB2_CN, dp_flag=1),
-                box(8, 8, 8, bc=(1, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1)):
+def test_adjoint_unsupported_for_lazy_projection():
+    for cfg in (periodic(8, 8, 8, scheme=AB2, dp_flag=0), periodic(8, 8, 8, scheme=AB2_CN, dp_flag=0)):
         with NSSolver(cfg) as s:
             rc = _abi.lib().ns_adjoint_set_state(s.h, None, None, None, None, None, None, None)
             assert rc == _abi.NS_ERR_UNSUPPORTED

@@ -4,13 +4,15 @@ For a linear map F and random x, y the identity <y, F x> = <F^T y, x> holds
 exactly in exact arithmetic; in FP64 both sides agree to rounding, so the bar
 is 1e-12 relative to the sum of the magnitudes of the products.  Every mirror
 is checked on walls (DCT, DST and mixed codes, AB2-CN for the sweeps and the
-CN boundary rows) and, where it applies, on a periodic box."""
+CN boundary rows) and, where it applies, on a periodic box; the sweeps and CN
+rows also on periodic lines (the reduced cyclic solve, with the reference's
+row-1 coupling and with the cyclic operator)."""
 import ctypes
 
 import numpy as np
 import pytest
 
-from ns_3d_staggered_dp_amd import AB2, AB2_CN, NSConfig, _abi
+from ns_3d_staggered_dp_amd import AB2, AB2_CN, FLAG_CN_CYCLIC, NSConfig, _abi
 from ns_3d_staggered_dp_amd.grid import field_shapes, plane_set_length
 from ns_3d_staggered_dp_amd.solver import NSSolver, _ptr
 
@@ -29,6 +31,12 @@ WALLS = [
 ]
 PERIODIC = lambda: NSConfig(nx=16, ny=12, nz=10, dx=2 * np.pi / 16, dy=0.4, dz=0.5, nu=0.02, dt=2e-3, bc=(1, 1, 1),
                             pbc=(1, 1, 1), scheme=AB2, dp_flag=1)
+# AB2-CN sweeps with periodic lines: the reference's row-1 coupling, the cyclic operator, periodic x beside walls
+CN_PERIODIC = [
+    ("periodic_cn", lambda: PERIODIC().replace(scheme=AB2_CN)),
+    ("periodic_cn_cyclic", lambda: PERIODIC().replace(scheme=AB2_CN, flags=FLAG_CN_CYCLIC)),
+    ("cn_periodic_x", lambda: box(12, 10, 8, bc=(1, 2, 4), pbc=(3, 3, 2), scheme=AB2_CN)),
+]
 
 
 def rnd(rng, shape):
@@ -61,7 +69,7 @@ def test_velocity_fill_adjoint(name, mk):
     check(list(zip(y, fx)), list(zip(yb, x)) + [(pb, planes)])
 
 
-@pytest.mark.parametrize("name,mk", WALLS, ids=[c[0] for c in WALLS])
+@pytest.mark.parametrize("name,mk", WALLS + CN_PERIODIC, ids=[c[0] for c in WALLS + CN_PERIODIC])
 @pytest.mark.parametrize("d", [1, 2, 3])
 def test_cn_rows_and_sweep_adjoint(name, mk, d):
     cfg = mk()
@@ -152,12 +160,9 @@ def test_pressure_fill_and_projection_adjoint(name, mk, dp_flag):
 
 
 def test_adjoint_mirrors_reject_unsupported():
-    per_cn = NSConfig(nx=8, ny=8, nz=8, dx=0.3, dy=0.3, dz=0.3, nu=0.05, dt=2e-3, bc=(1, 1, 1), pbc=(1, 1, 1),
-                      scheme=AB2_CN, dp_flag=1)
-    with NSSolver(per_cn) as s:
-        sh = field_shapes(8, 8, 8)
+    # the sweep's adjoint needs the CN systems of an AB2-CN context
+    with NSSolver(PERIODIC()) as s:
+        sh = field_shapes(16, 12, 10)
         a = [np.zeros(sh[k], order="F") for k in ("u", "v", "w")]
         assert _abi.lib().ns_adi_sweep_adj(s.h, *(_ptr(x) for x in a), 1) == _abi.NS_ERR_UNSUPPORTED
-        pb = np.zeros(plane_set_length(8, 8, 8))
-        assert _abi.lib().ns_vel_bc_staggered_cn2_adj(s.h, *(_ptr(x) for x in a), _ptr(pb), 1, 1) == \
-            _abi.NS_ERR_UNSUPPORTED
+        assert _abi.lib().ns_adi_sweep_adj(s.h, *(_ptr(x) for x in a), 4) == _abi.NS_ERR_INVALID
