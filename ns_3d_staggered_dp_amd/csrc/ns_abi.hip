@@ -203,6 +203,96 @@ std::vector<double> thomas_solve(const std::vector<double> &lo, const std::vecto
     return r;
 }
 
+// Partitioned solve of a wall line (N rows) in K blocks of at most
+// PART_ROWS rows, block k = rows [kN/K, (k+1)N/K).  Each block is solved on
+// its own (local Thomas factors loL, invL, cpL, restarted at the block's first
+// row); the couplings to the neighbouring blocks' end values enter through
+// the spikes V (block k's solve of lo_{s_k} e_first) and W (of up_{e_k}
+// e_last): x = y - x_{s_k - 1} V - x_{e_k + 1} W.  The 2K end values solve
+// (I + M) z = ends, M holding the spikes' end entries; G = (I + M)^{-1}.
+// Every table depends only on the matrix, which is the same for every line.
+// Returns K (0: the line is too long for PART_MAX_BLOCKS blocks) and fills
+// tab = loL | invL | cpL | V | W (N each) | G (2K x 2K, row-major).
+int part_tables(const std::vector<double> &lo, const std::vector<double> &d, const std::vector<double> &up,
+                std::vector<double> &tab) {
+    const int N = (int)d.size();
+    const int K = (N + PART_ROWS - 1) / PART_ROWS;
+    if (K > PART_MAX_BLOCKS || N < 2 * K) return 0;
+    const int K2 = 2 * K;
+    tab.assign((size_t)5 * N + (size_t)K2 * K2, 0.0);
+    double *loL = tab.data(), *invL = loL + N, *cpL = invL + N, *V = cpL + N, *W = V + N, *G = W + N;
+    auto start = [&](int k) { return (k * N) / K; };
+    for (int k = 0; k < K; k++)
+        for (int i = start(k); i < start(k + 1); i++) {
+            const bool first = (i == start(k));
+            const double den = first ? d[i] : d[i] - lo[i] * cpL[i - 1];
+            invL[i] = 1.0 / den;
+            cpL[i] = up[i] * invL[i];
+            loL[i] = first ? 0.0 : lo[i];
+        }
+    // local solve of block k on rhs e (rows of the block)
+    auto lsolve = [&](int k, std::vector<double> r) {
+        const int a = start(k), L = start(k + 1) - a;
+        double dd = 0.0;
+        for (int j = 0; j < L; j++) r[j] = dd = (r[j] - loL[a + j] * dd) * invL[a + j];
+        for (int j = L - 2; j >= 0; j--) r[j] = r[j] - cpL[a + j] * r[j + 1];
+        return r;
+    };
+    std::vector<double> M((size_t)K2 * K2, 0.0);
+    for (int k = 0; k < K; k++) {
+        const int a = start(k), b = start(k + 1), L = b - a;
+        if (k > 0) {
+            std::vector<double> e(L, 0.0);
+            e[0] = lo[a];
+            e = lsolve(k, e);
+            for (int j = 0; j < L; j++) V[a + j] = e[j];
+            M[(size_t)(2 * k) * K2 + 2 * k - 1] += e[0];
+            M[(size_t)(2 * k + 1) * K2 + 2 * k - 1] += e[L - 1];
+        }
+        if (k < K - 1) {
+            std::vector<double> e(L, 0.0);
+            e[L - 1] = up[b - 1];
+            e = lsolve(k, e);
+            for (int j = 0; j < L; j++) W[a + j] = e[j];
+            M[(size_t)(2 * k) * K2 + 2 * k + 2] += e[0];
+            M[(size_t)(2 * k + 1) * K2 + 2 * k + 2] += e[L - 1];
+        }
+    }
+    // G = (I + M)^{-1} by Gauss-Jordan with partial pivoting
+    std::vector<double> A((size_t)K2 * K2);
+    for (int i = 0; i < K2; i++)
+        for (int j = 0; j < K2; j++) {
+            A[(size_t)i * K2 + j] = M[(size_t)i * K2 + j] + (i == j ? 1.0 : 0.0);
+            G[(size_t)i * K2 + j] = (i == j) ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < K2; c++) {
+        int pr = c;
+        for (int r = c + 1; r < K2; r++)
+            if (fabs(A[(size_t)r * K2 + c]) > fabs(A[(size_t)pr * K2 + c])) pr = r;
+        if (A[(size_t)pr * K2 + c] == 0.0) return 0;
+        if (pr != c)
+            for (int j = 0; j < K2; j++) {
+                std::swap(A[(size_t)pr * K2 + j], A[(size_t)c * K2 + j]);
+                std::swap(G[(size_t)pr * K2 + j], G[(size_t)c * K2 + j]);
+            }
+        const double piv = 1.0 / A[(size_t)c * K2 + c];
+        for (int j = 0; j < K2; j++) {
+            A[(size_t)c * K2 + j] *= piv;
+            G[(size_t)c * K2 + j] *= piv;
+        }
+        for (int r = 0; r < K2; r++) {
+            if (r == c) continue;
+            const double f = A[(size_t)r * K2 + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < K2; j++) {
+                A[(size_t)r * K2 + j] -= f * A[(size_t)c * K2 + j];
+                G[(size_t)r * K2 + j] -= f * G[(size_t)c * K2 + j];
+            }
+        }
+    }
+    return K;
+}
+
 // device copy of a coefficient table: into the table the system held before
 // (a rebuild for a new dt: the line lengths of a context never change, and the
 // caller has drained the stream) or into a new one
@@ -295,6 +385,15 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         }
         thomas_factor(lo, d, up, inv, cp);
         S.z = nullptr;
+        // partitioned solve (lines of at most PART_MAX_BLOCKS * PART_ROWS rows)
+        std::vector<double> tab;
+        S.K = part_tables(lo, d, up, tab);
+        if (S.K) {
+            double *dt_;
+            // the table length depends only on N, which a rebuild keeps
+            if ((rc0 = dstore(c, old.pt, &dt_, tab))) return rc0;
+            S.pt = dt_;
+        }
         // twisted factorization at tk = N/2: the bottom-up elimination is Thomas
         // on the reversed system (sub- and super-diagonal exchanged)
         const int tk = N / 2;
@@ -1791,6 +1890,10 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     long cols = 0;
     if (kind == 0)
         for (ns_ctx *c : b->ctx) cols += predict_tile_columns(c->cfg);
+    // sweeps: one launch shape for all boxes, the partitioned solve only if every box has it
+    bool part = true;
+    if (kind == 6)
+        for (ns_ctx *c : b->ctx) part = part && sweep_partitioned(c->cn[arg][0], c->cn[arg][1]);
     for (int i = 0; i < n; i++) {
         ns_ctx *c = b->ctx[i];
         const ns_config &f = c->cfg;
@@ -1817,8 +1920,10 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         } else if (kind == 6) {
             size_t l = 0;
             nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l,
-                                   b->knobs[b->cls[i]].adix_lsh);
+                                   b->knobs[b->cls[i]].adix_lsh, part);
             lds = l > lds ? l : lds;
+            const int kw = sweep_params_waves(dst);
+            variant = kw > variant ? kw : variant;
         } else if (kind == 3) {
             nb = make_pr_bc_params(g, f.pbc, c->planes, d, c->dp, arg, dst);
         } else {
@@ -1873,7 +1978,7 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
             case 2: launch_dct_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             case 3: launch_pr_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 5: launch_cn2_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
-            case 6: launch_adi_batch(ph.arg, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
+            case 6: launch_adi_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
         }
     }

@@ -173,7 +173,172 @@ struct SweepParams {
     LineSys sys[3];  // per component (A for the normal one, B otherwise)
     double *a[3];
     int lsh, coff;   // x sweep: log2 lines per block, LDS offset of the coefficients
+                     // (partitioned: of the block-end values)
+    int kw;          // partitioned wall solve: waves per block (0: twisted / cyclic path)
 };
+
+// ---- partitioned wall solve (LineSys::K blocks per line, wave k = block k) ----
+//
+// A block of kw waves takes 64 lines (lane = line); wave k loads rows
+// [s_k, s_{k+1}) of its lane's line into registers, solves them locally,
+// publishes the two end values, and after one barrier forms the couplings
+// P = x_{s_k - 1}, Q = x_{e_k + 1} from the 2K end values of the line (rows
+// of G) and writes x = y - P V - Q W.  Every table address is uniform over the
+// wave (the block index k is), so the coefficient loads are scalar loads, and
+// a lane holds at most PART_ROWS rows, so no LDS bounds the resident waves.
+
+// local solve of one block in registers, y[0..R); returns y[R-1] (the last
+// forward result, which back substitution keeps)
+__device__ __forceinline__ double part_local(double (&y)[PART_ROWS], int R, const double *loL, const double *invL,
+                                             const double *cpL) {
+    double d = 0.0;
+#pragma unroll
+    for (int j = 0; j < PART_ROWS; j++)
+        if (j < R) {
+            d = (y[j] - loL[j] * d) * invL[j];
+            y[j] = d;
+        }
+#pragma unroll
+    for (int j = PART_ROWS - 2; j >= 0; j--)
+        if (j + 1 < R) y[j] = y[j] - cpL[j] * y[j + 1];
+    return d;
+}
+
+// couplings of block k from the line's 2K end values (ends[j * 64 + lane])
+__device__ __forceinline__ void part_couple(int k, int K, const double *G, const double *ends, int lane, double &Pk,
+                                            double &Qk) {
+    const int K2 = 2 * K;
+    const double *gp = G + (size_t)(k > 0 ? 2 * k - 1 : 0) * K2;
+    const double *gq = G + (size_t)(k < K - 1 ? 2 * k + 2 : 0) * K2;
+    Pk = 0.0;
+    Qk = 0.0;
+    for (int j = 0; j < K2; j++) {
+        const double e = ends[j * 64 + lane];
+        Pk = Pk + gp[j] * e;
+        Qk = Qk + gq[j] * e;
+    }
+    if (k == 0) Pk = 0.0;
+    if (k == K - 1) Qk = 0.0;
+}
+
+// y / z sweeps: lane = line (consecutive x), so every row load of a wave is
+// 64 consecutive doubles
+template <int AX>
+__device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, const int blk) {
+    extern __shared__ double sm[];
+    const int c = blockIdx.y;
+    int n[3];
+    comp_dims(P.g, c, n);
+    const int o2 = (AX == 2) ? 1 : 2;
+    const int stride = (AX == 1) ? n[0] : n[0] * n[1];
+    const LineSys &S = P.sys[c];
+    const int K = S.K, N = S.N;
+    const unsigned nl = (unsigned)n[0] * (unsigned)n[o2];
+    if ((unsigned)blk * 64u >= nl) return;  // the whole block: this component has fewer lines
+    const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const unsigned t = (unsigned)blk * 64u + lane;
+    const bool valid = t < nl;
+    const bool act = k < K;
+    const int s0 = act ? (k * N) / K : 0;
+    const int R = act ? ((k + 1) * N) / K - s0 : 0;
+    int id[3] = {1, 1, 1};
+    if (valid) {
+        const unsigned q = t / (unsigned)n[0];
+        id[0] = (int)(t - q * (unsigned)n[0]) + 1;
+        id[o2] = (int)q + 1;
+    }
+    double *x = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]) + (size_t)stride * s0;
+    const double *tb = S.pt;
+    double y[PART_ROWS];
+#pragma unroll
+    for (int j = 0; j < PART_ROWS; j++) y[j] = (j < R && valid) ? x[(size_t)stride * j] : 0.0;
+    if (act) {
+        const double yl = part_local(y, R, tb + s0, tb + N + s0, tb + 2 * N + s0);
+        sm[(2 * k) * 64 + lane] = y[0];
+        sm[(2 * k + 1) * 64 + lane] = yl;
+    }
+    __syncthreads();
+    if (!act) return;
+    double Pk, Qk;
+    part_couple(k, K, tb + 5 * N, sm, lane, Pk, Qk);
+    const double *V = tb + 3 * N + s0, *W = tb + 4 * N + s0;
+#pragma unroll
+    for (int j = 0; j < PART_ROWS; j++)
+        if (j < R && valid) x[(size_t)stride * j] = (y[j] - Pk * V[j]) - Qk * W[j];
+}
+
+// x sweeps: the block stages its 64 lines (one contiguous span) in LDS rows
+// of odd length LS >= N, coalesced both ways; the block-end values follow at
+// P.coff
+__device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int blk) {
+    extern __shared__ double sm[];
+    const int c = blockIdx.y;
+    int n[3];
+    comp_dims(P.g, c, n);
+    const int N = n[0];
+    const LineSys &S = P.sys[c];
+    const int K = S.K;
+    const int nl = n[1] * n[2];
+    const int l0 = blk * 64;
+    if (l0 >= nl) return;  // the whole block: this component has fewer lines
+    const int cnt = (nl - l0 < 64) ? nl - l0 : 64;
+    const int LS = N | 1, tot = cnt * N;
+    double *a = P.a[c] + (size_t)l0 * N;
+    const float rN = 1.0f / (float)N;  // f < 2^24: float quotient within one
+    auto slot = [&](int f) {
+        int l = (int)((float)f * rN);
+        if (l * N > f) l--;
+        else if ((l + 1) * N <= f) l++;
+        return l * LS + (f - l * N);
+    };
+    const int nth = blockDim.x;
+    for (int f0 = threadIdx.x; f0 < tot; f0 += nth * SB) {
+        double v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; u++) v[u] = (f0 + nth * u < tot) ? a[f0 + nth * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < SB; u++)
+            if (f0 + nth * u < tot) sm[slot(f0 + nth * u)] = v[u];
+    }
+    __syncthreads();
+    const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const bool act = k < K;
+    const int s0 = act ? (k * N) / K : 0;
+    const int R = act ? ((k + 1) * N) / K - s0 : 0;
+    double *row = sm + (lane < cnt ? lane : 0) * LS + s0;
+    double *ends = sm + P.coff;
+    const double *tb = S.pt;
+    double y[PART_ROWS];
+#pragma unroll
+    for (int j = 0; j < PART_ROWS; j++) y[j] = (j < R) ? row[j] : 0.0;
+    if (act) {
+        const double yl = part_local(y, R, tb + s0, tb + N + s0, tb + 2 * N + s0);
+        ends[(2 * k) * 64 + lane] = y[0];
+        ends[(2 * k + 1) * 64 + lane] = yl;
+    }
+    __syncthreads();
+    if (act) {
+        double Pk, Qk;
+        part_couple(k, K, tb + 5 * N, ends, lane, Pk, Qk);
+        const double *V = tb + 3 * N + s0, *W = tb + 4 * N + s0;
+        if (lane < cnt) {
+#pragma unroll
+            for (int j = 0; j < PART_ROWS; j++)
+                if (j < R) row[j] = (y[j] - Pk * V[j]) - Qk * W[j];
+        }
+    }
+    __syncthreads();
+    for (int f0 = threadIdx.x; f0 < tot; f0 += nth * SB) {
+        double v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; u++) v[u] = (f0 + nth * u < tot) ? sm[slot(f0 + nth * u)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < SB; u++)
+            if (f0 + nth * u < tot) a[f0 + nth * u] = v[u];
+    }
+}
 
 // y / z sweeps: one thread per line, lines walked in global memory (the
 // threads of a wave hold consecutive x, so every chunk load is coalesced and
@@ -321,6 +486,26 @@ __global__ void __launch_bounds__(64) k_adi_strided(SweepParams P) {
 }
 __global__ void __launch_bounds__(64) k_adi_x(SweepParams P) { adi_x_body(P, blockIdx.x); }
 
+constexpr int PART_THREADS = 64 * PART_MAX_BLOCKS;
+template <int AX>
+__global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided(SweepParams P) {
+    adi_part_strided_body<AX>(P, blockIdx.x);
+}
+__global__ void __launch_bounds__(PART_THREADS) k_adi_part_x(SweepParams P) { adi_part_x_body(P, blockIdx.x); }
+template <int AX>
+__global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided_batch(const SweepParams *__restrict__ Ps,
+                                                                         const int *__restrict__ prefix, int nbox) {
+    int blk;
+    const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
+    adi_part_strided_body<AX>(P, blk);
+}
+__global__ void __launch_bounds__(PART_THREADS) k_adi_part_x_batch(const SweepParams *__restrict__ Ps,
+                                                                  const int *__restrict__ prefix, int nbox) {
+    int blk;
+    const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
+    adi_part_x_body(P, blk);
+}
+
 template <int AX>
 __global__ void __launch_bounds__(64) k_adi_strided_batch(const SweepParams *__restrict__ Ps,
                                                           const int *__restrict__ prefix, int nbox) {
@@ -347,7 +532,7 @@ size_t sweep_params_size() { return sizeof(SweepParams); }
 // parameters of the sweep along axis dir-1; returns the block count (grid.y = 3
 // components) and the dynamic LDS size
 int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      void *out, size_t *lds, int x_lsh) {
+                      void *out, size_t *lds, int x_lsh, bool part) {
     SweepParams &P = *(SweepParams *)out;
     P = SweepParams{};
     P.g = g;
@@ -355,6 +540,22 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
     for (int c = 0; c < 3; c++) P.sys[c] = (c == P.ax) ? sysA : sysB;
     P.a[0] = u; P.a[1] = v; P.a[2] = w;
     const int ax = P.ax;
+    if (part && sweep_partitioned(sysA, sysB)) {
+        // 64 lines per block, one wave per partition block
+        P.kw = sysA.K > sysB.K ? sysA.K : sysB.K;
+        size_t nl = 0;
+        int lsmax = 0;
+        for (int c = 0; c < 3; c++) {
+            int n[3];
+            comp_dims(g, c, n);
+            const size_t m = (ax == 0) ? (size_t)n[1] * n[2] : (size_t)n[0] * ((ax == 1) ? n[2] : n[1]);
+            nl = m > nl ? m : nl;
+            lsmax = (n[0] | 1) > lsmax ? (n[0] | 1) : lsmax;
+        }
+        P.coff = (ax == 0) ? 64 * lsmax : 0;
+        *lds = ((size_t)P.coff + (size_t)2 * P.kw * 64) * sizeof(double);
+        return (int)((nl + 63) / 64);
+    }
     if (ax == 0) {
         // longest x line, lines per block (up to 64 within 64 KiB), blocks of the largest component
         const int nmax = g.nx + 2;
@@ -388,8 +589,27 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
     return (int)((nl + per - 1) / per);
 }
 
-void launch_adi_batch(int ax, const void *params, const int *prefix, int nbox, int nblk, size_t lds, hipStream_t st) {
+bool sweep_partitioned(const LineSys &sysA, const LineSys &sysB) {
+    return !sysA.periodic && !sysB.periodic && sysA.K > 0 && sysB.K > 0;
+}
+
+int sweep_params_waves(const void *params) { return ((const SweepParams *)params)->kw; }
+
+void launch_adi_batch(int ax, int kw, const void *params, const int *prefix, int nbox, int nblk, size_t lds,
+                      hipStream_t st) {
     const SweepParams *Ps = (const SweepParams *)params;
+    if (kw > 0) {
+        const dim3 grid(nblk, 3), blk(64 * kw);
+        if (ax == 0) {
+            set_lds(k_adi_part_x_batch, lds);
+            hipLaunchKernelGGL(k_adi_part_x_batch, grid, blk, lds, st, Ps, prefix, nbox);
+        } else if (ax == 1) {
+            hipLaunchKernelGGL(k_adi_part_strided_batch<1>, grid, blk, lds, st, Ps, prefix, nbox);
+        } else {
+            hipLaunchKernelGGL(k_adi_part_strided_batch<2>, grid, blk, lds, st, Ps, prefix, nbox);
+        }
+        return;
+    }
     if (ax == 0) {
         set_lds(k_adi_x_batch, lds);
         hipLaunchKernelGGL(k_adi_x_batch, dim3(nblk, 3), dim3(64), lds, st, Ps, prefix, nbox);
@@ -407,6 +627,18 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
     SweepParams P;
     size_t lds = 0;
     const int nb = make_sweep_params(g, dir, sysA, sysB, u, v, w, &P, &lds);
+    if (P.kw > 0) {
+        const dim3 grid(nb, 3), blk(64 * P.kw);
+        if (dir == 1) {
+            set_lds(k_adi_part_x, lds);
+            hipLaunchKernelGGL(k_adi_part_x, grid, blk, lds, st, P);
+        } else if (dir == 2) {
+            hipLaunchKernelGGL(k_adi_part_strided<1>, grid, blk, lds, st, P);
+        } else {
+            hipLaunchKernelGGL(k_adi_part_strided<2>, grid, blk, lds, st, P);
+        }
+        return;
+    }
     if (dir == 1) {
         set_lds(k_adi_x, lds);
         hipLaunchKernelGGL(k_adi_x, dim3(nb, 3), dim3(64), lds, st, P);

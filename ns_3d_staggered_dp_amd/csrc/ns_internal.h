@@ -82,7 +82,15 @@ struct LineSys {
     int tk;
     double ak, ck, gk;
     const double *loR, *invR, *cpR;
+    // partitioned solve (walls): K blocks of rows [kN/K, (k+1)N/K), one wave
+    // per block; pt = loL | invL | cpL | V | W (N each) | G (2K x 2K); K = 0:
+    // not partitioned
+    int K;
+    const double *pt;
 };
+// partitioned wall solve: rows per block (registers per lane) and blocks per line
+constexpr int PART_ROWS = 32;
+constexpr int PART_MAX_BLOCKS = 16;
 
 struct Ctx;
 
@@ -146,9 +154,15 @@ void launch_cn2_batch(const void *params, const int *prefix, int nbox, int nblk,
 // ADI sweep along axis dir-1 (ns_adi.hip): per-box parameters, block count (grid.y = 3), LDS size
 size_t sweep_params_size();
 // x_lsh: at most 2^x_lsh lines per x-sweep block (6: 64)
+// part: use the partitioned wall solve where both systems have one
 int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      void *out, size_t *lds, int x_lsh = 6);
-void launch_adi_batch(int ax, const void *params, const int *prefix, int nbox, int nblk, size_t lds, hipStream_t st);
+                      void *out, size_t *lds, int x_lsh = 6, bool part = true);
+bool sweep_partitioned(const LineSys &sysA, const LineSys &sysB);
+// waves per block of a sweep's parameters (0: twisted / cyclic path)
+int sweep_params_waves(const void *params);
+// kw: waves per block of the partitioned solve (every box partitioned), 0: twisted / cyclic path
+void launch_adi_batch(int ax, int kw, const void *params, const int *prefix, int nbox, int nblk, size_t lds,
+                      hipStream_t st);
 struct PoissonPlans;
 size_t pass_params_size();
 // lsh_drop: lines per block reduced by 2^lsh_drop from the default (a launch-shape choice only)
