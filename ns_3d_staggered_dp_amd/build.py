@@ -39,12 +39,40 @@ FLAGS = [
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(os.path.basename(src))[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    stem = os.path.splitext(os.path.basename(src))[0]
+    obj = os.path.join(BUILD, stem + ".o")
+    # the compiler's per-kernel resource report (registers, scratch, occupancy)
+    cmd = [HIPCC, *FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
+    with open(os.path.join(BUILD, stem + ".resources.txt"), "w") as f:
+        f.write(r.stderr)
     return obj
+
+
+def kernel_resources() -> dict[str, dict[str, int]]:
+    """Per-kernel VGPRs, scratch bytes per lane and occupancy (waves per SIMD)
+    from the resource reports of the last build (empty if there is none)."""
+    out: dict[str, dict[str, int]] = {}
+    keys = {"VGPRs": "vgprs", "ScratchSize [bytes/lane]": "scratch", "Occupancy [waves/SIMD]": "occupancy"}
+    for src in SOURCES + EXTRA_SOURCES:
+        path = os.path.join(BUILD, os.path.splitext(os.path.basename(src))[0] + ".resources.txt")
+        if not os.path.exists(path):
+            continue
+        name = None
+        for line in open(path):
+            if "remark:" not in line:
+                continue
+            text = line.split("remark:", 1)[1].split("[-Rpass", 1)[0].strip()
+            if text.startswith("Function Name:"):
+                name = text.split(":", 1)[1].strip()
+                out[name] = {}
+            elif name is not None:
+                for k, v in keys.items():
+                    if text.startswith(k + ":"):
+                        out[name][v] = int(text.split(":", 1)[1])
+    return out
 
 
 HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")

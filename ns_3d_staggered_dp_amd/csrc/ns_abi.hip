@@ -576,8 +576,14 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
             for (int ax = 0; ax < 3; ax++) cap_vel(c, ax);
     } else {
         for (int dir = 1; dir <= 3; dir++) {
-            launch_vel_bc_cn2(g, f.bc[dir - 1], dir, c->planes, c->us, c->vs, c->ws, c->st);
-            launch_adi_sweep(g, dir, c->cn[dir - 1][0], c->cn[dir - 1][1], c->us, c->vs, c->ws, c->st);
+            // a partitioned sweep reads the boundary rows (vel_bc_staggered_CN2) from the planes itself
+            const LineSys &sa = c->cn[dir - 1][0], &sb = c->cn[dir - 1][1];
+            if (sweep_partitioned(sa, sb)) {
+                launch_adi_sweep(g, dir, sa, sb, c->us, c->vs, c->ws, c->st, c->planes, &c->L);
+            } else {
+                launch_vel_bc_cn2(g, f.bc[dir - 1], dir, c->planes, c->us, c->vs, c->ws, c->st);
+                launch_adi_sweep(g, dir, sa, sb, c->us, c->vs, c->ws, c->st);
+            }
             if (cap) cap_vel(c, dir - 1);  // post-sweep planes (big_periodic_3D.f90:433-489)
         }
     }
@@ -1213,9 +1219,15 @@ int ns_adjoint_step(ns_ctx *c, const double *u, const double *v, const double *w
         // the transpose of its boundary rows (vel_bc_staggered_CN2)
         if ((rc = adj_cn_systems(c))) return rc;
         for (int dir = 3; dir >= 1; dir--) {
-            launch_adi_sweep(g, dir, c->cnT[dir - 1][0], c->cnT[dir - 1][1], c->ausb[0], c->ausb[1], c->ausb[2],
-                             c->st);
-            launch_adj_cn2(g, dir - 1, f.bc[dir - 1], c->ausb, c->apbar, c->st);
+            // a partitioned solve moves the boundary rows' adjoints to the planes itself
+            const LineSys &sa = c->cnT[dir - 1][0], &sb = c->cnT[dir - 1][1];
+            if (sweep_partitioned(sa, sb)) {
+                launch_adi_sweep(g, dir, sa, sb, c->ausb[0], c->ausb[1], c->ausb[2], c->st, nullptr, &c->L,
+                                 c->apbar);
+            } else {
+                launch_adi_sweep(g, dir, sa, sb, c->ausb[0], c->ausb[1], c->ausb[2], c->st);
+                launch_adj_cn2(g, dir - 1, f.bc[dir - 1], c->ausb, c->apbar, c->st);
+            }
         }
     }
     // S1^T: predictor
@@ -1874,6 +1886,13 @@ void free_phases(std::vector<ns_batch::Phase> &ph) {
     ph.clear();
 }
 
+// every box's sweep along ax is partitioned (one launch shape for all boxes)
+bool batch_sweep_partitioned(const ns_batch *b, int ax) {
+    for (const ns_ctx *c : b->ctx)
+        if (!sweep_partitioned(c->cn[ax][0], c->cn[ax][1])) return false;
+    return true;
+}
+
 // parameters of every box for one launch, uploaded with their block prefix
 int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_batch::Phase> &out) {
     const int n = (int)b->ctx.size();
@@ -1891,9 +1910,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     if (kind == 0)
         for (ns_ctx *c : b->ctx) cols += predict_tile_columns(c->cfg);
     // sweeps: one launch shape for all boxes, the partitioned solve only if every box has it
-    bool part = true;
-    if (kind == 6)
-        for (ns_ctx *c : b->ctx) part = part && sweep_partitioned(c->cn[arg][0], c->cn[arg][1]);
+    const bool part = kind == 6 && batch_sweep_partitioned(b, arg);
     for (int i = 0; i < n; i++) {
         ns_ctx *c = b->ctx[i];
         const ns_config &f = c->cfg;
@@ -1920,7 +1937,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         } else if (kind == 6) {
             size_t l = 0;
             nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l,
-                                   b->knobs[b->cls[i]].adix_lsh, part);
+                                   b->knobs[b->cls[i]].adix_lsh, part, part ? c->planes : nullptr, &c->L);
             lds = l > lds ? l : lds;
             const int kw = sweep_params_waves(dst);
             variant = kw > variant ? kw : variant;
@@ -1955,9 +1972,12 @@ int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out
     int rc;
     if ((rc = add_phase(b, 0, 0, first_step, out))) return rc;
     if (b->ctx[0]->cfg.scheme == NS_AB2_CN) {
-        // per axis: boundary rows, then the sweep (x, y, z)
-        for (int ax = 0; ax < 3; ax++)
-            if ((rc = add_phase(b, 5, ax, first_step, out)) || (rc = add_phase(b, 6, ax, first_step, out))) return rc;
+        // per axis: boundary rows, then the sweep (x, y, z); a partitioned
+        // sweep reads the boundary rows from the planes itself
+        for (int ax = 0; ax < 3; ax++) {
+            if (!batch_sweep_partitioned(b, ax) && (rc = add_phase(b, 5, ax, first_step, out))) return rc;
+            if ((rc = add_phase(b, 6, ax, first_step, out))) return rc;
+        }
     } else {
         for (int ax = 0; ax < 3; ax++)
             if ((rc = add_phase(b, 1, ax, first_step, out))) return rc;

@@ -175,6 +175,14 @@ struct SweepParams {
     int lsh, coff;   // x sweep: log2 lines per block, LDS offset of the coefficients
                      // (partitioned: of the block-end values)
     int kw;          // partitioned wall solve: waves per block (0: twisted / cyclic path)
+    // partitioned solve: right-hand sides of the lines' first and last rows
+    // per component (the boundary rows vel_bc_staggered_CN2 would write; one
+    // value per line, in line order), or null: the array holds them
+    const double *rb[3][2];
+    // partitioned solve: the lines' first and last rows go to these (added, one
+    // value per line) and zeros to the arrays (the transposed boundary rows of
+    // the adjoint), or null: the arrays take them
+    double *wb[3][2];
 };
 
 // ---- partitioned wall solve (LineSys::K blocks per line, wave k = block k) ----
@@ -202,6 +210,51 @@ __device__ __forceinline__ double part_local(double (&y)[PART_ROWS], int R, cons
     for (int j = PART_ROWS - 2; j >= 0; j--)
         if (j + 1 < R) y[j] = y[j] - cpL[j] * y[j + 1];
     return d;
+}
+
+// rows [s_k, s_k + R) of a lane's line into registers (get(j): row s_k + j);
+// the line's first and last rows come from rb when it is set (line index t)
+template <typename Get>
+__device__ __forceinline__ void part_load(double (&y)[PART_ROWS], int R, bool valid, int k, int K,
+                                          const double *const (&rb)[2], unsigned t, Get get) {
+    const double *r0 = (k == 0) ? rb[0] : nullptr;  // uniform over the wave
+    const double *rl = (k == K - 1) ? rb[1] : nullptr;
+    const double first = (r0 && valid) ? r0[t] : 0.0;
+    const double last = (rl && valid) ? rl[t] : 0.0;
+#pragma unroll
+    for (int j = 0; j < PART_ROWS; j++) {
+        const bool f = r0 && j == 0, l = rl && j == R - 1;
+        double v = (j < R && valid && !f && !l) ? get(j) : 0.0;
+        if (f) v = first;
+        if (l) v = last;
+        y[j] = v;
+    }
+}
+
+// stores rows [s_k, s_k + R) of a lane's line (put(j, v): row s_k + j); the
+// line's first and last rows are added to wb (line index t) when it is set and
+// stored as zeros
+template <typename Put>
+__device__ __forceinline__ void part_store(const double (&x)[PART_ROWS], int R, bool valid, int k, int K,
+                                           double *const (&wb)[2], unsigned t, Put put) {
+    if (!valid) return;
+    double *w0 = (k == 0) ? wb[0] : nullptr;  // uniform over the wave
+    double *wl = (k == K - 1) ? wb[1] : nullptr;
+#pragma unroll
+    for (int j = 0; j < PART_ROWS; j++) {
+        if (j < R) {
+            double v = x[j];
+            if (w0 && j == 0) {
+                w0[t] += v;
+                v = 0.0;
+            }
+            if (wl && j == R - 1) {
+                wl[t] += v;
+                v = 0.0;
+            }
+            put(j, v);
+        }
+    }
 }
 
 // couplings of block k from the line's 2K end values (ends[j * 64 + lane])
@@ -251,8 +304,7 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
     double *x = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]) + (size_t)stride * s0;
     const double *tb = S.pt;
     double y[PART_ROWS];
-#pragma unroll
-    for (int j = 0; j < PART_ROWS; j++) y[j] = (j < R && valid) ? x[(size_t)stride * j] : 0.0;
+    part_load(y, R, valid, k, K, P.rb[c], t, [&](int j) { return x[(size_t)stride * j]; });
     if (act) {
         const double yl = part_local(y, R, tb + s0, tb + N + s0, tb + 2 * N + s0);
         sm[(2 * k) * 64 + lane] = y[0];
@@ -265,7 +317,8 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
     const double *V = tb + 3 * N + s0, *W = tb + 4 * N + s0;
 #pragma unroll
     for (int j = 0; j < PART_ROWS; j++)
-        if (j < R && valid) x[(size_t)stride * j] = (y[j] - Pk * V[j]) - Qk * W[j];
+        if (j < R) y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
+    part_store(y, R, valid, k, K, P.wb[c], t, [&](int j, double v) { x[(size_t)stride * j] = v; });
 }
 
 // x sweeps: the block stages its 64 lines (one contiguous span) in LDS rows
@@ -311,8 +364,7 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
     double *ends = sm + P.coff;
     const double *tb = S.pt;
     double y[PART_ROWS];
-#pragma unroll
-    for (int j = 0; j < PART_ROWS; j++) y[j] = (j < R) ? row[j] : 0.0;
+    part_load(y, R, lane < cnt, k, K, P.rb[c], (unsigned)(l0 + lane), [&](int j) { return row[j]; });
     if (act) {
         const double yl = part_local(y, R, tb + s0, tb + N + s0, tb + 2 * N + s0);
         ends[(2 * k) * 64 + lane] = y[0];
@@ -323,11 +375,10 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
         double Pk, Qk;
         part_couple(k, K, tb + 5 * N, ends, lane, Pk, Qk);
         const double *V = tb + 3 * N + s0, *W = tb + 4 * N + s0;
-        if (lane < cnt) {
 #pragma unroll
-            for (int j = 0; j < PART_ROWS; j++)
-                if (j < R) row[j] = (y[j] - Pk * V[j]) - Qk * W[j];
-        }
+        for (int j = 0; j < PART_ROWS; j++)
+            if (j < R) y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
+        part_store(y, R, lane < cnt, k, K, P.wb[c], (unsigned)(l0 + lane), [&](int j, double v) { row[j] = v; });
     }
     __syncthreads();
     for (int f0 = threadIdx.x; f0 < tot; f0 += nth * SB) {
@@ -494,13 +545,14 @@ __global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided(SweepParams P
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_x(SweepParams P) { adi_part_x_body(P, blockIdx.x); }
 template <int AX>
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided_batch(const SweepParams *__restrict__ Ps,
-                                                                         const int *__restrict__ prefix, int nbox) {
+                                                                                  const int *__restrict__ prefix,
+                                                                                  int nbox) {
     int blk;
     const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
     adi_part_strided_body<AX>(P, blk);
 }
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_x_batch(const SweepParams *__restrict__ Ps,
-                                                                  const int *__restrict__ prefix, int nbox) {
+                                                                           const int *__restrict__ prefix, int nbox) {
     int blk;
     const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
     adi_part_x_body(P, blk);
@@ -532,7 +584,8 @@ size_t sweep_params_size() { return sizeof(SweepParams); }
 // parameters of the sweep along axis dir-1; returns the block count (grid.y = 3
 // components) and the dynamic LDS size
 int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      void *out, size_t *lds, int x_lsh, bool part) {
+                      void *out, size_t *lds, int x_lsh, bool part, const double *planes, const PlaneLayout *L,
+                      double *planes_out) {
     SweepParams &P = *(SweepParams *)out;
     P = SweepParams{};
     P.g = g;
@@ -554,6 +607,16 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
         }
         P.coff = (ax == 0) ? 64 * lsmax : 0;
         *lds = ((size_t)P.coff + (size_t)2 * P.kw * 64) * sizeof(double);
+        for (int c = 0; c < 3; c++) {
+            if (planes) {
+                P.rb[c][0] = planes + L->off[6 * c + 2 * ax];
+                P.rb[c][1] = planes + L->off[6 * c + 2 * ax + 1];
+            }
+            if (planes_out) {
+                P.wb[c][0] = planes_out + L->off[6 * c + 2 * ax];
+                P.wb[c][1] = planes_out + L->off[6 * c + 2 * ax + 1];
+            }
+        }
         return (int)((nl + 63) / 64);
     }
     if (ax == 0) {
@@ -623,10 +686,10 @@ void launch_adi_batch(int ax, int kw, const void *params, const int *prefix, int
 }
 
 void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      hipStream_t st) {
+                      hipStream_t st, const double *planes, const PlaneLayout *L, double *planes_out) {
     SweepParams P;
     size_t lds = 0;
-    const int nb = make_sweep_params(g, dir, sysA, sysB, u, v, w, &P, &lds);
+    const int nb = make_sweep_params(g, dir, sysA, sysB, u, v, w, &P, &lds, 6, true, planes, L, planes_out);
     if (P.kw > 0) {
         const dim3 grid(nb, 3), blk(64 * P.kw);
         if (dir == 1) {

@@ -394,88 +394,204 @@ __device__ __forceinline__ double convdiff_at(const AdjPred &P, int c, int i, in
     return P.nu * dif - conv;
 }
 
-// The same sum for a non-ghost face, with one linear base index per array and
-// constant neighbour strides (every access is inside the arrays there).
-__device__ __forceinline__ double convdiff_inner(const AdjPred &P, int c, int i, int j, int k, const int nu_[3],
-                                                 const int nv_[3], const int nw_[3]) {
-    const double *U = P.u[0], *V = P.u[1], *W = P.u[2];
-    const double *GU = P.gcb[0], *GV = P.gcb[1], *GW = P.gcb[2];  // convective part
-    const double *GD = P.gb[c];                                    // diffusive part, component c
-    const double rx = P.rd3[0], ry = P.rd3[1], rz = P.rd3[2];
-    const long su1 = nu_[0], su2 = (long)nu_[0] * nu_[1];
-    const long sv1 = nv_[0], sv2 = (long)nv_[0] * nv_[1];
-    const long sw1 = nw_[0], sw2 = (long)nw_[0] * nw_[1];
-    const long qu = (i - 1) + su1 * (j - 1) + su2 * (k - 1);
-    const long qv = (i - 1) + sv1 * (j - 1) + sv2 * (k - 1);
-    const long qw = (i - 1) + sw1 * (j - 1) + sw2 * (k - 1);
-    double dif, conv;
-    if (c == 0) {
-        dif = (GD[qu - 1] - 2.0 * GD[qu] + GD[qu + 1]) * P.rd2[0] + (GD[qu - su1] - 2.0 * GD[qu] + GD[qu + su1]) * P.rd2[1] +
-              (GD[qu - su2] - 2.0 * GD[qu] + GD[qu + su2]) * P.rd2[2];
-        double X1 = (GU[qu] - GU[qu + 1]) * rx;
-        if (P.per[0] && i == P.g.nx) {
-            dif += GD[qu - (i - 1)] * P.rd2[0];
-            X1 -= GU[qu - (i - 1)] * rx;
-        }
-        const double X0 = (GU[qu - 1] - GU[qu]) * rx;
-        conv = 0.5 * (2.0 * (0.5 * (U[qu + 1] + U[qu])) * X1 + 2.0 * (0.5 * (U[qu] + U[qu - 1])) * X0);
-        const double ef1 = (GU[qu] - GU[qu + su1]) * ry + (GV[qv] - GV[qv + 1]) * rx;
-        const double ef0 = (GU[qu - su1] - GU[qu]) * ry + (GV[qv - sv1] - GV[qv - sv1 + 1]) * rx;
-        conv += 0.5 * (ef1 * (0.5 * (V[qv + 1] + V[qv])) + ef0 * (0.5 * (V[qv - sv1 + 1] + V[qv - sv1])));
-        const double ff1 = (GU[qu] - GU[qu + su2]) * rz + (GW[qw] - GW[qw + 1]) * rx;
-        const double ff0 = (GU[qu - su2] - GU[qu]) * rz + (GW[qw - sw2] - GW[qw - sw2 + 1]) * rx;
-        conv += 0.5 * (ff1 * (0.5 * (W[qw + 1] + W[qw])) + ff0 * (0.5 * (W[qw - sw2 + 1] + W[qw - sw2])));
-    } else if (c == 1) {
-        dif = (GD[qv - 1] - 2.0 * GD[qv] + GD[qv + 1]) * P.rd2[0] + (GD[qv - sv1] - 2.0 * GD[qv] + GD[qv + sv1]) * P.rd2[1] +
-              (GD[qv - sv2] - 2.0 * GD[qv] + GD[qv + sv2]) * P.rd2[2];
-        double X1 = (GV[qv] - GV[qv + sv1]) * ry;
-        if (P.per[1] && j == P.g.ny) {
-            dif += GD[qv - (long)(j - 1) * sv1] * P.rd2[1];
-            X1 -= GV[qv - (long)(j - 1) * sv1] * ry;
-        }
-        const double X0 = (GV[qv - sv1] - GV[qv]) * ry;
-        conv = 0.5 * (2.0 * (0.5 * (V[qv + sv1] + V[qv])) * X1 + 2.0 * (0.5 * (V[qv] + V[qv - sv1])) * X0);
-        const double ef1 = (GU[qu] - GU[qu + su1]) * ry + (GV[qv] - GV[qv + 1]) * rx;
-        const double ef0 = (GU[qu - 1] - GU[qu - 1 + su1]) * ry + (GV[qv - 1] - GV[qv]) * rx;
-        conv += 0.5 * (ef1 * (0.5 * (U[qu + su1] + U[qu])) + ef0 * (0.5 * (U[qu - 1 + su1] + U[qu - 1])));
-        const double gf1 = (GV[qv] - GV[qv + sv2]) * rz + (GW[qw] - GW[qw + sw1]) * ry;
-        const double gf0 = (GV[qv - sv2] - GV[qv]) * rz + (GW[qw - sw2] - GW[qw - sw2 + sw1]) * ry;
-        conv += 0.5 * (gf1 * (0.5 * (W[qw + sw1] + W[qw])) + gf0 * (0.5 * (W[qw - sw2 + sw1] + W[qw - sw2])));
+// The same sum along z columns: a block takes a 64 x 4 tile of (i, j)
+// columns and each thread walks z planes [k0, k1] of its column.  The values a
+// face reads at k - 1 and k + 1 of its own or a neighbouring column (the z
+// stencil, the z edge fluxes) stay in registers from plane to plane, and the
+// in-plane neighbours of consecutive planes are read by the same block close
+// together in time, so each array streams about once.  Non-ghost faces use the
+// sums below with one linear index per array (every access inside the arrays
+// there; the expression order is that of convdiff_at); ghost faces use
+// convdiff_at.
+constexpr int CD_TX = 64, CD_TY = 4;
+
+// the ghost faces of component c (i, j or k at either end of its array):
+// the two z end planes, then the two y end rows of the other planes, then the
+// two x end columns of the rest
+__global__ void __launch_bounds__(AT) k_adj_convdiff_surface(AdjPred P) {
+    const int c = blockIdx.y;
+    int nu_[3], nv_[3], nw_[3];
+    dims_of(P.g, 0, nu_);
+    dims_of(P.g, 1, nv_);
+    dims_of(P.g, 2, nw_);
+    const int *nc = (c == 0) ? nu_ : (c == 1) ? nv_ : nw_;
+    const long n0 = nc[0], n1 = nc[1], n2 = nc[2];
+    const long sz = 2 * n0 * n1, sy = 2 * n0 * (n2 - 2), sx = 2 * (n1 - 2) * (n2 - 2);
+    long t = (long)blockIdx.x * AT + threadIdx.x;
+    int i, j, k;
+    if (t < sz) {
+        k = (t < n0 * n1) ? 1 : (int)n2;
+        t %= n0 * n1;
+        i = (int)(t % n0) + 1;
+        j = (int)(t / n0) + 1;
+    } else if ((t -= sz) < sy) {
+        j = (t < n0 * (n2 - 2)) ? 1 : (int)n1;
+        t %= n0 * (n2 - 2);
+        i = (int)(t % n0) + 1;
+        k = (int)(t / n0) + 2;
+    } else if ((t -= sy) < sx) {
+        i = (t < (n1 - 2) * (n2 - 2)) ? 1 : (int)n0;
+        t %= (n1 - 2) * (n2 - 2);
+        j = (int)(t % (n1 - 2)) + 2;
+        k = (int)(t / (n1 - 2)) + 2;
     } else {
-        dif = (GD[qw - 1] - 2.0 * GD[qw] + GD[qw + 1]) * P.rd2[0] + (GD[qw - sw1] - 2.0 * GD[qw] + GD[qw + sw1]) * P.rd2[1] +
-              (GD[qw - sw2] - 2.0 * GD[qw] + GD[qw + sw2]) * P.rd2[2];
-        double X1 = (GW[qw] - GW[qw + sw2]) * rz;
-        if (P.per[2] && k == P.g.nz) {
-            dif += GD[qw - (long)(k - 1) * sw2] * P.rd2[2];
-            X1 -= GW[qw - (long)(k - 1) * sw2] * rz;
-        }
-        const double X0 = (GW[qw - sw2] - GW[qw]) * rz;
-        conv = 0.5 * (2.0 * (0.5 * (W[qw + sw2] + W[qw])) * X1 + 2.0 * (0.5 * (W[qw] + W[qw - sw2])) * X0);
-        const double ff1 = (GU[qu] - GU[qu + su2]) * rz + (GW[qw] - GW[qw + 1]) * rx;
-        const double ff0 = (GU[qu - 1] - GU[qu - 1 + su2]) * rz + (GW[qw - 1] - GW[qw]) * rx;
-        conv += 0.5 * (ff1 * (0.5 * (U[qu + su2] + U[qu])) + ff0 * (0.5 * (U[qu - 1 + su2] + U[qu - 1])));
-        const double gf1 = (GV[qv] - GV[qv + sv2]) * rz + (GW[qw] - GW[qw + sw1]) * ry;
-        const double gf0 = (GV[qv - sv1] - GV[qv - sv1 + sv2]) * rz + (GW[qw - sw1] - GW[qw]) * ry;
-        conv += 0.5 * (gf1 * (0.5 * (V[qv + sv2] + V[qv])) + gf0 * (0.5 * (V[qv - sv1 + sv2] + V[qv - sv1])));
+        return;
     }
-    return P.nu * dif - conv;
+    P.ub[c][(size_t)(i - 1) + (size_t)n0 * ((size_t)(j - 1) + (size_t)n1 * (size_t)(k - 1))] +=
+        convdiff_at<true>(P, c, i, j, k, nu_, nv_, nw_);
 }
 
-// one launch per component (a launch for all three at once measured slower)
-__global__ void __launch_bounds__(AT) k_adj_convdiff(AdjPred P, int c) {
+template <int C>
+__global__ void __launch_bounds__(CD_TX * CD_TY) k_adj_convdiff_z(AdjPred P, int ntx, int nty, int nzc, int cz) {
     const Grid g = P.g;
     int nu_[3], nv_[3], nw_[3];
     dims_of(g, 0, nu_);
     dims_of(g, 1, nv_);
     dims_of(g, 2, nw_);
-    const int *nc = (c == 0) ? nu_ : (c == 1) ? nv_ : nw_;
-    const size_t tot = (size_t)nc[0] * nc[1] * nc[2];
-    const size_t q = (size_t)xcd_slab_block(blockIdx.x, gridDim.x) * AT + threadIdx.x;
-    if (q >= tot) return;
-    int i, j, k;
-    unflat(q, nc, i, j, k);
-    const bool inner = i >= 2 && i <= nc[0] - 1 && j >= 2 && j <= nc[1] - 1 && k >= 2 && k <= nc[2] - 1;
-    P.ub[c][q] += inner ? convdiff_inner(P, c, i, j, k, nu_, nv_, nw_) : convdiff_at<true>(P, c, i, j, k, nu_, nv_, nw_);
+    const int *nc = (C == 0) ? nu_ : (C == 1) ? nv_ : nw_;
+    const int nb = ntx * nty * nzc;
+    const int b = xcd_slab_block(blockIdx.x, gridDim.x);
+    if (b >= nb) return;
+    // tiles and chunks cover the non-ghost faces i, j, k in [2, n - 1] only: the
+    // ghost faces (the box's surface) are k_adj_convdiff_surface's
+    const int tx = b % ntx, ty = (b / ntx) % nty, kz = b / (ntx * nty);
+    const int i = tx * CD_TX + (int)(threadIdx.x & 63) + 2, j = ty * CD_TY + (int)(threadIdx.x >> 6) + 2;
+    const int k0 = kz * cz + 2, k1 = (k0 + cz - 1 < nc[2] - 1) ? k0 + cz - 1 : nc[2] - 1;
+    if (i > nc[0] - 1 || j > nc[1] - 1) return;
+    const long sc1 = nc[0], sc2 = (long)nc[0] * nc[1];
+    double *out = P.ub[C] + (i - 1) + sc1 * (j - 1);
+    const double *U = P.u[0], *V = P.u[1], *W = P.u[2];
+    const double *GU = P.gcb[0], *GV = P.gcb[1], *GW = P.gcb[2];  // convective part
+    const double *GD = P.gb[C];                                    // diffusive part, component C
+    const double rx = P.rd3[0], ry = P.rd3[1], rz = P.rd3[2];
+    const long su1 = nu_[0], su2 = (long)nu_[0] * nu_[1];
+    const long sv1 = nv_[0], sv2 = (long)nv_[0] * nv_[1];
+    const long sw1 = nw_[0], sw2 = (long)nw_[0] * nw_[1];
+    // plane-1 offsets of the column (i, j) in each array
+    const long cu = (i - 1) + su1 * (j - 1), cv = (i - 1) + sv1 * (j - 1), cw = (i - 1) + sw1 * (j - 1);
+    // value at plane kk of a column, 0 outside the array's planes
+    auto zk = [](const double *A, long col, long s2, int n2, int kk) {
+        return (kk >= 1 && kk <= n2) ? A[col + s2 * (kk - 1)] : 0.0;
+    };
+    const int nzC = nc[2];
+    if constexpr (C == 0) {
+        // windows: GD, GU at (i, j) planes k-1..k+1; GW, W at (i, j), (i+1, j) planes k-1, k
+        double gdm = zk(GD, cu, su2, nzC, k0 - 1), gdc = zk(GD, cu, su2, nzC, k0), gdp = zk(GD, cu, su2, nzC, k0 + 1);
+        double gum = zk(GU, cu, su2, nzC, k0 - 1), guc = zk(GU, cu, su2, nzC, k0), gup = zk(GU, cu, su2, nzC, k0 + 1);
+        double gw0m = zk(GW, cw, sw2, nw_[2], k0 - 1), gw0c = zk(GW, cw, sw2, nw_[2], k0);
+        double gw1m = zk(GW, cw + 1, sw2, nw_[2], k0 - 1), gw1c = zk(GW, cw + 1, sw2, nw_[2], k0);
+        double w0m = zk(W, cw, sw2, nw_[2], k0 - 1), w0c = zk(W, cw, sw2, nw_[2], k0);
+        double w1m = zk(W, cw + 1, sw2, nw_[2], k0 - 1), w1c = zk(W, cw + 1, sw2, nw_[2], k0);
+        for (int k = k0; k <= k1; k++) {
+            const long qu = cu + su2 * (k - 1), qv = cv + sv2 * (k - 1);
+            {
+                double dif = (GD[qu - 1] - 2.0 * gdc + GD[qu + 1]) * P.rd2[0] +
+                             (GD[qu - su1] - 2.0 * gdc + GD[qu + su1]) * P.rd2[1] + (gdm - 2.0 * gdc + gdp) * P.rd2[2];
+                double X1 = (guc - GU[qu + 1]) * rx;
+                if (P.per[0] && i == g.nx) {
+                    dif += GD[qu - (i - 1)] * P.rd2[0];
+                    X1 -= GU[qu - (i - 1)] * rx;
+                }
+                const double X0 = (GU[qu - 1] - guc) * rx;
+                double conv = 0.5 * (2.0 * (0.5 * (U[qu + 1] + U[qu])) * X1 + 2.0 * (0.5 * (U[qu] + U[qu - 1])) * X0);
+                const double ef1 = (guc - GU[qu + su1]) * ry + (GV[qv] - GV[qv + 1]) * rx;
+                const double ef0 = (GU[qu - su1] - guc) * ry + (GV[qv - sv1] - GV[qv - sv1 + 1]) * rx;
+                conv += 0.5 * (ef1 * (0.5 * (V[qv + 1] + V[qv])) + ef0 * (0.5 * (V[qv - sv1 + 1] + V[qv - sv1])));
+                const double ff1 = (guc - gup) * rz + (gw0c - gw1c) * rx;
+                const double ff0 = (gum - guc) * rz + (gw0m - gw1m) * rx;
+                conv += 0.5 * (ff1 * (0.5 * (w1c + w0c)) + ff0 * (0.5 * (w1m + w0m)));
+                out[sc2 * (k - 1)] += P.nu * dif - conv;
+            }
+            gdm = gdc; gdc = gdp; gdp = zk(GD, cu, su2, nzC, k + 2);
+            gum = guc; guc = gup; gup = zk(GU, cu, su2, nzC, k + 2);
+            gw0m = gw0c; gw0c = zk(GW, cw, sw2, nw_[2], k + 1);
+            gw1m = gw1c; gw1c = zk(GW, cw + 1, sw2, nw_[2], k + 1);
+            w0m = w0c; w0c = zk(W, cw, sw2, nw_[2], k + 1);
+            w1m = w1c; w1c = zk(W, cw + 1, sw2, nw_[2], k + 1);
+        }
+    } else if constexpr (C == 1) {
+        // windows: GD, GV at (i, j) planes k-1..k+1; GW, W at (i, j), (i, j+1) planes k-1, k
+        double gdm = zk(GD, cv, sv2, nzC, k0 - 1), gdc = zk(GD, cv, sv2, nzC, k0), gdp = zk(GD, cv, sv2, nzC, k0 + 1);
+        double gvm = zk(GV, cv, sv2, nzC, k0 - 1), gvc = zk(GV, cv, sv2, nzC, k0), gvp = zk(GV, cv, sv2, nzC, k0 + 1);
+        double gw0m = zk(GW, cw, sw2, nw_[2], k0 - 1), gw0c = zk(GW, cw, sw2, nw_[2], k0);
+        double gw1m = zk(GW, cw + sw1, sw2, nw_[2], k0 - 1), gw1c = zk(GW, cw + sw1, sw2, nw_[2], k0);
+        double w0m = zk(W, cw, sw2, nw_[2], k0 - 1), w0c = zk(W, cw, sw2, nw_[2], k0);
+        double w1m = zk(W, cw + sw1, sw2, nw_[2], k0 - 1), w1c = zk(W, cw + sw1, sw2, nw_[2], k0);
+        for (int k = k0; k <= k1; k++) {
+            const long qu = cu + su2 * (k - 1), qv = cv + sv2 * (k - 1);
+            {
+                double dif = (GD[qv - 1] - 2.0 * gdc + GD[qv + 1]) * P.rd2[0] +
+                             (GD[qv - sv1] - 2.0 * gdc + GD[qv + sv1]) * P.rd2[1] + (gdm - 2.0 * gdc + gdp) * P.rd2[2];
+                double X1 = (gvc - GV[qv + sv1]) * ry;
+                if (P.per[1] && j == g.ny) {
+                    dif += GD[qv - (long)(j - 1) * sv1] * P.rd2[1];
+                    X1 -= GV[qv - (long)(j - 1) * sv1] * ry;
+                }
+                const double X0 = (GV[qv - sv1] - gvc) * ry;
+                double conv =
+                    0.5 * (2.0 * (0.5 * (V[qv + sv1] + V[qv])) * X1 + 2.0 * (0.5 * (V[qv] + V[qv - sv1])) * X0);
+                const double ef1 = (GU[qu] - GU[qu + su1]) * ry + (gvc - GV[qv + 1]) * rx;
+                const double ef0 = (GU[qu - 1] - GU[qu - 1 + su1]) * ry + (GV[qv - 1] - gvc) * rx;
+                conv += 0.5 * (ef1 * (0.5 * (U[qu + su1] + U[qu])) + ef0 * (0.5 * (U[qu - 1 + su1] + U[qu - 1])));
+                const double gf1 = (gvc - gvp) * rz + (gw0c - gw1c) * ry;
+                const double gf0 = (gvm - gvc) * rz + (gw0m - gw1m) * ry;
+                conv += 0.5 * (gf1 * (0.5 * (w1c + w0c)) + gf0 * (0.5 * (w1m + w0m)));
+                out[sc2 * (k - 1)] += P.nu * dif - conv;
+            }
+            gdm = gdc; gdc = gdp; gdp = zk(GD, cv, sv2, nzC, k + 2);
+            gvm = gvc; gvc = gvp; gvp = zk(GV, cv, sv2, nzC, k + 2);
+            gw0m = gw0c; gw0c = zk(GW, cw, sw2, nw_[2], k + 1);
+            gw1m = gw1c; gw1c = zk(GW, cw + sw1, sw2, nw_[2], k + 1);
+            w0m = w0c; w0c = zk(W, cw, sw2, nw_[2], k + 1);
+            w1m = w1c; w1c = zk(W, cw + sw1, sw2, nw_[2], k + 1);
+        }
+    } else {
+        // windows: GD, GW, W at (i, j) planes k-1..k+1; GU, U at (i, j), (i-1, j)
+        // and GV, V at (i, j), (i, j-1), planes k, k+1
+        double gdm = zk(GD, cw, sw2, nzC, k0 - 1), gdc = zk(GD, cw, sw2, nzC, k0), gdp = zk(GD, cw, sw2, nzC, k0 + 1);
+        double gwm = zk(GW, cw, sw2, nzC, k0 - 1), gwc = zk(GW, cw, sw2, nzC, k0), gwp = zk(GW, cw, sw2, nzC, k0 + 1);
+        double wm = zk(W, cw, sw2, nzC, k0 - 1), wc = zk(W, cw, sw2, nzC, k0), wp = zk(W, cw, sw2, nzC, k0 + 1);
+        double gu0c = zk(GU, cu, su2, nu_[2], k0), gu0p = zk(GU, cu, su2, nu_[2], k0 + 1);
+        double gu1c = zk(GU, cu - 1, su2, nu_[2], k0), gu1p = zk(GU, cu - 1, su2, nu_[2], k0 + 1);
+        double u0c = zk(U, cu, su2, nu_[2], k0), u0p = zk(U, cu, su2, nu_[2], k0 + 1);
+        double u1c = zk(U, cu - 1, su2, nu_[2], k0), u1p = zk(U, cu - 1, su2, nu_[2], k0 + 1);
+        double gv0c = zk(GV, cv, sv2, nv_[2], k0), gv0p = zk(GV, cv, sv2, nv_[2], k0 + 1);
+        double gv1c = zk(GV, cv - sv1, sv2, nv_[2], k0), gv1p = zk(GV, cv - sv1, sv2, nv_[2], k0 + 1);
+        double v0c = zk(V, cv, sv2, nv_[2], k0), v0p = zk(V, cv, sv2, nv_[2], k0 + 1);
+        double v1c = zk(V, cv - sv1, sv2, nv_[2], k0), v1p = zk(V, cv - sv1, sv2, nv_[2], k0 + 1);
+        for (int k = k0; k <= k1; k++) {
+            const long qw = cw + sw2 * (k - 1);
+            {
+                double dif = (GD[qw - 1] - 2.0 * gdc + GD[qw + 1]) * P.rd2[0] +
+                             (GD[qw - sw1] - 2.0 * gdc + GD[qw + sw1]) * P.rd2[1] + (gdm - 2.0 * gdc + gdp) * P.rd2[2];
+                double X1 = (gwc - gwp) * rz;
+                if (P.per[2] && k == g.nz) {
+                    dif += GD[qw - (long)(k - 1) * sw2] * P.rd2[2];
+                    X1 -= GW[qw - (long)(k - 1) * sw2] * rz;
+                }
+                const double X0 = (gwm - gwc) * rz;
+                double conv = 0.5 * (2.0 * (0.5 * (wp + wc)) * X1 + 2.0 * (0.5 * (wc + wm)) * X0);
+                const double ff1 = (gu0c - gu0p) * rz + (gwc - GW[qw + 1]) * rx;
+                const double ff0 = (gu1c - gu1p) * rz + (GW[qw - 1] - gwc) * rx;
+                conv += 0.5 * (ff1 * (0.5 * (u0p + u0c)) + ff0 * (0.5 * (u1p + u1c)));
+                const double gf1 = (gv0c - gv0p) * rz + (gwc - GW[qw + sw1]) * ry;
+                const double gf0 = (gv1c - gv1p) * rz + (GW[qw - sw1] - gwc) * ry;
+                conv += 0.5 * (gf1 * (0.5 * (v0p + v0c)) + gf0 * (0.5 * (v1p + v1c)));
+                out[sc2 * (k - 1)] += P.nu * dif - conv;
+            }
+            gdm = gdc; gdc = gdp; gdp = zk(GD, cw, sw2, nzC, k + 2);
+            gwm = gwc; gwc = gwp; gwp = zk(GW, cw, sw2, nzC, k + 2);
+            wm = wc; wc = wp; wp = zk(W, cw, sw2, nzC, k + 2);
+            gu0c = gu0p; gu0p = zk(GU, cu, su2, nu_[2], k + 2);
+            gu1c = gu1p; gu1p = zk(GU, cu - 1, su2, nu_[2], k + 2);
+            u0c = u0p; u0p = zk(U, cu, su2, nu_[2], k + 2);
+            u1c = u1p; u1p = zk(U, cu - 1, su2, nu_[2], k + 2);
+            gv0c = gv0p; gv0p = zk(GV, cv, sv2, nv_[2], k + 2);
+            gv1c = gv1p; gv1p = zk(GV, cv - sv1, sv2, nv_[2], k + 2);
+            v0c = v0p; v0p = zk(V, cv, sv2, nv_[2], k + 2);
+            v1c = v1p; v1p = zk(V, cv - sv1, sv2, nv_[2], k + 2);
+        }
+    }
 }
 
 __global__ void __launch_bounds__(AT) k_block_sum(const double *a, size_t n, double *partial) {
@@ -694,7 +810,29 @@ void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, i
     const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
     for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_pred_prep, dim3(ablocks(tot[c])), dim3(AT), 0, st, P, c);
     if (dp_flag) hipLaunchKernelGGL(k_adj_pred_p, dim3(ablocks(g.np())), dim3(AT), 0, st, P);
-    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_convdiff, dim3(round8(ablocks(tot[c]))), dim3(AT), 0, st, P, c);
+    // the box's surface faces, then the z columns of the others: chunks of cz
+    // planes, about one round of 4 resident tiles per CU
+    long surf = 0;
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        const long m = 2L * n[0] * n[1] + 2L * n[0] * (n[2] - 2) + 2L * (n[1] - 2) * (n[2] - 2);
+        surf = m > surf ? m : surf;
+    }
+    hipLaunchKernelGGL(k_adj_convdiff_surface, dim3(ablocks((size_t)surf), 3), dim3(AT), 0, st, P);
+    for (int c = 0; c < 3; c++) {
+        int n[3];
+        comp_dims(g, c, n);
+        const int ntx = (n[0] - 2 + CD_TX - 1) / CD_TX, nty = (n[1] - 2 + CD_TY - 1) / CD_TY;
+        if (ntx < 1 || nty < 1 || n[2] < 3) continue;
+        int cz = (int)(((long)(n[2] - 2) * ntx * nty + 1023) / 1024);
+        cz = cz < 4 ? 4 : cz > 32 ? 32 : cz;
+        const int nzc = (n[2] - 2 + cz - 1) / cz;
+        const dim3 grid(round8((unsigned)(ntx * nty * nzc))), blk(CD_TX * CD_TY);
+        if (c == 0) hipLaunchKernelGGL(k_adj_convdiff_z<0>, grid, blk, 0, st, P, ntx, nty, nzc, cz);
+        else if (c == 1) hipLaunchKernelGGL(k_adj_convdiff_z<1>, grid, blk, 0, st, P, ntx, nty, nzc, cz);
+        else hipLaunchKernelGGL(k_adj_convdiff_z<2>, grid, blk, 0, st, P, ntx, nty, nzc, cz);
+    }
 }
 
 }  // namespace ns

@@ -89,7 +89,7 @@ struct LineSys {
     const double *pt;
 };
 // partitioned wall solve: rows per block (registers per lane) and blocks per line
-constexpr int PART_ROWS = 32;
+constexpr int PART_ROWS = 28;
 constexpr int PART_MAX_BLOCKS = 16;
 
 struct Ctx;
@@ -154,9 +154,15 @@ void launch_cn2_batch(const void *params, const int *prefix, int nbox, int nblk,
 // ADI sweep along axis dir-1 (ns_adi.hip): per-box parameters, block count (grid.y = 3), LDS size
 size_t sweep_params_size();
 // x_lsh: at most 2^x_lsh lines per x-sweep block (6: 64)
-// part: use the partitioned wall solve where both systems have one
+// part: use the partitioned wall solve where both systems have one; planes
+// (partitioned solve only): the lines' first and last rows are read from the
+// plane set (the rows vel_bc_staggered_CN2 sets) instead of the arrays;
+// planes_out (partitioned solve only): the solution's first and last rows are
+// added to that plane set and zeros stored in their place (the transpose of
+// those boundary rows, after an adjoint solve)
 int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      void *out, size_t *lds, int x_lsh = 6, bool part = true);
+                      void *out, size_t *lds, int x_lsh = 6, bool part = true, const double *planes = nullptr,
+                      const PlaneLayout *L = nullptr, double *planes_out = nullptr);
 bool sweep_partitioned(const LineSys &sysA, const LineSys &sysB);
 // waves per block of a sweep's parameters (0: twisted / cyclic path)
 int sweep_params_waves(const void *params);
@@ -276,7 +282,9 @@ void launch_add_at(double *a, size_t q, double v, hipStream_t st);
 void launch_add_partials(double *a, size_t q, const double *partial, int nb, double scale, hipStream_t st);
 
 // ---- ADI (ns_adi.hip) ----
+// planes, planes_out: as for make_sweep_params (used only when the sweep is partitioned)
 void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      hipStream_t st);
+                      hipStream_t st, const double *planes = nullptr, const PlaneLayout *L = nullptr,
+                      double *planes_out = nullptr);
 
 }  // namespace ns
