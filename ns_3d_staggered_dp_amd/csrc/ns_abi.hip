@@ -1924,7 +1924,8 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
             nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr, c->p,
                                      c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst, zc_hint);
         } else if (kind == 1) {
-            nb = make_vel_bc_params(g, f.bc, c->planes, c->us, c->vs, c->ws, arg, dst);
+            const double d0[3] = {0.0, 0.0, 0.0};
+            nb = make_fill_params(g, false, f.bc, c->planes, d0, c->us, c->vs, c->ws, 0, dst);
         } else if (kind == 2) {
             size_t l = 0;
             nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, c->us, c->vs, c->ws, nullptr, c->planes,
@@ -1942,7 +1943,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
             const int kw = sweep_params_waves(dst);
             variant = kw > variant ? kw : variant;
         } else if (kind == 3) {
-            nb = make_pr_bc_params(g, f.pbc, c->planes, d, c->dp, arg, dst);
+            nb = make_fill_params(g, true, f.pbc, c->planes, d, c->dp, nullptr, nullptr, 0, dst);
         } else {
             const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
             nb = make_project_params(g, d, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
@@ -1978,14 +1979,12 @@ int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out
             if (!batch_sweep_partitioned(b, ax) && (rc = add_phase(b, 5, ax, first_step, out))) return rc;
             if ((rc = add_phase(b, 6, ax, first_step, out))) return rc;
         }
-    } else {
-        for (int ax = 0; ax < 3; ax++)
-            if ((rc = add_phase(b, 1, ax, first_step, out))) return rc;
+    } else if ((rc = add_phase(b, 1, 0, first_step, out))) {  // the whole velocity fill
+        return rc;
     }
     for (int pass = 0; pass < 5; pass++)
         if ((rc = add_phase(b, 2, pass, first_step, out))) return rc;
-    for (int ax = 0; ax < 3; ax++)
-        if ((rc = add_phase(b, 3, ax, first_step, out))) return rc;
+    if ((rc = add_phase(b, 3, 0, first_step, out))) return rc;  // the whole pressure fill
     return add_phase(b, 4, 0, first_step, out);
 }
 
@@ -1994,9 +1993,9 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
     for (const auto &ph : phases) {
         switch (ph.kind) {
             case 0: launch_predict_batch(false, ph.params, ph.prefix, n, ph.nblk, b->st); break;
-            case 1: launch_vel_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 1: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 3, b->st); break;
             case 2: launch_dct_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
-            case 3: launch_pr_bc_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 3: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 1, b->st); break;
             case 5: launch_cn2_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 6: launch_adi_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;

@@ -138,15 +138,15 @@ void launch_predict_batch(bool lazy, const void *params, const int *prefix, int 
 size_t bc_params_size();
 size_t proj_params_size();
 // fill / projection parameters of one box; return the block count (0: nothing to launch)
-int make_vel_bc_params(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w, int ax,
-                       void *out);
-int make_pr_bc_params(const Grid &g, const int pbc[3], const double *planes, const double d[3], double *dp, int ax,
-                      void *out);
 int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag, const double *us, const double *vs,
                         const double *ws, const double *dp, const double *gauge, double *u, double *v, double *w,
                         double *p, void *out);
-void launch_vel_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
-void launch_pr_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
+// a whole velocity (pres false: a0..a2 = u, v, w) or pressure (a0 = dp) ghost
+// fill, axes first_axis..2, in one launch; returns the block count (grid.y = 3
+// components or 1)
+int make_fill_params(const Grid &g, bool pres, const int bc[3], const double *planes, const double d[3], double *a0,
+                     double *a1, double *a2, int first_axis, void *out);
+void launch_whole_fill_batch(const void *params, const int *prefix, int nbox, int nblk, int ncomp, hipStream_t st);
 void launch_project_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st);
 // AB2-CN boundary rows (vel_bc_staggered_CN2) along axis ax; 0 blocks: nothing to launch
 int make_cn2_params(const Grid &g, int bc, int ax, const double *planes, double *u, double *v, double *w, void *out);

@@ -153,72 +153,142 @@ __global__ void __launch_bounds__(256) k_conv_diff(CDParams P) {
 
 struct BCParams {
     int nx, ny, nz;
-    int ax;
+    int ax;        // axis of a one-axis launch; first filled axis of a whole fill
     int bc;
-    int pbc_mode;  // 0 velocity fill, 1 CN2 rows, 2 pressure fill
+    int pbc_mode;  // 0 velocity fill, 1 CN2 rows, 2 pressure fill (one axis); 3 velocity, 4 pressure (whole fill)
     double d;
+    int bcs[3];    // whole fill: codes per axis
+    double ds[3];  // whole pressure fill: spacings (code 3 rows)
     PlaneLayout L;
     const double *planes;
     double *a[3];
 };
 
+// ---- a whole ghost fill (x, y, z passes) in one launch ----
+// vel_bc_staggered (NS_functions.f90:213-310) and pr_bc_staggered
+// (:1403-1464) assign the end entries of every line along x, then y, then z,
+// each from values the earlier passes left.  The final value of an entry is
+// the last pass that assigns it, applied to what the earlier passes produced;
+// following that chain back ends at entries no pass assigns (the substituted
+// indices are source rows 2, n-1 or the periodic normal face 1, which no pass
+// assigns).  So one thread per assigned entry computes its final value from
+// unassigned entries only (no thread reads an entry another writes), with the
+// operations and their order of the three passes: bitwise their result.
+
+// is entry m of an axis line (length N) assigned by that axis' pass?
+__device__ __forceinline__ bool fill_assigned(bool pres, bool normal, int bc, int m, int N) {
+    if (m != 1 && m != N) return false;
+    if (pres) return bc >= 1 && bc <= 4;
+    if (bc == 1) return normal ? m == N : true;
+    return bc == 2 || bc == 4;  // velocity code 3: the pass leaves the line
+}
+
+// the pass' value for the low (m = 1) or high (m = N) end; src(m'): the line's
+// value at m' before the pass, b: the line's plane entry (codes 2-4)
+template <class Src>
+__device__ __forceinline__ double fill_rule(bool pres, bool normal, int bc, bool lo, int N, double d, const double *b,
+                                            Src src) {
+    if (!pres) {
+        if (bc == 1) return normal ? src(1) : (lo ? src(N - 1) : src(2));
+        if (bc == 2) return normal ? *b : (lo ? 2 * *b - src(2) : 2 * *b - src(N - 1));
+        return *b;
+    }
+    if (bc == 1) return lo ? src(N - 1) : src(2);
+    if (bc == 2) return lo ? 2 * *b - src(2) : 2 * *b - src(N - 1);
+    if (bc == 3) return lo ? -d * *b + src(2) : d * *b + src(N - 1);
+    return *b;
+}
+
+// entry (i, j, k) of the box surface enumerated by t: the two z end planes,
+// the two y end rows of the other planes, the two x end columns of the rest
+struct Entry {
+    int i, j, k;
+    bool ok;
+};
+__device__ __forceinline__ Entry surface_entry(size_t t, size_t n0, size_t n1, size_t n2) {
+    const size_t sz = 2 * n0 * n1, sy = 2 * n0 * (n2 - 2), sx = 2 * (n1 - 2) * (n2 - 2);
+    if (t < sz) {
+        const size_t r = t % (n0 * n1);
+        return {(int)(r % n0) + 1, (int)(r / n0) + 1, (t < n0 * n1) ? 1 : (int)n2, true};
+    }
+    if ((t -= sz) < sy) {
+        const size_t r = t % (n0 * (n2 - 2));
+        return {(int)(r % n0) + 1, (t < n0 * (n2 - 2)) ? 1 : (int)n1, (int)(r / n0) + 2, true};
+    }
+    if ((t -= sy) < sx) {
+        const size_t r = t % ((n1 - 2) * (n2 - 2));
+        return {(t < (n1 - 2) * (n2 - 2)) ? 1 : (int)n0, (int)(r % (n1 - 2)) + 2, (int)(r / (n1 - 2)) + 2, true};
+    }
+    return {0, 0, 0, false};
+}
+
+__host__ __device__ inline size_t surface_size(const int n[3]) {
+    return 2 * (size_t)n[0] * n[1] + 2 * (size_t)n[0] * (n[2] - 2) + 2 * (size_t)(n[1] - 2) * (n[2] - 2);
+}
+
+// component c (velocity) or dp (pressure); passes along axes >= P.ax
+__device__ __forceinline__ void whole_fill_body(const BCParams &P, const size_t t, const int c) {
+    const bool pres = (P.pbc_mode == 4);
+    // every array and parameter index below is a constant (a run-time index
+    // would copy the parameters to scratch memory)
+    const int n0 = P.nx + 2 - (!pres && c == 0), n1 = P.ny + 2 - (!pres && c == 1), n2 = P.nz + 2 - (!pres && c == 2);
+    const Entry e = surface_entry(t, n0, n1, n2);
+    if (!e.ok) return;
+    const int i = e.i, j = e.j, k = e.k;
+    double *A = pres ? P.a[0] : (c == 0) ? P.a[0] : (c == 1) ? P.a[1] : P.a[2];
+    auto asg = [&](int ax, int m) {
+        const int na = (ax == 0) ? n0 : (ax == 1) ? n1 : n2;
+        const int bc = (ax == 0) ? P.bcs[0] : (ax == 1) ? P.bcs[1] : P.bcs[2];
+        return ax >= P.ax && fill_assigned(pres, c == ax, bc, m, na);
+    };
+    if (!asg(0, i) && !asg(1, j) && !asg(2, k)) return;
+    // the plane entry of an axis' low or high end for the line through (o1, o2)
+    auto plane = [&](int ax, bool lo, size_t pl) -> const double * {
+        if (((ax == 0) ? P.bcs[0] : (ax == 1) ? P.bcs[1] : P.bcs[2]) == 1) return nullptr;
+        const int e = 2 * ax;
+        const long o = pres ? (lo ? P.L.off[18 + e] : P.L.off[19 + e])
+                       : (c == 0) ? (lo ? P.L.off[e] : P.L.off[e + 1])
+                       : (c == 1) ? (lo ? P.L.off[6 + e] : P.L.off[7 + e])
+                                  : (lo ? P.L.off[12 + e] : P.L.off[13 + e]);
+        return P.planes + o + pl;
+    };
+    auto vx = [&](int i_, int j_, int k_) -> double {
+        if (asg(0, i_))
+            return fill_rule(pres, c == 0, P.bcs[0], i_ == 1, n0, P.ds[0],
+                             plane(0, i_ == 1, (size_t)(j_ - 1) + (size_t)(k_ - 1) * n1),
+                             [&](int m) { return A[FIX(m, j_, k_, n0, n1)]; });
+        return A[FIX(i_, j_, k_, n0, n1)];
+    };
+    auto vy = [&](int i_, int j_, int k_) -> double {
+        if (asg(1, j_))
+            return fill_rule(pres, c == 1, P.bcs[1], j_ == 1, n1, P.ds[1],
+                             plane(1, j_ == 1, (size_t)(i_ - 1) + (size_t)(k_ - 1) * n0),
+                             [&](int m) { return vx(i_, m, k_); });
+        return vx(i_, j_, k_);
+    };
+    double v;
+    if (asg(2, k))
+        v = fill_rule(pres, c == 2, P.bcs[2], k == 1, n2, P.ds[2],
+                      plane(2, k == 1, (size_t)(i - 1) + (size_t)(j - 1) * n0), [&](int m) { return vy(i, j, m); });
+    else
+        v = vy(i, j, k);
+    A[FIX(i, j, k, n0, n1)] = v;
+}
+
+__global__ void __launch_bounds__(256) k_whole_fill(BCParams P) {
+    whole_fill_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) k_whole_fill_batch(const BCParams *__restrict__ Ps,
+                                                          const int *__restrict__ prefix, int nbox) {
+    int blk;
+    const BCParams &P = batch_params(Ps, prefix, nbox, blk);
+    whole_fill_body(P, (size_t)blk * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
 __device__ __forceinline__ void other_axes(int ax, int &o1, int &o2) {
     o1 = (ax == 0) ? 1 : 0;
     o2 = (ax == 2) ? 1 : 2;
-}
-
-// velocity ghost/wall fill along one axis (NS_functions.f90:235-308), all three
-// components; blockIdx.y selects the component
-__device__ __forceinline__ void vel_bc_body(const BCParams &P, const size_t t, const int c) {
-    Grid g{P.nx, P.ny, P.nz};
-    int o1, o2;
-    other_axes(P.ax, o1, o2);
-    {
-        int n[3];
-        comp_dims(g, c, n);
-        size_t tot = (size_t)n[o1] * n[o2];
-        if (t >= tot) return;
-        int pp = (int)(t % n[o1]) + 1, qq = (int)(t / n[o1]) + 1;
-        int i1[3], i2[3];
-        i1[o1] = i2[o1] = pp;
-        i1[o2] = i2[o2] = qq;
-        double *A = P.a[c];
-        const int N = n[P.ax];
-        const bool normal = (c == P.ax);
-        const double *b1 = P.planes ? P.planes + P.L.off[6 * c + 2 * P.ax] : nullptr;
-        const double *bn = P.planes ? P.planes + P.L.off[6 * c + 2 * P.ax + 1] : nullptr;
-        const size_t pl = t;
-#define AT(ix) A[FIX(ix[0], ix[1], ix[2], n[0], n[1])]
-        if (P.bc == 1) {
-            if (normal) { i1[P.ax] = N; i2[P.ax] = 1; AT(i1) = AT(i2); }
-            else {
-                i1[P.ax] = 1; i2[P.ax] = N - 1; AT(i1) = AT(i2);
-                i1[P.ax] = N; i2[P.ax] = 2; AT(i1) = AT(i2);
-            }
-        } else if (P.bc == 2) {
-            if (normal) {
-                i1[P.ax] = 1; AT(i1) = b1[pl];
-                i1[P.ax] = N; AT(i1) = bn[pl];
-            } else {
-                i1[P.ax] = 1; i2[P.ax] = 2; AT(i1) = 2 * b1[pl] - AT(i2);
-                i1[P.ax] = N; i2[P.ax] = N - 1; AT(i1) = 2 * bn[pl] - AT(i2);
-            }
-        } else if (P.bc == 4) {
-            i1[P.ax] = 1; AT(i1) = b1[pl];
-            i1[P.ax] = N; AT(i1) = bn[pl];
-        }
-    }
-}
-
-__global__ void __launch_bounds__(256) k_vel_bc_axis(BCParams P) {
-    vel_bc_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
-}
-
-__global__ void __launch_bounds__(256) k_vel_bc_axis_batch(const BCParams *__restrict__ Ps, const int *__restrict__ prefix,
-                                                           int nbox) {
-    int blk;
-    const BCParams &P = batch_params(Ps, prefix, nbox, blk);
-    vel_bc_body(P, (size_t)blk * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 // boundary rows before an ADI sweep (vel_bc_staggered_CN2, NS_functions.f90:313-410)
@@ -257,46 +327,6 @@ __global__ void __launch_bounds__(256) k_cn2_axis_batch(const BCParams *__restri
     int blk;
     const BCParams &P = batch_params(Ps, prefix, nbox, blk);
     cn2_body(P, (size_t)blk * blockDim.x + threadIdx.x);
-}
-
-// pressure ghost fill along one axis (pr_bc_staggered, NS_functions.f90:1416-1462)
-__device__ __forceinline__ void pr_bc_body(const BCParams &P, const size_t t) {
-    int n[3] = {P.nx + 2, P.ny + 2, P.nz + 2};
-    int o1, o2;
-    other_axes(P.ax, o1, o2);
-    if (t >= (size_t)n[o1] * n[o2]) return;
-    int i1[3], i2[3];
-    i1[o1] = i2[o1] = (int)(t % n[o1]) + 1;
-    i1[o2] = i2[o2] = (int)(t / n[o1]) + 1;
-    double *A = P.a[0];
-    const int N = n[P.ax];
-    const double *b1 = (P.bc != 1) ? P.planes + P.L.off[18 + 2 * P.ax] : nullptr;
-    const double *bn = (P.bc != 1) ? P.planes + P.L.off[19 + 2 * P.ax] : nullptr;
-    if (P.bc == 1) {
-        i1[P.ax] = 1; i2[P.ax] = N - 1; AT(i1) = AT(i2);
-        i1[P.ax] = N; i2[P.ax] = 2; AT(i1) = AT(i2);
-    } else if (P.bc == 2) {
-        i1[P.ax] = 1; i2[P.ax] = 2; AT(i1) = 2 * b1[t] - AT(i2);
-        i1[P.ax] = N; i2[P.ax] = N - 1; AT(i1) = 2 * bn[t] - AT(i2);
-    } else if (P.bc == 3) {
-        i1[P.ax] = 1; i2[P.ax] = 2; AT(i1) = -P.d * b1[t] + AT(i2);
-        i1[P.ax] = N; i2[P.ax] = N - 1; AT(i1) = P.d * bn[t] + AT(i2);
-    } else if (P.bc == 4) {
-        i1[P.ax] = 1; AT(i1) = b1[t];
-        i1[P.ax] = N; AT(i1) = bn[t];
-    }
-#undef AT
-}
-
-__global__ void __launch_bounds__(256) k_pr_bc_axis(BCParams P) {
-    pr_bc_body(P, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ void __launch_bounds__(256) k_pr_bc_axis_batch(const BCParams *__restrict__ Ps, const int *__restrict__ prefix,
-                                                          int nbox) {
-    int blk;
-    const BCParams &P = batch_params(Ps, prefix, nbox, blk);
-    pr_bc_body(P, (size_t)blk * blockDim.x + threadIdx.x);
 }
 
 // Poisson RHS boundary terms along one axis (NS_functions.f90:1362-1398)
@@ -737,6 +767,38 @@ void launch_periodic_fill(const Grid &g, int na, const int comp[3], double *cons
 
 }  // namespace
 
+int make_fill_params(const Grid &g, bool pres, const int bc[3], const double *planes, const double d[3], double *a0,
+                     double *a1, double *a2, int first_axis, void *out) {
+    BCParams &P = *(BCParams *)out;
+    P = BCParams{};
+    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.L = plane_layout(g.nx, g.ny, g.nz);
+    P.planes = planes;
+    P.a[0] = a0; P.a[1] = a1; P.a[2] = a2;
+    P.pbc_mode = pres ? 4 : 3;
+    P.ax = first_axis;
+    bool any = false;
+    for (int a = 0; a < 3; a++) {
+        P.bcs[a] = bc[a];
+        P.ds[a] = d[a];
+        if (a >= first_axis && (pres || bc[a] != 3)) any = true;
+    }
+    if (!any) return 0;
+    size_t mx = 0;
+    for (int c = 0; c < (pres ? 1 : 3); c++) {
+        int n[3];
+        comp_dims(g, pres ? 3 : c, n);
+        const size_t m = surface_size(n);
+        mx = m > mx ? m : mx;
+    }
+    return (int)nblocks(mx);
+}
+
+void launch_whole_fill_batch(const void *params, const int *prefix, int nbox, int nblk, int ncomp, hipStream_t st) {
+    hipLaunchKernelGGL(k_whole_fill_batch, dim3(nblk, ncomp), dim3(256), 0, st, (const BCParams *)params, prefix,
+                       nbox);
+}
+
 void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w,
                    hipStream_t st, int first_axis) {
     if (bc[0] == 1 && bc[1] == 1 && bc[2] == 1) {
@@ -746,18 +808,9 @@ void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double 
         return;
     }
     BCParams P;
-    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
-    P.L = plane_layout(g.nx, g.ny, g.nz);
-    P.planes = planes;
-    P.a[0] = u; P.a[1] = v; P.a[2] = w;
-    P.pbc_mode = 0;
-    P.d = 0;
-    for (int ax = first_axis; ax < 3; ax++) {
-        P.ax = ax;
-        P.bc = bc[ax];
-        if (bc[ax] == 3) continue;
-        hipLaunchKernelGGL(k_vel_bc_axis, dim3(nblocks(max_plane(g, ax)), 3), dim3(256), 0, st, P);
-    }
+    const double d[3] = {0.0, 0.0, 0.0};
+    const int nb = make_fill_params(g, false, bc, planes, d, u, v, w, first_axis, &P);
+    if (nb) hipLaunchKernelGGL(k_whole_fill, dim3(nb, 3), dim3(256), 0, st, P);
 }
 
 int make_cn2_params(const Grid &g, int bc, int ax, const double *planes, double *u, double *v, double *w, void *out) {
@@ -793,20 +846,9 @@ void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double 
         return;
     }
     BCParams P;
-    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
-    P.L = plane_layout(g.nx, g.ny, g.nz);
-    P.planes = planes;
-    P.a[0] = dp; P.a[1] = nullptr; P.a[2] = nullptr;
-    P.pbc_mode = 2;
-    double d[3] = {dx, dy, dz};
-    int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
-    for (int ax = first_axis; ax < 3; ax++) {
-        P.ax = ax;
-        P.bc = pbc[ax];
-        P.d = d[ax];
-        int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
-        hipLaunchKernelGGL(k_pr_bc_axis, dim3(nblocks((size_t)n[o1] * n[o2])), dim3(256), 0, st, P);
-    }
+    const double d[3] = {dx, dy, dz};
+    const int nb = make_fill_params(g, true, pbc, planes, d, dp, nullptr, nullptr, first_axis, &P);
+    if (nb) hipLaunchKernelGGL(k_whole_fill, dim3(nb, 1), dim3(256), 0, st, P);
 }
 
 void launch_modify_rhs(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz,
@@ -970,36 +1012,6 @@ void launch_subbox_mean(const Grid &gp, const Grid &gs, const int o[3], const do
 size_t bc_params_size() { return sizeof(BCParams); }
 size_t proj_params_size() { return sizeof(ProjParams); }
 
-int make_vel_bc_params(const Grid &g, const int bc[3], const double *planes, double *u, double *v, double *w, int ax,
-                       void *out) {
-    BCParams &P = *(BCParams *)out;
-    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
-    P.L = plane_layout(g.nx, g.ny, g.nz);
-    P.planes = planes;
-    P.a[0] = u; P.a[1] = v; P.a[2] = w;
-    P.pbc_mode = 0;
-    P.d = 0;
-    P.ax = ax;
-    P.bc = bc[ax];
-    return bc[ax] == 3 ? 0 : (int)nblocks(max_plane(g, ax));
-}
-
-int make_pr_bc_params(const Grid &g, const int pbc[3], const double *planes, const double d[3], double *dp, int ax,
-                      void *out) {
-    BCParams &P = *(BCParams *)out;
-    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
-    P.L = plane_layout(g.nx, g.ny, g.nz);
-    P.planes = planes;
-    P.a[0] = dp; P.a[1] = nullptr; P.a[2] = nullptr;
-    P.pbc_mode = 2;
-    P.ax = ax;
-    P.bc = pbc[ax];
-    P.d = d[ax];
-    const int n[3] = {g.nx + 2, g.ny + 2, g.nz + 2};
-    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
-    return (int)nblocks((size_t)n[o1] * n[o2]);
-}
-
 int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag, const double *us, const double *vs,
                         const double *ws, const double *dp, const double *gauge, double *u, double *v, double *w,
                         double *p, void *out) {
@@ -1021,12 +1033,6 @@ int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag
     return (int)round8(nblocks(g.np(), 256 * PROJ_PER));
 }
 
-void launch_vel_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_vel_bc_axis_batch, dim3(nblk, 3), dim3(256), 0, st, (const BCParams *)params, prefix, nbox);
-}
-void launch_pr_bc_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_pr_bc_axis_batch, dim3(nblk), dim3(256), 0, st, (const BCParams *)params, prefix, nbox);
-}
 void launch_project_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_project_batch, dim3(nblk, 4), dim3(256), 0, st, (const ProjParams *)params, prefix, nbox);
 }
