@@ -1849,7 +1849,8 @@ struct ns_batch {
         int *prefix = nullptr;
         int nblk = 0;
         size_t lds = 0;
-        int variant = 0;  // DCT pass: kernel variant (OR over the boxes)
+        int variant = 0;  // DCT pass: kernel variant (OR over the boxes); sweep: waves per block
+        int nbox = 0;     // boxes of this launch (a subset for sweeps grouped by blocks per line)
     };
     std::vector<Phase> phases;  // one steady (non-first) step
     hipGraphExec_t graph = nullptr;
@@ -1894,8 +1895,15 @@ bool batch_sweep_partitioned(const ns_batch *b, int ax) {
 }
 
 // parameters of every box for one launch, uploaded with their block prefix
-int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_batch::Phase> &out) {
-    const int n = (int)b->ctx.size();
+// sel: the boxes of this launch (null: all)
+int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_batch::Phase> &out,
+              const std::vector<int> *sel = nullptr) {
+    std::vector<int> all;
+    if (!sel) {
+        for (int i = 0; i < (int)b->ctx.size(); i++) all.push_back(i);
+        sel = &all;
+    }
+    const int n = (int)sel->size();
     size_t psz = kind == 0 ? predict_params_size() : kind == 2 ? pass_params_size()
                            : kind == 4 ? proj_params_size() : kind == 6 ? sweep_params_size() : bc_params_size();
     std::vector<char> host((size_t)n * psz);
@@ -1908,18 +1916,20 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     // chunk re-reads 3 slices)
     long cols = 0;
     if (kind == 0)
-        for (ns_ctx *c : b->ctx) cols += predict_tile_columns(c->cfg);
+        for (int bi : *sel) cols += predict_tile_columns(b->ctx[bi]->cfg);
     // sweeps: one launch shape for all boxes, the partitioned solve only if every box has it
     const bool part = kind == 6 && batch_sweep_partitioned(b, arg);
     for (int i = 0; i < n; i++) {
-        ns_ctx *c = b->ctx[i];
+        const int bi = (*sel)[i];
+        ns_ctx *c = b->ctx[bi];
+        const int cls = b->cls[bi];
         const ns_config &f = c->cfg;
         const Grid &g = c->g;
         const double d[3] = {f.dx, f.dy, f.dz};
         void *dst = host.data() + (size_t)i * psz;
         int nb = 0;
         if (kind == 0) {
-            const long z = cols / ((long)predict_resident_blocks(false) * b->knobs[b->cls[i]].rounds);
+            const long z = cols / ((long)predict_resident_blocks(false) * b->knobs[cls].rounds);
             const int zc_hint = (int)(z > 8 ? z : 8);
             nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr, c->p,
                                      c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst, zc_hint);
@@ -1930,7 +1940,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
             size_t l = 0;
             nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, c->us, c->vs, c->ws, nullptr, c->planes,
                                       c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l,
-                                      b->knobs[b->cls[i]].dct_drop);
+                                      b->knobs[cls].dct_drop);
             lds = l > lds ? l : lds;
             variant |= dct_params_variant(dst);
         } else if (kind == 5) {
@@ -1938,7 +1948,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         } else if (kind == 6) {
             size_t l = 0;
             nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l,
-                                   b->knobs[b->cls[i]].adix_lsh, part, part ? c->planes : nullptr, &c->L);
+                                   b->knobs[cls].adix_lsh, part, part ? c->planes : nullptr, &c->L);
             lds = l > lds ? l : lds;
             const int kw = sweep_params_waves(dst);
             variant = kw > variant ? kw : variant;
@@ -1958,6 +1968,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     ph.nblk = prefix[n];
     ph.lds = lds;
     ph.variant = variant;
+    ph.nbox = n;
     int rc;
     if ((rc = balloc(&ph.params, host.size())) || (rc = balloc(&ph.prefix, prefix.size() * sizeof(int)))) {
         if (ph.params) (void)hipFree(ph.params);
@@ -1977,7 +1988,19 @@ int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out
         // sweep reads the boundary rows from the planes itself
         for (int ax = 0; ax < 3; ax++) {
             if (!batch_sweep_partitioned(b, ax) && (rc = add_phase(b, 5, ax, first_step, out))) return rc;
-            if ((rc = add_phase(b, 6, ax, first_step, out))) return rc;
+            if (!batch_sweep_partitioned(b, ax)) {
+                if ((rc = add_phase(b, 6, ax, first_step, out))) return rc;
+                continue;
+            }
+            // one launch per blocks-per-line count: each sized (waves, LDS) for its own boxes
+            for (int K = 1; K <= PART_MAX_BLOCKS; K++) {
+                std::vector<int> sel;
+                for (int i = 0; i < (int)b->ctx.size(); i++) {
+                    const LineSys &sa = b->ctx[i]->cn[ax][0], &sb = b->ctx[i]->cn[ax][1];
+                    if ((sa.K > sb.K ? sa.K : sb.K) == K) sel.push_back(i);
+                }
+                if (!sel.empty() && (rc = add_phase(b, 6, ax, first_step, out, &sel))) return rc;
+            }
         }
     } else if ((rc = add_phase(b, 1, 0, first_step, out))) {  // the whole velocity fill
         return rc;
@@ -1989,8 +2012,8 @@ int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out
 }
 
 void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
-    const int n = (int)b->ctx.size();
     for (const auto &ph : phases) {
+        const int n = ph.nbox;
         switch (ph.kind) {
             case 0: launch_predict_batch(false, ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 1: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 3, b->st); break;
