@@ -59,6 +59,9 @@ CASES = [
     ("walls_cn_first", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), True),
     ("walls_cn_mixed_dst", lambda: box(10, 8, 9, bc=(2, 4, 4), pbc=(3, 2, 2), scheme=AB2_CN, dp_flag=1), False),
     ("walls_cn_long_lines", lambda: box(40, 6, 36, bc=(4, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), False),
+    # 9 partitioned blocks per x line; y lines over 448 rows (twisted solve)
+    ("walls_cn_k9_lines", lambda: box(250, 5, 4, bc=(2, 2, 4), pbc=(3, 3, 2), scheme=AB2_CN, dp_flag=1), False),
+    ("walls_cn_twisted_lines", lambda: box(4, 480, 3, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1), False),
     # periodic CN lines: the reference's row-1 coupling (default) and the cyclic operator
     ("periodic_cn", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=1), False),
     ("periodic_cn_first", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=1), True),

@@ -205,12 +205,15 @@ def test_step_walls(oracle, scheme, bc, pbc):
         assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
 
 
-@pytest.mark.parametrize("n", [(126, 12, 10), (12, 126, 10), (12, 10, 126)], ids=["long_x", "long_y", "long_z"])
+@pytest.mark.parametrize("n", [(126, 12, 10), (12, 126, 10), (12, 10, 126), (250, 6, 5), (6, 250, 5), (6, 5, 250),
+                               (480, 4, 3), (4, 480, 3)],
+                         ids=["long_x", "long_y", "long_z", "k9_x", "k9_y", "k9_z", "twisted_x", "twisted_y"])
 def test_step_walls_cn_long_lines(oracle, n):
-    """AB2-CN on walls with 126-cell lines along one axis: the x sweep's LDS
-    staging (32 lines per block, batched copies), the y/z sweeps' twisted
-    halves over several register chunks, and the DCT passes on long lines
-    with odd radices, against the oracle after 3 steps."""
+    """AB2-CN on walls with long lines along one axis: the partitioned sweeps
+    with 5 blocks per line (126 cells) and 9 (250 cells; the x sweep's LDS
+    staging of 64 lines), the twisted solve of lines over 448 rows (480
+    cells), and the DCT passes on long lines with odd radices, against the
+    oracle after 3 steps."""
     cfg = cfg_box(*n, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1).replace(dt=1e-3)
     u, v, w, p = rand_fields(cfg, 91)
     u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
