@@ -205,15 +205,19 @@ __global__ void __launch_bounds__(AT) k_adj_take_interior(Grid g, const double *
 // pr_bc_staggered_modify_rhs transposed (NS_functions.f90:1362-1398): one axis
 struct AdjRhs {
     Grid g;
-    int ax, pbc;
-    double d;
+    int pbc[3];
+    double d[3];
     PlaneLayout L;
     const double *rb;  // adjoint RHS (interior)
     double *pbar;
 };
+// blockIdx.y = axis (the three axes write disjoint planes)
 __global__ void __launch_bounds__(AT) k_adj_modify_rhs(AdjRhs P) {
+    const int ax = blockIdx.y;
+    const int pbc = P.pbc[ax];
+    if (pbc == 1) return;
     const int n[3] = {P.g.nx, P.g.ny, P.g.nz}, np_[3] = {P.g.nx + 2, P.g.ny + 2, P.g.nz + 2};
-    const int ax = P.ax, o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
     const size_t t = (size_t)blockIdx.x * AT + threadIdx.x;
     if (t >= (size_t)n[o1] * n[o2]) return;
     const int pp = (int)(t % n[o1]) + 1, qq = (int)(t / n[o1]) + 1;
@@ -225,14 +229,19 @@ __global__ void __launch_bounds__(AT) k_adj_modify_rhs(AdjRhs P) {
     const size_t pl = (size_t)pp + (size_t)np_[o1] * (size_t)qq;
     const double r1 = P.rb[FIX(i1[0], i1[1], i1[2], P.g.nx, P.g.ny)], rn = P.rb[FIX(iN[0], iN[1], iN[2], P.g.nx, P.g.ny)];
     double *b1 = P.pbar + P.L.off[18 + 2 * ax], *bn = P.pbar + P.L.off[19 + 2 * ax];
-    const double d = P.d, d2 = d * d;
-    if (P.pbc == 2) { b1[pl] += -2.0 * r1 / d2; bn[pl] += -2.0 * rn / d2; }
-    else if (P.pbc == 3) { b1[pl] += r1 / d; bn[pl] += -rn / d; }
-    else if (P.pbc == 4) { b1[pl] += -r1 / d2; bn[pl] += -rn / d2; }
+    const double d = P.d[ax], d2 = d * d;
+    if (pbc == 2) { b1[pl] += -2.0 * r1 / d2; bn[pl] += -2.0 * rn / d2; }
+    else if (pbc == 3) { b1[pl] += r1 / d; bn[pl] += -rn / d; }
+    else if (pbc == 4) { b1[pl] += -r1 / d2; bn[pl] += -rn / d2; }
 }
 
 // u* adjoint += div^T(r adjoint): r(a,b,c) = sum_c' (s(face+1) - s(face))/h / dt
-__global__ void __launch_bounds__(AT) k_adj_div(Grid g, int c, double rh_dt, const double *rb, double *usb) {
+// blockIdx.y = component c (uniform: the parameter selections below are scalar)
+__global__ void __launch_bounds__(AT) k_adj_div(Grid g, double3 rh_dt3, const double *rb, double *usb0, double *usb1,
+                                              double *usb2) {
+    const int c = blockIdx.y;
+    const double rh_dt = (c == 0) ? rh_dt3.x : (c == 1) ? rh_dt3.y : rh_dt3.z;
+    double *usb = (c == 0) ? usb0 : (c == 1) ? usb1 : usb2;
     int n[3];
     dims_of(g, c, n);
     const size_t tot = (size_t)n[0] * n[1] * n[2];
@@ -270,7 +279,9 @@ struct AdjPred {
     double *pb;                          // adjoint of p (accumulated)
 };
 
-__global__ void __launch_bounds__(AT) k_adj_pred_prep(AdjPred P, int c) {
+// blockIdx.y = component c
+__global__ void __launch_bounds__(AT) k_adj_pred_prep(AdjPred P) {
+    const int c = blockIdx.y;
     int n[3];
     dims_of(P.g, c, n);
     const size_t tot = (size_t)n[0] * n[1] * n[2];
@@ -694,22 +705,20 @@ void launch_adj_modify_rhs(const Grid &g, const int pbc[3], const double h[3], c
     R.L = plane_layout(g.nx, g.ny, g.nz);
     R.rb = rb;
     R.pbar = pbar;
-    const int n[3] = {g.nx, g.ny, g.nz};
+    if (!pbar || (pbc[0] == 1 && pbc[1] == 1 && pbc[2] == 1)) return;
     for (int ax = 0; ax < 3; ax++) {
-        if (pbc[ax] == 1 || !pbar) continue;
-        R.ax = ax;
-        R.pbc = pbc[ax];
-        R.d = h[ax];
-        const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
-        hipLaunchKernelGGL(k_adj_modify_rhs, dim3(ablocks((size_t)n[o1] * n[o2])), dim3(AT), 0, st, R);
+        R.pbc[ax] = pbc[ax];
+        R.d[ax] = h[ax];
     }
+    const size_t mx = std::max((size_t)g.ny * g.nz, std::max((size_t)g.nx * g.nz, (size_t)g.nx * g.ny));
+    hipLaunchKernelGGL(k_adj_modify_rhs, dim3(ablocks(mx), 3), dim3(AT), 0, st, R);
 }
 
 void launch_adj_div(const Grid &g, const double h[3], double dt, const double *rb, double *const usb[3],
                     hipStream_t st) {
-    const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
-    for (int c = 0; c < 3; c++)
-        hipLaunchKernelGGL(k_adj_div, dim3(ablocks(tot[c])), dim3(AT), 0, st, g, c, 1.0 / (h[c] * dt), rb, usb[c]);
+    const size_t mx = std::max(g.nu(), std::max(g.nv(), g.nw()));
+    const double3 r = make_double3(1.0 / (h[0] * dt), 1.0 / (h[1] * dt), 1.0 / (h[2] * dt));
+    hipLaunchKernelGGL(k_adj_div, dim3(ablocks(mx), 3), dim3(AT), 0, st, g, r, rb, usb[0], usb[1], usb[2]);
 }
 
 void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
@@ -808,7 +817,8 @@ void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, i
     }
     P.split = (gcb[0] != gb[0]) ? 1 : 0;
     const size_t tot[3] = {g.nu(), g.nv(), g.nw()};
-    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_adj_pred_prep, dim3(ablocks(tot[c])), dim3(AT), 0, st, P, c);
+    hipLaunchKernelGGL(k_adj_pred_prep, dim3(ablocks(std::max(tot[0], std::max(tot[1], tot[2]))), 3), dim3(AT), 0, st,
+                       P);
     if (dp_flag) hipLaunchKernelGGL(k_adj_pred_p, dim3(ablocks(g.np())), dim3(AT), 0, st, P);
     // the box's surface faces, then the z columns of the others: chunks of cz
     // planes, about one round of 4 resident tiles per CU
