@@ -2185,9 +2185,12 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
     // coordinate descent over (size class, knob); a change is kept when it
     // saves more than 1% of the step
     const int cand_rounds[4] = {4, 8, 16, 32}, cand_drop[3] = {0, 1, 2}, cand_lsh[4] = {6, 5, 4, 3};
+    // the x sweep's lines per block only shape the twisted / cyclic sweeps
+    const bool x_twisted = b->ctx[0]->cfg.scheme == NS_AB2_CN && !batch_sweep_partitioned(b, 0);
     for (int k = 0; k < 3 && !rc; k++) {
         if (!present[k]) continue;
         for (int knob = 0; knob < 3 && !rc; knob++) {
+            if (knob == 2 && !x_twisted) continue;
             int *val = knob == 0 ? &b->knobs[k].rounds : knob == 1 ? &b->knobs[k].dct_drop : &b->knobs[k].adix_lsh;
             const int *cand = knob == 0 ? cand_rounds : knob == 1 ? cand_drop : cand_lsh;
             const int nc = knob == 1 ? 3 : 4;

@@ -4,23 +4,27 @@
 // re_simulation_sub_tstep_interBC_combine.f90:656-701).
 //
 // Matrices (mat_CN NS_functions.f90:1245-1294, mat_CN_tridiagonal :1297-1342):
-// walls (bc 2/4) are plain tridiagonal and solved by Thomas over the whole
-// line with host-factored coefficients.  Periodic lines are reduced to the n
-// owned unknowns (the ghost rows of the dense matrix are u_N = u_1 + r_N,
-// v_1 = v_{n+1} + r_1, v_{n+2} = v_2 + r_{n+2}), which leaves a cyclic
-// tridiagonal system solved by Thomas plus a Sherman-Morrison rank-one
-// correction whose line-independent part is precomputed on the host.
+// walls (bc 2/4) are plain tridiagonal; every line of a direction and
+// component has the same matrix, so what depends on the matrix alone is
+// computed on the host.  Wall lines of up to PART_MAX_BLOCKS * PART_ROWS rows
+// use the partitioned solve (one wave per block of rows, couplings through
+// host-precomputed spikes); longer ones a twisted Thomas factorization (two
+// lanes per line).  Periodic lines are reduced to the n owned unknowns (the
+// ghost rows of the dense matrix are u_N = u_1 + r_N, v_1 = v_{n+1} + r_1,
+// v_{n+2} = v_2 + r_{n+2}), which leaves a cyclic tridiagonal system solved by
+// Thomas plus a Sherman-Morrison rank-one correction whose line-independent
+// part is precomputed on the host.
 #include "ns_internal.h"
 
 namespace ns {
 
 namespace {
 
-// One thread solves one line, so each elimination step would wait for its
-// own loads: lines and coefficients are read in register chunks of CHK
-// elements (one wait per chunk), the next chunk's data issued before the
-// current chunk's dependent arithmetic.
-constexpr int SB = 16;  // staging copies in flight per thread
+// Twisted and cyclic solves: one or two lanes walk a whole line, so each
+// elimination step would wait for its own loads: lines and coefficients are
+// read in register chunks of CHK elements (one wait per chunk), the next
+// chunk's data issued before the current chunk's dependent arithmetic.
+constexpr int SB = 16;  // staging copies in flight per thread (x lines staged in LDS)
 
 // forward elimination over y[0..m) (stride s): d_i = (r_i - lo_i d_{i-1}) inv_i,
 // stored to dout (stride so; y itself for an in-place solve); r_0 and (m > 1)
