@@ -211,10 +211,16 @@ std::vector<double> thomas_solve(const std::vector<double> &lo, const std::vecto
 // e_last): x = y - x_{s_k - 1} V - x_{e_k + 1} W.  The 2K end values solve
 // (I + M) z = ends, M holding the spikes' end entries; G = (I + M)^{-1}.
 // Every table depends only on the matrix, which is the same for every line.
+// cyclic: the system also couples row 0 to row N-1 (alpha) and row N-1 to
+// row 0 (beta), so block 0's left neighbour is block K-1 and block K-1's
+// right neighbour block 0 (the same spikes, with the corners as couplings).
 // Returns K (0: the line is too long for PART_MAX_BLOCKS blocks) and fills
 // tab = loL | invL | cpL | V | W (N each) | G (2K x 2K, row-major).
+// (The fewest blocks that fit the rows: more, shorter blocks fill the device
+// better for a single small box but cost every lane a longer coupling sum,
+// which slowed batches of many small boxes by 14%.)
 int part_tables(const std::vector<double> &lo, const std::vector<double> &d, const std::vector<double> &up,
-                std::vector<double> &tab) {
+                std::vector<double> &tab, bool cyclic = false, double alpha = 0.0, double beta = 0.0) {
     const int N = (int)d.size();
     const int K = (N + PART_ROWS - 1) / PART_ROWS;
     if (K > PART_MAX_BLOCKS || N < 2 * K) return 0;
@@ -241,21 +247,22 @@ int part_tables(const std::vector<double> &lo, const std::vector<double> &d, con
     std::vector<double> M((size_t)K2 * K2, 0.0);
     for (int k = 0; k < K; k++) {
         const int a = start(k), b = start(k + 1), L = b - a;
-        if (k > 0) {
+        const int kl = (k > 0) ? k - 1 : K - 1, kr = (k < K - 1) ? k + 1 : 0;
+        if (k > 0 || cyclic) {
             std::vector<double> e(L, 0.0);
-            e[0] = lo[a];
+            e[0] = (k > 0) ? lo[a] : alpha;
             e = lsolve(k, e);
             for (int j = 0; j < L; j++) V[a + j] = e[j];
-            M[(size_t)(2 * k) * K2 + 2 * k - 1] += e[0];
-            M[(size_t)(2 * k + 1) * K2 + 2 * k - 1] += e[L - 1];
+            M[(size_t)(2 * k) * K2 + 2 * kl + 1] += e[0];
+            M[(size_t)(2 * k + 1) * K2 + 2 * kl + 1] += e[L - 1];
         }
-        if (k < K - 1) {
+        if (k < K - 1 || cyclic) {
             std::vector<double> e(L, 0.0);
-            e[L - 1] = up[b - 1];
+            e[L - 1] = (k < K - 1) ? up[b - 1] : beta;
             e = lsolve(k, e);
             for (int j = 0; j < L; j++) W[a + j] = e[j];
-            M[(size_t)(2 * k) * K2 + 2 * k + 2] += e[0];
-            M[(size_t)(2 * k + 1) * K2 + 2 * k + 2] += e[L - 1];
+            M[(size_t)(2 * k) * K2 + 2 * kr] += e[0];
+            M[(size_t)(2 * k + 1) * K2 + 2 * kr] += e[L - 1];
         }
     }
     // G = (I + M)^{-1} by Gauss-Jordan with partial pivoting
@@ -357,6 +364,15 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         int rc = dstore(c, old.z, &dz_, z);
         if (rc) return rc;
         S.z = dz_;
+        // partitioned solve of the cyclic system itself (lines of at most
+        // PART_MAX_BLOCKS * PART_ROWS unknowns)
+        std::vector<double> tab;
+        S.K = part_tables(lo, d, up, tab, true, alpha, beta);
+        if (S.K) {
+            double *dt2;
+            if ((rc = dstore(c, old.pt, &dt2, tab))) return rc;
+            S.pt = dt2;
+        }
     } else if (bc == 2 || bc == 4) {
         const int N = typeB ? n + 2 : n + 1;
         S.periodic = 0;

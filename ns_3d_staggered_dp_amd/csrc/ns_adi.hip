@@ -187,6 +187,10 @@ struct SweepParams {
     // value per line) and zeros to the arrays (the transposed boundary rows of
     // the adjoint), or null: the arrays take them
     double *wb[3][2];
+    // periodic lines: gz_in, the ghost rows' right-hand sides are zero (the
+    // rows vel_bc_staggered_CN2 zeroes; not read); gz_out, the ghost rows are
+    // stored as zeros (the transpose of that, after an adjoint solve)
+    int gz_in, gz_out;
 };
 
 // ---- partitioned wall solve (LineSys::K blocks per line, wave k = block k) ----
@@ -262,11 +266,13 @@ __device__ __forceinline__ void part_store(const double (&x)[PART_ROWS], int R, 
 }
 
 // couplings of block k from the line's 2K end values (ends[j * 64 + lane])
-__device__ __forceinline__ void part_couple(int k, int K, const double *G, const double *ends, int lane, double &Pk,
-                                            double &Qk) {
+// (cyclic: block 0's left neighbour is block K-1, block K-1's right block 0)
+__device__ __forceinline__ void part_couple(int k, int K, bool cyclic, const double *G, const double *ends, int lane,
+                                            double &Pk, double &Qk) {
     const int K2 = 2 * K;
-    const double *gp = G + (size_t)(k > 0 ? 2 * k - 1 : 0) * K2;
-    const double *gq = G + (size_t)(k < K - 1 ? 2 * k + 2 : 0) * K2;
+    const int kl = (k > 0) ? k - 1 : K - 1, kr = (k < K - 1) ? k + 1 : 0;
+    const double *gp = G + (size_t)(2 * kl + 1) * K2;
+    const double *gq = G + (size_t)(2 * kr) * K2;
     Pk = 0.0;
     Qk = 0.0;
     for (int j = 0; j < K2; j++) {
@@ -274,8 +280,61 @@ __device__ __forceinline__ void part_couple(int k, int K, const double *G, const
         Pk = Pk + gp[j] * e;
         Qk = Qk + gq[j] * e;
     }
-    if (k == 0) Pk = 0.0;
-    if (k == K - 1) Qk = 0.0;
+    if (!cyclic && k == 0) Pk = 0.0;
+    if (!cyclic && k == K - 1) Qk = 0.0;
+}
+
+// the line's first unknown from its 2K end values (row 0 of G: z = G ends)
+__device__ __forceinline__ double part_first(int K, const double *G, const double *ends, int lane) {
+    double z = 0.0;
+    for (int j = 0; j < 2 * K; j++) z = z + G[j] * ends[j * 64 + lane];
+    return z;
+}
+
+// Periodic lines (reduced cyclic system on the m owned unknowns, rows off ..
+// off+m-1 of the line; the ghost rows' right-hand sides glo = row 0 (type B)
+// and ghi = row off+m enter the end rows, as in solve_line).  Adds the folds
+// to block 0's first and block K-1's last row.
+__device__ __forceinline__ void part_fold(double (&y)[PART_ROWS], int R, int k, int K, const LineSys &S, double glo,
+                                          double ghi) {
+    double ff, fl;
+    if (S.adj) {
+        ff = ghi;
+        fl = S.off ? glo : 0.0;
+    } else {
+        ff = S.off ? S.a * glo : (S.quirk ? S.a * ghi : 0.0);
+        fl = S.a * ghi;
+    }
+#pragma unroll
+    for (int j = 0; j < PART_ROWS; j++) {
+        if (k == 0 && j == 0) y[j] = y[j] + ff;
+        if (k == K - 1 && j == R - 1) y[j] = y[j] + fl;
+    }
+}
+
+// the ghost rows of a periodic line after its solve (solve_line's outputs):
+// put(r, v) stores row r of the line; x0 / xl are the solution's first and
+// last unknowns where this block holds them (z0: the first from the end
+// values, for the transposed type-A row, which block K-1 writes)
+template <typename Put>
+__device__ __forceinline__ void part_ghosts(int k, int K, const LineSys &S, bool zero, double x0, double xl, double z0,
+                                            double glo, double ghi, Put put) {
+    const int m = S.m;
+    if (S.off == 0) {
+        if (S.adj) {
+            if (k == K - 1) put(m, zero ? 0.0 : ((S.quirk ? S.a * z0 : 0.0) + S.a * xl) + ghi);
+        } else if (k == 0) {
+            put(m, zero ? 0.0 : x0 + ghi);
+        }
+        return;
+    }
+    if (S.adj) {
+        if (k == 0) put(0, zero ? 0.0 : S.a * x0 + glo);
+        if (k == K - 1) put(m + 1, zero ? 0.0 : S.a * xl + ghi);
+    } else {
+        if (k == 0) put(m + 1, zero ? 0.0 : x0 + ghi);
+        if (k == K - 1) put(0, zero ? 0.0 : xl + glo);
+    }
 }
 
 // y / z sweeps: lane = line (consecutive x), so every row load of a wave is
@@ -289,7 +348,8 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
     const int o2 = (AX == 2) ? 1 : 2;
     const int stride = (AX == 1) ? n[0] : n[0] * n[1];
     const LineSys &S = P.sys[c];
-    const int K = S.K, N = S.N;
+    const bool per = S.periodic;
+    const int K = S.K, N = per ? S.m : S.N, off = per ? S.off : 0;  // N: unknowns per line
     const unsigned nl = (unsigned)n[0] * (unsigned)n[o2];
     if ((unsigned)blk * 64u >= nl) return;  // the whole block: this component has fewer lines
     const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -305,10 +365,17 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
         id[0] = (int)(t - q * (unsigned)n[0]) + 1;
         id[o2] = (int)q + 1;
     }
-    double *x = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]) + (size_t)stride * s0;
+    double *line = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]);
+    double *x = line + (size_t)stride * (off + s0);
     const double *tb = S.pt;
+    double glo = 0.0, ghi = 0.0;
+    if (per && act && valid && !P.gz_in) {
+        if (off) glo = line[0];
+        ghi = line[(size_t)stride * (off + N)];
+    }
     double y[PART_ROWS];
     part_load(y, R, valid, k, K, P.rb[c], t, [&](int j) { return x[(size_t)stride * j]; });
+    if (per) part_fold(y, R, k, K, S, glo, ghi);
     if (act) {
         const double yl = part_local(y, R, tb + s0, tb + N + s0, tb + 2 * N + s0);
         sm[(2 * k) * 64 + lane] = y[0];
@@ -317,12 +384,21 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
     __syncthreads();
     if (!act) return;
     double Pk, Qk;
-    part_couple(k, K, tb + 5 * N, sm, lane, Pk, Qk);
+    part_couple(k, K, per, tb + 5 * N, sm, lane, Pk, Qk);
     const double *V = tb + 3 * N + s0, *W = tb + 4 * N + s0;
+    double xl = 0.0;
 #pragma unroll
     for (int j = 0; j < PART_ROWS; j++)
-        if (j < R) y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
+        if (j < R) {
+            y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
+            if (j == R - 1) xl = y[j];
+        }
     part_store(y, R, valid, k, K, P.wb[c], t, [&](int j, double v) { x[(size_t)stride * j] = v; });
+    if (per && valid) {
+        const double z0 = (S.adj && S.off == 0 && k == K - 1) ? part_first(K, tb + 5 * N, sm, lane) : 0.0;
+        part_ghosts(k, K, S, P.gz_out, y[0], xl, z0, glo, ghi,
+                    [&](int r, double v) { line[(size_t)stride * r] = v; });
+    }
 }
 
 // x sweeps: the block stages its 64 lines (one contiguous span) in LDS rows
@@ -333,9 +409,10 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
     const int c = blockIdx.y;
     int n[3];
     comp_dims(P.g, c, n);
-    const int N = n[0];
+    const int N = n[0];  // line length (the LDS rows hold whole lines)
     const LineSys &S = P.sys[c];
-    const int K = S.K;
+    const bool per = S.periodic;
+    const int K = S.K, NU = per ? S.m : N, off = per ? S.off : 0;  // NU: unknowns per line
     const int nl = n[1] * n[2];
     const int l0 = blk * 64;
     if (l0 >= nl) return;  // the whole block: this component has fewer lines
@@ -362,27 +439,42 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
     const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const bool act = k < K;
-    const int s0 = act ? (k * N) / K : 0;
-    const int R = act ? ((k + 1) * N) / K - s0 : 0;
-    double *row = sm + (lane < cnt ? lane : 0) * LS + s0;
+    const int s0 = act ? (k * NU) / K : 0;
+    const int R = act ? ((k + 1) * NU) / K - s0 : 0;
+    double *lrow = sm + (lane < cnt ? lane : 0) * LS;  // the lane's whole line
+    double *row = lrow + off + s0;
     double *ends = sm + P.coff;
     const double *tb = S.pt;
+    double glo = 0.0, ghi = 0.0;
+    if (per && act && !P.gz_in) {
+        if (off) glo = lrow[0];
+        ghi = lrow[off + NU];
+    }
     double y[PART_ROWS];
     part_load(y, R, lane < cnt, k, K, P.rb[c], (unsigned)(l0 + lane), [&](int j) { return row[j]; });
+    if (per) part_fold(y, R, k, K, S, glo, ghi);
     if (act) {
-        const double yl = part_local(y, R, tb + s0, tb + N + s0, tb + 2 * N + s0);
+        const double yl = part_local(y, R, tb + s0, tb + NU + s0, tb + 2 * NU + s0);
         ends[(2 * k) * 64 + lane] = y[0];
         ends[(2 * k + 1) * 64 + lane] = yl;
     }
     __syncthreads();
     if (act) {
         double Pk, Qk;
-        part_couple(k, K, tb + 5 * N, ends, lane, Pk, Qk);
-        const double *V = tb + 3 * N + s0, *W = tb + 4 * N + s0;
+        part_couple(k, K, per, tb + 5 * NU, ends, lane, Pk, Qk);
+        const double *V = tb + 3 * NU + s0, *W = tb + 4 * NU + s0;
+        double xl = 0.0;
 #pragma unroll
         for (int j = 0; j < PART_ROWS; j++)
-            if (j < R) y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
+            if (j < R) {
+                y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
+                if (j == R - 1) xl = y[j];
+            }
         part_store(y, R, lane < cnt, k, K, P.wb[c], (unsigned)(l0 + lane), [&](int j, double v) { row[j] = v; });
+        if (per && lane < cnt) {
+            const double z0 = (S.adj && S.off == 0 && k == K - 1) ? part_first(K, tb + 5 * NU, ends, lane) : 0.0;
+            part_ghosts(k, K, S, P.gz_out, y[0], xl, z0, glo, ghi, [&](int r, double v) { lrow[r] = v; });
+        }
     }
     __syncthreads();
     for (int f0 = threadIdx.x; f0 < tot; f0 += nth * SB) {
@@ -611,14 +703,20 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
         }
         P.coff = (ax == 0) ? 64 * lsmax : 0;
         *lds = ((size_t)P.coff + (size_t)2 * P.kw * 64) * sizeof(double);
-        for (int c = 0; c < 3; c++) {
-            if (planes) {
-                P.rb[c][0] = planes + L->off[6 * c + 2 * ax];
-                P.rb[c][1] = planes + L->off[6 * c + 2 * ax + 1];
-            }
-            if (planes_out) {
-                P.wb[c][0] = planes_out + L->off[6 * c + 2 * ax];
-                P.wb[c][1] = planes_out + L->off[6 * c + 2 * ax + 1];
+        if (sysA.periodic) {
+            // the boundary rows of periodic lines are their zeroed ghost rows
+            P.gz_in = planes ? 1 : 0;
+            P.gz_out = planes_out ? 1 : 0;
+        } else {
+            for (int c = 0; c < 3; c++) {
+                if (planes) {
+                    P.rb[c][0] = planes + L->off[6 * c + 2 * ax];
+                    P.rb[c][1] = planes + L->off[6 * c + 2 * ax + 1];
+                }
+                if (planes_out) {
+                    P.wb[c][0] = planes_out + L->off[6 * c + 2 * ax];
+                    P.wb[c][1] = planes_out + L->off[6 * c + 2 * ax + 1];
+                }
             }
         }
         return (int)((nl + 63) / 64);
@@ -656,9 +754,7 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
     return (int)((nl + per - 1) / per);
 }
 
-bool sweep_partitioned(const LineSys &sysA, const LineSys &sysB) {
-    return !sysA.periodic && !sysB.periodic && sysA.K > 0 && sysB.K > 0;
-}
+bool sweep_partitioned(const LineSys &sysA, const LineSys &sysB) { return sysA.K > 0 && sysB.K > 0; }
 
 int sweep_params_waves(const void *params) { return ((const SweepParams *)params)->kw; }
 
