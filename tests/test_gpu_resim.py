@@ -199,6 +199,34 @@ def test_set_dt_in_lazy_periodic_run(oracle):
         assert np.abs(got[name] - ref).max() <= 1e-11 * np.abs(ref).max(), name
 
 
+@pytest.mark.parametrize("bc,pbc", [((2, 4, 2), (3, 2, 3)), ((1, 1, 1), (1, 1, 1)), ((1, 2, 2), (3, 3, 3))],
+                         ids=["walls", "periodic", "periodic_x_walls_yz"])
+def test_set_dt_with_cn_sweeps(oracle, bc, pbc):
+    """dt changes between AB2-CN steps: the CN line systems (partitioned and
+    cyclic tables) are rebuilt in place for the new dt, against the oracle"""
+    cfg = NSConfig(nx=20, ny=14, nz=12, dx=0.31, dy=0.27, dz=0.23, nu=0.02, dt=2e-3, scheme=AB2_CN, bc=bc, pbc=pbc,
+                   dp_flag=1)
+    u, v, w, p = fields.random_ghosted(cfg, 61)
+    u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
+    planes = 0.2 * np.random.default_rng(62).standard_normal(plane_set_length(cfg.nx, cfg.ny, cfg.nz))
+    plan = ((2e-3, 3), (1e-3, 4), (4e-3, 2))
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, first=True)
+        s.set_bc_planes(planes)
+        for dt, n in plan:
+            s.set_dt(dt)
+            s.step(n)
+        got = s.download()
+    st = oracle.OracleState(cfg, u, v, w, p, first=True)
+    for dt, n in plan:
+        st.cfg = cfg.replace(dt=dt)
+        for _ in range(n):
+            st.step(planes)
+    for name in ("u", "v", "w", "p"):
+        ref = getattr(st, name)
+        assert np.abs(got[name] - ref).max() <= 1e-11 * np.abs(ref).max(), name
+
+
 def test_extract_subbox_and_sub_file(tmp_path):
     """device sub-box extraction == the idx_* slices of the downloaded fields
     (bitwise, dp gauged), written as a reference sub-box group and read back"""
