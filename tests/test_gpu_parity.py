@@ -228,6 +228,20 @@ def cfg_periodic(nx, ny, nz, **kw):
                     pbc=(1, 1, 1), flags=FLAG_GAUGE_CORNER, **kw)
 
 
+@pytest.mark.parametrize("n", [(480, 4, 4), (6, 4, 480)], ids=["long_x", "long_z"])
+def test_step_periodic_cn_long_lines(oracle, n):
+    """AB2-CN on periodic lines of 480 unknowns, beyond the partitioned solve:
+    the one-lane reduced cyclic solve with its Sherman-Morrison correction
+    (the separate periodic boundary-row launch), against the oracle after
+    3 steps"""
+    cfg = cfg_periodic(*n, scheme=AB2_CN, dp_flag=0)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=13)
+    out, st, div = run_both(oracle, cfg, u, v, w, p, 3)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+    assert div <= 1e-12
+
+
 @pytest.mark.parametrize("scheme,dp_flag", [(AB2, 0), (AB2_CN, 0), (AB2, 1), (EULER, 0)])
 def test_step_periodic_multi_tile(oracle, scheme, dp_flag):
     """periodic box spanning several predictor tiles (64 x 4 x 16 cells) with
