@@ -82,9 +82,10 @@ struct LineSys {
     int tk;
     double ak, ck, gk;
     const double *loR, *invR, *cpR;
-    // partitioned solve (walls): K blocks of rows [kN/K, (k+1)N/K), one wave
-    // per block; pt = loL | invL | cpL | V | W (N each) | G (2K x 2K); K = 0:
-    // not partitioned
+    // partitioned solve: K blocks of rows [kN/K, (k+1)N/K) of the N unknowns
+    // (walls: the whole line; periodic: the m owned unknowns, the first and
+    // last blocks coupled), one wave per block; pt = loL | invL | cpL | V | W
+    // (N each) | G (2K x 2K); K = 0: not partitioned
     int K;
     const double *pt;
 };
