@@ -30,7 +30,9 @@ def inputs(cfg, seed):
 @pytest.mark.parametrize("scheme,flags,bc,pbc", [(AB2, 0, (2, 2, 2), (3, 3, 3)), (EULER, 0, (4, 4, 4), (2, 2, 2)),
                                                 (AB2, FLAG_NO_GRAPH, (2, 4, 2), (3, 2, 3)),
                                                 (AB2_CN, 0, (2, 2, 2), (3, 3, 3)),
-                                                (AB2_CN, FLAG_NO_GRAPH, (4, 2, 4), (2, 3, 2))])
+                                                (AB2_CN, FLAG_NO_GRAPH, (4, 2, 4), (2, 3, 2)),
+                                                # periodic x velocity lines (cyclic partitioned sweeps) beside walls
+                                                (AB2_CN, 0, (1, 2, 4), (3, 3, 2))])
 def test_batch_equals_individual_steps(scheme, flags, bc, pbc):
     cfgs = [make(*s, scheme, flags, pbc, bc) for s in SIZES]
     data = [inputs(c, 60 + i) for i, c in enumerate(cfgs)]
