@@ -174,8 +174,9 @@ int ns_extract_subbox(ns_ctx *ctx, const ns_subbox *sb, int field, double *dst);
  * gradient side of the re-simulation) ----
  * Euler, AB2 and AB2-CN on every code the forward step takes (the ADI
  * sweeps' adjoints are solves with the transposed tridiagonal or reduced
- * cyclic systems), on contexts that project every step (dp_flag = 1, or
- * NS_FLAG_EAGER_PROJECTION); NS_ERR_UNSUPPORTED otherwise.  The
+ * cyclic systems), lazily projected contexts included: the adjoint works on
+ * the state ns_download_state returns and on the forward velocity passed to
+ * ns_adjoint_step, not on what the context stores between steps.  The
  * adjoint state is (u_b, v_b, w_b, p_b, hx_b, hy_b, hz_b), the gradients of a
  * scalar objective with respect to the state (u, v, w, p, rhs_*_previous)
  * at the current point of the reverse sweep, all in the ghosted shapes. */

@@ -1144,7 +1144,11 @@ static int adj_alloc(ns_ctx *c) {
 
 // explicit schemes on any codes; AB2-CN with wall velocity codes (2, 4) on
 // every axis, whose ADI systems are plain tridiagonal
-static bool adj_supported(const ns_ctx *c) { return !c->lazy && c->poisson_ok; }
+// The adjoint step works on the materialized state (u, v, w, p, H) that
+// ns_download_state returns and ns_upload_state takes, and on the forward
+// velocity passed in, never on the context's stored forward state, so lazily
+// projected contexts (which store u*, dp between steps) are supported too.
+static bool adj_supported(const ns_ctx *c) { return c->poisson_ok; }
 
 // transposed CN systems for the current dt
 static int adj_cn_systems(ns_ctx *c) {

@@ -55,6 +55,10 @@ CASES = [
     ("periodic_fft", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=1), False),
     ("periodic_gauge_eager", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=0,
                                               flags=FLAG_EAGER_PROJECTION | FLAG_GAUGE_CORNER), False),
+    # lazily projected contexts (the periodic parent's default: u*, dp stored between steps)
+    ("periodic_lazy", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=0), False),
+    ("periodic_lazy_gauge_first", lambda: periodic(16, 12, 10, scheme=AB2, dp_flag=0, flags=FLAG_GAUGE_CORNER), True),
+    ("periodic_lazy_cn", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=0, flags=FLAG_GAUGE_CORNER), False),
     ("walls_cn", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), False),
     ("walls_cn_first", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), True),
     ("walls_cn_mixed_dst", lambda: box(10, 8, 9, bc=(2, 4, 4), pbc=(3, 2, 2), scheme=AB2_CN, dp_flag=1), False),
@@ -100,23 +104,30 @@ def test_adjoint_step_exact_on_quadratic_map(name, mk, first):
     assert abs(fd - ad) <= 1e-10 * scale, (fd, ad, scale)
 
 
-def test_adjoint_unsupported_for_lazy_projection():
-    for cfg in (periodic(8, 8, 8, scheme=AB2, dp_flag=0), periodic(8, 8, 8, scheme=AB2_CN, dp_flag=0)):
-        with NSSolver(cfg) as s:
-            rc = _abi.lib().ns_adjoint_set_state(s.h, None, None, None, None, None, None, None)
-            assert rc == _abi.NS_ERR_UNSUPPORTED
+def test_adjoint_step_needs_state():
+    """ns_adjoint_step before ns_adjoint_set_state is an invalid call"""
+    cfg = periodic(8, 8, 8, scheme=AB2, dp_flag=0)
+    st, _ = rand_state(cfg, 3)
+    with NSSolver(cfg) as s:
+        rc = _abi.lib().ns_adjoint_step(s.h, _ptr(st["u"]), _ptr(st["v"]), _ptr(st["w"]), 0, None)
+        assert rc == _abi.NS_ERR_INVALID
 
 
-@pytest.mark.parametrize("scheme", [AB2, AB2_CN], ids=["ab2", "ab2_cn"])
-def test_rollout_gradient_matches_central_differences(scheme):
+@pytest.mark.parametrize("scheme,lazy", [(AB2, False), (AB2_CN, False), (AB2, True)],
+                         ids=["ab2", "ab2_cn", "periodic_lazy_ab2"])
+def test_rollout_gradient_matches_central_differences(scheme, lazy):
     """torch.autograd through 6 steps (checkpoint every 2, recompute), walls
-    with per-step planes: d/d(u0, v0, w0, planes) of a sub-domain mismatch,
-    checked in random directions against central differences (1e-6)."""
+    with per-step planes (or a lazily projected periodic box): d/d(u0, v0, w0,
+    planes) of a sub-domain mismatch, checked in random directions against
+    central differences (1e-6)."""
     import torch
 
     from ns_3d_staggered_dp_amd.adjoint import Rollout, make_rollout_function
 
-    cfg = box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=scheme, dp_flag=1)
+    if lazy:
+        cfg = periodic(12, 10, 8, scheme=scheme, dp_flag=0, flags=FLAG_GAUGE_CORNER)
+    else:
+        cfg = box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=scheme, dp_flag=1)
     nsteps = 6
     st, rng = rand_state(cfg, 41, amp=0.3)
     L = plane_set_length(cfg.nx, cfg.ny, cfg.nz)
