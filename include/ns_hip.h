@@ -210,7 +210,7 @@ int ns_batch_step(ns_batch *b, int nsteps);
  * per size class of the boxes (cells < 2^17, < 2^20, larger) it picks the
  * predictor's z-chunk length (in rounds of resident blocks: 4, 8, 16, 32),
  * the DCT passes' lines per block (default, /2, /4) and, when the x sweep is
- * not partitioned (periodic lines, lines over 448 rows), its lines per block
+ * not partitioned (lines over 640 rows), its lines per block
  * (64, 32, 16, 8), by coordinate descent over steady batch steps
  * timed on the device (trial_steps steps per trial, best of two); the boxes'
  * states are restored after every trial.  No choice changes any computed

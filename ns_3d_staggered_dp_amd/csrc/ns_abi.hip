@@ -203,8 +203,9 @@ std::vector<double> thomas_solve(const std::vector<double> &lo, const std::vecto
     return r;
 }
 
-// Partitioned solve of a wall line (N rows) in K blocks of at most
-// PART_ROWS rows, block k = rows [kN/K, (k+1)N/K).  Each block is solved on
+// Partitioned solve of a wall line (N rows) in K blocks of at most R rows
+// (PART_ROWS, or PART_ROWS_LONG when PART_MAX_BLOCKS blocks of PART_ROWS rows
+// do not reach N), block k = rows [kN/K, (k+1)N/K).  Each block is solved on
 // its own (local Thomas factors loL, invL, cpL, restarted at the block's first
 // row); the couplings to the neighbouring blocks' end values enter through
 // the spikes V (block k's solve of lo_{s_k} e_first) and W (of up_{e_k}
@@ -214,16 +215,20 @@ std::vector<double> thomas_solve(const std::vector<double> &lo, const std::vecto
 // cyclic: the system also couples row 0 to row N-1 (alpha) and row N-1 to
 // row 0 (beta), so block 0's left neighbour is block K-1 and block K-1's
 // right neighbour block 0 (the same spikes, with the corners as couplings).
-// Returns K (0: the line is too long for PART_MAX_BLOCKS blocks) and fills
+// Returns K (0: the line is too long for PART_MAX_BLOCKS blocks), sets R and fills
 // tab = loL | invL | cpL | V | W (N each) | G (2K x 2K, row-major).
 // (The fewest blocks that fit the rows: more, shorter blocks fill the device
 // better for a single small box but cost every lane a longer coupling sum,
 // which slowed batches of many small boxes by 14%.)
 int part_tables(const std::vector<double> &lo, const std::vector<double> &d, const std::vector<double> &up,
-                std::vector<double> &tab, bool cyclic = false, double alpha = 0.0, double beta = 0.0) {
+                std::vector<double> &tab, int &R, bool cyclic = false, double alpha = 0.0, double beta = 0.0) {
     const int N = (int)d.size();
-    const int K = (N + PART_ROWS - 1) / PART_ROWS;
-    if (K > PART_MAX_BLOCKS || N < 2 * K) return 0;
+    R = (N <= PART_MAX_BLOCKS * PART_ROWS) ? PART_ROWS : PART_ROWS_LONG;
+    const int K = (N + R - 1) / R;
+    if (K > PART_MAX_BLOCKS || N < 2 * K) {
+        R = 0;
+        return 0;
+    }
     const int K2 = 2 * K;
     tab.assign((size_t)5 * N + (size_t)K2 * K2, 0.0);
     double *loL = tab.data(), *invL = loL + N, *cpL = invL + N, *V = cpL + N, *W = V + N, *G = W + N;
@@ -365,9 +370,9 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         if (rc) return rc;
         S.z = dz_;
         // partitioned solve of the cyclic system itself (lines of at most
-        // PART_MAX_BLOCKS * PART_ROWS unknowns)
+        // PART_MAX_BLOCKS * PART_ROWS_LONG unknowns)
         std::vector<double> tab;
-        S.K = part_tables(lo, d, up, tab, true, alpha, beta);
+        S.K = part_tables(lo, d, up, tab, S.R, true, alpha, beta);
         if (S.K) {
             double *dt2;
             if ((rc = dstore(c, old.pt, &dt2, tab))) return rc;
@@ -401,9 +406,9 @@ int make_line_sys(ns_ctx *c, int ax, int typeB, LineSys &S, bool transpose = fal
         }
         thomas_factor(lo, d, up, inv, cp);
         S.z = nullptr;
-        // partitioned solve (lines of at most PART_MAX_BLOCKS * PART_ROWS rows)
+        // partitioned solve (lines of at most PART_MAX_BLOCKS * PART_ROWS_LONG rows)
         std::vector<double> tab;
-        S.K = part_tables(lo, d, up, tab);
+        S.K = part_tables(lo, d, up, tab, S.R);
         if (S.K) {
             double *dt_;
             // the table length depends only on N, which a rebuild keeps
@@ -1870,6 +1875,7 @@ struct ns_batch {
         int nblk = 0;
         size_t lds = 0;
         int variant = 0;  // DCT pass: kernel variant (OR over the boxes); sweep: waves per block
+        int rows = 0;     // sweep: rows per lane of the partitioned kernel (the most over the boxes)
         int nbox = 0;     // boxes of this launch (a subset for sweeps grouped by blocks per line)
     };
     std::vector<Phase> phases;  // one steady (non-first) step
@@ -1929,7 +1935,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     std::vector<char> host((size_t)n * psz);
     std::vector<int> prefix(n + 1, 0);
     size_t lds = 0;
-    int variant = 0;
+    int variant = 0, rows = 0;
     // predictor: blocks stream z chunks serially, so chunks of one length per
     // size class, short enough that the launch spans about `rounds` rounds of
     // resident blocks (a short tail), and at least 8 cells (the prologue of a
@@ -1970,8 +1976,9 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
             nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l,
                                    b->knobs[cls].adix_lsh, part, part ? c->planes : nullptr, &c->L);
             lds = l > lds ? l : lds;
-            const int kw = sweep_params_waves(dst);
+            const int kw = sweep_params_waves(dst), rw = sweep_params_rows(dst);
             variant = kw > variant ? kw : variant;
+            rows = rw > rows ? rw : rows;
         } else if (kind == 3) {
             nb = make_fill_params(g, true, f.pbc, c->planes, d, c->dp, nullptr, nullptr, 0, dst);
         } else {
@@ -1988,6 +1995,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     ph.nblk = prefix[n];
     ph.lds = lds;
     ph.variant = variant;
+    ph.rows = rows;
     ph.nbox = n;
     int rc;
     if ((rc = balloc(&ph.params, host.size())) || (rc = balloc(&ph.prefix, prefix.size() * sizeof(int)))) {
@@ -2040,7 +2048,9 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
             case 2: launch_dct_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             case 3: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 1, b->st); break;
             case 5: launch_cn2_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
-            case 6: launch_adi_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
+            case 6:
+                launch_adi_batch(ph.arg, ph.variant, ph.rows, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st);
+                break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
         }
     }

@@ -6,8 +6,8 @@
 // Matrices (mat_CN NS_functions.f90:1245-1294, mat_CN_tridiagonal :1297-1342):
 // walls (bc 2/4) are plain tridiagonal; every line of a direction and
 // component has the same matrix, so what depends on the matrix alone is
-// computed on the host.  Wall lines of up to PART_MAX_BLOCKS * PART_ROWS rows
-// use the partitioned solve (one wave per block of rows, couplings through
+// computed on the host.  Wall lines of up to PART_MAX_BLOCKS * PART_ROWS_LONG
+// rows use the partitioned solve (one wave per block of rows, couplings through
 // host-precomputed spikes); longer ones a twisted Thomas factorization (two
 // lanes per line).  Periodic lines are reduced to the n owned unknowns (the
 // ghost rows of the dense matrix are u_N = u_1 + r_N, v_1 = v_{n+1} + r_1,
@@ -179,6 +179,9 @@ struct SweepParams {
     int lsh, coff;   // x sweep: log2 lines per block, LDS offset of the coefficients
                      // (partitioned: of the block-end values)
     int kw;          // partitioned wall solve: waves per block (0: twisted / cyclic path)
+    int rc;          // partitioned: rows per lane of the kernel (PART_ROWS or PART_ROWS_LONG)
+    int xl;          // partitioned x sweep: lines per block
+    int gcol;        // twisted y / z sweep: elimination results in the line itself (LDS too small)
     // partitioned solve: right-hand sides of the lines' first and last rows
     // per component (the boundary rows vel_bc_staggered_CN2 would write; one
     // value per line, in line order), or null: the array holds them
@@ -201,36 +204,38 @@ struct SweepParams {
 // P = x_{s_k - 1}, Q = x_{e_k + 1} from the 2K end values of the line (rows
 // of G) and writes x = y - P V - Q W.  Every table address is uniform over the
 // wave (the block index k is), so the coefficient loads are scalar loads, and
-// a lane holds at most PART_ROWS rows, so no LDS bounds the resident waves.
+// a lane holds at most RC rows (PART_ROWS, or PART_ROWS_LONG for long lines),
+// so no LDS bounds the resident waves.
 
 // local solve of one block in registers, y[0..R); returns y[R-1] (the last
 // forward result, which back substitution keeps)
-__device__ __forceinline__ double part_local(double (&y)[PART_ROWS], int R, const double *loL, const double *invL,
+template <int RC>
+__device__ __forceinline__ double part_local(double (&y)[RC], int R, const double *loL, const double *invL,
                                              const double *cpL) {
     double d = 0.0;
 #pragma unroll
-    for (int j = 0; j < PART_ROWS; j++)
+    for (int j = 0; j < RC; j++)
         if (j < R) {
             d = (y[j] - loL[j] * d) * invL[j];
             y[j] = d;
         }
 #pragma unroll
-    for (int j = PART_ROWS - 2; j >= 0; j--)
+    for (int j = RC - 2; j >= 0; j--)
         if (j + 1 < R) y[j] = y[j] - cpL[j] * y[j + 1];
     return d;
 }
 
 // rows [s_k, s_k + R) of a lane's line into registers (get(j): row s_k + j);
 // the line's first and last rows come from rb when it is set (line index t)
-template <typename Get>
-__device__ __forceinline__ void part_load(double (&y)[PART_ROWS], int R, bool valid, int k, int K,
+template <int RC, typename Get>
+__device__ __forceinline__ void part_load(double (&y)[RC], int R, bool valid, int k, int K,
                                           const double *const (&rb)[2], unsigned t, Get get) {
     const double *r0 = (k == 0) ? rb[0] : nullptr;  // uniform over the wave
     const double *rl = (k == K - 1) ? rb[1] : nullptr;
     const double first = (r0 && valid) ? r0[t] : 0.0;
     const double last = (rl && valid) ? rl[t] : 0.0;
 #pragma unroll
-    for (int j = 0; j < PART_ROWS; j++) {
+    for (int j = 0; j < RC; j++) {
         const bool f = r0 && j == 0, l = rl && j == R - 1;
         double v = (j < R && valid && !f && !l) ? get(j) : 0.0;
         if (f) v = first;
@@ -242,14 +247,14 @@ __device__ __forceinline__ void part_load(double (&y)[PART_ROWS], int R, bool va
 // stores rows [s_k, s_k + R) of a lane's line (put(j, v): row s_k + j); the
 // line's first and last rows are added to wb (line index t) when it is set and
 // stored as zeros
-template <typename Put>
-__device__ __forceinline__ void part_store(const double (&x)[PART_ROWS], int R, bool valid, int k, int K,
+template <int RC, typename Put>
+__device__ __forceinline__ void part_store(const double (&x)[RC], int R, bool valid, int k, int K,
                                            double *const (&wb)[2], unsigned t, Put put) {
     if (!valid) return;
     double *w0 = (k == 0) ? wb[0] : nullptr;  // uniform over the wave
     double *wl = (k == K - 1) ? wb[1] : nullptr;
 #pragma unroll
-    for (int j = 0; j < PART_ROWS; j++) {
+    for (int j = 0; j < RC; j++) {
         if (j < R) {
             double v = x[j];
             if (w0 && j == 0) {
@@ -295,7 +300,8 @@ __device__ __forceinline__ double part_first(int K, const double *G, const doubl
 // off+m-1 of the line; the ghost rows' right-hand sides glo = row 0 (type B)
 // and ghi = row off+m enter the end rows, as in solve_line).  Adds the folds
 // to block 0's first and block K-1's last row.
-__device__ __forceinline__ void part_fold(double (&y)[PART_ROWS], int R, int k, int K, const LineSys &S, double glo,
+template <int RC>
+__device__ __forceinline__ void part_fold(double (&y)[RC], int R, int k, int K, const LineSys &S, double glo,
                                           double ghi) {
     double ff, fl;
     if (S.adj) {
@@ -306,7 +312,7 @@ __device__ __forceinline__ void part_fold(double (&y)[PART_ROWS], int R, int k, 
         fl = S.a * ghi;
     }
 #pragma unroll
-    for (int j = 0; j < PART_ROWS; j++) {
+    for (int j = 0; j < RC; j++) {
         if (k == 0 && j == 0) y[j] = y[j] + ff;
         if (k == K - 1 && j == R - 1) y[j] = y[j] + fl;
     }
@@ -339,7 +345,7 @@ __device__ __forceinline__ void part_ghosts(int k, int K, const LineSys &S, bool
 
 // y / z sweeps: lane = line (consecutive x), so every row load of a wave is
 // 64 consecutive doubles
-template <int AX>
+template <int AX, int RC>
 __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
     const int c = blockIdx.y;
@@ -373,7 +379,7 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
         if (off) glo = line[0];
         ghi = line[(size_t)stride * (off + N)];
     }
-    double y[PART_ROWS];
+    double y[RC];
     part_load(y, R, valid, k, K, P.rb[c], t, [&](int j) { return x[(size_t)stride * j]; });
     if (per) part_fold(y, R, k, K, S, glo, ghi);
     if (act) {
@@ -388,7 +394,7 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
     const double *V = tb + 3 * N + s0, *W = tb + 4 * N + s0;
     double xl = 0.0;
 #pragma unroll
-    for (int j = 0; j < PART_ROWS; j++)
+    for (int j = 0; j < RC; j++)
         if (j < R) {
             y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
             if (j == R - 1) xl = y[j];
@@ -401,9 +407,10 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
     }
 }
 
-// x sweeps: the block stages its 64 lines (one contiguous span) in LDS rows
-// of odd length LS >= N, coalesced both ways; the block-end values follow at
-// P.coff
+// x sweeps: the block stages its P.xl lines (one contiguous span; 64, or 32
+// or 16 when 64 long lines would not fit the LDS) in LDS rows of odd length
+// LS >= N, coalesced both ways; the block-end values follow at P.coff
+template <int RC>
 __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
     const int c = blockIdx.y;
@@ -414,9 +421,9 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
     const bool per = S.periodic;
     const int K = S.K, NU = per ? S.m : N, off = per ? S.off : 0;  // NU: unknowns per line
     const int nl = n[1] * n[2];
-    const int l0 = blk * 64;
+    const int l0 = blk * P.xl;
     if (l0 >= nl) return;  // the whole block: this component has fewer lines
-    const int cnt = (nl - l0 < 64) ? nl - l0 : 64;
+    const int cnt = (nl - l0 < P.xl) ? nl - l0 : P.xl;
     const int LS = N | 1, tot = cnt * N;
     double *a = P.a[c] + (size_t)l0 * N;
     const float rN = 1.0f / (float)N;  // f < 2^24: float quotient within one
@@ -450,7 +457,7 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
         if (off) glo = lrow[0];
         ghi = lrow[off + NU];
     }
-    double y[PART_ROWS];
+    double y[RC];
     part_load(y, R, lane < cnt, k, K, P.rb[c], (unsigned)(l0 + lane), [&](int j) { return row[j]; });
     if (per) part_fold(y, R, k, K, S, glo, ghi);
     if (act) {
@@ -465,7 +472,7 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
         const double *V = tb + 3 * NU + s0, *W = tb + 4 * NU + s0;
         double xl = 0.0;
 #pragma unroll
-        for (int j = 0; j < PART_ROWS; j++)
+        for (int j = 0; j < RC; j++)
             if (j < R) {
                 y[j] = (y[j] - Pk * V[j]) - Qk * W[j];
                 if (j == R - 1) xl = y[j];
@@ -528,10 +535,12 @@ __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int
     double *base = top ? x : x + stride * (N - 1);
     const long st = top ? stride : -stride;
     const int m = valid ? (top ? tk : N - 1 - tk) : 0;
-    // the elimination results of a half line stay in this lane's LDS column
+    // the elimination results of a half line stay in this lane's LDS column,
+    // or (lines too long for 64 columns) overwrite the half line in place
     extern __shared__ double sm[];
-    double *col = sm + threadIdx.x;
-    const double dk = thomas_fwd<16>(base, st, col, 64, m, top ? S.lo : S.loR, top ? S.inv : S.invR, false, 0.0, 0.0);
+    double *col = P.gcol ? base : sm + threadIdx.x;
+    const long cs = P.gcol ? st : 64;
+    const double dk = thomas_fwd<16>(base, st, col, cs, m, top ? S.lo : S.loR, top ? S.inv : S.invR, false, 0.0, 0.0);
     const double other = __shfl_xor(dk, 32);
     double xk = 0.0;
     if (valid) {
@@ -539,7 +548,7 @@ __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int
         xk = ((x[stride * tk] - S.ak * dt) - S.ck * eb) * S.gk;
     }
     if (valid && top) x[stride * tk] = xk;
-    thomas_bwd<16>(col, 64, base, st, valid ? (top ? tk + 1 : N - tk) : 0, top ? S.cp : S.cpR, xk);
+    thomas_bwd<16>(col, cs, base, st, valid ? (top ? tk + 1 : N - tk) : 0, top ? S.cp : S.cpR, xk);
 }
 
 // x sweeps: a block stages L = 2^lsh lines of one component in LDS (x lines
@@ -634,25 +643,28 @@ __global__ void __launch_bounds__(64) k_adi_strided(SweepParams P) {
 __global__ void __launch_bounds__(64) k_adi_x(SweepParams P) { adi_x_body(P, blockIdx.x); }
 
 constexpr int PART_THREADS = 64 * PART_MAX_BLOCKS;
-template <int AX>
+template <int AX, int RC>
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided(SweepParams P) {
-    adi_part_strided_body<AX>(P, blockIdx.x);
+    adi_part_strided_body<AX, RC>(P, blockIdx.x);
 }
-__global__ void __launch_bounds__(PART_THREADS) k_adi_part_x(SweepParams P) { adi_part_x_body(P, blockIdx.x); }
-template <int AX>
+template <int RC>
+__global__ void __launch_bounds__(PART_THREADS) k_adi_part_x(SweepParams P) { adi_part_x_body<RC>(P, blockIdx.x); }
+template <int AX, int RC>
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided_batch(const SweepParams *__restrict__ Ps,
                                                                                   const int *__restrict__ prefix,
                                                                                   int nbox) {
     int blk;
     const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
-    adi_part_strided_body<AX>(P, blk);
+    adi_part_strided_body<AX, RC>(P, blk);
 }
+template <int RC>
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_x_batch(const SweepParams *__restrict__ Ps,
                                                                            const int *__restrict__ prefix, int nbox) {
     int blk;
     const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
-    adi_part_x_body(P, blk);
+    adi_part_x_body<RC>(P, blk);
 }
+
 
 template <int AX>
 __global__ void __launch_bounds__(64) k_adi_strided_batch(const SweepParams *__restrict__ Ps,
@@ -673,6 +685,27 @@ void set_lds(K kern, size_t lds) {
     if (lds > 65536) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// one partitioned sweep launch with the kernel for RC rows per lane
+template <int RC>
+void launch_part(int ax, dim3 grid, dim3 blk, size_t lds, hipStream_t st, const SweepParams *P,
+                 const SweepParams *Ps, const int *prefix, int nbox) {
+    if (ax == 0) {
+        if (P) {
+            set_lds(k_adi_part_x<RC>, lds);
+            hipLaunchKernelGGL(k_adi_part_x<RC>, grid, blk, lds, st, *P);
+        } else {
+            set_lds(k_adi_part_x_batch<RC>, lds);
+            hipLaunchKernelGGL(k_adi_part_x_batch<RC>, grid, blk, lds, st, Ps, prefix, nbox);
+        }
+    } else if (ax == 1) {
+        if (P) hipLaunchKernelGGL((k_adi_part_strided<1, RC>), grid, blk, lds, st, *P);
+        else hipLaunchKernelGGL((k_adi_part_strided_batch<1, RC>), grid, blk, lds, st, Ps, prefix, nbox);
+    } else {
+        if (P) hipLaunchKernelGGL((k_adi_part_strided<2, RC>), grid, blk, lds, st, *P);
+        else hipLaunchKernelGGL((k_adi_part_strided_batch<2, RC>), grid, blk, lds, st, Ps, prefix, nbox);
+    }
+}
+
 }  // namespace
 
 size_t sweep_params_size() { return sizeof(SweepParams); }
@@ -690,8 +723,9 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
     P.a[0] = u; P.a[1] = v; P.a[2] = w;
     const int ax = P.ax;
     if (part && sweep_partitioned(sysA, sysB)) {
-        // 64 lines per block, one wave per partition block
+        // 64 lines per block (x: P.xl), one wave per partition block
         P.kw = sysA.K > sysB.K ? sysA.K : sysB.K;
+        P.rc = sysA.R > sysB.R ? sysA.R : sysB.R;
         size_t nl = 0;
         int lsmax = 0;
         for (int c = 0; c < 3; c++) {
@@ -701,7 +735,12 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
             nl = m > nl ? m : nl;
             lsmax = (n[0] | 1) > lsmax ? (n[0] | 1) : lsmax;
         }
-        P.coff = (ax == 0) ? 64 * lsmax : 0;
+        // x: the staged lines and the end values within the LDS (lines of at
+        // most PART_MAX_BLOCKS * PART_ROWS_LONG + 2 elements fit 16 at a time)
+        P.xl = 64;
+        while (ax == 0 && P.xl > 16 && ((size_t)P.xl * lsmax + (size_t)2 * P.kw * 64) * sizeof(double) > LDS_BYTES)
+            P.xl /= 2;
+        P.coff = (ax == 0) ? P.xl * lsmax : 0;
         *lds = ((size_t)P.coff + (size_t)2 * P.kw * 64) * sizeof(double);
         if (sysA.periodic) {
             // the boundary rows of periodic lines are their zeroed ghost rows
@@ -719,7 +758,7 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
                 }
             }
         }
-        return (int)((nl + 63) / 64);
+        return (int)((nl + P.xl - 1) / P.xl);
     }
     if (ax == 0) {
         // longest x line, lines per block (up to 64 within 64 KiB), blocks of the largest component
@@ -750,6 +789,10 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
         nmax = n[ax] > nmax ? n[ax] : nmax;
     }
     *lds = sysA.periodic ? 0 : (size_t)64 * (nmax / 2 + 1) * sizeof(double);
+    if (*lds > LDS_BYTES) {
+        P.gcol = 1;
+        *lds = 0;
+    }
     const size_t per = sysA.periodic ? 64 : 32;
     return (int)((nl + per - 1) / per);
 }
@@ -758,19 +801,15 @@ bool sweep_partitioned(const LineSys &sysA, const LineSys &sysB) { return sysA.K
 
 int sweep_params_waves(const void *params) { return ((const SweepParams *)params)->kw; }
 
-void launch_adi_batch(int ax, int kw, const void *params, const int *prefix, int nbox, int nblk, size_t lds,
-                      hipStream_t st) {
+int sweep_params_rows(const void *params) { return ((const SweepParams *)params)->rc; }
+
+void launch_adi_batch(int ax, int kw, int rows, const void *params, const int *prefix, int nbox, int nblk,
+                      size_t lds, hipStream_t st) {
     const SweepParams *Ps = (const SweepParams *)params;
     if (kw > 0) {
         const dim3 grid(nblk, 3), blk(64 * kw);
-        if (ax == 0) {
-            set_lds(k_adi_part_x_batch, lds);
-            hipLaunchKernelGGL(k_adi_part_x_batch, grid, blk, lds, st, Ps, prefix, nbox);
-        } else if (ax == 1) {
-            hipLaunchKernelGGL(k_adi_part_strided_batch<1>, grid, blk, lds, st, Ps, prefix, nbox);
-        } else {
-            hipLaunchKernelGGL(k_adi_part_strided_batch<2>, grid, blk, lds, st, Ps, prefix, nbox);
-        }
+        if (rows > PART_ROWS) launch_part<PART_ROWS_LONG>(ax, grid, blk, lds, st, nullptr, Ps, prefix, nbox);
+        else launch_part<PART_ROWS>(ax, grid, blk, lds, st, nullptr, Ps, prefix, nbox);
         return;
     }
     if (ax == 0) {
@@ -792,14 +831,8 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
     const int nb = make_sweep_params(g, dir, sysA, sysB, u, v, w, &P, &lds, 6, true, planes, L, planes_out);
     if (P.kw > 0) {
         const dim3 grid(nb, 3), blk(64 * P.kw);
-        if (dir == 1) {
-            set_lds(k_adi_part_x, lds);
-            hipLaunchKernelGGL(k_adi_part_x, grid, blk, lds, st, P);
-        } else if (dir == 2) {
-            hipLaunchKernelGGL(k_adi_part_strided<1>, grid, blk, lds, st, P);
-        } else {
-            hipLaunchKernelGGL(k_adi_part_strided<2>, grid, blk, lds, st, P);
-        }
+        if (P.rc > PART_ROWS) launch_part<PART_ROWS_LONG>(dir - 1, grid, blk, lds, st, &P, nullptr, nullptr, 0);
+        else launch_part<PART_ROWS>(dir - 1, grid, blk, lds, st, &P, nullptr, nullptr, 0);
         return;
     }
     if (dir == 1) {

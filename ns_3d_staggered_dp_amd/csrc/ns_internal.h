@@ -85,13 +85,21 @@ struct LineSys {
     // partitioned solve: K blocks of rows [kN/K, (k+1)N/K) of the N unknowns
     // (walls: the whole line; periodic: the m owned unknowns, the first and
     // last blocks coupled), one wave per block; pt = loL | invL | cpL | V | W
-    // (N each) | G (2K x 2K); K = 0: not partitioned
+    // (N each) | G (2K x 2K); K = 0: not partitioned.  R: the most rows of a
+    // block (PART_ROWS, or PART_ROWS_LONG for lines of more than
+    // PART_MAX_BLOCKS * PART_ROWS unknowns)
     int K;
+    int R;
     const double *pt;
 };
-// partitioned wall solve: rows per block (registers per lane) and blocks per line
+// partitioned solve: rows per block (registers per lane) and blocks per line;
+// lines too long for PART_MAX_BLOCKS blocks of PART_ROWS rows take blocks of
+// up to PART_ROWS_LONG rows (more registers per lane, fewer resident waves)
 constexpr int PART_ROWS = 28;
+constexpr int PART_ROWS_LONG = 40;
 constexpr int PART_MAX_BLOCKS = 16;
+// LDS of one workgroup on gfx950
+constexpr size_t LDS_BYTES = 160 * 1024;
 
 struct Ctx;
 
@@ -167,8 +175,10 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
 bool sweep_partitioned(const LineSys &sysA, const LineSys &sysB);
 // waves per block of a sweep's parameters (0: twisted / cyclic path)
 int sweep_params_waves(const void *params);
+// rows per lane of a partitioned sweep's kernel (PART_ROWS or PART_ROWS_LONG)
+int sweep_params_rows(const void *params);
 // kw: waves per block of the partitioned solve (every box partitioned), 0: twisted / cyclic path
-void launch_adi_batch(int ax, int kw, const void *params, const int *prefix, int nbox, int nblk, size_t lds,
+void launch_adi_batch(int ax, int kw, int rows, const void *params, const int *prefix, int nbox, int nblk, size_t lds,
                       hipStream_t st);
 struct PoissonPlans;
 size_t pass_params_size();

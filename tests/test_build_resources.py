@@ -19,10 +19,12 @@ def test_no_kernel_uses_scratch():
 
 
 def test_partitioned_sweeps_fit_five_waves():
+    """the partitioned sweeps with 28 rows per lane keep five waves per SIMD;
+    those with 40 (lines over 448 rows) four"""
     res = build.kernel_resources()
     if not res:
         pytest.skip("no resource reports: library not built here")
     part = {k: v for k, v in res.items() if "k_adi_part" in k}
-    assert len(part) == 6
+    assert len(part) == 12
     for k, v in part.items():
-        assert v["occupancy"] >= 5, (k, v)
+        assert v["occupancy"] >= (4 if "Li40E" in k else 5), (k, v)

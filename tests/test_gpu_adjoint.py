@@ -63,9 +63,12 @@ CASES = [
     ("walls_cn_first", lambda: box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), True),
     ("walls_cn_mixed_dst", lambda: box(10, 8, 9, bc=(2, 4, 4), pbc=(3, 2, 2), scheme=AB2_CN, dp_flag=1), False),
     ("walls_cn_long_lines", lambda: box(40, 6, 36, bc=(4, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1), False),
-    # 9 partitioned blocks per x line; y lines over 448 rows (twisted solve)
+    # 9 partitioned blocks per x line; x lines staged 32 per block; y lines in blocks of up to 40 rows;
+    # y lines over 640 rows (twisted solve)
     ("walls_cn_k9_lines", lambda: box(250, 5, 4, bc=(2, 2, 4), pbc=(3, 3, 2), scheme=AB2_CN, dp_flag=1), False),
-    ("walls_cn_twisted_lines", lambda: box(4, 480, 3, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1), False),
+    ("walls_cn_wide_x", lambda: box(320, 4, 3, bc=(2, 2, 4), pbc=(3, 3, 2), scheme=AB2_CN, dp_flag=1), False),
+    ("walls_cn_rows40_lines", lambda: box(4, 480, 3, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1), False),
+    ("walls_cn_twisted_lines", lambda: box(4, 700, 3, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1), False),
     # periodic CN lines: the reference's row-1 coupling (default) and the cyclic operator
     ("periodic_cn", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=1), False),
     ("periodic_cn_first", lambda: periodic(16, 12, 10, scheme=AB2_CN, dp_flag=1), True),
@@ -73,6 +76,10 @@ CASES = [
                                                   flags=FLAG_CN_CYCLIC | FLAG_EAGER_PROJECTION | FLAG_GAUGE_CORNER),
      False),
     ("cn_periodic_x_walls_yz", lambda: box(12, 10, 8, bc=(1, 2, 4), pbc=(3, 3, 2), scheme=AB2_CN, dp_flag=1), False),
+    # long periodic lines: x staged 32 per block (320), blocks of up to 40 rows (480), one-lane cyclic (700)
+    ("periodic_cn_wide_x", lambda: periodic(320, 4, 3, scheme=AB2_CN, dp_flag=1), False),
+    ("periodic_cn_rows40_x", lambda: periodic(480, 4, 3, scheme=AB2_CN, dp_flag=1), False),
+    ("periodic_cn_cyclic_x", lambda: periodic(700, 4, 3, scheme=AB2_CN, dp_flag=1), False),
 ]
 
 

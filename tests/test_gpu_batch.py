@@ -34,7 +34,18 @@ def inputs(cfg, seed):
                                                 # periodic x velocity lines (cyclic partitioned sweeps) beside walls
                                                 (AB2_CN, 0, (1, 2, 4), (3, 3, 2))])
 def test_batch_equals_individual_steps(scheme, flags, bc, pbc):
-    cfgs = [make(*s, scheme, flags, pbc, bc) for s in SIZES]
+    check_batch([make(*s, scheme, flags, pbc, bc) for s in SIZES])
+
+
+def test_batch_long_lines():
+    """AB2-CN boxes whose lines take different partitioned kernels in one
+    batch: x lines staged 32 per block (320, 350 cells), blocks of up to 40
+    rows (480 cells) and of 28 rows with the same block count (350 cells:
+    13 blocks, as 480), beside a short box"""
+    check_batch([make(*s, AB2_CN) for s in [(350, 4, 3), (480, 4, 3), (24, 16, 12), (4, 480, 3), (320, 4, 3)]])
+
+
+def check_batch(cfgs):
     data = [inputs(c, 60 + i) for i, c in enumerate(cfgs)]
     nsteps = 5
     ref = []
@@ -55,7 +66,7 @@ def test_batch_equals_individual_steps(scheme, flags, bc, pbc):
         for i, s in enumerate(solvers):
             got = s.download()
             for k in ("u", "v", "w", "p", "hx", "hy", "hz"):
-                assert np.array_equal(got[k], ref[i][k]), (SIZES[i], k)
+                assert np.array_equal(got[k], ref[i][k]), ((cfgs[i].nx, cfgs[i].ny, cfgs[i].nz), k)
     finally:
         for s in solvers:
             s.close()
