@@ -863,6 +863,9 @@ int ns_step(ns_ctx *c, int nsteps) {
         const bool steady = !c->first && !c->cap_on && !c->graph_off && !(c->cfg.flags & NS_FLAG_NO_GRAPH);
         if (steady && nsteps - s >= 2) {
             const int parity = (c->us == c->us_base) ? 0 : 1;
+            // a failed launch of an eager step before this one is reported, not
+            // cleared with the capture's errors
+            if (!c->graph[parity] && hipPeekAtLastError() != hipSuccess) return NS_ERR_HIP;
             if (!c->graph[parity] && build_graph(c, parity) != NS_OK) {
                 c->graph_off = 1;  // run eagerly from here on
                 continue;
