@@ -206,9 +206,9 @@ def test_step_walls(oracle, scheme, bc, pbc):
 
 
 @pytest.mark.parametrize("n", [(126, 12, 10), (12, 126, 10), (12, 10, 126), (250, 6, 5), (6, 250, 5), (6, 5, 250),
-                               (320, 5, 4), (480, 4, 3), (4, 480, 3), (4, 3, 480), (700, 4, 3), (4, 700, 3), (5, 4, 700)],
+                               (320, 5, 4), (480, 4, 3), (4, 480, 3), (4, 3, 480), (700, 4, 3), (4, 700, 3), (5, 4, 700), (2048, 3, 2), (3, 2048, 2), (3, 2, 2048)],
                          ids=["long_x", "long_y", "long_z", "k9_x", "k9_y", "k9_z", "wide_x", "rows40_x", "rows40_y",
-                              "rows40_z", "twisted_x", "twisted_y", "twisted_z"])
+                              "rows40_z", "twisted_x", "twisted_y", "twisted_z", "max_x", "max_y", "max_z"])
 def test_step_walls_cn_long_lines(oracle, n):
     """AB2-CN on walls with long lines along one axis: the partitioned sweeps
     with 5 blocks per line (126 cells) and 9 (250 cells; the x sweep's LDS
@@ -216,8 +216,9 @@ def test_step_walls_cn_long_lines(oracle, n):
     cells: 32 lines per block), lines over 448 rows in blocks of up to 40 rows
     (480 cells), the twisted solve of lines over 640 rows (700 cells; y and z
     lines too long for the LDS columns keep the elimination results in the
-    line itself), and the DCT passes on long lines with odd radices, against
-    the oracle after 3 steps."""
+    line itself), lines of the longest transform length (2048 cells), and the
+    DCT passes on long lines with odd radices, against the oracle after 3
+    steps."""
     cfg = cfg_box(*n, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1).replace(dt=1e-3)
     u, v, w, p = rand_fields(cfg, 91)
     u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
@@ -232,14 +233,15 @@ def cfg_periodic(nx, ny, nz, **kw):
                     pbc=(1, 1, 1), flags=FLAG_GAUGE_CORNER, **kw)
 
 
-@pytest.mark.parametrize("n", [(320, 4, 4), (480, 4, 4), (6, 4, 480), (700, 4, 4), (6, 4, 700)],
-                         ids=["wide_x", "rows40_x", "rows40_z", "cyclic_x", "cyclic_z"])
+@pytest.mark.parametrize("n", [(320, 4, 4), (480, 4, 4), (6, 4, 480), (700, 4, 4), (6, 4, 700), (2048, 3, 2),
+                               (4, 3, 2048)],
+                         ids=["wide_x", "rows40_x", "rows40_z", "cyclic_x", "cyclic_z", "max_x", "max_z"])
 def test_step_periodic_cn_long_lines(oracle, n):
     """AB2-CN on long periodic lines: the partitioned cyclic solve with x
     lines staged 32 per block (320 unknowns) and with blocks of up to 40 rows
-    (480), and beyond it (700) the one-lane reduced cyclic solve with its
-    Sherman-Morrison correction (the separate periodic boundary-row launch),
-    against the oracle after 3 steps"""
+    (480), and beyond it (700, and 2048, the longest transform length) the
+    one-lane reduced cyclic solve with its Sherman-Morrison correction (the
+    separate periodic boundary-row launch), against the oracle after 3 steps"""
     cfg = cfg_periodic(*n, scheme=AB2_CN, dp_flag=0)
     u, v, w, p = fields.decaying_turbulence(cfg, seed=13)
     out, st, div = run_both(oracle, cfg, u, v, w, p, 3)
