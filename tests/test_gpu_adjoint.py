@@ -85,7 +85,44 @@ CASES = [
 
 @pytest.mark.parametrize("name,mk,first", CASES, ids=[c[0] for c in CASES])
 def test_adjoint_step_exact_on_quadratic_map(name, mk, first):
-    cfg = mk()
+    check_exact(mk(), first)
+
+
+def _shapes(seed, count):
+    """seeded shapes: one axis of 7-smooth length in [60, 400], two in [2, 24]"""
+    def smooth(n):
+        for f in (2, 3, 5, 7):
+            while n % f == 0:
+                n //= f
+        return n == 1
+    long_ = [n for n in range(60, 401) if smooth(n)]
+    short = [n for n in range(2, 25) if smooth(n)]
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = [int(rng.choice(short)) for _ in range(3)]
+        n[int(rng.integers(3))] = int(rng.choice(long_))
+        out.append(tuple(n))
+    return out
+
+
+SHAPES = _shapes(77, 4)
+
+
+@pytest.mark.parametrize("n", SHAPES, ids=["x".join(map(str, n)) for n in SHAPES])
+@pytest.mark.parametrize("kind", ["walls_cn", "periodic_cn"])
+def test_adjoint_step_exact_random_shapes(n, kind):
+    """the exactness check on seeded box shapes with one long axis, whose
+    launch shapes (the adjoint z-march's chunks, the sweeps' blocks per line
+    and LDS staging, the transform passes' lines per block) differ"""
+    if kind == "walls_cn":
+        cfg = box(*n, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1)
+    else:
+        cfg = periodic(*n, scheme=AB2_CN, dp_flag=1)
+    check_exact(cfg, False)
+
+
+def check_exact(cfg, first):
     st, rng = rand_state(cfg, 31)
     L = plane_set_length(cfg.nx, cfg.ny, cfg.nz)
     planes = 0.2 * rng.standard_normal(L)

@@ -45,6 +45,32 @@ def test_batch_long_lines():
     check_batch([make(*s, AB2_CN) for s in [(350, 4, 3), (480, 4, 3), (24, 16, 12), (4, 480, 3), (320, 4, 3)]])
 
 
+def test_batch_random_shapes():
+    """a batch of seeded box shapes, each with one long axis (7-smooth, 60 to
+    400) and two short ones (2 to 24): sweeps grouped by blocks per line with
+    different rows per lane and x staging, transform passes with different
+    lines per block, in one launch each"""
+    rng = np.random.default_rng(5)
+    smooth = [n for n in range(2, 401) if all(p in (2, 3, 5, 7) for p in _factors(n))]
+    long_, short = [n for n in smooth if n >= 60], [n for n in smooth if n <= 24]
+    shapes = []
+    for _ in range(6):
+        n = [int(rng.choice(short)) for _ in range(3)]
+        n[int(rng.integers(3))] = int(rng.choice(long_))
+        shapes.append(tuple(n))
+    check_batch([make(*n, AB2_CN) for n in shapes])
+
+
+def _factors(n):
+    out, f = [], 2
+    while n > 1:
+        while n % f == 0:
+            out.append(f)
+            n //= f
+        f += 1
+    return out
+
+
 def check_batch(cfgs):
     data = [inputs(c, 60 + i) for i, c in enumerate(cfgs)]
     nsteps = 5
