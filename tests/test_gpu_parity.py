@@ -258,22 +258,24 @@ RANDOM_SHAPES = _random_shapes(2026, 10)
 
 @pytest.mark.parametrize("n", RANDOM_SHAPES, ids=["x".join(map(str, n)) for n in RANDOM_SHAPES])
 @pytest.mark.parametrize("kind", ["walls", "periodic"])
-def test_step_cn_random_shapes(oracle, n, kind):
-    """AB2-CN steps on seeded box shapes with one long axis (the launch shapes
-    of every sweep, fill and transform pass depend on the lengths: lines per
-    block, blocks per line, rows per lane, LDS staging), walls with mixed
-    codes and planes or triply periodic, against the oracle after 3 steps.
+@pytest.mark.parametrize("scheme", [AB2_CN, AB2], ids=["ab2_cn", "ab2"])
+def test_step_random_shapes(oracle, n, kind, scheme):
+    """AB2-CN and AB2 steps on seeded box shapes with one long axis (the
+    launch shapes of every sweep, fill, predictor tile and transform pass
+    depend on the lengths: lines per block, blocks per line, rows per lane,
+    LDS staging), walls with mixed codes and planes or triply periodic,
+    against the oracle after 3 steps.
     The periodic boxes start from smooth solenoidal fields, as the other
     periodic tests: from white noise the pressure of a 25 x 375 x 2 box came
     to 1.75e-11 of the oracle's (velocity within 1e-11), the rounding of the
     FFT solve on a box whose smallest and largest eigenvalues lie 1e5 apart."""
     if kind == "walls":
-        cfg = cfg_box(*n, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=AB2_CN, dp_flag=1).replace(dt=1e-3)
+        cfg = cfg_box(*n, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=scheme, dp_flag=1).replace(dt=1e-3)
         planes = rand_planes(cfg, 97, 0.2)
         u, v, w, p = rand_fields(cfg, 96)
         u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
     else:
-        cfg = cfg_periodic(*n, scheme=AB2_CN, dp_flag=0)
+        cfg = cfg_periodic(*n, scheme=scheme, dp_flag=0)
         planes = None
         u, v, w, p = fields.decaying_turbulence(cfg, seed=96)
     out, st, div = run_both(oracle, cfg, u, v, w, p, 3, planes=planes)
