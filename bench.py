@@ -179,6 +179,7 @@ def adjoint_cost(n, scheme_name="AB2-CN", steps=10):
         ub = torch.from_numpy(np.asfortranarray(u).ravel(order="F")).cuda()
         _abi.call("ns_adjoint_set_state", s.h, ctypes.c_void_p(ub.data_ptr()), None, None, None, None, None, None)
         pb = torch.zeros(L, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()  # the context stream does not wait for torch's
         args = [ctypes.c_void_p(t.data_ptr()) for t in (us, vs, ws)]
         _abi.call("ns_adjoint_step", s.h, *args, 0, ctypes.c_void_p(pb.data_ptr()))
         torch.cuda.synchronize()

@@ -19,6 +19,11 @@
  *    re_simulation_sub_tstep_interBC_combine.f90:393-394).
  *  - One context per host thread; calls on one context are serialised on its
  *    HIP stream.  Contexts share no mutable state.
+ *  - The context stream is non-blocking: device buffers passed in are read
+ *    (and device outputs written) on it, so the caller completes the work
+ *    that produces an input before the call (synchronise the producing
+ *    stream, or make the context stream wait on an event), and reads a
+ *    device output after ns_synchronize.  Host buffers need nothing.
  */
 #ifndef NS_HIP_H
 #define NS_HIP_H

@@ -47,10 +47,20 @@ class Rollout:
     def close(self):
         self.s.close()
 
-    def _empty_state(self):
+    def _empty_state(self, zero=False):
         import torch
 
-        return {k: torch.zeros(int(np.prod(self.sh[_comp(k)])), dtype=torch.float64, device="cuda") for k in STATE}
+        alloc = torch.zeros if zero else torch.empty
+        return {k: alloc(int(np.prod(self.sh[_comp(k)])), dtype=torch.float64, device="cuda") for k in STATE}
+
+    @staticmethod
+    def _sync_torch():
+        # the context runs on its own non-blocking stream: torch's pending work
+        # on the tensors handed to it (inputs, freshly allocated outputs) must
+        # be complete first
+        import torch
+
+        torch.cuda.current_stream().synchronize()
 
     def _download(self, out):
         _abi.call("ns_download_state", self.s.h, *(ctypes.c_void_p(out[k].data_ptr()) for k in STATE))
@@ -73,6 +83,7 @@ class Rollout:
         self.planes = planes
         self.ckpt = {}
         st = {k: state0[k].detach().clone() for k in STATE}
+        self._sync_torch()
         self._upload(st, first)
         for n in range(self.n):
             if n % self.K == 0:
@@ -98,9 +109,10 @@ class Rollout:
         import torch
 
         g = {k: (grad_state[k].contiguous() if grad_state.get(k) is not None else None) for k in STATE}
+        pbar = torch.zeros(self.n * self.L, dtype=torch.float64, device="cuda")
+        self._sync_torch()
         _abi.call("ns_adjoint_set_state", self.s.h, *(ctypes.c_void_p(g[k].data_ptr()) if g[k] is not None else None
                                                       for k in STATE))
-        pbar = torch.zeros(self.n * self.L, dtype=torch.float64, device="cuda")
         starts = sorted(self.ckpt)
         for s0 in reversed(starts):
             s1 = min(s0 + self.K, self.n)
@@ -137,7 +149,7 @@ def make_rollout_function():
     class NSRollout(torch.autograd.Function):
         @staticmethod
         def forward(ctx, u0, v0, w0, planes, rollout: Rollout):
-            st = rollout._empty_state()
+            st = rollout._empty_state(zero=True)
             st["u"].copy_(u0)
             st["v"].copy_(v0)
             st["w"].copy_(w0)

@@ -33,13 +33,17 @@ def _ptr(a):
         return ctypes.c_void_p(a.ctypes.data)
     # torch tensor (device or host)
     try:
-        import torch  # noqa: F401
+        import torch
     except ImportError:  # pragma: no cover
         raise TypeError(f"unsupported array type {type(a)}")
     if a.dtype.__repr__() != "torch.float64":
         raise TypeError("tensors must be float64")
     if not a.is_contiguous():
         raise ValueError("tensors must be contiguous")
+    if a.is_cuda:
+        # the context works on its own non-blocking stream: torch's pending
+        # work on the tensor must be complete before the library touches it
+        torch.cuda.current_stream(a.device).synchronize()
     return ctypes.c_void_p(a.data_ptr())
 
 

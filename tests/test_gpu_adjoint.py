@@ -106,7 +106,7 @@ def _shapes(seed, count):
     return out
 
 
-SHAPES = _shapes(77, 4)
+SHAPES = _shapes(77, 4) + [(64, 64, 64), (128, 128, 128)]
 
 
 @pytest.mark.parametrize("n", SHAPES, ids=["x".join(map(str, n)) for n in SHAPES])

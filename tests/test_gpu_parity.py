@@ -284,6 +284,22 @@ def test_step_random_shapes(oracle, n, kind, scheme):
     assert div <= 1e-12
 
 
+@pytest.mark.parametrize("n", [64, 96, 128])
+@pytest.mark.parametrize("scheme", [AB2_CN, AB2], ids=["ab2_cn", "ab2"])
+def test_step_walls_cube(oracle, n, scheme):
+    """re-simulation boxes at their working sizes (n^3 walls, DCT Poisson,
+    dp_flag 1): every axis long enough for several partition blocks, predictor
+    tiles, transform groups and adjoint chunks at once, against the oracle
+    after 3 steps (the random planes carry a net flux through the walls, so
+    the divergence is not zero and is not checked)"""
+    cfg = NSConfig.cube(n, nu=1e-3, dt=5e-4, scheme=scheme, bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=21)
+    planes = rand_planes(cfg, 22, 0.05)
+    out, st, div = run_both(oracle, cfg, u, v, w, p, 3, planes=planes)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+
+
 def cfg_periodic(nx, ny, nz, **kw):
     return NSConfig(nx=nx, ny=ny, nz=nz, dx=2 * np.pi / nx, dy=0.5, dz=0.35, nu=2e-3, dt=2e-3, bc=(1, 1, 1),
                     pbc=(1, 1, 1), flags=FLAG_GAUGE_CORNER, **kw)

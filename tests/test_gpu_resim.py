@@ -30,6 +30,7 @@ def gpu_parent(scheme, nsteps, n=20, dt=4e-3, every=1):
     L = plane_set_length(a, b, c)
     nsets = nsteps // every
     buf = torch.zeros(nsets * L, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()  # the context's stream does not wait for torch's fill
     with NSSolver(cfg) as s:
         s.upload(u, v, w, p, first=True)
         s.step(3)
