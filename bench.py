@@ -57,6 +57,11 @@ def parse():
                     help="sub-boxes of the batched re-simulation measurement (sizes 32..128, seed 4); 0 skips it")
     ap.add_argument("--resim-only", action="store_true",
                     help="run only the adjoint / batch measurements (for profiling them) and print them as one JSON line")
+    ap.add_argument("--pipeline", action="store_true",
+                    help="run the re-simulation pipeline end to end (a 512^3 periodic AB2 parent capturing a 128^3 "
+                         "sub-box every 4th step for 100 steps, then the sub-box re-simulated with walls, AB2-CN, "
+                         "dt/4, 400 sub-steps, planes linearly interpolated), print its timings and errors as one "
+                         "JSON line and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -262,6 +267,76 @@ def batch_throughput(nbox, scheme_name="AB2-CN", steps=20):
             "speedup_vs_eager": round(res["eager"] / res["batch"], 3)}
 
 
+def resim_pipeline(N=512, sub=(101, 201, 151, 128, 128, 128), dt0=5e-4, every=4, parent_steps=100):
+    """The re-simulation workload end to end on one device: the parent
+    (decaying turbulence, big_periodic_3D.f90:107's dp_flag 0, corner gauge)
+    captures the sub-box's plane set every `every` steps
+    (big_periodic_3D.f90:433-489,664-712); the sub-box starts from the parent's
+    state at the first capture and runs `every` * 4 sub-steps per capture
+    interval at dt0/4 with the planes interpolated linearly in time
+    (re_simulation_sub_tstep_interBC_combine.f90:388-396), and is compared
+    with the parent's sub-box at the end (the reference's error table,
+    :826-833)."""
+    import ctypes
+    import math
+
+    import torch
+
+    from ns_3d_staggered_dp_amd import AB2, AB2_CN, FLAG_GAUGE_CORNER, NSConfig, _abi, fields
+    from ns_3d_staggered_dp_amd.grid import plane_set_length
+    from ns_3d_staggered_dp_amd.solver import NSSolver
+
+    x0, y0, z0, a, b, c = sub
+    h = 2 * np.pi / N
+    cfg = NSConfig(nx=N, ny=N, nz=N, dx=h, dy=h, dz=h, nu=1e-3, dt=dt0, scheme=AB2, bc=(1, 1, 1), pbc=(1, 1, 1),
+                   dp_flag=0, flags=FLAG_GAUGE_CORNER)
+    L = plane_set_length(a, b, c)
+    nsets = parent_steps // every + 1
+    buf = torch.zeros(nsets * L, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    keys = ("u", "v", "w", "p", "hx", "hy", "hz")
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=2)
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, first=True)
+        del u, v, w, p
+        s.step(3)
+        sb = _abi.SubBox(x0, y0, z0, a, b, c)
+        _abi.call("ns_capture_begin", s.h, ctypes.byref(sb), ctypes.c_void_p(buf.data_ptr()), nsets, every)
+        s.step(every)  # the first capture: the sub-box's start
+        start = {k: s.extract_subbox(x0, y0, z0, a, b, c, k) for k in keys}
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(parent_steps)
+        s.synchronize()
+        tpar = time.perf_counter() - t0
+        cnt = ctypes.c_int()
+        _abi.call("ns_capture_end", s.h, ctypes.byref(cnt))
+        end = {k: s.extract_subbox(x0, y0, z0, a, b, c, k) for k in ("u", "v", "w", "p")}
+    snaps = buf.view(nsets, L)
+    scfg = NSConfig(nx=a, ny=b, nz=c, dx=h, dy=h, dz=h, nu=cfg.nu, dt=dt0 / 4, scheme=AB2_CN, bc=(2, 2, 2),
+                    pbc=(3, 3, 3), dp_flag=1)
+    nsub = parent_steps * 4
+    with NSSolver(scfg) as r:
+        r.upload(*(start[k] for k in keys), first=False)
+        r.synchronize()
+        t0 = time.perf_counter()
+        for i in range(1, nsub + 1):
+            t = i * dt0 / 4
+            k = min(int(math.floor(t / (every * dt0))), nsets - 2)
+            r.bc_interp(snaps[k], snaps[k + 1], (t - k * every * dt0) / (every * dt0))
+            r.step(1)
+        r.synchronize()
+        tre = time.perf_counter() - t0
+        err, _ = r.error_metrics(end["u"], end["v"], end["w"], end["p"])
+    return {"parent": f"{N}^3 periodic decaying turbulence, AB2, FFT, dp_flag 0, dt={dt0}, capture every {every}",
+            "parent_steps": parent_steps, "parent_ms_per_step": round(tpar / parent_steps * 1e3, 3),
+            "parent_mcell_steps_s": round(N ** 3 * parent_steps / tpar / 1e6, 1), "captured_sets": cnt.value,
+            "sub_box": f"{a}x{b}x{c} at ({x0},{y0},{z0}), walls bc=2 pbc=3, AB2-CN, DCT, dp_flag 1, dt={dt0 / 4}",
+            "sub_steps": nsub, "sub_ms_per_step": round(tre / nsub * 1e3, 4),
+            "sub_mcell_steps_s": round(a * b * c * nsub / tre / 1e6, 1), "sub_total_ms": round(tre * 1e3, 1),
+            "err_max_rel_uvw": [float(x) for x in err[:3]], "err_mean_rel_uvw": [float(x) for x in err[4:7]]}
+
+
 def copy_calibration():
     import torch
 
@@ -310,6 +385,9 @@ def main():
     from ns_3d_staggered_dp_amd import _abi
     from ns_3d_staggered_dp_amd.solver import NSSolver
 
+    if a.pipeline:
+        print(json.dumps({"pipeline": resim_pipeline()}), flush=True)
+        return 0
     if a.resim_only:
         out = {}
         if a.adjoint_n > 0:
