@@ -236,6 +236,19 @@ int ns_check_finite(ns_ctx *ctx);
  *   err[7]    = max|oz - oz_sub| / rms(oz_sub), oz = diff(v,1,1)/dx - diff(u,1,2)/dy */
 int ns_error_metrics(ns_ctx *ctx, const double *u_sub, const double *v_sub, const double *w_sub,
                      const double *p_sub, double err[8], double rms[4]);
+/* ns_error_metrics without any allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory, e.g. a re-simulation loop
+ * computing its errors every step under a no-allocation rule).  u_sub..p_sub
+ * must be device arrays (NS_ERR_INVALID otherwise) and the context's device
+ * must be current.  The block partials live in a static 229 KB device array,
+ * one per GPU, shared by every call on that GPU: calls that overlap on one
+ * GPU (from several host threads or contexts) race and must be serialised
+ * by the caller.  Every box ns_create accepts fits (at most 2048 partial
+ * rows; the kernel walks larger boxes grid-stride).  Sums are combined on the
+ * device in double precision, so results agree with ns_error_metrics to
+ * rounding (its host combination uses extended precision). */
+int ns_error_metrics_noalloc(ns_ctx *ctx, const double *u_sub, const double *v_sub, const double *w_sub,
+                             const double *p_sub, double err[8], double rms[4]);
 
 /* ---- per-subroutine mirrors with the reference's arguments ---- */
 /* cal_conv NS_functions.f90:53-59; conv_* are (nx or nx-1,ny,nz) etc. */

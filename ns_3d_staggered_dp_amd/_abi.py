@@ -67,6 +67,7 @@ SIGNATURES = {
     "ns_max_div": [P, ctypes.POINTER(D)],
     "ns_check_finite": [P],
     "ns_error_metrics": [P, P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)],
+    "ns_error_metrics_noalloc": [P, P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)],
     "ns_resim_advance_spline": [P, P, I, D, D, I],
     "ns_bc_interp_spline": [P, P, I, D, D],
     "ns_spline_weights": [P, I, D, P],

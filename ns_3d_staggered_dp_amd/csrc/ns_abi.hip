@@ -18,6 +18,7 @@ struct ns_ctx {
     Grid g;
     PlaneLayout L;
     hipStream_t st = nullptr;
+    int dev = 0;  // device current at ns_create (owns the stream and every buffer)
     // resident state (Fortran ghosted layout)
     double *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
     double *hx = nullptr, *hy = nullptr, *hz = nullptr;
@@ -641,6 +642,11 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
     if (!valid_cfg(cfg)) return NS_ERR_INVALID;
     ns_ctx *c = new ns_ctx();
     c->cfg = *cfg;
+    if (hipGetDevice(&c->dev) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return NS_ERR_HIP;
+    }
     c->g = Grid{cfg->nx, cfg->ny, cfg->nz};
     c->L = plane_layout(cfg->nx, cfg->ny, cfg->nz);
     int rc = NS_OK;
@@ -1391,6 +1397,41 @@ int ns_error_metrics(ns_ctx *c, const double *u_sub, const double *v_sub, const 
     }
     for (int cc = 0; cc < 3; cc++) err[4 + cc] = (double)(sabs[cc] / n[cc]) / rms[cc];
     err[7] = mx[4] / sqrt((double)(ssq[4] / noz));
+    return NS_OK;
+}
+
+int ns_error_metrics_noalloc(ns_ctx *c, const double *u_sub, const double *v_sub, const double *w_sub,
+                             const double *p_sub, double *err, double *rms) {
+    if (!c || !u_sub || !v_sub || !w_sub || !p_sub || !err || !rms) return NS_ERR_INVALID;
+    // device arrays only: staging a host array would allocate
+    if (!is_dev_ptr(u_sub) || !is_dev_ptr(v_sub) || !is_dev_ptr(w_sub) || !is_dev_ptr(p_sub)) return NS_ERR_INVALID;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != c->dev) return NS_ERR_INVALID;
+    double *scr = nullptr;  // this device's copy of the static scratch
+    int rc = metric_scratch(&scr);
+    if (rc) return rc;
+    materialize(c);
+    const Grid &g = c->g;
+    const double *a[4] = {c->u, c->v, c->w, c->p};
+    const double *b[4] = {u_sub, v_sub, w_sub, p_sub};
+    // the partial kernel walks the arrays grid-stride, so every box fits in
+    // NOALLOC_METRIC_BLOCKS rows; the check guards the scratch's extent
+    const int nb = launch_err_partials(g, c->cfg.dx, c->cfg.dy, a, b, scr, NOALLOC_METRIC_BLOCKS, c->st);
+    if (nb < 1 || nb > NOALLOC_METRIC_BLOCKS) return NS_ERR_INVALID;
+    double *row = scr + (size_t)NOALLOC_METRIC_BLOCKS * NS_METRIC_PARTS;
+    launch_err_combine(scr, nb, row, c->st);
+    double q[NS_METRIC_PARTS];
+    HIPCHK(hipMemcpyAsync(q, row, sizeof(q), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    const double n[4] = {(double)g.nu(), (double)g.nv(), (double)g.nw(), (double)g.np()};
+    const double noz = (double)((size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 2));
+    for (int cc = 0; cc < 4; cc++) {
+        rms[cc] = sqrt(q[3 * cc + 2] / n[cc]);
+        err[cc] = q[3 * cc] / rms[cc];
+    }
+    for (int cc = 0; cc < 3; cc++) err[4 + cc] = q[3 * cc + 1] / n[cc] / rms[cc];
+    err[7] = q[12] / sqrt(q[13] / noz);
     return NS_OK;
 }
 

@@ -262,6 +262,12 @@ void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
 #define NS_METRIC_PARTS 14
 int launch_err_partials(const Grid &g, double dx, double dy, const double *const a[4], const double *const b[4],
                         double *partial, int max_blocks, hipStream_t st);
+// rows of block partials held by the static scratch of ns_error_metrics_noalloc
+#define NOALLOC_METRIC_BLOCKS 2048
+// out[0..NS_METRIC_PARTS) := the nb partial rows combined (max or sum per entry), one block
+void launch_err_combine(const double *partial, int nb, double *out, hipStream_t st);
+// device address of that scratch on the current device (looked up once per device)
+int metric_scratch(double **out);
 
 // ---- discrete adjoint of a step (ns_adjoint.hip) ----
 void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag, const double *const ub[3],

@@ -4,9 +4,20 @@
 // x_sub^2, and for the z vorticity oz = diff(v,1,1)/dx - diff(u,1,2)/dy the
 // max of |oz - oz_sub| and the sum of oz_sub^2.  The host combines the block
 // partials (max, and sums in extended precision).
+//
+// ns_error_metrics_noalloc keeps its block partials in `scratch`, a
+// file-scope device array (one copy per GPU, held until the process ends),
+// and combines them on the device, so it needs no allocation per call.
+#include <mutex>
+
 #include "ns_internal.h"
 
 namespace ns {
+
+// block partials [NOALLOC_METRIC_BLOCKS][NS_METRIC_PARTS], then the combined
+// row (2049 x 14 doubles = 229 KB).  In host code `scratch` names a host-side
+// stand-in: host code uses only the address metric_scratch() returns.
+static __device__ __align__(256) double scratch[(NOALLOC_METRIC_BLOCKS + 1) * NS_METRIC_PARTS];
 
 namespace {
 
@@ -90,6 +101,61 @@ int launch_err_partials(const Grid &g, double dx, double dy, const double *const
     if (nb > (size_t)max_blocks) nb = max_blocks;
     hipLaunchKernelGGL(k_err_partials, dim3((unsigned)nb), dim3(MT), 0, st, P);
     return (int)nb;
+}
+
+namespace {
+// one block: out[q] = max or sum over the nb rows of partial (max for the
+// max|a-b| entries, sum otherwise), rows taken in a fixed order
+__global__ void __launch_bounds__(MT) k_err_combine(const double *__restrict__ partial, int nb,
+                                                    double *__restrict__ out) {
+    __shared__ double red[MT];
+    for (int q = 0; q < NS_METRIC_PARTS; q++) {
+        const bool is_max = (q == 12) || (q < 12 && q % 3 == 0);
+        double a = 0.0;
+        for (int r = threadIdx.x; r < nb; r += MT) {
+            const double x = partial[(size_t)r * NS_METRIC_PARTS + q];
+            a = is_max ? fmax(a, x) : a + x;
+        }
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int s = MT / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s)
+                red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                          : red[threadIdx.x] + red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = red[0];
+        __syncthreads();
+    }
+}
+}  // namespace
+
+void launch_err_combine(const double *partial, int nb, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_err_combine, dim3(1), dim3(MT), 0, st, partial, nb, out);
+}
+
+int metric_scratch(double **out) {
+    // device address of `scratch` on the current device, looked up once per device
+    constexpr int MAXDEV = 64;
+    static std::mutex mtx;
+    static double *addr[MAXDEV] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return NS_ERR_HIP;
+    }
+    if (dev < 0 || dev >= MAXDEV) return NS_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(mtx);
+    if (!addr[dev]) {
+        double *ptr = nullptr;
+        if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return NS_ERR_HIP;
+        }
+        addr[dev] = ptr;
+    }
+    *out = addr[dev];
+    return NS_OK;
 }
 
 }  // namespace ns
