@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_256.json"))
     ap.add_argument("--adjoint-n", type=int, default=128,
@@ -349,6 +349,34 @@ def copy_calibration():
     return 0
 
 
+def host_cpu():
+    """model name and core counts of this host (lscpu; /proc/cpuinfo as a fallback)"""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "1"))
+        cores = int(kv.get("Core(s) per socket", "0"))
+        info.update(sockets=sockets, cores_per_socket=cores, threads_per_core=int(kv.get("Thread(s) per core", "1")),
+                    physical_cores=sockets * cores)
+    except Exception:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return info
+
+
 def cpu_baseline(cfg, u, v, w, p, nsteps):
     from oracle import oracle as O
 
@@ -366,6 +394,7 @@ def cpu_baseline(cfg, u, v, w, p, nsteps):
         "kind": "port",
         "sample": f"{nsteps} AB2 steps of the same {cfg.nx}^3 field after 1 untimed Euler step "
                   f"({dt:.1f} s, oracle/ns_oracle.c, OpenMP)",
+        "host": host_cpu(),
     }
 
 

@@ -62,6 +62,10 @@ struct ns_ctx {
     // explicit schemes alias agb)
     double *agcb[3] = {nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
+    // device copies of host arguments of the mirror entry points, one buffer
+    // per argument position, grown on demand and kept until ns_destroy, so a
+    // host that calls the mirrors every step allocates only on the first call
+    std::vector<std::pair<double *, size_t>> stage;
 };
 
 namespace {
@@ -465,51 +469,60 @@ bool is_dev_ptr(const void *p) {
     return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
 }
 
-// host-or-device argument staged onto the device for a mirror call
+// host-or-device argument staged onto the device for a mirror call; a host
+// argument goes through the context's staging buffer of its position (slot)
 struct Arg {
     double *dev = nullptr;
     void *user = nullptr;
     size_t n = 0;
-    bool owned = false;
+    bool staged = false;
+    int slot = -1;
 };
 
 int arg_in(ns_ctx *c, Arg &a, const double *p, size_t n, bool copy_in) {
     a.user = (void *)p;
     a.n = n;
-    if (!p) return NS_ERR_INVALID;
+    if (!p || a.slot < 0) return NS_ERR_INVALID;
     if (is_dev_ptr(p)) {
         a.dev = (double *)p;
-        a.owned = false;
+        a.staged = false;
         return NS_OK;
     }
-    if (hipMalloc(&a.dev, n * sizeof(double)) != hipSuccess) {
-        (void)hipGetLastError();
-        return NS_ERR_NOMEM;
+    if (c->stage.size() <= (size_t)a.slot) c->stage.resize((size_t)a.slot + 1, {nullptr, 0});
+    auto &buf = c->stage[(size_t)a.slot];
+    if (buf.second < n) {
+        if (buf.first) {
+            // earlier calls' copies through this buffer are ordered on the stream
+            HIPCHK(hipStreamSynchronize(c->st));
+            (void)hipFree(buf.first);
+        }
+        buf = {nullptr, 0};
+        void *q = nullptr;
+        if (hipMalloc(&q, (n ? n : 1) * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            return NS_ERR_NOMEM;
+        }
+        buf = {(double *)q, n};
     }
-    a.owned = true;
+    a.dev = buf.first;
+    a.staged = true;
     if (copy_in) HIPCHK(hipMemcpyAsync(a.dev, p, n * sizeof(double), hipMemcpyHostToDevice, c->st));
     return NS_OK;
 }
 
 int arg_out(ns_ctx *c, Arg &a) {
-    if (a.owned) HIPCHK(hipMemcpyAsync(a.user, a.dev, a.n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (a.staged) HIPCHK(hipMemcpyAsync(a.user, a.dev, a.n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     return NS_OK;
 }
 
-void arg_free(Arg &a) {
-    if (a.owned && a.dev) (void)hipFree(a.dev);
-    a.dev = nullptr;
-}
-
-// std::deque keeps references to earlier elements valid across push_back
-// (a std::vector would reallocate and leave them dangling)
+// the arguments of one call, each with its own staging slot.  std::deque
+// keeps references to earlier elements valid across push_back (a std::vector
+// would reallocate and leave them dangling)
 struct ArgList {
     std::deque<Arg> v;
-    ~ArgList() {
-        for (auto &a : v) arg_free(a);
-    }
     Arg &add() {
         v.emplace_back();
+        v.back().slot = (int)v.size() - 1;
         return v.back();
     }
     Arg &back() { return v.back(); }
@@ -729,6 +742,8 @@ int ns_destroy(ns_ctx *c) {
     for (int i = 0; i < 2; i++)
         if (c->graph[i]) (void)hipGraphExecDestroy(c->graph[i]);
     for (void *q : c->allocs) (void)hipFree(q);
+    for (auto &b : c->stage)
+        if (b.first) (void)hipFree(b.first);
     if (c->snaps) (void)hipFree(c->snaps);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -1270,7 +1285,7 @@ int ns_adjoint_step(ns_ctx *c, const double *u, const double *v, const double *w
                        c->apb, c->st);
     if (planes_bar) HIPCHK(hipMemcpyAsync(planes_bar, c->apbar, (size_t)c->L.total * 8, hipMemcpyDefault, c->st));
     // host outputs and staged host inputs need the stream drained; device ones stay stream-ordered
-    if ((planes_bar && !is_dev_ptr(planes_bar)) || AU.owned || AV.owned || AW.owned)
+    if ((planes_bar && !is_dev_ptr(planes_bar)) || AU.staged || AV.staged || AW.staged)
         HIPCHK(hipStreamSynchronize(c->st));
     return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
 }

@@ -17,7 +17,8 @@ from ns_3d_staggered_dp_amd.config import CConfig, NSConfig
 from ns_3d_staggered_dp_amd.grid import field_shapes, plane_set_length
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libns_oracle.so")
+# NS_ORACLE_LIB selects another build of the same source (the sanitizer build)
+LIB_PATH = os.environ.get("NS_ORACLE_LIB", os.path.join(HERE, "_build", "libns_oracle.so"))
 _lib = None
 
 _D = ctypes.POINTER(ctypes.c_double)

@@ -200,3 +200,69 @@ def test_rollout_gradient_matches_central_differences(scheme, lazy):
         ad = sum(float((g * dd).sum()) for g, dd in zip(grads, d))
         assert abs(fd - ad) <= 1e-6 * abs(ad), (trial, fd, ad)
     ro.close()
+
+
+def _rollout_inputs(cfg, nsteps, seed):
+    import torch
+
+    st, rng = rand_state(cfg, seed, amp=0.3)
+    L = plane_set_length(cfg.nx, cfg.ny, cfg.nz)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).cuda()
+    x = [dev(st["u"]), dev(st["v"]), dev(st["w"]), dev(0.1 * rng.standard_normal(nsteps * L))]
+    target = [dev(0.3 * rng.standard_normal(st[k].shape)) for k in ("u", "v", "w")]
+    return x, target
+
+
+def test_rollout_two_applications_before_backward():
+    """two applications on one Rollout, then one backward of a loss that
+    uses both: each application reverses from its own checkpoints and planes,
+    so the gradients equal those of each application differentiated alone"""
+    import torch
+
+    from ns_3d_staggered_dp_amd.adjoint import Rollout, make_rollout_function
+
+    cfg = box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2_CN, dp_flag=1)
+    nsteps = 4
+    (xa, ta), (xb, tb) = _rollout_inputs(cfg, nsteps, 61), _rollout_inputs(cfg, nsteps, 62)
+    ro = Rollout(cfg, nsteps, checkpoint_every=2)
+    F = make_rollout_function()
+
+    def mismatch(xs, target):
+        return 0.5 * sum(((o - t) ** 2).sum() for o, t in zip(F.apply(*xs, ro), target))
+
+    alone = []
+    for x, t in ((xa, ta), (xb, tb)):
+        xs = [a.clone().requires_grad_(True) for a in x]
+        alone.append(torch.autograd.grad(mismatch(xs, t), xs))
+    xsa = [a.clone().requires_grad_(True) for a in xa]
+    xsb = [a.clone().requires_grad_(True) for a in xb]
+    together = torch.autograd.grad(mismatch(xsa, ta) + mismatch(xsb, tb), xsa + xsb)
+    ro.close()
+    for g, ref in zip(together, alone[0] + alone[1]):
+        assert torch.equal(g, ref)
+
+
+def test_rollout_without_planes_steps_with_zero_planes():
+    """planes=None means zero plane sets, whatever set the context held
+    before; the planes slot of the gradient is then None"""
+    import torch
+
+    from ns_3d_staggered_dp_amd.adjoint import Rollout, make_rollout_function
+
+    cfg = box(12, 10, 8, bc=(2, 2, 2), pbc=(3, 3, 3), scheme=AB2, dp_flag=1)
+    nsteps = 4
+    x, target = _rollout_inputs(cfg, nsteps, 63)
+    ro = Rollout(cfg, nsteps, checkpoint_every=2)
+    F = make_rollout_function()
+    with torch.no_grad():
+        F.apply(*x, ro)  # leaves the last of x's non-zero plane sets in the context
+    zeros = torch.zeros_like(x[3])
+    results = []
+    for pl in (None, zeros):
+        xs = [a.clone().requires_grad_(True) for a in x[:3]]
+        out = F.apply(*xs, pl, ro)
+        loss = 0.5 * sum(((o - t) ** 2).sum() for o, t in zip(out, target))
+        results.append(([o.detach() for o in out], torch.autograd.grad(loss, xs)))
+    ro.close()
+    for a, b in zip(results[0][0] + list(results[0][1]), results[1][0] + list(results[1][1])):
+        assert torch.equal(a, b)

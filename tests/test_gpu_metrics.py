@@ -45,3 +45,36 @@ def test_error_metrics_after_steps_periodic():
                                                  ref["p"], cfg.dx, cfg.dy)
     assert np.allclose(rms, rms_ref, rtol=1e-13, atol=0)
     assert np.allclose(err, err_ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("case", ["clean", "nan_in_u", "inf_in_p", "nan_in_history", "lazy_periodic_nan"])
+def test_check_finite_reports_injected_nonfinite(case):
+    """ns_check_finite returns NS_ERR_NONFINITE for a state holding a NaN or
+    an Inf, uploaded (or carried through steps that spread it), and NS_OK
+    for a clean one"""
+    from ns_3d_staggered_dp_amd import FLAG_GAUGE_CORNER, _abi
+
+    if case == "lazy_periodic_nan":
+        cfg = NSConfig(nx=32, ny=16, nz=12, dx=2 * np.pi / 32, dy=0.4, dz=0.5, nu=2e-3, dt=2e-3, scheme=AB2,
+                       flags=FLAG_GAUGE_CORNER)
+        u, v, w, p = fields.decaying_turbulence(cfg, seed=12)
+    else:
+        cfg = NSConfig(nx=20, ny=12, nz=10, dx=0.31, dy=0.27, dz=0.23, nu=0.01, dt=1e-3, scheme=AB2, bc=(2, 2, 2),
+                       pbc=(3, 3, 3), dp_flag=1)
+        u, v, w, p = fields.random_ghosted(cfg, 93)
+        u, v, w = 0.3 * u, 0.3 * v, 0.3 * w
+    hx = np.zeros_like(u)
+    if case in ("nan_in_u", "lazy_periodic_nan"):
+        u[5, 3, 4] = np.nan
+    elif case == "inf_in_p":
+        p[2, 2, 2] = np.inf
+    elif case == "nan_in_history":
+        hx[6, 5, 4] = np.nan
+    want = _abi.NS_OK if case == "clean" else _abi.NS_ERR_NONFINITE
+    with NSSolver(cfg) as s:
+        s.upload(u, v, w, p, hx, None, None, first=False)
+        # the history enters the state only through a step
+        if case != "nan_in_history":
+            assert _abi.lib().ns_check_finite(s.h) == want
+        s.step(2)
+        assert _abi.lib().ns_check_finite(s.h) == want

@@ -407,3 +407,80 @@ def test_graph_replay_matches_eager(case):
     for a, b in zip(*runs):
         for k in ("u", "v", "w", "p", "hx", "hy", "hz"):
             assert np.array_equal(a[k], b[k]), k
+
+
+# ---- boundary branches against independent numpy statements (tests/manufactured.py) ----
+
+def test_manufactured_mixed_poisson_gpu(oracle):
+    """ns_pr_bc_staggered_modify_rhs + ns_dct_poisson_solver on the
+    reference's manufactured DST-2 / DCT-2 problem (test_subroutine.f90:280-297):
+    second-order convergence to the exact solution, and the oracle's solve
+    to 1e-12"""
+    from manufactured import manufactured_case
+
+    from ns_3d_staggered_dp_amd.grid import pack_planes
+
+    errs = []
+    for n in (16, 32, 64):
+        h, rhs, pl, phi = manufactured_case(n)
+        nz = rhs.shape[2]
+        planes = pack_planes(n, n, nz, pl)
+        cfg = NSConfig(nx=n, ny=n, nz=nz, dx=h, dy=h, dz=h, bc=(2, 2, 2), pbc=(2, 3, 3), dp_flag=1)
+        r = rhs.copy(order="F")
+        with NSSolver(cfg) as s:
+            s.pr_bc_staggered_modify_rhs(r, planes)
+            sol = s.DCT_poisson_solver(r, 0.0)
+        ro = rhs.copy(order="F")
+        oracle.pr_bc_modify_rhs(ro, planes, (2, 3, 3), h, h, h)
+        assert np.array_equal(r, ro)
+        assert rel(sol, oracle.poisson_dct(ro, (2, 3, 3), h, h, h, 0.0)) <= 1e-12
+        errs.append(np.abs(sol - phi).max())
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(orders > 1.95), (errs, orders)
+
+
+@pytest.mark.parametrize("pbc", [(2, 2, 2), (3, 3, 3), (4, 4, 4), (2, 3, 4)])
+def test_modify_rhs_moves_the_fill_terms_gpu(pbc):
+    """ns_pr_bc_staggered_modify_rhs adds exactly the planes' part of the
+    Laplacian of the field ns_pr_bc_staggered fills"""
+    from manufactured import interior_laplacian
+
+    cfg = cfg_box(9, 7, 6, pbc=pbc)
+    rng = np.random.default_rng(31)
+    p = np.asfortranarray(rng.standard_normal((11, 9, 8)))
+    planes = rng.standard_normal(plane_set_length(9, 7, 6))
+    a, b = p.copy(order="F"), p.copy(order="F")
+    d = (cfg.dx, cfg.dy, cfg.dz)
+    with NSSolver(cfg) as s:
+        s.pr_bc_staggered(a, planes)
+        s.pr_bc_staggered(b, np.zeros_like(planes))
+        r = np.asfortranarray(interior_laplacian(a, *d))
+        s.pr_bc_staggered_modify_rhs(r, planes)
+    ref = interior_laplacian(b, *d)
+    assert np.abs(r - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("bc", [2, 4])
+@pytest.mark.parametrize("direction", [1, 2, 3])
+def test_cn_rows_and_sweep_solve_the_line_equations_gpu(bc, direction):
+    """ns_vel_bc_staggered_cn2 (codes 2, 4) writes the planes into the line
+    ends and ns_adi_sweep then satisfies the CN line equations stated in numpy"""
+    from manufactured import cn2_rows, cn_line_residual
+
+    from ns_3d_staggered_dp_amd.grid import unpack_planes
+
+    cfg = cfg_box(12, 10, 8, bc=(bc, bc, bc), pbc=(3, 3, 3) if bc == 2 else (2, 2, 2), scheme=AB2_CN,
+                  dp_flag=1).replace(dt=0.05, nu=0.5)
+    u, v, w, _ = rand_fields(cfg, 40 + direction)
+    planes = np.random.default_rng(50 + direction).standard_normal(plane_set_length(12, 10, 8))
+    r = cn2_rows(u, v, w, unpack_planes(12, 10, 8, planes), direction)
+    with NSSolver(cfg) as s:
+        s.vel_bc_staggered_CN2(u, v, w, planes, bc, direction)
+        for x, ref in zip((u, v, w), r):
+            assert np.array_equal(x, ref)
+        s.adi_sweep(u, v, w, direction)
+    h = (cfg.dx, cfg.dy, cfg.dz)[direction - 1]
+    c = 0.5 * cfg.dt * cfg.nu / (h * h)
+    for comp, (x, rr) in enumerate(zip((u, v, w), r)):
+        res = cn_line_residual(x, rr, direction - 1, c, comp == direction - 1, bc)
+        assert res <= 1e-12 * np.abs(rr).max(), (comp, res)

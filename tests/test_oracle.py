@@ -466,3 +466,72 @@ def test_captured_planes_reproduce_the_parent(oracle, scheme):
             ref = snaps[k][name]
             err = np.abs(getattr(st, name) - ref).max() / np.sqrt(np.mean(ref ** 2))
             assert err < 1e-12, (k, name, err)
+
+
+# ---------------- boundary branches pinned by independent statements ----------------
+def test_manufactured_mixed_poisson_converges(oracle):
+    """the reference's manufactured DST-2 / DCT-2 problem (x: Dirichlet on
+    the cell wall, y: Neumann; test_subroutine.f90:280-297, extruded in z):
+    pr_bc_staggered_modify_rhs + DCT_poisson_solver converge to the exact
+    solution at second order (the interior stencil is exact on this cubic,
+    the wall treatments are second order)"""
+    from manufactured import manufactured_case
+
+    errs = []
+    for n in (16, 32, 64):
+        h, rhs, pl, phi = manufactured_case(n)
+        planes = pack_planes(n, n, rhs.shape[2], pl)
+        r = rhs.copy(order="F")
+        oracle.pr_bc_modify_rhs(r, planes, (2, 3, 3), h, h, h)
+        sol = oracle.poisson_dct(r, (2, 3, 3), h, h, h, 0.0)
+        errs.append(np.abs(sol - phi).max())
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(orders > 1.95), (errs, orders)
+    assert errs[-1] < 2e-4, errs
+
+
+@pytest.mark.parametrize("pbc", [(2, 2, 2), (3, 3, 3), (4, 4, 4), (2, 3, 4)])
+def test_modify_rhs_moves_the_fill_terms(oracle, pbc):
+    """the terms pr_bc_staggered_modify_rhs adds (NS_functions.f90:1345-1400)
+    are exactly the planes' part of the Laplacian of the field that
+    pr_bc_staggered fills (:1403-1464): lap(fill(p, planes)) + terms ==
+    lap(fill(p, 0)), for each code"""
+    from manufactured import interior_laplacian
+
+    n, d = (9, 7, 6), (0.31, 0.27, 0.23)
+    rng = np.random.default_rng(31)
+    p = np.asfortranarray(rng.standard_normal((n[0] + 2, n[1] + 2, n[2] + 2)))
+    planes = rng.standard_normal(plane_set_length(*n))
+    a, b = p.copy(order="F"), p.copy(order="F")
+    oracle.pr_bc(a, planes, pbc, *d)
+    oracle.pr_bc(b, np.zeros_like(planes), pbc, *d)
+    r = np.asfortranarray(interior_laplacian(a, *d))
+    oracle.pr_bc_modify_rhs(r, planes, pbc, *d)
+    ref = interior_laplacian(b, *d)
+    assert np.abs(r - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("bc", [2, 4])
+@pytest.mark.parametrize("direction", [1, 2, 3])
+def test_cn_rows_and_sweep_solve_the_line_equations(oracle, bc, direction):
+    """vel_bc_staggered_CN2 puts the planes into the line ends
+    (NS_functions.f90:337-405, codes 2 and 4) and the sweep then satisfies the
+    CN line equations of mat_CN_tridiagonal (:1297-1342) stated in numpy"""
+    from manufactured import cn2_rows, cn_line_residual
+
+    from ns_3d_staggered_dp_amd.grid import unpack_planes
+
+    cfg = NSConfig(nx=12, ny=10, nz=8, dx=0.31, dy=0.27, dz=0.23, nu=0.5, dt=0.05, bc=(bc, bc, bc),
+                   pbc=(3, 3, 3) if bc == 2 else (2, 2, 2), scheme=AB2_CN)
+    u, v, w, _ = fields.random_ghosted(cfg, 40 + direction)
+    planes = np.random.default_rng(50 + direction).standard_normal(plane_set_length(12, 10, 8))
+    r = cn2_rows(u, v, w, unpack_planes(12, 10, 8, planes), direction)
+    oracle.vel_bc_cn2(u, v, w, planes, bc, direction)
+    for x, ref in zip((u, v, w), r):
+        assert np.array_equal(x, ref)
+    oracle.adi_sweep(u, v, w, cfg, direction)
+    h = (cfg.dx, cfg.dy, cfg.dz)[direction - 1]
+    c = 0.5 * cfg.dt * cfg.nu / (h * h)
+    for comp, (x, rr) in enumerate(zip((u, v, w), r)):
+        res = cn_line_residual(x, rr, direction - 1, c, comp == direction - 1, bc)
+        assert res <= 1e-12 * np.abs(rr).max(), (comp, res)
