@@ -7,10 +7,13 @@
 // through z ("2.5-D" blocking).  Each component keeps a ring of four z-slices
 // of its halo'd 66 x 6 tile in LDS, so every input value is fetched from
 // memory about 1.5 times (the x/y halo, mostly served by L2) instead of once
-// per stencil use.  Each z step stores the slices loaded into registers two
-// steps earlier, issues the loads for two steps ahead (and of the AB2 history
-// of the next cell), passes one barrier and computes; four slots make that
-// single barrier sufficient.  Only the points of the reference's
+// per stencil use.  Each z step stores the slices loaded into registers one
+// step earlier, issues the loads of the next step (and the AB2 history of
+// the next cell), passes one barrier and computes; four slots make that
+// single barrier sufficient.  Every load and LDS store is unconditional
+// straight-line code (clamped addresses), so the compiler waits for exactly
+// the loads a store needs and the next step's loads stay in flight across
+// the barrier and the compute.  Only the points of the reference's
 // assignment ranges are computed (x_range0 etc., big_periodic_3D.f90:126-146);
 // the ghost entries of u* are rewritten by the boundary fill or discarded by
 // the ADI sweeps in every scheme, so they are left untouched, and so are the
@@ -116,6 +119,7 @@ struct Tiles {
     double v[NVS][VH][HX];
     double w[NVS][HY][HX];
     double d[LAZY ? NDS : 1][DYT][DXT];
+    double spare[64];  // target of the slice stores of lanes past a tile's end
 };
 
 // one block's column of tiles; b is the block's index within its box
@@ -149,11 +153,18 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
     unsigned int offd[NE];     // dp tile
     short ldsd[NE];
     short plo[3][NE], phi[3][NE];  // dp tile positions of the lazy projection (phi < 0: next dp slice, w)
+    // Lanes past a tile's end load the tile's last element (the same address
+    // as a lane that needs it, so no extra traffic) and store into T.spare:
+    // every load and LDS store is then unconditional, straight-line code,
+    // which lets the compiler count the outstanding loads exactly instead of
+    // draining them all (a branch around a load makes it wait for every
+    // earlier load, including the prefetch for two steps ahead).
     for (int s_ = 0; s_ < NE; s_++) {
-        const int e = threadIdx.x + s_ * NT;
+        const int e0 = threadIdx.x + s_ * NT;
         // u: HY rows of UW; column HX is u(nx+1)
         {
-            const bool ok = e < HY * UW;
+            const bool ok = e0 < HY * UW;
+            const int e = ok ? e0 : HY * UW - 1;
             const int la = e % UW, lb = e / UW;
             int gi = (la == HX) ? P.nx + 1 : ox[0] + la, gj = oy[0] + lb;
             if (gi == 0 && P.per[0]) gi = P.nx;
@@ -168,7 +179,8 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
         }
         // v: VH rows of HX; row HY is v(ny+1)
         {
-            const bool ok = e < VH * HX;
+            const bool ok = e0 < VH * HX;
+            const int e = ok ? e0 : VH * HX - 1;
             const int la = e % HX, lb = e / HX;
             int gi = ox[1] + la, gj = (lb == HY) ? P.ny + 1 : oy[1] + lb;
             if (gj == 0 && P.per[1]) gj = P.ny;
@@ -182,7 +194,8 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
         }
         // w: HY rows of HX
         {
-            const bool ok = e < HY * HX;
+            const bool ok = e0 < HY * HX;
+            const int e = ok ? e0 : HY * HX - 1;
             const int la = e % HX, lb = e / HX;
             const int gi = clampi(ox[2] + la, 1, nn[2][0]), gj = clampi(oy[2] + lb, 1, nn[2][1]);
             off[2][s_] = (unsigned int)((gi - 1) + nn[2][0] * (gj - 1));
@@ -192,7 +205,8 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
         }
         // dp: DYT rows of DXT starting at (a0, b0)
         {
-            const bool ok = LAZY && e < DYT * DXT;
+            const bool ok = LAZY && e0 < DYT * DXT;
+            const int e = (e0 < DYT * DXT) ? e0 : DYT * DXT - 1;
             const int la = e % DXT, lb = e / DXT;
             int gi = a0 + la, gj = b0 + lb;
             if (gi == 0 && P.per[0]) gi = P.nx;
@@ -221,26 +235,27 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
         double d[NE];
     };
     // vel slices: u, v at su, w at sw
+    double *const spare = &T.spare[threadIdx.x & 63];
     auto fetch_vel = [&](VRegs &R, int su, int sw) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const double *src = P.src[c] + (size_t)(kvel(c, c == 2 ? sw : su) - 1) * ps[c];
 #pragma unroll
-            for (int s_ = 0; s_ < NE; s_++)
-                if (lds[c][s_] >= 0) R.v[c][s_] = src[off[c][s_]];
+            for (int s_ = 0; s_ < NE; s_++) R.v[c][s_] = src[off[c][s_]];
         }
     };
     auto fetch_dp = [&](DRegs &D, int s) {
         const double *src = P.dp + (size_t)(kdp(s) - 1) * pp;
 #pragma unroll
-        for (int s_ = 0; s_ < NE; s_++)
-            if (ldsd[s_] >= 0) D.d[s_] = src[offd[s_]];
+        for (int s_ = 0; s_ < NE; s_++) D.d[s_] = src[offd[s_]];
     };
     auto store_dp = [&](const DRegs &D, int s) {
         double *dst = &T.d[s % NDS][0][0];
 #pragma unroll
-        for (int s_ = 0; s_ < NE; s_++)
-            if (ldsd[s_] >= 0) dst[ldsd[s_]] = P.gauge ? D.d[s_] - gauge : D.d[s_];
+        for (int s_ = 0; s_ < NE; s_++) {
+            double *q = (ldsd[s_] >= 0) ? dst + ldsd[s_] : spare;
+            *q = D.d[s_] - gauge;
+        }
     };
     // projection u = u* - dt*(dp(+1) - dp)/d (big_periodic_3D.f90:562-567), with the
     // dp slices (gauge already applied) taken from the LDS ring
@@ -252,7 +267,6 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             const double d = (c == 0) ? P.dx : (c == 1) ? P.dy : P.dz;
 #pragma unroll
             for (int s_ = 0; s_ < NE; s_++) {
-                if (lds[c][s_] < 0) continue;
                 double x = R.v[c][s_];
                 if (LAZY) {
                     const double *dl = &T.d[s % NDS][0][0];
@@ -260,7 +274,8 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
                     const double gr = dh[phi[c][s_]] - dl[plo[c][s_]];
                     x = x - qdiv(P.dt * gr, d, P.rd[c]);
                 }
-                dst[lds[c][s_]] = x;
+                double *q = (lds[c][s_] >= 0) ? dst + lds[c][s_] : spare;
+                *q = x;
             }
         }
     };
@@ -268,68 +283,63 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
     // output cell of this thread and its first-slice offsets
     const int a = a0 + tx, bb = b0 + ty;
     const bool act = a < P.nx && bb < P.ny;
-    const bool needH = P.scheme != NS_EULER && !P.first;
     size_t q0[3];
     bool on[3];
+    // lanes past the box read a valid cell's history (discarded)
+    const int ac = min(a, P.nx - 1), bc = min(bb, P.ny - 1);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int own = (c == 0) ? a : bb;
         on[c] = act && (c == 2 || P.per[c] || own != 0);
-        q0[c] = FIX(gidx(c, 0, a), gidx(c, 1, bb), gidx(c, 2, 0), nn[c][0], nn[c][1]);
+        q0[c] = FIX(gidx(c, 0, ac), gidx(c, 1, bc), gidx(c, 2, 0), nn[c][0], nn[c][1]);
     }
+    // the history of cells this thread does not compute is loaded and never
+    // used (nor is it used by the first / Euler steps), so the load needs no
+    // condition: a select on the loaded value would wait for the load at once
     auto fetch_h = [&](double hr[3], int az) {
+        const size_t k = (size_t)min(az, P.nz - 1);  // past the column end: a valid cell, not used
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const bool own_ok = (c != 2) || P.per[2] || az != 0;
-            if (needH && on[c] && own_ok) hr[c] = P.h[c][q0[c] + (size_t)az * ps[c]];
-            else hr[c] = 0.0;
-        }
+        for (int c = 0; c < 3; c++) hr[c] = P.h[c][q0[c] + k * ps[c]];
     };
 
     // prologue: dp slices z0..z0+2, then u, v slices z0+1, z0+2 and w slices z0, z0+1
-    // Two register sets (A: even steps, B: odd steps) keep the loads of a
-    // step's slices in flight for two steps before they are stored.
-    VRegs RA, RB;
-    DRegs DA, DB;
+    // into LDS, then the registers of the first step's store phase
+    VRegs R;
+    DRegs D;
     if (LAZY) {
         for (int s = z0; s < z0 + 3; s++) {
-            fetch_dp(DA, s);
-            store_dp(DA, s);
+            fetch_dp(D, s);
+            store_dp(D, s);
         }
         __syncthreads();
     }
     for (int q = 0; q < 2; q++) {
-        fetch_vel(RA, z0 + 1 + q, z0 + q);
-        store_vel(RA, z0 + 1 + q, z0 + q);
+        fetch_vel(R, z0 + 1 + q, z0 + q);
+        store_vel(R, z0 + 1 + q, z0 + q);
     }
     if (LAZY) {
-        fetch_dp(DA, z0 + 3);
+        fetch_dp(D, z0 + 3);
         __syncthreads();  // dp slot (z0+3) % 3 was read above
-        store_dp(DA, z0 + 3);
-        fetch_dp(DA, z0 + 4);
-        fetch_dp(DB, z0 + 5);
+        store_dp(D, z0 + 3);
+        fetch_dp(D, z0 + 4);
     }
-    fetch_vel(RA, z0 + 3, z0 + 2);
-    fetch_vel(RB, z0 + 4, z0 + 3);
-    double hn[3];
-    fetch_h(hn, z0);
+    fetch_vel(R, z0 + 3, z0 + 2);
+    double hreg[3];
+    fetch_h(hreg, z0);
     if (LAZY) __syncthreads();  // dp slice z0+3 visible to the first store phase
 
     const double dx = P.dx, dy = P.dy, dz = P.dz, dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
     const double rx = P.rd[0], ry = P.rd[1], rz = P.rd[2], rx2 = P.rd2[0], ry2 = P.rd2[1], rz2 = P.rd2[2];
     const double dt = P.dt, nu = P.nu, dpf = (double)P.dp_flag;
-    auto zstep = [&](const int az, VRegs &R, DRegs &D) {
-        // store phase: this step's newest slices (u, v at az+3, w at az+2; dp at az+4)
+    // one z step: store the slices loaded one step earlier (u, v at az+3,
+    // w at az+2; dp at az+4), load the next step's into the same registers
+    // (they overlap the barrier and the compute; past the end of the chunk
+    // they read clamped, valid addresses and are not used) and compute
+    auto zstep = [&](const int az) {
         if (LAZY) store_dp(D, az + 4);
         store_vel(R, az + 3, az + 2);
-        double hc[3] = {hn[0], hn[1], hn[2]};
-        // loads for two steps ahead (same register set) and the next AB2 history;
-        // they overlap the barrier and the compute below
-        if (az + 2 < z1) {
-            fetch_vel(R, az + 5, az + 4);
-            if (LAZY) fetch_dp(D, az + 6);
-        }
-        if (az + 1 < z1) fetch_h(hn, az + 1);
+        fetch_vel(R, az + 4, az + 3);
+        if (LAZY) fetch_dp(D, az + 5);
         __syncthreads();
         if (act) {
 #define LU(gi, gj, gk) T.u[(gk) % NVS][(gj) - oy[0]][(gi) - ox[0]]
@@ -426,7 +436,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
                 } else if (P.scheme == NS_AB2) {
                     rhs = nu * df - cv;
                     if (P.first) val = dt * (1.0 * rhs - dpf * dpd) + c0;
-                    else val = dt * (1.5 * rhs - 0.5 * hc[c] - dpf * dpd) + c0;
+                    else val = dt * (1.5 * rhs - 0.5 * hreg[c] - dpf * dpd) + c0;
                     H[q] = rhs;
                 } else {
                     if (P.first) {
@@ -434,7 +444,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
                         val = dt * (rhs - dpf * dpd) + c0;
                     } else {
                         rhs = 0.5 * nu * df - 1.5 * cv;
-                        val = dt * (rhs - 0.5 * hc[c] - dpf * dpd) + c0;
+                        val = dt * (rhs - 0.5 * hreg[c] - dpf * dpd) + c0;
                     }
                     H[q] = -cv;
                 }
@@ -447,11 +457,15 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
 #undef LV
 #undef LW
         }
+        // the next cell's AB2 history, loaded after its last use here: it
+        // lands during the next step's store phase and conv/diff compute.
+        // The empty asm (a compiler-only memory barrier, no instruction)
+        // keeps the compiler from sinking the load to its use in the next
+        // step, where it would wait for every outstanding load.
+        fetch_h(hreg, az + 1);
+        asm volatile("" ::: "memory");
     };
-    for (int az = z0; az < z1; az += 2) {
-        zstep(az, RA, DA);
-        if (az + 1 < z1) zstep(az + 1, RB, DB);
-    }
+    for (int az = z0; az < z1; az++) zstep(az);
 }
 
 template <bool LAZY>
