@@ -32,12 +32,15 @@ PHASES = ["predictor", "vel_bc", "div_fft_x_fwd", "fft_y_fwd", "fft_z_solve", "f
           "pr_bc", "project"]
 ARRAYS_PER_CELL = {"predictor": 13, "vel_bc": 0, "div_fft_x_fwd": 4, "fft_y_fwd": 2, "fft_z_solve": 2,
                    "fft_y_inv": 2, "fft_x_inv_dp": 2, "pr_bc": 0, "project": 0}
+# of which read
+ARRAYS_READ = {"predictor": 7, "div_fft_x_fwd": 3, "fft_y_fwd": 1, "fft_z_solve": 1, "fft_y_inv": 1, "fft_x_inv_dp": 1}
 # the reference pipeline's budget (explicit projection pass): 32 arrays = 256 B/cell-step
 REFERENCE_BUDGET_B = 256
 KERNEL_NAMES = {"predictor": "k_predict_tiled<true>", "div_fft_x_fwd": "k_fft_x_fwd<false,512>",
                 "fft_y_fwd": "k_fft_yz<1,-1,0,false,2048>", "fft_z_solve": "k_fft_yz<2,-1,1,false,1024>",
                 "fft_y_inv": "k_fft_yz<1,1,0,false,2048>", "fft_x_inv_dp": "k_fft_x_inv<false,1024>", "project": "k_project"}
-COPY_CAL_BYTES = 2 ** 30  # the device-to-device copy used to scale the HBM counters
+COPY_CAL_BYTES = 2 ** 30  # bytes each way of the streaming copy (ns_stream_copy) that scales the traffic counters
+CAL_KERNEL = "k_stream_copy"
 
 
 def parse():
@@ -90,8 +93,9 @@ def short_kernel_name(name):
 
 
 def summarize_profile(d, traffic_json):
-    """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per
-    dispatch), each scaled by what the same counter reads for the 1 GiB copy."""
+    """Per-kernel traffic between the L2s and the memory fabric (Infinity Cache hits
+    included) from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch),
+    each scaled so that the library's streaming copy of 1 GiB reads and writes 1 GiB."""
     import csv
     import glob
 
@@ -105,9 +109,8 @@ def summarize_profile(d, traffic_json):
 
     fetch, write = counter("FETCH_SIZE", "FETCH_SIZE"), counter("WRITE_SIZE", "WRITE_SIZE")
     cf, cw = counter("cal_FETCH_SIZE", "FETCH_SIZE"), counter("cal_WRITE_SIZE", "WRITE_SIZE")
-    copy = [k for k in cf if "copyBuffer" in k]
-    f_fac = (COPY_CAL_BYTES / 1024) / max(cf[k] for k in copy)
-    w_fac = (COPY_CAL_BYTES / 1024) / max(cw[k] for k in [k for k in cw if "copyBuffer" in k])
+    f_fac = (COPY_CAL_BYTES / 1024) / cf[CAL_KERNEL]
+    w_fac = (COPY_CAL_BYTES / 1024) / cw[CAL_KERNEL]
     stats = {}
     for f in glob.glob(os.path.join(d, "kt", "*kernel_stats.csv")):
         for r in csv.DictReader(open(f)):
@@ -123,22 +126,28 @@ def summarize_profile(d, traffic_json):
         traffic[kname] = int(rd + wr)
         alg = ARRAYS_PER_CELL.get(phase, 0) * 8 * n3 if phase != "project" else 8 * 8 * n3
         calls, us = stats.get(kname, (0, float("nan")))
+        nr, nw = ARRAYS_READ.get(phase, 0), ARRAYS_PER_CELL.get(phase, 0) - ARRAYS_READ.get(phase, 0)
+        if phase == "project":
+            nr, nw = 4, 4
+        ar, aw = nr * 8 * n3, nw * 8 * n3
         rows.append(f"| {kname} | {calls} | {us:.1f} | {alg / 1e6:.0f} | {rd / 1e6:.0f} | {wr / 1e6:.0f} | "
-                    f"{(rd + wr) / alg:.2f} | {alg / (us * 1e-6) / 1e9:.0f} | {(rd + wr) / (us * 1e-6) / 1e9:.0f} |")
+                    f"{rd / ar:.2f} | {wr / aw:.2f} | {alg / (us * 1e-6) / 1e9:.0f} | {(rd + wr) / (us * 1e-6) / 1e9:.0f} |")
     if rows:
         json.dump(traffic, open(traffic_json, "w"), indent=1)
-    print(f"FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} (1 GiB copy)")
+    print(f"FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} ({CAL_KERNEL}, 1 GiB each way)")
     if rows:
-        print("| kernel | calls | avg us | alg MB/launch | HBM read MB | HBM write MB | traffic/alg | alg GB/s | HBM GB/s |")
-        print("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
+        print("| kernel | calls | avg us | alg MB/launch | L2-fabric read MB | L2-fabric write MB | read/alg | "
+              "write/alg | alg GB/s | L2-fabric GB/s |")
+        print("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
         print("\n".join(rows))
         print()
     # every kernel of the run: counter traffic per launch and the rate it implies
-    print("| kernel (all launches of the run) | calls | avg us | HBM read MB | HBM write MB | HBM GB/s |")
+    print("| kernel (all launches of the run) | calls | avg us | L2-fabric read MB | L2-fabric write MB | "
+          "L2-fabric GB/s |")
     print("|---|---:|---:|---:|---:|---:|")
     allk = []
     for kname in set(fetch) & set(write):
-        if "copyBuffer" in kname or "fillBuffer" in kname or kname not in stats:
+        if kname == CAL_KERNEL or "fillBuffer" in kname or kname not in stats:
             continue
         rd, wr = fetch[kname] * f_fac * 1024, write[kname] * w_fac * 1024
         calls, us = stats[kname]
@@ -338,12 +347,22 @@ def resim_pipeline(N=512, sub=(101, 201, 151, 128, 128, 128), dt0=5e-4, every=4,
 
 
 def copy_calibration():
+    """three launches of the library's streaming copy (k_stream_copy) of 1 GiB:
+    a hand-written kernel whose traffic is known, so the FETCH_SIZE and
+    WRITE_SIZE counters it reads give their scale"""
+    import ctypes
+
     import torch
+
+    from ns_3d_staggered_dp_amd import NSConfig, _abi
+    from ns_3d_staggered_dp_amd.solver import NSSolver
 
     a = torch.ones(COPY_CAL_BYTES // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
+    torch.cuda.synchronize()
+    with NSSolver(NSConfig.cube(8)) as s:
+        for _ in range(3):
+            _abi.call("ns_stream_copy", s.h, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), a.numel())
     torch.cuda.synchronize()
     print("copy calibration done", float(b[0]), flush=True)
     return 0

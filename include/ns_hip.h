@@ -250,6 +250,12 @@ int ns_error_metrics(ns_ctx *ctx, const double *u_sub, const double *v_sub, cons
 int ns_error_metrics_noalloc(ns_ctx *ctx, const double *u_sub, const double *v_sub, const double *w_sub,
                              const double *p_sub, double err[8], double rms[4]);
 
+/* ---- measurement support (no counterpart in the reference) ----
+ * dst[i] = src[i] for i < n (device arrays), one launch on the context's
+ * stream: a hand-written streaming kernel whose traffic is known (8n bytes
+ * read, 8n written), used to scale the hardware traffic counters. */
+int ns_stream_copy(ns_ctx *ctx, const double *src, double *dst, long n);
+
 /* ---- per-subroutine mirrors with the reference's arguments ---- */
 /* cal_conv NS_functions.f90:53-59; conv_* are (nx or nx-1,ny,nz) etc. */
 int ns_cal_conv(ns_ctx *ctx, const double *u, const double *v, const double *w, int bc_x, int bc_y, int bc_z,

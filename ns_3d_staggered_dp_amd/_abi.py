@@ -66,6 +66,7 @@ SIGNATURES = {
     "ns_capture_end": [P, ctypes.POINTER(I)],
     "ns_max_div": [P, ctypes.POINTER(D)],
     "ns_check_finite": [P],
+    "ns_stream_copy": [P, P, P, ctypes.c_long],
     "ns_error_metrics": [P, P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)],
     "ns_error_metrics_noalloc": [P, P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)],
     "ns_resim_advance_spline": [P, P, I, D, D, I],

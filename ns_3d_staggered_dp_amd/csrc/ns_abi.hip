@@ -1450,6 +1450,14 @@ int ns_error_metrics_noalloc(ns_ctx *c, const double *u_sub, const double *v_sub
     return NS_OK;
 }
 
+int ns_stream_copy(ns_ctx *c, const double *src, double *dst, long n) {
+    if (!c || !src || !dst || n < 0) return NS_ERR_INVALID;
+    if (!is_dev_ptr(src) || !is_dev_ptr(dst)) return NS_ERR_INVALID;
+    launch_stream_copy(src, dst, n, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
+    return NS_OK;
+}
+
 int ns_check_finite(ns_ctx *c) {
     if (!c) return NS_ERR_INVALID;
     materialize(c);

@@ -268,6 +268,8 @@ int launch_err_partials(const Grid &g, double dx, double dy, const double *const
 void launch_err_combine(const double *partial, int nb, double *out, hipStream_t st);
 // device address of that scratch on the current device (looked up once per device)
 int metric_scratch(double **out);
+// dst[i] = src[i], i < n (the traffic-counter calibration kernel)
+void launch_stream_copy(const double *src, double *dst, long n, hipStream_t st);
 
 // ---- discrete adjoint of a step (ns_adjoint.hip) ----
 void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag, const double *const ub[3],

@@ -134,6 +134,20 @@ void launch_err_combine(const double *partial, int nb, double *out, hipStream_t 
     hipLaunchKernelGGL(k_err_combine, dim3(1), dim3(MT), 0, st, partial, nb, out);
 }
 
+namespace {
+// dst[i] = src[i]: a kernel of known traffic (reads and writes 8 B per element)
+__global__ void __launch_bounds__(256) k_stream_copy(const double *__restrict__ src, double *__restrict__ dst, long n) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = src[i];
+}
+}  // namespace
+
+void launch_stream_copy(const double *src, double *dst, long n, hipStream_t st) {
+    long nb = (n + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)nb), dim3(256), 0, st, src, dst, n);
+}
+
 int metric_scratch(double **out) {
     // device address of `scratch` on the current device, looked up once per device
     constexpr int MAXDEV = 64;
