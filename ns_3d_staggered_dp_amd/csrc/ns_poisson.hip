@@ -338,11 +338,20 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
     if (other >= ((AXIS == 1) ? g.nz : g.ny)) return;  // padding of the grid (uniform per block)
     const int kx0 = (blk - other * ngrp) << lsh;
     const int cnt = L * N;
-#pragma unroll 4
-    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
-        const int l = f & (L - 1), pos = f >> lsh;
-        const int kx = kx0 + l;
-        A[f] = (kx < nxh) ? P.spec[spec_index<AXIS>(g, nxhp, kx, pos, other)] : make_double2(0.0, 0.0);
+    {
+        // every load of the thread issued before any is used; indices past the
+        // block's lines are clamped to valid addresses (their values are not used)
+        double2 v[EL / FFT_NT];
+#pragma unroll
+        for (int u = 0; u < EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            v[u] = P.spec[spec_index<AXIS>(g, nxhp, min(kx0 + (f & (L - 1)), nxh - 1), min(f >> lsh, N - 1), other)];
+        }
+#pragma unroll
+        for (int u = 0; u < EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            if (f < cnt) A[f] = (kx0 + (f & (L - 1)) < nxh) ? v[u] : make_double2(0.0, 0.0);
+        }
     }
     __syncthreads();
     const Lay ly{1, L};
@@ -571,6 +580,22 @@ __device__ __forceinline__ YZGroup yz_group(const PassParams &P, int blk) {
 template <int AXIS>
 __device__ __forceinline__ int dct_n(const Grid &g) { return AXIS == 0 ? g.nx : AXIS == 1 ? g.ny : g.nz; }
 
+// a y/z block's 2L real lines of work (element f = threadIdx.x + u*FFT_NT of the
+// block: line f & (2L-1), position f >> (lsh+1)), every load of the thread
+// issued before any is used; indices past the lines are clamped to valid
+// addresses (their values are not used)
+template <int EL>
+__device__ __forceinline__ void yz_load(const PassParams &P, const YZGroup &G, int lsh, int N,
+                                        double (&v)[2 * EL / FFT_NT]) {
+    const int rm = (2 << lsh) - 1, xl = P.g.nx - 1;
+#pragma unroll
+    for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+        const int f = threadIdx.x + u * FFT_NT;
+        const int pos = min(f >> (lsh + 1), N - 1), x = min(G.x0 + (f & rm), xl);
+        v[u] = P.work[G.base + G.stride * pos + x];
+    }
+}
+
 // forward DCT-II / DST-II along x (fused with the RHS) or y, coefficients to work
 template <int AXIS, bool ODD, int EL>
 __device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk) {
@@ -599,14 +624,17 @@ __device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk)
         }
     } else {
         const YZGroup G = yz_group<AXIS>(P, blk);
-        const int rm = 2 * L - 1;
-#pragma unroll 4
-        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
-            const int rline = f & rm, pos = f >> (lsh + 1);
-            const int x = G.x0 + rline;
-            double val = (x < g.nx) ? P.work[G.base + G.stride * pos + x] : 0.0;
-            if (dst && (pos & 1) == 0) val = -val;
-            Ad[2 * L * mk_pos(pos, N) + rline] = val;
+        double v[2 * EL / FFT_NT];
+        yz_load<EL>(P, G, lsh, N, v);
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            if (f < cnt) {
+                const int rline = f & (2 * L - 1), pos = f >> (lsh + 1);
+                double val = (G.x0 + rline < g.nx) ? v[u] : 0.0;
+                if (dst && (pos & 1) == 0) val = -val;
+                Ad[2 * L * mk_pos(pos, N) + rline] = val;
+            }
         }
     }
     __syncthreads();
@@ -656,13 +684,19 @@ __device__ __forceinline__ void dct_solve_body(const PassParams &P, const int bl
     const YZGroup G = yz_group<2>(P, blk);
     const int yj = idiv(blk, P.dGrp);
     const int cnt = 2 * L * N, rm = 2 * L - 1;
-#pragma unroll 4
-    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
-        const int rline = f & rm, pos = f >> (lsh + 1);
-        const int x = G.x0 + rline;
-        double val = (x < g.nx) ? P.work[G.base + G.stride * pos + x] : 0.0;
-        if (dst && (pos & 1) == 0) val = -val;
-        Ad[2 * L * mk_pos(pos, N) + rline] = val;
+    {
+        double v[2 * EL / FFT_NT];
+        yz_load<EL>(P, G, lsh, N, v);
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            if (f < cnt) {
+                const int rline = f & rm, pos = f >> (lsh + 1);
+                double val = (G.x0 + rline < g.nx) ? v[u] : 0.0;
+                if (dst && (pos & 1) == 0) val = -val;
+                Ad[2 * L * mk_pos(pos, N) + rline] = val;
+            }
+        }
     }
     __syncthreads();
     fft_ip<ODD, EL>(A, P.pl, ly, lsh, tws, -1);
@@ -729,23 +763,36 @@ __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk)
     for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
     const int cnt = 2 * L * N, rm = 2 * L - 1;
     // coefficients in DCT order: a DST-II line is stored reversed
+    double v[2 * EL / FFT_NT];
     if (AXIS == 0) {
         const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
-#pragma unroll 4
-        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
-            const int rline = idiv(f, P.dN), pos = f - rline * N;
-            const int rl = rl0 + rline;
-            const double val = (rl < RL) ? P.work[(size_t)rl * N + pos] : 0.0;
-            Ad[2 * ((rline >> 1) * P.LS + (dst ? N - 1 - pos : pos)) + (rline & 1)] = val;
+        // the block's lines are one contiguous span of work
+        const size_t s0 = (size_t)rl0 * N, last = (size_t)RL * N - 1;
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const size_t q = s0 + threadIdx.x + u * FFT_NT;
+            v[u] = P.work[q < last ? q : last];
+        }
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            if (f < cnt) {
+                const int rline = idiv(f, P.dN), pos = f - rline * N;
+                const double val = (rl0 + rline < RL) ? v[u] : 0.0;
+                Ad[2 * ((rline >> 1) * P.LS + (dst ? N - 1 - pos : pos)) + (rline & 1)] = val;
+            }
         }
     } else {
         const YZGroup G = yz_group<AXIS>(P, blk);
-#pragma unroll 4
-        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
-            const int rline = f & rm, pos = f >> (lsh + 1);
-            const int x = G.x0 + rline;
-            const double val = (x < g.nx) ? P.work[G.base + G.stride * pos + x] : 0.0;
-            Ad[2 * L * (dst ? N - 1 - pos : pos) + rline] = val;
+        yz_load<EL>(P, G, lsh, N, v);
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            if (f < cnt) {
+                const int rline = f & rm, pos = f >> (lsh + 1);
+                const double val = (G.x0 + rline < g.nx) ? v[u] : 0.0;
+                Ad[2 * L * (dst ? N - 1 - pos : pos) + rline] = val;
+            }
         }
     }
     __syncthreads();
