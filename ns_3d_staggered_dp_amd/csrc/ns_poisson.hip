@@ -278,16 +278,25 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd(FFTPass P) {
     const int RL = g.ny * g.nz;
     const int rl0 = blockIdx.x * 2 * L;
     const int cnt = 2 * L * N;
-#pragma unroll 4
-    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
-        const int rline = idiv(f, P.dN), x = f - rline * N;
-        const int rl = rl0 + rline;
-        double val = 0.0;
-        if (rl < RL) {
-            const int k = idiv(rl, P.dNy);
-            val = div_rhs(P, x, rl - k * g.ny, k);
+    {
+        // every load of the thread issued before any is used; rows past the
+        // grid are clamped to its last row (their values are not used)
+        double v[2 * EL / FFT_NT];
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            const int rline = idiv(f, P.dN), x = f - rline * N;
+            const int rl = min(rl0 + rline, RL - 1), k = idiv(rl, P.dNy);
+            v[u] = div_rhs(P, x, rl - k * g.ny, k);
         }
-        ((double *)&A[(rline >> 1) * LS + x])[rline & 1] = val;
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            if (f < cnt) {
+                const int rline = idiv(f, P.dN), x = f - rline * N;
+                ((double *)&A[(rline >> 1) * LS + x])[rline & 1] = (rl0 + rline < RL) ? v[u] : 0.0;
+            }
+        }
     }
     __syncthreads();
     fft_ip<ODD, EL>(A, P.pl, Lay{LS, 1}, P.lsh, tws, -1);
@@ -392,14 +401,27 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
     // Z[m] = Xa[m] + i Xb[m] of the paired lines, Hermitian extension for m >= nxh;
     // c2r ignores the imaginary parts of DC and Nyquist
     const int cnt = L * N;
-#pragma unroll 4
-    for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+    // every load of the thread issued before any is used; rows past the grid
+    // are clamped to its last row (their values are not used)
+    double2 va[EL / FFT_NT], vb[EL / FFT_NT];
+#pragma unroll
+    for (int u = 0; u < EL / FFT_NT; u++) {
+        const int f = threadIdx.x + u * FFT_NT;
+        const int l = idiv(f, P.dN), m = f - l * N;
+        const int ra = rl0 + 2 * l, km = (m >= nxh) ? N - m : m;
+        va[u] = P.spec[(size_t)min(ra, RL - 1) * P.nxhp + km];
+        vb[u] = P.spec[(size_t)min(ra + 1, RL - 1) * P.nxhp + km];
+    }
+#pragma unroll
+    for (int u = 0; u < EL / FFT_NT; u++) {
+        const int f = threadIdx.x + u * FFT_NT;
+        if (f >= cnt) break;
         const int l = idiv(f, P.dN), m = f - l * N;
         const int ra = rl0 + 2 * l;
         const bool cj = m >= nxh;
         const int km = cj ? N - m : m;
-        double2 a = (ra < RL) ? P.spec[(size_t)ra * P.nxhp + km] : make_double2(0.0, 0.0);
-        double2 b = (ra + 1 < RL) ? P.spec[(size_t)(ra + 1) * P.nxhp + km] : make_double2(0.0, 0.0);
+        double2 a = (ra < RL) ? va[u] : make_double2(0.0, 0.0);
+        double2 b = (ra + 1 < RL) ? vb[u] : make_double2(0.0, 0.0);
         if (km == 0 || 2 * km == N) {
             a.y = 0.0;
             b.y = 0.0;
@@ -610,17 +632,25 @@ __device__ __forceinline__ void dct_fwd_body(const PassParams &P, const int blk)
     const int cnt = 2 * L * N;
     if (AXIS == 0) {
         const int RL = g.ny * g.nz, rl0 = blk * 2 * L;
-#pragma unroll 2
-        for (int f = threadIdx.x; f < cnt; f += FFT_NT) {
+        // every load of the thread issued before any is used; rows past the
+        // grid are clamped to its last row (their values are not used)
+        double v[2 * EL / FFT_NT];
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
             const int rline = idiv(f, P.dN), x = f - rline * N;
-            const int rl = rl0 + rline;
-            double val = 0.0;
-            if (rl < RL) {
-                const int k = idiv(rl, P.dNy);
-                val = wall_rhs(P, x, rl - k * g.ny, k);
+            const int rl = min(rl0 + rline, RL - 1), k = idiv(rl, P.dNy);
+            v[u] = wall_rhs(P, x, rl - k * g.ny, k);
+        }
+#pragma unroll
+        for (int u = 0; u < 2 * EL / FFT_NT; u++) {
+            const int f = threadIdx.x + u * FFT_NT;
+            if (f < cnt) {
+                const int rline = idiv(f, P.dN), x = f - rline * N;
+                double val = (rl0 + rline < RL) ? v[u] : 0.0;
                 if (dst && (x & 1) == 0) val = -val;
+                Ad[2 * ((rline >> 1) * P.LS + mk_pos(x, N)) + (rline & 1)] = val;
             }
-            Ad[2 * ((rline >> 1) * P.LS + mk_pos(x, N)) + (rline & 1)] = val;
         }
     } else {
         const YZGroup G = yz_group<AXIS>(P, blk);
