@@ -87,7 +87,10 @@ long ns_plane_set_length(int nx, int ny, int nz);
 int ns_upload_state(ns_ctx *ctx, const double *u, const double *v, const double *w, const double *p,
                     const double *hx, const double *hy, const double *hz, int first);
 int ns_download_state(ns_ctx *ctx, double *u, double *v, double *w, double *p, double *hx, double *hy, double *hz);
-/* u*, v*, w* and dp of the last step (dp gauged like the driver's). Any may be NULL. */
+/* u*, v*, w* and dp of the last step (dp gauged like the driver's). Any may be NULL.
+ * All-periodic lazy AB2/Euler steps leave the ghost entries of u*, v*, w*
+ * unfilled (their kernels read the periodic images instead); this call, like
+ * ns_state_pointers and the downloads, fills them first. */
 int ns_download_work(ns_ctx *ctx, double *u_star, double *v_star, double *w_star, double *dp);
 int ns_synchronize(ns_ctx *ctx);
 /* device pointers of the resident state, in the order u,v,w,p,hx,hy,hz,u*,v*,w*,dp; valid

@@ -27,6 +27,11 @@ struct ns_ctx {
     // lazy projection (dp_flag == 0): the state is (u*, dp) and u = u* - dt*grad(dp)
     // is formed inside the next predictor; u, v, w, p are materialised on demand
     int lazy = 0, stale = 0;
+    // all-periodic lazy AB2/Euler steps leave the ghost entries of u*, v*, w*
+    // unfilled (star_fill = 1) and read them through their periodic images;
+    // ghost_user: the velocity's ghosts are the caller's (just uploaded), so
+    // the next predictor reads them as they are
+    int star_fill = 0, ghost_user = 0;
     double *planes = nullptr;   // current plane set
     double *snaps = nullptr;    // snapshot history for ns_resim_advance
     size_t snaps_len = 0;
@@ -545,12 +550,12 @@ inline size_t comp_size(const Grid &g, int c) { return c == 0 ? g.nu() : c == 1 
 
 int run_poisson(ns_ctx *c, const double *us, const double *vs, const double *ws, const double *rhs_in,
                 const double *planes, const double *mean_dev, double mean_host, double *dp_out, double *sol_out,
-                hipEvent_t *ev = nullptr, bool dp_ghost_x = false) {
+                hipEvent_t *ev = nullptr, bool dp_ghost_x = false, bool wrap_vel = false) {
     const ns_config &f = c->cfg;
     if (!c->poisson_ok) return NS_ERR_UNSUPPORTED;
     if (!c->pp.dct)
         poisson_fft_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, c->spec, dp_out, sol_out, c->st, ev,
-                          dp_ghost_x);
+                          dp_ghost_x, wrap_vel);
     else
         poisson_dct_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, planes, mean_dev, mean_host,
                           c->work, dp_out, sol_out, c->st, ev);
@@ -574,9 +579,25 @@ void cap_pressure(ns_ctx *c, const double *gauge) {
     launch_subbox_mean(c->g, c->cap_g, c->cap_o, gauge, c->dp, c->cap_partial, dst + Ls.off[24], c->st);
 }
 
+// the steps of this context skip the velocity ghost fill: lazy, every
+// velocity axis periodic, and no ADI sweeps (AB2-CN's periodic sweeps write
+// the ghost rows themselves)
+bool skips_star_fill(const ns_ctx *c) {
+    const ns_config &f = c->cfg;
+    return c->lazy && f.bc[0] == 1 && f.bc[1] == 1 && f.bc[2] == 1 && f.scheme != NS_AB2_CN;
+}
+
+// fill the ghost entries of u*, v*, w* where the last steps left them
+void fill_star(ns_ctx *c) {
+    if (!c->star_fill) return;
+    launch_vel_bc(c->g, c->cfg.bc, c->planes, c->us, c->vs, c->ws, c->st);
+    c->star_fill = 0;
+}
+
 // lazy mode: form u, v, w, p from the current (u*, dp)
 void materialize(ns_ctx *c) {
     if (!c->lazy || !c->stale) return;
+    fill_star(c);
     const ns_config &f = c->cfg;
     const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
     launch_project(c->g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
@@ -593,10 +614,11 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     if (ev) (void)hipEventRecord(ev[0], c->st);
     // the periodic x ghosts of dp are written by the x inverse transform
     const bool pghost = !c->pp.dct && f.pbc[0] == 1;
+    const bool skip = skips_star_fill(c);
     if (c->lazy) {
         const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
         launch_predict_tiled(f, c->first, true, c->us, c->vs, c->ws, c->dp, gauge, c->p, c->hx, c->hy, c->hz, c->us2,
-                             c->vs2, c->ws2, c->st);
+                             c->vs2, c->ws2, c->st, skip && !c->ghost_user);
         std::swap(c->us, c->us2);
         std::swap(c->vs, c->vs2);
         std::swap(c->ws, c->ws2);
@@ -606,7 +628,12 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     }
     if (ev) (void)hipEventRecord(ev[1], c->st);
     if (f.scheme != NS_AB2_CN) {
-        launch_vel_bc(g, f.bc, c->planes, c->us, c->vs, c->ws, c->st);
+        if (!skip || cap) {
+            launch_vel_bc(g, f.bc, c->planes, c->us, c->vs, c->ws, c->st);
+            c->star_fill = 0;
+        } else {
+            c->star_fill = 1;
+        }
         if (cap)
             for (int ax = 0; ax < 3; ax++) cap_vel(c, ax);
     } else {
@@ -624,7 +651,8 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     }
     if (ev) (void)hipEventRecord(ev[2], c->st);
     int rc = run_poisson(c, c->us, c->vs, c->ws, nullptr, c->planes, c->planes + c->L.off[24], 0.0, c->dp, nullptr,
-                         ev ? ev + 3 : nullptr, pghost);
+                         ev ? ev + 3 : nullptr, pghost, skip);
+    c->ghost_user = 0;
     if (rc) return rc;
     launch_pr_bc(g, f.pbc, c->planes, f.dx, f.dy, f.dz, c->dp, c->st, pghost ? 1 : 0);
     if (ev) (void)hipEventRecord(ev[8], c->st);
@@ -778,6 +806,8 @@ int ns_upload_state(ns_ctx *c, const double *u, const double *v, const double *w
         HIPCHK(hipMemcpyAsync(c->ws, c->w, g.nw() * 8, hipMemcpyDeviceToDevice, c->st));
         HIPCHK(hipMemsetAsync(c->dp, 0, g.np() * 8, c->st));
         c->stale = 0;
+        c->star_fill = 0;
+        c->ghost_user = 1;
     }
     HIPCHK(hipStreamSynchronize(c->st));
     c->first = first ? 1 : 0;
@@ -802,6 +832,7 @@ int ns_download_state(ns_ctx *c, double *u, double *v, double *w, double *p, dou
 int ns_download_work(ns_ctx *c, double *u_star, double *v_star, double *w_star, double *dp) {
     if (!c) return NS_ERR_INVALID;
     const Grid &g = c->g;
+    fill_star(c);
     if (u_star) HIPCHK(hipMemcpyAsync(u_star, c->us, g.nu() * 8, hipMemcpyDefault, c->st));
     if (v_star) HIPCHK(hipMemcpyAsync(v_star, c->vs, g.nv() * 8, hipMemcpyDefault, c->st));
     if (w_star) HIPCHK(hipMemcpyAsync(w_star, c->ws, g.nw() * 8, hipMemcpyDefault, c->st));
@@ -824,6 +855,7 @@ int ns_download_work(ns_ctx *c, double *u_star, double *v_star, double *w_star, 
 
 int ns_state_pointers(ns_ctx *c, double **p) {
     if (!c || !p) return NS_ERR_INVALID;
+    fill_star(c);
     double *q[11] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz, c->us, c->vs, c->ws, c->dp};
     for (int i = 0; i < 11; i++) p[i] = q[i];
     return NS_OK;
@@ -881,7 +913,8 @@ int ns_step(ns_ctx *c, int nsteps) {
     if (!c || nsteps < 0) return NS_ERR_INVALID;
     int s = 0;
     while (s < nsteps) {
-        const bool steady = !c->first && !c->cap_on && !c->graph_off && !(c->cfg.flags & NS_FLAG_NO_GRAPH);
+        const bool steady = !c->first && !c->cap_on && !c->graph_off && !(c->cfg.flags & NS_FLAG_NO_GRAPH) &&
+                            !c->ghost_user;
         if (steady && nsteps - s >= 2) {
             const int parity = (c->us == c->us_base) ? 0 : 1;
             // a failed launch of an eager step before this one is reported, not
@@ -895,6 +928,7 @@ int ns_step(ns_ctx *c, int nsteps) {
             // host-side effects of the two steps (one_step's, minus the buffer
             // swaps, which cancel): u, v, w, p must be re-formed on demand
             if (c->lazy) c->stale = 1;
+            if (skips_star_fill(c)) c->star_fill = 1;
             c->first = 0;
             s += 2;
             continue;
@@ -1104,6 +1138,7 @@ int ns_set_dt(ns_ctx *c, double dt) {
         HIPCHK(hipMemcpyAsync(c->vs, c->v, g.nv() * 8, hipMemcpyDeviceToDevice, c->st));
         HIPCHK(hipMemcpyAsync(c->ws, c->w, g.nw() * 8, hipMemcpyDeviceToDevice, c->st));
         HIPCHK(hipMemsetAsync(c->dp, 0, g.np() * 8, c->st));
+        c->star_fill = 0;
     }
     c->cfg.dt = dt;
     c->cnT_ok = 0;
@@ -1297,6 +1332,7 @@ int ns_extract_subbox(ns_ctx *c, const ns_subbox *sb, int field, double *dst) {
     const int comp_of[11] = {0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2};
     const int comp = comp_of[field];
     if (field <= NS_FIELD_P) materialize(c);
+    fill_star(c);
     double *src[11] = {c->u, c->v, c->w, c->p, c->us, c->vs, c->ws, c->dp, c->hx, c->hy, c->hz};
     int n[3];
     comp_dims(c->g, comp, n);

@@ -105,10 +105,12 @@ struct Ctx;
 
 // ---- stencil / BC / projection launchers (ns_stencil.hip) ----
 // fused predictor (ns_predict.hip); lazy: (u,v,w) are the previous u*,v*,w* and the
-// projection with dp (and the corner gauge, if non-null) is applied on load
+// projection with dp (and the corner gauge, if non-null) is applied on load;
+// wrap (all-periodic boxes): the velocity's ghost entries are read through
+// their periodic images, so they need not have been filled
 void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                           const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                          double *us, double *vs, double *ws, hipStream_t st);
+                          double *us, double *vs, double *ws, hipStream_t st, bool wrap = false);
 
 // ---- batched launches: many boxes, one launch per kernel ----
 // box of block blk: the last i with prefix[i] <= blk (prefix ascending, prefix[0] = 0)
@@ -139,7 +141,7 @@ __device__ __forceinline__ const T &batch_params(const T *params, const int *pre
 size_t predict_params_size();
 int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                        double *us, double *vs, double *ws, void *out, int zc_hint);
+                        double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap = false);
 // tile columns x z cells of a box (the predictor's serial work units) and resident predictor blocks
 long predict_tile_columns(const ns_config &c);
 int predict_resident_blocks(bool lazy);
@@ -248,10 +250,12 @@ struct PoissonPlans {
 // row stride (complex) of the periodic half spectrum [z][y][kx]
 int spectrum_row(int nx);
 // ev (optional): 5 events recorded after the five passes
+// wrap_vel: every velocity axis periodic, so the divergence reads the high
+// own-axis faces through their periodic copies (the ghosts need not be filled)
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
                        double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr,
-                       bool dp_ghost_x = false);
+                       bool dp_ghost_x = false, bool wrap_vel = false);
 void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in,
                        const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,

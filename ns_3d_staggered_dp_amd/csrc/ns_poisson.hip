@@ -246,6 +246,7 @@ struct FFTPass {
     double *dp_out;               // ghosted dp (interior written) or null
     int dp_ghost_x;               // also write dp's periodic x ghosts
     double *sol_out;              // interior (nx,ny,nz) or null
+    int wrap;                     // all-periodic velocity: read u(nx+1), v(ny+1), w(nz+1) as u(1), v(1), w(1)
 };
 
 // a/d with r = RN(1/d): one product and one FMA correction step (the
@@ -262,9 +263,14 @@ __device__ __forceinline__ double div_rhs(const FFTPass &P, int a, int b, int c)
     if (P.rhs) return P.rhs[(size_t)a + (size_t)g.nx * ((size_t)b + (size_t)g.ny * c)];
     const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
     const int A = a + 1, B = b + 1, C = c + 1;
-    double s = qdiv(P.us[FIX(A + 1, B + 1, C + 1, U1, U2)] - P.us[FIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
-    s = s + qdiv(P.vs[FIX(A + 1, B + 1, C + 1, V1, V2)] - P.vs[FIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
-    s = s + qdiv(P.ws[FIX(A + 1, B + 1, C + 1, W1, W2)] - P.ws[FIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
+    // the high face of each component's own axis: its periodic copy of face 1
+    // when every velocity axis is periodic (the only ghosts read here)
+    const int iu = (P.wrap && A == g.nx) ? 1 : A + 1;
+    const int jv = (P.wrap && B == g.ny) ? 1 : B + 1;
+    const int kw = (P.wrap && C == g.nz) ? 1 : C + 1;
+    double s = qdiv(P.us[FIX(iu, B + 1, C + 1, U1, U2)] - P.us[FIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
+    s = s + qdiv(P.vs[FIX(A + 1, jv, C + 1, V1, V2)] - P.vs[FIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
+    s = s + qdiv(P.ws[FIX(A + 1, B + 1, kw, W1, W2)] - P.ws[FIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
     return qdiv(s, P.dt, P.rdt);
 }
 
@@ -910,8 +916,10 @@ IDiv make_idiv(int d) { return IDiv{d, 1.0f / (float)d}; }
 
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in, double2 *spec,
-                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev, bool dp_ghost_x) {
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev, bool dp_ghost_x,
+                       bool wrap_vel) {
     FFTPass P{};
+    P.wrap = wrap_vel ? 1 : 0;
     P.g = g;
     P.nxh = g.nx / 2 + 1;
     P.nxhp = spectrum_row(g.nx);

@@ -59,6 +59,7 @@ struct TileParams {
     double rd[3], rd2[3];  // 1/dx, 1/dy, 1/dz and 1/dx^2, ... rounded on the host
     int scheme, first, dp_flag;
     int per[3];          // own-axis periodicity of u, v, w (bc == 1)
+    int wrap;            // every axis periodic: ghost reads take the interior image
     int zc;              // z cells per block
     int tiles_x, tiles_y, tiles_z, ntiles;
     const double *src[3];  // u, v, w (or u*, v*, w* when lazy)
@@ -85,6 +86,16 @@ __device__ __forceinline__ int gidx(int c, int ax, int a) { return a + ((ax == c
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// In an all-periodic box every ghost entry of u, v, w is a copy of the
+// interior value at its index wrapped along every axis (vel_bc_staggered with
+// bc 1, NS_functions.f90:213-310): along a component's own axis (n entries)
+// entry n is entry 1, along the others entry 1 is n-1 and entry n is 2.
+// Reading through this map makes the ghosts' contents irrelevant.
+__device__ __forceinline__ int pwrap(int i, int n, bool own) {
+    if (own) return (i == n) ? 1 : i;
+    return (i == 1) ? n - 1 : ((i == n) ? 2 : i);
+}
+
 // one velocity value with the lazy projection applied, read straight from
 // memory (used for the periodic z wrap of w, once per column)
 template <bool LAZY>
@@ -94,7 +105,13 @@ __device__ __forceinline__ double load_val(const TileParams &P, int c, int gi, i
     gi = clampi(gi, 1, n[0]);
     gj = clampi(gj, 1, n[1]);
     gk = clampi(gk, 1, n[2]);
-    double x = P.src[c][FIX(gi, gj, gk, n[0], n[1])];
+    int vi = gi, vj = gj, vk = gk;
+    if (P.wrap) {
+        vi = pwrap(vi, n[0], c == 0);
+        vj = pwrap(vj, n[1], c == 1);
+        vk = pwrap(vk, n[2], c == 2);
+    }
+    double x = P.src[c][FIX(vi, vj, vk, n[0], n[1])];
     if (LAZY) {
         const int P1 = P.nx + 2, P2 = P.ny + 2;
         int b[3] = {gi, gj, gk};
@@ -168,6 +185,10 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             const int la = e % UW, lb = e / UW;
             int gi = (la == HX) ? P.nx + 1 : ox[0] + la, gj = oy[0] + lb;
             if (gi == 0 && P.per[0]) gi = P.nx;
+            if (P.wrap) {
+                gi = pwrap(gi, nn[0][0], true);
+                gj = pwrap(gj, nn[0][1], false);
+            }
             gi = clampi(gi, 1, nn[0][0]);
             gj = clampi(gj, 1, nn[0][1]);
             off[0][s_] = (unsigned int)((gi - 1) + nn[0][0] * (gj - 1));
@@ -184,6 +205,10 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             const int la = e % HX, lb = e / HX;
             int gi = ox[1] + la, gj = (lb == HY) ? P.ny + 1 : oy[1] + lb;
             if (gj == 0 && P.per[1]) gj = P.ny;
+            if (P.wrap) {
+                gi = pwrap(gi, nn[1][0], false);
+                gj = pwrap(gj, nn[1][1], true);
+            }
             gi = clampi(gi, 1, nn[1][0]);
             gj = clampi(gj, 1, nn[1][1]);
             off[1][s_] = (unsigned int)((gi - 1) + nn[1][0] * (gj - 1));
@@ -197,7 +222,13 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             const bool ok = e0 < HY * HX;
             const int e = ok ? e0 : HY * HX - 1;
             const int la = e % HX, lb = e / HX;
-            const int gi = clampi(ox[2] + la, 1, nn[2][0]), gj = clampi(oy[2] + lb, 1, nn[2][1]);
+            int gi = ox[2] + la, gj = oy[2] + lb;
+            if (P.wrap) {
+                gi = pwrap(gi, nn[2][0], false);
+                gj = pwrap(gj, nn[2][1], false);
+            }
+            gi = clampi(gi, 1, nn[2][0]);
+            gj = clampi(gj, 1, nn[2][1]);
             off[2][s_] = (unsigned int)((gi - 1) + nn[2][0] * (gj - 1));
             lds[2][s_] = ok ? (short)(lb * HX + la) : (short)-1;
             plo[2][s_] = (short)((lb + 1) * DXT + la + 1);
@@ -222,6 +253,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
     // z index of a velocity slice (0 -> nz for periodic w) and of a dp slice
     auto kvel = [&](int c, int s) {
         if (c == 2 && s == 0 && P.per[2]) s = P.nz;
+        if (P.wrap) s = pwrap(s, nn[c][2], c == 2);
         return clampi(s, 1, nn[c][2]);
     };
     auto kdp = [&](int s) {
@@ -533,7 +565,7 @@ int predict_resident_blocks(bool lazy) { return resident_blocks(lazy); }
 
 int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                        double *us, double *vs, double *ws, void *out, int zc_hint) {
+                        double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap) {
     TileParams &P = *(TileParams *)out;
     P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
     P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
@@ -544,6 +576,7 @@ int make_predict_params(const ns_config &c, int first, bool lazy, const double *
     }
     P.scheme = c.scheme; P.first = first; P.dp_flag = c.dp_flag;
     for (int a = 0; a < 3; a++) P.per[a] = (c.bc[a] == 1);
+    P.wrap = (wrap && P.per[0] && P.per[1] && P.per[2]) ? 1 : 0;
     P.tiles_x = (c.nx + TX - 1) / TX;
     P.tiles_y = (c.ny + TY - 1) / TY;
     if (zc_hint > 0) {
@@ -574,9 +607,9 @@ void launch_predict_batch(bool lazy, const void *params, const int *prefix, int 
 
 void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                           const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                          double *us, double *vs, double *ws, hipStream_t st) {
+                          double *us, double *vs, double *ws, hipStream_t st, bool wrap) {
     TileParams P;
-    const int nblk = make_predict_params(c, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P, 0);
+    const int nblk = make_predict_params(c, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P, 0, wrap);
     if (lazy)
         hipLaunchKernelGGL(k_predict_tiled<true>, dim3(nblk), dim3(TX * TY), 0, st, P);
     else
