@@ -22,7 +22,8 @@ FLAG_GAUGE_CORNER = 1
 FLAG_CN_CYCLIC = 2
 # every step as individual launches (no HIP graph replay of steady steps)
 FLAG_NO_GRAPH = 4
-# project every step even with dp_flag = 0 (needed by the discrete adjoint)
+# project every step instead of folding the projection of u into the next
+# predictor (the fold: dp_flag = 0, or dp_flag = 1 with DCT codes and no gauge)
 FLAG_EAGER_PROJECTION = 8
 
 PI = 3.1415926535897932  # big_periodic_3D.f90:6

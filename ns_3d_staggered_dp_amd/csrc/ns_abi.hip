@@ -32,6 +32,10 @@ struct ns_ctx {
     // ghost_user: the velocity's ghosts are the caller's (just uploaded), so
     // the next predictor reads them as they are
     int star_fill = 0, ghost_user = 0;
+    // lazy with dp_flag = 1 (DCT solve, no gauge): p is kept current by the
+    // steps themselves, p += dp where dp is formed (x inverse pass, pressure
+    // fill), so only u, v, w wait for the projection
+    int pacc = 0;
     double *planes = nullptr;   // current plane set
     double *snaps = nullptr;    // snapshot history for ns_resim_advance
     size_t snaps_len = 0;
@@ -550,7 +554,7 @@ inline size_t comp_size(const Grid &g, int c) { return c == 0 ? g.nu() : c == 1 
 
 int run_poisson(ns_ctx *c, const double *us, const double *vs, const double *ws, const double *rhs_in,
                 const double *planes, const double *mean_dev, double mean_host, double *dp_out, double *sol_out,
-                hipEvent_t *ev = nullptr, bool dp_ghost_x = false, bool wrap_vel = false) {
+                hipEvent_t *ev = nullptr, bool dp_ghost_x = false, bool wrap_vel = false, double *p_acc = nullptr) {
     const ns_config &f = c->cfg;
     if (!c->poisson_ok) return NS_ERR_UNSUPPORTED;
     if (!c->pp.dct)
@@ -558,7 +562,7 @@ int run_poisson(ns_ctx *c, const double *us, const double *vs, const double *ws,
                           dp_ghost_x, wrap_vel);
     else
         poisson_dct_solve(c->g, c->pp, f.dx, f.dy, f.dz, f.dt, us, vs, ws, rhs_in, planes, mean_dev, mean_host,
-                          c->work, dp_out, sol_out, c->st, ev);
+                          c->work, dp_out, sol_out, c->st, ev, p_acc);
     return NS_OK;
 }
 
@@ -600,8 +604,8 @@ void materialize(ns_ctx *c) {
     fill_star(c);
     const ns_config &f = c->cfg;
     const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
-    launch_project(c->g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
-                   c->st);
+    launch_project(c->g, f.dx, f.dy, f.dz, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w,
+                   c->pacc ? nullptr : c->p, c->st);
     c->stale = 0;
 }
 
@@ -650,11 +654,12 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
         }
     }
     if (ev) (void)hipEventRecord(ev[2], c->st);
+    double *pacc = c->pacc ? c->p : nullptr;
     int rc = run_poisson(c, c->us, c->vs, c->ws, nullptr, c->planes, c->planes + c->L.off[24], 0.0, c->dp, nullptr,
-                         ev ? ev + 3 : nullptr, pghost, skip);
+                         ev ? ev + 3 : nullptr, pghost, skip, pacc);
     c->ghost_user = 0;
     if (rc) return rc;
-    launch_pr_bc(g, f.pbc, c->planes, f.dx, f.dy, f.dz, c->dp, c->st, pghost ? 1 : 0);
+    launch_pr_bc(g, f.pbc, c->planes, f.dx, f.dy, f.dz, c->dp, c->st, pghost ? 1 : 0, pacc);
     if (ev) (void)hipEventRecord(ev[8], c->st);
     const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
     if (cap) cap_pressure(c, gauge);
@@ -755,9 +760,15 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
     // lazy projection (dp folded into the next predictor) needs periodic pressure
     // on every periodic velocity axis: the predictor reads dp(n+1) as dp(1)
     c->lazy = (cfg->dp_flag == 0);
+    // dp_flag = 1: lazy where p can be formed with dp (the DCT solve's x
+    // inverse pass and the pressure fill write every entry of dp once), with
+    // no corner gauge (whose value is known only after the whole solve)
+    const bool nonper = cfg->pbc[0] != 1 && cfg->pbc[1] != 1 && cfg->pbc[2] != 1;
+    if (cfg->dp_flag == 1 && nonper && !(cfg->flags & NS_FLAG_GAUGE_CORNER)) c->lazy = c->pacc = 1;
     for (int a = 0; a < 3; a++)
         if (cfg->bc[a] == 1 && cfg->pbc[a] != 1) c->lazy = 0;
     if (cfg->flags & NS_FLAG_EAGER_PROJECTION) c->lazy = 0;
+    if (!c->lazy || !c->poisson_ok) c->lazy = c->pacc = 0;
     HIPCHK(hipMemsetAsync(c->planes, 0, (size_t)c->L.total * 8, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     *out = c;
@@ -2170,13 +2181,27 @@ int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
         const ns_ctx *c = ctxs[i];
         if (!c) return NS_ERR_INVALID;
         const ns_config &f = c->cfg;
-        // one kernel family per launch: same scheme and codes, projected every step, DCT Poisson
-        if (c->lazy || !c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
+        // one kernel family per launch: same scheme and codes, DCT Poisson
+        if (!c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
         if (f.scheme != f0.scheme || f.dp_flag != f0.dp_flag || (f.flags & ~NS_FLAG_NO_GRAPH) !=
             (f0.flags & ~NS_FLAG_NO_GRAPH))
             return NS_ERR_UNSUPPORTED;
         for (int a = 0; a < 3; a++)
             if (f.bc[a] != f0.bc[a] || f.pbc[a] != f0.pbc[a]) return NS_ERR_UNSUPPORTED;
+    }
+    // the batch's launches project every step: a lazily projected context
+    // forms u, v, w, p now and steps eagerly from here on
+    for (int i = 0; i < n; i++) {
+        ns_ctx *c = ctxs[i];
+        if (!c->lazy) continue;
+        materialize(c);
+        c->lazy = c->pacc = c->star_fill = c->ghost_user = 0;
+        for (int k = 0; k < 2; k++)
+            if (c->graph[k]) {
+                (void)hipGraphExecDestroy(c->graph[k]);
+                c->graph[k] = nullptr;
+            }
+        HIPCHK(hipStreamSynchronize(c->st));
     }
     ns_batch *b = new ns_batch();
     b->ctx.assign(ctxs, ctxs + n);

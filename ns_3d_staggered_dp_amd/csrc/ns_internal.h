@@ -188,7 +188,8 @@ size_t pass_params_size();
 int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                          const double *us, const double *vs, const double *ws, const double *rhs_in,
                          const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
-                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds, int lsh_drop = 0);
+                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds, int lsh_drop = 0,
+                         double *p_acc = nullptr);
 // kernel variant of a box's pass (bit 1: odd radices, bit 0: lines over 1024);
 // a batch launch uses the bitwise OR over its boxes
 int dct_params_variant(const void *params);
@@ -202,8 +203,9 @@ void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double 
                    int first_axis = 0);
 void launch_vel_bc_cn2(const Grid &g, int bc, int dir, const double *planes, double *u, double *v, double *w,
                        hipStream_t st);
+// p_acc (non-periodic codes only): also p_acc += dp at every filled entry
 void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,
-                  hipStream_t st, int first_axis = 0);
+                  hipStream_t st, int first_axis = 0, double *p_acc = nullptr);
 void launch_modify_rhs(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz,
                        double *rhs, hipStream_t st);
 void launch_divergence(const Grid &g, double dx, double dy, double dz, double dt, const double *us, const double *vs,
@@ -259,7 +261,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
 void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in,
                        const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
-                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr);
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev = nullptr,
+                       double *p_acc = nullptr);
 
 // ---- re-simulation error metrics (ns_metrics.hip) ----
 // partials per block: for u, v, w, p: max|a-b|, sum|a-b|, sum b^2; then max|oz-oz_b|, sum oz_b^2

@@ -501,6 +501,7 @@ struct PassParams {
     double mean_dp;              // zero-mode source from the host
     double *dp_out;              // ghosted dp (interior written) or null
     double *sol_out;             // interior (nx,ny,nz) or null
+    double *p_acc;               // ghosted p: p += dp at the interior (lazy dp_flag = 1), or null
 };
 
 // divergence RHS at interior cell (a,b,c), 0-based (big_periodic_3D.f90:498-501),
@@ -845,7 +846,10 @@ __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk)
             double val = Ad[2 * ((rline >> 1) * P.LS + mk_pos(pos, N)) + (rline & 1)] * inv;
             if (dst && (pos & 1) == 0) val = -val;
             const int k = idiv(rl, P.dNy), j = rl - k * g.ny;
-            if (P.dp_out) P.dp_out[FIX(pos + 2, j + 2, k + 2, g.nx + 2, g.ny + 2)] = val;
+            const size_t q = FIX(pos + 2, j + 2, k + 2, g.nx + 2, g.ny + 2);
+            if (P.dp_out) P.dp_out[q] = val;
+            // p = 1*p + dp, the projection's pressure update, where dp is formed
+            if (P.p_acc) P.p_acc[q] = P.p_acc[q] + val;
             if (P.sol_out) P.sol_out[(size_t)rl * N + pos] = val;
         }
     } else {
@@ -1071,9 +1075,11 @@ void dct_launch_batch_v(int v, const PassParams *Ps, const int *prefix, int nbox
 int make_dct_pass_params(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                          const double *us, const double *vs, const double *ws, const double *rhs_in,
                          const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
-                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds, int lsh_drop) {
+                         double *dp_out, double *sol_out, int pass, void *out, size_t *lds, int lsh_drop,
+                         double *p_acc) {
     PassParams &P = *(PassParams *)out;
     P = PassParams{};
+    P.p_acc = (pass == 4) ? p_acc : nullptr;
     P.g = g;
     P.rhs = rhs_in;
     P.us = us; P.vs = vs; P.ws = ws;
@@ -1158,12 +1164,12 @@ void launch_dct_batch(int pass, int variant, const void *params, const int *pref
 void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
                        const double *us, const double *vs, const double *ws, const double *rhs_in,
                        const double *planes, const double *mean_dp_dev, double mean_dp_host, double *work,
-                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev) {
+                       double *dp_out, double *sol_out, hipStream_t st, hipEvent_t *ev, double *p_acc) {
     for (int pass = 0; pass < 5; pass++) {
         PassParams P;
         size_t lds = 0;
         const int nb = make_dct_pass_params(g, pl, dx, dy, dz, dt, us, vs, ws, rhs_in, planes, mean_dp_dev,
-                                            mean_dp_host, work, dp_out, sol_out, pass, &P, &lds);
+                                            mean_dp_host, work, dp_out, sol_out, pass, &P, &lds, 0, p_acc);
         launch_dct_pass(pass, &P, nb, lds, st);
         if (ev) (void)hipEventRecord(ev[pass], st);
     }

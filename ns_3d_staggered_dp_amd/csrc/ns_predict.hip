@@ -453,7 +453,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
                 }
                 const size_t q = q0[c] + (size_t)az * ps[c];
                 double dpd = 0.0;
-                if (!LAZY && P.dp_flag) {
+                if (P.dp_flag) {
                     int bq[3] = {gi, gj, gk};
                     bq[c]++;
                     const double dd = (c == 0) ? dx : (c == 1) ? dy : dz;

@@ -273,6 +273,10 @@ __device__ __forceinline__ void whole_fill_body(const BCParams &P, const size_t 
     else
         v = vy(i, j, k);
     A[FIX(i, j, k, n0, n1)] = v;
+    // pressure fill with an accumulation target (the lazily projected
+    // dp_flag = 1 step): p = 1*p + dp at the ghost entry, as the projection
+    // would form it
+    if (pres && P.a[1]) P.a[1][FIX(i, j, k, n0, n1)] = P.a[1][FIX(i, j, k, n0, n1)] + v;
 }
 
 __global__ void __launch_bounds__(256) k_whole_fill(BCParams P) {
@@ -838,7 +842,7 @@ void launch_cn2_batch(const void *params, const int *prefix, int nbox, int nblk,
 }
 
 void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double dx, double dy, double dz, double *dp,
-                  hipStream_t st, int first_axis) {
+                  hipStream_t st, int first_axis, double *p_acc) {
     if (pbc[0] == 1 && pbc[1] == 1 && pbc[2] == 1) {
         const int comp[3] = {3, 3, 3};
         double *const a[3] = {dp, nullptr, nullptr};
@@ -847,7 +851,7 @@ void launch_pr_bc(const Grid &g, const int pbc[3], const double *planes, double 
     }
     BCParams P;
     const double d[3] = {dx, dy, dz};
-    const int nb = make_fill_params(g, true, pbc, planes, d, dp, nullptr, nullptr, first_axis, &P);
+    const int nb = make_fill_params(g, true, pbc, planes, d, dp, p_acc, nullptr, first_axis, &P);
     if (nb) hipLaunchKernelGGL(k_whole_fill, dim3(nb, 1), dim3(256), 0, st, P);
 }
 
@@ -881,7 +885,8 @@ void launch_project(const Grid &g, double dx, double dy, double dz, double dt, i
     ProjParams P;
     const double d[3] = {dx, dy, dz};
     const int nb = make_project_params(g, d, dt, dp_flag, us, vs, ws, dp, gauge, u, v, w, p, &P);
-    hipLaunchKernelGGL(k_project, dim3(nb, 4), dim3(256), 0, st, P);
+    // p null: the velocity only (the pressure is already formed)
+    hipLaunchKernelGGL(k_project, dim3(nb, p ? 4 : 3), dim3(256), 0, st, P);
 }
 
 void launch_get_velpr_bc(const Grid &g, const int bc[3], const int pbc[3], double dx, double dy, double dz, double dt,

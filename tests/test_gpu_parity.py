@@ -409,6 +409,30 @@ def test_lazy_periodic_steps_skip_the_velocity_fill(oracle):
         assert periodic_ghosts_ok(wk[k], c), k
 
 
+@pytest.mark.parametrize("scheme", [AB2, AB2_CN])
+def test_walls_dpflag1_lazy_matches_eager_projection(oracle, scheme):
+    """wall boxes with dp_flag = 1 fold the projection of u into the next
+    predictor and form p where dp is formed (DCT x inverse pass, pressure
+    fill).  Against the oracle, and against the same steps projected every
+    step (FLAG_EAGER_PROJECTION): the lazy predictor's projection divides
+    with the corrected reciprocal, so the two agree to rounding, not bitwise"""
+    from ns_3d_staggered_dp_amd import FLAG_EAGER_PROJECTION
+
+    cfg = cfg_box(70, 20, 18, bc=(2, 4, 2), pbc=(3, 2, 3), scheme=scheme, dp_flag=1).replace(dt=1e-3)
+    u, v, w, p = (0.3 * x for x in rand_fields(cfg, 92))
+    planes = rand_planes(cfg, 93, 0.2)
+    out, st, div = run_both(oracle, cfg, u, v, w, p, 7, planes=planes)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+    with NSSolver(cfg.replace(flags=FLAG_EAGER_PROJECTION)) as s:
+        s.upload(u, v, w, p, first=True)
+        s.set_bc_planes(planes)
+        s.step(7)
+        eager = s.download()
+    for k in ("u", "v", "w", "p", "hx", "hy", "hz"):
+        assert rel(out[k], eager[k]) <= 1e-13, (k, rel(out[k], eager[k]))
+
+
 @pytest.mark.parametrize("case", ["periodic_ab2_lazy", "periodic_ab2cn", "walls_dct_dpflag"])
 def test_graph_replay_matches_eager(case):
     """ns_step replays steady steps from a HIP graph (two steps per replay, one
