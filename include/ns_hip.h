@@ -58,8 +58,7 @@ extern "C" {
  * projection of u into the next predictor.  Without it the projection is
  * folded with dp_flag = 0, and with dp_flag = 1 where the pressure codes are
  * all non-periodic (DCT solve) and NS_FLAG_GAUGE_CORNER is off; p is then
- * formed where dp is.  A context joined to a batch projects every step from
- * then on. */
+ * formed where dp is. */
 #define NS_FLAG_EAGER_PROJECTION 8
 
 /* Run configuration: the driver parameter block (big_periodic_3D.f90:9-21,
@@ -208,9 +207,9 @@ int ns_adjoint_step(ns_ctx *ctx, const double *u, const double *v, const double 
  * per-box block counts), steady steps replayed from one HIP graph ----
  * The contexts must share scheme (Euler, AB2 or AB2-CN), bc/pbc codes,
  * dp_flag and flags and use the DCT Poisson solver (walls); sizes may
- * differ.  A lazily projected context forms u, v, w, p when it joins and
- * projects every step from then on (the batch's launches have no lazy
- * form).  Each context keeps its own state and current plane set;
+ * differ.  Lazily projected boxes step lazily in the batch too; where the
+ * boxes differ in projection mode, the lazy ones form u, v, w, p when they
+ * join and project every step from then on.  Each context keeps its own state and current plane set;
  * ns_batch_step(b, n) leaves every context as ns_step(ctx, n) would.  A batch
  * holds its contexts' device pointers and time steps: destroy it before any
  * of them, and re-create it after ns_set_dt on any of them. */

@@ -1992,8 +1992,13 @@ struct ns_batch {
         int rows = 0;     // sweep: rows per lane of the partitioned kernel (the most over the boxes)
         int nbox = 0;     // boxes of this launch (a subset for sweeps grouped by blocks per line)
     };
-    std::vector<Phase> phases;  // one steady (non-first) step
-    hipGraphExec_t graph = nullptr;
+    // one steady (non-first) step per parity of the boxes' u* buffers: with
+    // the projection folded into the predictor (lazy) each step reads one
+    // buffer and writes the other, so the tables of consecutive steps alternate
+    std::vector<Phase> phases[2];
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int lazy = 0;
+    std::vector<double *> sig;  // each box's current u* when phases[0] was built
     // launch-shape choices per size class of the boxes (cells < 2^17, < 2^20,
     // larger): predictor z chunks sized for `rounds` rounds of resident
     // blocks, DCT lines per block reduced by 2^dct_drop, 2^adix_lsh lines per
@@ -2068,33 +2073,44 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
         const double d[3] = {f.dx, f.dy, f.dz};
         void *dst = host.data() + (size_t)i * psz;
         int nb = 0;
+        // lazy: the predictor reads u* and dp (the previous step's) and writes
+        // the other u* buffer, which the rest of the step works on; p is formed
+        // where dp is (DCT x inverse pass, pressure fill) when dp_flag = 1
+        double *const nus = b->lazy ? c->us2 : c->us, *const nvs = b->lazy ? c->vs2 : c->vs,
+                      *const nws = b->lazy ? c->ws2 : c->ws;
+        double *const pacc = (b->lazy && c->pacc) ? c->p : nullptr;
         if (kind == 0) {
-            const long z = cols / ((long)predict_resident_blocks(false) * b->knobs[cls].rounds);
+            const long z = cols / ((long)predict_resident_blocks(b->lazy != 0) * b->knobs[cls].rounds);
             const int zc_hint = (int)(z > 8 ? z : 8);
-            nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr, c->p,
-                                     c->hx, c->hy, c->hz, c->us, c->vs, c->ws, dst, zc_hint);
+            const double *gauge = (b->lazy && (f.flags & NS_FLAG_GAUGE_CORNER)) ? c->dp : nullptr;
+            if (b->lazy)
+                nb = make_predict_params(f, first_step ? c->first : 0, true, c->us, c->vs, c->ws, c->dp, gauge, c->p,
+                                         c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint);
+            else
+                nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr,
+                                         c->p, c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint);
         } else if (kind == 1) {
             const double d0[3] = {0.0, 0.0, 0.0};
-            nb = make_fill_params(g, false, f.bc, c->planes, d0, c->us, c->vs, c->ws, 0, dst);
+            nb = make_fill_params(g, false, f.bc, c->planes, d0, nus, nvs, nws, 0, dst);
         } else if (kind == 2) {
             size_t l = 0;
-            nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, c->us, c->vs, c->ws, nullptr, c->planes,
+            nb = make_dct_pass_params(g, c->pp, f.dx, f.dy, f.dz, f.dt, nus, nvs, nws, nullptr, c->planes,
                                       c->planes + c->L.off[24], 0.0, c->work, c->dp, nullptr, arg, dst, &l,
-                                      b->knobs[cls].dct_drop);
+                                      b->knobs[cls].dct_drop, pacc);
             lds = l > lds ? l : lds;
             variant |= dct_params_variant(dst);
         } else if (kind == 5) {
-            nb = make_cn2_params(g, f.bc[arg], arg, c->planes, c->us, c->vs, c->ws, dst);
+            nb = make_cn2_params(g, f.bc[arg], arg, c->planes, nus, nvs, nws, dst);
         } else if (kind == 6) {
             size_t l = 0;
-            nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], c->us, c->vs, c->ws, dst, &l,
+            nb = make_sweep_params(g, arg + 1, c->cn[arg][0], c->cn[arg][1], nus, nvs, nws, dst, &l,
                                    b->knobs[cls].adix_lsh, part, part ? c->planes : nullptr, &c->L);
             lds = l > lds ? l : lds;
             const int kw = sweep_params_waves(dst), rw = sweep_params_rows(dst);
             variant = kw > variant ? kw : variant;
             rows = rw > rows ? rw : rows;
         } else if (kind == 3) {
-            nb = make_fill_params(g, true, f.pbc, c->planes, d, c->dp, nullptr, nullptr, 0, dst);
+            nb = make_fill_params(g, true, f.pbc, c->planes, d, c->dp, pacc, nullptr, 0, dst);
         } else {
             const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
             nb = make_project_params(g, d, f.dt, f.dp_flag, c->us, c->vs, c->ws, c->dp, gauge, c->u, c->v, c->w, c->p,
@@ -2150,6 +2166,7 @@ int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out
     for (int pass = 0; pass < 5; pass++)
         if ((rc = add_phase(b, 2, pass, first_step, out))) return rc;
     if ((rc = add_phase(b, 3, 0, first_step, out))) return rc;  // the whole pressure fill
+    if (b->lazy) return NS_OK;                                     // projected by the next predictor
     return add_phase(b, 4, 0, first_step, out);
 }
 
@@ -2157,7 +2174,7 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
     for (const auto &ph : phases) {
         const int n = ph.nbox;
         switch (ph.kind) {
-            case 0: launch_predict_batch(false, ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 0: launch_predict_batch(b->lazy != 0, ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 1: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 3, b->st); break;
             case 2: launch_dct_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             case 3: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 1, b->st); break;
@@ -2167,6 +2184,60 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
                 break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
         }
+    }
+}
+
+// exchange every box's two u* buffers (host pointers; lazy steps alternate them)
+void swap_star(ns_batch *b) {
+    for (ns_ctx *c : b->ctx) {
+        std::swap(c->us, c->us2);
+        std::swap(c->vs, c->vs2);
+        std::swap(c->ws, c->ws2);
+    }
+}
+
+void drop_steady(ns_batch *b) {
+    for (int k = 0; k < 2; k++) {
+        free_phases(b->phases[k]);
+        if (b->graph[k]) {
+            (void)hipGraphExecDestroy(b->graph[k]);
+            b->graph[k] = nullptr;
+        }
+    }
+}
+
+// the steady-step tables for the boxes' current u* buffers (phases[0]) and,
+// lazy, for the exchanged ones (phases[1])
+int build_steady(ns_batch *b) {
+    drop_steady(b);
+    int rc = build_phases(b, false, b->phases[0]);
+    if (!rc && b->lazy) {
+        swap_star(b);
+        rc = build_phases(b, false, b->phases[1]);
+        swap_star(b);
+    }
+    b->sig.clear();
+    for (ns_ctx *c : b->ctx) b->sig.push_back(c->us);
+    return rc;
+}
+
+// which steady tables match the boxes' buffers now: 0, 1, or -1 (none: a box
+// was stepped on its own since they were built)
+int steady_parity(const ns_batch *b) {
+    bool p0 = true, p1 = b->lazy != 0;
+    for (size_t i = 0; i < b->ctx.size(); i++) {
+        p0 = p0 && b->ctx[i]->us == b->sig[i];
+        p1 = p1 && b->ctx[i]->us2 == b->sig[i];
+    }
+    return p0 ? 0 : (p1 ? 1 : -1);
+}
+
+// host-side effects of batch steps on each box
+void after_batch_steps(ns_batch *b) {
+    for (ns_ctx *c : b->ctx) {
+        c->first = 0;
+        c->ghost_user = 0;
+        if (b->lazy) c->stale = 1;
     }
 }
 
@@ -2189,9 +2260,12 @@ int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
         for (int a = 0; a < 3; a++)
             if (f.bc[a] != f0.bc[a] || f.pbc[a] != f0.pbc[a]) return NS_ERR_UNSUPPORTED;
     }
-    // the batch's launches project every step: a lazily projected context
-    // forms u, v, w, p now and steps eagerly from here on
-    for (int i = 0; i < n; i++) {
+    // one projection mode per batch: where the boxes differ (a box that left
+    // an earlier batch projects every step), the lazy ones form u, v, w, p and
+    // project every step from here on
+    bool all_lazy = true;
+    for (int i = 0; i < n; i++) all_lazy = all_lazy && ctxs[i]->lazy;
+    for (int i = 0; i < n && !all_lazy; i++) {
         ns_ctx *c = ctxs[i];
         if (!c->lazy) continue;
         materialize(c);
@@ -2205,6 +2279,7 @@ int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
     }
     ns_batch *b = new ns_batch();
     b->ctx.assign(ctxs, ctxs + n);
+    b->lazy = all_lazy ? 1 : 0;
     for (ns_ctx *c : b->ctx) {
         const size_t cells = c->g.ni();
         b->cls.push_back(cells < ((size_t)1 << 17) ? 0 : cells < ((size_t)1 << 20) ? 1 : 2);
@@ -2214,7 +2289,7 @@ int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
         delete b;
         return NS_ERR_HIP;
     }
-    int rc = build_phases(b, false, b->phases);
+    int rc = build_steady(b);
     if (rc) {
         ns_batch_destroy(b);
         return rc;
@@ -2226,8 +2301,7 @@ int ns_batch_create(ns_ctx *const *ctxs, int n, ns_batch **out) {
 int ns_batch_destroy(ns_batch *b) {
     if (!b) return NS_ERR_INVALID;
     if (b->st) (void)hipStreamSynchronize(b->st);
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    free_phases(b->phases);
+    drop_steady(b);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
     return NS_OK;
@@ -2249,25 +2323,39 @@ int ns_batch_step(ns_batch *b, int nsteps) {
         free_phases(ph);
         if (rc) return rc;
         HIPCHK(e);
-        for (ns_ctx *c : b->ctx) c->first = 0;
+        if (b->lazy) swap_star(b);
+        after_batch_steps(b);
         s = 1;
     }
+    if (s >= nsteps) return NS_OK;
+    int par = steady_parity(b);
+    if (par < 0) {
+        int rc = build_steady(b);
+        if (rc) return rc;
+        par = 0;
+    }
     const bool use_graph = !(b->ctx[0]->cfg.flags & NS_FLAG_NO_GRAPH);
-    if (use_graph && !b->graph && s < nsteps) {
-        hipGraph_t gr = nullptr;
-        if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            run_phases(b, b->phases);
-            if (hipStreamEndCapture(b->st, &gr) == hipSuccess && gr) {
-                if (hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0) != hipSuccess) b->graph = nullptr;
-            }
-            if (gr) (void)hipGraphDestroy(gr);
-        }
-        (void)hipGetLastError();
-    }
     for (; s < nsteps; s++) {
-        if (use_graph && b->graph) HIPCHK(hipGraphLaunch(b->graph, b->st));
-        else run_phases(b, b->phases);
+        if (use_graph && !b->graph[par]) {
+            hipGraph_t gr = nullptr;
+            if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                run_phases(b, b->phases[par]);
+                if (hipStreamEndCapture(b->st, &gr) == hipSuccess && gr) {
+                    if (hipGraphInstantiate(&b->graph[par], gr, nullptr, nullptr, 0) != hipSuccess)
+                        b->graph[par] = nullptr;
+                }
+                if (gr) (void)hipGraphDestroy(gr);
+            }
+            (void)hipGetLastError();
+        }
+        if (use_graph && b->graph[par]) HIPCHK(hipGraphLaunch(b->graph[par], b->st));
+        else run_phases(b, b->phases[par]);
+        if (b->lazy) {
+            swap_star(b);
+            par ^= 1;
+        }
     }
+    after_batch_steps(b);
     HIPCHK(hipStreamSynchronize(b->st));
     return hipGetLastError() == hipSuccess ? NS_OK : NS_ERR_HIP;
 }
@@ -2282,12 +2370,24 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
         for (double *q : saved) (void)hipFree(q);
         saved.clear();
     };
+    // (lazy boxes: u*, dp, p and the histories, with u* in the buffers it is in now)
     std::vector<int> first;
-    for (ns_ctx *c : b->ctx) {
+    auto state = [&](ns_ctx *c, double *a[8], size_t n[8]) {
         const Grid &g = c->g;
-        const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
-        double *src[7] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz};
-        for (int i = 0; i < 7; i++) {
+        const size_t m[8] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw(), g.np()};
+        double *q[8] = {b->lazy ? c->us : c->u, b->lazy ? c->vs : c->v, b->lazy ? c->ws : c->w, c->p,
+                        c->hx, c->hy, c->hz, b->lazy ? c->dp : nullptr};
+        for (int i = 0; i < 8; i++) {
+            a[i] = q[i];
+            n[i] = m[i];
+        }
+    };
+    for (ns_ctx *c : b->ctx) {
+        double *src[8];
+        size_t n[8];
+        state(c, src, n);
+        for (int i = 0; i < 8; i++) {
+            if (!src[i]) continue;
             double *q = nullptr;
             if (balloc(&q, n[i] * sizeof(double))) {
                 release();
@@ -2298,16 +2398,31 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
         }
         first.push_back(c->first);
     }
+    // parity of the u* buffers relative to the saved state
+    int flips = 0;
     auto restore = [&]() {
+        if (flips & 1) swap_star(b);
+        flips = 0;
         size_t k = 0;
         for (size_t j = 0; j < b->ctx.size(); j++) {
             ns_ctx *c = b->ctx[j];
-            const Grid &g = c->g;
-            const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
-            double *dst[7] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz};
-            for (int i = 0; i < 7; i++, k++)
+            double *dst[8];
+            size_t n[8];
+            state(c, dst, n);
+            for (int i = 0; i < 8; i++) {
+                if (!dst[i]) continue;
                 (void)hipMemcpyAsync(dst[i], saved[k], n[i] * sizeof(double), hipMemcpyDeviceToDevice, b->st);
+                k++;
+            }
             c->first = first[j];
+        }
+    };
+    // one steady step with the trial tables (lazy: those of the buffers' parity)
+    auto step_with = [&](std::vector<ns_batch::Phase> (&ph)[2]) {
+        run_phases(b, ph[flips & 1]);
+        if (b->lazy) {
+            swap_star(b);
+            flips++;
         }
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2318,13 +2433,18 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
     }
     // ms of trial_steps steady steps with the current knobs (best of two runs)
     auto trial = [&](float &ms) -> int {
-        std::vector<ns_batch::Phase> ph;
-        int rc = build_phases(b, false, ph);
+        std::vector<ns_batch::Phase> ph[2];
+        int rc = build_phases(b, false, ph[0]);
+        if (!rc && b->lazy) {
+            swap_star(b);
+            rc = build_phases(b, false, ph[1]);
+            swap_star(b);
+        }
         ms = 1e30f;
         for (int rep = 0; rep < 2 && !rc; rep++) {
-            run_phases(b, ph);  // warm-up of this shape
+            step_with(ph);  // warm-up of this shape
             (void)hipEventRecord(e0, b->st);
-            for (int s = 0; s < trial_steps; s++) run_phases(b, ph);
+            for (int s = 0; s < trial_steps; s++) step_with(ph);
             (void)hipEventRecord(e1, b->st);
             restore();
             if (hipEventSynchronize(e1) != hipSuccess) rc = NS_ERR_HIP;
@@ -2332,7 +2452,8 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
             if (!rc && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t < ms) ms = t;
         }
         (void)hipStreamSynchronize(b->st);
-        free_phases(ph);
+        free_phases(ph[0]);
+        free_phases(ph[1]);
         (void)hipGetLastError();
         return rc;
     };
@@ -2371,13 +2492,8 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
     (void)hipEventDestroy(e1);
     release();
     if (rc) return rc;
-    // the steady-step tables and graph with the chosen shapes
-    free_phases(b->phases);
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
-    if ((rc = build_phases(b, false, b->phases))) return rc;
+    // the steady-step tables with the chosen shapes (graphs captured on the next steps)
+    if ((rc = build_steady(b))) return rc;
     if (knobs_out)
         for (int k = 0; k < 3; k++) {
             knobs_out[3 * k] = b->knobs[k].rounds;
