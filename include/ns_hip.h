@@ -220,13 +220,14 @@ int ns_batch_step(ns_batch *b, int nsteps);
 /* Autotuner of the batch's launch shapes (no counterpart in the reference):
  * per size class of the boxes (cells < 2^17, < 2^20, larger) it picks the
  * predictor's z-chunk length (in rounds of resident blocks: 4, 8, 16, 32),
- * the DCT passes' lines per block (default, /2, /4) and, when the x sweep is
- * not partitioned (lines over 640 rows), its lines per block
+ * the DCT passes' lines per block (default, /2, /4), the predictor's tile
+ * (each box's shape with fewer idle lanes, 64 x 4, or 32 x 8) and, when the
+ * x sweep is not partitioned (lines over 640 rows), its lines per block
  * (64, 32, 16, 8), by coordinate descent over steady batch steps
  * timed on the device (trial_steps steps per trial, best of two); the boxes'
  * states are restored after every trial.  No choice changes any computed
- * value.  knobs_out (9 ints or null): rounds, DCT drop, x-sweep log2 lines,
- * per class. */
+ * value.  knobs_out (12 ints or null): rounds, DCT drop, x-sweep log2 lines,
+ * tile (0 per box, 64, 32), per class. */
 int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out);
 int ns_capture_end(ns_ctx *ctx, int *nsets_written);
 /* max |div u| over interior cells (big_periodic_3D.f90:576-579) */

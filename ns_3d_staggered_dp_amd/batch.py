@@ -43,10 +43,12 @@ class Batch:
     def tune(self, trial_steps: int = 3) -> dict:
         """Pick the launch shapes per size class by timing steady batch steps
         (ns_batch_tune); the boxes' states are unchanged.  Returns the choice
-        per class: {"small"|"medium"|"large": (rounds, dct_drop, x_sweep_lsh)}."""
-        k = (ctypes.c_int * 9)()
+        per class: {"small"|"medium"|"large": (rounds, dct_drop, x_sweep_lsh,
+        tile)}, tile 0 meaning each box's own shape (the one with fewer idle
+        lanes), else the predictor tile width (64 or 32)."""
+        k = (ctypes.c_int * 12)()
         _abi.call("ns_batch_tune", self.h, int(trial_steps), k)
-        return {name: (k[3 * i], k[3 * i + 1], k[3 * i + 2]) for i, name in enumerate(("small", "medium", "large"))}
+        return {name: tuple(k[4 * i + j] for j in range(4)) for i, name in enumerate(("small", "medium", "large"))}
 
     def close(self):
         if self.h:

@@ -2003,8 +2003,9 @@ struct ns_batch {
     // larger): predictor z chunks sized for `rounds` rounds of resident
     // blocks, DCT lines per block reduced by 2^dct_drop, 2^adix_lsh lines per
     // x-sweep block.  None of them changes any value computed.
+    // Predictor tile per class: 0 each box's shape with fewer idle lanes, or 64 / 32 wide.
     struct Knobs {
-        int rounds = 8, dct_drop = 0, adix_lsh = 6;
+        int rounds = 8, dct_drop = 0, adix_lsh = 6, tile = 0;
     };
     Knobs knobs[3];
     std::vector<int> cls;  // size class of each box
@@ -2061,7 +2062,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     // chunk re-reads 3 slices)
     long cols = 0;
     if (kind == 0)
-        for (int bi : *sel) cols += predict_tile_columns(b->ctx[bi]->cfg);
+        for (int bi : *sel) cols += predict_tile_columns(b->ctx[bi]->cfg, arg);
     // sweeps: one launch shape for all boxes, the partitioned solve only if every box has it
     const bool part = kind == 6 && batch_sweep_partitioned(b, arg);
     for (int i = 0; i < n; i++) {
@@ -2080,15 +2081,15 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
                       *const nws = b->lazy ? c->ws2 : c->ws;
         double *const pacc = (b->lazy && c->pacc) ? c->p : nullptr;
         if (kind == 0) {
-            const long z = cols / ((long)predict_resident_blocks(b->lazy != 0) * b->knobs[cls].rounds);
+            const long z = cols / ((long)predict_resident_blocks(b->lazy != 0, arg) * b->knobs[cls].rounds);
             const int zc_hint = (int)(z > 8 ? z : 8);
             const double *gauge = (b->lazy && (f.flags & NS_FLAG_GAUGE_CORNER)) ? c->dp : nullptr;
             if (b->lazy)
                 nb = make_predict_params(f, first_step ? c->first : 0, true, c->us, c->vs, c->ws, c->dp, gauge, c->p,
-                                         c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint);
+                                         c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint, false, arg);
             else
                 nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr,
-                                         c->p, c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint);
+                                         c->p, c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint, false, arg);
         } else if (kind == 1) {
             const double d0[3] = {0.0, 0.0, 0.0};
             nb = make_fill_params(g, false, f.bc, c->planes, d0, nus, nvs, nws, 0, dst);
@@ -2140,7 +2141,13 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
 
 int build_phases(ns_batch *b, bool first_step, std::vector<ns_batch::Phase> &out) {
     int rc;
-    if ((rc = add_phase(b, 0, 0, first_step, out))) return rc;
+    // one predictor launch per tile shape (arg: the tile width)
+    for (int tile : {64, 32}) {
+        std::vector<int> sel;
+        for (int i = 0; i < (int)b->ctx.size(); i++)
+            if (predict_shape(b->ctx[i]->cfg, b->knobs[b->cls[i]].tile) == tile) sel.push_back(i);
+        if (!sel.empty() && (rc = add_phase(b, 0, tile, first_step, out, &sel))) return rc;
+    }
     if (b->ctx[0]->cfg.scheme == NS_AB2_CN) {
         // per axis: boundary rows, then the sweep (x, y, z); a partitioned
         // sweep reads the boundary rows from the planes itself
@@ -2174,7 +2181,7 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
     for (const auto &ph : phases) {
         const int n = ph.nbox;
         switch (ph.kind) {
-            case 0: launch_predict_batch(b->lazy != 0, ph.params, ph.prefix, n, ph.nblk, b->st); break;
+            case 0: launch_predict_batch(b->lazy != 0, ph.arg, ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 1: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 3, b->st); break;
             case 2: launch_dct_batch(ph.arg, ph.variant, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st); break;
             case 3: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 1, b->st); break;
@@ -2463,16 +2470,20 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
     for (int k : b->cls) present[k] = true;
     // coordinate descent over (size class, knob); a change is kept when it
     // saves more than 1% of the step
-    const int cand_rounds[4] = {4, 8, 16, 32}, cand_drop[3] = {0, 1, 2}, cand_lsh[4] = {6, 5, 4, 3};
+    const int cand_rounds[4] = {4, 8, 16, 32}, cand_drop[3] = {0, 1, 2}, cand_lsh[4] = {6, 5, 4, 3},
+              cand_tile[3] = {0, 64, 32};
     // the x sweep's lines per block only shape the twisted / cyclic sweeps
     const bool x_twisted = b->ctx[0]->cfg.scheme == NS_AB2_CN && !batch_sweep_partitioned(b, 0);
     for (int k = 0; k < 3 && !rc; k++) {
         if (!present[k]) continue;
-        for (int knob = 0; knob < 3 && !rc; knob++) {
+        for (int knob = 0; knob < 4 && !rc; knob++) {
             if (knob == 2 && !x_twisted) continue;
-            int *val = knob == 0 ? &b->knobs[k].rounds : knob == 1 ? &b->knobs[k].dct_drop : &b->knobs[k].adix_lsh;
-            const int *cand = knob == 0 ? cand_rounds : knob == 1 ? cand_drop : cand_lsh;
-            const int nc = knob == 1 ? 3 : 4;
+            int *val = knob == 0   ? &b->knobs[k].rounds
+                       : knob == 1 ? &b->knobs[k].dct_drop
+                       : knob == 2 ? &b->knobs[k].adix_lsh
+                                   : &b->knobs[k].tile;
+            const int *cand = knob == 0 ? cand_rounds : knob == 1 ? cand_drop : knob == 2 ? cand_lsh : cand_tile;
+            const int nc = (knob == 1 || knob == 3) ? 3 : 4;
             const int keep = *val;
             int best_v = keep;
             for (int i = 0; i < nc && !rc; i++) {
@@ -2496,9 +2507,10 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
     if ((rc = build_steady(b))) return rc;
     if (knobs_out)
         for (int k = 0; k < 3; k++) {
-            knobs_out[3 * k] = b->knobs[k].rounds;
-            knobs_out[3 * k + 1] = b->knobs[k].dct_drop;
-            knobs_out[3 * k + 2] = b->knobs[k].adix_lsh;
+            knobs_out[4 * k] = b->knobs[k].rounds;
+            knobs_out[4 * k + 1] = b->knobs[k].dct_drop;
+            knobs_out[4 * k + 2] = b->knobs[k].adix_lsh;
+            knobs_out[4 * k + 3] = b->knobs[k].tile;
         }
     return NS_OK;
 }

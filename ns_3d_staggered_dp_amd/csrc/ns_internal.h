@@ -141,11 +141,16 @@ __device__ __forceinline__ const T &batch_params(const T *params, const int *pre
 size_t predict_params_size();
 int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                        double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap = false);
+                        double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap = false,
+                        int tile = 0);
 // tile columns x z cells of a box (the predictor's serial work units) and resident predictor blocks
-long predict_tile_columns(const ns_config &c);
-int predict_resident_blocks(bool lazy);
-void launch_predict_batch(bool lazy, const void *params, const int *prefix, int nbox, int nblocks, hipStream_t st);
+// tile width of a box's predictor launch: tile 64 or 32 as asked, 0: the
+// shape with fewer idle lanes (64 x 4 on ties)
+int predict_shape(const ns_config &c, int tile);
+long predict_tile_columns(const ns_config &c, int tile = 0);
+int predict_resident_blocks(bool lazy, int tile = 64);
+void launch_predict_batch(bool lazy, int tile, const void *params, const int *prefix, int nbox, int nblocks,
+                          hipStream_t st);
 size_t bc_params_size();
 size_t proj_params_size();
 // fill / projection parameters of one box; return the block count (0: nothing to launch)

@@ -41,17 +41,24 @@ namespace ns {
 
 namespace {
 
-constexpr int TX = 64, TY = 4;
-constexpr int HX = TX + 2, HY = TY + 2;
-constexpr int UW = HX + 1;   // u rows: local x = HX holds u(nx+1) (periodic wrap of the x average)
-constexpr int VH = HY + 1;   // v tile: local row HY holds v(ny+1) (periodic wrap of the y average)
-constexpr int DXT = HX + 1, DYT = HY + 1;  // dp tile, covering the lo/hi points of all three components
 constexpr int NVS = 4;       // velocity ring slots: one barrier per z step
 constexpr int NDS = 3;       // dp ring slots
-constexpr int NT = TX * TY;
+constexpr int NT = 256;      // threads per block, one per cell of the tile
 constexpr int NE = 2;        // load slots per thread and array (every tile has <= 2*NT elements)
-static_assert(HY * UW <= NE * NT && VH * HX <= NE * NT && HY * HX <= NE * NT && DYT * DXT <= NE * NT,
-              "tile larger than the per-thread load slots");
+
+// Tile shapes: TXV x (NT / TXV) cells per block; 64 x 4 by default, 32 x 8
+// where a box's width leaves fewer idle lanes that way (a 70-wide box: two
+// 64-wide tiles use 55% of their lanes, three 32-wide ones 73%)
+template <int TXV>
+struct Shape {
+    static constexpr int TX = TXV, TY = NT / TXV;
+    static constexpr int HX = TX + 2, HY = TY + 2;
+    static constexpr int UW = HX + 1;   // u rows: local x = HX holds u(nx+1) (periodic wrap of the x average)
+    static constexpr int VH = HY + 1;   // v tile: local row HY holds v(ny+1) (periodic wrap of the y average)
+    static constexpr int DXT = HX + 1, DYT = HY + 1;  // dp tile, covering the lo/hi points of all three components
+    static_assert(HY * UW <= NE * NT && VH * HX <= NE * NT && HY * HX <= NE * NT && DYT * DXT <= NE * NT,
+                  "tile larger than the per-thread load slots");
+};
 
 struct TileParams {
     int nx, ny, nz;
@@ -60,6 +67,7 @@ struct TileParams {
     int scheme, first, dp_flag;
     int per[3];          // own-axis periodicity of u, v, w (bc == 1)
     int wrap;            // every axis periodic: ghost reads take the interior image
+    int tx;              // tile width (64 or 32; the tile is tx x 256/tx cells)
     int zc;              // z cells per block
     int tiles_x, tiles_y, tiles_z, ntiles;
     const double *src[3];  // u, v, w (or u*, v*, w* when lazy)
@@ -130,19 +138,23 @@ __device__ __forceinline__ double load_val(const TileParams &P, int c, int gi, i
 }
 
 // LDS tiles of one block: rings of z-slices of the halo'd tiles
-template <bool LAZY>
+template <bool LAZY, int TXV>
 struct Tiles {
-    double u[NVS][HY][UW];
-    double v[NVS][VH][HX];
-    double w[NVS][HY][HX];
-    double d[LAZY ? NDS : 1][DYT][DXT];
+    using S = Shape<TXV>;
+    double u[NVS][S::HY][S::UW];
+    double v[NVS][S::VH][S::HX];
+    double w[NVS][S::HY][S::HX];
+    double d[LAZY ? NDS : 1][S::DYT][S::DXT];
     double spare[64];  // target of the slice stores of lanes past a tile's end
 };
 
 // one block's column of tiles; b is the block's index within its box
-template <bool LAZY>
+template <bool LAZY, int TXV>
 __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
-    __shared__ Tiles<LAZY> T;
+    using S = Shape<TXV>;
+    constexpr int TX = S::TX, TY = S::TY, HX = S::HX, HY = S::HY, UW = S::UW, VH = S::VH, DXT = S::DXT,
+                  DYT = S::DYT;
+    __shared__ Tiles<LAZY, TXV> T;
     // XCD-aware tile order: consecutive tiles of one XCD are x/y neighbours
     const int nb = P.ntiles;
     const int per_xcd = (nb + 7) / 8;
@@ -500,18 +512,18 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
     for (int az = z0; az < z1; az++) zstep(az);
 }
 
-template <bool LAZY>
+template <bool LAZY, int TXV>
 __global__ void __launch_bounds__(NT, 3) k_predict_tiled(TileParams P) {
-    predict_tile<LAZY>(P, blockIdx.x);
+    predict_tile<LAZY, TXV>(P, blockIdx.x);
 }
 
 // many boxes in one launch: prefix[i] is the first block of box i
-template <bool LAZY>
+template <bool LAZY, int TXV>
 __global__ void __launch_bounds__(NT, 3) k_predict_tiled_batch(const TileParams *__restrict__ Ps,
                                                                const int *__restrict__ prefix, int nbox) {
     int blk;
     const TileParams &P = batch_params(Ps, prefix, nbox, blk);
-    predict_tile<LAZY>(P, blk);
+    predict_tile<LAZY, TXV>(P, blk);
 }
 
 }  // namespace
@@ -519,15 +531,17 @@ __global__ void __launch_bounds__(NT, 3) k_predict_tiled_batch(const TileParams 
 namespace {
 
 // blocks of the predictor resident on the whole device at once (cached per device)
-int resident_blocks(bool lazy) {
-    static int cache[2][64] = {};
+int resident_blocks(bool lazy, int tx) {
+    static int cache[2][2][64] = {};
     int dev = 0;
     (void)hipGetDevice(&dev);
-    int &slots = cache[lazy ? 1 : 0][dev & 63];
+    int &slots = cache[lazy ? 1 : 0][tx == 32 ? 1 : 0][dev & 63];
     if (!slots) {
         int per_cu = 0, cus = 0;
-        if (lazy) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<true>, NT, 0);
-        else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<false>, NT, 0);
+        if (lazy && tx == 32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<true, 32>, NT, 0);
+        else if (lazy) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<true, 64>, NT, 0);
+        else if (tx == 32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<false, 32>, NT, 0);
+        else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_tiled<false, 64>, NT, 0);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         slots = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
     }
@@ -557,15 +571,24 @@ int choose_zc(int nz, long txy, int slots) {
 
 size_t predict_params_size() { return sizeof(TileParams); }
 
-long predict_tile_columns(const ns_config &c) {
-    return (long)((c.nx + TX - 1) / TX) * ((c.ny + TY - 1) / TY) * c.nz;
+int predict_shape(const ns_config &c, int tile) {
+    if (tile == 64 || tile == 32) return tile;
+    // the shape with fewer idle lanes (ties: 64 x 4, whose rows are longer)
+    const long a64 = (long)((c.nx + 63) / 64) * 64 * ((c.ny + 3) / 4) * 4;
+    const long a32 = (long)((c.nx + 31) / 32) * 32 * ((c.ny + 7) / 8) * 8;
+    return a32 < a64 ? 32 : 64;
 }
 
-int predict_resident_blocks(bool lazy) { return resident_blocks(lazy); }
+long predict_tile_columns(const ns_config &c, int tile) {
+    const int tx = predict_shape(c, tile), ty = NT / tx;
+    return (long)((c.nx + tx - 1) / tx) * ((c.ny + ty - 1) / ty) * c.nz;
+}
+
+int predict_resident_blocks(bool lazy, int tile) { return resident_blocks(lazy, tile == 32 ? 32 : 64); }
 
 int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
-                        double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap) {
+                        double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap, int tile) {
     TileParams &P = *(TileParams *)out;
     P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
     P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
@@ -577,15 +600,17 @@ int make_predict_params(const ns_config &c, int first, bool lazy, const double *
     P.scheme = c.scheme; P.first = first; P.dp_flag = c.dp_flag;
     for (int a = 0; a < 3; a++) P.per[a] = (c.bc[a] == 1);
     P.wrap = (wrap && P.per[0] && P.per[1] && P.per[2]) ? 1 : 0;
-    P.tiles_x = (c.nx + TX - 1) / TX;
-    P.tiles_y = (c.ny + TY - 1) / TY;
+    P.tx = predict_shape(c, tile);
+    const int ty = NT / P.tx;
+    P.tiles_x = (c.nx + P.tx - 1) / P.tx;
+    P.tiles_y = (c.ny + ty - 1) / ty;
     if (zc_hint > 0) {
         // equal chunks of about zc_hint cells
         const int tz = (c.nz + zc_hint - 1) / zc_hint;
         P.zc = (c.nz + tz - 1) / tz;
     } else {
         // a box alone: chunks that fill whole rounds of the resident blocks
-        P.zc = choose_zc(c.nz, (long)P.tiles_x * P.tiles_y, resident_blocks(lazy));
+        P.zc = choose_zc(c.nz, (long)P.tiles_x * P.tiles_y, resident_blocks(lazy, P.tx));
     }
     P.tiles_z = (c.nz + P.zc - 1) / P.zc;
     P.ntiles = P.tiles_x * P.tiles_y * P.tiles_z;
@@ -597,12 +622,17 @@ int make_predict_params(const ns_config &c, int first, bool lazy, const double *
     return ((P.ntiles + 7) / 8) * 8;
 }
 
-void launch_predict_batch(bool lazy, const void *params, const int *prefix, int nbox, int nblocks, hipStream_t st) {
+void launch_predict_batch(bool lazy, int tile, const void *params, const int *prefix, int nbox, int nblocks,
+                          hipStream_t st) {
     const TileParams *Ps = (const TileParams *)params;
-    if (lazy)
-        hipLaunchKernelGGL(k_predict_tiled_batch<true>, dim3(nblocks), dim3(TX * TY), 0, st, Ps, prefix, nbox);
+    if (lazy && tile == 32)
+        hipLaunchKernelGGL((k_predict_tiled_batch<true, 32>), dim3(nblocks), dim3(NT), 0, st, Ps, prefix, nbox);
+    else if (lazy)
+        hipLaunchKernelGGL((k_predict_tiled_batch<true, 64>), dim3(nblocks), dim3(NT), 0, st, Ps, prefix, nbox);
+    else if (tile == 32)
+        hipLaunchKernelGGL((k_predict_tiled_batch<false, 32>), dim3(nblocks), dim3(NT), 0, st, Ps, prefix, nbox);
     else
-        hipLaunchKernelGGL(k_predict_tiled_batch<false>, dim3(nblocks), dim3(TX * TY), 0, st, Ps, prefix, nbox);
+        hipLaunchKernelGGL((k_predict_tiled_batch<false, 64>), dim3(nblocks), dim3(NT), 0, st, Ps, prefix, nbox);
 }
 
 void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
@@ -610,10 +640,10 @@ void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double
                           double *us, double *vs, double *ws, hipStream_t st, bool wrap) {
     TileParams P;
     const int nblk = make_predict_params(c, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P, 0, wrap);
-    if (lazy)
-        hipLaunchKernelGGL(k_predict_tiled<true>, dim3(nblk), dim3(TX * TY), 0, st, P);
-    else
-        hipLaunchKernelGGL(k_predict_tiled<false>, dim3(nblk), dim3(TX * TY), 0, st, P);
+    if (lazy && P.tx == 32) hipLaunchKernelGGL((k_predict_tiled<true, 32>), dim3(nblk), dim3(NT), 0, st, P);
+    else if (lazy) hipLaunchKernelGGL((k_predict_tiled<true, 64>), dim3(nblk), dim3(NT), 0, st, P);
+    else if (P.tx == 32) hipLaunchKernelGGL((k_predict_tiled<false, 32>), dim3(nblk), dim3(NT), 0, st, P);
+    else hipLaunchKernelGGL((k_predict_tiled<false, 64>), dim3(nblk), dim3(NT), 0, st, P);
 }
 
 }  // namespace ns
