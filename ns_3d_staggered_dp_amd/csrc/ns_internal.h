@@ -144,8 +144,8 @@ int make_predict_params(const ns_config &c, int first, bool lazy, const double *
                         double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap = false,
                         int tile = 0);
 // tile columns x z cells of a box (the predictor's serial work units) and resident predictor blocks
-// tile width of a box's predictor launch: tile 64 or 32 as asked, 0: the
-// shape with fewer idle lanes (64 x 4 on ties)
+// tile width of a box's predictor launch: tile 64 or 32 as asked, 0: 32 x 8
+// for boxes up to 128 wide, else the shape with fewer idle lanes (64 x 4 on ties)
 int predict_shape(const ns_config &c, int tile);
 long predict_tile_columns(const ns_config &c, int tile = 0);
 int predict_resident_blocks(bool lazy, int tile = 64);

@@ -46,9 +46,9 @@ constexpr int NDS = 3;       // dp ring slots
 constexpr int NT = 256;      // threads per block, one per cell of the tile
 constexpr int NE = 2;        // load slots per thread and array (every tile has <= 2*NT elements)
 
-// Tile shapes: TXV x (NT / TXV) cells per block; 64 x 4 by default, 32 x 8
-// where a box's width leaves fewer idle lanes that way (a 70-wide box: two
-// 64-wide tiles use 55% of their lanes, three 32-wide ones 73%)
+// Tile shapes: TXV x (NT / TXV) cells per block; 64 x 4, or 32 x 8 for boxes
+// up to 128 wide and where a box's width leaves fewer idle lanes that way (a
+// 70-wide box: two 64-wide tiles use 55% of their lanes, three 32-wide ones 73%)
 template <int TXV>
 struct Shape {
     static constexpr int TX = TXV, TY = NT / TXV;
@@ -573,10 +573,13 @@ size_t predict_params_size() { return sizeof(TileParams); }
 
 int predict_shape(const ns_config &c, int tile) {
     if (tile == 64 || tile == 32) return tile;
-    // the shape with fewer idle lanes (ties: 64 x 4, whose rows are longer)
+    // boxes up to 128 wide: 32 x 8 (a smaller halo, 34 x 10 around 256 cells
+    // against 66 x 6, and fewer idle lanes on such widths); wider ones the
+    // shape with fewer idle lanes, 64 x 4 on ties (its longer rows stream
+    // better at 256^3: 0.421 against 0.432 ms)
     const long a64 = (long)((c.nx + 63) / 64) * 64 * ((c.ny + 3) / 4) * 4;
     const long a32 = (long)((c.nx + 31) / 32) * 32 * ((c.ny + 7) / 8) * 8;
-    return a32 < a64 ? 32 : 64;
+    return (c.nx <= 128 || a32 < a64) ? 32 : 64;
 }
 
 long predict_tile_columns(const ns_config &c, int tile) {
