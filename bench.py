@@ -185,7 +185,6 @@ def adjoint_cost(n, scheme_name="AB2-CN", steps=10):
         s.step(steps)
         s.synchronize()
         fwd = (time.perf_counter() - t0) / steps * 1e3
-        ptr = s.state_pointers()
         us, vs, ws = (torch.empty(int(x), dtype=torch.float64, device="cuda") for x in
                       ((n + 1) * (n + 2) ** 2, (n + 2) * (n + 1) * (n + 2), (n + 2) ** 2 * (n + 1)))
         _abi.call("ns_download_state", s.h, *(ctypes.c_void_p(t.data_ptr()) for t in (us, vs, ws)), None, None, None,

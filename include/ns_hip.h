@@ -97,8 +97,15 @@ int ns_download_work(ns_ctx *ctx, double *u_star, double *v_star, double *w_star
 int ns_synchronize(ns_ctx *ctx);
 /* device pointers of the resident state, in the order u,v,w,p,hx,hy,hz,u*,v*,w*,dp; valid
  * until the next step (u* alternates between two buffers when dp_flag == 0, and u,v,w,p
- * are then formed from u*, dp only by ns_download_state / ns_max_div / ns_check_finite) */
+ * are then formed from u*, dp only by ns_download_state / ns_max_div / ns_check_finite).
+ * The resident arrays keep the reference's index order and extents, but their x rows
+ * are ns_state_pitch() elements apart (whole 128-byte lines): entry (i,j,k) of an array
+ * with y extent n2 is at ptr[(i-1) + pitch*((j-1) + n2*(k-1))].  The copy calls
+ * (ns_upload_state, ns_download_state, ns_download_work) take and return dense arrays. */
 int ns_state_pointers(ns_ctx *ctx, double **ptrs11);
+/* row pitch (elements) of the resident arrays; no counterpart in the reference (its
+ * arrays are dense), for hosts that read ns_state_pointers' arrays themselves */
+int ns_state_pitch(ns_ctx *ctx, int *pitch);
 
 /* ---- fused time step (the loop body of big_periodic_3D.f90:326-571 and
  *      re_simulation_sub_tstep_interBC_combine.f90:508-802) ---- */

@@ -56,6 +56,7 @@ SIGNATURES = {
     "ns_download_work": [P, P, P, P, P],
     "ns_synchronize": [P],
     "ns_state_pointers": [P, P],
+    "ns_state_pitch": [P, P],
     "ns_set_bc_planes": [P, P],
     "ns_get_bc_planes": [P, P],
     "ns_bc_interp": [P, P, P, D],

@@ -15,11 +15,13 @@ using namespace ns;
 
 struct ns_ctx {
     ns_config cfg;
-    Grid g;
+    Grid g;   // the box in the layout of the resident state (dev_grid: line-aligned rows)
+    Grid gr;  // the box in the reference layout (caller arrays, mirrors, adjoint buffers)
     PlaneLayout L;
     hipStream_t st = nullptr;
     int dev = 0;  // device current at ns_create (owns the stream and every buffer)
-    // resident state (Fortran ghosted layout)
+    // resident state: the reference's ghosted arrays, with x rows padded to
+    // whole 128-byte lines (layout g)
     double *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
     double *hx = nullptr, *hy = nullptr, *hz = nullptr;
     double *us = nullptr, *vs = nullptr, *ws = nullptr, *dp = nullptr;
@@ -50,7 +52,7 @@ struct ns_ctx {
     int first = 1;
     // parent-run capture of a sub-box's boundary plane sets
     int cap_on = 0, cap_every = 1, cap_max = 0, cap_count = 0, cap_step = 0;
-    Grid cap_g{0, 0, 0};
+    Grid cap_g = Grid(0, 0, 0, 0, 0);
     int cap_o[3] = {0, 0, 0};
     long cap_len = 0;
     double *cap_dst = nullptr;
@@ -75,6 +77,9 @@ struct ns_ctx {
     // per argument position, grown on demand and kept until ns_destroy, so a
     // host that calls the mirrors every step allocates only on the first call
     std::vector<std::pair<double *, size_t>> stage;
+    // one dense array of the largest component: host transfers of the
+    // resident state pass through it (relayout on the device)
+    double *xfer = nullptr;
 };
 
 namespace {
@@ -107,6 +112,18 @@ int dupload(ns_ctx *c, T **p, const std::vector<T> &h) {
     int rc = dalloc(c, p, h.size());
     if (rc) return rc;
     HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return NS_OK;
+}
+
+// a component's ghosted array in layout g: the whole allocation is zeroed
+// (spare elements included) and *p is set to the array origin
+int dalloc_field(ns_ctx *c, double **p, const Grid &g, int comp) {
+    double *raw = nullptr;
+    const size_t n = field_alloc(g, comp);
+    int rc = dalloc(c, &raw, n);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(raw, 0, n * sizeof(double), c->st));
+    *p = raw + field_origin(g, comp);
     return NS_OK;
 }
 
@@ -524,6 +541,47 @@ int arg_out(ns_ctx *c, Arg &a) {
     return NS_OK;
 }
 
+// Transfers of one component's array between a caller's array in the
+// reference layout (host or device memory) and the resident state (layout
+// c->g): device arrays are relaid directly, host arrays through c->xfer.
+// Stream-ordered on c->st; a host destination is written when the stream
+// drains.
+int xfer_buffer(ns_ctx *c) {
+    return c->xfer ? NS_OK : dalloc(c, &c->xfer, c->gr.ref_np());
+}
+size_t ref_size(const Grid &g, int comp) {
+    return comp == 0 ? g.ref_nu() : comp == 1 ? g.ref_nv() : comp == 2 ? g.ref_nw() : g.ref_np();
+}
+int field_in(ns_ctx *c, double *own, const double *ext, int comp) {
+    int n[3];
+    comp_dims(c->g, comp, n);
+    const size_t rows = (size_t)n[1] * n[2];
+    const double *src = ext;
+    if (!is_dev_ptr(ext)) {
+        int rc = xfer_buffer(c);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->xfer, ext, ref_size(c->g, comp) * sizeof(double), hipMemcpyHostToDevice, c->st));
+        src = c->xfer;
+    }
+    launch_relayout(src, n[0], own, comp_pitch(c->g, comp), n[0], rows, nullptr, c->st);
+    return NS_OK;
+}
+// gauge (device, optional): subtracted from every value on the way out
+int field_out(ns_ctx *c, double *ext, const double *own, int comp, const double *gauge = nullptr) {
+    int n[3];
+    comp_dims(c->g, comp, n);
+    const size_t rows = (size_t)n[1] * n[2];
+    if (is_dev_ptr(ext)) {
+        launch_relayout(own, comp_pitch(c->g, comp), ext, n[0], n[0], rows, gauge, c->st);
+        return NS_OK;
+    }
+    int rc = xfer_buffer(c);
+    if (rc) return rc;
+    launch_relayout(own, comp_pitch(c->g, comp), c->xfer, n[0], n[0], rows, gauge, c->st);
+    HIPCHK(hipMemcpyAsync(ext, c->xfer, ref_size(c->g, comp) * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    return NS_OK;
+}
+
 // the arguments of one call, each with its own staging slot.  std::deque
 // keeps references to earlier elements valid across push_back (a std::vector
 // would reallocate and leave them dangling)
@@ -621,13 +679,13 @@ int one_step(ns_ctx *c, hipEvent_t *ev = nullptr) {
     const bool skip = skips_star_fill(c);
     if (c->lazy) {
         const double *gauge = (f.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
-        launch_predict_tiled(f, c->first, true, c->us, c->vs, c->ws, c->dp, gauge, c->p, c->hx, c->hy, c->hz, c->us2,
+        launch_predict_tiled(f, g, c->first, true, c->us, c->vs, c->ws, c->dp, gauge, c->p, c->hx, c->hy, c->hz, c->us2,
                              c->vs2, c->ws2, c->st, skip && !c->ghost_user);
         std::swap(c->us, c->us2);
         std::swap(c->vs, c->vs2);
         std::swap(c->ws, c->ws2);
     } else {
-        launch_predict_tiled(f, c->first, false, c->u, c->v, c->w, nullptr, nullptr, c->p, c->hx, c->hy, c->hz, c->us,
+        launch_predict_tiled(f, g, c->first, false, c->u, c->v, c->w, nullptr, nullptr, c->p, c->hx, c->hy, c->hz, c->us,
                              c->vs, c->ws, c->st);
     }
     if (ev) (void)hipEventRecord(ev[1], c->st);
@@ -693,7 +751,8 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
         delete c;
         return NS_ERR_HIP;
     }
-    c->g = Grid{cfg->nx, cfg->ny, cfg->nz};
+    c->g = dev_grid(cfg->nx, cfg->ny, cfg->nz);
+    c->gr = ref_grid(cfg->nx, cfg->ny, cfg->nz);
     c->L = plane_layout(cfg->nx, cfg->ny, cfg->nz);
     int rc = NS_OK;
     auto fail = [&](int r) {
@@ -708,11 +767,13 @@ int ns_create(const ns_config *cfg, ns_ctx **out) {
     const Grid &g = c->g;
     // element indices of the ghosted arrays are 32-bit in some kernels
     if (g.np() >= ((size_t)1 << 32)) return fail(NS_ERR_UNSUPPORTED);
-    if ((rc = dalloc(c, &c->u, g.nu())) || (rc = dalloc(c, &c->v, g.nv())) || (rc = dalloc(c, &c->w, g.nw())) ||
-        (rc = dalloc(c, &c->p, g.np())) || (rc = dalloc(c, &c->hx, g.nu())) || (rc = dalloc(c, &c->hy, g.nv())) ||
-        (rc = dalloc(c, &c->hz, g.nw())) || (rc = dalloc(c, &c->us, g.nu())) || (rc = dalloc(c, &c->vs, g.nv())) ||
-        (rc = dalloc(c, &c->ws, g.nw())) || (rc = dalloc(c, &c->dp, g.np())) ||
-        (rc = dalloc(c, &c->us2, g.nu())) || (rc = dalloc(c, &c->vs2, g.nv())) || (rc = dalloc(c, &c->ws2, g.nw())) ||
+    if ((rc = dalloc_field(c, &c->u, g, 0)) || (rc = dalloc_field(c, &c->v, g, 1)) ||
+        (rc = dalloc_field(c, &c->w, g, 2)) || (rc = dalloc_field(c, &c->p, g, 3)) ||
+        (rc = dalloc_field(c, &c->hx, g, 0)) || (rc = dalloc_field(c, &c->hy, g, 1)) ||
+        (rc = dalloc_field(c, &c->hz, g, 2)) || (rc = dalloc_field(c, &c->us, g, 0)) ||
+        (rc = dalloc_field(c, &c->vs, g, 1)) || (rc = dalloc_field(c, &c->ws, g, 2)) ||
+        (rc = dalloc_field(c, &c->dp, g, 3)) || (rc = dalloc_field(c, &c->us2, g, 0)) ||
+        (rc = dalloc_field(c, &c->vs2, g, 1)) || (rc = dalloc_field(c, &c->ws2, g, 2)) ||
         (rc = dalloc(c, &c->planes, (size_t)c->L.total)) || (rc = dalloc(c, &c->partial, (size_t)4096)) ||
         (rc = dalloc(c, &c->counter, (size_t)1)))
         return fail(rc);
@@ -799,17 +860,23 @@ int ns_upload_state(ns_ctx *c, const double *u, const double *v, const double *w
                     const double *hy, const double *hz, int first) {
     if (!c || !u || !v || !w) return NS_ERR_INVALID;
     const Grid &g = c->g;
-    HIPCHK(hipMemcpyAsync(c->u, u, g.nu() * 8, hipMemcpyDefault, c->st));
-    HIPCHK(hipMemcpyAsync(c->v, v, g.nv() * 8, hipMemcpyDefault, c->st));
-    HIPCHK(hipMemcpyAsync(c->w, w, g.nw() * 8, hipMemcpyDefault, c->st));
-    if (p) HIPCHK(hipMemcpyAsync(c->p, p, g.np() * 8, hipMemcpyDefault, c->st));
-    else HIPCHK(hipMemsetAsync(c->p, 0, g.np() * 8, c->st));
-    if (hx) HIPCHK(hipMemcpyAsync(c->hx, hx, g.nu() * 8, hipMemcpyDefault, c->st));
-    else HIPCHK(hipMemsetAsync(c->hx, 0, g.nu() * 8, c->st));
-    if (hy) HIPCHK(hipMemcpyAsync(c->hy, hy, g.nv() * 8, hipMemcpyDefault, c->st));
-    else HIPCHK(hipMemsetAsync(c->hy, 0, g.nv() * 8, c->st));
-    if (hz) HIPCHK(hipMemcpyAsync(c->hz, hz, g.nw() * 8, hipMemcpyDefault, c->st));
-    else HIPCHK(hipMemsetAsync(c->hz, 0, g.nw() * 8, c->st));
+    int rc;
+    if ((rc = field_in(c, c->u, u, 0)) || (rc = field_in(c, c->v, v, 1)) || (rc = field_in(c, c->w, w, 2))) return rc;
+    if (p) {
+        if ((rc = field_in(c, c->p, p, 3))) return rc;
+    } else {
+        HIPCHK(hipMemsetAsync(c->p, 0, g.np() * 8, c->st));
+    }
+    const double *hs[3] = {hx, hy, hz};
+    double *hd[3] = {c->hx, c->hy, c->hz};
+    const size_t hn[3] = {g.nu(), g.nv(), g.nw()};
+    for (int a = 0; a < 3; a++) {
+        if (hs[a]) {
+            if ((rc = field_in(c, hd[a], hs[a], a))) return rc;
+        } else {
+            HIPCHK(hipMemsetAsync(hd[a], 0, hn[a] * 8, c->st));
+        }
+    }
     if (c->lazy) {
         // u = u* - dt*grad(0) reproduces u exactly
         HIPCHK(hipMemcpyAsync(c->us, c->u, g.nu() * 8, hipMemcpyDeviceToDevice, c->st));
@@ -827,38 +894,30 @@ int ns_upload_state(ns_ctx *c, const double *u, const double *v, const double *w
 
 int ns_download_state(ns_ctx *c, double *u, double *v, double *w, double *p, double *hx, double *hy, double *hz) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
     materialize(c);
-    if (u) HIPCHK(hipMemcpyAsync(u, c->u, g.nu() * 8, hipMemcpyDefault, c->st));
-    if (v) HIPCHK(hipMemcpyAsync(v, c->v, g.nv() * 8, hipMemcpyDefault, c->st));
-    if (w) HIPCHK(hipMemcpyAsync(w, c->w, g.nw() * 8, hipMemcpyDefault, c->st));
-    if (p) HIPCHK(hipMemcpyAsync(p, c->p, g.np() * 8, hipMemcpyDefault, c->st));
-    if (hx) HIPCHK(hipMemcpyAsync(hx, c->hx, g.nu() * 8, hipMemcpyDefault, c->st));
-    if (hy) HIPCHK(hipMemcpyAsync(hy, c->hy, g.nv() * 8, hipMemcpyDefault, c->st));
-    if (hz) HIPCHK(hipMemcpyAsync(hz, c->hz, g.nw() * 8, hipMemcpyDefault, c->st));
+    double *dst[7] = {u, v, w, p, hx, hy, hz};
+    const double *src[7] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz};
+    const int comp[7] = {0, 1, 2, 3, 0, 1, 2};
+    for (int i = 0; i < 7; i++) {
+        if (!dst[i]) continue;
+        const int rc = field_out(c, dst[i], src[i], comp[i]);
+        if (rc) return rc;
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     return NS_OK;
 }
 
 int ns_download_work(ns_ctx *c, double *u_star, double *v_star, double *w_star, double *dp) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
     fill_star(c);
-    if (u_star) HIPCHK(hipMemcpyAsync(u_star, c->us, g.nu() * 8, hipMemcpyDefault, c->st));
-    if (v_star) HIPCHK(hipMemcpyAsync(v_star, c->vs, g.nv() * 8, hipMemcpyDefault, c->st));
-    if (w_star) HIPCHK(hipMemcpyAsync(w_star, c->ws, g.nw() * 8, hipMemcpyDefault, c->st));
-    if (dp) {
-        if (c->cfg.flags & NS_FLAG_GAUGE_CORNER) {
-            double *tmp = nullptr;
-            HIPCHK(hipMalloc(&tmp, g.np() * 8));
-            launch_sub_gauge(c->dp, c->dp, tmp, g.np(), c->st);
-            hipError_t e = hipMemcpyAsync(dp, tmp, g.np() * 8, hipMemcpyDefault, c->st);
-            (void)hipStreamSynchronize(c->st);
-            (void)hipFree(tmp);
-            HIPCHK(e);
-        } else {
-            HIPCHK(hipMemcpyAsync(dp, c->dp, g.np() * 8, hipMemcpyDefault, c->st));
-        }
+    double *dst[4] = {u_star, v_star, w_star, dp};
+    const double *src[4] = {c->us, c->vs, c->ws, c->dp};
+    // dp gauged like the driver's: dp - dp(1,1,1)
+    const double *gauge = (c->cfg.flags & NS_FLAG_GAUGE_CORNER) ? c->dp : nullptr;
+    for (int i = 0; i < 4; i++) {
+        if (!dst[i]) continue;
+        const int rc = field_out(c, dst[i], src[i], i, i == 3 ? gauge : nullptr);
+        if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(c->st));
     return NS_OK;
@@ -869,6 +928,12 @@ int ns_state_pointers(ns_ctx *c, double **p) {
     fill_star(c);
     double *q[11] = {c->u, c->v, c->w, c->p, c->hx, c->hy, c->hz, c->us, c->vs, c->ws, c->dp};
     for (int i = 0; i < 11; i++) p[i] = q[i];
+    return NS_OK;
+}
+
+int ns_state_pitch(ns_ctx *c, int *pitch) {
+    if (!c || !pitch) return NS_ERR_INVALID;
+    *pitch = c->g.pc;  // u-shaped arrays share it (dev_grid: pu == pc)
     return NS_OK;
 }
 
@@ -1194,7 +1259,7 @@ int ns_history_restore(ns_ctx *c) {
 
 static int adj_alloc(ns_ctx *c) {
     if (c->ab[0]) return NS_OK;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const size_t n[3] = {g.nu(), g.nv(), g.nw()};
     const bool cn = c->cfg.scheme == NS_AB2_CN;
     int rc;
@@ -1244,7 +1309,7 @@ int ns_adjoint_set_state(ns_ctx *c, const double *ub, const double *vb, const do
     if (!adj_supported(c)) return NS_ERR_UNSUPPORTED;
     int rc = adj_alloc(c);
     if (rc) return rc;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
     const double *src[7] = {ub, vb, wb, pb, hxb, hyb, hzb};
     double *dst[7] = {c->ab[0], c->ab[1], c->ab[2], c->apb, c->ahb[0], c->ahb[1], c->ahb[2]};
@@ -1260,7 +1325,7 @@ int ns_adjoint_get_state(ns_ctx *c, double *ub, double *vb, double *wb, double *
                          double *hzb) {
     if (!c) return NS_ERR_INVALID;
     if (!c->ab[0]) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const size_t n[7] = {g.nu(), g.nv(), g.nw(), g.np(), g.nu(), g.nv(), g.nw()};
     double *dst[7] = {ub, vb, wb, pb, hxb, hyb, hzb};
     const double *src[7] = {c->ab[0], c->ab[1], c->ab[2], c->apb, c->ahb[0], c->ahb[1], c->ahb[2]};
@@ -1275,7 +1340,7 @@ int ns_adjoint_step(ns_ctx *c, const double *u, const double *v, const double *w
     if (!adj_supported(c)) return NS_ERR_UNSUPPORTED;
     if (!c->ab[0]) return NS_ERR_INVALID;  // ns_adjoint_set_state first
     const ns_config &f = c->cfg;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const double h[3] = {f.dx, f.dy, f.dz};
     int per[3];
     for (int a = 0; a < 3; a++) per[a] = (f.bc[a] == 1);
@@ -1358,7 +1423,8 @@ int ns_extract_subbox(ns_ctx *c, const ns_subbox *sb, int field, double *dst) {
     Arg &D = args.add();
     int rc = arg_in(c, D, dst, (size_t)e[0] * e[1] * e[2], false);
     if (rc) return rc;
-    launch_extract_box(src[field], n, o, e, gauge, D.dev, c->st);
+    const int pn[3] = {comp_pitch(c->g, comp), n[1], n[2]};  // row pitch and rows of the resident array
+    launch_extract_box(src[field], pn, o, e, gauge, D.dev, c->st);
     if ((rc = arg_out(c, D))) return rc;
     HIPCHK(hipStreamSynchronize(c->st));
     return NS_OK;
@@ -1376,7 +1442,7 @@ int ns_capture_begin(ns_ctx *c, const ns_subbox *sb, double *dst, int max_sets, 
         int rc = dalloc(c, &c->cap_partial, (size_t)1024);
         if (rc) return rc;
     }
-    c->cap_g = Grid{sb->nx, sb->ny, sb->nz};
+    c->cap_g = ref_grid(sb->nx, sb->ny, sb->nz);
     c->cap_o[0] = sb->x0 - 1;
     c->cap_o[1] = sb->y0 - 1;
     c->cap_o[2] = sb->z0 - 1;
@@ -1430,8 +1496,8 @@ int ns_error_metrics(ns_ctx *c, const double *u_sub, const double *v_sub, const 
     Arg &A2 = args.add();
     Arg &A3 = args.add();
     int rc;
-    if ((rc = arg_in(c, A0, u_sub, g.nu(), true)) || (rc = arg_in(c, A1, v_sub, g.nv(), true)) ||
-        (rc = arg_in(c, A2, w_sub, g.nw(), true)) || (rc = arg_in(c, A3, p_sub, g.np(), true)))
+    if ((rc = arg_in(c, A0, u_sub, g.ref_nu(), true)) || (rc = arg_in(c, A1, v_sub, g.ref_nv(), true)) ||
+        (rc = arg_in(c, A2, w_sub, g.ref_nw(), true)) || (rc = arg_in(c, A3, p_sub, g.ref_np(), true)))
         return rc;
     const double *a[4] = {c->u, c->v, c->w, c->p};
     const double *b[4] = {A0.dev, A1.dev, A2.dev, A3.dev};
@@ -1451,7 +1517,7 @@ int ns_error_metrics(ns_ctx *c, const double *u_sub, const double *v_sub, const 
         mx[4] = q[12] > mx[4] ? q[12] : mx[4];
         ssq[4] += q[13];
     }
-    const double n[4] = {(double)g.nu(), (double)g.nv(), (double)g.nw(), (double)g.np()};
+    const double n[4] = {(double)g.ref_nu(), (double)g.ref_nv(), (double)g.ref_nw(), (double)g.ref_np()};
     const double noz = (double)((size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 2));
     for (int cc = 0; cc < 4; cc++) {
         rms[cc] = sqrt((double)(ssq[cc] / n[cc]));
@@ -1486,7 +1552,7 @@ int ns_error_metrics_noalloc(ns_ctx *c, const double *u_sub, const double *v_sub
     double q[NS_METRIC_PARTS];
     HIPCHK(hipMemcpyAsync(q, row, sizeof(q), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    const double n[4] = {(double)g.nu(), (double)g.nv(), (double)g.nw(), (double)g.np()};
+    const double n[4] = {(double)g.ref_nu(), (double)g.ref_nv(), (double)g.ref_nw(), (double)g.ref_np()};
     const double noz = (double)((size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 2));
     for (int cc = 0; cc < 4; cc++) {
         rms[cc] = sqrt(q[3 * cc + 2] / n[cc]);
@@ -1524,7 +1590,7 @@ int ns_check_finite(ns_ctx *c) {
 int ns_cal_conv(ns_ctx *c, const double *u, const double *v, const double *w, int bc_x, int bc_y, int bc_z, double dx,
                 double dy, double dz, double *conv_x, double *conv_y, double *conv_z) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     int bc[3] = {bc_x, bc_y, bc_z};
     for (int a = 0; a < 3; a++)
         if (bc[a] < 1 || bc[a] > 4) return NS_ERR_INVALID;
@@ -1551,7 +1617,7 @@ int ns_cal_conv(ns_ctx *c, const double *u, const double *v, const double *w, in
 int ns_cal_diff(ns_ctx *c, const double *u, const double *v, const double *w, int bc_x, int bc_y, int bc_z, double dx2,
                 double dy2, double dz2, double *diff_x, double *diff_y, double *diff_z) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     int bc[3] = {bc_x, bc_y, bc_z};
     for (int a = 0; a < 3; a++)
         if (bc[a] < 1 || bc[a] > 4) return NS_ERR_INVALID;
@@ -1578,7 +1644,7 @@ int ns_cal_diff(ns_ctx *c, const double *u, const double *v, const double *w, in
 int ns_vel_bc_staggered(ns_ctx *c, double *u_star, double *v_star, double *w_star, const double *planes, int bc_x,
                         int bc_y, int bc_z) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     int bc[3] = {bc_x, bc_y, bc_z};
     for (int a = 0; a < 3; a++) {
         if (bc[a] < 1 || bc[a] > 4) return NS_ERR_INVALID;
@@ -1605,7 +1671,7 @@ int ns_vel_bc_staggered_cn2(ns_ctx *c, double *u_star, double *v_star, double *w
                             int dir) {
     if (!c || dir < 1 || dir > 3 || bc < 1 || bc > 4) return NS_ERR_INVALID;
     if ((bc == 2 || bc == 4) && !planes) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     ArgList A;
     int rc;
     Arg &au = A.add(); if ((rc = arg_in(c, au, u_star, g.nu(), true))) return rc;
@@ -1626,7 +1692,7 @@ int ns_vel_bc_staggered_cn2(ns_ctx *c, double *u_star, double *v_star, double *w
 int ns_adi_sweep(ns_ctx *c, double *u_star, double *v_star, double *w_star, int dir) {
     if (!c || dir < 1 || dir > 3) return NS_ERR_INVALID;
     if (c->cfg.scheme != NS_AB2_CN) return NS_ERR_UNSUPPORTED;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     ArgList A;
     int rc;
     Arg &au = A.add(); if ((rc = arg_in(c, au, u_star, g.nu(), true))) return rc;
@@ -1641,7 +1707,7 @@ int ns_adi_sweep(ns_ctx *c, double *u_star, double *v_star, double *w_star, int 
 int ns_get_velpr_bc(ns_ctx *c, const double *u_bc, const double *v_bc, const double *w_bc, const double *p_bc, double dt,
                     int dp_flag, int using_ustar, const double *p_previous, double *planes) {
     if (!c || !planes) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     const int corr = !(dp_flag || using_ustar);
     ArgList A;
@@ -1666,7 +1732,7 @@ int ns_get_velpr_bc(ns_ctx *c, const double *u_bc, const double *v_bc, const dou
 
 int ns_get_pr_bc(ns_ctx *c, const double *p_bc, double *planes) {
     if (!c || !planes) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     ArgList A;
     int rc;
@@ -1680,7 +1746,7 @@ int ns_get_pr_bc(ns_ctx *c, const double *p_bc, double *planes) {
 
 int ns_divergence(ns_ctx *c, const double *u_star, const double *v_star, const double *w_star, double dt, double *rhs) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     ArgList A;
     int rc;
@@ -1696,7 +1762,7 @@ int ns_divergence(ns_ctx *c, const double *u_star, const double *v_star, const d
 
 int ns_pr_bc_staggered_modify_rhs(ns_ctx *c, double *rhs, const double *planes) {
     if (!c || !planes) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     ArgList A;
     int rc;
@@ -1711,7 +1777,7 @@ int ns_pr_bc_staggered_modify_rhs(ns_ctx *c, double *rhs, const double *planes) 
 int ns_poisson_fft(ns_ctx *c, const double *rhs, double *sol) {
     if (!c) return NS_ERR_INVALID;
     if (!c->poisson_ok || c->pp.dct) return NS_ERR_UNSUPPORTED;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     ArgList A;
     int rc;
@@ -1727,7 +1793,7 @@ int ns_poisson_fft(ns_ctx *c, const double *rhs, double *sol) {
 int ns_dct_poisson_solver(ns_ctx *c, double *rhs, double mean_dp) {
     if (!c) return NS_ERR_INVALID;
     if (!c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     ArgList A;
     int rc;
@@ -1741,7 +1807,7 @@ int ns_dct_poisson_solver(ns_ctx *c, double *rhs, double mean_dp) {
 
 int ns_pr_bc_staggered(ns_ctx *c, double *dp, const double *planes) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     bool need = f.pbc[0] != 1 || f.pbc[1] != 1 || f.pbc[2] != 1;
     if (need && !planes) return NS_ERR_INVALID;
@@ -1763,7 +1829,7 @@ int ns_pr_bc_staggered(ns_ctx *c, double *dp, const double *planes) {
 int ns_project(ns_ctx *c, const double *u_star, const double *v_star, const double *w_star, const double *dp, double dt,
                int dp_flag, double *u, double *v, double *w, double *p) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     ArgList A;
     int rc;
@@ -1797,7 +1863,7 @@ int stage_planes_b(ns_ctx *c, ArgList &A, double *planes_b, double **dev) {
 int ns_vel_bc_staggered_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, double *planes_b, int bc_x, int bc_y,
                             int bc_z) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     int bc[3] = {bc_x, bc_y, bc_z};
     for (int a = 0; a < 3; a++) {
@@ -1825,7 +1891,7 @@ int ns_vel_bc_staggered_cn2_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b
     if (!c || dir < 1 || dir > 3 || bc < 1 || bc > 4) return NS_ERR_INVALID;
     if (bc == 3) return NS_OK;  // the forward leaves the lines as they are
     if (!planes_b) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     ArgList A;
     int rc;
     Arg &au = A.add(); if ((rc = arg_in(c, au, u_b, g.nu(), true))) return rc;
@@ -1846,7 +1912,7 @@ int ns_adi_sweep_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, int dir) 
     if (c->cfg.scheme != NS_AB2_CN) return NS_ERR_UNSUPPORTED;
     int rc = adj_cn_systems(c);
     if (rc) return rc;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     ArgList A;
     Arg &au = A.add(); if ((rc = arg_in(c, au, u_b, g.nu(), true))) return rc;
     Arg &av = A.add(); if ((rc = arg_in(c, av, v_b, g.nv(), true))) return rc;
@@ -1859,7 +1925,7 @@ int ns_adi_sweep_adj(ns_ctx *c, double *u_b, double *v_b, double *w_b, int dir) 
 
 int ns_divergence_adj(ns_ctx *c, const double *rhs_b, double dt, double *u_b, double *v_b, double *w_b) {
     if (!c || !rhs_b) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     const double h[3] = {f.dx, f.dy, f.dz};
     ArgList A;
@@ -1877,7 +1943,7 @@ int ns_divergence_adj(ns_ctx *c, const double *rhs_b, double dt, double *u_b, do
 
 int ns_pr_bc_staggered_modify_rhs_adj(ns_ctx *c, const double *rhs_b, double *planes_b) {
     if (!c || !rhs_b || !planes_b) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     const double h[3] = {f.dx, f.dy, f.dz};
     ArgList A;
@@ -1899,7 +1965,7 @@ int ns_poisson_fft_adj(ns_ctx *c, const double *sol_b, double *rhs_b) {
 int ns_dct_poisson_solver_adj(ns_ctx *c, double *rhs_b, double *mean_dp_b) {
     if (!c || !rhs_b) return NS_ERR_INVALID;
     if (!c->poisson_ok || !c->pp.dct) return NS_ERR_UNSUPPORTED;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     if (!c->mpartial) {
         int rc = dalloc(c, &c->mpartial, (size_t)2048 * NS_METRIC_PARTS);
@@ -1928,7 +1994,7 @@ int ns_dct_poisson_solver_adj(ns_ctx *c, double *rhs_b, double *mean_dp_b) {
 
 int ns_pr_bc_staggered_adj(ns_ctx *c, double *dp_b, double *planes_b) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     const bool need = f.pbc[0] != 1 || f.pbc[1] != 1 || f.pbc[2] != 1;
     if (need && !planes_b) return NS_ERR_INVALID;
@@ -1949,7 +2015,7 @@ int ns_pr_bc_staggered_adj(ns_ctx *c, double *dp_b, double *planes_b) {
 int ns_project_adj(ns_ctx *c, const double *u_b, const double *v_b, const double *w_b, double *p_b, double dt,
                    int dp_flag, double *us_b, double *vs_b, double *ws_b, double *dp_b) {
     if (!c) return NS_ERR_INVALID;
-    const Grid &g = c->g;
+    const Grid &g = c->gr;
     const ns_config &f = c->cfg;
     const double h[3] = {f.dx, f.dy, f.dz};
     ArgList A;
@@ -1990,6 +2056,7 @@ struct ns_batch {
         size_t lds = 0;
         int variant = 0;  // DCT pass: kernel variant (OR over the boxes); sweep: waves per block
         int rows = 0;     // sweep: rows per lane of the partitioned kernel (the most over the boxes)
+        int dense = 1;    // sweep: every box of the launch keeps the dense layout
         int nbox = 0;     // boxes of this launch (a subset for sweeps grouped by blocks per line)
     };
     // one steady (non-first) step per parity of the boxes' u* buffers: with
@@ -2085,10 +2152,10 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
             const int zc_hint = (int)(z > 8 ? z : 8);
             const double *gauge = (b->lazy && (f.flags & NS_FLAG_GAUGE_CORNER)) ? c->dp : nullptr;
             if (b->lazy)
-                nb = make_predict_params(f, first_step ? c->first : 0, true, c->us, c->vs, c->ws, c->dp, gauge, c->p,
+                nb = make_predict_params(f, c->g, first_step ? c->first : 0, true, c->us, c->vs, c->ws, c->dp, gauge, c->p,
                                          c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint, false, arg);
             else
-                nb = make_predict_params(f, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr,
+                nb = make_predict_params(f, c->g, first_step ? c->first : 0, false, c->u, c->v, c->w, nullptr, nullptr,
                                          c->p, c->hx, c->hy, c->hz, nus, nvs, nws, dst, zc_hint, false, arg);
         } else if (kind == 1) {
             const double d0[3] = {0.0, 0.0, 0.0};
@@ -2127,6 +2194,7 @@ int add_phase(ns_batch *b, int kind, int arg, bool first_step, std::vector<ns_ba
     ph.lds = lds;
     ph.variant = variant;
     ph.rows = rows;
+    for (int bi : *sel) ph.dense = ph.dense && b->ctx[bi]->g.dense();
     ph.nbox = n;
     int rc;
     if ((rc = balloc(&ph.params, host.size())) || (rc = balloc(&ph.prefix, prefix.size() * sizeof(int)))) {
@@ -2187,7 +2255,8 @@ void run_phases(ns_batch *b, const std::vector<ns_batch::Phase> &phases) {
             case 3: launch_whole_fill_batch(ph.params, ph.prefix, n, ph.nblk, 1, b->st); break;
             case 5: launch_cn2_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
             case 6:
-                launch_adi_batch(ph.arg, ph.variant, ph.rows, ph.params, ph.prefix, n, ph.nblk, ph.lds, b->st);
+                launch_adi_batch(ph.arg, ph.variant, ph.rows, ph.dense != 0, ph.params, ph.prefix, n, ph.nblk, ph.lds,
+                                 b->st);
                 break;
             default: launch_project_batch(ph.params, ph.prefix, n, ph.nblk, b->st); break;
         }

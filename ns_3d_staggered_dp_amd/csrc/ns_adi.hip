@@ -194,6 +194,13 @@ struct SweepParams {
     // rows vel_bc_staggered_CN2 zeroes; not read); gz_out, the ghost rows are
     // stored as zeros (the transpose of that, after an adjoint solve)
     int gz_in, gz_out;
+    // partitioned y / z sweeps: lanes walk the storage slots of a plane of
+    // each component's lines (slot s is element s - xs[c] of the rows of
+    // lines, row pitch px), so the 64 lanes of a wavefront read and write
+    // whole 128-byte lines of each array row; slots of the spare elements at
+    // the row ends idle.  xs[c] is the array origin's offset from its
+    // line-aligned start.
+    int xs[3];
 };
 
 // ---- partitioned wall solve (LineSys::K blocks per line, wave k = block k) ----
@@ -344,34 +351,56 @@ __device__ __forceinline__ void part_ghosts(int k, int K, const LineSys &S, bool
 }
 
 // y / z sweeps: lane = line (consecutive x), so every row load of a wave is
-// 64 consecutive doubles
-template <int AX, int RC>
+// 64 consecutive doubles.  DENSE: the arrays have the dense layout (lane t
+// holds line t); otherwise lanes walk the storage slots of the aligned layout
+// (SweepParams::xs), so a wave's row accesses start on a 128-byte line.  The
+// two are separate instantiations because the slot arithmetic costs the dense
+// kernel two registers and with them a resident wave per SIMD.
+template <int AX, int RC, bool DENSE>
 __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
     const int c = blockIdx.y;
     int n[3];
     comp_dims(P.g, c, n);
     const int o2 = (AX == 2) ? 1 : 2;
-    const int stride = (AX == 1) ? n[0] : n[0] * n[1];
+    const int px = DENSE ? n[0] : comp_pitch(P.g, c);
+    const unsigned xs = DENSE ? 0u : (unsigned)P.xs[c];
+    const int stride = (AX == 1) ? px : px * n[1];
     const LineSys &S = P.sys[c];
     const bool per = S.periodic;
     const int K = S.K, N = per ? S.m : S.N, off = per ? S.off : 0;  // N: unknowns per line
-    const unsigned nl = (unsigned)n[0] * (unsigned)n[o2];
+    const unsigned nl = (unsigned)px * (unsigned)n[o2] + xs;  // line slots
     if ((unsigned)blk * 64u >= nl) return;  // the whole block: this component has fewer lines
     const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const unsigned t = (unsigned)blk * 64u + lane;
-    const bool valid = t < nl;
+    const unsigned ts = (unsigned)blk * 64u + lane;
+    bool valid = ts < nl;
     const bool act = k < K;
     const int s0 = act ? (k * N) / K : 0;
     const int R = act ? ((k + 1) * N) / K - s0 : 0;
     int id[3] = {1, 1, 1};
-    if (valid) {
-        const unsigned q = t / (unsigned)n[0];
-        id[0] = (int)(t - q * (unsigned)n[0]) + 1;
-        id[o2] = (int)q + 1;
+    unsigned t = ts;  // the line's index in plane order (x fastest)
+    if (DENSE) {
+        if (valid) {
+            const unsigned q = t / (unsigned)n[0];
+            id[0] = (int)(t - q * (unsigned)n[0]) + 1;
+            id[o2] = (int)q + 1;
+        }
+    } else {
+        // slot ts holds element e = ts - xs of the plane's rows of lines (e
+        // wraps past nl when ts < xs); spare slots at the row ends idle
+        const unsigned e = ts - xs;
+        valid = e < nl - xs;
+        const unsigned q = valid ? e / (unsigned)px : 0u;
+        const int xr = (int)(e - q * (unsigned)px);
+        valid = valid && xr < n[0];
+        if (valid) {
+            id[0] = xr + 1;
+            id[o2] = (int)q + 1;
+        }
+        t = (unsigned)xr + (unsigned)n[0] * q;
     }
-    double *line = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]);
+    double *line = P.a[c] + GIX(id[0], id[1], id[2], px, n[1]);
     double *x = line + (size_t)stride * (off + s0);
     const double *tb = S.pt;
     double glo = 0.0, ghi = 0.0;
@@ -409,8 +438,10 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
 
 // x sweeps: the block stages its P.xl lines (one contiguous span; 64, or 32
 // or 16 when 64 long lines would not fit the LDS) in LDS rows of odd length
-// LS >= N, coalesced both ways; the block-end values follow at P.coff
-template <int RC>
+// LS >= N, coalesced both ways; the block-end values follow at P.coff.
+// DENSE: rows N apart; otherwise the aligned layout's storage slots are
+// walked (spare slots skipped)
+template <int RC, bool DENSE>
 __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
     const int c = blockIdx.y;
@@ -424,14 +455,22 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
     const int l0 = blk * P.xl;
     if (l0 >= nl) return;  // the whole block: this component has fewer lines
     const int cnt = (nl - l0 < P.xl) ? nl - l0 : P.xl;
-    const int LS = N | 1, tot = cnt * N;
-    double *a = P.a[c] + (size_t)l0 * N;
-    const float rN = 1.0f / (float)N;  // f < 2^24: float quotient within one
-    auto slot = [&](int f) {
-        int l = (int)((float)f * rN);
-        if (l * N > f) l--;
-        else if ((l + 1) * N <= f) l++;
-        return l * LS + (f - l * N);
+    const int LS = N | 1, px = DENSE ? N : comp_pitch(P.g, c), xs = DENSE ? 0 : P.xs[c];
+    // the block's lines are rows l0.. of the array: walked as storage slots
+    // from the line-aligned start before row l0 (slot g holds element
+    // g - xs of the rows, row pitch px), so every wavefront access starts on
+    // a 128-byte line; slots outside the rows' N entries are skipped
+    double *a = P.a[c] + (size_t)l0 * px - xs;
+    const int tot = cnt * px + xs;
+    const float rp = 1.0f / (float)px;  // g < 2^24: float quotient within one
+    auto slot = [&](int g) {  // LDS slot of storage slot g, or -1
+        const int f = g - xs;
+        int l = (int)((float)f * rp);
+        if (l * px > f) l--;
+        else if ((l + 1) * px <= f) l++;
+        const int pos = f - l * px;
+        if (DENSE) return l * LS + pos;
+        return (f >= 0 && pos < N) ? l * LS + pos : -1;
     };
     const int nth = blockDim.x;
     for (int f0 = threadIdx.x; f0 < tot; f0 += nth * SB) {
@@ -439,8 +478,14 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
 #pragma unroll
         for (int u = 0; u < SB; u++) v[u] = (f0 + nth * u < tot) ? a[f0 + nth * u] : 0.0;
 #pragma unroll
-        for (int u = 0; u < SB; u++)
-            if (f0 + nth * u < tot) sm[slot(f0 + nth * u)] = v[u];
+        for (int u = 0; u < SB; u++) {
+            if (DENSE) {
+                if (f0 + nth * u < tot) sm[slot(f0 + nth * u)] = v[u];
+            } else {
+                const int q = slot(f0 + nth * u);
+                if (f0 + nth * u < tot && q >= 0) sm[q] = v[u];
+            }
+        }
     }
     __syncthreads();
     const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -486,11 +531,25 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
     __syncthreads();
     for (int f0 = threadIdx.x; f0 < tot; f0 += nth * SB) {
         double v[SB];
+        if (DENSE) {
 #pragma unroll
-        for (int u = 0; u < SB; u++) v[u] = (f0 + nth * u < tot) ? sm[slot(f0 + nth * u)] : 0.0;
+            for (int u = 0; u < SB; u++) v[u] = (f0 + nth * u < tot) ? sm[slot(f0 + nth * u)] : 0.0;
 #pragma unroll
-        for (int u = 0; u < SB; u++)
-            if (f0 + nth * u < tot) a[f0 + nth * u] = v[u];
+            for (int u = 0; u < SB; u++)
+                if (f0 + nth * u < tot) a[f0 + nth * u] = v[u];
+        } else {
+            unsigned ok = 0;  // bit u: slot f0 + nth*u holds an entry
+#pragma unroll
+            for (int u = 0; u < SB; u++) {
+                const int q = slot(f0 + nth * u);
+                const bool in = f0 + nth * u < tot && q >= 0;
+                ok |= (unsigned)in << u;
+                v[u] = in ? sm[q] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < SB; u++)
+                if ((ok >> u) & 1u) a[f0 + nth * u] = v[u];
+        }
     }
 }
 
@@ -509,7 +568,8 @@ __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int
     int n[3];
     comp_dims(P.g, c, n);
     const int o2 = (AX == 2) ? 1 : 2;
-    const long stride = (AX == 1) ? (long)n[0] : (long)n[0] * n[1];
+    const int px = comp_pitch(P.g, c);
+    const long stride = (AX == 1) ? (long)px : (long)px * n[1];
     const LineSys &S = P.sys[c];
     const size_t nl = (size_t)n[0] * n[o2];
     if (S.periodic) {
@@ -519,7 +579,7 @@ __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int
         id[0] = (int)(t % n[0]) + 1;
         id[o2] = (int)(t / n[0]) + 1;
         id[AX] = 1;
-        solve_line<16>(P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]), stride, S);
+        solve_line<16>(P.a[c] + GIX(id[0], id[1], id[2], px, n[1]), stride, S);
         return;
     }
     const bool top = threadIdx.x < 32;
@@ -531,7 +591,7 @@ __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int
         id[o2] = (int)(t / n[0]) + 1;
     }
     const int N = S.N, tk = S.tk;
-    double *x = P.a[c] + FIX(id[0], id[1], id[2], n[0], n[1]);
+    double *x = P.a[c] + GIX(id[0], id[1], id[2], px, n[1]);
     double *base = top ? x : x + stride * (N - 1);
     const long st = top ? stride : -stride;
     const int m = valid ? (top ? tk : N - 1 - tk) : 0;
@@ -582,22 +642,28 @@ __device__ __forceinline__ void adi_x_body(const SweepParams &P, const int blk) 
     const long l0 = (long)blk << lsh;
     if (l0 >= nl) return;  // this component has fewer lines (uniform per block)
     const int cnt = (int)(nl - l0 < L ? nl - l0 : L);
-    const int LS = N + 1, tot = cnt * N;
-    const size_t base = (size_t)l0 * N;
-    const float rN = 1.0f / (float)N;  // f < 2^24: float quotient within one
-    auto slot = [&](int f) {
-        int l = (int)((float)f * rN);
-        if (l * N > f) l--;
-        else if ((l + 1) * N <= f) l++;
-        return l * LS + (f - l * N);
+    const int LS = N + 1, px = comp_pitch(P.g, c);
+    // the block's rows walked as storage slots from the row origin (slot f:
+    // element f of the rows, pitch px); slots past a row's N entries skipped
+    const size_t base = (size_t)l0 * px;
+    const int tot = cnt * px;
+    const float rp = 1.0f / (float)px;  // f < 2^24: float quotient within one
+    auto slot = [&](int f) {  // LDS slot of storage slot f, or -1
+        int l = (int)((float)f * rp);
+        if (l * px > f) l--;
+        else if ((l + 1) * px <= f) l++;
+        const int pos = f - l * px;
+        return pos < N ? l * LS + pos : -1;
     };
     for (int f0 = threadIdx.x; f0 < tot; f0 += 64 * SB) {
         double v[SB];
 #pragma unroll
         for (int u = 0; u < SB; u++) v[u] = (f0 + 64 * u < tot) ? a[base + f0 + 64 * u] : 0.0;
 #pragma unroll
-        for (int u = 0; u < SB; u++)
-            if (f0 + 64 * u < tot) sm[slot(f0 + 64 * u)] = v[u];
+        for (int u = 0; u < SB; u++) {
+            const int q = slot(f0 + 64 * u);
+            if (f0 + 64 * u < tot && q >= 0) sm[q] = v[u];
+        }
     }
     __syncthreads();
     if (S.periodic) {
@@ -628,11 +694,17 @@ __device__ __forceinline__ void adi_x_body(const SweepParams &P, const int blk) 
     __syncthreads();
     for (int f0 = threadIdx.x; f0 < tot; f0 += 64 * SB) {
         double v[SB];
+        unsigned ok = 0;  // bit u: slot f0 + 64*u holds an entry
 #pragma unroll
-        for (int u = 0; u < SB; u++) v[u] = (f0 + 64 * u < tot) ? sm[slot(f0 + 64 * u)] : 0.0;
+        for (int u = 0; u < SB; u++) {
+            const int q = slot(f0 + 64 * u);
+            const bool in = f0 + 64 * u < tot && q >= 0;
+            ok |= (unsigned)in << u;
+            v[u] = in ? sm[q] : 0.0;
+        }
 #pragma unroll
         for (int u = 0; u < SB; u++)
-            if (f0 + 64 * u < tot) a[base + f0 + 64 * u] = v[u];
+            if ((ok >> u) & 1u) a[base + f0 + 64 * u] = v[u];
     }
 }
 
@@ -643,26 +715,28 @@ __global__ void __launch_bounds__(64) k_adi_strided(SweepParams P) {
 __global__ void __launch_bounds__(64) k_adi_x(SweepParams P) { adi_x_body(P, blockIdx.x); }
 
 constexpr int PART_THREADS = 64 * PART_MAX_BLOCKS;
-template <int AX, int RC>
+template <int AX, int RC, bool DENSE>
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided(SweepParams P) {
-    adi_part_strided_body<AX, RC>(P, blockIdx.x);
+    adi_part_strided_body<AX, RC, DENSE>(P, blockIdx.x);
 }
-template <int RC>
-__global__ void __launch_bounds__(PART_THREADS) k_adi_part_x(SweepParams P) { adi_part_x_body<RC>(P, blockIdx.x); }
-template <int AX, int RC>
+template <int RC, bool DENSE>
+__global__ void __launch_bounds__(PART_THREADS) k_adi_part_x(SweepParams P) {
+    adi_part_x_body<RC, DENSE>(P, blockIdx.x);
+}
+template <int AX, int RC, bool DENSE>
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_strided_batch(const SweepParams *__restrict__ Ps,
                                                                                   const int *__restrict__ prefix,
                                                                                   int nbox) {
     int blk;
     const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
-    adi_part_strided_body<AX, RC>(P, blk);
+    adi_part_strided_body<AX, RC, DENSE>(P, blk);
 }
-template <int RC>
+template <int RC, bool DENSE>
 __global__ void __launch_bounds__(PART_THREADS) k_adi_part_x_batch(const SweepParams *__restrict__ Ps,
                                                                            const int *__restrict__ prefix, int nbox) {
     int blk;
     const SweepParams &P = batch_params(Ps, prefix, nbox, blk);
-    adi_part_x_body<RC>(P, blk);
+    adi_part_x_body<RC, DENSE>(P, blk);
 }
 
 
@@ -685,25 +759,32 @@ void set_lds(K kern, size_t lds) {
     if (lds > 65536) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-// one partitioned sweep launch with the kernel for RC rows per lane
-template <int RC>
-void launch_part(int ax, dim3 grid, dim3 blk, size_t lds, hipStream_t st, const SweepParams *P,
-                 const SweepParams *Ps, const int *prefix, int nbox) {
+// one partitioned sweep launch with the kernel for RC rows per lane and
+// the layout (DENSE: every array of the launch dense)
+template <int RC, bool DENSE>
+void launch_part_l(int ax, dim3 grid, dim3 blk, size_t lds, hipStream_t st, const SweepParams *P,
+                   const SweepParams *Ps, const int *prefix, int nbox) {
     if (ax == 0) {
         if (P) {
-            set_lds(k_adi_part_x<RC>, lds);
-            hipLaunchKernelGGL(k_adi_part_x<RC>, grid, blk, lds, st, *P);
+            set_lds(k_adi_part_x<RC, DENSE>, lds);
+            hipLaunchKernelGGL((k_adi_part_x<RC, DENSE>), grid, blk, lds, st, *P);
         } else {
-            set_lds(k_adi_part_x_batch<RC>, lds);
-            hipLaunchKernelGGL(k_adi_part_x_batch<RC>, grid, blk, lds, st, Ps, prefix, nbox);
+            set_lds(k_adi_part_x_batch<RC, DENSE>, lds);
+            hipLaunchKernelGGL((k_adi_part_x_batch<RC, DENSE>), grid, blk, lds, st, Ps, prefix, nbox);
         }
     } else if (ax == 1) {
-        if (P) hipLaunchKernelGGL((k_adi_part_strided<1, RC>), grid, blk, lds, st, *P);
-        else hipLaunchKernelGGL((k_adi_part_strided_batch<1, RC>), grid, blk, lds, st, Ps, prefix, nbox);
+        if (P) hipLaunchKernelGGL((k_adi_part_strided<1, RC, DENSE>), grid, blk, lds, st, *P);
+        else hipLaunchKernelGGL((k_adi_part_strided_batch<1, RC, DENSE>), grid, blk, lds, st, Ps, prefix, nbox);
     } else {
-        if (P) hipLaunchKernelGGL((k_adi_part_strided<2, RC>), grid, blk, lds, st, *P);
-        else hipLaunchKernelGGL((k_adi_part_strided_batch<2, RC>), grid, blk, lds, st, Ps, prefix, nbox);
+        if (P) hipLaunchKernelGGL((k_adi_part_strided<2, RC, DENSE>), grid, blk, lds, st, *P);
+        else hipLaunchKernelGGL((k_adi_part_strided_batch<2, RC, DENSE>), grid, blk, lds, st, Ps, prefix, nbox);
     }
+}
+template <int RC>
+void launch_part(int ax, bool dense, dim3 grid, dim3 blk, size_t lds, hipStream_t st, const SweepParams *P,
+                 const SweepParams *Ps, const int *prefix, int nbox) {
+    if (dense) launch_part_l<RC, true>(ax, grid, blk, lds, st, P, Ps, prefix, nbox);
+    else launch_part_l<RC, false>(ax, grid, blk, lds, st, P, Ps, prefix, nbox);
 }
 
 }  // namespace
@@ -721,6 +802,7 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
     P.ax = dir - 1;
     for (int c = 0; c < 3; c++) P.sys[c] = (c == P.ax) ? sysA : sysB;
     P.a[0] = u; P.a[1] = v; P.a[2] = w;
+    for (int c = 0; c < 3; c++) P.xs[c] = field_origin(g, c);
     const int ax = P.ax;
     if (part && sweep_partitioned(sysA, sysB)) {
         // 64 lines per block (x: P.xl), one wave per partition block
@@ -731,7 +813,9 @@ int make_sweep_params(const Grid &g, int dir, const LineSys &sysA, const LineSys
         for (int c = 0; c < 3; c++) {
             int n[3];
             comp_dims(g, c, n);
-            const size_t m = (ax == 0) ? (size_t)n[1] * n[2] : (size_t)n[0] * ((ax == 1) ? n[2] : n[1]);
+            // y / z: storage slots of a plane of lines (origin offset included)
+            const size_t m = (ax == 0) ? (size_t)n[1] * n[2]
+                                       : (size_t)comp_pitch(g, c) * ((ax == 1) ? n[2] : n[1]) + P.xs[c];
             nl = m > nl ? m : nl;
             lsmax = (n[0] | 1) > lsmax ? (n[0] | 1) : lsmax;
         }
@@ -803,13 +887,13 @@ int sweep_params_waves(const void *params) { return ((const SweepParams *)params
 
 int sweep_params_rows(const void *params) { return ((const SweepParams *)params)->rc; }
 
-void launch_adi_batch(int ax, int kw, int rows, const void *params, const int *prefix, int nbox, int nblk,
-                      size_t lds, hipStream_t st) {
+void launch_adi_batch(int ax, int kw, int rows, bool dense, const void *params, const int *prefix, int nbox,
+                      int nblk, size_t lds, hipStream_t st) {
     const SweepParams *Ps = (const SweepParams *)params;
     if (kw > 0) {
         const dim3 grid(nblk, 3), blk(64 * kw);
-        if (rows > PART_ROWS) launch_part<PART_ROWS_LONG>(ax, grid, blk, lds, st, nullptr, Ps, prefix, nbox);
-        else launch_part<PART_ROWS>(ax, grid, blk, lds, st, nullptr, Ps, prefix, nbox);
+        if (rows > PART_ROWS) launch_part<PART_ROWS_LONG>(ax, dense, grid, blk, lds, st, nullptr, Ps, prefix, nbox);
+        else launch_part<PART_ROWS>(ax, dense, grid, blk, lds, st, nullptr, Ps, prefix, nbox);
         return;
     }
     if (ax == 0) {
@@ -831,8 +915,9 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
     const int nb = make_sweep_params(g, dir, sysA, sysB, u, v, w, &P, &lds, 6, true, planes, L, planes_out);
     if (P.kw > 0) {
         const dim3 grid(nb, 3), blk(64 * P.kw);
-        if (P.rc > PART_ROWS) launch_part<PART_ROWS_LONG>(dir - 1, grid, blk, lds, st, &P, nullptr, nullptr, 0);
-        else launch_part<PART_ROWS>(dir - 1, grid, blk, lds, st, &P, nullptr, nullptr, 0);
+        const bool dense = g.dense();
+        if (P.rc > PART_ROWS) launch_part<PART_ROWS_LONG>(dir - 1, dense, grid, blk, lds, st, &P, nullptr, nullptr, 0);
+        else launch_part<PART_ROWS>(dir - 1, dense, grid, blk, lds, st, &P, nullptr, nullptr, 0);
         return;
     }
     if (dir == 1) {

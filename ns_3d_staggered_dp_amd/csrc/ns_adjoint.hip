@@ -22,6 +22,11 @@
 // adjoint is formed from the output adjoints around it and spread back to the
 // faces it averages, so every adjoint value is gathered (no atomics, the
 // result does not depend on thread order).
+//
+// The adjoint arrays are dense (the reference layout: row pitch = row length),
+// and these kernels index them so; every launcher takes a dense Grid.
+#include <cassert>
+
 #include "ns_internal.h"
 
 namespace ns {
@@ -35,7 +40,7 @@ inline unsigned int ablocks(size_t n) { return (unsigned int)((n + AT - 1) / AT)
 // 1-based read of a ghosted array, 0 outside it
 __device__ __forceinline__ double rd(const double *a, const int n[3], int i, int j, int k) {
     if (i < 1 || i > n[0] || j < 1 || j > n[1] || k < 1 || k > n[2]) return 0.0;
-    return a[FIX(i, j, k, n[0], n[1])];
+    return a[GIX(i, j, k, n[0], n[1])];
 }
 
 // the same without the range test, for points whose every access is in range
@@ -146,7 +151,7 @@ __global__ void __launch_bounds__(AT) k_adj_fill(const AdjFill P) {
     auto at = [&](int pos) -> double & {
         int q[3] = {id[0], id[1], id[2]};
         q[ax] = pos;
-        return a[FIX(q[0], q[1], q[2], n[0], n[1])];
+        return a[GIX(q[0], q[1], q[2], n[0], n[1])];
     };
     const bool pres = comp == 3;
     const long o1p = pres ? P.L.off[18 + 2 * ax] : P.L.off[6 * comp + 2 * ax];
@@ -199,7 +204,7 @@ __global__ void __launch_bounds__(AT) k_adj_take_interior(Grid g, const double *
     const int a = (int)(q % g.nx) + 1;
     const size_t s = q / g.nx;
     const int b = (int)(s % g.ny) + 1, c = (int)(s / g.ny) + 1;
-    r[q] = dpb[FIX(a + 1, b + 1, c + 1, g.nx + 2, g.ny + 2)];
+    r[q] = dpb[GIX(a + 1, b + 1, c + 1, g.nx + 2, g.ny + 2)];
 }
 
 // pr_bc_staggered_modify_rhs transposed (NS_functions.f90:1362-1398): one axis
@@ -227,7 +232,7 @@ __global__ void __launch_bounds__(AT) k_adj_modify_rhs(AdjRhs P) {
     i1[ax] = 1;
     iN[ax] = n[ax];
     const size_t pl = (size_t)pp + (size_t)np_[o1] * (size_t)qq;
-    const double r1 = P.rb[FIX(i1[0], i1[1], i1[2], P.g.nx, P.g.ny)], rn = P.rb[FIX(iN[0], iN[1], iN[2], P.g.nx, P.g.ny)];
+    const double r1 = P.rb[GIX(i1[0], i1[1], i1[2], P.g.nx, P.g.ny)], rn = P.rb[GIX(iN[0], iN[1], iN[2], P.g.nx, P.g.ny)];
     double *b1 = P.pbar + P.L.off[18 + 2 * ax], *bn = P.pbar + P.L.off[19 + 2 * ax];
     const double d = P.d[ax], d2 = d * d;
     if (pbc == 2) { b1[pl] += -2.0 * r1 / d2; bn[pl] += -2.0 * rn / d2; }
@@ -657,6 +662,7 @@ void launch_add_at(double *a, size_t q, double v, hipStream_t st) {
 
 void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag, const double *const ub[3],
                         double *const usb[3], double *pb, double *dpb, hipStream_t st) {
+    assert(g.dense());
     AdjProj P;
     P.g = g;
     P.dt = dt;
@@ -676,6 +682,7 @@ void launch_adj_project(const Grid &g, double dt, const double h[3], int dp_flag
 
 void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3], double *const a[3], double *pbar,
                      hipStream_t st) {
+    assert(g.dense());
     AdjFill P;
     P.g = g;
     P.comp = comp;
@@ -695,11 +702,13 @@ void launch_adj_fill(const Grid &g, int comp, const int bc[3], const double h[3]
 }
 
 void launch_adj_take_interior(const Grid &g, const double *dpb, double *r, hipStream_t st) {
+    assert(g.dense());
     hipLaunchKernelGGL(k_adj_take_interior, dim3(ablocks(g.ni())), dim3(AT), 0, st, g, dpb, r);
 }
 
 void launch_adj_modify_rhs(const Grid &g, const int pbc[3], const double h[3], const double *rb, double *pbar,
                            hipStream_t st) {
+    assert(g.dense());
     AdjRhs R;
     R.g = g;
     R.L = plane_layout(g.nx, g.ny, g.nz);
@@ -716,6 +725,7 @@ void launch_adj_modify_rhs(const Grid &g, const int pbc[3], const double h[3], c
 
 void launch_adj_div(const Grid &g, const double h[3], double dt, const double *rb, double *const usb[3],
                     hipStream_t st) {
+    assert(g.dense());
     const size_t mx = std::max(g.nu(), std::max(g.nv(), g.nw()));
     const double3 r = make_double3(1.0 / (h[0] * dt), 1.0 / (h[1] * dt), 1.0 / (h[2] * dt));
     hipLaunchKernelGGL(k_adj_div, dim3(ablocks(mx), 3), dim3(AT), 0, st, g, r, rb, usb[0], usb[1], usb[2]);
@@ -723,6 +733,7 @@ void launch_adj_div(const Grid &g, const double h[3], double dt, const double *r
 
 void launch_adj_rhs_div(const Grid &g, const int pbc[3], const double h[3], double dt, const double *rb,
                         double *const usb[3], double *pbar, hipStream_t st) {
+    assert(g.dense());
     launch_adj_modify_rhs(g, pbc, h, rb, pbar, st);
     launch_adj_div(g, h, dt, rb, usb, st);
 }
@@ -747,9 +758,9 @@ __global__ void __launch_bounds__(AT) k_adj_cn2(Grid g, int ax, int bc, PlaneLay
     id[o2] = (int)(t / n[o1]) + 1;
     double *A = (c == 0) ? a0 : (c == 1) ? a1 : a2;
     id[ax] = 1;
-    const size_t q1 = FIX(id[0], id[1], id[2], n[0], n[1]);
+    const size_t q1 = GIX(id[0], id[1], id[2], n[0], n[1]);
     id[ax] = n[ax];
-    const size_t qn = FIX(id[0], id[1], id[2], n[0], n[1]);
+    const size_t qn = GIX(id[0], id[1], id[2], n[0], n[1]);
     if (bc == 1) {
         // the forward zeroes the ghost rows (row N; row 1 too for the tangential components)
         if (c != ax) A[q1] = 0.0;
@@ -765,6 +776,7 @@ __global__ void __launch_bounds__(AT) k_adj_cn2(Grid g, int ax, int bc, PlaneLay
 }  // namespace
 
 void launch_adj_cn2(const Grid &g, int ax, int bc, double *const a[3], double *pbar, hipStream_t st) {
+    assert(g.dense());
     if (bc != 1 && bc != 2 && bc != 4) return;  // code 3: the forward leaves the lines as they are
     size_t mx = 0;
     for (int c = 0; c < 3; c++) {
@@ -782,6 +794,7 @@ void launch_adj_predict(const Grid &g, const int per[3], double dt, double nu, i
                         int scheme, int first, const double *const u[3], const double *const usb[3],
                         double *const hb[3], double *const gb[3], double *const gcb[3], double *const ub[3],
                         double *pb, hipStream_t st) {
+    assert(g.dense());
     AdjPred P;
     P.g = g;
     for (int a = 0; a < 3; a++) {

@@ -6,19 +6,79 @@
 
 #include "ns_hip.h"
 
-// 1-based Fortran index into an (n1,n2,*) column-major array
-#define FIX(i, j, k, n1, n2) ((size_t)((i) - 1) + (size_t)(n1) * ((size_t)((j) - 1) + (size_t)(n2) * (size_t)((k) - 1)))
+// 1-based Fortran index into a column-major array whose x rows are px
+// elements apart and whose planes hold n2 rows (px = n1 for a dense (n1,n2,*)
+// array)
+#define GIX(i, j, k, px, n2) ((size_t)((i) - 1) + (size_t)(px) * ((size_t)((j) - 1) + (size_t)(n2) * (size_t)((k) - 1)))
 
 namespace ns {
 
+// Box dimensions and the memory layout of its ghosted arrays.  pu is the x
+// row pitch (in elements) of the u-shaped arrays (nx+1 entries per row), pc
+// that of v, w, p (nx+2 entries); planes hold exactly ny+2 (v: ny+1) rows, so
+// an array spans pitch * rows * planes elements from its origin.  The
+// reference layout has pu = nx+1, pc = nx+2 (dense arrays, as a Fortran host
+// holds them); a context keeps its own state in the aligned layout of
+// dev_grid().
 struct Grid {
     int nx, ny, nz;
-    __host__ __device__ size_t nu() const { return (size_t)(nx + 1) * (ny + 2) * (nz + 2); }
-    __host__ __device__ size_t nv() const { return (size_t)(nx + 2) * (ny + 1) * (nz + 2); }
-    __host__ __device__ size_t nw() const { return (size_t)(nx + 2) * (ny + 2) * (nz + 1); }
-    __host__ __device__ size_t np() const { return (size_t)(nx + 2) * (ny + 2) * (nz + 2); }
+    int pu, pc;
+    Grid() = default;
+    __host__ __device__ Grid(int x, int y, int z, int pu_, int pc_) : nx(x), ny(y), nz(z), pu(pu_), pc(pc_) {}
+    // elements from an array's origin to the end of its last row
+    __host__ __device__ size_t nu() const { return (size_t)pu * (ny + 2) * (nz + 2); }
+    __host__ __device__ size_t nv() const { return (size_t)pc * (ny + 1) * (nz + 2); }
+    __host__ __device__ size_t nw() const { return (size_t)pc * (ny + 2) * (nz + 1); }
+    __host__ __device__ size_t np() const { return (size_t)pc * (ny + 2) * (nz + 2); }
     __host__ __device__ size_t ni() const { return (size_t)nx * ny * nz; }
+    // entries of the reference (dense) layout
+    __host__ __device__ size_t ref_nu() const { return (size_t)(nx + 1) * (ny + 2) * (nz + 2); }
+    __host__ __device__ size_t ref_nv() const { return (size_t)(nx + 2) * (ny + 1) * (nz + 2); }
+    __host__ __device__ size_t ref_nw() const { return (size_t)(nx + 2) * (ny + 2) * (nz + 1); }
+    __host__ __device__ size_t ref_np() const { return (size_t)(nx + 2) * (ny + 2) * (nz + 2); }
+    __host__ __device__ bool dense() const { return pu == nx + 1 && pc == nx + 2; }
 };
+
+// row pitch of a component's arrays: c = 0 u, 1 v, 2 w, 3 p
+__host__ __device__ inline int comp_pitch(const Grid &g, int c) { return c == 0 ? g.pu : g.pc; }
+
+// The reference layout (dense Fortran arrays).
+inline Grid ref_grid(int nx, int ny, int nz) { return Grid(nx, ny, nz, nx + 1, nx + 2); }
+
+// A context's own layout.  Row pitches are a whole number of 128-byte lines
+// (an odd number, so that the rows of a plane spread over the memory
+// channels), and the array origins are placed so that the first computed
+// point of every row is the first element of a line: u(1) (u is computed
+// from face 1 for periodic x, face 2 for walls) and v(2), w(2), p(2).  A
+// wavefront's stores of 64 consecutive points of a row then cover whole
+// lines; on MI355X a stream whose stores straddle line boundaries runs about
+// 40% slower (3.7 against 5.3 TB/s for 7 reads and 6 writes per point).
+// The spare elements at the row ends are never read as data.
+constexpr int ROW_ALIGN = 16;  // doubles per 128-byte line
+inline int ghost_pitch(int nx) {
+    int lines = (nx + 2 + ROW_ALIGN - 1) / ROW_ALIGN;
+    if ((lines & 1) == 0) lines++;
+    return lines * ROW_ALIGN;
+}
+// The aligned layout costs pitch - (nx+2) spare elements per row.  Boxes
+// whose rows it would lengthen by more than 1/16 keep the dense layout:
+// there the spares are a large share of every sweep's lanes and staged rows
+// (128 wide: 11%, 32 wide: 41%), and such boxes' arrays largely stay in the
+// 256 MB Infinity Cache, where partial-line stores cost little (measured at
+// 128^3: walls AB2-CN 0.276 ms dense, 0.300 ms aligned; at 256^3 the aligned
+// periodic AB2 step is 4% faster).
+inline Grid dev_grid(int nx, int ny, int nz) {
+    const int p = ghost_pitch(nx);
+    if (16 * (p - (nx + 2)) > nx + 2) return ref_grid(nx, ny, nz);
+    return Grid(nx, ny, nz, p, p);
+}
+// offset (elements) of a component's array origin from its line-aligned allocation
+inline int field_origin(const Grid &g, int c) { return (g.dense() || c == 0) ? 0 : ROW_ALIGN - 1; }
+// elements to allocate for a component's array (origin offset included)
+inline size_t field_alloc(const Grid &g, int c) {
+    const size_t n = c == 0 ? g.nu() : c == 1 ? g.nv() : c == 2 ? g.nw() : g.np();
+    return n + (g.dense() ? 0 : ROW_ALIGN);
+}
 
 // x / d for x < 2^32 and 1 <= d < 2^20, with rd = 1.0/d: the double product
 // is within 2^-20 of x/d, so truncation gives the quotient or, when d divides
@@ -39,9 +99,11 @@ __device__ __forceinline__ int xcd_slab_block(int b, int nb) { return (b & 7) * 
 inline unsigned int round8(unsigned int nb) { return (nb + 7u) & ~7u; }
 
 // component dims: c=0 u, 1 v, 2 w, 3 p
+// (written without a run-time array index, which would place n in scratch memory)
 __host__ __device__ inline void comp_dims(const Grid &g, int c, int n[3]) {
-    n[0] = g.nx + 2; n[1] = g.ny + 2; n[2] = g.nz + 2;
-    if (c < 3) n[c] -= 1;
+    n[0] = g.nx + 2 - (c == 0);
+    n[1] = g.ny + 2 - (c == 1);
+    n[2] = g.nz + 2 - (c == 2);
 }
 
 // packed plane-set layout (18 velocity + 6 pressure planes + mean_dp)
@@ -108,7 +170,7 @@ struct Ctx;
 // projection with dp (and the corner gauge, if non-null) is applied on load;
 // wrap (all-periodic boxes): the velocity's ghost entries are read through
 // their periodic images, so they need not have been filled
-void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+void launch_predict_tiled(const ns_config &c, const Grid &g, int first, bool lazy, const double *u, const double *v, const double *w,
                           const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
                           double *us, double *vs, double *ws, hipStream_t st, bool wrap = false);
 
@@ -139,7 +201,7 @@ __device__ __forceinline__ const T &batch_params(const T *params, const int *pre
 // predictor parameters of one box written to out (predict_params_size() bytes); returns its
 // block count.  zc_hint: z cells per block (0: chosen for the box alone on the device)
 size_t predict_params_size();
-int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+int make_predict_params(const ns_config &c, const Grid &g, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
                         double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap = false,
                         int tile = 0);
@@ -185,8 +247,9 @@ int sweep_params_waves(const void *params);
 // rows per lane of a partitioned sweep's kernel (PART_ROWS or PART_ROWS_LONG)
 int sweep_params_rows(const void *params);
 // kw: waves per block of the partitioned solve (every box partitioned), 0: twisted / cyclic path
-void launch_adi_batch(int ax, int kw, int rows, const void *params, const int *prefix, int nbox, int nblk, size_t lds,
-                      hipStream_t st);
+// dense: every box of the launch has the dense layout (the kernels for it)
+void launch_adi_batch(int ax, int kw, int rows, bool dense, const void *params, const int *prefix, int nbox, int nblk,
+                      size_t lds, hipStream_t st);
 struct PoissonPlans;
 size_t pass_params_size();
 // lsh_drop: lines per block reduced by 2^lsh_drop from the default (a launch-shape choice only)
@@ -230,6 +293,10 @@ void launch_axpby(const double *a, const double *b, double th, double *out, long
 void launch_wsum(const double *a, long stride, const double *w, int m, double *out, long n, hipStream_t st);
 void launch_copy_interior(const Grid &g, const double *sol, double *dp, hipStream_t st);
 void launch_sub_gauge(const double *src, const double *gauge, double *dst, size_t n, hipStream_t st);
+// rows of n0 entries from src (row pitch ps) to dst (row pitch pd), minus *gauge
+// if gauge is set; rows * n0 < 2^32
+void launch_relayout(const double *src, int ps, double *dst, int pd, int n0, size_t rows, const double *gauge,
+                     hipStream_t st);
 // box of extent e at 0-based origin o of a ghosted array with leading dims n[0], n[1]
 void launch_extract_box(const double *src, const int n[3], const int o[3], const int e[3], const double *gauge,
                         double *dst, hipStream_t st);
@@ -272,6 +339,7 @@ void poisson_dct_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
 // ---- re-simulation error metrics (ns_metrics.hip) ----
 // partials per block: for u, v, w, p: max|a-b|, sum|a-b|, sum b^2; then max|oz-oz_b|, sum oz_b^2
 #define NS_METRIC_PARTS 14
+// a (the run's u, v, w, p) in the layout of g, b (the parent's) dense
 int launch_err_partials(const Grid &g, double dx, double dy, const double *const a[4], const double *const b[4],
                         double *partial, int max_blocks, hipStream_t st);
 // rows of block partials held by the static scratch of ns_error_metrics_noalloc

@@ -24,11 +24,12 @@ namespace {
 constexpr int MT = 256;
 
 struct MetricParams {
-    Grid g;
+    Grid g;              // layout of a (the run's arrays); b is dense
     double dx, dy;
     const double *a[4];  // u, v, w, p of the run
     const double *b[4];  // u_sub, v_sub, w_sub, p_sub of the parent
-    size_t n[4], noz, total;
+    size_t n[4], noz, total;  // entries of each array, of oz, and the most of them
+    int n0[4], pa[4];    // row length and a's row pitch of each component
     double *partial;     // [block][NS_METRIC_PARTS]
 };
 
@@ -43,7 +44,9 @@ __global__ void __launch_bounds__(MT) k_err_partials(MetricParams P) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             if (e < P.n[c]) {
-                const double x = P.a[c][e], y = P.b[c][e];
+                // entry e of the dense layout; in a, its row r = e / n0 starts pa * r
+                const size_t r = e / (size_t)P.n0[c];
+                const double x = P.a[c][e + r * (size_t)(P.pa[c] - P.n0[c])], y = P.b[c][e];
                 const double d = fabs(x - y);
                 acc[3 * c] = fmax(acc[3 * c], d);
                 acc[3 * c + 1] += d;
@@ -54,9 +57,11 @@ __global__ void __launch_bounds__(MT) k_err_partials(MetricParams P) {
             const int i = (int)(e % O1) + 1;
             const int j = (int)((e / O1) % O2) + 1;
             const int k = (int)(e / ((size_t)O1 * O2)) + 1;
-            const size_t v0 = FIX(i, j, k, V1, V2), v1 = FIX(i + 1, j, k, V1, V2);
-            const size_t u0 = FIX(i, j, k, U1, U2), u1 = FIX(i, j + 1, k, U1, U2);
-            const double oz = (P.a[1][v1] - P.a[1][v0]) / P.dx - (P.a[0][u1] - P.a[0][u0]) / P.dy;
+            const size_t v0 = GIX(i, j, k, V1, V2), v1 = GIX(i + 1, j, k, V1, V2);
+            const size_t u0 = GIX(i, j, k, U1, U2), u1 = GIX(i, j + 1, k, U1, U2);
+            const size_t av0 = GIX(i, j, k, P.pa[1], V2), av1 = GIX(i + 1, j, k, P.pa[1], V2);
+            const size_t au0 = GIX(i, j, k, P.pa[0], U2), au1 = GIX(i, j + 1, k, P.pa[0], U2);
+            const double oz = (P.a[1][av1] - P.a[1][av0]) / P.dx - (P.a[0][au1] - P.a[0][au0]) / P.dy;
             const double ozs = (P.b[1][v1] - P.b[1][v0]) / P.dx - (P.b[0][u1] - P.b[0][u0]) / P.dy;
             acc[12] = fmax(acc[12], fabs(oz - ozs));
             acc[13] += ozs * ozs;
@@ -86,12 +91,14 @@ int launch_err_partials(const Grid &g, double dx, double dy, const double *const
     P.g = g;
     P.dx = dx;
     P.dy = dy;
-    const size_t n[4] = {g.nu(), g.nv(), g.nw(), g.np()};
+    const size_t n[4] = {g.ref_nu(), g.ref_nv(), g.ref_nw(), g.ref_np()};
     P.total = 0;
     for (int c = 0; c < 4; c++) {
         P.a[c] = a[c];
         P.b[c] = b[c];
         P.n[c] = n[c];
+        P.n0[c] = (c == 0) ? g.nx + 1 : g.nx + 2;
+        P.pa[c] = comp_pitch(g, c);
         P.total = n[c] > P.total ? n[c] : P.total;
     }
     P.noz = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 2);

@@ -261,16 +261,16 @@ __device__ __forceinline__ double qdiv(double a, double d, double r) {
 __device__ __forceinline__ double div_rhs(const FFTPass &P, int a, int b, int c) {
     const Grid &g = P.g;
     if (P.rhs) return P.rhs[(size_t)a + (size_t)g.nx * ((size_t)b + (size_t)g.ny * c)];
-    const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+    const int U1 = g.pu, U2 = g.ny + 2, V1 = g.pc, V2 = g.ny + 1, W1 = g.pc, W2 = g.ny + 2;
     const int A = a + 1, B = b + 1, C = c + 1;
     // the high face of each component's own axis: its periodic copy of face 1
     // when every velocity axis is periodic (the only ghosts read here)
     const int iu = (P.wrap && A == g.nx) ? 1 : A + 1;
     const int jv = (P.wrap && B == g.ny) ? 1 : B + 1;
     const int kw = (P.wrap && C == g.nz) ? 1 : C + 1;
-    double s = qdiv(P.us[FIX(iu, B + 1, C + 1, U1, U2)] - P.us[FIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
-    s = s + qdiv(P.vs[FIX(A + 1, jv, C + 1, V1, V2)] - P.vs[FIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
-    s = s + qdiv(P.ws[FIX(A + 1, B + 1, kw, W1, W2)] - P.ws[FIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
+    double s = qdiv(P.us[GIX(iu, B + 1, C + 1, U1, U2)] - P.us[GIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
+    s = s + qdiv(P.vs[GIX(A + 1, jv, C + 1, V1, V2)] - P.vs[GIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
+    s = s + qdiv(P.ws[GIX(A + 1, B + 1, kw, W1, W2)] - P.ws[GIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
     return qdiv(s, P.dt, P.rdt);
 }
 
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
         const double val = ((rline & 1) ? z.y : z.x) * sc;
         const int k = idiv(rl, P.dNy), j = rl - k * g.ny;
         if (P.dp_out) {
-            const size_t q = FIX(x + 2, j + 2, k + 2, g.nx + 2, g.ny + 2);
+            const size_t q = GIX(x + 2, j + 2, k + 2, g.pc, g.ny + 2);
             P.dp_out[q] = val;
             // x part of pr_bc_staggered (pbc 1): dp(1) = dp(nx+1), dp(nx+2) = dp(2)
             if (P.dp_ghost_x) {
@@ -513,11 +513,11 @@ __device__ __forceinline__ double wall_rhs(const PassParams &P, int a, int b, in
     if (P.rhs) {
         s = P.rhs[(size_t)a + (size_t)g.nx * ((size_t)b + (size_t)g.ny * c)];
     } else {
-        const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+        const int U1 = g.pu, U2 = g.ny + 2, V1 = g.pc, V2 = g.ny + 1, W1 = g.pc, W2 = g.ny + 2;
         const int A = a + 1, B = b + 1, C = c + 1;
-        s = qdiv(P.us[FIX(A + 1, B + 1, C + 1, U1, U2)] - P.us[FIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
-        s = s + qdiv(P.vs[FIX(A + 1, B + 1, C + 1, V1, V2)] - P.vs[FIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
-        s = s + qdiv(P.ws[FIX(A + 1, B + 1, C + 1, W1, W2)] - P.ws[FIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
+        s = qdiv(P.us[GIX(A + 1, B + 1, C + 1, U1, U2)] - P.us[GIX(A, B + 1, C + 1, U1, U2)], P.dx, P.rdx);
+        s = s + qdiv(P.vs[GIX(A + 1, B + 1, C + 1, V1, V2)] - P.vs[GIX(A + 1, B, C + 1, V1, V2)], P.dy, P.rdy);
+        s = s + qdiv(P.ws[GIX(A + 1, B + 1, C + 1, W1, W2)] - P.ws[GIX(A + 1, B + 1, C, W1, W2)], P.dz, P.rdz);
         s = qdiv(s, P.dt, P.rdt);
     }
     if (P.planes) {
@@ -846,7 +846,7 @@ __device__ __forceinline__ void dct_inv_body(const PassParams &P, const int blk)
             double val = Ad[2 * ((rline >> 1) * P.LS + mk_pos(pos, N)) + (rline & 1)] * inv;
             if (dst && (pos & 1) == 0) val = -val;
             const int k = idiv(rl, P.dNy), j = rl - k * g.ny;
-            const size_t q = FIX(pos + 2, j + 2, k + 2, g.nx + 2, g.ny + 2);
+            const size_t q = GIX(pos + 2, j + 2, k + 2, g.pc, g.ny + 2);
             if (P.dp_out) P.dp_out[q] = val;
             // p = 1*p + dp, the projection's pressure update, where dp is formed
             if (P.p_acc) P.p_acc[q] = P.p_acc[q] + val;

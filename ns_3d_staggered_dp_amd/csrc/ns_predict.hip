@@ -62,6 +62,7 @@ struct Shape {
 
 struct TileParams {
     int nx, ny, nz;
+    int pu, pc;          // row pitches of the u-shaped arrays and of v, w, p
     double dx, dy, dz, nu, dt;
     double rd[3], rd2[3];  // 1/dx, 1/dy, 1/dz and 1/dx^2, ... rounded on the host
     int scheme, first, dp_flag;
@@ -88,6 +89,7 @@ __device__ __forceinline__ void cdims(const TileParams &P, int c, int n[3]) {
     n[0] = P.nx + 2; n[1] = P.ny + 2; n[2] = P.nz + 2;
     n[c] -= 1;
 }
+__device__ __forceinline__ int cpitch(const TileParams &P, int c) { return c == 0 ? P.pu : P.pc; }
 
 // global 1-based index of interior cell a (0-based) along axis ax for component c
 __device__ __forceinline__ int gidx(int c, int ax, int a) { return a + ((ax == c) ? 1 : 2); }
@@ -119,12 +121,12 @@ __device__ __forceinline__ double load_val(const TileParams &P, int c, int gi, i
         vj = pwrap(vj, n[1], c == 1);
         vk = pwrap(vk, n[2], c == 2);
     }
-    double x = P.src[c][FIX(vi, vj, vk, n[0], n[1])];
+    double x = P.src[c][GIX(vi, vj, vk, cpitch(P, c), n[1])];
     if (LAZY) {
-        const int P1 = P.nx + 2, P2 = P.ny + 2;
+        const int P1 = P.pc, P2 = P.ny + 2;
         int b[3] = {gi, gj, gk};
         b[c]++;
-        double hi = P.dp[FIX(b[0], b[1], b[2], P1, P2)], lo = P.dp[FIX(gi, gj, gk, P1, P2)];
+        double hi = P.dp[GIX(b[0], b[1], b[2], P1, P2)], lo = P.dp[GIX(gi, gj, gk, P1, P2)];
         if (P.gauge) {
             const double g = *P.gauge;
             hi = hi - g;
@@ -167,7 +169,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
     const int z1 = min(z0 + P.zc, P.nz);
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int P1 = P.nx + 2, P2 = P.ny + 2;
-    const size_t pp = (size_t)P1 * P2;
+    const size_t pp = (size_t)P.pc * P2;
     // tile origins (global index of local 0) per component; the dp tile starts at (a0, b0)
     const int ox[3] = {a0, a0 + 1, a0 + 1}, oy[3] = {b0 + 1, b0, b0 + 1};
     int nn[3][3];
@@ -203,7 +205,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             }
             gi = clampi(gi, 1, nn[0][0]);
             gj = clampi(gj, 1, nn[0][1]);
-            off[0][s_] = (unsigned int)((gi - 1) + nn[0][0] * (gj - 1));
+            off[0][s_] = (unsigned int)((gi - 1) + P.pu * (gj - 1));
             lds[0][s_] = ok ? (short)(lb * UW + la) : (short)-1;
             // dp(nx+1) == dp(1) and dp(nx+2) == dp(2) after the periodic pressure fill
             const int dx0 = (la == HX) ? 1 : la;
@@ -223,7 +225,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             }
             gi = clampi(gi, 1, nn[1][0]);
             gj = clampi(gj, 1, nn[1][1]);
-            off[1][s_] = (unsigned int)((gi - 1) + nn[1][0] * (gj - 1));
+            off[1][s_] = (unsigned int)((gi - 1) + P.pc * (gj - 1));
             lds[1][s_] = ok ? (short)(lb * HX + la) : (short)-1;
             const int dy0 = (lb == HY) ? 1 : lb;
             plo[1][s_] = (short)(dy0 * DXT + la + 1);
@@ -241,7 +243,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             }
             gi = clampi(gi, 1, nn[2][0]);
             gj = clampi(gj, 1, nn[2][1]);
-            off[2][s_] = (unsigned int)((gi - 1) + nn[2][0] * (gj - 1));
+            off[2][s_] = (unsigned int)((gi - 1) + P.pc * (gj - 1));
             lds[2][s_] = ok ? (short)(lb * HX + la) : (short)-1;
             plo[2][s_] = (short)((lb + 1) * DXT + la + 1);
             phi[2][s_] = plo[2][s_];
@@ -256,11 +258,11 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
             if (gj == 0 && P.per[1]) gj = P.ny;
             gi = clampi(gi, 1, P1);
             gj = clampi(gj, 1, P2);
-            offd[s_] = (unsigned int)((gi - 1) + P1 * (gj - 1));
+            offd[s_] = (unsigned int)((gi - 1) + P.pc * (gj - 1));
             ldsd[s_] = ok ? (short)e : (short)-1;
         }
     }
-    const size_t ps[3] = {(size_t)nn[0][0] * nn[0][1], (size_t)nn[1][0] * nn[1][1], (size_t)nn[2][0] * nn[2][1]};
+    const size_t ps[3] = {(size_t)P.pu * nn[0][1], (size_t)P.pc * nn[1][1], (size_t)P.pc * nn[2][1]};
     const double gauge = (LAZY && P.gauge) ? *P.gauge : 0.0;
     // z index of a velocity slice (0 -> nz for periodic w) and of a dp slice
     auto kvel = [&](int c, int s) {
@@ -335,7 +337,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
     for (int c = 0; c < 3; c++) {
         const int own = (c == 0) ? a : bb;
         on[c] = act && (c == 2 || P.per[c] || own != 0);
-        q0[c] = FIX(gidx(c, 0, ac), gidx(c, 1, bc), gidx(c, 2, 0), nn[c][0], nn[c][1]);
+        q0[c] = GIX(gidx(c, 0, ac), gidx(c, 1, bc), gidx(c, 2, 0), cpitch(P, c), nn[c][1]);
     }
     // the history of cells this thread does not compute is loaded and never
     // used (nor is it used by the first / Euler steps), so the load needs no
@@ -469,7 +471,7 @@ __device__ __forceinline__ void predict_tile(const TileParams P, const int b) {
                     int bq[3] = {gi, gj, gk};
                     bq[c]++;
                     const double dd = (c == 0) ? dx : (c == 1) ? dy : dz;
-                    dpd = qdiv(P.p[FIX(bq[0], bq[1], bq[2], P1, P2)] - P.p[FIX(gi, gj, gk, P1, P2)], dd, P.rd[c]);
+                    dpd = qdiv(P.p[GIX(bq[0], bq[1], bq[2], P.pc, P2)] - P.p[GIX(gi, gj, gk, P.pc, P2)], dd, P.rd[c]);
                 }
                 double *H = P.h[c];
                 double rhs, val;
@@ -589,11 +591,12 @@ long predict_tile_columns(const ns_config &c, int tile) {
 
 int predict_resident_blocks(bool lazy, int tile) { return resident_blocks(lazy, tile == 32 ? 32 : 64); }
 
-int make_predict_params(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+int make_predict_params(const ns_config &c, const Grid &g, int first, bool lazy, const double *u, const double *v, const double *w,
                         const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
                         double *us, double *vs, double *ws, void *out, int zc_hint, bool wrap, int tile) {
     TileParams &P = *(TileParams *)out;
     P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
+    P.pu = g.pu; P.pc = g.pc;
     P.dx = c.dx; P.dy = c.dy; P.dz = c.dz; P.nu = c.nu; P.dt = c.dt;
     const double d[3] = {c.dx, c.dy, c.dz};
     for (int a = 0; a < 3; a++) {
@@ -638,11 +641,11 @@ void launch_predict_batch(bool lazy, int tile, const void *params, const int *pr
         hipLaunchKernelGGL((k_predict_tiled_batch<false, 64>), dim3(nblocks), dim3(NT), 0, st, Ps, prefix, nbox);
 }
 
-void launch_predict_tiled(const ns_config &c, int first, bool lazy, const double *u, const double *v, const double *w,
+void launch_predict_tiled(const ns_config &c, const Grid &g, int first, bool lazy, const double *u, const double *v, const double *w,
                           const double *dp, const double *gauge, const double *p, double *hx, double *hy, double *hz,
                           double *us, double *vs, double *ws, hipStream_t st, bool wrap) {
     TileParams P;
-    const int nblk = make_predict_params(c, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P, 0, wrap);
+    const int nblk = make_predict_params(c, g, first, lazy, u, v, w, dp, gauge, p, hx, hy, hz, us, vs, ws, &P, 0, wrap);
     if (lazy && P.tx == 32) hipLaunchKernelGGL((k_predict_tiled<true, 32>), dim3(nblk), dim3(NT), 0, st, P);
     else if (lazy) hipLaunchKernelGGL((k_predict_tiled<true, 64>), dim3(nblk), dim3(NT), 0, st, P);
     else if (P.tx == 32) hipLaunchKernelGGL((k_predict_tiled<false, 32>), dim3(nblk), dim3(NT), 0, st, P);

@@ -16,11 +16,12 @@ namespace {
 struct FieldsC {
     const double *u, *v, *w;
     int nx, ny, nz;
+    int pu, pc;  // row pitches of u and of v, w
 };
 
-#define UU(i, j, k) f.u[FIX(i, j, k, f.nx + 1, f.ny + 2)]
-#define VV(i, j, k) f.v[FIX(i, j, k, f.nx + 2, f.ny + 1)]
-#define WW(i, j, k) f.w[FIX(i, j, k, f.nx + 2, f.ny + 2)]
+#define UU(i, j, k) f.u[GIX(i, j, k, f.pu, f.ny + 2)]
+#define VV(i, j, k) f.v[GIX(i, j, k, f.pc, f.ny + 1)]
+#define WW(i, j, k) f.w[GIX(i, j, k, f.pc, f.ny + 2)]
 
 // edge fluxes shared by two components: E = avg_y(u)*avg_x(v) on xy-edges,
 // F = avg_z(u)*avg_x(w) on xz-edges, G = avg_z(v)*avg_y(w) on yz-edges
@@ -123,6 +124,7 @@ __device__ __forceinline__ bool in_range(int c, const int bc[3], int nx, int ny,
 
 struct CDParams {
     int nx, ny, nz;
+    int pu, pc;
     double h0, h1, h2;
     int bc[3];
     int which;  // 0 conv, 1 diff
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(256) k_conv_diff(CDParams P) {
     int cc = (int)(r / m[1]) + 1;
     int id[3] = {a + 1, b + 1, cc + 1};
     id[c] = (P.bc[c] == 1) ? id[c] - 1 : id[c];  // own axis: periodic 1..n, walls 2..n
-    FieldsC f{P.u, P.v, P.w, P.nx, P.ny, P.nz};
+    FieldsC f{P.u, P.v, P.w, P.nx, P.ny, P.nz, P.pu, P.pc};
     double cv, df;
     // conv mirror gets (dx,dy,dz); diff mirror gets (dx2,dy2,dz2) as cal_diff does
     conv_diff(f, c, P.bc, id[0], id[1], id[2], P.h0, P.h1, P.h2, P.h0, P.h1, P.h2, cv, df);
@@ -153,6 +155,7 @@ __global__ void __launch_bounds__(256) k_conv_diff(CDParams P) {
 
 struct BCParams {
     int nx, ny, nz;
+    int pu, pc;    // row pitches of u and of v, w, p
     int ax;        // axis of a one-axis launch; first filled axis of a whole fill
     int bc;
     int pbc_mode;  // 0 velocity fill, 1 CN2 rows, 2 pressure fill (one axis); 3 velocity, 4 pressure (whole fill)
@@ -232,6 +235,7 @@ __device__ __forceinline__ void whole_fill_body(const BCParams &P, const size_t 
     // every array and parameter index below is a constant (a run-time index
     // would copy the parameters to scratch memory)
     const int n0 = P.nx + 2 - (!pres && c == 0), n1 = P.ny + 2 - (!pres && c == 1), n2 = P.nz + 2 - (!pres && c == 2);
+    const int px = (!pres && c == 0) ? P.pu : P.pc;
     const Entry e = surface_entry(t, n0, n1, n2);
     if (!e.ok) return;
     const int i = e.i, j = e.j, k = e.k;
@@ -256,8 +260,8 @@ __device__ __forceinline__ void whole_fill_body(const BCParams &P, const size_t 
         if (asg(0, i_))
             return fill_rule(pres, c == 0, P.bcs[0], i_ == 1, n0, P.ds[0],
                              plane(0, i_ == 1, (size_t)(j_ - 1) + (size_t)(k_ - 1) * n1),
-                             [&](int m) { return A[FIX(m, j_, k_, n0, n1)]; });
-        return A[FIX(i_, j_, k_, n0, n1)];
+                             [&](int m) { return A[GIX(m, j_, k_, px, n1)]; });
+        return A[GIX(i_, j_, k_, px, n1)];
     };
     auto vy = [&](int i_, int j_, int k_) -> double {
         if (asg(1, j_))
@@ -272,11 +276,11 @@ __device__ __forceinline__ void whole_fill_body(const BCParams &P, const size_t 
                       plane(2, k == 1, (size_t)(i - 1) + (size_t)(j - 1) * n0), [&](int m) { return vy(i, j, m); });
     else
         v = vy(i, j, k);
-    A[FIX(i, j, k, n0, n1)] = v;
+    A[GIX(i, j, k, px, n1)] = v;
     // pressure fill with an accumulation target (the lazily projected
     // dp_flag = 1 step): p = 1*p + dp at the ghost entry, as the projection
     // would form it
-    if (pres && P.a[1]) P.a[1][FIX(i, j, k, n0, n1)] = P.a[1][FIX(i, j, k, n0, n1)] + v;
+    if (pres && P.a[1]) P.a[1][GIX(i, j, k, px, n1)] = P.a[1][GIX(i, j, k, px, n1)] + v;
 }
 
 __global__ void __launch_bounds__(256) k_whole_fill(BCParams P) {
@@ -297,12 +301,13 @@ __device__ __forceinline__ void other_axes(int ax, int &o1, int &o2) {
 
 // boundary rows before an ADI sweep (vel_bc_staggered_CN2, NS_functions.f90:313-410)
 __device__ __forceinline__ void cn2_body(const BCParams &P, const size_t t) {
-    Grid g{P.nx, P.ny, P.nz};
+    const Grid g(P.nx, P.ny, P.nz, P.pu, P.pc);
     int o1, o2;
     other_axes(P.ax, o1, o2);
     for (int c = 0; c < 3; c++) {
         int n[3];
         comp_dims(g, c, n);
+        const int px = comp_pitch(g, c);
         size_t tot = (size_t)n[o1] * n[o2];
         if (t >= tot) continue;
         int id[3];
@@ -311,13 +316,13 @@ __device__ __forceinline__ void cn2_body(const BCParams &P, const size_t t) {
         double *A = P.a[c];
         const int N = n[P.ax];
         if (P.bc == 1) {
-            if (c != P.ax) { id[P.ax] = 1; A[FIX(id[0], id[1], id[2], n[0], n[1])] = 0.0; }
-            id[P.ax] = N; A[FIX(id[0], id[1], id[2], n[0], n[1])] = 0.0;
+            if (c != P.ax) { id[P.ax] = 1; A[GIX(id[0], id[1], id[2], px, n[1])] = 0.0; }
+            id[P.ax] = N; A[GIX(id[0], id[1], id[2], px, n[1])] = 0.0;
         } else if (P.bc == 2 || P.bc == 4) {
             const double *b1 = P.planes + P.L.off[6 * c + 2 * P.ax];
             const double *bn = P.planes + P.L.off[6 * c + 2 * P.ax + 1];
-            id[P.ax] = 1; A[FIX(id[0], id[1], id[2], n[0], n[1])] = b1[t];
-            id[P.ax] = N; A[FIX(id[0], id[1], id[2], n[0], n[1])] = bn[t];
+            id[P.ax] = 1; A[GIX(id[0], id[1], id[2], px, n[1])] = b1[t];
+            id[P.ax] = N; A[GIX(id[0], id[1], id[2], px, n[1])] = bn[t];
         }
     }
 }
@@ -348,8 +353,8 @@ __global__ void __launch_bounds__(256) k_modify_rhs_axis(BCParams P) {
     iN[P.ax] = n[P.ax];
     size_t pl = (size_t)pp + (size_t)np_[o1] * (size_t)qq;  // interior (2:np-1) of the full plane
     const double *b1 = P.planes + P.L.off[18 + 2 * P.ax], *bn = P.planes + P.L.off[19 + 2 * P.ax];
-    double *r1 = &P.a[0][FIX(i1[0], i1[1], i1[2], P.nx, P.ny)];
-    double *rn = &P.a[0][FIX(iN[0], iN[1], iN[2], P.nx, P.ny)];
+    double *r1 = &P.a[0][GIX(i1[0], i1[1], i1[2], P.nx, P.ny)];
+    double *rn = &P.a[0][GIX(iN[0], iN[1], iN[2], P.nx, P.ny)];
     const double d = P.d, d2 = P.d * P.d;
     if (P.bc == 2) { *r1 = *r1 - 2 * b1[pl] / d2; *rn = *rn - 2 * bn[pl] / d2; }
     else if (P.bc == 3) { *r1 = *r1 + b1[pl] / d; *rn = *rn - bn[pl] / d; }
@@ -364,10 +369,10 @@ __global__ void __launch_bounds__(256) k_divergence(Grid g, double dx, double dy
     int a = (int)(q % g.nx) + 1;
     size_t r = q / g.nx;
     int b = (int)(r % g.ny) + 1, c = (int)(r / g.ny) + 1;
-    const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
-    double s = (us[FIX(a + 1, b + 1, c + 1, U1, U2)] - us[FIX(a, b + 1, c + 1, U1, U2)]) / dx;
-    s = s + (vs[FIX(a + 1, b + 1, c + 1, V1, V2)] - vs[FIX(a + 1, b, c + 1, V1, V2)]) / dy;
-    s = s + (ws[FIX(a + 1, b + 1, c + 1, W1, W2)] - ws[FIX(a + 1, b + 1, c, W1, W2)]) / dz;
+    const int U1 = g.pu, U2 = g.ny + 2, V1 = g.pc, V2 = g.ny + 1, W1 = g.pc, W2 = g.ny + 2;
+    double s = (us[GIX(a + 1, b + 1, c + 1, U1, U2)] - us[GIX(a, b + 1, c + 1, U1, U2)]) / dx;
+    s = s + (vs[GIX(a + 1, b + 1, c + 1, V1, V2)] - vs[GIX(a + 1, b, c + 1, V1, V2)]) / dy;
+    s = s + (ws[GIX(a + 1, b + 1, c + 1, W1, W2)] - ws[GIX(a + 1, b + 1, c, W1, W2)]) / dz;
     rhs[q] = s / dt;
 }
 
@@ -378,31 +383,39 @@ struct ProjParams {
     const double *s[3];
     const double *dp, *gauge;
     double *o[4];
-    double r0[3], r1[3];  // 1/n0 and 1/n1 of the velocity components' dims
+    double rp[4], r1[4];  // 1/row pitch and 1/rows per plane of each component's arrays
+    int org[4];           // offset of each component's array origin from its line-aligned start
 };
 
-// one element q of component c (c = 3: p) of the projection
-// (big_periodic_3D.f90:562-567): u = u* - dt*(dp(i+1)-dp(i))/dx, p = dp_flag*p + dp
-__device__ __forceinline__ void project_body(const ProjParams &P, const size_t q, const int c) {
+// storage slot t of component c (c = 3: p) of the projection
+// (big_periodic_3D.f90:562-567): u = u* - dt*(dp(i+1)-dp(i))/dx, p = dp_flag*p + dp.
+// Slot t is element t - org[c] of the arrays, so the 64 slots of a wavefront
+// are one or more whole 128-byte lines; slots before the origin and the spare
+// elements at the row ends are skipped.
+__device__ __forceinline__ void project_body(const ProjParams &P, const size_t t, const int c) {
     int n[3];
     comp_dims(P.g, c, n);
-    const size_t tot = (size_t)n[0] * n[1] * n[2];
-    if (q >= tot) return;
+    const int px = comp_pitch(P.g, c);
+    const long qs = (long)t - P.org[c];
+    if (qs < 0 || (size_t)qs >= (size_t)px * n[1] * n[2]) return;
+    // arrays hold fewer than 2^32 elements (checked at context creation)
+    const unsigned qq = (unsigned)qs;
+    const unsigned r = udiv_rd(qq, (unsigned)px, P.rp[c]);
+    const int i = (int)(qq - r * (unsigned)px) + 1;
+    if (i > n[0]) return;
+    const size_t q = qq;
     const double gc = P.gauge ? *P.gauge : 0.0;
     if (c == 3) {
         double dpv = P.gauge ? P.dp[q] - gc : P.dp[q];
         P.o[3][q] = (double)P.dp_flag * P.o[3][q] + dpv;
         return;
     }
-    // arrays hold fewer than 2^32 elements (checked at context creation)
-    const unsigned qq = (unsigned)q;
-    const unsigned r = udiv_rd(qq, (unsigned)n[0], P.r0[c]);
     const unsigned kk = udiv_rd(r, (unsigned)n[1], P.r1[c]);
-    const int i = (int)(qq - r * (unsigned)n[0]) + 1, j = (int)(r - kk * (unsigned)n[1]) + 1, k = (int)kk + 1;
+    const int j = (int)(r - kk * (unsigned)n[1]) + 1, k = (int)kk + 1;
     int b[3] = {i, j, k};
     b[c]++;
-    const int P1 = P.g.nx + 2, P2 = P.g.ny + 2;
-    double hi = P.dp[FIX(b[0], b[1], b[2], P1, P2)], lo = P.dp[FIX(i, j, k, P1, P2)];
+    const int P1 = P.g.pc, P2 = P.g.ny + 2;
+    double hi = P.dp[GIX(b[0], b[1], b[2], P1, P2)], lo = P.dp[GIX(i, j, k, P1, P2)];
     if (P.gauge) { hi = hi - gc; lo = lo - gc; }
     double gr = hi - lo;
     P.o[c][q] = P.s[c][q] - P.dt * gr / P.d[c];
@@ -429,6 +442,7 @@ __global__ void __launch_bounds__(256) k_project_batch(const ProjParams *__restr
 
 struct GetBCParams {
     int nx, ny, nz;
+    int pu, pc;  // row pitches of u and of v, w, p
     int ax, bc, corr;
     double d[3], dt;
     PlaneLayout L;
@@ -439,14 +453,15 @@ struct GetBCParams {
 
 // velocity planes from a (parent) field (get_velpr_bc, NS_functions.f90:446-503)
 __global__ void __launch_bounds__(256) k_get_vel_bc_axis(GetBCParams P) {
-    Grid g{P.nx, P.ny, P.nz};
+    const Grid g(P.nx, P.ny, P.nz, P.pu, P.pc);
     int o1, o2;
     other_axes(P.ax, o1, o2);
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int P1 = P.nx + 2, P2 = P.ny + 2;
+    const int P1 = P.pc, P2 = P.ny + 2;
     for (int c = 0; c < 3; c++) {
         int n[3];
         comp_dims(g, c, n);
+        const int px = comp_pitch(g, c);
         if (t >= (size_t)n[o1] * n[o2]) continue;
         int id[3];
         id[o1] = (int)(t % n[o1]) + 1;
@@ -455,11 +470,11 @@ __global__ void __launch_bounds__(256) k_get_vel_bc_axis(GetBCParams P) {
         auto val = [&](int pos) -> double {
             int a[3] = {id[0], id[1], id[2]};
             a[P.ax] = pos;
-            double x = P.src[c][FIX(a[0], a[1], a[2], n[0], n[1])];
+            double x = P.src[c][GIX(a[0], a[1], a[2], px, n[1])];
             if (P.corr) {
                 int b[3] = {a[0], a[1], a[2]};
                 b[c]++;
-                double gr = P.pprev[FIX(b[0], b[1], b[2], P1, P2)] - P.pprev[FIX(a[0], a[1], a[2], P1, P2)];
+                double gr = P.pprev[GIX(b[0], b[1], b[2], P1, P2)] - P.pprev[GIX(a[0], a[1], a[2], P1, P2)];
                 x = x + gr / P.d[c] * P.dt;
             }
             return x;
@@ -485,7 +500,7 @@ __global__ void __launch_bounds__(256) k_get_pr_bc_axis(GetBCParams P) {
     auto val = [&](int pos) -> double {
         int a[3] = {id[0], id[1], id[2]};
         a[P.ax] = pos;
-        return p[FIX(a[0], a[1], a[2], n[0], n[1])];
+        return p[GIX(a[0], a[1], a[2], P.pc, n[1])];
     };
     double *b1 = P.planes + P.L.off[18 + 2 * P.ax], *bn = P.planes + P.L.off[19 + 2 * P.ax];
     double d = P.d[P.ax];
@@ -522,7 +537,21 @@ __global__ void __launch_bounds__(256) k_copy_interior(Grid g, const double *__r
     int a = (int)(q % g.nx) + 1;
     size_t r = q / g.nx;
     int b = (int)(r % g.ny) + 1, c = (int)(r / g.ny) + 1;
-    dp[FIX(a + 1, b + 1, c + 1, g.nx + 2, g.ny + 2)] = sol[q];
+    dp[GIX(a + 1, b + 1, c + 1, g.pc, g.ny + 2)] = sol[q];
+}
+
+// dst row r = src row r [- *gauge], n0 entries per row, rows ps (src) and pd
+// (dst) elements apart: an array between the reference and a context's layout
+__global__ void __launch_bounds__(256) k_relayout(const double *__restrict__ src, int ps, double *__restrict__ dst,
+                                                  int pd, int n0, unsigned tot, double rn0, const double *gauge) {
+    const double gc = gauge ? *gauge : 0.0;
+    for (unsigned e = blockIdx.x * 256u + threadIdx.x; e < tot; e += gridDim.x * 256u) {
+        const unsigned r = udiv_rd(e, (unsigned)n0, rn0);
+        const unsigned i = e - r * (unsigned)n0;
+        double x = src[(size_t)r * ps + i];
+        if (gauge) x = x - gc;
+        dst[(size_t)r * pd + i] = x;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_sub_gauge(const double *__restrict__ src, const double *gauge,
@@ -532,8 +561,8 @@ __global__ void __launch_bounds__(256) k_sub_gauge(const double *__restrict__ sr
 }
 
 // dst(e0,e1,e2) = src(o0+1.., o1+1.., o2+1..) [- *gauge]: a box of a ghosted
-// (n0,n1,*) array, 0-based origin o
-__global__ void __launch_bounds__(256) k_extract_box(const double *__restrict__ src, int n0, int n1, int o0, int o1,
+// array of row pitch px and n1 rows per plane, 0-based origin o
+__global__ void __launch_bounds__(256) k_extract_box(const double *__restrict__ src, int px, int n1, int o0, int o1,
                                                      int o2, int e0, int e1, int e2, const double *gauge,
                                                      double *__restrict__ dst) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -542,7 +571,7 @@ __global__ void __launch_bounds__(256) k_extract_box(const double *__restrict__ 
     const int i = (int)(q % e0);
     const size_t r = q / e0;
     const int j = (int)(r % e1), k = (int)(r / e1);
-    double x = src[FIX(o0 + i + 1, o1 + j + 1, o2 + k + 1, n0, n1)];
+    double x = src[GIX(o0 + i + 1, o1 + j + 1, o2 + k + 1, px, n1)];
     if (gauge) x = x - *gauge;
     dst[q] = x;
 }
@@ -552,14 +581,14 @@ __global__ void __launch_bounds__(256) k_max_div(Grid g, double dx, double dy, d
                                                  double *partial) {
     __shared__ double red[256];
     double m = 0.0;
-    const int U1 = g.nx + 1, U2 = g.ny + 2, V1 = g.nx + 2, V2 = g.ny + 1, W1 = g.nx + 2, W2 = g.ny + 2;
+    const int U1 = g.pu, U2 = g.ny + 2, V1 = g.pc, V2 = g.ny + 1, W1 = g.pc, W2 = g.ny + 2;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < g.ni(); q += (size_t)gridDim.x * blockDim.x) {
         int a = (int)(q % g.nx) + 1;
         size_t r = q / g.nx;
         int b = (int)(r % g.ny) + 1, c = (int)(r / g.ny) + 1;
-        double s = (u[FIX(a + 1, b + 1, c + 1, U1, U2)] - u[FIX(a, b + 1, c + 1, U1, U2)]) / dx;
-        s = s + (v[FIX(a + 1, b + 1, c + 1, V1, V2)] - v[FIX(a + 1, b, c + 1, V1, V2)]) / dy;
-        s = s + (w[FIX(a + 1, b + 1, c + 1, W1, W2)] - w[FIX(a + 1, b + 1, c, W1, W2)]) / dz;
+        double s = (u[GIX(a + 1, b + 1, c + 1, U1, U2)] - u[GIX(a, b + 1, c + 1, U1, U2)]) / dx;
+        s = s + (v[GIX(a + 1, b + 1, c + 1, V1, V2)] - v[GIX(a + 1, b, c + 1, V1, V2)]) / dy;
+        s = s + (w[GIX(a + 1, b + 1, c + 1, W1, W2)] - w[GIX(a + 1, b + 1, c, W1, W2)]) / dz;
         m = fmax(m, fabs(s));
     }
     red[threadIdx.x] = m;
@@ -594,6 +623,8 @@ struct CapParams {
     double *planes;
 };
 
+__device__ __forceinline__ int sel3(const int a[3], int k) { return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]); }
+
 __global__ void __launch_bounds__(256) k_capture_axis(CapParams P) {
     int o1, o2;
     other_axes(P.ax, o1, o2);
@@ -604,17 +635,21 @@ __global__ void __launch_bounds__(256) k_capture_axis(CapParams P) {
         int ns_[3], np_[3];
         comp_dims(P.gs, cc, ns_);
         comp_dims(P.gp, cc, np_);
-        if (t >= (size_t)ns_[o1] * ns_[o2]) continue;
-        int id[3];
-        id[o1] = (int)(t % ns_[o1]);
-        id[o2] = (int)(t / ns_[o1]);
-        const int N = ns_[P.ax];
+        const int ppx = comp_pitch(P.gp, cc);
+        // every small array is read through selects (a run-time index would
+        // place it in scratch memory)
+        const int n1 = sel3(ns_, o1), n2 = sel3(ns_, o2);
+        if (t >= (size_t)n1 * n2) continue;
+        const int p1 = (int)(t % n1), p2 = (int)(t / n1);  // 0-based along o1, o2
+        const int N = sel3(ns_, P.ax);
         const double *A = P.src[c];
         const double gc = P.gauge ? *P.gauge : 0.0;
         auto val = [&](int pos) -> double {  // pos is 1-based along ax in the sub-box
-            int a[3] = {id[0] + P.o[0], id[1] + P.o[1], id[2] + P.o[2]};
-            a[P.ax] = pos - 1 + P.o[P.ax];
-            double x = A[(size_t)a[0] + (size_t)np_[0] * ((size_t)a[1] + (size_t)np_[1] * (size_t)a[2])];
+            const int ax = P.ax, q = pos - 1;
+            const int a0 = (ax == 0 ? q : (o1 == 0 ? p1 : p2)) + P.o[0];
+            const int a1 = (ax == 1 ? q : (o1 == 1 ? p1 : p2)) + P.o[1];
+            const int a2 = (ax == 2 ? q : (o1 == 2 ? p1 : p2)) + P.o[2];
+            double x = A[(size_t)a0 + (size_t)ppx * ((size_t)a1 + (size_t)np_[1] * (size_t)a2)];
             return P.gauge ? x - gc : x;
         };
         if (P.what == 0) {
@@ -636,7 +671,7 @@ __global__ void __launch_bounds__(256) k_subbox_partial(CapParams P, double *par
     const size_t n = P.gs.ni();
     const double gc = P.gauge ? *P.gauge : 0.0;
     double s = 0.0;
-    const int P1 = P.gp.nx + 2, P2 = P.gp.ny + 2;
+    const int P1 = P.gp.pc, P2 = P.gp.ny + 2;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         int a = (int)(q % P.gs.nx);
         size_t r = q / P.gs.nx;
@@ -686,6 +721,7 @@ void launch_conv_diff(const Grid &g, const int bc[3], double h0, double h1, doub
                       const double *v, const double *w, double *ox, double *oy, double *oz, hipStream_t st) {
     CDParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.pu = g.pu; P.pc = g.pc;
     P.h0 = h0; P.h1 = h1; P.h2 = h2;
     for (int a = 0; a < 3; a++) P.bc[a] = bc[a];
     P.which = which_diff;
@@ -747,7 +783,8 @@ __global__ void __launch_bounds__(256) k_periodic_fill(const PerFill P) {
     }
     if (src[0] == id[0] && src[1] == id[1] && src[2] == id[2]) return;
     double *A = P.a[k];
-    A[FIX(id[0], id[1], id[2], n[0], n[1])] = A[FIX(src[0], src[1], src[2], n[0], n[1])];
+    const int px = comp_pitch(P.g, c < 3 ? c : 3);
+    A[GIX(id[0], id[1], id[2], px, n[1])] = A[GIX(src[0], src[1], src[2], px, n[1])];
 }
 
 void launch_periodic_fill(const Grid &g, int na, const int comp[3], double *const a[3], int first_axis,
@@ -776,6 +813,7 @@ int make_fill_params(const Grid &g, bool pres, const int bc[3], const double *pl
     BCParams &P = *(BCParams *)out;
     P = BCParams{};
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.pu = g.pu; P.pc = g.pc;
     P.L = plane_layout(g.nx, g.ny, g.nz);
     P.planes = planes;
     P.a[0] = a0; P.a[1] = a1; P.a[2] = a2;
@@ -820,6 +858,7 @@ void launch_vel_bc(const Grid &g, const int bc[3], const double *planes, double 
 int make_cn2_params(const Grid &g, int bc, int ax, const double *planes, double *u, double *v, double *w, void *out) {
     BCParams &P = *(BCParams *)out;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.pu = g.pu; P.pc = g.pc;
     P.L = plane_layout(g.nx, g.ny, g.nz);
     P.planes = planes;
     P.a[0] = u; P.a[1] = v; P.a[2] = w;
@@ -859,6 +898,7 @@ void launch_modify_rhs(const Grid &g, const int pbc[3], const double *planes, do
                        double *rhs, hipStream_t st) {
     BCParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.pu = g.pu; P.pc = g.pc;
     P.L = plane_layout(g.nx, g.ny, g.nz);
     P.planes = planes;
     P.a[0] = rhs; P.a[1] = nullptr; P.a[2] = nullptr;
@@ -894,6 +934,7 @@ void launch_get_velpr_bc(const Grid &g, const int bc[3], const int pbc[3], doubl
                          const double *pprev, double *planes, hipStream_t st) {
     GetBCParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.pu = g.pu; P.pc = g.pc;
     P.d[0] = dx; P.d[1] = dy; P.d[2] = dz;
     P.dt = dt;
     P.corr = corr && (bc[0] != 1 || bc[1] != 1 || bc[2] != 1);
@@ -914,6 +955,7 @@ void launch_get_pr_bc(const Grid &g, const int pbc[3], double dx, double dy, dou
                       double *planes, hipStream_t st) {
     GetBCParams P;
     P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+    P.pu = g.pu; P.pc = g.pc;
     P.d[0] = dx; P.d[1] = dy; P.d[2] = dz;
     P.dt = 0;
     P.corr = 0;
@@ -955,6 +997,16 @@ void launch_extract_box(const double *src, const int n[3], const int o[3], const
     const size_t tot = (size_t)e[0] * e[1] * e[2];
     hipLaunchKernelGGL(k_extract_box, dim3(nblocks(tot)), dim3(256), 0, st, src, n[0], n[1], o[0], o[1], o[2], e[0],
                        e[1], e[2], gauge, dst);
+}
+
+void launch_relayout(const double *src, int ps, double *dst, int pd, int n0, size_t rows, const double *gauge,
+                     hipStream_t st) {
+    const size_t tot = (size_t)n0 * rows;
+    size_t nb = nblocks(tot);
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_relayout, dim3((unsigned)nb), dim3(256), 0, st, src, ps, dst, pd, n0, (unsigned)tot,
+                       1.0 / n0, gauge);
 }
 
 void launch_sub_gauge(const double *src, const double *gauge, double *dst, size_t n, hipStream_t st) {
@@ -1029,13 +1081,15 @@ int make_project_params(const Grid &g, const double d[3], double dt, int dp_flag
     P.dp = dp;
     P.gauge = gauge;
     P.o[0] = u; P.o[1] = v; P.o[2] = w; P.o[3] = p;
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < 4; c++) {
         int n[3];
         comp_dims(g, c, n);
-        P.r0[c] = 1.0 / n[0];
+        P.rp[c] = 1.0 / comp_pitch(g, c);
         P.r1[c] = 1.0 / n[1];
+        P.org[c] = field_origin(g, c);
     }
-    return (int)round8(nblocks(g.np(), 256 * PROJ_PER));
+    // storage slots of the largest array, its origin offset included
+    return (int)round8(nblocks(g.np() + (size_t)field_origin(g, 3), 256 * PROJ_PER));
 }
 
 void launch_project_batch(const void *params, const int *prefix, int nbox, int nblk, hipStream_t st) {

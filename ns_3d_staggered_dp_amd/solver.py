@@ -114,6 +114,13 @@ class NSSolver:
         keys = ["u", "v", "w", "p", "hx", "hy", "hz", "u_star", "v_star", "w_star", "dp"]
         return {k: arr[i] for i, k in enumerate(keys)}
 
+    def state_pitch(self):
+        """Row pitch (elements) of the arrays state_pointers() points at: entry
+        (i, j, k) of an array with y extent n2 is at (i-1) + pitch*((j-1) + n2*(k-1))."""
+        out = ctypes.c_int(0)
+        _abi.call("ns_state_pitch", self.h, ctypes.byref(out))
+        return out.value
+
     def synchronize(self):
         _abi.call("ns_synchronize", self.h)
 

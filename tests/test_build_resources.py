@@ -25,6 +25,6 @@ def test_partitioned_sweeps_fit_five_waves():
     if not res:
         pytest.skip("no resource reports: library not built here")
     part = {k: v for k, v in res.items() if "k_adi_part" in k}
-    assert len(part) == 12
+    assert len(part) == 24  # x, y, z x single / batch x 28 / 40 rows x dense / aligned layout
     for k, v in part.items():
         assert v["occupancy"] >= (4 if "Li40E" in k else 5), (k, v)
