@@ -216,7 +216,11 @@ int ns_adjoint_step(ns_ctx *ctx, const double *u, const double *v, const double 
  * dp_flag and flags and use the DCT Poisson solver (walls); sizes may
  * differ.  Lazily projected boxes step lazily in the batch too; where the
  * boxes differ in projection mode, the lazy ones form u, v, w, p when they
- * join and project every step from then on.  Each context keeps its own state and current plane set;
+ * join and project every step from then on.  A batch of 16 or more boxes
+ * deals them into groups of about equal cell counts (n/8 groups, at most 4),
+ * each with its own stream, tables and graphs, and queues every group's steps
+ * before waiting for all, so the device runs the groups' kernels side by
+ * side.  Each context keeps its own state and current plane set;
  * ns_batch_step(b, n) leaves every context as ns_step(ctx, n) would.  A batch
  * holds its contexts' device pointers and time steps: destroy it before any
  * of them, and re-create it after ns_set_dt on any of them. */

@@ -120,10 +120,20 @@ def test_box_sizes_are_transform_sizes():
     assert box_sizes(64, seed=4) == sizes
 
 
-def test_tuned_batch_keeps_state_and_equals_individual_steps():
+def test_batch_groups_equal_individual_steps():
+    """24 boxes: the batch deals them into three groups of about equal cell
+    counts, each stepped on its own stream; every box still steps bitwise as
+    on its own, fresh (Euler first step) and restarted boxes alike"""
+    sizes = box_sizes(24, 8, 36, seed=11)
+    check_batch([make(*sz, AB2_CN) for sz in sizes])
+
+
+@pytest.mark.parametrize("sizes", [[(12, 10, 8), (70, 64, 40), (40, 36, 48), (96, 90, 126)],  # three size classes
+                                   box_sizes(16, 10, 40, seed=9)],  # 16 boxes: two groups on their own streams
+                         ids=["classes", "two_groups"])
+def test_tuned_batch_keeps_state_and_equals_individual_steps(sizes):
     """ns_batch_tune times launch shapes on the boxes' own state and restores
     it; the tuned batch still steps bitwise like the boxes on their own."""
-    sizes = [(12, 10, 8), (70, 64, 40), (40, 36, 48), (96, 90, 126)]  # small, medium, large classes
     cfgs = [make(*sz, AB2_CN) for sz in sizes]
     data = [inputs(c, 80 + i) for i, c in enumerate(cfgs)]
     ref = []
