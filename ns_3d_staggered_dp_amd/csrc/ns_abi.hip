@@ -2586,10 +2586,10 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
             release();
             return NS_ERR_HIP;
         }
-    // ms of trial_steps steady steps of every group at once with the current
-    // knobs (best of two runs), from an event on group 0's stream that the
-    // others wait for to one after all of them
-    auto trial = [&](float &ms) -> int {
+    // ms of nst steady steps of every group at once with the current knobs
+    // (best of reps runs), from an event on group 0's stream that the others
+    // wait for to one after all of them
+    auto trial = [&](float &ms, int nst, int reps) -> int {
         std::vector<std::vector<ns_batch::Phase>> ph(2 * ng);
         int rc = NS_OK;
         for (int gi = 0; gi < ng && !rc; gi++) {
@@ -2615,11 +2615,11 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
             }
         };
         ms = 1e30f;
-        for (int rep = 0; rep < 2 && !rc; rep++) {
+        for (int rep = 0; rep < reps && !rc; rep++) {
             step_all();  // warm-up of this shape
             (void)hipEventRecord(ev[0], G[0]->st);
             for (int gi = 1; gi < ng; gi++) (void)hipStreamWaitEvent(G[gi]->st, ev[0], 0);
-            for (int s2 = 0; s2 < trial_steps; s2++) step_all();
+            for (int s2 = 0; s2 < nst; s2++) step_all();
             for (int gi = 1; gi < ng; gi++) {
                 (void)hipEventRecord(ev[1 + gi], G[gi]->st);
                 (void)hipStreamWaitEvent(G[0]->st, ev[1 + gi], 0);
@@ -2635,8 +2635,9 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
         (void)hipGetLastError();
         return rc;
     };
+    const ns_batch::Knobs start[3] = {b->knobs[0], b->knobs[1], b->knobs[2]};
     float best = 0.0f;
-    int rc = trial(best);
+    int rc = trial(best, trial_steps, 2);
     bool present[3] = {false, false, false};
     for (int k : b->cls) present[k] = true;
     // coordinate descent over (size class, knob); a change is kept when it
@@ -2663,7 +2664,7 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
                 if (cand[i] == keep) continue;
                 *val = cand[i];
                 float t = 0.0f;
-                rc = trial(t);
+                rc = trial(t, trial_steps, 2);
                 if (!rc && t < 0.99f * best) {
                     best = t;
                     best_v = cand[i];
@@ -2671,6 +2672,22 @@ int ns_batch_tune(ns_batch *b, int trial_steps, int *knobs_out) {
             }
             *val = best_v;
         }
+    }
+    // the short trials are noisier than the gains they look for: a choice
+    // that differs from the starting shapes is kept only if longer runs of
+    // both (twice the steps, best of three) still find it more than 1% faster
+    bool changed = false;
+    for (int k = 0; k < 3; k++)
+        changed = changed || b->knobs[k].rounds != start[k].rounds || b->knobs[k].dct_drop != start[k].dct_drop ||
+                  b->knobs[k].adix_lsh != start[k].adix_lsh || b->knobs[k].tile != start[k].tile;
+    if (!rc && changed) {
+        const ns_batch::Knobs chosen[3] = {b->knobs[0], b->knobs[1], b->knobs[2]};
+        float t_new = 0.0f, t_old = 0.0f;
+        rc = trial(t_new, 2 * trial_steps, 3);
+        for (int k = 0; k < 3; k++) b->knobs[k] = start[k];
+        if (!rc) rc = trial(t_old, 2 * trial_steps, 3);
+        if (!rc && t_new < 0.99f * t_old)
+            for (int k = 0; k < 3; k++) b->knobs[k] = chosen[k];
     }
     for (auto &x : ev) (void)hipEventDestroy(x);
     release();

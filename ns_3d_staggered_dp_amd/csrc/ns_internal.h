@@ -61,15 +61,15 @@ inline int ghost_pitch(int nx) {
     return lines * ROW_ALIGN;
 }
 // The aligned layout costs pitch - (nx+2) spare elements per row.  Boxes
-// whose rows it would lengthen by more than 1/16 keep the dense layout:
-// there the spares are a large share of every sweep's lanes and staged rows
-// (128 wide: 11%, 32 wide: 41%), and such boxes' arrays largely stay in the
-// 256 MB Infinity Cache, where partial-line stores cost little (measured at
-// 128^3: walls AB2-CN 0.276 ms dense, 0.300 ms aligned; at 256^3 the aligned
-// periodic AB2 step is 4% faster).
+// narrower than 222 cells, or whose rows it would lengthen by more than 1/16,
+// keep the dense layout: there the spares are a large share of every sweep's
+// lanes and staged rows (128 wide: 11%, 32 wide: 41%), and such boxes'
+// arrays largely stay in the 256 MB Infinity Cache, where partial-line stores
+// cost little (measured at 128^3: walls AB2-CN 0.276 ms dense, 0.300 ms
+// aligned; at 256^3 the aligned periodic AB2 step is 2-4% faster).
 inline Grid dev_grid(int nx, int ny, int nz) {
     const int p = ghost_pitch(nx);
-    if (16 * (p - (nx + 2)) > nx + 2) return ref_grid(nx, ny, nz);
+    if (nx < 222 || 16 * (p - (nx + 2)) > nx + 2) return ref_grid(nx, ny, nz);
     return Grid(nx, ny, nz, p, p);
 }
 // offset (elements) of a component's array origin from its line-aligned allocation
