@@ -115,6 +115,41 @@ template <int R> struct SmallDFTOdd {
         v[0] = x0;
     }
 };
+// v *= W16^m = exp(s*2*pi*i*m/16), m a compile-time constant after unrolling
+__device__ __forceinline__ double2 w16mul(double2 a, int m, int s) {
+    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    switch (m) {
+        case 0: return a;
+        case 1: return cmul(a, make_double2(c1, s * s1));
+        case 2: return cmul(a, make_double2(h, s * h));
+        case 3: return cmul(a, make_double2(s1, s * c1));
+        case 4: return cmuli(a, s);
+        case 6: return cmul(a, make_double2(-h, s * h));
+        default: return cmul(a, make_double2(-c1, -s * s1));  // m = 9
+    }
+}
+// 16 = 4 x 4: inner DFTs over r2 (r = r1 + 4*r2), twiddles W16^(r1*k2),
+// outer DFTs over r1 give X[k2 + 4*k1]
+template <> struct SmallDFT<16> {
+    __device__ static void run(double2 *v, int s) {
+        double2 t[4][4];
+#pragma unroll
+        for (int r1 = 0; r1 < 4; r1++) {
+#pragma unroll
+            for (int r2 = 0; r2 < 4; r2++) t[r1][r2] = v[r1 + 4 * r2];
+            SmallDFT<4>::run(t[r1], s);
+#pragma unroll
+            for (int k2 = 1; k2 < 4; k2++) t[r1][k2] = w16mul(t[r1][k2], r1 * k2, s);
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) {
+            double2 u[4] = {t[0][k2], t[1][k2], t[2][k2], t[3][k2]};
+            SmallDFT<4>::run(u, s);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) v[k2 + 4 * k1] = u[k1];
+        }
+    }
+};
 template <> struct SmallDFT<3> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<3>::run(v, s); } };
 template <> struct SmallDFT<5> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<5>::run(v, s); } };
 template <> struct SmallDFT<7> { __device__ static void run(double2 *v, int s) { SmallDFTOdd<7>::run(v, s); } };
@@ -392,6 +427,120 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_yz(FFTPass P) {
         const int l = f & (L - 1), pos = f >> lsh;
         const int kx = kx0 + l;
         if (kx < nxh) P.spec[spec_index<AXIS>(g, nxhp, kx, pos, other)] = A[f];
+    }
+}
+
+// y/z lines of square length N = R*R as a two-step (four-step without the
+// transposes) transform held in registers: with n = n1 + R*n2 and
+// k = k2 + R*k1,
+//   X[k2 + R*k1] = sum_n1 W_R^(n1*k1) * W_N^(n1*k2) * sum_n2 x[n1 + R*n2] W_R^(n2*k2).
+// Thread (l, q) of a block of L lines (l fastest, so a wavefront's loads
+// cover L consecutive wavenumbers) loads x[q + R*n2] straight from memory
+// (q = n1), runs the inner R-point DFT, twiddles, and passes its column
+// through LDS once; the thread then holding k2 = q runs the outer DFT and
+// owns X[q + R*k1].  The inverse mirrors the steps, so a z visit with the
+// modal division (MODE 2) ends with each thread storing exactly the
+// positions it loaded, through two LDS exchanges and two barriers (the
+// Stockham passes above make a round trip through LDS per radix stage).
+// MODE 0: forward; 1: inverse; 2: forward, division, inverse.
+template <int AXIS, int MODE, int R, int L>
+__global__ void __launch_bounds__(R * L) k_fft_sq(FFTPass P) {
+    // LDS: element (k2, n1, line) at k2*RS + n1*L + line; rows padded for
+    // L < 8 so the 8 lanes of one 128-byte LDS cycle fall in distinct banks
+    constexpr int RS = R * L + (L < 8 ? L : 0);
+    __shared__ double2 S[R * RS];
+    const Grid g = P.g;
+    const int nxh = P.nxh, nxhp = P.nxhp;
+    constexpr int G = (L < 8) ? 8 / L : 1;
+    const int ngrp = ((((nxh + L - 1) / L) + G - 1) / G) * G;
+    int blk = (int)blockIdx.x;
+    if (G > 1) {
+        // blocks sharing a 128-byte line of a spectrum row run on one XCD
+        const int sl = blk >> 3, xcd = blk & 7;
+        blk = ((sl / G) * 8 + xcd) * G + (sl % G);
+    }
+    const int other = blk / ngrp;
+    if (other >= ((AXIS == 1) ? g.nz : g.ny)) return;  // padding of the grid (uniform per block)
+    const int l = threadIdx.x & (L - 1), q = threadIdx.x / L;
+    const int kx = (blk - other * ngrp) * L + l;
+    const bool act = kx < nxh;
+    const int kxc = act ? kx : nxh - 1;  // idle lanes read a valid address
+    // 32-bit element offsets: the spectrum of a box with a 64- or 256-long
+    // y or z axis has fewer than 2^31 elements (every axis is at most 2048)
+    const unsigned sp = (AXIS == 1) ? (unsigned)nxhp : (unsigned)g.ny * (unsigned)nxhp;  // line stride
+    const unsigned step = sp * R;                                                         // n2 stride
+    const unsigned b0 = (AXIS == 1) ? (unsigned)other * (unsigned)g.ny * (unsigned)nxhp : (unsigned)other * (unsigned)nxhp;
+    const unsigned bl = b0 + (unsigned)q * sp + (unsigned)kxc;  // element (kxc, pos = q)
+    const double2 *__restrict__ tw = P.pl.tw;
+    // z visit: the eigenvalue parts staged in LDS before the data loads are
+    // issued (visible after the first barrier)
+    __shared__ double Ez[MODE == 2 ? R * R : 1];
+    double exy = 0.0;
+    if (MODE == 2) {
+        for (int f = threadIdx.x; f < R * R; f += R * L) Ez[f] = P.ez[f];
+        exy = P.ex[kxc] + P.ey[other];
+    }
+    double2 v[R];
+    if (MODE != 1) {
+#pragma unroll
+        for (int n2 = 0; n2 < R; n2++) v[n2] = P.spec[bl + n2 * step];
+        SmallDFT<R>::run(v, -1);
+#pragma unroll
+        for (int k2 = 1; k2 < R; k2++) v[k2] = cmul(v[k2], tw[q * k2]);
+#pragma unroll
+        for (int k2 = 0; k2 < R; k2++) S[k2 * RS + q * L + l] = v[k2];
+        __syncthreads();
+#pragma unroll
+        for (int n1 = 0; n1 < R; n1++) v[n1] = S[q * RS + n1 * L + l];
+        SmallDFT<R>::run(v, -1);
+        // v[k1] = X[q + R*k1]
+        if (MODE == 0) {
+            if (act) {
+                // the store addresses are formed here, not kept live from the loads
+                unsigned bs = bl;
+                asm volatile("" : "+v"(bs));
+#pragma unroll
+                for (int k1 = 0; k1 < R; k1++) P.spec[bs + k1 * step] = v[k1];
+            }
+            return;
+        }
+        // modal division (z pass): eig = -4*((ex + ey) + ez); mode (0,0,0) -> 0
+#pragma unroll
+        for (int k1 = 0; k1 < R; k1++) {
+            const int kz = q + R * k1;
+            double e = exy + Ez[kz];
+            e = -4.0 * e;
+            // branch-free: mode (0,0,0) has e = 0 and is replaced by zero
+            const double r = 1.0 / e;
+            const bool zero = (kx | other | kz) == 0;
+            v[k1] = make_double2(zero ? 0.0 : v[k1].x * r, zero ? 0.0 : v[k1].y * r);
+        }
+    } else {
+#pragma unroll
+        for (int k1 = 0; k1 < R; k1++) v[k1] = P.spec[bl + k1 * step];
+    }
+    // inverse: the thread holds X[q + R*k1]; inner DFT over k1, twiddle
+    // conj(W_N^(n1*q)), exchange, outer DFT over k2 gives x[q + R*n2].  The
+    // LDS slots written here are the ones this thread read above, so no
+    // barrier is needed before the writes.
+    SmallDFT<R>::run(v, +1);
+#pragma unroll
+    for (int n1 = 1; n1 < R; n1++) {
+        const double2 w = tw[n1 * q];
+        v[n1] = cmul(v[n1], make_double2(w.x, -w.y));
+    }
+#pragma unroll
+    for (int n1 = 0; n1 < R; n1++) S[q * RS + n1 * L + l] = v[n1];
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) v[k2] = S[k2 * RS + q * L + l];
+    SmallDFT<R>::run(v, +1);
+    if (act) {
+        // the store addresses are formed here, not kept live from the loads
+        unsigned bs = bl;
+        asm volatile("" : "+v"(bs));
+#pragma unroll
+        for (int n2 = 0; n2 < R; n2++) P.spec[bs + n2 * step] = v[n2];
     }
 }
 
@@ -916,6 +1065,23 @@ int fft_row(int n) { return n + ((2 - n) % 16 + 16) % 16; }
 
 IDiv make_idiv(int d) { return IDiv{d, 1.0f / (float)d}; }
 
+// square line lengths with a register-resident two-step kernel: 64 and 256
+bool sq_ok(int n) { return n == 64 || n == 256; }
+template <int AXIS, int MODE, int R, int L>
+void launch_sq_l(int others, hipStream_t st, const FFTPass &P) {
+    constexpr int G = (L < 8) ? 8 / L : 1;
+    const long ngrp = (((P.nxh + L - 1) / L + G - 1) / G) * G;
+    const long nb = ngrp * others, q = 8L * G;
+    hipLaunchKernelGGL((k_fft_sq<AXIS, MODE, R, L>), dim3((unsigned)((nb + q - 1) / q * q)), dim3(R * L), 0, st, P);
+}
+// 8 wavenumbers per block: each row segment one whole 128-byte line (the
+// 256^3 z visit measured 55.0 us, against 58.3 us with 4 per block)
+template <int AXIS, int MODE>
+void launch_sq(int n, int others, hipStream_t st, const FFTPass &P) {
+    if (n == 64) launch_sq_l<AXIS, MODE, 8, 8>(others, st, P);
+    else launch_sq_l<AXIS, MODE, 16, 8>(others, st, P);
+}
+
 }  // namespace
 
 void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double dy, double dz, double dt,
@@ -970,14 +1136,16 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     // y forward
     P.pl = pl.py;
     P.lsh = fft_lsh(g.ny);
-    if (pow2(g.ny)) launch_lds(k_fft_yz<1, -1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    if (sq_ok(g.ny)) launch_sq<1, 0>(g.ny, g.nz, st, P);
+    else if (pow2(g.ny)) launch_lds(k_fft_yz<1, -1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     else launch_lds(k_fft_yz<1, -1, 0, true, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[1], st);
     // z forward + modal division + z inverse
     P.pl = pl.pz;
     const bool zs_small = g.nz <= FFT_ZS_ELEMS;
     P.lsh = fft_lsh(g.nz, zs_small ? FFT_ZS_ELEMS : FFT_ELEMS);
-    if (zs_small) {
+    if (sq_ok(g.nz)) launch_sq<2, 2>(g.nz, g.ny, st, P);
+    else if (zs_small) {
         if (pow2(g.nz)) launch_lds(k_fft_yz<2, -1, 1, false, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
         else launch_lds(k_fft_yz<2, -1, 1, true, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
     } else {
@@ -988,7 +1156,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     // y inverse
     P.pl = pl.py;
     P.lsh = fft_lsh(g.ny);
-    if (pow2(g.ny)) launch_lds(k_fft_yz<1, 1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
+    if (sq_ok(g.ny)) launch_sq<1, 1>(g.ny, g.nz, st, P);
+    else if (pow2(g.ny)) launch_lds(k_fft_yz<1, 1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     else launch_lds(k_fft_yz<1, 1, 0, true, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[3], st);
     // x inverse (c2r) writing dp
