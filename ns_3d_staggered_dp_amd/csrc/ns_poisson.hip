@@ -356,6 +356,82 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd(FFTPass P) {
     }
 }
 
+// x forward with the divergence for rows that are a multiple of the block
+// size long (nx = 256, 512) in groups of 2L rows of one z plane: each
+// velocity value of the group is loaded once per use in another cell only
+// where that cell is in another plane (w).  u(i+1) is read from the u rows
+// staged in LDS, v(j+1) is the same thread's value of the next row (N is a
+// multiple of the block size, so a thread's elements of consecutive rows
+// share their x), and the group's last row takes v of the row after it.
+// (The generic kernel above loads six values per cell.)
+template <bool ODD, int EL, int RPR>
+__global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd_rs(FFTPass P) {
+    extern __shared__ double2 smem[];
+    const Grid g = P.g;
+    const int N = g.nx, L = 1 << P.lsh, LS = P.LS, nxh = P.nxh, R2 = 2 * L;
+    double2 *A = smem, *tws = smem + (size_t)L * LS;
+    double *Us = (double *)(tws + N);  // R2 rows of N+1 (u faces 1..nx+1)
+    for (int f = threadIdx.x; f < N; f += FFT_NT) tws[f] = P.pl.tw[f];
+    constexpr int PER = 2 * EL / FFT_NT;
+    // RPR = N / FFT_NT elements of one row per thread; PER = R2 * RPR
+    const int rl0 = blockIdx.x * R2;
+    const int c = rl0 / g.ny, y0 = rl0 - c * g.ny;
+    const size_t U1 = g.pu, U2 = g.ny + 2, V1 = g.pc, V2 = g.ny + 1, W1 = g.pc, W2 = g.ny + 2;
+    const unsigned u1 = g.pu, v1 = g.pc;
+    // high faces of the group: w's next plane and v's next row, face 1 again
+    // at the end of the box when every velocity axis is periodic
+    const size_t kw = (P.wrap && c == g.nz - 1) ? 0 : c + 1;
+    const size_t yh = (P.wrap && y0 + R2 == g.ny) ? 0 : y0 + R2;
+    const double *ub = P.us + U1 * ((y0 + 1) + U2 * (size_t)(c + 1));
+    const double *vb = P.vs + 1 + V1 * (y0 + V2 * (size_t)(c + 1));
+    const double *vhb = P.vs + 1 + V1 * (yh + V2 * (size_t)(c + 1));
+    const double *wb0 = P.ws + 1 + W1 * ((y0 + 1) + W2 * (size_t)c);
+    const double *wb1 = P.ws + 1 + W1 * ((y0 + 1) + W2 * kw);
+    double cu[PER], cv[PER], w0[PER], w1[PER], vh[PER];
+    // element u of the thread: row u / RPR, x threadIdx.x + (u % RPR) * FFT_NT
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int r = u / RPR, a = threadIdx.x + (u % RPR) * FFT_NT;
+        cu[u] = ub[a + u1 * r];
+        cv[u] = vb[a + v1 * r];
+        w0[u] = wb0[a + v1 * r];
+        w1[u] = wb1[a + v1 * r];
+        if (u < RPR) vh[u] = vhb[a];
+    }
+    const double uh = ub[N + u1 * min((int)threadIdx.x, R2 - 1)];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int r = u / RPR, a = threadIdx.x + (u % RPR) * FFT_NT;
+        Us[r * (N + 1) + a] = cu[u];
+    }
+    if ((int)threadIdx.x < R2) Us[threadIdx.x * (N + 1) + N] = uh;
+    __syncthreads();
+    // the divergence as in div_rhs: ((du/dx + dv/dy) + dw/dz) / dt
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int r = u / RPR, a = threadIdx.x + (u % RPR) * FFT_NT;
+        const double du = Us[r * (N + 1) + ((P.wrap && a == N - 1) ? 0 : a + 1)] - cu[u];
+        const double dv = ((u + RPR < PER) ? cv[(u + RPR) % PER] : vh[u % RPR]) - cv[u];
+        double s = qdiv(du, P.dx, P.rdx);
+        s = s + qdiv(dv, P.dy, P.rdy);
+        s = s + qdiv(w1[u] - w0[u], P.dz, P.rdz);
+        ((double *)&A[(r >> 1) * LS + a])[r & 1] = qdiv(s, P.dt, P.rdt);
+    }
+    __syncthreads();
+    fft_ip<ODD, EL>(A, P.pl, Lay{LS, 1}, P.lsh, tws, -1);
+    // split the paired transform into the two half spectra
+    const int cnto = R2 * nxh;
+    for (int f = threadIdx.x; f < cnto; f += FFT_NT) {
+        const int rline = idiv(f, P.dNxh), kx = f - rline * nxh;
+        const double2 *z = A + (rline >> 1) * LS;
+        const double2 x0 = z[kx], x1 = z[kx == 0 ? 0 : N - kx];
+        double2 X;
+        if ((rline & 1) == 0) X = make_double2((x0.x + x1.x) * 0.5, (x0.y - x1.y) * 0.5);
+        else X = make_double2((x0.y + x1.y) * 0.5, (x1.x - x0.x) * 0.5);
+        P.spec[(size_t)(rl0 + rline) * P.nxhp + kx] = X;
+    }
+}
+
 // strided complex lines: along y (stride nxhp) for fixed (kx,z), or along z
 // (stride nxhp*ny) for fixed (kx,y); L consecutive kx per block, held
 // line-interleaved in LDS (element (l,pos) at pos*L + l).  Spectrum rows are
@@ -1125,7 +1201,13 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     P.LS = fft_row(g.nx);
     const unsigned nbx = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
     auto pow2 = [](int n) { return (n & (n - 1)) == 0; };
-    if (xf_small) {
+    if (xf_small && !rhs_in && g.nx % FFT_NT == 0 && g.ny % (2 << P.lsh) == 0 &&
+        (2 << P.lsh) * (g.nx / FFT_NT) == 2 * FFT_XF_ELEMS / FFT_NT) {
+        // rows of 256 or 512: 2L rows fill the block's elements exactly
+        const size_t lds = lds_x() + (size_t)(2 << P.lsh) * (g.nx + 1) * sizeof(double);
+        if (g.nx == FFT_NT) launch_lds(k_fft_x_fwd_rs<false, FFT_XF_ELEMS, 1>, nbx, lds, st, P);
+        else launch_lds(k_fft_x_fwd_rs<false, FFT_XF_ELEMS, 2>, nbx, lds, st, P);
+    } else if (xf_small) {
         if (pow2(g.nx)) launch_lds(k_fft_x_fwd<false, FFT_XF_ELEMS>, nbx, lds_x(), st, P);
         else launch_lds(k_fft_x_fwd<true, FFT_XF_ELEMS>, nbx, lds_x(), st, P);
     } else {

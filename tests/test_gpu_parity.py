@@ -386,6 +386,21 @@ def periodic_ghosts_ok(a, own):
     return True
 
 
+@pytest.mark.parametrize("n", [(256, 8, 6), (512, 4, 6), (256, 12, 5)])
+def test_periodic_rows_of_256_and_512(oracle, n):
+    """x forward pass over rows of 256 and 512 cells (row groups of one plane,
+    u staged in LDS, the next row's v from the same thread, the group's last
+    row reading the row after it, also across the periodic end): lazy AB2
+    (ghosts read through their periodic images) and AB2-CN (filled ghosts)"""
+    for scheme in (AB2, AB2_CN):
+        cfg = cfg_periodic(*n, scheme=scheme, dp_flag=0)
+        u, v, w, p = fields.decaying_turbulence(cfg, seed=17)
+        out, st, div = run_both(oracle, cfg, u, v, w, p, 4)
+        for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+            assert rel(out[k], ref) <= 1e-11, (scheme, k, rel(out[k], ref))
+        assert div <= 1e-12
+
+
 def test_lazy_periodic_steps_skip_the_velocity_fill(oracle):
     """all-periodic lazy AB2 steps leave the ghost entries of u*, v*, w*
     unfilled and read them through their periodic images.  Against the oracle
