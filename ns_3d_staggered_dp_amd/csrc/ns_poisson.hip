@@ -630,38 +630,35 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
     const int RL = g.ny * g.nz;
     const int rl0 = blockIdx.x * 2 * L;
     // Z[m] = Xa[m] + i Xb[m] of the paired lines, Hermitian extension for m >= nxh;
-    // c2r ignores the imaginary parts of DC and Nyquist
-    const int cnt = L * N;
+    // c2r ignores the imaginary parts of DC and Nyquist.  Each stored
+    // coefficient k is loaded once and gives Z[k] and Z[N-k].
+    const int cnt = L * nxh;
     // every load of the thread issued before any is used; rows past the grid
     // are clamped to its last row (their values are not used)
     double2 va[EL / FFT_NT], vb[EL / FFT_NT];
 #pragma unroll
     for (int u = 0; u < EL / FFT_NT; u++) {
-        const int f = threadIdx.x + u * FFT_NT;
-        const int l = idiv(f, P.dN), m = f - l * N;
-        const int ra = rl0 + 2 * l, km = (m >= nxh) ? N - m : m;
-        va[u] = P.spec[(size_t)min(ra, RL - 1) * P.nxhp + km];
-        vb[u] = P.spec[(size_t)min(ra + 1, RL - 1) * P.nxhp + km];
+        const int f = min((int)threadIdx.x + u * FFT_NT, cnt - 1);
+        const int l = idiv(f, P.dNxh), k = f - l * nxh;
+        const int ra = rl0 + 2 * l;
+        va[u] = P.spec[(size_t)min(ra, RL - 1) * P.nxhp + k];
+        vb[u] = P.spec[(size_t)min(ra + 1, RL - 1) * P.nxhp + k];
     }
 #pragma unroll
     for (int u = 0; u < EL / FFT_NT; u++) {
         const int f = threadIdx.x + u * FFT_NT;
         if (f >= cnt) break;
-        const int l = idiv(f, P.dN), m = f - l * N;
+        const int l = idiv(f, P.dNxh), k = f - l * nxh;
         const int ra = rl0 + 2 * l;
-        const bool cj = m >= nxh;
-        const int km = cj ? N - m : m;
         double2 a = (ra < RL) ? va[u] : make_double2(0.0, 0.0);
         double2 b = (ra + 1 < RL) ? vb[u] : make_double2(0.0, 0.0);
-        if (km == 0 || 2 * km == N) {
+        if (k == 0 || 2 * k == N) {
             a.y = 0.0;
             b.y = 0.0;
         }
-        if (cj) {
-            a.y = -a.y;
-            b.y = -b.y;
-        }
-        A[l * LS + m] = make_double2(a.x - b.y, a.y + b.x);
+        A[l * LS + k] = make_double2(a.x - b.y, a.y + b.x);
+        // the conjugate-symmetric partner N-k (k = 1 .. ceil(N/2)-1)
+        if (k > 0 && 2 * k < N) A[l * LS + N - k] = make_double2(a.x + b.y, b.x - a.y);
     }
     __syncthreads();
     fft_ip<ODD, EL>(A, P.pl, Lay{LS, 1}, P.lsh, tws, +1);
