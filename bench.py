@@ -36,9 +36,9 @@ ARRAYS_PER_CELL = {"predictor": 13, "vel_bc": 0, "div_fft_x_fwd": 4, "fft_y_fwd"
 ARRAYS_READ = {"predictor": 7, "div_fft_x_fwd": 3, "fft_y_fwd": 1, "fft_z_solve": 1, "fft_y_inv": 1, "fft_x_inv_dp": 1}
 # the reference pipeline's budget (explicit projection pass): 32 arrays = 256 B/cell-step
 REFERENCE_BUDGET_B = 256
-KERNEL_NAMES = {"predictor": "k_predict_tiled<true,64>", "div_fft_x_fwd": "k_fft_x_fwd<false,512>",
-                "fft_y_fwd": "k_fft_yz<1,-1,0,false,2048>", "fft_z_solve": "k_fft_yz<2,-1,1,false,1024>",
-                "fft_y_inv": "k_fft_yz<1,1,0,false,2048>", "fft_x_inv_dp": "k_fft_x_inv<false,1024>", "project": "k_project"}
+KERNEL_NAMES = {"predictor": "k_predict_tiled<true,64>", "div_fft_x_fwd": "k_fft_x_fwd_rs<false,512,1>",
+                "fft_y_fwd": "k_fft_sq<1,0,16,8>", "fft_z_solve": "k_fft_sq<2,2,16,8>",
+                "fft_y_inv": "k_fft_sq<1,1,16,8>", "fft_x_inv_dp": "k_fft_x_inv<false,1024>", "project": "k_project"}
 COPY_CAL_BYTES = 2 ** 30  # bytes each way of the streaming copy (ns_stream_copy) that scales the traffic counters
 CAL_KERNEL = "k_stream_copy"
 
