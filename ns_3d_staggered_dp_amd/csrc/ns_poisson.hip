@@ -620,6 +620,139 @@ __global__ void __launch_bounds__(R * L) k_fft_sq(FFTPass P) {
     }
 }
 
+// y/z lines of 512 = 8^3 points as a three-step transform held in
+// registers (used for the z visit; the Stockham y passes were faster at
+// 512^3): 64 threads per line, thread t holding the 8 points t + 64*c.
+// With t = a + 8*m1 (n = a + 8*m1 + 64*m2), the 64-point transforms of the
+// 8 decimated sequences x[a + 8*m] are two 8-point steps with twiddles
+// W_64^(m1*kk2), then twiddles W_512^(a*kk) and an 8-point step over a
+// give X[kk2 + 8*kk1 + 64*k1], the thread t' = kk2 + 8*kk1 holding k1 = 0..7.
+// Two LDS exchanges per direction; the inverse mirrors the steps, so the z
+// visit (MODE 2) ends with each thread storing the positions it loaded.
+// MODE 0: forward; 1: inverse; 2: forward, division, inverse.
+template <int AXIS, int MODE, int L>
+__global__ void __launch_bounds__(64 * L) k_fft_cube(FFTPass P) {
+    // element (i, j, k, line) at ((k*8 + j)*8 + i)*L + line
+    __shared__ double2 S[512 * L];
+    auto E = [](int i, int j, int k, int l) { return ((k * 8 + j) * 8 + i) * L + l; };
+    const Grid g = P.g;
+    const int nxh = P.nxh, nxhp = P.nxhp;
+    constexpr int G = (L < 8) ? 8 / L : 1;
+    const int ngrp = ((((nxh + L - 1) / L) + G - 1) / G) * G;
+    int blk = (int)blockIdx.x;
+    if (G > 1) {
+        // blocks sharing a 128-byte line of a spectrum row run on one XCD
+        const int sl = blk >> 3, xcd = blk & 7;
+        blk = ((sl / G) * 8 + xcd) * G + (sl % G);
+    }
+    const int other = blk / ngrp;
+    if (other >= ((AXIS == 1) ? g.nz : g.ny)) return;  // padding of the grid (uniform per block)
+    const int l = threadIdx.x % L, t = threadIdx.x / L;
+    const int lo = t & 7, hi = t >> 3;
+    const int kx = (blk - other * ngrp) * L + l;
+    const bool act = kx < nxh;
+    const int kxc = act ? kx : nxh - 1;  // idle lanes read a valid address
+    // 32-bit element offsets: the spectrum of a box with a 512-long y or z
+    // axis has fewer than 2^31 elements (every axis is at most 2048)
+    const unsigned sp = (AXIS == 1) ? (unsigned)nxhp : (unsigned)g.ny * (unsigned)nxhp;
+    const unsigned step = sp * 64;
+    const unsigned b0 = (AXIS == 1) ? (unsigned)other * (unsigned)g.ny * (unsigned)nxhp : (unsigned)other * (unsigned)nxhp;
+    const unsigned bl = b0 + (unsigned)t * sp + (unsigned)kxc;  // element (kxc, position t)
+    const double2 *__restrict__ tw = P.pl.tw;
+    __shared__ double Ez[MODE == 2 ? 512 : 1];
+    double exy = 0.0;
+    if (MODE == 2) {
+        for (int f = threadIdx.x; f < 512; f += 64 * L) Ez[f] = P.ez[f];
+        exy = P.ex[kxc] + P.ey[other];
+    }
+    double2 v[8];
+    if (MODE != 1) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) v[c] = P.spec[bl + c * step];
+        // thread (a, m1) = (lo, hi): DFT over m2, twiddle W_64^(m1*kk2)
+        SmallDFT<8>::run(v, -1);
+#pragma unroll
+        for (int k = 1; k < 8; k++) v[k] = cmul(v[k], tw[8 * hi * k]);
+#pragma unroll
+        for (int k = 0; k < 8; k++) S[E(lo, hi, k, l)] = v[k];
+        __syncthreads();
+        // thread (a, kk2) = (lo, hi): DFT over m1, twiddle W_512^(a*(kk2 + 8*kk1))
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = S[E(lo, m, hi, l)];
+        SmallDFT<8>::run(v, -1);
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = cmul(v[k], tw[lo * (hi + 8 * k)]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; k++) S[E(lo, hi, k, l)] = v[k];
+        __syncthreads();
+        // thread (kk2, kk1) = (lo, hi): DFT over a gives X[t + 64*k1]
+#pragma unroll
+        for (int a = 0; a < 8; a++) v[a] = S[E(a, lo, hi, l)];
+        SmallDFT<8>::run(v, -1);
+        if (MODE == 0) {
+            if (act) {
+                // the store addresses are formed here, not kept live from the loads
+                unsigned bs = bl;
+                asm volatile("" : "+v"(bs));
+#pragma unroll
+                for (int k = 0; k < 8; k++) P.spec[bs + k * step] = v[k];
+            }
+            return;
+        }
+        // modal division (z pass): eig = -4*((ex + ey) + ez); mode (0,0,0) -> 0
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int kz = t + 64 * k;
+            double e = exy + Ez[kz];
+            e = -4.0 * e;
+            // branch-free: mode (0,0,0) has e = 0 and is replaced by zero
+            const double r = 1.0 / e;
+            const bool zero = (kx | other | kz) == 0;
+            v[k] = make_double2(zero ? 0.0 : v[k].x * r, zero ? 0.0 : v[k].y * r);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = P.spec[bl + k * step];
+    }
+    // inverse, thread (kk2, kk1) = (lo, hi) holding X[t + 64*k1]: DFT over
+    // k1, conjugate twiddle; the LDS slots written first are the ones this
+    // thread read last (MODE 2), so no barrier is needed before them
+    SmallDFT<8>::run(v, +1);
+#pragma unroll
+    for (int a = 1; a < 8; a++) {
+        const double2 w = tw[a * (lo + 8 * hi)];
+        v[a] = cmul(v[a], make_double2(w.x, -w.y));
+    }
+#pragma unroll
+    for (int a = 0; a < 8; a++) S[E(a, lo, hi, l)] = v[a];
+    __syncthreads();
+    // thread (a, kk2) = (lo, hi): DFT over kk1, conjugate twiddle W_64^(m1*kk2)
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = S[E(lo, hi, k, l)];
+    SmallDFT<8>::run(v, +1);
+#pragma unroll
+    for (int m = 1; m < 8; m++) {
+        const double2 w = tw[8 * m * hi];
+        v[m] = cmul(v[m], make_double2(w.x, -w.y));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; m++) S[E(lo, m, hi, l)] = v[m];
+    __syncthreads();
+    // thread (a, m1) = (lo, hi): DFT over kk2 gives x[t + 64*m2]
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = S[E(lo, hi, k, l)];
+    SmallDFT<8>::run(v, +1);
+    if (act) {
+        // the store addresses are formed here, not kept live from the loads
+        unsigned bs = bl;
+        asm volatile("" : "+v"(bs));
+#pragma unroll
+        for (int c = 0; c < 8; c++) P.spec[bs + c * step] = v[c];
+    }
+}
+
 template <bool ODD, int EL>
 __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
     extern __shared__ double2 smem[];
@@ -1152,7 +1285,12 @@ void launch_sq_l(int others, hipStream_t st, const FFTPass &P) {
 template <int AXIS, int MODE>
 void launch_sq(int n, int others, hipStream_t st, const FFTPass &P) {
     if (n == 64) launch_sq_l<AXIS, MODE, 8, 8>(others, st, P);
-    else launch_sq_l<AXIS, MODE, 16, 8>(others, st, P);
+    else if (n == 256) launch_sq_l<AXIS, MODE, 16, 8>(others, st, P);
+    else if constexpr (MODE == 2) {
+        // 512: the three-step kernel, 8 wavenumbers (512 threads) per block
+        const long nb = (long)((P.nxh + 7) / 8) * others;
+        hipLaunchKernelGGL((k_fft_cube<AXIS, MODE, 8>), dim3((unsigned)((nb + 7) / 8 * 8)), dim3(512), 0, st, P);
+    }
 }
 
 }  // namespace
@@ -1223,7 +1361,9 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     P.pl = pl.pz;
     const bool zs_small = g.nz <= FFT_ZS_ELEMS;
     P.lsh = fft_lsh(g.nz, zs_small ? FFT_ZS_ELEMS : FFT_ELEMS);
-    if (sq_ok(g.nz)) launch_sq<2, 2>(g.nz, g.ny, st, P);
+    // z visits of 512-long lines too (measured at 512^3: 875.6 -> 560.0 us;
+    // the y passes of 512 keep the Stockham kernel, 427/423 against 439/470 us)
+    if (sq_ok(g.nz) || g.nz == 512) launch_sq<2, 2>(g.nz, g.ny, st, P);
     else if (zs_small) {
         if (pow2(g.nz)) launch_lds(k_fft_yz<2, -1, 1, false, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
         else launch_lds(k_fft_yz<2, -1, 1, true, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
