@@ -134,8 +134,10 @@ def test_poisson_fft(oracle, n):
     assert rel(g, r) <= 1e-12
 
 
+# including 64- and 128-long z lines with ragged row groups (nx = 20, 12, 70, 15)
 @pytest.mark.parametrize("pbc", [(3, 3, 3), (2, 2, 2), (2, 3, 2)])
-@pytest.mark.parametrize("n", [(16, 16, 16), (12, 10, 14), (48, 20, 28), (70, 36, 20), (128, 24, 40)])
+@pytest.mark.parametrize("n", [(16, 16, 16), (12, 10, 14), (48, 20, 28), (70, 36, 20), (128, 24, 40), (20, 10, 128),
+                               (12, 6, 64), (70, 8, 128), (15, 8, 64)])
 def test_dct_poisson(oracle, pbc, n):
     cfg = cfg_box(*n, bc=(2, 2, 2), pbc=pbc)
     rhs = np.asfortranarray(np.random.default_rng(61).standard_normal(n))
