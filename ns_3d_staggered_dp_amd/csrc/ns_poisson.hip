@@ -180,7 +180,8 @@ constexpr int FFT_ELEMS = 2048;  // complex elements per block (L*N), at most
 constexpr int FFT_XF_ELEMS = FFT_ELEMS / 4, FFT_XI_ELEMS = FFT_ELEMS / 2;
 // DCT passes: 1024 elements per block
 constexpr int DCT_ELEMS = 1024;
-// periodic z pass (two transforms per visit; measured at 256^3)
+// Stockham periodic z pass (two transforms per visit; measured at 256^3 before
+// the register-resident kernels took the 64-, 256- and 512-long lines)
 constexpr int FFT_ZS_ELEMS = 1024;
 
 // x / d for 0 <= x < 2^24 with r = 1/d in single precision (one correction)
