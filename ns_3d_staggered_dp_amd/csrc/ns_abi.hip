@@ -609,8 +609,6 @@ bool valid_cfg(const ns_config *f) {
     return true;
 }
 
-inline size_t comp_size(const Grid &g, int c) { return c == 0 ? g.nu() : c == 1 ? g.nv() : c == 2 ? g.nw() : g.np(); }
-
 int run_poisson(ns_ctx *c, const double *us, const double *vs, const double *ws, const double *rhs_in,
                 const double *planes, const double *mean_dev, double mean_host, double *dp_out, double *sol_out,
                 hipEvent_t *ev = nullptr, bool dp_ghost_x = false, bool wrap_vel = false, double *p_acc = nullptr) {

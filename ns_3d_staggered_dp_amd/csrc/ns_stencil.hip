@@ -109,19 +109,6 @@ __device__ __forceinline__ void conv_diff(const FieldsC &f, int c, const int bc[
     else conv_diff_w(f, bc[2] == 1, i, j, k, dx, dy, dz, dx2, dy2, dz2, conv, dif);
 }
 
-// range of the rhs assignment (x_range0/x_range1 etc., big_periodic_3D.f90:126-146,356-358)
-__device__ __forceinline__ bool in_range(int c, const int bc[3], int nx, int ny, int nz, int i, int j, int k) {
-    int id[3] = {i, j, k}, n[3] = {nx, ny, nz};
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        int lo, hi;
-        if (a == c) { lo = (bc[a] == 1) ? 1 : 2; hi = n[a]; }
-        else { lo = 2; hi = n[a] + 1; }
-        if (id[a] < lo || id[a] > hi) return false;
-    }
-    return true;
-}
-
 struct CDParams {
     int nx, ny, nz;
     int pu, pc;
