@@ -506,6 +506,9 @@ def main():
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4),
                           "frac_of_256B_pipeline_bound": round(REFERENCE_BUDGET_B * value * 1e6 / (HBM_PEAK_GBS * 1e9), 4)},
         "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
+        # each pass's algorithmic bytes (arrays x 8 B x n^3) over its HIP-event time, as a fraction of 8 TB/s
+        "phase_frac": {k: round(ARRAYS_PER_CELL[k] * 8 * n3 / (phase_ms[k] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+                       for k in PHASES if ARRAYS_PER_CELL.get(k, 0) > 0 and phase_ms[k] > 0},
         "checks": {"finite": bool(finite), "max_div": maxdiv},
     }
     if a.adjoint_n > 0:
