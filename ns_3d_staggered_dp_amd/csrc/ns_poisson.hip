@@ -757,7 +757,12 @@ __global__ void __launch_bounds__(64 * L) k_fft_cube(FFTPass P) {
     }
 }
 
-template <bool ODD, int EL>
+// HALF: a power-of-two N with L*N == EL, so the coefficients 0..N/2-1 of the
+// block's lines fill EL/(2*FFT_NT) load slots exactly and the Nyquist
+// coefficient of each line is one more load of the first L lanes (the
+// generic path clamps and issues EL/FFT_NT slots of loads for L*(N/2+1)
+// coefficients)
+template <bool ODD, int EL, bool HALF = false>
 __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
     extern __shared__ double2 smem[];
     const Grid g = P.g;
@@ -769,23 +774,33 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_inv(FFTPass P) {
     // Z[m] = Xa[m] + i Xb[m] of the paired lines, Hermitian extension for m >= nxh;
     // c2r ignores the imaginary parts of DC and Nyquist.  Each stored
     // coefficient k is loaded once and gives Z[k] and Z[N-k].
-    const int cnt = L * nxh;
+    // HALF: coefficients 0..N/2-1 here (f = l*(N/2) + k), Nyquist below
+    const int kn = HALF ? N / 2 : nxh;
+    const int cnt = L * kn;
+    constexpr int NS = HALF ? EL / (2 * FFT_NT) : EL / FFT_NT;
     // every load of the thread issued before any is used; rows past the grid
     // are clamped to its last row (their values are not used)
-    double2 va[EL / FFT_NT], vb[EL / FFT_NT];
+    double2 va[NS], vb[NS];
 #pragma unroll
-    for (int u = 0; u < EL / FFT_NT; u++) {
+    for (int u = 0; u < NS; u++) {
         const int f = min((int)threadIdx.x + u * FFT_NT, cnt - 1);
-        const int l = idiv(f, P.dNxh), k = f - l * nxh;
+        const int l = HALF ? f >> __builtin_ctz(kn) : idiv(f, P.dNxh), k = f - l * kn;
         const int ra = rl0 + 2 * l;
         va[u] = P.spec[(size_t)min(ra, RL - 1) * P.nxhp + k];
         vb[u] = P.spec[(size_t)min(ra + 1, RL - 1) * P.nxhp + k];
     }
+    if (HALF && (int)threadIdx.x < L) {
+        // Nyquist coefficient of line threadIdx.x: imaginary parts ignored
+        const int ra = rl0 + 2 * threadIdx.x;
+        const double an = (ra < RL) ? P.spec[(size_t)ra * P.nxhp + N / 2].x : 0.0;
+        const double bn = (ra + 1 < RL) ? P.spec[(size_t)(ra + 1) * P.nxhp + N / 2].x : 0.0;
+        A[threadIdx.x * LS + N / 2] = make_double2(an, bn);
+    }
 #pragma unroll
-    for (int u = 0; u < EL / FFT_NT; u++) {
+    for (int u = 0; u < NS; u++) {
         const int f = threadIdx.x + u * FFT_NT;
         if (f >= cnt) break;
-        const int l = idiv(f, P.dNxh), k = f - l * nxh;
+        const int l = HALF ? f >> __builtin_ctz(kn) : idiv(f, P.dNxh), k = f - l * kn;
         const int ra = rl0 + 2 * l;
         double2 a = (ra < RL) ? va[u] : make_double2(0.0, 0.0);
         double2 b = (ra + 1 < RL) ? vb[u] : make_double2(0.0, 0.0);
@@ -1389,7 +1404,9 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     P.LS = fft_row(g.nx);
     const unsigned nbxi = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
     if (xi_small) {
-        if (pow2(g.nx)) launch_lds(k_fft_x_inv<false, FFT_XI_ELEMS>, nbxi, lds_x(), st, P);
+        if (pow2(g.nx) && (g.nx << P.lsh) == FFT_XI_ELEMS)
+            launch_lds(k_fft_x_inv<false, FFT_XI_ELEMS, true>, nbxi, lds_x(), st, P);
+        else if (pow2(g.nx)) launch_lds(k_fft_x_inv<false, FFT_XI_ELEMS>, nbxi, lds_x(), st, P);
         else launch_lds(k_fft_x_inv<true, FFT_XI_ELEMS>, nbxi, lds_x(), st, P);
     } else {
         if (pow2(g.nx)) launch_lds(k_fft_x_inv<false, FFT_ELEMS>, nbxi, lds_x(), st, P);
