@@ -38,7 +38,7 @@ ARRAYS_READ = {"predictor": 7, "div_fft_x_fwd": 3, "fft_y_fwd": 1, "fft_z_solve"
 REFERENCE_BUDGET_B = 256
 KERNEL_NAMES = {"predictor": "k_predict_tiled<true,64>", "div_fft_x_fwd": "k_fft_x_fwd_rs<false,512,1>",
                 "fft_y_fwd": "k_fft_sq<1,0,16,8>", "fft_z_solve": "k_fft_sq<2,2,16,8>",
-                "fft_y_inv": "k_fft_sq<1,1,16,8>", "fft_x_inv_dp": "k_fft_x_inv<false,1024>", "project": "k_project"}
+                "fft_y_inv": "k_fft_sq<1,1,16,8>", "fft_x_inv_dp": "k_fft_x_inv<false,1024,true>", "project": "k_project"}
 COPY_CAL_BYTES = 2 ** 30  # bytes each way of the streaming copy (ns_stream_copy) that scales the traffic counters
 CAL_KERNEL = "k_stream_copy"
 

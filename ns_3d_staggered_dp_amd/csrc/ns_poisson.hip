@@ -402,13 +402,13 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd_rs(FFTPass P) {
         w1[u] = wb1[a + v1 * r];
         if (u < RPR) vh[u] = vhb[a];
     }
-    const double uh = ub[N + u1 * min((int)threadIdx.x, R2 - 1)];
 #pragma unroll
     for (int u = 0; u < PER; u++) {
         const int r = u / RPR, a = threadIdx.x + (u % RPR) * FFT_NT;
         Us[r * (N + 1) + a] = cu[u];
     }
-    if ((int)threadIdx.x < R2) Us[threadIdx.x * (N + 1) + N] = uh;
+    // the rows' high ghost faces u(nx+1): one load of the first R2 lanes
+    if ((int)threadIdx.x < R2) Us[threadIdx.x * (N + 1) + N] = ub[N + u1 * threadIdx.x];
     __syncthreads();
     // the divergence as in div_rhs: ((du/dx + dv/dy) + dw/dz) / dt
 #pragma unroll
