@@ -361,13 +361,12 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd(FFTPass P) {
 }
 
 // x forward with the divergence for rows that are a multiple of the block
-// size long (nx = 256, 512) in groups of 2L rows of one z plane: each
-// velocity value of the group is loaded once per use in another cell only
-// where that cell is in another plane (w).  u(i+1) is read from the u rows
-// staged in LDS, v(j+1) is the same thread's value of the next row (N is a
-// multiple of the block size, so a thread's elements of consecutive rows
-// share their x), and the group's last row takes v of the row after it.
-// (The generic kernel above loads six values per cell.)
+// size long (nx = 256, 512) in groups of 2L rows of one z plane.  Each u and
+// v value the group needs is loaded once, w for both of its planes: u(i+1)
+// is read from the u rows staged in LDS, v(j+1) is the same thread's value
+// of the next row (N is a multiple of the block size, so a thread's elements
+// of consecutive rows share their x), and the group's last row takes v of
+// the row after it.  (The generic kernel above loads six values per cell.)
 template <bool ODD, int EL, int RPR>
 __global__ void __launch_bounds__(FFT_NT) k_fft_x_fwd_rs(FFTPass P) {
     extern __shared__ double2 smem[];
