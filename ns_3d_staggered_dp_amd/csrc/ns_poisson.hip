@@ -1403,7 +1403,8 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     P.LS = fft_row(g.nx);
     const unsigned nbxi = (unsigned)((RL + (2 << P.lsh) - 1) / (2 << P.lsh));
     if (xi_small) {
-        if (pow2(g.nx) && (g.nx << P.lsh) == FFT_XI_ELEMS)
+        // (the Nyquist loads take one lane per line: at most FFT_NT lines, nx >= 4)
+        if (pow2(g.nx) && (g.nx << P.lsh) == FFT_XI_ELEMS && (1 << P.lsh) <= FFT_NT)
             launch_lds(k_fft_x_inv<false, FFT_XI_ELEMS, true>, nbxi, lds_x(), st, P);
         else if (pow2(g.nx)) launch_lds(k_fft_x_inv<false, FFT_XI_ELEMS>, nbxi, lds_x(), st, P);
         else launch_lds(k_fft_x_inv<true, FFT_XI_ELEMS>, nbxi, lds_x(), st, P);

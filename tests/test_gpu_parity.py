@@ -124,7 +124,8 @@ def test_divergence_and_projection_bitwise(oracle):
 # 64 and 256 long y/z lines take the register-resident two-step kernel, 512
 # long ones the three-step kernel (ragged wavenumber groups: nx = 10, 42, 20, 6)
 @pytest.mark.parametrize("n", [(16, 16, 16), (12, 10, 8), (32, 8, 24), (64, 64, 64), (10, 64, 256), (42, 256, 64),
-                               (6, 256, 12), (20, 12, 256), (6, 512, 10), (20, 6, 512), (10, 512, 512)])
+                               (6, 256, 12), (20, 12, 256), (6, 512, 10), (20, 6, 512), (10, 512, 512), (2, 8, 6),
+                               (4, 6, 8), (2, 256, 4)])
 def test_poisson_fft(oracle, n):
     cfg = cfg_box(*n)
     rhs = np.asfortranarray(np.random.default_rng(51).standard_normal(n))
