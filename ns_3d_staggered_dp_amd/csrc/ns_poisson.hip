@@ -1346,6 +1346,9 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
         const long nb = ngrp * others, q = 8L * G;
         return (unsigned)((nb + q - 1) / q * q);
     };
+    // the register-resident y/z kernels index the spectrum with 32-bit element
+    // offsets (true for every box whose y or z axis they take; checked here)
+    const bool spec32 = (size_t)g.nz * g.ny * P.nxhp < ((size_t)1 << 31);
     // x forward (fused with the divergence)
     P.pl = pl.px;
     // lines longer than the x-pass cap use the full-size register sets
@@ -1371,7 +1374,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     // y forward
     P.pl = pl.py;
     P.lsh = fft_lsh(g.ny);
-    if (sq_ok(g.ny)) launch_sq<1, 0>(g.ny, g.nz, st, P);
+    if (sq_ok(g.ny) && spec32) launch_sq<1, 0>(g.ny, g.nz, st, P);
     else if (pow2(g.ny)) launch_lds(k_fft_yz<1, -1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     else launch_lds(k_fft_yz<1, -1, 0, true, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[1], st);
@@ -1381,7 +1384,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     P.lsh = fft_lsh(g.nz, zs_small ? FFT_ZS_ELEMS : FFT_ELEMS);
     // z visits of 512-long lines too (measured at 512^3: 875.6 -> 560.0 us;
     // the y passes of 512 keep the Stockham kernel, 427/423 against 439/470 us)
-    if (sq_ok(g.nz) || g.nz == 512) launch_sq<2, 2>(g.nz, g.ny, st, P);
+    if ((sq_ok(g.nz) || g.nz == 512) && spec32) launch_sq<2, 2>(g.nz, g.ny, st, P);
     else if (zs_small) {
         if (pow2(g.nz)) launch_lds(k_fft_yz<2, -1, 1, false, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
         else launch_lds(k_fft_yz<2, -1, 1, true, FFT_ZS_ELEMS>, nb_yz(g.ny), lds_yz(g.nz), st, P);
@@ -1393,7 +1396,7 @@ void poisson_fft_solve(const Grid &g, const PoissonPlans &pl, double dx, double 
     // y inverse
     P.pl = pl.py;
     P.lsh = fft_lsh(g.ny);
-    if (sq_ok(g.ny)) launch_sq<1, 1>(g.ny, g.nz, st, P);
+    if (sq_ok(g.ny) && spec32) launch_sq<1, 1>(g.ny, g.nz, st, P);
     else if (pow2(g.ny)) launch_lds(k_fft_yz<1, 1, 0, false, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     else launch_lds(k_fft_yz<1, 1, 0, true, FFT_ELEMS>, nb_yz(g.nz), lds_yz(g.ny), st, P);
     if (ev) (void)hipEventRecord(ev[3], st);
