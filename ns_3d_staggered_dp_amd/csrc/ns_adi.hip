@@ -201,6 +201,7 @@ struct SweepParams {
     // the row ends idle.  xs[c] is the array origin's offset from its
     // line-aligned start.
     int xs[3];
+    int c0;          // first component of the launch (blockIdx.y counts from it)
 };
 
 // ---- partitioned wall solve (LineSys::K blocks per line, wave k = block k) ----
@@ -359,7 +360,7 @@ __device__ __forceinline__ void part_ghosts(int k, int K, const LineSys &S, bool
 template <int AX, int RC, bool DENSE>
 __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
-    const int c = blockIdx.y;
+    const int c = blockIdx.y + P.c0;
     int n[3];
     comp_dims(P.g, c, n);
     const int o2 = (AX == 2) ? 1 : 2;
@@ -444,7 +445,7 @@ __device__ __forceinline__ void adi_part_strided_body(const SweepParams &P, cons
 template <int RC, bool DENSE>
 __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
-    const int c = blockIdx.y;
+    const int c = blockIdx.y + P.c0;
     int n[3];
     comp_dims(P.g, c, n);
     const int N = n[0];  // line length (the LDS rows hold whole lines)
@@ -564,7 +565,7 @@ __device__ __forceinline__ void adi_part_x_body(const SweepParams &P, const int 
 // per line.  Periodic lines: one lane per line (reduced cyclic solve).
 template <int AX>
 __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int blk) {
-    const int c = blockIdx.y;
+    const int c = blockIdx.y + P.c0;
     int n[3];
     comp_dims(P.g, c, n);
     const int o2 = (AX == 2) ? 1 : 2;
@@ -619,7 +620,7 @@ __device__ __forceinline__ void adi_strided_body(const SweepParams &P, const int
 __device__ __forceinline__ void adi_x_body(const SweepParams &P, const int blk) {
     extern __shared__ double sm[];
     const int lsh = P.lsh, coff = P.coff;
-    const int c = blockIdx.y;
+    const int c = blockIdx.y + P.c0;
     int n[3];
     comp_dims(P.g, c, n);
     const int N = n[0], L = 1 << lsh;
@@ -909,12 +910,13 @@ void launch_adi_batch(int ax, int kw, int rows, bool dense, const void *params, 
 }
 
 void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
-                      hipStream_t st, const double *planes, const PlaneLayout *L, double *planes_out) {
+                      hipStream_t st, const double *planes, const PlaneLayout *L, double *planes_out, int c0, int nc) {
     SweepParams P;
     size_t lds = 0;
     const int nb = make_sweep_params(g, dir, sysA, sysB, u, v, w, &P, &lds, 6, true, planes, L, planes_out);
+    P.c0 = c0;
     if (P.kw > 0) {
-        const dim3 grid(nb, 3), blk(64 * P.kw);
+        const dim3 grid(nb, nc), blk(64 * P.kw);
         const bool dense = g.dense();
         if (P.rc > PART_ROWS) launch_part<PART_ROWS_LONG>(dir - 1, dense, grid, blk, lds, st, &P, nullptr, nullptr, 0);
         else launch_part<PART_ROWS>(dir - 1, dense, grid, blk, lds, st, &P, nullptr, nullptr, 0);
@@ -922,13 +924,13 @@ void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys
     }
     if (dir == 1) {
         set_lds(k_adi_x, lds);
-        hipLaunchKernelGGL(k_adi_x, dim3(nb, 3), dim3(64), lds, st, P);
+        hipLaunchKernelGGL(k_adi_x, dim3(nb, nc), dim3(64), lds, st, P);
     } else if (dir == 2) {
         set_lds(k_adi_strided<1>, lds);
-        hipLaunchKernelGGL(k_adi_strided<1>, dim3(nb, 3), dim3(64), lds, st, P);
+        hipLaunchKernelGGL(k_adi_strided<1>, dim3(nb, nc), dim3(64), lds, st, P);
     } else {
         set_lds(k_adi_strided<2>, lds);
-        hipLaunchKernelGGL(k_adi_strided<2>, dim3(nb, 3), dim3(64), lds, st, P);
+        hipLaunchKernelGGL(k_adi_strided<2>, dim3(nb, nc), dim3(64), lds, st, P);
     }
 }
 

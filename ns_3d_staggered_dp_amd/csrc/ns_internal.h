@@ -382,8 +382,9 @@ void launch_add_partials(double *a, size_t q, const double *partial, int nb, dou
 
 // ---- ADI (ns_adi.hip) ----
 // planes, planes_out: as for make_sweep_params (used only when the sweep is partitioned)
+// c0, nc: the components swept (blockIdx.y = component - c0)
 void launch_adi_sweep(const Grid &g, int dir, const LineSys &sysA, const LineSys &sysB, double *u, double *v, double *w,
                       hipStream_t st, const double *planes = nullptr, const PlaneLayout *L = nullptr,
-                      double *planes_out = nullptr);
+                      double *planes_out = nullptr, int c0 = 0, int nc = 3);
 
 }  // namespace ns

@@ -316,6 +316,24 @@ def test_step_walls_cube(oracle, n, scheme):
         assert div <= 1e-12, div
 
 
+def test_step_walls_256_component_streams(oracle):
+    """boxes of 2^24 cells or more take the three velocity components' ADI
+    sweeps on three streams (fork and join events on the context stream, also
+    inside the graph that replays steady steps): a 256^3 wall box, AB2-CN,
+    against the oracle after 3 steps (an eager first step, then a two-step
+    graph), with the uniform-divergence check of the cube test"""
+    import plane_sets
+
+    cfg = NSConfig.cube(256, nu=1e-3, dt=5e-4, scheme=AB2_CN, bc=(2, 2, 2), pbc=(3, 3, 3), dp_flag=1)
+    u, v, w, p = fields.decaying_turbulence(cfg, seed=23)
+    planes = plane_sets.zero_flux(cfg, rand_planes(cfg, 24, 0.05))
+    out, st, div = run_both(oracle, cfg, u, v, w, p, 3, planes=planes)
+    for k, ref in (("u", st.u), ("v", st.v), ("w", st.w), ("p", st.p)):
+        assert rel(out[k], ref) <= 1e-11, (k, rel(out[k], ref))
+    d = plane_sets.divergence(cfg, out["u"], out["v"], out["w"])
+    assert np.abs(d - d.mean()).max() <= 1e-12
+
+
 def cfg_periodic(nx, ny, nz, **kw):
     return NSConfig(nx=nx, ny=ny, nz=nz, dx=2 * np.pi / nx, dy=0.5, dz=0.35, nu=2e-3, dt=2e-3, bc=(1, 1, 1),
                     pbc=(1, 1, 1), flags=FLAG_GAUGE_CORNER, **kw)
